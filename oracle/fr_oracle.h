@@ -1,0 +1,154 @@
+/*
+ * fr_oracle.h -- CPU restatement of the Frealign v9.12 per-particle hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP
+ * implementation in frealign_amd/csrc.  Only tests/, __graft_entry__.smoke()
+ * and the cpu_baseline leg of bench.py may load it; the product path never
+ * calls it.
+ *
+ * Layout conventions (identical to the reference's Fortran storage, 0-based):
+ *   half-plane  : JC x NSAM complex64, index l + JC*m, l = x-frequency 0..NSAM/2,
+ *                 m = wrapped y-frequency (m >= JC means m-NSAM); ctfapply.f:64-76
+ *   CTFF        : 2 half-planes, CTFR then CTFL (offset IC = NSAM*JC); ctfapply.f:78-79
+ *   real image  : NSAM x NSAM float, row pitch NSAM+2 (in-place r2c layout)
+ *   half-volume : JC x NSAM x NSAM complex64, index l + JC*(m + NSAM*n); shift.f:34
+ * Complex arrays are interleaved (re, im).
+ *
+ * Every value-producing routine exists in two precisions:
+ *   or32_*  FP32 arithmetic in the reference's operation order (the restatement)
+ *   or64_*  the same algorithm with FP64 values (the error yardstick)
+ * Coordinates, rotation matrices and Miller indices are always FP32, exactly as
+ * in the reference, so both variants visit the same voxels.
+ *
+ * Parity pinning (tests/test_oracle_*.py):
+ *   pinned against the flang-compiled reference routines (oracle/_ref):
+ *     ctf, maskcos, maskcos2, pweight, ainterpo3ds, pextract, pinsert
+ *     (pinsert_trilin + symmetry loop), searchang/limitsymm, random,
+ *     getsymmat, sheltest, storeshift
+ *   pinned by closed forms and invariants (the reference's copies need FFTW,
+ *   which this image lacks): the 2D FFTs (vs numpy.fft), ctfapply (composition
+ *   of pinned parts + FFT), cc3m (self-correlation closed form, scale
+ *   invariance, FP64 agreement), presb (FSC-style closed form), ccp/psearch
+ *   (brute-force direct DFT on small boxes, quadrant permutation identities).
+ */
+#ifndef FR_ORACLE_H
+#define FR_ORACLE_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- precision-independent pieces (fr_oracle_common.c) ---- */
+
+/* random.f:8-32 (L'Ecuyer LCG). *is is updated in place. */
+float or_random(int32_t *is);
+
+/* Rotation matrix of pextract.f:30-51 / cc3m.f:48-65: ZYZ Euler, divided by
+ * |amag| unless amag == 0 (PEXTRACT does not divide), then DM <- MATMUL_T(DM,SM)
+ * when sm != NULL (matmul_t.f:17-24).  dm is column-major 3x3 (Fortran DM(1..9)). */
+void or_dm(float phi, float theta, float psi, float amag, const float *sm, float dm[9]);
+
+/* matmul_t.f:8-26: C <- (A^T B)^T computed in the reference order. */
+void or_matmul_t(const float a[9], const float b[9], float c[9]);
+
+/* getsymmat.f:8-97 for C<n> and D<n> (and T/O/I tables).  Returns nsym;
+ * fills symop (3x3 column-major per operator), jsym (rotational order per
+ * operator, checksym.f) and *iredun.  asym: 'C','D','T','O','I'. */
+int or_getsymmat(char asym, int nasym, float *symop, int *jsym, int *iredun);
+
+/* searchang.f:8-150 + limitsymm.f:8-51.  Fills sang (3 x nsang radians, may be
+ * NULL to count only) and returns nsang; *iquadmax and *dang_out as computed.
+ * mask: PMASK(1..5). */
+int or_searchang(float rmm, float ri, float dangin, char asym, int nasym,
+                 const int mask[5], int asymstore, float *sang, int *iquadmax,
+                 float *dang_out);
+
+/* Unnormalised 2D real FFTs in the reference's FFTW layout (fftw_plans.f):
+ * forward e^{-i}, backward e^{+i}; real data NSAM x NSAM, row pitch NSAM+2,
+ * in place.  Computed in FP64; the float entry points round on output.
+ * The backward transform follows the row-column c2r semantic: the imaginary
+ * parts of the x=0 and x=NSAM/2 columns are ignored after the y pass. */
+void or_rfft2_f(int nsam, float *data);   /* r2c in place */
+void or_irfft2_f(int nsam, float *data);  /* c2r in place */
+void or_rfft2_d(int nsam, double *data);
+void or_irfft2_d(int nsam, double *data);
+
+/* ---- precision-dependent pieces (fr_oracle.c compiled as or32_ / or64_) ---- */
+#define OR_DECLARE(P, R)                                                        \
+  /* ctf.f:8-64; out[2] = (re, im) */                                            \
+  void P##ctf(float cs, float wl, float wgh1, float wgh2in, float df1,          \
+              float df2, float angast, float thetatr, int ix, int iy,           \
+              float tx, float ty, R *out);                                      \
+  /* maskcos.f:8-55 (centre NSAM/2+1); outd pitch NSAM+2 */                     \
+  void P##maskcos(int nsam, R *outd, float ri2, float ri3, float rih,           \
+                  float halfw, float amagp);                                     \
+  /* maskcos2.f:8-64 (centre at the FFT origin) */                              \
+  void P##maskcos2(int nsam, R *outd, float ri2, float ri3, float rih,          \
+                   float halfw, float amagp);                                    \
+  /* pweight.f:8-98; spec and outc may alias */                                 \
+  void P##pweight(int nsam, const R *spec, R *outc, const float *ssnr,          \
+                  const R *ctff, int ictf);                                     \
+  /* ctfapply.f:8-161 (non-helical, XSTD >= 0).  spec: half-plane of the r2c   \
+   * FFT of the particle.  Outputs ctff (2 half-planes), outc (masked,          \
+   * SNR-weighted), datc (shifted, unmasked, origin-centred). */                \
+  void P##ctfapply(int nsam, const float *spec, float shx, float shy,           \
+                   float cs, float wl, float wgh1, float wgh2, float df1,       \
+                   float df2, float angast, float thetatr, float amagp,         \
+                   float rih, float halfw, float ri2, float ri3,                \
+                   const float *pssnr, int ictf, R *ctff, R *outc, R *datc);    \
+  /* ainterpo3ds.f:8-99 (trilinear, Hermitian half-volume); nsam = IPAD*NSAM */ \
+  void P##ainterpo3ds(int nsam, const float *a3df, float xx, float yy,          \
+                      float zz, R *out);                                        \
+  /* pextract.f:8-101 (IEWALD=0, IRAD!=0, IPAD=1) */                            \
+  void P##pextract(int nsam, R *outc, const float *a3df, float phi,             \
+                   float theta, float psi, const R *ctff, const float *sm);     \
+  /* cc3m.f:8-169 (IEWALD=0, IBUF=0, IPAD=1, IRAD!=0) */                        \
+  double P##cc3m(int nsam, float amag, const R *spec, const float *a3df,        \
+                 int maxr1, int maxr2, float phi, float theta, float psi,       \
+                 float shx, float shy, float rbfact, const float *fsct,         \
+                 const R *ctff, float ri2, float ri3, float rih, float halfw,   \
+                 const float *sm);                                              \
+  /* presb.f:8-160 (non-helical).  rbin[2*NSAM] updated like the reference. */  \
+  void P##presb(int nsam, float amag, const R *spec, const float *a3df,         \
+                float phi, float theta, float psi, float shx, float shy,        \
+                R *rbin, const R *ctff, float ri2, float ri3, float rih,        \
+                float halfw, const float *sm);                                  \
+  /* pinsert.f:8-195 with pinsert_trilin.f:8-244 (INTERP=1, IEWALD=0,          \
+   * NNSTAT=0, IMP=1).  Accumulates into b3df (complex), t3df, *nksum, *ttd.    \
+   * occ is the ON = OCC/SIG2N factor of a3d3_s.f:44. */                        \
+  void P##pinsert(int nsam, float rrec, float amag, const R *spec,              \
+                  const R *ctff, R *b3df, R *t3df, double *ttd, float phi,      \
+                  float theta, float psi, float shx, float shy, float presa,    \
+                  float pbc, float boff, int nsym, const int *jsym,             \
+                  const float *symop, int64_t *nksum, float psize, float occ,   \
+                  const float *sm, int nonsym);                                 \
+  /* storeshift.f:43-102 (IFSC=0, NNSTAT=0): half-maps A,D -> B,C */            \
+  void P##storeshift(int nsam, double std, double vtd, int iredun,              \
+                     const R *a3df, const R *s3df, const R *d3df,               \
+                     const R *v3df, R *b3df, R *c3df);                          \
+  /* sheltest.f:8-122 (FSC part, NNSTAT=0); returns nstp */                     \
+  int P##sheltest(int nsam, int istp, const R *a3df, const R *d3df,             \
+                  const R *s3df, const R *v3df, float fmask, int *npix,         \
+                  int *npixt, double *dfsc, R *pssnr);                          \
+  /* ccp.f:8-275 with rotate2d.f and pad.f (IRAD!=0, IPAD=1, IEWALD=0).        \
+   * spec: B-factor weighted particle half-plane.  On return *psi, *shx, *shy  \
+   * hold the best in-plane angle and shift and the score is returned. */       \
+  double P##ccp(int nsam, float amag, const R *spec, const float *a3df,         \
+                float dangin, int maxr1, int maxr2, float phi, float theta,     \
+                float *psi, float *shx, float *shy, const int mask[5],          \
+                const R *ctff, int iquadmax, const float *sm);                  \
+  /* psearch.f:8-121.  testpar: 6 x ipmax; returns CCMAX. */                   \
+  double P##psearch(int nsam, float amag, const R *spec, const float *a3df,     \
+                    float dangin, int maxr1, int maxr2, float *phi,             \
+                    float *theta, float *psi, float *dshx, float *dshy,         \
+                    const float *sang, int nsang, int iquadmax,                 \
+                    const int mask[5], float rbfact, const R *ctff,             \
+                    float *testpar, int ipmax, int32_t *iran, const float *sm);
+
+OR_DECLARE(or32_, float)
+OR_DECLARE(or64_, double)
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,252 @@
+"""Pin the CPU oracle to the reference's own routines, compiled by flang from
+the reference sources (oracle/_ref).  These are the FFT-free hot-path
+routines; FFT-using ones are pinned in test_oracle_math.py.
+
+FP32 oracle vs reference: integer outputs (grid, symmetry orders, counts)
+must be bit-exact; float outputs are compared bit-exactly where the routine
+is a straight FP32 restatement and to 1e-6 relative where flang's complex
+arithmetic may order operations differently.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+rng = np.random.default_rng(1234)
+
+
+def half_volume(nsam, seed):
+    r = np.random.default_rng(seed)
+    v = r.standard_normal((nsam, nsam, nsam)).astype(np.float32)
+    f = np.fft.rfftn(v).astype(np.complex64)  # (z, y, x//2+1), x fastest
+    return np.ascontiguousarray(f).reshape(-1)
+
+
+def half_plane(nsam, seed):
+    r = np.random.default_rng(seed)
+    jc = nsam // 2 + 1
+    return (r.standard_normal(nsam * jc) + 1j * r.standard_normal(nsam * jc)).astype(np.complex64)
+
+
+@pytest.mark.parametrize("asym,nasym", [("C", 1), ("D", 7), ("C", 7), ("T", 0), ("O", 0), ("I", 0)])
+@pytest.mark.parametrize("dang", [5.0, 7.5, 15.0])
+def test_searchang_bitexact(ref_lib, asym, nasym, dang):
+    a, iqa = O.searchang(dang, asym, nasym, lib="oracle")
+    b, iqb = O.searchang(dang, asym, nasym, lib="ref")
+    assert iqa == iqb
+    assert a.shape == b.shape
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def test_searchang_counts(ref_lib):
+    # the view counts quoted for the workloads, from the reference itself
+    assert len(O.searchang(5.0, "D", 7, lib="ref")[0]) == 130
+    assert len(O.searchang(5.0, "C", 1, lib="ref")[0]) == 1626
+    assert len(O.searchang(15.0, "C", 1, lib="ref")[0]) == 178
+
+
+@pytest.mark.parametrize("asym,nasym", [("C", 1), ("C", 5), ("D", 7), ("T", 0), ("O", 0), ("I", 0)])
+def test_getsymmat(ref_lib, asym, nasym):
+    r = O.getsymmat(asym, nasym, lib="ref")
+    o = O.getsymmat(asym, nasym)
+    assert r[0] == o[0] and r[3] == o[3]
+    assert np.array_equal(r[2][: r[0]], o[2][: o[0]])
+    assert np.array_equal(r[1].view(np.int32), o[1].view(np.int32))
+
+
+def test_random_lcg(ref_lib):
+    L = O.oracle()
+    for seed in (-1, -37, -59, 12345):
+        s1, s2 = C.c_int(seed), C.c_int(seed)
+        for _ in range(50):
+            a = L.or_random(C.byref(s1))
+            out = C.c_float(0)
+            ref_lib.ref_random(C.byref(s2), C.byref(out))
+            assert s1.value == s2.value
+            assert np.float32(a).view(np.int32) == np.float32(out.value).view(np.int32)
+
+
+@pytest.mark.parametrize("wgh2", [0.07, -0.07, -1.0])
+def test_ctf_bitexact(ref_lib, wgh2):
+    L = O.oracle()
+    cs, wl, wgh1 = 2.7e7, 0.01969, float(np.sqrt(1 - 0.07 ** 2))
+    out_o = np.zeros(2, np.float32)
+    out_r = np.zeros(2, np.float32)
+    for trial in range(200):
+        ix, iy = int(rng.integers(-64, 65)), int(rng.integers(-64, 65))
+        df1, df2 = float(rng.uniform(1e4, 3e4)), float(rng.uniform(1e4, 3e4))
+        ang = float(rng.uniform(-3, 3))
+        th = float(rng.uniform(1e-4, 2e-4))
+        tx, ty = (0.0, 0.0) if trial % 2 else (float(rng.uniform(-1, 1)), float(rng.uniform(-1, 1)))
+        L.or32_ctf(cs, wl, wgh1, wgh2, df1, df2, ang, th, ix, iy, tx, ty, out_o)
+        args = [O.byref_f(v) for v in (cs, wl, wgh1, wgh2, df1, df2, ang, th)]
+        ref_lib.ref_ctf(*args, O.byref_i(ix), O.byref_i(iy), O.byref_f(tx), O.byref_f(ty), out_r)
+        assert np.array_equal(out_o.view(np.int32), out_r.view(np.int32)), (ix, iy)
+
+
+@pytest.mark.parametrize("nsam", [32, 64])
+@pytest.mark.parametrize("origin", [0, 1])
+def test_maskcos_bitexact(ref_lib, nsam, origin):
+    L = O.oracle()
+    img = rng.standard_normal(nsam * (nsam + 2)).astype(np.float32)
+    a, b = img.copy(), img.copy()
+    ri = nsam * 0.35
+    rih, rii = ri - 3.0, ri + 3.0
+    args = (float(rih * rih), float(rii * rii), float(rih), 6.0, 1.0)
+    (L.or32_maskcos2 if origin else L.or32_maskcos)(nsam, a, *args)
+    ref_lib.ref_maskcos(O.byref_i(nsam), b, *[O.byref_f(v) for v in args], O.byref_i(origin))
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+@pytest.mark.parametrize("nsam", [32, 64])
+@pytest.mark.parametrize("ictf", [0, 1])
+def test_pweight(ref_lib, nsam, ictf):
+    """Includes the aliased Nyquist row and the JC+1 bin quirk."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    spec = half_plane(nsam, 7)
+    ctff = np.concatenate([half_plane(nsam, 8), half_plane(nsam, 9)]) * np.complex64(0.3)
+    ssnr = rng.uniform(0.1, 2.0, jc + 1).astype(np.float32)
+    a, b = spec.copy(), spec.copy()
+    L.or32_pweight(nsam, a, a, ssnr, ctff, ictf)
+    ref_lib.ref_pweight(O.byref_i(nsam), b, ssnr, ctff, O.byref_i(ictf))
+    np.testing.assert_allclose(a, b, rtol=2e-6, atol=1e-7)
+    exact = np.mean(a.view(np.int32) == b.view(np.int32))
+    assert exact > 0.99
+
+
+@pytest.mark.parametrize("nsam", [16, 32])
+def test_ainterpo3ds(ref_lib, nsam):
+    L = O.oracle()
+    vol = half_volume(nsam, 3)
+    out_o = np.zeros(2, np.float32)
+    out_r = np.zeros(2, np.float32)
+    h = nsam / 2
+    pts = list(rng.uniform(-h, h, (300, 3)))
+    # integer and boundary coordinates: negative integers, x at the JC-1 clamp
+    pts += [np.array(p, dtype=np.float64) for p in
+            [(-2.0, -3.0, 1.0), (0.0, -h, h - 0.5), (h - 0.25, 2.0, -1.0), (-h + 0.1, 0.0, 0.0),
+             (0.0, 0.0, 0.0), (3.0, -4.0, -5.0)]]
+    for p in pts:
+        x, y, z = (float(np.float32(v)) for v in p)
+        L.or32_ainterpo3ds(nsam, vol, x, y, z, out_o)
+        ref_lib.ref_ainterpo3ds(O.byref_i(nsam), vol, O.byref_f(x), O.byref_f(y),
+                                O.byref_f(z), out_r)
+        assert np.array_equal(out_o.view(np.int32), out_r.view(np.int32)), (x, y, z)
+
+
+@pytest.mark.parametrize("nsam", [16, 32])
+def test_pextract(ref_lib, nsam):
+    L = O.oracle()
+    vol = half_volume(nsam, 5)
+    jc = nsam // 2 + 1
+    ctff = np.concatenate([half_plane(nsam, 10), half_plane(nsam, 11)])
+    for trial in range(5):
+        phi, theta, psi = (float(v) for v in rng.uniform(-np.pi, np.pi, 3))
+        a = np.zeros(nsam * jc, np.complex64)
+        b = np.zeros(nsam * jc, np.complex64)
+        L.or32_pextract(nsam, a, vol, phi, theta, psi, ctff, O.IDENTITY)
+        ref_lib.ref_pextract(O.byref_i(nsam), b, vol, O.byref_f(phi), O.byref_f(theta),
+                             O.byref_f(psi), ctff, O.IDENTITY)
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+def _insert(lib, which, nsam, spec, ctff, par, sym, occ=1.0, rrec=0.5):
+    jc = nsam // 2 + 1
+    b3 = np.zeros(nsam * nsam * jc, np.complex64)
+    t3 = np.zeros(nsam * nsam * jc, np.float32)
+    ttd = C.c_double(0.0)
+    nk = C.c_int64(0)
+    nsym, symop, jsym, _ = sym
+    phi, theta, psi, shx, shy = par
+    presa, pbc, boff, psize = 0.6, 5.0, 40.0, 1.5
+    if which == "oracle":
+        lib.or32_pinsert(nsam, rrec, 1.0, spec, ctff, b3, t3, C.byref(ttd), phi, theta, psi,
+                         shx, shy, presa, pbc, boff, nsym, jsym, symop, C.byref(nk), psize, occ,
+                         O.IDENTITY, 0)
+    else:
+        f = O.byref_f
+        lib.ref_pinsert(O.byref_i(nsam), f(rrec), f(1.0), spec, ctff, b3, t3, C.byref(ttd),
+                        f(phi), f(theta), f(psi), f(shx), f(shy), f(presa), f(pbc), f(boff),
+                        O.byref_i(nsym), jsym, symop, C.byref(nk), f(psize), f(occ), O.IDENTITY,
+                        O.byref_i(0))
+    return b3, t3, ttd.value, nk.value
+
+
+@pytest.mark.parametrize("nsam,asym,nasym", [(16, "C", 1), (32, "C", 1), (32, "D", 7), (32, "O", 0)])
+def test_pinsert(ref_lib, nsam, asym, nasym):
+    """Voxel set and count bit-exact (NKSUM), values to FP32 rounding."""
+    L = O.oracle()
+    sym = O.getsymmat(asym, nasym)
+    spec = half_plane(nsam, 21)
+    ctff = np.concatenate([half_plane(nsam, 22), half_plane(nsam, 23)]) * np.complex64(0.5)
+    for trial in range(3):
+        par = tuple(float(v) for v in rng.uniform(-np.pi, np.pi, 3)) + \
+            tuple(float(v) for v in rng.uniform(-0.3, 0.3, 2))
+        bo, to, tto, nko = _insert(L, "oracle", nsam, spec, ctff, par, sym)
+        br, tr, ttr, nkr = _insert(ref_lib, "ref", nsam, spec, ctff, par, sym)
+        assert nko == nkr
+        # the FP32 restatement reproduces the reference bit for bit
+        assert np.array_equal(bo.view(np.int32), br.view(np.int32))
+        assert np.array_equal(to.view(np.int32), tr.view(np.int32))
+        assert tto == ttr
+
+
+def test_pinsert_integer_angles_exact(ref_lib):
+    """Integer-aligned orientations land on grid points: hits the INT()/negative rule."""
+    L = O.oracle()
+    nsam = 16
+    sym = O.getsymmat("C", 1)
+    spec = half_plane(nsam, 31)
+    ctff = np.concatenate([half_plane(nsam, 32), half_plane(nsam, 33)])
+    for par in [(0.0, 0.0, 0.0, 0.0, 0.0), (np.pi / 2, np.pi / 2, 0.0, 0.1, -0.2),
+                (np.pi, 0.0, np.pi / 2, 0.0, 0.0), (0.0, np.pi, 0.0, 0.0, 0.0)]:
+        par = tuple(float(np.float32(v)) for v in par)
+        bo, to, tto, nko = _insert(L, "oracle", nsam, spec, ctff, par, sym)
+        br, tr, ttr, nkr = _insert(ref_lib, "ref", nsam, spec, ctff, par, sym)
+        assert nko == nkr
+        assert np.array_equal(to.view(np.int32), tr.view(np.int32))
+        assert np.array_equal(bo.view(np.int32), br.view(np.int32))
+
+
+@pytest.mark.parametrize("nsam", [16, 32])
+def test_storeshift(ref_lib, nsam):
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    bo, co = np.zeros(n, np.complex64), np.zeros(n, np.complex64)
+    br, cr = np.zeros(n, np.complex64), np.zeros(n, np.complex64)
+    L.or32_storeshift(nsam, 0.37, 0.41, 14, a, s, d, v, bo, co)
+    ref_lib.ref_storeshift(O.byref_i(nsam), C.byref(C.c_double(0.37)), C.byref(C.c_double(0.41)),
+                           O.byref_i(14), a, s, d, v, br, cr)
+    np.testing.assert_allclose(bo, br, rtol=1e-6)
+    np.testing.assert_allclose(co, cr, rtol=1e-6)
+
+
+@pytest.mark.parametrize("nsam,istp", [(16, 1), (32, 1), (32, 3)])
+def test_sheltest_fsc(ref_lib, nsam, istp):
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (a + 0.7 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    npo, npto = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
+    dfo, pso = np.zeros(jc + 2), np.zeros(jc + 2, np.float32)
+    nst = L.or32_sheltest(nsam, istp, a, d, s, v, 1.0, npo, npto, dfo, pso)
+    npr, nptr = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
+    dfr, fsc, psr = np.zeros(jc + 2), np.zeros(jc + 2, np.float32), np.zeros(jc + 2, np.float32)
+    nstr = C.c_int(0)
+    ref_lib.ref_sheltest(O.byref_i(nsam), O.byref_i(istp), a, d, s, v, O.byref_f(1.0), npr, nptr,
+                         dfr, fsc, psr, C.byref(nstr))
+    assert nst == nstr.value
+    assert np.array_equal(npo[:nst], npr[:nst]) and np.array_equal(npto[:nst], nptr[:nst])
+    np.testing.assert_allclose(dfo[:nst], dfr[:nst], rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(pso[:nst], psr[:nst], rtol=1e-5)
