@@ -1,0 +1,73 @@
+"""ctypes binding of libfrealign_hip.so (the C-ABI in include/frealign_hip.h).
+
+The library is built in-tree by ``python -m frealign_amd.build``.  There is no
+fallback: if the library is missing every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfrealign_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "frealign_hip.h")
+
+_lib = None
+
+vp, ci, cf, cd = C.c_void_p, C.c_int, C.c_float, C.c_double
+pi_, pf, pd = C.POINTER(C.c_int), C.POINTER(C.c_float), C.POINTER(C.c_double)
+
+SIGNATURES = {
+    "fr_version": ([], C.c_char_p),
+    "fr_last_error": ([], C.c_char_p),
+    "fr_create": ([ci, C.POINTER(vp)], ci),
+    "fr_destroy": ([vp], ci),
+    "fr_set_stream": ([vp, vp], ci),
+    "fr_synchronize": ([vp], ci),
+    "fr_search_grid": ([cf, cf, cf, C.c_char, ci, vp, ci, vp, ci, pi_, pi_], ci),
+    "fr_point_group": ([C.c_char, ci, vp, vp, pi_, pi_], ci),
+    "fr_rotation": ([ci, vp, cf, vp, vp], ci),
+    "fr_lcg_random": ([pi_], cf),
+    "fr_fft2d_r2c": ([vp, ci, ci, vp], ci),
+    "fr_fft2d_c2r": ([vp, ci, ci, vp], ci),
+    "fr_prep_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp], ci),
+    "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
+    "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
+    "fr_insert_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp, vp,
+                         vp], ci),
+    "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
+    "fr_storeshift": ([vp, ci, cd, cd, ci, vp, vp, vp, vp, vp, vp], ci),
+}
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeError(
+                f"{LIB_PATH} is missing: build it with `python -m frealign_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != 0:
+        msg = lib().fr_last_error().decode(errors="replace")
+        raise NativeError(f"{what}: {msg}" if what else msg)
+
+
+def declared_symbols() -> list[str]:
+    """Every function name declared in include/frealign_hip.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(fr_[a-z0-9_]+)\s*\(", txt)))

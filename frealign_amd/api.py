@@ -1,0 +1,261 @@
+"""Host-side mirror of Frealign's hot-path operators over the gfx950 library.
+
+The functions keep the reference's names and argument meaning (radians,
+phase-unit shifts, the RECB record of A3D3, PMASK, ...) and operate on
+device tensors (torch is used only for device memory and streams).  Every
+call goes through libfrealign_hip.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+IDENTITY_SM = np.eye(3, dtype=np.float32).reshape(-1, order="F").copy()
+
+
+def _p(t) -> int:
+    """Raw address of a tensor / ndarray (or 0 for None)."""
+    if t is None:
+        return 0
+    if isinstance(t, torch.Tensor):
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return t.data_ptr()
+    if isinstance(t, np.ndarray):
+        if not t.flags["C_CONTIGUOUS"]:
+            raise ValueError("array must be C-contiguous")
+        return t.ctypes.data
+    raise TypeError(type(t))
+
+
+def _dev(t: torch.Tensor, dtype, name: str) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return t
+
+
+class Context:
+    """One fr_ctx on one device; launches go to torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        N.check(N.lib().fr_create(device, C.byref(h)), "fr_create")
+        self.h = h
+        self.device = device
+
+    def _sync_stream(self):
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        N.check(N.lib().fr_set_stream(self.h, C.c_void_p(s)), "fr_set_stream")
+
+    def close(self):
+        if self.h:
+            N.lib().fr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- FFT
+    def fft_r2c(self, nsam: int, data: torch.Tensor):
+        _dev(data, torch.float32, "data")
+        nimg = data.numel() // (nsam * (nsam + 2))
+        self._sync_stream()
+        N.check(N.lib().fr_fft2d_r2c(self.h, nsam, nimg, _p(data)), "fr_fft2d_r2c")
+
+    def fft_c2r(self, nsam: int, data: torch.Tensor):
+        _dev(data, torch.float32, "data")
+        nimg = data.numel() // (nsam * (nsam + 2))
+        self._sync_stream()
+        N.check(N.lib().fr_fft2d_c2r(self.h, nsam, nimg, _p(data)), "fr_fft2d_c2r")
+
+    # ------------------------------------------------------------- CTFAPPLY
+    def ctfapply(self, nsam: int, images: torch.Tensor, ppar: torch.Tensor, pssnr: torch.Tensor,
+                 gpar, ictf: int = 1):
+        """LMAIN's r2c + CTFAPPLY for a batch.  images: (nb, NSAM, NSAM+2) float32,
+        destroyed.  ppar: (nb, 8) = SHX, SHY, DFMID1, DFMID2, ANGAST, AMAGP, 0, 0.
+        gpar: CS, WL, WGH1, WGH2, THETATR, RIH, HALFW, RI2, RI3, TX, TY.
+        Returns CTFF (nb, 2, NSAM, JC) c64, OUTC (nb, NSAM, JC), DATC (nb, NSAM, JC)."""
+        jc = nsam // 2 + 1
+        _dev(images, torch.float32, "images")
+        nb = images.numel() // (nsam * (nsam + 2))
+        _dev(ppar, torch.float32, "ppar")
+        _dev(pssnr, torch.float32, "pssnr")
+        if ppar.numel() != 8 * nb:
+            raise ValueError("ppar must be (nb, 8)")
+        if pssnr.numel() < jc + 1:
+            raise ValueError("pssnr needs JC+1 entries")
+        g = np.asarray(gpar, dtype=np.float32)
+        if g.size != 11:
+            raise ValueError("gpar needs 11 values")
+        dev = images.device
+        ctff = torch.empty((nb, 2, nsam, jc), dtype=torch.complex64, device=dev)
+        outc = torch.empty((nb, nsam, jc), dtype=torch.complex64, device=dev)
+        datc = torch.empty((nb, nsam, jc), dtype=torch.complex64, device=dev)
+        self._sync_stream()
+        N.check(N.lib().fr_prep_batch(self.h, nsam, nb, _p(images), _p(ppar), _p(pssnr), _p(g),
+                                      ictf, _p(ctff), _p(outc), _p(datc)), "fr_prep_batch")
+        return ctff, outc, datc
+
+    # ------------------------------------------------------------- PEXTRACT
+    def pextract(self, nsam: int, a3df: torch.Tensor, dm: torch.Tensor, ctff: torch.Tensor,
+                 pidx: torch.Tensor | None = None) -> torch.Tensor:
+        jc = nsam // 2 + 1
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(dm, torch.float32, "dm")
+        _dev(ctff, torch.complex64, "ctff")
+        n = dm.numel() // 9
+        npart = ctff.numel() // (2 * nsam * jc)
+        if pidx is not None:
+            _dev(pidx, torch.int32, "pidx")
+            if pidx.numel() != n or (n and int(pidx.max()) >= npart) or (n and int(pidx.min()) < 0):
+                raise ValueError("pidx out of range")
+        elif n > npart:
+            raise ValueError("more sections than CTF tables")
+        if a3df.numel() != jc * nsam * nsam:
+            raise ValueError("a3df must be a JC x NSAM x NSAM half-volume")
+        out = torch.empty((n, nsam, jc), dtype=torch.complex64, device=a3df.device)
+        self._sync_stream()
+        N.check(N.lib().fr_pextract_batch(self.h, nsam, n, _p(a3df), _p(dm), _p(pidx), _p(ctff),
+                                          _p(out)), "fr_pextract_batch")
+        return out
+
+    # ---------------------------------------------------------------- CC3M
+    def cc3m(self, nsam: int, a3df, spec, ctff, pidx, dm, shift, amag, maxr1: int, maxr2: int,
+             rbfact: float, fsct, ring) -> torch.Tensor:
+        jc = nsam // 2 + 1
+        for t, nm, dt in ((a3df, "a3df", torch.complex64), (spec, "spec", torch.complex64),
+                          (ctff, "ctff", torch.complex64), (pidx, "pidx", torch.int32),
+                          (dm, "dm", torch.float32), (shift, "shift", torch.float32),
+                          (amag, "amag", torch.float32), (fsct, "fsct", torch.float32)):
+            _dev(t, dt, nm)
+        n = pidx.numel()
+        npart = spec.numel() // (nsam * jc)
+        if dm.numel() != 9 * n or shift.numel() != 2 * n or amag.numel() != n:
+            raise ValueError("dm/shift/amag sizes do not match pidx")
+        if n and (int(pidx.max()) >= npart or int(pidx.min()) < 0):
+            raise ValueError("pidx out of range")
+        if ctff.numel() != 2 * spec.numel() or fsct.numel() < nsam // 2 + 1:
+            raise ValueError("ctff/fsct sizes")
+        r = np.asarray(ring, dtype=np.float32)
+        out = torch.empty(n, dtype=torch.float32, device=a3df.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cc3m_batch(self.h, nsam, n, _p(a3df), _p(spec), _p(ctff), _p(pidx),
+                                      _p(dm), _p(shift), _p(amag), maxr1, maxr2, float(rbfact),
+                                      _p(fsct), _p(r), _p(out)), "fr_cc3m_batch")
+        return out
+
+    # ------------------------------------------------------------ insertion
+    def insert(self, nsam: int, datc, ctff, recb: np.ndarray, half: np.ndarray, sym, pbc: float,
+               boff: float, psize: float, vols: "HalfMaps"):
+        _dev(datc, torch.complex64, "datc")
+        _dev(ctff, torch.complex64, "ctff")
+        recb = np.ascontiguousarray(recb, dtype=np.float32)
+        half = np.ascontiguousarray(half, dtype=np.int32)
+        nb = recb.shape[0]
+        jc = nsam // 2 + 1
+        if datc.numel() < nb * nsam * jc or ctff.numel() < 2 * nb * nsam * jc:
+            raise ValueError("datc/ctff hold fewer particles than recb")
+        if half.shape != (nb,):
+            raise ValueError("half must be (nb,)")
+        nsym, symop, jsym, _ = sym
+        self._sync_stream()
+        N.check(N.lib().fr_insert_batch(
+            self.h, nsam, nb, _p(datc), _p(ctff), _p(recb), _p(half), nsym, _p(jsym), _p(symop),
+            float(pbc), float(boff), float(psize), _p(vols.num[0]), _p(vols.ctf2[0]),
+            _p(vols.num[1]), _p(vols.ctf2[1]), _p(vols.nksum), _p(vols.ttd)), "fr_insert_batch")
+
+    # ----------------------------------------------------------- statistics
+    def sheltest(self, nsam: int, a, d, s, v, istp: int = 1, fmask: float = 1.0):
+        jc = nsam // 2 + 1
+        cap = jc + 2
+        npix = np.zeros(cap, np.int32)
+        npixt = np.zeros(cap, np.int32)
+        dfsc = np.zeros(cap, np.float64)
+        pssnr = np.zeros(cap, np.float32)
+        nstp = C.c_int(0)
+        self._sync_stream()
+        N.check(N.lib().fr_sheltest(self.h, nsam, istp, _p(a), _p(d), _p(s), _p(v), float(fmask),
+                                    _p(npix), _p(npixt), _p(dfsc), _p(pssnr), C.byref(nstp)),
+                "fr_sheltest")
+        k = nstp.value
+        return dict(npix=npix[:k], npixt=npixt[:k], fsc=dfsc[:k], pssnr=pssnr[:k])
+
+    def storeshift(self, nsam: int, std: float, vtd: float, iredun: int, a, s, d, v):
+        b = torch.empty_like(a)
+        c = torch.empty_like(d)
+        self._sync_stream()
+        N.check(N.lib().fr_storeshift(self.h, nsam, float(std), float(vtd), int(iredun), _p(a),
+                                      _p(s), _p(d), _p(v), _p(b), _p(c)), "fr_storeshift")
+        return b, c
+
+
+@dataclass
+class HalfMaps:
+    """The two half-map accumulators of A3D3 (A3DF/S3DF and D3DF/V3DF) with
+    their NKSUM1 / TTD1 scalars, resident on the device."""
+    nsam: int
+    num: tuple
+    ctf2: tuple
+    nksum: torch.Tensor
+    ttd: torch.Tensor
+
+    @staticmethod
+    def zeros(nsam: int, device="cuda") -> "HalfMaps":
+        jc = nsam // 2 + 1
+        shp = (nsam, nsam, jc)
+        num = tuple(torch.zeros(shp, dtype=torch.complex64, device=device) for _ in range(2))
+        ctf2 = tuple(torch.zeros(shp, dtype=torch.float32, device=device) for _ in range(2))
+        return HalfMaps(nsam, num, ctf2, torch.zeros(2, dtype=torch.int64, device=device),
+                        torch.zeros(2, dtype=torch.float64, device=device))
+
+
+# ---------------------------------------------------------------- geometry
+def search_grid(dang: float, asym: str = "C", nasym: int = 1, pmask=(1, 1, 1, 1, 1),
+                rmm: float = 0.1, ri: float = 50.0, asymstore: bool = True):
+    """SEARCHANG (searchang.f): returns (sang (n,3) radians float32, iquadmax)."""
+    m = np.asarray(pmask, dtype=np.int32)
+    n, iq = C.c_int(0), C.c_int(0)
+    N.check(N.lib().fr_search_grid(rmm, ri, dang, asym.encode(), nasym, _p(m), int(asymstore),
+                                   None, 0, C.byref(n), C.byref(iq)), "fr_search_grid")
+    sang = np.zeros((max(n.value, 1), 3), np.float32)
+    N.check(N.lib().fr_search_grid(rmm, ri, dang, asym.encode(), nasym, _p(m), int(asymstore),
+                                   _p(sang), n.value, C.byref(n), C.byref(iq)), "fr_search_grid")
+    return sang[: n.value], iq.value
+
+
+def point_group(asym: str, nasym: int):
+    """GETSYMMAT: (nsym, symop (72,) float32, jsym (8,) int32, iredun)."""
+    symop = np.zeros(72, np.float32)
+    jsym = np.zeros(8, np.int32)
+    nsym, ired = C.c_int(0), C.c_int(0)
+    N.check(N.lib().fr_point_group(asym.encode(), nasym, _p(symop), _p(jsym), C.byref(nsym),
+                                   C.byref(ired)), "fr_point_group")
+    return nsym.value, symop, jsym, ired.value
+
+
+def rotation(euler, amag: float = 1.0, sm=IDENTITY_SM) -> np.ndarray:
+    """DM matrices (n, 9) of cc3m.f:48-65 (amag=0: pextract.f form)."""
+    e = np.ascontiguousarray(np.asarray(euler, dtype=np.float32).reshape(-1, 3))
+    dm = np.zeros((e.shape[0], 9), np.float32)
+    smv = None if sm is None else np.ascontiguousarray(sm, dtype=np.float32)
+    N.check(N.lib().fr_rotation(e.shape[0], _p(e), float(amag), _p(smv) if smv is not None else None,
+                                _p(dm)), "fr_rotation")
+    return dm
+
+
+def lcg_random(iran: int):
+    s = C.c_int(iran)
+    v = N.lib().fr_lcg_random(C.byref(s))
+    return v, s.value

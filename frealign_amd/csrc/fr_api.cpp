@@ -1,0 +1,349 @@
+// fr_api.cpp -- the C-ABI entry points (declared in include/frealign_hip.h).
+//
+// Every entry point validates its shapes on the host before any launch,
+// returns 0 on success or a negative status with a message retrievable from
+// fr_last_error(); nothing falls back to a CPU path.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/frealign_hip.h"
+#include "fr_common.h"
+#include "fr_host.h"
+#include "fr_kernels.h"
+
+struct fr_ctx : public fr::Ctx {
+  std::vector<fr::InsertTask> host_tasks;
+  void* slots[4] = {};
+  size_t slot_bytes[4] = {};
+  void* slot(int k, size_t bytes, int* err);
+};
+
+namespace fr {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int fail(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return -1;
+}
+
+int Ctx::ensure_scratch(size_t bytes) {
+  if (bytes <= scratch_bytes) return 0;
+  if (scratch) {
+    FR_HIP(hipStreamSynchronize(stream));
+    FR_HIP(hipFree(scratch));
+    scratch = nullptr;
+    scratch_bytes = 0;
+  }
+  FR_HIP(hipMalloc(&scratch, bytes));
+  scratch_bytes = bytes;
+  return 0;
+}
+
+const float2* Ctx::twiddles(int n) {
+  if (n < 1 || n > 1024) return nullptr;
+  if (tw[n]) return tw[n];
+  std::vector<float2> h(n);
+  for (int k = 0; k < n; k++) {
+    double a = -2.0 * M_PI * (double)k / (double)n;
+    h[k] = make_float2((float)cos(a), (float)sin(a));
+  }
+  float2* d = nullptr;
+  if (hipMalloc(&d, sizeof(float2) * n) != hipSuccess) return nullptr;
+  if (hipMemcpy(d, h.data(), sizeof(float2) * n, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return nullptr;
+  }
+  tw[n] = d;
+  return d;
+}
+
+Ctx::~Ctx() {
+  if (scratch) (void)hipFree(scratch);
+  for (auto* p : tw)
+    if (p) (void)hipFree(p);
+}
+
+}  // namespace fr
+
+void* fr_ctx::slot(int k, size_t bytes, int* err) {
+  if (bytes <= slot_bytes[k]) return slots[k];
+  if (slots[k]) {
+    if (hipStreamSynchronize(stream) != hipSuccess || hipFree(slots[k]) != hipSuccess) {
+      *err = fr::fail("scratch release failed");
+      return nullptr;
+    }
+    slots[k] = nullptr;
+    slot_bytes[k] = 0;
+  }
+  if (hipMalloc(&slots[k], bytes) != hipSuccess) {
+    *err = fr::fail("hipMalloc of %zu scratch bytes failed", bytes);
+    return nullptr;
+  }
+  slot_bytes[k] = bytes;
+  return slots[k];
+}
+
+using namespace fr;
+
+#define CTX_CHECK(c) FR_CHECK((c) != nullptr, "null fr_ctx")
+#define SIZE_CHECK(nsam) \
+  FR_CHECK(fft_supported(nsam) && (nsam) >= 8 && (nsam) <= 1024, "unsupported box size %d", nsam)
+
+extern "C" {
+
+const char* fr_version(void) { return "frealign-mi355x 0.1 (gfx950)"; }
+const char* fr_last_error(void) { return g_err.c_str(); }
+
+int fr_create(int device, fr_ctx** out) {
+  FR_CHECK(out != nullptr, "null output pointer");
+  int n = 0;
+  FR_HIP(hipGetDeviceCount(&n));
+  FR_CHECK(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+  FR_HIP(hipSetDevice(device));
+  fr_ctx* c = new fr_ctx();
+  c->device = device;
+  c->stream = nullptr;
+  *out = c;
+  return 0;
+}
+
+int fr_destroy(fr_ctx* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (int k = 0; k < 4; k++)
+    if (c->slots[k]) (void)hipFree(c->slots[k]);
+  delete c;
+  return 0;
+}
+
+int fr_set_stream(fr_ctx* c, void* stream) {
+  CTX_CHECK(c);
+  c->stream = reinterpret_cast<hipStream_t>(stream);
+  return 0;
+}
+
+int fr_synchronize(fr_ctx* c) {
+  CTX_CHECK(c);
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+// ------------------------------------------------------------ host geometry
+int fr_search_grid(float rmm, float ri, float dangin, char asym, int nasym, const int* pmask,
+                   int asymstore, float* sang, int cap, int* nsang, int* iquadmax) {
+  FR_CHECK(pmask && nsang && iquadmax, "null argument");
+  int n = search_grid(rmm, ri, dangin, asym, nasym, pmask, asymstore, sang, cap, iquadmax);
+  FR_CHECK(n >= 0, "invalid symmetry '%c'", asym);
+  *nsang = n;
+  return 0;
+}
+
+int fr_point_group(char asym, int nasym, float* symop, int* jsym, int* nsym, int* iredun) {
+  FR_CHECK(symop && jsym && nsym && iredun, "null argument");
+  FR_CHECK(point_group(asym, nasym, symop, jsym, nsym, iredun) == 0,
+           "invalid point group %c%d", asym, nasym);
+  return 0;
+}
+
+int fr_rotation(int n, const float* euler, float amag, const float* sm, float* dm) {
+  FR_CHECK(n >= 0 && euler && dm, "null argument");
+  for (int i = 0; i < n; i++)
+    rotation(euler[3 * i], euler[3 * i + 1], euler[3 * i + 2], amag, sm, dm + 9 * i);
+  return 0;
+}
+
+float fr_lcg_random(int* iran) { return lcg_random(iran); }
+
+// ------------------------------------------------------------------- FFTs
+int fr_fft2d_r2c(fr_ctx* c, int nsam, int nimg, float* d_data) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  return fft_r2c_2d(*c, nsam, nimg, d_data, (size_t)nsam * (nsam + 2));
+}
+
+int fr_fft2d_c2r(fr_ctx* c, int nsam, int nimg, float* d_data) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  return fft_c2r_2d(*c, nsam, nimg, d_data, (size_t)nsam * (nsam + 2));
+}
+
+// ------------------------------------------------------- particle preparation
+int fr_prep_batch(fr_ctx* c, int nsam, int nb, float* d_data, const float* d_ppar,
+                  const float* d_pssnr, const float* gpar, int ictf, float* d_ctff,
+                  float* d_outc, float* d_datc) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(nb >= 0 && gpar && d_data && d_ppar && d_pssnr && d_ctff && d_outc && d_datc,
+           "null argument");
+  FR_CHECK(d_outc != d_data, "OUTC must not alias the image buffer");
+  PrepGlobals g{gpar[0], gpar[1], gpar[2], gpar[3], gpar[4], gpar[5], gpar[6],
+                gpar[7], gpar[8], gpar[9], gpar[10], ictf};
+  FR_CHECK(g.halfw != 0.0f, "HALFW must be non-zero");
+  return launch_prep(*c, nsam, nb, d_data, d_ppar, d_pssnr, g, d_ctff, d_outc, d_datc);
+}
+
+// ------------------------------------------------------ extraction / scoring
+int fr_pextract_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_dm,
+                      const int* d_pidx, const float* d_ctff, float* d_outc) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(n >= 0 && d_a3df && d_dm && d_ctff && d_outc, "null argument");
+  return launch_pextract(*c, nsam, n, d_a3df, d_dm, d_pidx, d_ctff, d_outc);
+}
+
+int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                  const float* d_ctff, const int* d_pidx, const float* d_dm,
+                  const float* d_shift, const float* d_amag, int maxr1, int maxr2, float rbfact,
+                  const float* d_fsct, const float* ring, float* d_score) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(n >= 0 && d_a3df && d_spec && d_ctff && d_pidx && d_dm && d_shift && d_amag &&
+               d_fsct && ring && d_score,
+           "null argument");
+  FR_CHECK(maxr1 >= 0 && maxr2 > maxr1, "bad resolution band %d..%d", maxr1, maxr2);
+  if (n == 0) return 0;
+  const size_t img = (size_t)nsam * (nsam + 2);
+  // process in chunks so the FFT work buffer stays bounded
+  const int chunk = (int)((size_t)512 * 1024 * 1024 / (img * sizeof(float)));
+  const int step = chunk < 1 ? 1 : (chunk > 65535 ? 65535 : chunk);
+  int err = 0;
+  float* work = (float*)c->slot(0, img * sizeof(float) * (size_t)(n < step ? n : step), &err);
+  if (!work) return err;
+  Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
+  for (int o = 0; o < n; o += step) {
+    a.n = n - o < step ? n - o : step;
+    if (int r = launch_cc3m(*c, a, d_a3df, d_spec, d_ctff, d_pidx + o, d_dm + 9 * (size_t)o,
+                            d_shift + 2 * (size_t)o, d_amag + o, d_fsct, work, d_score + o))
+      return r;
+  }
+  return 0;
+}
+
+// -------------------------------------------------------------- insertion
+int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const float* d_ctff,
+                    const float* recb, const int* half, int nsym, const int* jsym,
+                    const float* symop, float pbc, float boff, float psize, float* d_num0,
+                    float* d_ctf2_0, float* d_num1, float* d_ctf2_1, long long* d_nksum,
+                    double* d_ttd) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(nb >= 0 && d_datc && d_ctff && recb && half && jsym && symop && d_num0 && d_ctf2_0 &&
+               d_num1 && d_ctf2_1 && d_nksum && d_ttd,
+           "null argument");
+  FR_CHECK(nsym >= 1 && nsym <= 8, "nsym %d out of range", nsym);
+  FR_CHECK(pbc != 0.0f && psize != 0.0f, "PBC and PSIZE must be non-zero");
+  int ncopy = 1;
+  for (int k = 0; k < nsym; k++) {
+    FR_CHECK(jsym[k] >= 1 && jsym[k] <= 60, "bad rotational order %d", jsym[k]);
+    ncopy *= jsym[k];
+  }
+  FR_HIP(hipStreamSynchronize(c->stream));  // host task buffer reuse
+  auto& tasks = c->host_tasks;
+  tasks.clear();
+  const int nsamh = nsam / 2;
+  std::vector<float> copies(9 * (size_t)ncopy);
+  for (int b = 0; b < nb; b++) {
+    const float* r = recb + 10 * (size_t)b;
+    FR_CHECK(half[b] == 0 || half[b] == 1, "half-map flag must be 0 or 1");
+    FR_CHECK(r[7] != 0.0f, "SIG2N of particle %d is zero", b);
+    float on = r[6] / r[7];  // a3d3_s.f:44
+    if (on == 0.0f) continue;  // pinsert.f:40
+    FR_CHECK(r[5] != 0.0f, "AMAG of particle %d is zero", b);
+    // pinsert.f:42-62
+    float pref = pbc / 100.0f;
+    int irrec = (int)((float)nsam * r[9]);
+    FR_CHECK(irrec >= 0 && irrec <= nsam, "RREC of particle %d out of range", b);
+    float tmp = (r[8] / pref - boff / pbc) / (psize * psize);
+    if (tmp > 2.0f) tmp = 2.0f;
+    tmp = 2.0f * tmp / ((float)nsamh * (float)nsamh);
+    float tm[9];
+    rotation(r[0], r[1], r[2], r[5], nullptr, tm);
+    int nc = symmetry_copies(tm, nsym, jsym, symop, copies.data());
+    FR_CHECK(nc == ncopy, "symmetry enumeration mismatch");
+    for (int k = 0; k < nc; k++) {
+      InsertTask t;
+      memcpy(t.dm, copies.data() + 9 * k, sizeof t.dm);
+      t.shx = r[3];
+      t.shy = r[4];
+      t.tmp = tmp;
+      t.occ = on;
+      t.part = b;
+      t.half = half[b];
+      t.irrec2 = irrec * irrec;
+      t.pad_ = 0;
+      tasks.push_back(t);
+    }
+  }
+  if (tasks.empty()) return 0;
+  int err = 0;
+  InsertTask* d_tasks = (InsertTask*)c->slot(1, sizeof(InsertTask) * tasks.size(), &err);
+  if (!d_tasks) return err;
+  FR_HIP(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(InsertTask) * tasks.size(),
+                        hipMemcpyHostToDevice, c->stream));
+  return launch_insert(*c, nsam, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0, d_ctf2_0,
+                       d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum), d_ttd);
+}
+
+// ------------------------------------------------------------- statistics
+int fr_sheltest(fr_ctx* c, int nsam, int istp, const float* d_a, const float* d_d,
+                const float* d_s, const float* d_v, float fmask, int* npix, int* npixt,
+                double* dfsc, float* pssnr, int* nstp) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(istp >= 1 && d_a && d_d && d_s && d_v && npix && npixt && dfsc && pssnr && nstp,
+           "null argument");
+  const int jc = nsam / 2 + 1;
+  int ns = jc / istp;
+  if (ns * istp < jc) ns++;
+  int err = 0;
+  double* acc = (double*)c->slot(2, sizeof(double) * 6 * ns, &err);
+  if (!acc) return err;
+  if (int r = launch_shell_sums(*c, nsam, istp, ns, d_a, d_d, d_s, d_v, acc)) return r;
+  std::vector<double> h(6 * (size_t)ns);
+  FR_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                        c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  for (int i = 0; i < ns; i++) {  // sheltest.f:90-112
+    const double* a = h.data() + 6 * i;
+    npix[i] = (int)a[4];
+    npixt[i] = (int)a[5];
+    double d = a[0];
+    float ps = (float)a[3];
+    if (npix[i] != 0) {
+      d = d / sqrt(a[1] * a[2]);
+      float f = (float)d;
+      float a1 = fabsf(f) < 1.0f ? (float)(2.0 * fabs(d) / (1.0 - fabs(d))) : 1000.0f;
+      if (ps != 0.0f) ps = a1 * (float)npix[i] / ps * fmask;
+    }
+    dfsc[i] = d;
+    pssnr[i] = ps;
+  }
+  *nstp = ns;
+  return 0;
+}
+
+int fr_storeshift(fr_ctx* c, int nsam, double std_, double vtd, int iredun, const float* d_a,
+                  const float* d_s, const float* d_d, const float* d_v, float* d_b, float* d_c) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a && d_s && d_d && d_v && d_b && d_c, "null argument");
+  FR_CHECK(std_ != 0.0 && vtd != 0.0, "STD or VTD zero: no acceptable particles");
+  return launch_storeshift(*c, nsam, std_, vtd, iredun, d_a, d_s, d_d, d_v, d_b, d_c);
+}
+
+}  // extern "C"

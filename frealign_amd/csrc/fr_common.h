@@ -1,0 +1,63 @@
+// fr_common.h -- shared definitions for the gfx950 kernels and the C-ABI glue.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+namespace fr {
+
+// ---------------------------------------------------------------- errors
+void set_error(const std::string& msg);
+int fail(const char* fmt, ...);
+#define FR_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) return ::fr::fail("%s:%d %s: %s", __FILE__, __LINE__,   \
+                                            #call, hipGetErrorString(e_));        \
+  } while (0)
+#define FR_CHECK(cond, ...)                                                       \
+  do {                                                                            \
+    if (!(cond)) return ::fr::fail(__VA_ARGS__);                                  \
+  } while (0)
+
+// ---------------------------------------------------------------- context
+struct Ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  // grow-only device scratch owned by the context
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // FFT twiddle tables, one per length (W_N^k = e^{-2 pi i k/N}, k < N)
+  float2* tw[1025] = {};
+  int ensure_scratch(size_t bytes);
+  const float2* twiddles(int n);
+  ~Ctx();
+};
+
+// ---------------------------------------------------------------- complex
+struct cf {
+  float re, im;
+};
+__host__ __device__ inline cf cmake(float a, float b) { return cf{a, b}; }
+__host__ __device__ inline cf cmul(cf a, cf b) {
+  return cf{a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__host__ __device__ inline cf cconj(cf a) { return cf{a.re, -a.im}; }
+__host__ __device__ inline cf cadd(cf a, cf b) { return cf{a.re + b.re, a.im + b.im}; }
+__host__ __device__ inline cf csub(cf a, cf b) { return cf{a.re - b.re, a.im - b.im}; }
+__host__ __device__ inline cf cscale(cf a, float s) { return cf{a.re * s, a.im * s}; }
+
+// Wrapped frequency index -> storage row (JJ=J+1; IF (JJ.LT.1) JJ=JJ+NSAM, 0-based)
+__host__ __device__ inline int wrapn(int j, int n) { return j < 0 ? j + n : j; }
+
+// ---------------------------------------------------------------- FFT API (fr_fft.hip)
+// Batched 2D real FFTs in the reference's in-place layout: nimg images of
+// NSAM x NSAM reals with row pitch NSAM+2, consecutive images `img_stride`
+// floats apart.  Forward is unnormalised e^{-i}; backward is unnormalised e^{+i}
+// with the row-column c2r semantic (imaginary parts of the x=0 and x=NSAM/2
+// columns ignored after the column pass).
+int fft_r2c_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride);
+int fft_c2r_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride);
+bool fft_supported(int nsam);
+
+}  // namespace fr

@@ -1,0 +1,305 @@
+// fr_fft.hip -- batched 2D real FFTs for the particle and projection planes.
+//
+// Replaces the FFTW 3.3 plans of fftw_plans.f:12-13 (fftwf_plan_dft_r2c_2d /
+// fftwf_plan_dft_c2r_2d, in place, unnormalised) for the box sizes the hot
+// path uses: N = 2^a 3^b 5^c, N <= 1024 (64, 96, 128, 160, 256 and the 4x
+// padded 1024 of pad.f).
+//
+// Structure: row pass (two real rows packed into one complex sequence),
+// column pass over the JC half-plane columns, both as Stockham autosort FFTs
+// in LDS with radix-4/2/3/5 butterflies and an accurate twiddle table
+// (W_N^k computed on the host in FP64).
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "fr_common.h"
+
+namespace fr {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kMaxPass = 12;
+
+struct Plan {
+  int n;
+  int npass;
+  int radix[kMaxPass];
+};
+
+bool make_plan(int n, Plan* p) {
+  if (n < 2 || n > 1024) return false;
+  p->n = n;
+  p->npass = 0;
+  int m = n;
+  while (m % 4 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 4; m /= 4; }
+  while (m % 2 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 2; m /= 2; }
+  while (m % 3 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 3; m /= 3; }
+  while (m % 5 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 5; m /= 5; }
+  return m == 1;
+}
+
+__device__ inline float2 f2(float a, float b) { return make_float2(a, b); }
+__device__ inline float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+__device__ inline float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+__device__ inline float2 mul2(float2 a, float2 b) {
+  return f2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline float2 scl2(float2 a, float s) { return f2(a.x * s, a.y * s); }
+// multiply by i*sign
+__device__ inline float2 muli(float2 a, int sign) { return sign > 0 ? f2(-a.y, a.x) : f2(a.y, -a.x); }
+
+template <int R>
+__device__ inline void butterfly(float2* u, int sign) {
+  if constexpr (R == 2) {
+    float2 a = u[0], b = u[1];
+    u[0] = add2(a, b);
+    u[1] = sub2(a, b);
+  } else if constexpr (R == 4) {
+    float2 a0 = add2(u[0], u[2]), a1 = sub2(u[0], u[2]);
+    float2 b0 = add2(u[1], u[3]), jt = muli(sub2(u[1], u[3]), sign);
+    u[0] = add2(a0, b0);
+    u[2] = sub2(a0, b0);
+    u[1] = add2(a1, jt);
+    u[3] = sub2(a1, jt);
+  } else if constexpr (R == 3) {
+    const float h = 0.86602540378443864676f;
+    float2 s = add2(u[1], u[2]), d = sub2(u[1], u[2]);
+    float2 v0 = add2(u[0], s);
+    float2 m = sub2(u[0], scl2(s, 0.5f));
+    float2 nn = scl2(muli(d, sign), h);
+    u[0] = v0;
+    u[1] = add2(m, nn);
+    u[2] = sub2(m, nn);
+  } else if constexpr (R == 5) {
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    float2 a1 = add2(u[1], u[4]), b1 = sub2(u[1], u[4]);
+    float2 a2 = add2(u[2], u[3]), b2 = sub2(u[2], u[3]);
+    float2 v0 = add2(add2(u[0], a1), a2);
+    float2 p1 = add2(u[0], add2(scl2(a1, c1), scl2(a2, c2)));
+    float2 q1 = muli(add2(scl2(b1, s1), scl2(b2, s2)), sign);
+    float2 p2 = add2(u[0], add2(scl2(a1, c2), scl2(a2, c1)));
+    float2 q2 = muli(sub2(scl2(b1, s2), scl2(b2, s1)), sign);
+    u[0] = v0;
+    u[1] = add2(p1, q1);
+    u[4] = sub2(p1, q1);
+    u[2] = add2(p2, q2);
+    u[3] = sub2(p2, q2);
+  }
+}
+
+template <int R>
+__device__ inline void stockham_pass(const float2* in, float2* out, int n, int ld, int nseq, int p,
+                                     int sign, const float2* __restrict__ tw) {
+  const int T = n / R;
+  const int stride = n / (p * R);
+  for (int b = threadIdx.x; b < nseq * T; b += blockDim.x) {
+    int s = b / T, i = b - s * T;
+    int k = i % p;
+    const float2* x = in + s * ld;
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = x[i + r * T];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      int e = (r * k * stride) % n;
+      float2 w = tw[e];
+      if (sign > 0) w.y = -w.y;
+      u[r] = mul2(u[r], w);
+    }
+    butterfly<R>(u, sign);
+    float2* y = out + s * ld;
+    int base = (i - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; q++) y[base + q * p] = u[q];
+  }
+}
+
+// In-LDS FFT of nseq sequences (stride ld); returns the buffer holding the result.
+__device__ float2* lds_fft(float2* buf, float2* tmp, const Plan& pl, int ld, int nseq, int sign,
+                           const float2* __restrict__ tw) {
+  int p = 1;
+  for (int ps = 0; ps < pl.npass; ps++) {
+    int R = pl.radix[ps];
+    switch (R) {
+      case 4: stockham_pass<4>(buf, tmp, pl.n, ld, nseq, p, sign, tw); break;
+      case 2: stockham_pass<2>(buf, tmp, pl.n, ld, nseq, p, sign, tw); break;
+      case 3: stockham_pass<3>(buf, tmp, pl.n, ld, nseq, p, sign, tw); break;
+      default: stockham_pass<5>(buf, tmp, pl.n, ld, nseq, p, sign, tw); break;
+    }
+    __syncthreads();
+    float2* t = buf;
+    buf = tmp;
+    tmp = t;
+    p *= R;
+  }
+  return buf;
+}
+
+// ---- row pass, forward: rows (2q, 2q+1) -> z = a + i b -> FFT -> unpack to half spectra
+__global__ void __launch_bounds__(kThreads) k_rows_r2c(float* data, Plan pl, int nrows_per_img,
+                                                       size_t img_stride, int npairs, int nseq,
+                                                       const float2* __restrict__ tw) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n, ld = n + 1;
+  float2* buf = smem;
+  float2* tmp = smem + nseq * ld;
+  const int pair0 = blockIdx.x * nseq;
+  const int pitch = n + 2;
+  for (int e = threadIdx.x; e < nseq * n; e += blockDim.x) {
+    int s = e / n, x = e - s * n;
+    int pr = pair0 + s;
+    float2 v = f2(0.f, 0.f);
+    if (pr < npairs) {
+      int img = (2 * pr) / nrows_per_img, row = (2 * pr) % nrows_per_img;
+      const float* ra = data + img * img_stride + (size_t)row * pitch;
+      v = f2(ra[x], ra[x + pitch]);
+    }
+    buf[s * ld + x] = v;
+  }
+  __syncthreads();
+  float2* res = lds_fft(buf, tmp, pl, ld, nseq, -1, tw);
+  const int jc = n / 2 + 1;
+  for (int e = threadIdx.x; e < nseq * jc; e += blockDim.x) {
+    int s = e / jc, k = e - s * jc;
+    int pr = pair0 + s;
+    if (pr >= npairs) continue;
+    float2 zk = res[s * ld + k];
+    float2 zm = res[s * ld + ((n - k) % n)];
+    // Xa = (Z_k + conj Z_-k)/2 ; Xb = (Z_k - conj Z_-k)/(2i)
+    float2 xa = f2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    float2 xb = f2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    int img = (2 * pr) / nrows_per_img, row = (2 * pr) % nrows_per_img;
+    float* ra = data + img * img_stride + (size_t)row * pitch;
+    ra[2 * k] = xa.x;
+    ra[2 * k + 1] = xa.y;
+    ra[pitch + 2 * k] = xb.x;
+    ra[pitch + 2 * k + 1] = xb.y;
+  }
+}
+
+// ---- row pass, backward: half spectra of rows (2q, 2q+1) -> real rows
+__global__ void __launch_bounds__(kThreads) k_rows_c2r(float* data, Plan pl, int nrows_per_img,
+                                                       size_t img_stride, int npairs, int nseq,
+                                                       const float2* __restrict__ tw) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n, ld = n + 1, jc = n / 2 + 1, pitch = n + 2;
+  float2* buf = smem;
+  float2* tmp = smem + nseq * ld;
+  const int pair0 = blockIdx.x * nseq;
+  for (int e = threadIdx.x; e < nseq * jc; e += blockDim.x) {
+    int s = e / jc, k = e - s * jc;
+    int pr = pair0 + s;
+    float2 a = f2(0.f, 0.f), b = f2(0.f, 0.f);
+    if (pr < npairs) {
+      int img = (2 * pr) / nrows_per_img, row = (2 * pr) % nrows_per_img;
+      const float* ra = data + img * img_stride + (size_t)row * pitch;
+      a = f2(ra[2 * k], ra[2 * k + 1]);
+      b = f2(ra[pitch + 2 * k], ra[pitch + 2 * k + 1]);
+    }
+    if (k == 0 || 2 * k == n) { a.y = 0.f; b.y = 0.f; }
+    // Z_k = A_k + i B_k ; Z_{n-k} = conj(A_k) + i conj(B_k)
+    buf[s * ld + k] = f2(a.x - b.y, a.y + b.x);
+    if (k > 0 && 2 * k < n) buf[s * ld + (n - k)] = f2(a.x + b.y, -a.y + b.x);
+  }
+  __syncthreads();
+  float2* res = lds_fft(buf, tmp, pl, ld, nseq, +1, tw);
+  for (int e = threadIdx.x; e < nseq * pitch; e += blockDim.x) {
+    int s = e / pitch, x = e - s * pitch;
+    int pr = pair0 + s;
+    if (pr >= npairs) continue;
+    int img = (2 * pr) / nrows_per_img, row = (2 * pr) % nrows_per_img;
+    float* ra = data + img * img_stride + (size_t)row * pitch;
+    float2 z = x < n ? res[s * ld + x] : f2(0.f, 0.f);
+    ra[x] = z.x;
+    ra[pitch + x] = z.y;
+  }
+}
+
+// ---- column pass over the half-plane: JC columns of length n (complex, stride JC)
+__global__ void __launch_bounds__(kThreads) k_cols(float* data, Plan pl, size_t img_stride, int cw,
+                                                   int sign, const float2* __restrict__ tw) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n, ld = n + 1, jc = n / 2 + 1;
+  float2* buf = smem;
+  float2* tmp = smem + cw * ld;
+  const int x0 = blockIdx.x * cw;
+  const int ncol = min(cw, jc - x0);
+  float2* base = reinterpret_cast<float2*>(data + blockIdx.y * img_stride);
+  for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
+    int y = e / cw, c = e - y * cw;
+    float2 v = f2(0.f, 0.f);
+    if (c < ncol) v = base[(size_t)y * jc + x0 + c];
+    buf[c * ld + y] = v;
+  }
+  __syncthreads();
+  float2* res = lds_fft(buf, tmp, pl, ld, cw, sign, tw);
+  for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
+    int y = e / cw, c = e - y * cw;
+    if (c < ncol) base[(size_t)y * jc + x0 + c] = res[c * ld + y];
+  }
+}
+
+int row_seqs(int n) {
+  int s = 2048 / n;
+  return s < 1 ? 1 : s;
+}
+
+}  // namespace
+
+bool fft_supported(int nsam) {
+  Plan p;
+  return (nsam % 2 == 0) && make_plan(nsam, &p);
+}
+
+static int rows_pass(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, bool fwd) {
+  Plan pl;
+  FR_CHECK(make_plan(nsam, &pl) && nsam % 2 == 0, "FFT length %d not supported", nsam);
+  const float2* tw = c.twiddles(nsam);
+  FR_CHECK(tw != nullptr, "twiddle table allocation failed");
+  int npairs = nimg * (nsam / 2);
+  int nseq = row_seqs(nsam);
+  size_t lds = sizeof(float2) * 2 * nseq * (nsam + 1);
+  dim3 grid((npairs + nseq - 1) / nseq);
+  if (fwd)
+    hipLaunchKernelGGL(k_rows_r2c, grid, dim3(kThreads), lds, c.stream, data, pl, nsam, img_stride,
+                       npairs, nseq, tw);
+  else
+    hipLaunchKernelGGL(k_rows_c2r, grid, dim3(kThreads), lds, c.stream, data, pl, nsam, img_stride,
+                       npairs, nseq, tw);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+static int cols_pass(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, int sign) {
+  Plan pl;
+  FR_CHECK(make_plan(nsam, &pl), "FFT length %d not supported", nsam);
+  const float2* tw = c.twiddles(nsam);
+  FR_CHECK(tw != nullptr, "twiddle table allocation failed");
+  int jc = nsam / 2 + 1;
+  int cw = 2048 / nsam;
+  if (cw < 1) cw = 1;
+  if (cw > jc) cw = jc;
+  size_t lds = sizeof(float2) * 2 * cw * (nsam + 1);
+  FR_CHECK(nimg <= 65535, "too many images in one FFT batch (%d)", nimg);
+  dim3 grid((jc + cw - 1) / cw, nimg);
+  hipLaunchKernelGGL(k_cols, grid, dim3(kThreads), lds, c.stream, data, pl, img_stride, cw, sign, tw);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int fft_r2c_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride) {
+  if (nimg <= 0) return 0;
+  if (int r = rows_pass(c, nsam, nimg, data, img_stride, true)) return r;
+  return cols_pass(c, nsam, nimg, data, img_stride, -1);
+}
+
+int fft_c2r_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride) {
+  if (nimg <= 0) return 0;
+  if (int r = cols_pass(c, nsam, nimg, data, img_stride, +1)) return r;
+  return rows_pass(c, nsam, nimg, data, img_stride, false);
+}
+
+}  // namespace fr

@@ -1,0 +1,55 @@
+// fr_kernels.h -- launchers shared between the kernel files and the C-ABI glue.
+#pragma once
+#include "fr_common.h"
+
+namespace fr {
+
+// One insertion task: a particle under one symmetry copy (pinsert.f:74-93).
+struct InsertTask {
+  float dm[6];   // DM(1..6): the in-plane axes of the rotated plane
+  float shx, shy;  // phase-ramp shift (DSHX, DSHY of a3d3.f:84-85)
+  float tmp;     // B-factor exponent of pinsert.f:57-62
+  float occ;     // ON = OCC/SIG2N (a3d3_s.f:44)
+  int part;      // particle slot in the batch
+  int half;      // 0: first half-map (A3DF/S3DF), 1: second (D3DF/V3DF)
+  int irrec2;    // INT(NSAM*RREC)**2
+  int pad_;
+};
+
+int launch_insert(Ctx& c, int nsam, int ntask, const InsertTask* d_tasks, const float* d_datc,
+                  const float* d_ctff, float* num0, float* ctf2_0, float* num1, float* ctf2_1,
+                  unsigned long long* d_nksum, double* d_ttd);
+
+// FP64 shell sums of sheltest.f:48-88 (istp-wide shells); acc is nstp x 6 doubles:
+// DFSC, MEAN12, MEAN22, PSSNR-sum, NPIX, NPIXT.
+int launch_shell_sums(Ctx& c, int nsam, int istp, int nstp, const float* a, const float* d,
+                      const float* s, const float* v, double* acc);
+int launch_storeshift(Ctx& c, int nsam, double std, double vtd, int iredun, const float* a,
+                      const float* s, const float* d, const float* v, float* b, float* cc);
+
+// Particle preparation (lmain.f:459 r2c + ctfapply.f:8-161)
+struct PrepGlobals {
+  float cs, wl, wgh1, wgh2, thetatr, rih, halfw, ri2, ri3, tx, ty;
+  int ictf;
+};
+int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const float* pssnr,
+                const PrepGlobals& g, float* ctff, float* outc, float* datc);
+
+// Central-section extraction (pextract.f:8-101), one output plane per item.
+int launch_pextract(Ctx& c, int nsam, int n, const float* a3df, const float* dm, const int* pidx,
+                    const float* ctff, float* outc);
+
+// CC3M scores (cc3m.f:8-169) for n (particle, orientation, shift) items.
+struct Cc3mArgs {
+  int nsam, n, maxr1, maxr2;
+  float rbfact, ri2, ri3, rih, halfw;
+};
+// Scale by `scal`, then MASKCOS (origin=0) / MASKCOS2 (origin=1); one workgroup per image.
+__global__ void k_scale_mask(int nsam, float* data, size_t img_stride, float scal,
+                             const float* amag_arr, int amag_stride, float ri2, float ri3,
+                             float rih, float halfw, int origin);
+int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec, const float* ctff,
+                const int* pidx, const float* dm, const float* shift, const float* amag,
+                const float* fsct, float* work, float* score);
+
+}  // namespace fr

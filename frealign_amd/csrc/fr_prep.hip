@@ -1,0 +1,263 @@
+// fr_prep.hip -- batched particle preparation.
+//
+// Replaces, for a batch of particles, the r2c FFT of the particle image in
+// LMAIN (lmain.f:459) followed by CTFAPPLY (ctfapply.f:8-161): CTF table
+// (ctf.f), shift phase ramp, c2r, scale, MASKCOS (maskcos.f), r2c,
+// (-1)^(l+m) origin shift and PWEIGHT (pweight.f), including PWEIGHT's
+// in-place double weighting of the Nyquist row and its JC+1 shell aliasing
+// into bin 1's count (the reference calls PWEIGHT(NSAM,OUTC,OUTC,...)).
+#include <hip/hip_runtime.h>
+
+#include "fr_common.h"
+#include "fr_kernels.h"
+
+namespace fr {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr float kTwoPi = 6.2831853071796f;
+constexpr float kPi = 3.1415926535897f;
+
+// ctf.f:8-64
+__device__ inline cf ctf_dev(const PrepGlobals& g, float df1, float df2, float angast,
+                             float thetatr, int ix, int iy) {
+  float wgh2 = fabsf(g.wgh2);
+  cf out;
+  if (g.wgh2 != -1.0f) {
+    float radf = (float)(ix * ix + iy * iy);
+    if (radf != 0.0f) {
+      float rad = sqrtf(radf);
+      float angle = rad * thetatr;
+      float angspt = atan2f((float)iy, (float)ix);
+      float c1 = kTwoPi * angle * angle / (2.0f * g.wl);
+      float c2 = c1 * g.cs * angle * angle / 2.0f;
+      float ccos = cosf(2.0f * (angspt - angast));
+      float df = -0.5f * (df1 + df2 + ccos * (df1 - df2));
+      float chi = c1 * df + c2;
+      // beam tilt (ctf.f:44)
+      chi = chi + kTwoPi * g.cs * angle * angle * ((float)ix * g.tx + (float)iy * g.ty) * thetatr /
+                      g.wl / 1000.0f;
+      float s, c;
+      sincosf(chi, &s, &c);
+      out = cf{g.wgh1 * s - wgh2 * c, -g.wgh1 * c - wgh2 * s};
+    } else {
+      out = cf{-wgh2, 0.0f};
+    }
+    if (g.wgh2 < 0.0f) out = cf{-out.re, -out.im};
+  } else {
+    out = cf{1.0f, 0.0f};
+  }
+  return out;
+}
+
+// ctfapply.f:64-80: CTF table and shifted spectrum; one thread per half-plane point
+__global__ void __launch_bounds__(kThreads)
+    k_ctf_shift(int nsam, int nb, const float2* __restrict__ spec, const float* __restrict__ ppar,
+                PrepGlobals g, float2* ctff, float2* outc, float2* datc) {
+  const int jc = nsam / 2 + 1;
+  const size_t ic = (size_t)nsam * jc;
+  const size_t gid = blockIdx.x * (size_t)kThreads + threadIdx.x;
+  if (gid >= ic * nb) return;
+  const int b = (int)(gid / ic);
+  const int id = (int)(gid - (size_t)b * ic);
+  const int l = id % jc, m = id / jc;
+  const int mm = m >= jc ? m - nsam : m;
+  const float* pp = ppar + 8 * b;
+  const float shx = pp[0], shy = pp[1], df1 = pp[2], df2 = pp[3], ang = pp[4], amagp = pp[5];
+  const float thetatrp = g.thetatr / amagp;
+  float phase = shx * (float)l + shy * (float)mm;
+  float sn, cs;
+  sincosf(phase, &sn, &cs);
+  cf cr = ctf_dev(g, df1, df2, ang, thetatrp, l, mm);
+  cf cl = ctf_dev(g, df1, df2, ang, thetatrp, -l, -mm);
+  float2 s = spec[(size_t)b * ic + id];
+  cf v = cmul(cf{s.x, s.y}, cf{cs, sn});
+  outc[(size_t)b * ic + id] = make_float2(v.re, v.im);
+  datc[(size_t)b * ic + id] = make_float2(v.re, v.im);
+  ctff[(size_t)b * 2 * ic + id] = make_float2(cr.re, cr.im);
+  ctff[(size_t)b * 2 * ic + ic + id] = make_float2(cl.re, cl.im);
+}
+
+__device__ inline float block_sum(float v, float* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  float t = 0.0f;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += red[k];
+  return t;
+}
+
+}  // namespace
+
+// ctfapply.f:107-122 and maskcos.f / maskcos2.f: scale, then cosine-edged
+// real-space mask.  origin=0 centres the disc at NSAM/2+1 (MASKCOS), origin=1
+// at the FFT origin (MASKCOS2).  One workgroup per image.
+__global__ void __launch_bounds__(kThreads)
+    k_scale_mask(int nsam, float* data, size_t img_stride, float scal, const float* amag_arr,
+                 int amag_stride, float ri2, float ri3, float rih, float halfw, int origin) {
+  __shared__ float red[kThreads / 64];
+  __shared__ int redi[kThreads / 64];
+  float* img = data + blockIdx.x * img_stride;
+  const float amagp = amag_arr[(size_t)blockIdx.x * amag_stride];
+  const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
+  const int nmid = origin ? 1 : nsam / 2 + 1, nsamh = nsam / 2, pitch = nsam + 2;
+  float sum = 0.0f;
+  int isum = 0;
+  for (int e = threadIdx.x; e < nsam * nsam; e += blockDim.x) {
+    int i = e / nsam + 1, jx = e % nsam + 1;
+    int ii = i, jj = jx;
+    if (origin) {
+      if (ii > nsamh) ii -= nsam;
+      if (jj > nsamh) jj -= nsam;
+    }
+    float d1 = (float)(ii - nmid), d2 = (float)(jj - nmid);
+    float crad2;
+    {
+#pragma clang fp contract(off)
+      crad2 = d2 * d2 + d1 * d1;
+    }
+    float* px = img + (size_t)(i - 1) * pitch + (jx - 1);
+    float v = *px * scal;
+    *px = v;
+    if (crad2 > ri2p && crad2 <= ri3p) {
+      sum += v;
+      isum++;
+    }
+  }
+  float tot = block_sum(sum, red);
+  // integer count reduction
+  int ci = isum;
+  for (int off = 32; off > 0; off >>= 1) ci += __shfl_down(ci, off, 64);
+  if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = ci;
+  __syncthreads();
+  int itot = 0;
+  for (int k = 0; k < (int)(blockDim.x >> 6); k++) itot += redi[k];
+  const float mean = itot ? tot / (float)itot : tot;
+  for (int e = threadIdx.x; e < nsam * nsam; e += blockDim.x) {
+    int i = e / nsam + 1, jx = e % nsam + 1;
+    int ii = i, jj = jx;
+    if (origin) {
+      if (ii > nsamh) ii -= nsam;
+      if (jj > nsamh) jj -= nsam;
+    }
+    float d1 = (float)(ii - nmid), d2 = (float)(jj - nmid);
+    float crad2;
+    {
+#pragma clang fp contract(off)
+      crad2 = d2 * d2 + d1 * d1;
+    }
+    float* px = img + (size_t)(i - 1) * pitch + (jx - 1);
+    if (crad2 > ri2p && crad2 <= ri3p) {
+      float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+      *px = *px * edge + (1.0f - edge) * mean;
+    } else if (crad2 > ri3p) {
+      *px = mean;
+    }
+  }
+}
+
+namespace {
+
+// ctfapply.f:144-158: origin shift of DATC and OUTC, then PWEIGHT on OUTC.
+// One workgroup per particle.  Shell bins are 1-based as in pweight.f.
+__global__ void __launch_bounds__(kThreads)
+    k_origin_pweight(int nsam, const float* __restrict__ pssnr, int ictf,
+                     const float2* __restrict__ ctff, float2* outc, float2* datc) {
+  __shared__ float rbuf[2 * 520 + 4];
+  const int jc = nsam / 2 + 1;
+  const size_t ic = (size_t)nsam * jc;
+  float2* oc = outc + blockIdx.x * ic;
+  float2* dc = datc + blockIdx.x * ic;
+  const float2* cf_ = ctff + blockIdx.x * 2 * ic;
+  for (int k = threadIdx.x; k < 2 * jc + 2; k += blockDim.x) rbuf[k] = 0.0f;
+  __syncthreads();
+  // origin shift, and the shell sums of pweight.f:31-42 over J=-JC+1..JC-1
+  // (the Nyquist row is addressed by both J=-(JC-1) and J=+(JC-1))
+  for (int e = threadIdx.x; e < (int)ic; e += blockDim.x) {
+    int l = e % jc, m = e / jc;
+    int mm = m >= jc ? m - nsam : m;
+    float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
+    float2 o = oc[e], d = dc[e];
+    o.x *= sg; o.y *= sg;
+    d.x *= sg; d.y *= sg;
+    oc[e] = o;
+    dc[e] = d;
+    int itemp = l * l + mm * mm;
+    if (itemp < jc * jc) {
+      int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+      float h = hypotf(o.x, o.y);
+      int visits = (2 * m == nsam) ? 2 : 1;
+      atomicAdd(&rbuf[ibin], visits * (h * h));
+      atomicAdd(&rbuf[ibin + jc], (float)visits);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i <= jc; i++) {
+      if (rbuf[i + jc] != 0.0f) rbuf[i] = sqrtf(rbuf[i] / rbuf[i + jc]);
+      if (rbuf[i] == 0.0f) rbuf[i] = 1.0f;
+    }
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < (int)ic; e += blockDim.x) {
+    int l = e % jc, m = e / jc;
+    int mm = m >= jc ? m - nsam : m;
+    int itemp = l * l + mm * mm;
+    float2 o = oc[e];
+    if (itemp < jc * jc) {
+      int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+      float snr;
+      if (ictf == 1) {
+        float2 cr = cf_[e], cl = cf_[e + ic];
+        float re = cr.x + cl.x, im = cr.y - cl.y;
+        float h = hypotf(re, im);
+        snr = pssnr[ibin - 1] * (h * h);
+      } else {
+        snr = pssnr[ibin - 1];
+      }
+      float wgt = snr / (snr + 1.0f) / rbuf[ibin];
+      o.x *= wgt;
+      o.y *= wgt;
+      if (2 * m == nsam) { o.x *= wgt; o.y *= wgt; }
+    } else {
+      o = make_float2(0.0f, 0.0f);
+    }
+    oc[e] = o;
+  }
+}
+
+}  // namespace
+
+int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const float* pssnr,
+                const PrepGlobals& g, float* ctff, float* outc, float* datc) {
+  if (nb <= 0) return 0;
+  FR_CHECK(nsam / 2 + 1 <= 520, "box too large for PWEIGHT bins");
+  const int jc = nsam / 2 + 1;
+  const size_t ic = (size_t)nsam * jc;
+  const size_t img = (size_t)nsam * (nsam + 2);
+  // lmain.f:459: r2c of the particle image, in place
+  if (int r = fft_r2c_2d(c, nsam, nb, data, img)) return r;
+  size_t tot = ic * nb;
+  hipLaunchKernelGGL(k_ctf_shift, dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, c.stream, nsam, nb, reinterpret_cast<const float2*>(data), ppar, g,
+                     reinterpret_cast<float2*>(ctff), reinterpret_cast<float2*>(outc),
+                     reinterpret_cast<float2*>(datc));
+  FR_HIP(hipGetLastError());
+  if (int r = fft_c2r_2d(c, nsam, nb, outc, img)) return r;
+  // ctfapply.f:107-122; per-particle AMAGP is column 5 of ppar
+  hipLaunchKernelGGL(k_scale_mask, dim3(nb), dim3(kThreads), 0, c.stream, nsam, outc, img,
+                     1.0f / (float)nsam / (float)nsam, ppar + 5, 8, g.ri2, g.ri3, g.rih, g.halfw,
+                     0);
+  FR_HIP(hipGetLastError());
+  if (int r = fft_r2c_2d(c, nsam, nb, outc, img)) return r;
+  hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kThreads), 0, c.stream, nsam, pssnr, g.ictf,
+                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<float2*>(outc),
+                     reinterpret_cast<float2*>(datc));
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fr

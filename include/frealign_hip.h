@@ -1,0 +1,133 @@
+/*
+ * frealign_hip.h -- C-ABI of the MI355X (gfx950) implementation of Frealign
+ * v9.12's per-particle hot path.
+ *
+ * Conventions
+ *   - Plain C types only.  Arguments named d_* are device pointers (hipMalloc
+ *     or any allocator on the context's device); everything else is host memory.
+ *   - Work is enqueued on the context's stream (fr_set_stream; default: the
+ *     null stream) and is asynchronous unless a function returns host results.
+ *   - Return value: 0 on success, negative on error; fr_last_error() gives the
+ *     message (per thread).  The Fortran glue maps a non-zero status to STOP,
+ *     as the reference does for its own fatal conditions.
+ *   - Storage follows the reference (column-major, x fastest, 0-based here):
+ *       half-plane  JC x NSAM complex64, JC = NSAM/2+1          (ctfapply.f:64-76)
+ *       CTFF        2 half-planes: CTFR then CTFL               (ctfapply.f:78-79)
+ *       image       NSAM x NSAM float32, row pitch NSAM+2 (in-place r2c layout)
+ *       half-volume JC x NSAM x NSAM complex64 (+ float32 CTF^2) (shift.f:34)
+ *     complex arrays are interleaved float pairs and passed as float*.
+ *   - Angles in radians; shifts in phase units (2*pi*pixels/NSAM), as after
+ *     the unit conversion of card10.f:275-291.
+ */
+#ifndef FREALIGN_HIP_H
+#define FREALIGN_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fr_ctx fr_ctx;
+
+const char *fr_version(void);
+const char *fr_last_error(void);
+
+/* Context: one per device and host thread.  Owns grow-only scratch memory. */
+int fr_create(int device, fr_ctx **out);
+int fr_destroy(fr_ctx *ctx);
+int fr_set_stream(fr_ctx *ctx, void *hip_stream);
+int fr_synchronize(fr_ctx *ctx);
+
+/* ---------------------------------------------------------------- geometry
+ * Host functions; their FP32 results are bit-identical to the reference's. */
+
+/* Replaces SEARCHANG + LIMITSYMM (searchang.f:8-150, limitsymm.f:8-51).
+ * pmask: PMASK(1..5).  Writes up to cap triples (phi, theta, psi) to sang and
+ * the full count to *nsang (call with cap=0 to size). */
+int fr_search_grid(float rmm, float ri, float dangin, char asym, int nasym, const int *pmask,
+                   int asymstore, float *sang, int cap, int *nsang, int *iquadmax);
+
+/* Replaces GETSYMMAT + CHECKSYM (getsymmat.f:8-97, checksym.f:8-44):
+ * symop 8 x 9 floats (column-major 3x3), jsym 8 ints. */
+int fr_point_group(char asym, int nasym, float *symop, int *jsym, int *nsym, int *iredun);
+
+/* Rotation matrices DM of cc3m.f:48-65 (amag != 0: divided by |AMAG|) or
+ * pextract.f:30-51 (amag == 0), then DM <- MATMUL_T(DM, SM) when sm != NULL
+ * (matmul_t.f).  euler: n x (phi, theta, psi); dm: n x 9. */
+int fr_rotation(int n, const float *euler, float amag, const float *sm, float *dm);
+
+/* RANDOM (random.f:8-32): the L'Ecuyer generator PSEARCH jitters its grid with. */
+float fr_lcg_random(int *iran);
+
+/* ---------------------------------------------------------------- FFTs
+ * Replaces fftwf_plan_dft_r2c_2d / _c2r_2d as used through FFTW_FWD / FFTW_BWD
+ * (fftw_plans.f:12-13,47,62): nimg images back to back (stride NSAM*(NSAM+2)
+ * floats), in place, unnormalised. */
+int fr_fft2d_r2c(fr_ctx *ctx, int nsam, int nimg, float *d_data);
+int fr_fft2d_c2r(fr_ctx *ctx, int nsam, int nimg, float *d_data);
+
+/* ---------------------------------------------------- particle preparation
+ * Replaces, per particle, the r2c FFT in LMAIN (lmain.f:459) and CTFAPPLY
+ * (ctfapply.f:8-161; called at lmain.f:509-514) with MASKCOS and PWEIGHT.
+ *   d_data  in: nb images (destroyed; holds the r2c spectra on return)
+ *   d_ppar  nb x 8: SHX, SHY (phase units), DFMID1, DFMID2 (A), ANGAST (rad),
+ *           AMAGP, 0, 0
+ *   d_pssnr PSSNR(1..JC+1) (pweight.f reads one past JC; see card10.f:81-90)
+ *   gpar    11 host floats: CS (A), WL (A), WGH1, WGH2, THETATR, RIH, HALFW,
+ *           RI2, RI3, TX, TY  (lmain.f:275, frealign_v9.f:526-531)
+ *   outputs d_ctff (nb x 2 half-planes), d_outc (nb, masked + SNR-weighted),
+ *           d_datc (nb, shifted, unmasked, origin-centred) */
+int fr_prep_batch(fr_ctx *ctx, int nsam, int nb, float *d_data, const float *d_ppar,
+                  const float *d_pssnr, const float *gpar, int ictf, float *d_ctff,
+                  float *d_outc, float *d_datc);
+
+/* ---------------------------------------------------- extraction / scoring */
+
+/* Replaces PEXTRACT (pextract.f:8-101; IEWALD=0, IPAD=1, IRAD>0): n central
+ * sections, item k uses d_dm[9k..] (from fr_rotation with amag=0, sm=identity)
+ * and the CTF of particle d_pidx[k] (d_pidx may be NULL: particle k). */
+int fr_pextract_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_dm,
+                      const int *d_pidx, const float *d_ctff, float *d_outc);
+
+/* Replaces CC3M (cc3m.f:8-169; IEWALD=0, IBUF=0): n scores.  Item k compares
+ * particle d_pidx[k] (d_spec: OUTC of fr_prep_batch, d_ctff) with the section
+ * d_dm[9k..] (fr_rotation with amag=AMAG, sm=identity) at shift
+ * d_shift[2k..] and masks with AMAG d_amag[k].  ring: RI2, RI3, RIH, HALFW.
+ * Requires MAXR2 < NSAM/2. */
+int fr_cc3m_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
+                  const float *d_ctff, const int *d_pidx, const float *d_dm,
+                  const float *d_shift, const float *d_amag, int maxr1, int maxr2, float rbfact,
+                  const float *d_fsct, const float *ring, float *d_score);
+
+/* -------------------------------------------------------------- insertion
+ * Replaces the A3D3 flush (a3d3.f:107-143) -> A3D3_S -> PINSERT ->
+ * PINSERT_TRILIN (pinsert.f:8-195, pinsert_trilin.f:8-244; INTERP=1,
+ * IEWALD=0, NNSTAT=0) for nb buffered particles.
+ *   recb  host nb x 10, the RECB record of a3d3.f:81-90:
+ *         PHI, THETA, PSI, DSHX, DSHY, AMAGP, OCC, SIG2N, PRESA, RREC
+ *   half  host nb: 0 -> first half-map (A3DF/S3DF), 1 -> second (D3DF/V3DF)
+ *   symop/jsym/nsym from fr_point_group; PBC, BOFF, PSIZE as in pinsert.f.
+ * Accumulates into the half-map volumes and into d_nksum[half] (NKSUM1) and
+ * d_ttd[half] (TTD1, FP64). */
+int fr_insert_batch(fr_ctx *ctx, int nsam, int nb, const float *d_datc, const float *d_ctff,
+                    const float *recb, const int *half, int nsym, const int *jsym,
+                    const float *symop, float pbc, float boff, float psize, float *d_num0,
+                    float *d_ctf2_0, float *d_num1, float *d_ctf2_1, long long *d_nksum,
+                    double *d_ttd);
+
+/* ------------------------------------------------------------- statistics */
+
+/* Replaces SHELTEST's FSC/PSSNR part (sheltest.f:8-122, NNSTAT=0) with FP64
+ * shell sums on the GPU.  Host outputs per shell: npix, npixt, dfsc, pssnr. */
+int fr_sheltest(fr_ctx *ctx, int nsam, int istp, const float *d_a, const float *d_d,
+                const float *d_s, const float *d_v, float fmask, int *npix, int *npixt,
+                double *dfsc, float *pssnr, int *nstp);
+
+/* Replaces STORESHIFT's half-map normalisation (storeshift.f:43-66, IFSC=0,
+ * NNSTAT=0): B = A*(-1)^(l+m+n)/(S+STD*IREDUN)/NSAM^3, C likewise from D, V. */
+int fr_storeshift(fr_ctx *ctx, int nsam, double std_, double vtd, int iredun, const float *d_a,
+                  const float *d_s, const float *d_d, const float *d_v, float *d_b, float *d_c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

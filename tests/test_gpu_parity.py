@@ -1,0 +1,327 @@
+"""GPU parity: the HIP path (through libfrealign_hip.so) against the CPU oracle
+on identical seeded inputs.
+
+Bars (written in each test):
+  - Miller-index mapping / voxel set / NKSUM / grid indices: bit-exact
+  - CC3M / PRESB scores: 1e-4 relative to the FP32 oracle
+  - inserted numerator and CTF^2 volumes: 1e-5 relative L2 (FP32 atomics)
+  - every result's error is also reported against the FP64 oracle (or64_*)
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle_lib as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no HIP device", allow_module_level=True)
+
+from frealign_amd import api  # noqa: E402
+
+rng = np.random.default_rng(20261202)
+DEV = "cuda"
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.complex128 if np.iscomplexobj(a) else np.float64)
+    b = np.asarray(b, dtype=np.complex128 if np.iscomplexobj(b) else np.float64)
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def half_volume(nsam, seed):
+    r = np.random.default_rng(seed)
+    v = r.standard_normal((nsam, nsam, nsam)).astype(np.float32)
+    return np.ascontiguousarray(np.fft.rfftn(v).astype(np.complex64))
+
+
+def gpar_for(nsam, psize=1.5):
+    wl = 12.26 / np.sqrt(300000.0 + 0.9785 * 300000.0 ** 2 / 1e6)
+    ri = 0.4 * nsam
+    halfw = 6.0
+    rih, rii = ri - halfw / 2, ri + halfw / 2
+    w = 0.07
+    return np.array([2.7e7, wl, np.sqrt(1 - w * w), w, wl / (psize * nsam), rih, halfw, rih * rih,
+                     rii * rii, 0.0, 0.0], np.float32)
+
+
+def pssnr_for(nsam, ri=None):
+    jc = nsam // 2 + 1
+    ri = 0.4 * nsam if ri is None else ri
+    p = np.zeros(jc + 1, np.float32)
+    for i in range(2, nsam // 2 + 1):  # card10.f:81-90
+        p[i - 1] = np.exp(-((i - 1) / nsam * ri / nsam) ** 2)
+    p[jc] = p[jc - 1]
+    return p
+
+
+# ------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("nsam", [64, 96, 128, 160, 256])
+def test_fft_r2c_c2r(ctx, nsam):
+    nimg = 3
+    img = rng.standard_normal((nimg, nsam, nsam)).astype(np.float32)
+    buf = np.zeros((nimg, nsam, nsam + 2), np.float32)
+    buf[:, :, :nsam] = img
+    d = torch.from_numpy(buf).to(DEV)
+    ctx.fft_r2c(nsam, d)
+    got = d.cpu().numpy().view(np.complex64).reshape(nimg, nsam, nsam // 2 + 1)
+    want = np.fft.rfft2(img.astype(np.float64))
+    err = rel_l2(got, want)
+    print(f"nsam={nsam} r2c rel L2 vs FP64 = {err:.2e}")
+    assert err < 2e-6
+    ctx.fft_c2r(nsam, d)
+    back = d.cpu().numpy()[:, :, :nsam] / (nsam * nsam)
+    assert rel_l2(back, img) < 2e-6
+
+
+# ------------------------------------------------------------- CTFAPPLY
+@pytest.mark.parametrize("nsam", [64, 96])
+def test_ctfapply_vs_oracle(ctx, nsam):
+    nb = 3
+    jc = nsam // 2 + 1
+    L = O.oracle()
+    g = gpar_for(nsam)
+    ps = pssnr_for(nsam)
+    img = rng.standard_normal((nb, nsam, nsam)).astype(np.float32)
+    buf = np.zeros((nb, nsam, nsam + 2), np.float32)
+    buf[:, :, :nsam] = img
+    ppar = np.zeros((nb, 8), np.float32)
+    ppar[:, 0:2] = rng.uniform(-0.3, 0.3, (nb, 2))
+    ppar[:, 2] = rng.uniform(10000, 30000, nb)
+    ppar[:, 3] = ppar[:, 2] + rng.uniform(0, 1000, nb)
+    ppar[:, 4] = rng.uniform(-1, 1, nb)
+    ppar[:, 5] = 1.0
+    ctff, outc, datc = ctx.ctfapply(nsam, torch.from_numpy(buf.copy()).to(DEV),
+                                    torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV),
+                                    g)
+    ctff, outc, datc = ctff.cpu().numpy(), outc.cpu().numpy(), datc.cpu().numpy()
+    for b in range(nb):
+        spec = buf[b].copy().reshape(-1)
+        L.or_rfft2_f(nsam, spec)
+        specc = spec.view(np.complex64).copy()
+        res = {}
+        for pre, ct in (("or32_", np.complex64), ("or64_", np.complex128)):
+            cf_ = np.zeros(2 * nsam * jc, ct)
+            oc = np.zeros(nsam * jc, ct)
+            dc = np.zeros(nsam * jc, ct)
+            getattr(L, pre + "ctfapply")(nsam, specc, *[float(x) for x in ppar[b, :2]], float(g[0]),
+                                         float(g[1]), float(g[2]), float(g[3]),
+                                         *[float(x) for x in ppar[b, 2:5]], float(g[4]),
+                                         float(ppar[b, 5]), float(g[5]), float(g[6]), float(g[7]),
+                                         float(g[8]), ps, 1, cf_, oc, dc)
+            res[pre] = (cf_, oc, dc)
+        for k, name in enumerate(("ctff", "outc", "datc")):
+            gpu = (ctff, outc, datc)[k][b].reshape(-1)
+            e32 = rel_l2(gpu, res["or32_"][k])
+            e64 = rel_l2(gpu, res["or64_"][k])
+            print(f"nsam={nsam} particle {b} {name}: rel L2 vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e}")
+            assert e32 < 1e-5, name
+
+
+# ------------------------------------------------------------- PEXTRACT
+@pytest.mark.parametrize("nsam", [32, 64])
+def test_pextract_vs_oracle(ctx, nsam):
+    jc = nsam // 2 + 1
+    L = O.oracle()
+    vol = half_volume(nsam, 41)
+    n = 6
+    eul = rng.uniform(-np.pi, np.pi, (n, 3)).astype(np.float32)
+    eul[0] = [0, 0, 0]
+    eul[1] = np.float32([np.pi / 2, np.pi / 2, 0])
+    ctff = (rng.standard_normal((n, 2, nsam, jc)) + 1j * rng.standard_normal((n, 2, nsam, jc))
+            ).astype(np.complex64)
+    dm = api.rotation(eul, amag=0.0)
+    out = ctx.pextract(nsam, torch.from_numpy(vol).to(DEV), torch.from_numpy(dm).to(DEV),
+                       torch.from_numpy(ctff).to(DEV)).cpu().numpy()
+    for k in range(n):
+        w32 = np.zeros(nsam * jc, np.complex64)
+        w64 = np.zeros(nsam * jc, np.complex128)
+        L.or32_pextract(nsam, w32, vol.reshape(-1), *[float(x) for x in eul[k]],
+                        ctff[k].reshape(-1), O.IDENTITY)
+        L.or64_pextract(nsam, w64, vol.reshape(-1), *[float(x) for x in eul[k]],
+                        ctff[k].reshape(-1).astype(np.complex128), O.IDENTITY)
+        g = out[k].reshape(-1)
+        # same support and same voxels: a wrong corner would be an O(1) error
+        assert np.array_equal(g != 0, w32 != 0)
+        e32, e64 = rel_l2(g, w32), rel_l2(g, w64)
+        worst = np.max(np.abs(g - w32) / (np.abs(w32) + 1e-3 * np.abs(w32).max()))
+        print(f"nsam={nsam} item {k}: rel L2 vs FP32 {e32:.2e} vs FP64 {e64:.2e} worst {worst:.1e}")
+        assert e32 < 1e-5 and worst < 1e-4
+
+
+# ----------------------------------------------------------------- CC3M
+@pytest.mark.parametrize("nsam", [64])
+def test_cc3m_vs_oracle(ctx, nsam):
+    jc = nsam // 2 + 1
+    L = O.oracle()
+    vol = half_volume(nsam, 43)
+    npart, nitem = 2, 8
+    spec = (rng.standard_normal((npart, nsam, jc)) + 1j * rng.standard_normal((npart, nsam, jc))
+            ).astype(np.complex64)
+    ctff = (0.5 + 0.1 * rng.standard_normal((npart, 2, nsam, jc))).astype(np.complex64)
+    pidx = rng.integers(0, npart, nitem).astype(np.int32)
+    eul = rng.uniform(-np.pi, np.pi, (nitem, 3)).astype(np.float32)
+    shift = rng.uniform(-0.2, 0.2, (nitem, 2)).astype(np.float32)
+    amag = np.ones(nitem, np.float32)
+    dm = api.rotation(eul, amag=1.0)
+    maxr1, maxr2, rbfact = 2, 24, 0.001
+    fsct = np.zeros(nsam // 2 + 1, np.float32)
+    fsct[: nsam // 4] = 1.0
+    ri = 0.4 * nsam
+    ring = np.array([(ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0], np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
+    got = ctx.cc3m(nsam, t(vol), t(spec), t(ctff), t(pidx), t(dm), t(shift), t(amag), maxr1,
+                   maxr2, rbfact, t(fsct), ring).cpu().numpy()
+    for k in range(nitem):
+        p = pidx[k]
+        args = [float(x) for x in eul[k]] + [float(shift[k, 0]), float(shift[k, 1]), rbfact]
+        s32 = L.or32_cc3m(nsam, 1.0, spec[p].reshape(-1), vol.reshape(-1), maxr1, maxr2, *args,
+                          fsct, ctff[p].reshape(-1), *[float(x) for x in ring], O.IDENTITY)
+        s64 = L.or64_cc3m(nsam, 1.0, spec[p].reshape(-1).astype(np.complex128), vol.reshape(-1),
+                          maxr1, maxr2, *args, fsct, ctff[p].reshape(-1).astype(np.complex128),
+                          *[float(x) for x in ring], O.IDENTITY)
+        r32 = abs(got[k] - s32) / abs(s32)
+        print(f"item {k}: gpu {got[k]:.7f} or32 {s32:.7f} or64 {s64:.7f} rel32 {r32:.1e} "
+              f"rel64 {abs(got[k] - s64) / abs(s64):.1e}")
+        assert r32 < 1e-4
+
+
+# ------------------------------------------------------------ insertion
+def _oracle_insert(L, pre, nsam, datc, ctff, recb, half, sym, pbc, boff, psize):
+    jc = nsam // 2 + 1
+    ct, rt = (np.complex64, np.float32) if pre == "or32_" else (np.complex128, np.float64)
+    num = [np.zeros(nsam * nsam * jc, ct) for _ in range(2)]
+    c2 = [np.zeros(nsam * nsam * jc, rt) for _ in range(2)]
+    ttd = [C.c_double(0.0), C.c_double(0.0)]
+    nk = [C.c_int64(0), C.c_int64(0)]
+    nsym, symop, jsym, _ = sym
+    for b in range(recb.shape[0]):
+        r = recb[b]
+        h = int(half[b])
+        on = np.float32(r[6]) / np.float32(r[7])
+        getattr(L, pre + "pinsert")(nsam, float(r[9]), float(r[5]), datc[b].reshape(-1).astype(ct),
+                                    ctff[b].reshape(-1).astype(ct), num[h], c2[h], C.byref(ttd[h]),
+                                    float(r[0]), float(r[1]), float(r[2]), float(r[3]), float(r[4]),
+                                    float(r[8]), pbc, boff, nsym, jsym, symop, C.byref(nk[h]),
+                                    psize, float(on), O.IDENTITY, 0)
+    return num, c2, [t.value for t in ttd], [k.value for k in nk]
+
+
+@pytest.mark.parametrize("nsam,asym,nasym", [(32, "C", 1), (64, "C", 1), (64, "D", 7)])
+def test_insert_vs_oracle(ctx, nsam, asym, nasym):
+    jc = nsam // 2 + 1
+    L = O.oracle()
+    nb = 6
+    datc = (rng.standard_normal((nb, nsam, jc)) + 1j * rng.standard_normal((nb, nsam, jc))
+            ).astype(np.complex64)
+    ctff = (0.6 + 0.2 * (rng.standard_normal((nb, 2, nsam, jc))
+                         + 1j * rng.standard_normal((nb, 2, nsam, jc)))).astype(np.complex64)
+    recb = np.zeros((nb, 10), np.float32)
+    recb[:, 0:3] = rng.uniform(-np.pi, np.pi, (nb, 3))
+    recb[0, 0:3] = [0.0, 0.0, 0.0]  # integer-aligned plane: exercises the INT() rules
+    recb[:, 3:5] = rng.uniform(-0.3, 0.3, (nb, 2))
+    recb[:, 5] = 1.0
+    recb[:, 6] = 1.0
+    recb[:, 7] = rng.uniform(0.5, 2.0, nb)
+    recb[:, 8] = rng.uniform(0.2, 0.8, nb)
+    recb[:, 9] = 0.5 if nsam < 64 else 0.45
+    half = (np.arange(nb) % 2).astype(np.int32)
+    sym = api.point_group(asym, nasym)
+    pbc, boff, psize = 5.0, 40.0, 1.5
+    vols = api.HalfMaps.zeros(nsam)
+    ctx.insert(nsam, torch.from_numpy(datc).to(DEV), torch.from_numpy(ctff).to(DEV), recb, half,
+               sym, pbc, boff, psize, vols)
+    torch.cuda.synchronize()
+    n32, c32, t32, k32 = _oracle_insert(L, "or32_", nsam, datc, ctff, recb, half, sym, pbc, boff,
+                                        psize)
+    n64, c64, t64, _ = _oracle_insert(L, "or64_", nsam, datc, ctff, recb, half, sym, pbc, boff,
+                                      psize)
+    nk = vols.nksum.cpu().numpy()
+    td = vols.ttd.cpu().numpy()
+    for h in range(2):
+        gnum = vols.num[h].cpu().numpy().reshape(-1)
+        gc2 = vols.ctf2[h].cpu().numpy().reshape(-1)
+        assert nk[h] == k32[h], "NKSUM must be bit-exact"
+        assert np.array_equal(gc2 != 0, c32[h] != 0), "voxel set must be identical"
+        e_num, e_c2 = rel_l2(gnum, n32[h]), rel_l2(gc2, c32[h])
+        print(f"{asym}{nasym} nsam={nsam} half {h}: NKSUM {nk[h]}; numerator rel L2 {e_num:.2e} "
+              f"(FP64 {rel_l2(gnum, n64[h]):.2e}); CTF^2 {e_c2:.2e} (FP64 {rel_l2(gc2, c64[h]):.2e});"
+              f" TTD rel {abs(td[h] - t32[h]) / t32[h]:.1e}")
+        assert e_num < 1e-5 and e_c2 < 1e-5
+        assert abs(td[h] - t32[h]) <= 1e-9 * abs(t32[h])
+
+
+def test_insert_deterministic_support(ctx):
+    """Two runs touch the same voxels; sums agree to FP32 atomics rounding."""
+    nsam, nb = 64, 4
+    jc = nsam // 2 + 1
+    datc = (rng.standard_normal((nb, nsam, jc)) + 1j * rng.standard_normal((nb, nsam, jc))
+            ).astype(np.complex64)
+    ctff = np.ones((nb, 2, nsam, jc), np.complex64) * 0.5
+    recb = np.zeros((nb, 10), np.float32)
+    recb[:, 0:3] = rng.uniform(-np.pi, np.pi, (nb, 3))
+    recb[:, 5:8] = 1.0
+    recb[:, 8] = 0.5
+    recb[:, 9] = 0.5
+    half = np.zeros(nb, np.int32)
+    sym = api.point_group("C", 1)
+    outs = []
+    for _ in range(2):
+        v = api.HalfMaps.zeros(nsam)
+        ctx.insert(nsam, torch.from_numpy(datc).to(DEV), torch.from_numpy(ctff).to(DEV), recb,
+                   half, sym, 5.0, 40.0, 1.5, v)
+        outs.append((v.num[0].cpu().numpy(), v.nksum.cpu().numpy()))
+    assert np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0] != 0, outs[1][0] != 0)
+    assert rel_l2(outs[0][0], outs[1][0]) < 1e-6
+
+
+# ------------------------------------------------------------ statistics
+@pytest.mark.parametrize("nsam,istp", [(32, 1), (64, 1), (64, 3)])
+def test_sheltest_vs_oracle(ctx, nsam, istp):
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (a + 0.8 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    t = lambda x: torch.from_numpy(x).to(DEV)  # noqa: E731
+    got = ctx.sheltest(nsam, t(a), t(d), t(s), t(v), istp=istp)
+    npo, npto = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
+    dfo, pso = np.zeros(jc + 2), np.zeros(jc + 2, np.float64)
+    nst = L.or64_sheltest(nsam, istp, a.astype(np.complex128), d.astype(np.complex128),
+                          s.astype(np.float64), v.astype(np.float64), 1.0, npo, npto, dfo, pso)
+    assert len(got["fsc"]) == nst
+    assert np.array_equal(got["npix"], npo[:nst]) and np.array_equal(got["npixt"], npto[:nst])
+    err = np.max(np.abs(got["fsc"] - dfo[:nst]))
+    print(f"nsam={nsam} istp={istp}: max |FSC_gpu - FSC_fp64| = {err:.1e}")
+    assert err < 1e-9
+    np.testing.assert_allclose(got["pssnr"], pso[:nst], rtol=1e-5)
+
+
+def test_storeshift_vs_oracle(ctx):
+    L = O.oracle()
+    nsam = 32
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    t = lambda x: torch.from_numpy(x).to(DEV)  # noqa: E731
+    b, c = ctx.storeshift(nsam, 0.37, 0.41, 14, t(a), t(s), t(d), t(v))
+    bo, co = np.zeros(n, np.complex64), np.zeros(n, np.complex64)
+    L.or32_storeshift(nsam, 0.37, 0.41, 14, a, s, d, v, bo, co)
+    assert rel_l2(b.cpu().numpy(), bo) < 1e-6 and rel_l2(c.cpu().numpy(), co) < 1e-6
