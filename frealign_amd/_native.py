@@ -38,6 +38,11 @@ SIGNATURES = {
                          vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_storeshift": ([vp, ci, cd, cd, ci, vp, vp, vp, vp, vp, vp], ci),
+    "fr_profile_enable": ([vp, ci], ci),
+    "fr_profile_read": ([vp, ci, pd, C.POINTER(C.c_longlong)], ci),
+    "fr_search_layout": ([vp, ci, ci, ci, pi_, pi_], ci),
+    "fr_search_bank": ([vp, ci, ci, ci, vp, ci, vp, ci, vp], ci),
+    "fr_search_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, ci, vp, vp, ci, vp, vp, vp], ci),
 }
 
 

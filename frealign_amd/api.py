@@ -67,6 +67,19 @@ class Context:
         except Exception:
             pass
 
+    # ------------------------------------------------------------ profiling
+    PROF = {"search_gemm": 0, "insert": 1, "cc3m": 2, "prep": 3}
+
+    def profile(self, on: bool = True):
+        N.check(N.lib().fr_profile_enable(self.h, int(on)), "fr_profile_enable")
+
+    def profile_read(self, name: str):
+        """(total_ms, launches) of one kernel family since the last read."""
+        ms, n = C.c_double(0), C.c_longlong(0)
+        N.check(N.lib().fr_profile_read(self.h, self.PROF[name], C.byref(ms), C.byref(n)),
+                "fr_profile_read")
+        return ms.value, n.value
+
     # ---------------------------------------------------------------- FFT
     def fft_r2c(self, nsam: int, data: torch.Tensor):
         _dev(data, torch.float32, "data")
@@ -175,6 +188,56 @@ class Context:
             self.h, nsam, nb, _p(datc), _p(ctff), _p(recb), _p(half), nsym, _p(jsym), _p(symop),
             float(pbc), float(boff), float(psize), _p(vols.num[0]), _p(vols.ctf2[0]),
             _p(vols.num[1]), _p(vols.ctf2[1]), _p(vols.nksum), _p(vols.ttd)), "fr_insert_batch")
+
+    # --------------------------------------------------------------- search
+    def search_layout(self, nsam: int, maxr1: int, maxr2: int):
+        kc, kpad = C.c_int(0), C.c_int(0)
+        N.check(N.lib().fr_search_layout(self.h, nsam, maxr1, maxr2, C.byref(kc), C.byref(kpad)),
+                "fr_search_layout")
+        return kc.value, kpad.value
+
+    def search_bank(self, nsam: int, maxr1: int, maxr2: int, a3df, dm: torch.Tensor,
+                    dtype=torch.bfloat16) -> torch.Tensor:
+        """CTF-free template bank (norient, kpad) for the orientations dm (norient, 9)."""
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(dm, torch.float32, "dm")
+        if dtype not in (torch.bfloat16, torch.float32):
+            raise ValueError("bank dtype must be bfloat16 or float32")
+        _, kpad = self.search_layout(nsam, maxr1, maxr2)
+        n = dm.numel() // 9
+        bank = torch.empty((n, kpad), dtype=dtype, device=a3df.device)
+        self._sync_stream()
+        N.check(N.lib().fr_search_bank(self.h, nsam, maxr1, maxr2, _p(a3df), n, _p(dm),
+                                       1 if dtype == torch.bfloat16 else 0, _p(bank)),
+                "fr_search_bank")
+        return bank
+
+    def search(self, nsam: int, maxr1: int, maxr2: int, rbfact: float, bank: torch.Tensor,
+               outc, ctff, topk: int = 32, group: int = 1):
+        """Top-k views (orientation, shift row, score) per particle against the bank;
+        `group` consecutive orientations form one view (its in-plane angles)."""
+        _dev(outc, torch.complex64, "outc")
+        _dev(ctff, torch.complex64, "ctff")
+        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
+            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        jc = nsam // 2 + 1
+        np_ = outc.numel() // (nsam * jc)
+        if ctff.numel() != 2 * outc.numel():
+            raise ValueError("ctff must hold two half-planes per particle")
+        _, kpad = self.search_layout(nsam, maxr1, maxr2)
+        if bank.dim() != 2 or bank.shape[1] != kpad:
+            raise ValueError(f"bank rows must be {kpad} long")
+        dev = outc.device
+        top_o = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_r = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_s = torch.empty((np_, topk), dtype=torch.float32, device=dev)
+        self._sync_stream()
+        N.check(N.lib().fr_search_batch(self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0],
+                                        _p(bank), 1 if bank.dtype == torch.bfloat16 else 0,
+                                        int(group), np_,
+                                        _p(outc), _p(ctff), topk, _p(top_o), _p(top_r),
+                                        _p(top_s)), "fr_search_batch")
+        return top_o, top_r, top_s
 
     # ----------------------------------------------------------- statistics
     def sheltest(self, nsam: int, a, d, s, v, istp: int = 1, fmask: float = 1.0):

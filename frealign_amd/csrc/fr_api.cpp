@@ -16,12 +16,19 @@
 #include "fr_common.h"
 #include "fr_host.h"
 #include "fr_kernels.h"
+#include "fr_search.h"
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  void* slots[4] = {};
-  size_t slot_bytes[4] = {};
+  void* slots[8] = {};
+  size_t slot_bytes[8] = {};
   void* slot(int k, size_t bytes, int* err);
+  // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
+  int skey[3] = {0, 0, 0};
+  int kc = 0, kpad = 0;
+  int* d_kidx = nullptr;
+  float* d_shx = nullptr;
+  float* d_shy = nullptr;
 };
 
 namespace fr {
@@ -71,7 +78,28 @@ const float2* Ctx::twiddles(int n) {
   return d;
 }
 
+int Ctx::prof_begin(int name) {
+  if (!prof_on || nprof >= 4096) return -1;
+  ProfRec& r = prof[nprof];
+  if (hipEventCreate(&r.a) != hipSuccess) return -1;
+  if (hipEventCreate(&r.b) != hipSuccess) {
+    (void)hipEventDestroy(r.a);
+    return -1;
+  }
+  r.name = name;
+  (void)hipEventRecord(r.a, stream);
+  return nprof++;
+}
+
+void Ctx::prof_end(int slot) {
+  if (slot >= 0) (void)hipEventRecord(prof[slot].b, stream);
+}
+
 Ctx::~Ctx() {
+  for (int i = 0; i < nprof; i++) {
+    (void)hipEventDestroy(prof[i].a);
+    (void)hipEventDestroy(prof[i].b);
+  }
   if (scratch) (void)hipFree(scratch);
   for (auto* p : tw)
     if (p) (void)hipFree(p);
@@ -106,6 +134,38 @@ using namespace fr;
 extern "C" {
 
 const char* fr_version(void) { return "frealign-mi355x 0.1 (gfx950)"; }
+
+int fr_profile_enable(fr_ctx* c, int on) {
+  CTX_CHECK(c);
+  c->prof_on = on != 0;
+  return 0;
+}
+
+int fr_profile_read(fr_ctx* c, int which, double* total_ms, long long* count) {
+  CTX_CHECK(c);
+  FR_CHECK(which >= 0 && which < kProfN && total_ms && count, "bad profile query");
+  FR_HIP(hipStreamSynchronize(c->stream));
+  double t = 0.0;
+  long long n = 0;
+  int keep = 0;
+  for (int i = 0; i < c->nprof; i++) {
+    auto& r = c->prof[i];
+    if (r.name == which) {
+      float ms = 0.f;
+      FR_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      t += ms;
+      n++;
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    } else {
+      c->prof[keep++] = r;
+    }
+  }
+  c->nprof = keep;
+  *total_ms = t;
+  *count = n;
+  return 0;
+}
 const char* fr_last_error(void) { return g_err.c_str(); }
 
 int fr_create(int device, fr_ctx** out) {
@@ -125,8 +185,11 @@ int fr_destroy(fr_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < 8; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
+  if (c->d_kidx) (void)hipFree(c->d_kidx);
+  if (c->d_shx) (void)hipFree(c->d_shx);
+  if (c->d_shy) (void)hipFree(c->d_shy);
   delete c;
   return 0;
 }
@@ -194,7 +257,10 @@ int fr_prep_batch(fr_ctx* c, int nsam, int nb, float* d_data, const float* d_ppa
   PrepGlobals g{gpar[0], gpar[1], gpar[2], gpar[3], gpar[4], gpar[5], gpar[6],
                 gpar[7], gpar[8], gpar[9], gpar[10], ictf};
   FR_CHECK(g.halfw != 0.0f, "HALFW must be non-zero");
-  return launch_prep(*c, nsam, nb, d_data, d_ppar, d_pssnr, g, d_ctff, d_outc, d_datc);
+  int pslot = c->prof_begin(kProfPrep);
+  int r = launch_prep(*c, nsam, nb, d_data, d_ppar, d_pssnr, g, d_ctff, d_outc, d_datc);
+  c->prof_end(pslot);
+  return r;
 }
 
 // ------------------------------------------------------ extraction / scoring
@@ -225,12 +291,14 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   float* work = (float*)c->slot(0, img * sizeof(float) * (size_t)(n < step ? n : step), &err);
   if (!work) return err;
   Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
+  int pslot = c->prof_begin(kProfCc3m);
   for (int o = 0; o < n; o += step) {
     a.n = n - o < step ? n - o : step;
     if (int r = launch_cc3m(*c, a, d_a3df, d_spec, d_ctff, d_pidx + o, d_dm + 9 * (size_t)o,
                             d_shift + 2 * (size_t)o, d_amag + o, d_fsct, work, d_score + o))
       return r;
   }
+  c->prof_end(pslot);
   return 0;
 }
 
@@ -295,8 +363,11 @@ int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const floa
   if (!d_tasks) return err;
   FR_HIP(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(InsertTask) * tasks.size(),
                         hipMemcpyHostToDevice, c->stream));
-  return launch_insert(*c, nsam, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0, d_ctf2_0,
-                       d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum), d_ttd);
+  int pslot = c->prof_begin(kProfInsert);
+  int r = launch_insert(*c, nsam, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0, d_ctf2_0,
+                        d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum), d_ttd);
+  c->prof_end(pslot);
+  return r;
 }
 
 // ------------------------------------------------------------- statistics
@@ -344,6 +415,122 @@ int fr_storeshift(fr_ctx* c, int nsam, double std_, double vtd, int iredun, cons
   FR_CHECK(d_a && d_s && d_d && d_v && d_b && d_c, "null argument");
   FR_CHECK(std_ != 0.0 && vtd != 0.0, "STD or VTD zero: no acceptable particles");
   return launch_storeshift(*c, nsam, std_, vtd, iredun, d_a, d_s, d_d, d_v, d_b, d_c);
+}
+
+// ------------------------------------------------------------------ search
+// The correlation annulus of ccp.f:110-131 and 194-216 on the half-plane:
+// I >= 0, max(MAXR1^2, 1) <= I^2+J^2 < MAXR2^2 (DC is outside the NSAMR
+// correlation loop), listed I-major.
+static int search_annulus(int nsam, int maxr1, int maxr2, std::vector<int>& kidx) {
+  kidx.clear();
+  const int jc = nsam / 2 + 1;
+  int lo = maxr1 * maxr1 < 1 ? 1 : maxr1 * maxr1, hi = maxr2 * maxr2;
+  for (int i = 0; i <= jc - 2; i++)
+    for (int j = -jc + 1; j <= jc - 1; j++) {
+      int it = i * i + j * j;
+      if (it >= lo && it < hi && abs(j) != jc - 1) {
+        kidx.push_back(i);
+        kidx.push_back(j);
+      }
+    }
+  return (int)kidx.size() / 2;
+}
+
+static int search_tables(fr_ctx* c, int nsam, int maxr1, int maxr2) {
+  if (c->skey[0] == nsam && c->skey[1] == maxr1 && c->skey[2] == maxr2) return 0;
+  int nsamh = nsam / 2, nsamrh = nsamh / 2;
+  if (2 * nsamrh != nsamh) nsamrh = (nsamh + 1) / 2;  // ccp.f:49-52
+  const int nsamr = 2 * nsamrh;
+  FR_CHECK(maxr2 <= nsamrh, "search band MAXR2=%d exceeds the NSAMR/2 grid (%d)", maxr2, nsamrh);
+  std::vector<int> kidx;
+  int kc = search_annulus(nsam, maxr1, maxr2, kidx);
+  FR_CHECK(kc > 0, "empty search annulus");
+  int half = (kc + 15) / 16 * 16;
+  kidx.resize(2 * (size_t)half, 0);
+  std::vector<float> shx(2 * (size_t)half * 15, 0.f), shy(2 * (size_t)half * 15, 0.f);
+  for (int k = 0; k < kc; k++)
+    for (int s = 0; s < 15; s++) {
+      double ax = 2.0 * M_PI * kidx[2 * k] * (s - 7) / nsamr;
+      double ay = 2.0 * M_PI * kidx[2 * k + 1] * (s - 7) / nsamr;
+      shx[2 * (k * 15 + s)] = (float)cos(ax);
+      shx[2 * (k * 15 + s) + 1] = (float)sin(ax);
+      shy[2 * (k * 15 + s)] = (float)cos(ay);
+      shy[2 * (k * 15 + s) + 1] = (float)sin(ay);
+    }
+  FR_HIP(hipStreamSynchronize(c->stream));
+  if (c->d_kidx) (void)hipFree(c->d_kidx);
+  if (c->d_shx) (void)hipFree(c->d_shx);
+  if (c->d_shy) (void)hipFree(c->d_shy);
+  c->d_kidx = nullptr;
+  c->d_shx = c->d_shy = nullptr;
+  FR_HIP(hipMalloc(&c->d_kidx, sizeof(int) * kidx.size()));
+  FR_HIP(hipMalloc(&c->d_shx, sizeof(float) * shx.size()));
+  FR_HIP(hipMalloc(&c->d_shy, sizeof(float) * shy.size()));
+  FR_HIP(hipMemcpy(c->d_kidx, kidx.data(), sizeof(int) * kidx.size(), hipMemcpyHostToDevice));
+  FR_HIP(hipMemcpy(c->d_shx, shx.data(), sizeof(float) * shx.size(), hipMemcpyHostToDevice));
+  FR_HIP(hipMemcpy(c->d_shy, shy.data(), sizeof(float) * shy.size(), hipMemcpyHostToDevice));
+  c->kc = kc;
+  c->kpad = 2 * half;
+  c->skey[0] = nsam;
+  c->skey[1] = maxr1;
+  c->skey[2] = maxr2;
+  return 0;
+}
+
+int fr_search_layout(fr_ctx* c, int nsam, int maxr1, int maxr2, int* kc, int* kpad) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(kc && kpad, "null argument");
+  if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  *kc = c->kc;
+  *kpad = c->kpad;
+  return 0;
+}
+
+int fr_search_bank(fr_ctx* c, int nsam, int maxr1, int maxr2, const float* d_a3df, int norient,
+                   const float* d_dm, int dtype, void* d_bank) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_dm && d_bank && norient >= 0, "null argument");
+  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  return launch_bank_build(*c, nsam, norient, c->kc, c->kpad, d_a3df, d_dm, c->d_kidx, d_bank,
+                           dtype == 1);
+}
+
+int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int norient,
+                    const void* d_bank, int dtype, int group, int np, const float* d_outc,
+                    const float* d_ctff, int topk, int* d_top_o, int* d_top_row,
+                    float* d_top_score) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_bank && d_outc && d_ctff && d_top_o && d_top_row && d_top_score, "null argument");
+  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(np >= 0 && norient > 0 && topk >= 1 && topk <= 32, "bad sizes");
+  FR_CHECK(group >= 1 && norient % group == 0, "orientations must come in whole view groups");
+  if (np == 0) return 0;
+  if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  const int half = c->kpad / 2;
+  int err = 0;
+  float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
+  if (!ap) return err;
+  float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
+  if (!c2) return err;
+  float* sum2 = (float*)c->slot(5, sizeof(float) * (size_t)np, &err);
+  if (!sum2) return err;
+  float* score = (float*)c->slot(6, sizeof(float) * (size_t)np * norient, &err);
+  if (!score) return err;
+  int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * norient, &err);
+  if (!brow) return err;
+  if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
+                                  d_ctff, c->d_kidx, ap, c2, sum2))
+    return r;
+  int pslot = c->prof_begin(kProfSearchGemm);
+  if (int r = launch_search_gemm(*c, norient, c->kpad, np, d_bank, dtype == 1, ap, c2, sum2,
+                                 c->d_shx, c->d_shy, score, brow, nullptr))
+    return r;
+  c->prof_end(pslot);
+  return launch_topk(*c, np, norient, group, topk, score, brow, d_top_o, d_top_row, d_top_score);
 }
 
 }  // extern "C"

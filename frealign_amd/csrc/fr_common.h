@@ -29,6 +29,17 @@ struct Ctx {
   size_t scratch_bytes = 0;
   // FFT twiddle tables, one per length (W_N^k = e^{-2 pi i k/N}, k < N)
   float2* tw[1025] = {};
+  // optional per-kernel timing: HIP events recorded on the launch stream
+  // around the named launches (see prof_begin / prof_end)
+  bool prof_on = false;
+  struct ProfRec {
+    int name;
+    hipEvent_t a, b;
+  };
+  ProfRec prof[4096];
+  int nprof = 0;
+  int prof_begin(int name);
+  void prof_end(int slot);
   int ensure_scratch(size_t bytes);
   const float2* twiddles(int n);
   ~Ctx();
@@ -49,6 +60,9 @@ __host__ __device__ inline cf cscale(cf a, float s) { return cf{a.re * s, a.im *
 
 // Wrapped frequency index -> storage row (JJ=J+1; IF (JJ.LT.1) JJ=JJ+NSAM, 0-based)
 __host__ __device__ inline int wrapn(int j, int n) { return j < 0 ? j + n : j; }
+
+// timed kernel families (fr_profile_read)
+enum ProfName { kProfSearchGemm = 0, kProfInsert = 1, kProfCc3m = 2, kProfPrep = 3, kProfN = 4 };
 
 // ---------------------------------------------------------------- FFT API (fr_fft.hip)
 // Batched 2D real FFTs in the reference's in-place layout: nimg images of

@@ -1,0 +1,474 @@
+// fr_search.hip -- exhaustive orientation x translation search as an MFMA GEMM.
+//
+// The reference's PSEARCH/CCP (psearch.f:8-121, ccp.f:8-275) correlates,
+// for every view and in-plane angle, the B-factor-weighted particle spectrum
+// with the CTF-multiplied projection and reads the cross-correlation map on
+// the NSAMR = NSAM/2 grid inside +-6 pixels (ccp.f:229-252).  Here the same
+// quantity is written as a dot product over the resolution annulus k:
+//
+//   cc(p, o, s) = sum_k w_k Re( X_p(k) conj(c_p(k)) e^{i 2pi k.s/NSAMR} conj(T_o(k)) )
+//
+// with X_p = CBUF of psearch.f:50-59, c_p = CTFR + conj(CTFL), w_k = 2 for
+// I > 0 and 1 for I = 0 (the c2r convention of the NSAMR transform), and
+// T_o(k) the CTF-free central section at orientation o.  The CTF goes to the
+// particle side, so the templates T_o are shared by every particle: rows are
+// the 15x15 shift hypotheses s of one particle (generated on the fly in LDS
+// from the particle vector and separable phase tables), columns are
+// templates, K is the real-interleaved annulus.  The epilogue does what
+// ccp.f:225-252 does with the map: argmax over the inner 13x13 (first max
+// wins, x outer), the 3x3 box sum around it, and the normalisation by
+// sqrt(sum |c T|^2) sqrt(sum |X|^2) / 9.
+//
+// The element type is bf16 (v_mfma_f32_16x16x32_bf16, FP32 accumulation) or
+// fp32 (v_mfma_f32_16x16x4f32, the plain FP32 path).
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include "fr_common.h"
+#include "fr_extract.h"
+#include "fr_search.h"
+
+namespace fr {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kRows = 256;     // 15x15 = 225 shift rows, padded
+constexpr int kCols = 128;     // templates per block
+constexpr int kKStep = 32;     // reals per K step (16 complex coefficients)
+constexpr int kShift = 15;     // shift grid edge (-7..7 on the NSAMR grid)
+constexpr int kInner = 6;      // argmax window (-6..6), ccp.f:229-241
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
+
+template <typename T>
+struct Elem;
+template <>
+struct Elem<__bf16> {
+  static constexpr int kPad = 40;  // LDS row pitch in elements (80 B: conflict-free b128 reads)
+  static __device__ inline __bf16 cvt(float v) { return (__bf16)v; }
+  static __device__ inline float f(__bf16 v) { return (float)v; }
+};
+template <>
+struct Elem<float> {
+  static constexpr int kPad = 36;
+  static __device__ inline float cvt(float v) { return v; }
+  static __device__ inline float f(float v) { return v; }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- bank
+// T_o(k) for every orientation o and annulus point k (AINTERPO3DS of the
+// plane DM_o*(I,J)), stored real-interleaved with row pitch kpad.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_bank_build(int nsam, int norient, int kc, int kpad, const float2* __restrict__ a3df,
+                 const float* __restrict__ dm, const int2* __restrict__ kidx, T* bank) {
+  const size_t gid = blockIdx.x * (size_t)kThreads + threadIdx.x;
+  const int half = kpad / 2;
+  if (gid >= (size_t)norient * half) return;
+  const int o = (int)(gid / half), k = (int)(gid - (size_t)o * half);
+  T* row = bank + (size_t)o * kpad;
+  if (k >= kc) {
+    row[2 * k] = Elem<T>::cvt(0.f);
+    row[2 * k + 1] = Elem<T>::cvt(0.f);
+    return;
+  }
+  int2 ij = kidx[k];
+  cf v = ainterpo_dev(nsam, a3df, dm + 9 * (size_t)o, ij.x, ij.y);
+  row[2 * k] = Elem<T>::cvt(v.re);
+  row[2 * k + 1] = Elem<T>::cvt(v.im);
+}
+
+// ------------------------------------------------------- particle features
+// psearch.f:50-59 (CBUF), the CTF of ccp.f:125-129 folded onto the particle,
+// the c2r weights, |c|^2 for the normalisation and SUM2 of ccp.f:212.
+__global__ void __launch_bounds__(kThreads)
+    k_search_feats(int nsam, int kc, int kpad, int maxr22, float rbfact,
+                   const float2* __restrict__ outc, const float2* __restrict__ ctff,
+                   const int2* __restrict__ kidx, float2* ap, float* c2, float* sum2) {
+  const int jc = nsam / 2 + 1, p = blockIdx.x, half = kpad / 2;
+  const size_t ic = (size_t)nsam * jc;
+  const float2* sp = outc + (size_t)p * ic;
+  const float2* cp = ctff + (size_t)p * 2 * ic;
+  for (int k = threadIdx.x; k < half; k += blockDim.x) {
+    float2 a = make_float2(0.f, 0.f);
+    float cc = 0.f;
+    if (k < kc) {
+      int2 ij = kidx[k];
+      size_t id = (size_t)ij.x + (size_t)jc * wrapn(ij.y, nsam);
+      float w = expf(-rbfact * (float)(ij.x * ij.x + ij.y * ij.y));
+      float2 s = sp[id];
+      float2 cr = cp[id], cl = cp[id + ic];
+      float cre = cr.x + cl.x, cim = cr.y - cl.y;  // c = CTFR + conj(CTFL)
+      float wk = ij.x > 0 ? 2.0f : 1.0f;
+      float xr = s.x * w, xi = s.y * w;
+      // w_k * X * conj(c)
+      a = make_float2(wk * (xr * cre + xi * cim), wk * (xi * cre - xr * cim));
+      cc = cre * cre + cim * cim;
+    }
+    ap[(size_t)p * half + k] = a;
+    c2[(size_t)p * half + k] = cc;
+  }
+  // SUM2: |CBUF|^2 over 1 <= r^2 < MAXR2^2 on the half-plane (ccp.f:194-216)
+  __shared__ float red[kThreads / 64];
+  float acc = 0.f;
+  const int r = (int)sqrtf((float)maxr22) + 1;
+  for (int e = threadIdx.x; e < (r + 1) * (2 * r + 1); e += blockDim.x) {
+    int i = e / (2 * r + 1), j = e % (2 * r + 1) - r;
+    int it = i * i + j * j;
+    if (it < 1 || it >= maxr22) continue;
+    size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
+    float w = expf(-rbfact * (float)it);
+    float2 s = sp[id];
+    acc += (s.x * w) * (s.x * w) + (s.y * w) * (s.y * w);
+  }
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sum2[p] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ---------------------------------------------------------------- GEMM
+template <typename T>
+__device__ inline void mma_step(const T* As, const T* Bs, floatx4 (&acc)[4][8], int wave,
+                                int lane);
+
+template <>
+__device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
+                                        floatx4 (&acc)[4][8], int wave, int lane) {
+  constexpr int P = Elem<__bf16>::kPad;
+  const int lr = lane & 15, kq = (lane >> 4) * 8;
+  bf16x8 a[4], b[8];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+    a[rt] = *reinterpret_cast<const bf16x8*>(As + (wave * 64 + rt * 16 + lr) * P + kq);
+#pragma unroll
+  for (int ct = 0; ct < 8; ct++)
+    b[ct] = *reinterpret_cast<const bf16x8*>(Bs + (ct * 16 + lr) * P + kq);
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int ct = 0; ct < 8; ct++)
+      acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+}
+
+template <>
+__device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4 (&acc)[4][8],
+                                       int wave, int lane) {
+  constexpr int P = Elem<float>::kPad;
+  const int lr = lane & 15, kq = lane >> 4;
+#pragma unroll
+  for (int kk = 0; kk < kKStep; kk += 4) {
+    float a[4], b[8];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++) a[rt] = As[(wave * 64 + rt * 16 + lr) * P + kk + kq];
+#pragma unroll
+    for (int ct = 0; ct < 8; ct++) b[ct] = Bs[(ct * 16 + lr) * P + kk + kq];
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int ct = 0; ct < 8; ct++)
+        acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+  }
+}
+
+template <typename T>
+__host__ __device__ constexpr size_t lds_tab_offset() {
+  constexpr size_t ops = (size_t)(kRows + kCols) * Elem<T>::kPad * sizeof(T);
+  constexpr size_t ep = (size_t)kRows * 65 * sizeof(float);
+  return ((ops > ep ? ops : ep) + 15) / 16 * 16;
+}
+
+template <typename T>
+constexpr size_t lds_total() {
+  return lds_tab_offset<T>() + 2 * 16 * kShift * sizeof(float2) + 16 * sizeof(float) +
+         2 * kCols * sizeof(float);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_search_gemm(int norient, int kpad, int np, int ntile, const T* __restrict__ bank,
+                  const float2* __restrict__ ap, const float* __restrict__ c2,
+                  const float* __restrict__ sum2, const float2* __restrict__ shx_tab,
+                  const float2* __restrict__ shy_tab, float* score, int* best_row,
+                  float* raw_peak) {
+  constexpr int P = Elem<T>::kPad;
+  // XCD-aware mapping: blocks are dealt round-robin to the 8 XCDs; each XCD
+  // walks its own template tiles with the particles innermost, so a template
+  // tile stays resident in that XCD's L2 while every particle uses it.
+  const int b = blockIdx.x;
+  const int xcd = b & 7, j = b >> 3;
+  const int tile = (j / np) * 8 + xcd;
+  const int p = j % np;
+  if (tile >= ntile) return;
+  const int o0 = tile * kCols;
+
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  T* As = reinterpret_cast<T*>(smem_raw);
+  T* Bs = As + kRows * P;
+  // phase tables, |c|^2 and the normalisation partials live after the larger
+  // of the operand region and the epilogue buffer (which aliases the operands)
+  float2* tabx = reinterpret_cast<float2*>(smem_raw + lds_tab_offset<T>());  // [16 k][15 sx]
+  float2* taby = tabx + 16 * kShift;                                         // [16 k][15 sy]
+  float* c2s = reinterpret_cast<float*>(taby + 16 * kShift);  // [16]
+  float* nrm = c2s + 16;                                      // [2][kCols]
+
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int half = kpad / 2;
+  const float2* apv = ap + (size_t)p * half;
+  const float* c2v = c2 + (size_t)p * half;
+
+  floatx4 acc[4][8];
+#pragma unroll
+  for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+    for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float my_nrm = 0.f;
+  const int row = tid;  // A-generation row owned by this thread
+  const int rsx = row / kShift, rsy = row % kShift;
+
+  for (int k0 = 0; k0 < half; k0 += 16) {
+    // stage the chunk's particle x shift-x phases and the shift-y phases
+    for (int e = tid; e < 16 * kShift; e += kThreads) {
+      int kk = e / kShift, sx = e % kShift;
+      float2 a = apv[k0 + kk];
+      float2 ex = shx_tab[(size_t)(k0 + kk) * kShift + sx];
+      tabx[e] = make_float2(a.x * ex.x - a.y * ex.y, a.x * ex.y + a.y * ex.x);
+      taby[e] = shy_tab[(size_t)(k0 + kk) * kShift + sx];
+    }
+    if (tid < 16) c2s[tid] = c2v[k0 + tid];
+    // template chunk: kCols rows x 32 reals
+    for (int e = tid; e < kCols * 8; e += kThreads) {
+      int c = e >> 3, q = e & 7;  // 8 groups of 4 reals per template row
+      int o = o0 + c;
+      const T* src = bank + (size_t)(o < norient ? o : 0) * kpad + 2 * k0 + 4 * q;
+      T* dst = Bs + c * P + 4 * q;
+      if (o < norient) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) dst[u] = src[u];
+      } else {
+#pragma unroll
+        for (int u = 0; u < 4; u++) dst[u] = Elem<T>::cvt(0.f);
+      }
+    }
+    __syncthreads();
+    // A chunk: row (sx, sy) = X'(k) e^{i k.s}
+    {
+      T* dst = As + row * P;
+      if (row < kShift * kShift) {
+#pragma unroll 4
+        for (int kk = 0; kk < 16; kk++) {
+          float2 u = tabx[kk * kShift + rsx], v = taby[kk * kShift + rsy];
+          dst[2 * kk] = Elem<T>::cvt(u.x * v.x - u.y * v.y);
+          dst[2 * kk + 1] = Elem<T>::cvt(u.x * v.y + u.y * v.x);
+        }
+      } else {
+#pragma unroll 4
+        for (int kk = 0; kk < 32; kk++) dst[kk] = Elem<T>::cvt(0.f);
+      }
+    }
+    // normalisation: sum_k |c_p(k)|^2 |T_o(k)|^2 for this thread's template
+    {
+      int c = tid & (kCols - 1), h = tid >> 7;
+      const T* src = Bs + c * P + 16 * h;
+#pragma unroll
+      for (int kk = 0; kk < 8; kk++) {
+        float re = Elem<T>::f(src[2 * kk]), im = Elem<T>::f(src[2 * kk + 1]);
+        my_nrm += c2s[8 * h + kk] * (re * re + im * im);
+      }
+    }
+    __syncthreads();
+    mma_step<T>(As, Bs, acc, wave, lane);
+    __syncthreads();
+  }
+
+  // ------------------------------------------------------------ epilogue
+  nrm[(tid >> 7) * kCols + (tid & (kCols - 1))] = my_nrm;
+  float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
+  const float s2 = sum2[p];
+#pragma unroll
+  for (int hc = 0; hc < 2; hc++) {
+    __syncthreads();
+#pragma unroll
+    for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+      for (int ct = 0; ct < 4; ct++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          int r = wave * 64 + rt * 16 + (lane >> 4) * 4 + i;
+          int c = ct * 16 + (lane & 15);
+          ep[r * 65 + c] = acc[rt][ct + 4 * hc][i];
+        }
+    __syncthreads();
+    if (tid < 64) {
+      const int c = tid, o = o0 + hc * 64 + c;
+      // ccp.f:229-241: argmax over IX (outer) and IY (inner) in -6..6, first max wins
+      float best = -3.0e38f;
+      int bx = 0, by = 0;
+      for (int ix = -kInner; ix <= kInner; ix++)
+        for (int iy = -kInner; iy <= kInner; iy++) {
+          float v = ep[((ix + 7) * kShift + (iy + 7)) * 65 + c];
+          if (v > best) { best = v; bx = ix; by = iy; }
+        }
+      // ccp.f:242-252: 3x3 box sum
+      float box = 0.f;
+      for (int dx = -1; dx <= 1; dx++)
+        for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
+      if (o < norient) {
+        float n = nrm[hc * 64 + c] + nrm[kCols + hc * 64 + c];
+        float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
+        size_t out = (size_t)p * norient + o;
+        score[out] = cc;
+        best_row[out] = (bx + 7) * kShift + (by + 7);
+        if (raw_peak) raw_peak[out] = best;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- top-K
+// CCP returns one (psi, shift) per view: the first maximum over its in-plane
+// angles in (IPSI outer, IQUAD inner) order (ccp.f:144-264); PSEARCH then
+// keeps the IPMAX best views by strict '>' in grid order (psearch.f:89-102),
+// so ties keep the earlier view.  Orientations come in groups of `group`
+// consecutive entries per view.  One workgroup per particle: each thread
+// keeps a sorted top-K of its views, then a bitonic merge.
+__global__ void __launch_bounds__(kThreads)
+    k_topk(int norient, int group, int topk, const float* __restrict__ score,
+           const int* __restrict__ best_row, int* top_o, int* top_row, float* top_s) {
+  constexpr int kK = 32;
+  __shared__ float sv[kThreads * kK];
+  __shared__ int si[kThreads * kK];
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const float* sc = score + (size_t)p * norient;
+  const int ngroup = norient / group;
+  float v[kK];
+  int ix[kK];
+#pragma unroll
+  for (int q = 0; q < kK; q++) { v[q] = -3.4e38f; ix[q] = 0x7fffffff; }
+  for (int g = tid; g < ngroup; g += kThreads) {
+    float x = -3.4e38f;
+    int xo = g * group;
+    for (int u = 0; u < group; u++) {
+      float y = sc[g * group + u];
+      if (y > x) { x = y; xo = g * group + u; }
+    }
+    if (!(x > v[kK - 1])) continue;
+    int q = kK - 1;
+    while (q > 0 && x > v[q - 1]) { v[q] = v[q - 1]; ix[q] = ix[q - 1]; q--; }
+    v[q] = x;
+    ix[q] = xo;
+  }
+#pragma unroll
+  for (int q = 0; q < kK; q++) { sv[tid * kK + q] = v[q]; si[tid * kK + q] = ix[q]; }
+  __syncthreads();
+  // bitonic sort of kThreads*kK entries, descending score, ascending index on ties
+  const int n = kThreads * kK;
+  for (int size = 2; size <= n; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int e = tid; e < n / 2; e += kThreads) {
+        int lo = 2 * e - (e & (stride - 1));
+        int hi = lo + stride;
+        bool desc = ((lo & size) == 0);
+        float a = sv[lo], bb = sv[hi];
+        int ia = si[lo], ib = si[hi];
+        bool a_first = (a > bb) || (a == bb && ia < ib);
+        if (a_first != desc) {
+          sv[lo] = bb; sv[hi] = a;
+          si[lo] = ib; si[hi] = ia;
+        }
+      }
+      __syncthreads();
+    }
+  for (int q = tid; q < topk; q += kThreads) {
+    int o = si[q];
+    bool ok = o >= 0 && o < norient;
+    top_o[(size_t)p * topk + q] = ok ? o : -1;
+    top_row[(size_t)p * topk + q] = ok ? best_row[(size_t)p * norient + o] : -1;
+    top_s[(size_t)p * topk + q] = sv[q];
+  }
+}
+
+// ------------------------------------------------------------- launchers
+size_t search_gemm_lds(bool bf16) { return bf16 ? lds_total<__bf16>() : lds_total<float>(); }
+
+int launch_bank_build(Ctx& c, int nsam, int norient, int kc, int kpad, const float* a3df,
+                      const float* dm, const int* kidx, void* bank, bool bf16) {
+  size_t tot = (size_t)norient * (kpad / 2);
+  dim3 grid((unsigned)((tot + kThreads - 1) / kThreads));
+  if (bf16)
+    hipLaunchKernelGGL(k_bank_build<__bf16>, grid, dim3(kThreads), 0, c.stream, nsam, norient, kc,
+                       kpad, reinterpret_cast<const float2*>(a3df), dm,
+                       reinterpret_cast<const int2*>(kidx), reinterpret_cast<__bf16*>(bank));
+  else
+    hipLaunchKernelGGL(k_bank_build<float>, grid, dim3(kThreads), 0, c.stream, nsam, norient, kc,
+                       kpad, reinterpret_cast<const float2*>(a3df), dm,
+                       reinterpret_cast<const int2*>(kidx), reinterpret_cast<float*>(bank));
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_search_feats(Ctx& c, int nsam, int np, int kc, int kpad, int maxr22, float rbfact,
+                        const float* outc, const float* ctff, const int* kidx, float* ap,
+                        float* c2, float* sum2) {
+  hipLaunchKernelGGL(k_search_feats, dim3(np), dim3(kThreads), 0, c.stream, nsam, kc, kpad, maxr22,
+                     rbfact, reinterpret_cast<const float2*>(outc),
+                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<const int2*>(kidx),
+                     reinterpret_cast<float2*>(ap), c2, sum2);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_search_gemm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
+                       const float* ap, const float* c2, const float* sum2, const float* shx_tab,
+                       const float* shy_tab, float* score, int* best_row, float* raw_peak) {
+  const int ntile = (norient + kCols - 1) / kCols;
+  const int ntile8 = (ntile + 7) / 8 * 8;
+  const size_t nblk = (size_t)ntile8 * np;
+  FR_CHECK(nblk < (1u << 31), "search grid too large");
+  const size_t lds = search_gemm_lds(bf16);
+  FR_CHECK(lds <= 160 * 1024, "LDS budget exceeded (%zu)", lds);
+  if (bf16) {
+    static bool attr = false;
+    if (!attr) {
+      FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<__bf16>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kThreads), lds, c.stream,
+                       norient, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank),
+                       reinterpret_cast<const float2*>(ap), c2, sum2,
+                       reinterpret_cast<const float2*>(shx_tab),
+                       reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
+  } else {
+    static bool attr = false;
+    if (!attr) {
+      FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<float>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      attr = true;
+    }
+    hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kThreads), lds, c.stream,
+                       norient, kpad, np, ntile, reinterpret_cast<const float*>(bank),
+                       reinterpret_cast<const float2*>(ap), c2, sum2,
+                       reinterpret_cast<const float2*>(shx_tab),
+                       reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
+  }
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_topk(Ctx& c, int np, int norient, int group, int topk, const float* score,
+                const int* best_row, int* top_o, int* top_row, float* top_s) {
+  FR_CHECK(topk >= 1 && topk <= 32, "topk must be in 1..32");
+  FR_CHECK(group >= 1 && norient % group == 0, "orientation count %d is not a multiple of the "
+           "per-view group %d", norient, group);
+  hipLaunchKernelGGL(k_topk, dim3(np), dim3(kThreads), 0, c.stream, norient, group, topk, score,
+                     best_row, top_o, top_row, top_s);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fr

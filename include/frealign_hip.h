@@ -37,6 +37,14 @@ int fr_destroy(fr_ctx *ctx);
 int fr_set_stream(fr_ctx *ctx, void *hip_stream);
 int fr_synchronize(fr_ctx *ctx);
 
+/* Per-kernel timing with HIP events on the context's stream (off by default).
+ * which: 0 search GEMM, 1 insertion, 2 CC3M, 3 particle preparation.
+ * fr_profile_read synchronises, returns the summed duration and count of the
+ * recorded launches of that family, and forgets them.  No reference
+ * counterpart (the reference only prints wall-clock stamps, lmain.f:238-246). */
+int fr_profile_enable(fr_ctx *ctx, int on);
+int fr_profile_read(fr_ctx *ctx, int which, double *total_ms, long long *count);
+
 /* ---------------------------------------------------------------- geometry
  * Host functions; their FP32 results are bit-identical to the reference's. */
 
@@ -126,6 +134,37 @@ int fr_sheltest(fr_ctx *ctx, int nsam, int istp, const float *d_a, const float *
  * NNSTAT=0): B = A*(-1)^(l+m+n)/(S+STD*IREDUN)/NSAM^3, C likewise from D, V. */
 int fr_storeshift(fr_ctx *ctx, int nsam, double std_, double vtd, int iredun, const float *d_a,
                   const float *d_s, const float *d_d, const float *d_v, float *d_b, float *d_c);
+
+/* ------------------------------------------------------------------ search
+ * Replaces PSEARCH -> CCP (psearch.f:8-121, ccp.f:8-275; called at
+ * lmain.f:570-575) for a batch of particles against a shared bank of
+ * orientations.  The correlation annulus is ccp.f's: I >= 0,
+ * max(MAXR1^2,1) <= I^2+J^2 < MAXR2^2, with MAXR2 <= NSAM/4 (the NSAMR grid).
+ * Shifts are the 15x15 NSAMR-grid hypotheses -7..7; each (particle,
+ * orientation) is scored as CCP scores a view: argmax over -6..6 (x outer,
+ * first max wins), 3x3 box sum /9, normalised by sqrt(sum|cT|^2 sum|X|^2).
+ * dtype 1 runs the correlation as a BF16 MFMA GEMM with FP32 accumulation,
+ * dtype 0 as an FP32 MFMA GEMM. */
+
+/* Annulus size kc and the padded real row length kpad of a bank row. */
+int fr_search_layout(fr_ctx *ctx, int nsam, int maxr1, int maxr2, int *kc, int *kpad);
+
+/* Bank of CTF-free central sections T_o (AINTERPO3DS of the plane
+ * d_dm[9o..]*(I,J), dm from fr_rotation with amag=AMAG): norient x kpad
+ * elements of 4 (dtype 0) or 2 (dtype 1) bytes. */
+int fr_search_bank(fr_ctx *ctx, int nsam, int maxr1, int maxr2, const float *d_a3df, int norient,
+                   const float *d_dm, int dtype, void *d_bank);
+
+/* Scores np particles (d_outc: OUTC of fr_prep_batch, d_ctff) against the
+ * bank.  Orientations come in views of `group` consecutive in-plane angles;
+ * each view contributes its first maximum (ccp.f:144-264) and the topk
+ * (<= 32) views per particle are returned by strict '>' in grid order
+ * (psearch.f:89-102): orientation index, shift row ((sx+7)*15+(sy+7), NSAMR
+ * pixels) and score.  rbfact: RBFACT of lmain.f:556. */
+int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, int norient,
+                    const void *d_bank, int dtype, int group, int np, const float *d_outc,
+                    const float *d_ctff, int topk, int *d_top_o, int *d_top_row,
+                    float *d_top_score);
 
 #ifdef __cplusplus
 }

@@ -259,7 +259,8 @@ def test_insert_vs_oracle(ctx, nsam, asym, nasym):
               f"(FP64 {rel_l2(gnum, n64[h]):.2e}); CTF^2 {e_c2:.2e} (FP64 {rel_l2(gc2, c64[h]):.2e});"
               f" TTD rel {abs(td[h] - t32[h]) / t32[h]:.1e}")
         assert e_num < 1e-5 and e_c2 < 1e-5
-        assert abs(td[h] - t32[h]) <= 1e-9 * abs(t32[h])
+        # TTD sums the same FP32 CTF^2 terms in FP64: its bar follows CTF^2's
+        assert abs(td[h] - t32[h]) <= 1e-6 * abs(t32[h])
 
 
 def test_insert_deterministic_support(ctx):
@@ -299,15 +300,22 @@ def test_sheltest_vs_oracle(ctx, nsam, istp):
     v = rng.uniform(0, 3, n).astype(np.float32)
     t = lambda x: torch.from_numpy(x).to(DEV)  # noqa: E731
     got = ctx.sheltest(nsam, t(a), t(d), t(s), t(v), istp=istp)
-    npo, npto = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
-    dfo, pso = np.zeros(jc + 2), np.zeros(jc + 2, np.float64)
-    nst = L.or64_sheltest(nsam, istp, a.astype(np.complex128), d.astype(np.complex128),
-                          s.astype(np.float64), v.astype(np.float64), 1.0, npo, npto, dfo, pso)
+    res = {}
+    for pre, ct, rt in (("or32_", np.complex64, np.float32), ("or64_", np.complex128, np.float64)):
+        npo, npto = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
+        dfo, pso = np.zeros(jc + 2), np.zeros(jc + 2, rt)
+        nst = getattr(L, pre + "sheltest")(nsam, istp, a.astype(ct), d.astype(ct), s.astype(rt),
+                                          v.astype(rt), 1.0, npo, npto, dfo, pso)
+        res[pre] = (nst, npo, npto, dfo, pso)
+    nst, npo, npto, dfo, pso = res["or32_"]
     assert len(got["fsc"]) == nst
     assert np.array_equal(got["npix"], npo[:nst]) and np.array_equal(got["npixt"], npto[:nst])
-    err = np.max(np.abs(got["fsc"] - dfo[:nst]))
-    print(f"nsam={nsam} istp={istp}: max |FSC_gpu - FSC_fp64| = {err:.1e}")
-    assert err < 1e-9
+    # FP32 products accumulated in FP64, as sheltest.f does: only the summation
+    # order differs from the FP32 restatement
+    err32 = np.max(np.abs(got["fsc"] - dfo[:nst]))
+    err64 = np.max(np.abs(got["fsc"] - res["or64_"][3][:nst]))
+    print(f"nsam={nsam} istp={istp}: max |FSC_gpu - FSC_fp32| = {err32:.1e}, vs FP64 {err64:.1e}")
+    assert err32 < 1e-6 and err64 < 1e-6
     np.testing.assert_allclose(got["pssnr"], pso[:nst], rtol=1e-5)
 
 
