@@ -37,7 +37,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-views", type=int, default=4)
+    ap.add_argument("--cpu-views", type=int, default=16)
     return ap.parse_args()
 
 
@@ -51,6 +51,10 @@ TOPK = 32
 PEAK_BF16 = 2.5e15   # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main():
@@ -88,6 +92,7 @@ def main():
     kc, kpad = ctx.search_layout(NSAM, maxr1, maxr2)
     norient = gs16.norient
     setup_s = time.time() - t0
+    log(f"setup {setup_s:.1f}s: {norient} orientations, annulus {kc}, batch {B}")
 
     def batch(i):
         lo = (i * B) % npool
@@ -124,6 +129,7 @@ def main():
     prep_ms, prep_n = ctx.profile_read("prep")
     ctx.profile(False)
     pps16 = B * args.steps / dt16
+    log(f"bf16 path: {pps16:.1f} particles/s")
 
     # accuracy of the BF16 path against the truth (modulo D7)
     errs = []
@@ -143,6 +149,7 @@ def main():
     torch.cuda.synchronize()
     dt32 = time.perf_counter() - t2
     pps32 = B * args.fp32_steps / dt32
+    log(f"fp32 path: {pps32:.1f} particles/s")
     agree = []
     for (o16, _, _, _, _), (o32, _, _, _, _) in zip(results, res32):
         agree.append(search.angular_error_deg(gs16.euler[o16], gs32.euler[o32], mats))
@@ -160,7 +167,7 @@ def main():
             if x * x + y * y <= irrec * irrec:
                 npts += 1
     ins_parts = 0
-    for o, sh, lo, ctff, datc in results:
+    for o, sh, lo, ctff, datc in results[:1]:
         recb = np.zeros((B, 10), np.float32)
         recb[:, 0:3] = gs16.euler[o]
         recb[:, 3:5] = sh
@@ -177,6 +184,7 @@ def main():
     ctx.profile(False)
     ins_bytes = 216.0 * npts * sym[3] * ins_parts
     ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
+    log(f"insertion: {ins_gbps:.1f} GB/s")
 
     # ------------------------------------------------------------ roofline
     gemm_avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
@@ -190,6 +198,7 @@ def main():
     # --------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
+        log("cpu baseline")
         cpu = cpu_baseline(args.cpu_views, parts, a3df, ctff_for_cpu=None, optics=optics,
                            gpar=gpar, maxr1=maxr1, maxr2=maxr2, norient_views=norient // gs16.group,
                            ctx=ctx, pssnr=pssnr)
