@@ -175,18 +175,35 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
   }
 }
 
+// LDS layout of k_search_gemm: two operand buffers (A rows x P, B rows x P),
+// three phase-table buffers, then the normalisation partials.  The epilogue
+// map [kRows][65] floats aliases the start of the region.
 template <typename T>
-__host__ __device__ constexpr size_t lds_tab_offset() {
-  constexpr size_t ops = (size_t)(kRows + kCols) * Elem<T>::kPad * sizeof(T);
-  constexpr size_t ep = (size_t)kRows * 65 * sizeof(float);
-  return ((ops > ep ? ops : ep) + 15) / 16 * 16;
-}
+struct Lds {
+  static constexpr size_t a_bytes = (size_t)kRows * Elem<T>::kPad * sizeof(T);
+  static constexpr size_t b_bytes = (size_t)kCols * Elem<T>::kPad * sizeof(T);
+  static constexpr size_t op_bytes = a_bytes + b_bytes;                   // one buffer
+  static constexpr size_t tab_one = 2 * 16 * kShift * sizeof(float2) + 16 * sizeof(float);
+  static constexpr size_t tab_off = 2 * op_bytes;
+  static constexpr size_t ep_bytes = (size_t)kRows * 65 * sizeof(float);
+  static constexpr size_t end_tab = tab_off + 3 * tab_one;
+  static constexpr size_t nrm_off = ((end_tab > ep_bytes ? end_tab : ep_bytes) + 15) / 16 * 16;
+  static constexpr size_t total = nrm_off + 4 * kCols * sizeof(float) * 2;
+};
 
 template <typename T>
 constexpr size_t lds_total() {
-  return lds_tab_offset<T>() + 2 * 16 * kShift * sizeof(float2) + 16 * sizeof(float) +
-         2 * kCols * sizeof(float);
+  return Lds<T>::total;
 }
+
+// B chunk transfer: kCols templates x 32 reals, in 16-byte pieces.
+template <typename T>
+struct BPieces {
+  static constexpr int per_row = 32 * sizeof(T) / 16;  // 4 (bf16) or 8 (fp32)
+  static constexpr int total = kCols * per_row;         // 512 or 1024
+  static constexpr int per_thread = total / kThreads;   // 2 or 4
+  static constexpr int elems = 16 / sizeof(T);          // 8 or 4 elements per piece
+};
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
@@ -196,6 +213,8 @@ __global__ void __launch_bounds__(kThreads)
                   const float2* __restrict__ shy_tab, float* score, int* best_row,
                   float* raw_peak) {
   constexpr int P = Elem<T>::kPad;
+  using L = Lds<T>;
+  using BP = BPieces<T>;
   // XCD-aware mapping: blocks are dealt round-robin to the 8 XCDs; each XCD
   // walks its own template tiles with the particles innermost, so a template
   // tile stays resident in that XCD's L2 while every particle uses it.
@@ -207,17 +226,18 @@ __global__ void __launch_bounds__(kThreads)
   const int o0 = tile * kCols;
 
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
-  T* As = reinterpret_cast<T*>(smem_raw);
-  T* Bs = As + kRows * P;
-  // phase tables, |c|^2 and the normalisation partials live after the larger
-  // of the operand region and the epilogue buffer (which aliases the operands)
-  float2* tabx = reinterpret_cast<float2*>(smem_raw + lds_tab_offset<T>());  // [16 k][15 sx]
-  float2* taby = tabx + 16 * kShift;                                         // [16 k][15 sy]
-  float* c2s = reinterpret_cast<float*>(taby + 16 * kShift);  // [16]
-  float* nrm = c2s + 16;                                      // [2][kCols]
+  auto As = [&](int buf) { return reinterpret_cast<T*>(smem_raw + buf * L::op_bytes); };
+  auto Bs = [&](int buf) { return reinterpret_cast<T*>(smem_raw + buf * L::op_bytes + L::a_bytes); };
+  auto tabx = [&](int buf) {
+    return reinterpret_cast<float2*>(smem_raw + L::tab_off + buf * L::tab_one);
+  };
+  auto taby = [&](int buf) { return tabx(buf) + 16 * kShift; };
+  auto c2s = [&](int buf) { return reinterpret_cast<float*>(taby(buf) + 16 * kShift); };
+  float* nrm = reinterpret_cast<float*>(smem_raw + L::nrm_off);  // [pieces-per-row][kCols]
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int half = kpad / 2;
+  const int nchunk = half / 16;
   const float2* apv = ap + (size_t)p * half;
   const float* c2v = c2 + (size_t)p * half;
 
@@ -226,67 +246,143 @@ __global__ void __launch_bounds__(kThreads)
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
-  float my_nrm = 0.f;
-  const int row = tid;  // A-generation row owned by this thread
-  const int rsx = row / kShift, rsy = row % kShift;
 
-  for (int k0 = 0; k0 < half; k0 += 16) {
-    // stage the chunk's particle x shift-x phases and the shift-y phases
-    for (int e = tid; e < 16 * kShift; e += kThreads) {
-      int kk = e / kShift, sx = e % kShift;
-      float2 a = apv[k0 + kk];
-      float2 ex = shx_tab[(size_t)(k0 + kk) * kShift + sx];
-      tabx[e] = make_float2(a.x * ex.x - a.y * ex.y, a.x * ex.y + a.y * ex.x);
-      taby[e] = shy_tab[(size_t)(k0 + kk) * kShift + sx];
-    }
-    if (tid < 16) c2s[tid] = c2v[k0 + tid];
-    // template chunk: kCols rows x 32 reals
-    for (int e = tid; e < kCols * 8; e += kThreads) {
-      int c = e >> 3, q = e & 7;  // 8 groups of 4 reals per template row
+  // this thread's B pieces: piece q of templates c0 + u * (kCols / per_thread)
+  const int bq = tid % BP::per_row;
+  const int bc0 = tid / BP::per_row;
+  constexpr int bstep = kThreads / BP::per_row;
+  float my_nrm[BP::per_thread];
+#pragma unroll
+  for (int u = 0; u < BP::per_thread; u++) my_nrm[u] = 0.f;
+  uint4 breg[BP::per_thread];
+  // phase-table staging registers: entries e = tid and tid + 256 of 2 x 240 (+16 |c|^2)
+  float2 treg[2];
+  float creg = 0.f;
+
+  auto load_b = [&](int chunk) {
+#pragma unroll
+    for (int u = 0; u < BP::per_thread; u++) {
+      int c = bc0 + u * bstep;
       int o = o0 + c;
-      const T* src = bank + (size_t)(o < norient ? o : 0) * kpad + 2 * k0 + 4 * q;
-      T* dst = Bs + c * P + 4 * q;
-      if (o < norient) {
-#pragma unroll
-        for (int u = 0; u < 4; u++) dst[u] = src[u];
-      } else {
-#pragma unroll
-        for (int u = 0; u < 4; u++) dst[u] = Elem<T>::cvt(0.f);
-      }
+      if (o < norient)
+        breg[u] = *reinterpret_cast<const uint4*>(bank + (size_t)o * kpad + 32 * chunk +
+                                                   bq * BP::elems);
+      else
+        breg[u] = make_uint4(0u, 0u, 0u, 0u);
     }
-    __syncthreads();
-    // A chunk: row (sx, sy) = X'(k) e^{i k.s}
-    {
-      T* dst = As + row * P;
-      if (row < kShift * kShift) {
-#pragma unroll 4
-        for (int kk = 0; kk < 16; kk++) {
-          float2 u = tabx[kk * kShift + rsx], v = taby[kk * kShift + rsy];
-          dst[2 * kk] = Elem<T>::cvt(u.x * v.x - u.y * v.y);
-          dst[2 * kk + 1] = Elem<T>::cvt(u.x * v.y + u.y * v.x);
+  };
+  auto store_b = [&](int buf) {
+    T* dst = Bs(buf);
+#pragma unroll
+    for (int u = 0; u < BP::per_thread; u++) {
+      int c = bc0 + u * bstep;
+      *reinterpret_cast<uint4*>(dst + c * P + bq * BP::elems) = breg[u];
+    }
+  };
+  // normalisation sum_k |c(k)|^2 |T_o(k)|^2 from the pieces this thread holds
+  auto norm_b = [&](int tbuf) {
+    const float* cc = c2s(tbuf);
+#pragma unroll
+    for (int u = 0; u < BP::per_thread; u++) {
+      const T* e = reinterpret_cast<const T*>(&breg[u]);
+      float acc_n = 0.f;
+#pragma unroll
+      for (int q = 0; q < BP::elems / 2; q++) {
+        float re = Elem<T>::f(e[2 * q]), im = Elem<T>::f(e[2 * q + 1]);
+        acc_n += cc[bq * (BP::elems / 2) + q] * (re * re + im * im);
+      }
+      my_nrm[u] += acc_n;
+    }
+  };
+  auto load_tab = [&](int chunk) {
+    int k0 = 16 * chunk;
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      int e = tid + u * kThreads;  // 0..479 used
+      float2 v = make_float2(0.f, 0.f);
+      if (e < 16 * kShift) {
+        int kk = e / kShift, sx = e % kShift;
+        float2 a = apv[k0 + kk];
+        float2 ex = shx_tab[(size_t)(k0 + kk) * kShift + sx];
+        v = make_float2(a.x * ex.x - a.y * ex.y, a.x * ex.y + a.y * ex.x);
+      } else if (e < 32 * kShift) {
+        int e2 = e - 16 * kShift;
+        v = shy_tab[(size_t)(k0 + e2 / kShift) * kShift + e2 % kShift];
+      }
+      treg[u] = v;
+    }
+    creg = tid < 16 ? c2v[k0 + tid] : 0.f;
+  };
+  auto store_tab = [&](int tbuf) {
+    float2* tx = tabx(tbuf);
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      int e = tid + u * kThreads;
+      if (e < 32 * kShift) tx[e] = treg[u];  // tabx and taby are contiguous
+    }
+    if (tid < 16) c2s(tbuf)[tid] = creg;
+  };
+  // A chunk: row (sx, sy) = X'(k) e^{i k.s}, written as 16-byte vectors
+  const int row = tid;
+  const int rsx = row / kShift, rsy = row % kShift;
+  auto gen_a = [&](int buf, int tbuf) {
+    T* dst = As(buf) + row * P;
+    const float2* tx = tabx(tbuf);
+    const float2* ty = taby(tbuf);
+    constexpr int E = 16 / sizeof(T);  // elements per 16-byte store
+#pragma unroll
+    for (int g = 0; g < 32 / E; g++) {
+      T v[E];
+#pragma unroll
+      for (int q = 0; q < E / 2; q++) {
+        int kk = g * (E / 2) + q;
+        float re = 0.f, im = 0.f;
+        if (row < kShift * kShift) {
+          float2 a = tx[kk * kShift + rsx], e = ty[kk * kShift + rsy];
+          re = a.x * e.x - a.y * e.y;
+          im = a.x * e.y + a.y * e.x;
         }
-      } else {
-#pragma unroll 4
-        for (int kk = 0; kk < 32; kk++) dst[kk] = Elem<T>::cvt(0.f);
+        v[2 * q] = Elem<T>::cvt(re);
+        v[2 * q + 1] = Elem<T>::cvt(im);
       }
+      *reinterpret_cast<uint4*>(dst + g * E) = *reinterpret_cast<const uint4*>(v);
     }
-    // normalisation: sum_k |c_p(k)|^2 |T_o(k)|^2 for this thread's template
-    {
-      int c = tid & (kCols - 1), h = tid >> 7;
-      const T* src = Bs + c * P + 16 * h;
-#pragma unroll
-      for (int kk = 0; kk < 8; kk++) {
-        float re = Elem<T>::f(src[2 * kk]), im = Elem<T>::f(src[2 * kk + 1]);
-        my_nrm += c2s[8 * h + kk] * (re * re + im * im);
-      }
+  };
+
+  // prologue: tables for chunks 0 and 1, operands for chunk 0
+  load_tab(0);
+  store_tab(0);
+  if (nchunk > 1) {
+    load_tab(1);
+    store_tab(1);
+  }
+  load_b(0);
+  store_b(0);
+  __syncthreads();
+  gen_a(0, 0);
+  norm_b(0);
+  __syncthreads();
+
+  for (int kc = 0; kc < nchunk; kc++) {
+    const int cur = kc & 1, nxt = cur ^ 1;
+    const bool more = kc + 1 < nchunk;
+    if (more) load_b(kc + 1);
+    if (kc + 2 < nchunk) load_tab(kc + 2);
+    mma_step<T>(As(cur), Bs(cur), acc, wave, lane);
+    if (more) {
+      gen_a(nxt, (kc + 1) % 3);
+      norm_b((kc + 1) % 3);
+      store_b(nxt);
     }
-    __syncthreads();
-    mma_step<T>(As, Bs, acc, wave, lane);
+    if (kc + 2 < nchunk) store_tab((kc + 2) % 3);
     __syncthreads();
   }
 
+  // normalisation partials: per_row pieces per template
+#pragma unroll
+  for (int u = 0; u < BP::per_thread; u++) nrm[bq * kCols + bc0 + u * bstep] = my_nrm[u];
+
   // ------------------------------------------------------------ epilogue
-  nrm[(tid >> 7) * kCols + (tid & (kCols - 1))] = my_nrm;
   float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
   const float s2 = sum2[p];
 #pragma unroll
@@ -318,7 +414,9 @@ __global__ void __launch_bounds__(kThreads)
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
       if (o < norient) {
-        float n = nrm[hc * 64 + c] + nrm[kCols + hc * 64 + c];
+        float n = 0.f;
+#pragma unroll
+        for (int q = 0; q < BP::per_row; q++) n += nrm[q * kCols + hc * 64 + c];
         float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         size_t out = (size_t)p * norient + o;
         score[out] = cc;

@@ -1,0 +1,188 @@
+! frealign_hip_mod.f90 -- ISO_C_BINDING interface to libfrealign_hip.so for a
+! Fortran host that keeps Frealign's card / par-file operator surface.
+!
+! Each interface is the C entry point of include/frealign_hip.h; the comment
+! names the reference routine it replaces.  Device buffers are TYPE(C_PTR)
+! (allocate them with fr_dev_alloc); host arrays are passed by reference as
+! the reference passes them.  A non-zero status is fatal, as STOP is in the
+! reference: use fr_check.
+module frealign_hip
+  use iso_c_binding
+  implicit none
+
+  interface
+    function fr_version() bind(C, name="fr_version")
+      import :: c_ptr
+      type(c_ptr) :: fr_version
+    end function
+
+    function fr_last_error() bind(C, name="fr_last_error")
+      import :: c_ptr
+      type(c_ptr) :: fr_last_error
+    end function
+
+    function fr_create(device, ctx) bind(C, name="fr_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: device
+      type(c_ptr) :: ctx
+      integer(c_int) :: fr_create
+    end function
+
+    function fr_destroy(ctx) bind(C, name="fr_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: fr_destroy
+    end function
+
+    function fr_synchronize(ctx) bind(C, name="fr_synchronize")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: fr_synchronize
+    end function
+
+    ! SEARCHANG + LIMITSYMM (searchang.f:8-150)
+    function fr_search_grid(rmm, ri, dangin, asym, nasym, pmask, asymstore, sang, cap, nsang, &
+                            iquadmax) bind(C, name="fr_search_grid")
+      import :: c_int, c_float, c_char
+      real(c_float), value :: rmm, ri, dangin
+      character(kind=c_char), value :: asym
+      integer(c_int), value :: nasym, asymstore, cap
+      integer(c_int) :: pmask(5), nsang, iquadmax
+      real(c_float) :: sang(3, *)
+      integer(c_int) :: fr_search_grid
+    end function
+
+    ! GETSYMMAT (getsymmat.f:8-97)
+    function fr_point_group(asym, nasym, symop, jsym, nsym, iredun) bind(C, name="fr_point_group")
+      import :: c_int, c_float, c_char
+      character(kind=c_char), value :: asym
+      integer(c_int), value :: nasym
+      real(c_float) :: symop(3, 3, 8)
+      integer(c_int) :: jsym(8), nsym, iredun
+      integer(c_int) :: fr_point_group
+    end function
+
+    ! rotation matrices of cc3m.f:48-65 / pextract.f:30-51
+    function fr_rotation(n, euler, amag, sm, dm) bind(C, name="fr_rotation")
+      import :: c_int, c_float
+      integer(c_int), value :: n
+      real(c_float) :: euler(3, *), sm(9), dm(9, *)
+      real(c_float), value :: amag
+      integer(c_int) :: fr_rotation
+    end function
+
+    ! LMAIN's r2c (lmain.f:459) + CTFAPPLY (ctfapply.f:8-161), batched
+    function fr_prep_batch(ctx, nsam, nb, d_data, d_ppar, d_pssnr, gpar, ictf, d_ctff, d_outc, &
+                           d_datc) bind(C, name="fr_prep_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_data, d_ppar, d_pssnr, d_ctff, d_outc, d_datc
+      integer(c_int), value :: nsam, nb, ictf
+      real(c_float) :: gpar(11)
+      integer(c_int) :: fr_prep_batch
+    end function
+
+    ! PEXTRACT (pextract.f:8-101)
+    function fr_pextract_batch(ctx, nsam, n, d_a3df, d_dm, d_pidx, d_ctff, d_outc) &
+        bind(C, name="fr_pextract_batch")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_dm, d_pidx, d_ctff, d_outc
+      integer(c_int), value :: nsam, n
+      integer(c_int) :: fr_pextract_batch
+    end function
+
+    ! CC3M (cc3m.f:8-169)
+    function fr_cc3m_batch(ctx, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, &
+                           maxr1, maxr2, rbfact, d_fsct, ring, d_score) bind(C, name="fr_cc3m_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, d_fsct, &
+                            d_score
+      integer(c_int), value :: nsam, n, maxr1, maxr2
+      real(c_float), value :: rbfact
+      real(c_float) :: ring(4)
+      integer(c_int) :: fr_cc3m_batch
+    end function
+
+    ! A3D3 flush -> PINSERT -> PINSERT_TRILIN (a3d3.f:107-143, pinsert.f, pinsert_trilin.f)
+    function fr_insert_batch(ctx, nsam, nb, d_datc, d_ctff, recb, half, nsym, jsym, symop, pbc, &
+                             boff, psize, d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd) &
+        bind(C, name="fr_insert_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_datc, d_ctff, d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd
+      integer(c_int), value :: nsam, nb, nsym
+      real(c_float) :: recb(10, *), symop(3, 3, *)
+      integer(c_int) :: half(*), jsym(*)
+      real(c_float), value :: pbc, boff, psize
+      integer(c_int) :: fr_insert_batch
+    end function
+
+    ! SHELTEST (sheltest.f:8-122)
+    function fr_sheltest(ctx, nsam, istp, d_a, d_d, d_s, d_v, fmask, npix, npixt, dfsc, pssnr, &
+                         nstp) bind(C, name="fr_sheltest")
+      import :: c_int, c_float, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a, d_d, d_s, d_v
+      integer(c_int), value :: nsam, istp
+      real(c_float), value :: fmask
+      integer(c_int) :: npix(*), npixt(*), nstp
+      real(c_double) :: dfsc(*)
+      real(c_float) :: pssnr(*)
+      integer(c_int) :: fr_sheltest
+    end function
+
+    ! STORESHIFT (storeshift.f:43-66)
+    function fr_storeshift(ctx, nsam, std, vtd, iredun, d_a, d_s, d_d, d_v, d_b, d_c) &
+        bind(C, name="fr_storeshift")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a, d_s, d_d, d_v, d_b, d_c
+      integer(c_int), value :: nsam, iredun
+      real(c_double), value :: std, vtd
+      integer(c_int) :: fr_storeshift
+    end function
+
+    ! PSEARCH -> CCP (psearch.f:8-121, ccp.f:8-275)
+    function fr_search_layout(ctx, nsam, maxr1, maxr2, kc, kpad) bind(C, name="fr_search_layout")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsam, maxr1, maxr2
+      integer(c_int) :: kc, kpad
+      integer(c_int) :: fr_search_layout
+    end function
+
+    function fr_search_bank(ctx, nsam, maxr1, maxr2, d_a3df, norient, d_dm, dtype, d_bank) &
+        bind(C, name="fr_search_bank")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_dm, d_bank
+      integer(c_int), value :: nsam, maxr1, maxr2, norient, dtype
+      integer(c_int) :: fr_search_bank
+    end function
+
+    function fr_search_batch(ctx, nsam, maxr1, maxr2, rbfact, norient, d_bank, dtype, group, np, &
+                             d_outc, d_ctff, topk, d_top_o, d_top_row, d_top_score) &
+        bind(C, name="fr_search_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_bank, d_outc, d_ctff, d_top_o, d_top_row, d_top_score
+      integer(c_int), value :: nsam, maxr1, maxr2, norient, dtype, group, np, topk
+      real(c_float), value :: rbfact
+      integer(c_int) :: fr_search_batch
+    end function
+  end interface
+
+contains
+
+  ! STOP on a non-zero status with the library's message, as the reference
+  ! stops on its own fatal conditions.
+  subroutine fr_check(status, what)
+    integer(c_int), intent(in) :: status
+    character(len=*), intent(in) :: what
+    character(kind=c_char), pointer :: msg(:)
+    integer :: n
+    if (status == 0) return
+    call c_f_pointer(fr_last_error(), msg, [512])
+    n = 0
+    do while (n < 512)
+      if (msg(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write(*, '(A,": ",512A1)') what, msg(1:n)
+    stop 'frealign_hip: fatal error'
+  end subroutine
+end module frealign_hip
