@@ -34,7 +34,7 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kRows = 256;     // 15x15 = 225 shift rows, padded
-constexpr int kCols = 128;     // templates per block
+constexpr int kCols = 256;     // templates per block
 constexpr int kKStep = 32;     // reals per K step (16 complex coefficients)
 constexpr int kShift = 15;     // shift grid edge (-7..7 on the NSAMR grid)
 constexpr int kInner = 6;      // argmax window (-6..6), ccp.f:229-241
@@ -131,23 +131,83 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) sum2[p] = red[0] + red[1] + red[2] + red[3];
 }
 
-// ---------------------------------------------------------------- GEMM
+// ---------------------------------------------------------------- norms
+// sum_k |c_p(k)|^2 |T_o(k)|^2 for every (particle, orientation): the
+// normalisation SUM of ccp.f:211-214 with the CTF folded on the particle.
+// Tiles of 64 particles x 128 templates; K staged through LDS.
 template <typename T>
-__device__ inline void mma_step(const T* As, const T* Bs, floatx4 (&acc)[4][8], int wave,
+__global__ void __launch_bounds__(kThreads)
+    k_search_norm(int norient, int kpad, int np, const T* __restrict__ bank,
+                  const float* __restrict__ c2, float* nrm) {
+  constexpr int TP = 64, TO = 128, KS = 32;  // particles, templates, complex per step
+  __shared__ float cs[TP][KS + 1];
+  __shared__ float ts[TO][KS + 1];
+  const int half = kpad / 2;
+  const int o0 = blockIdx.x * TO, p0 = blockIdx.y * TP;
+  const int tid = threadIdx.x;
+  // thread -> (4 particles) x (8 templates)
+  const int tp = (tid & 15) * 4, to = (tid >> 4) * 8;
+  float acc[4][8] = {};
+  for (int k0 = 0; k0 < half; k0 += KS) {
+    for (int e = tid; e < TP * KS; e += kThreads) {
+      int pp = e / KS, kk = e % KS;
+      int p = p0 + pp, k = k0 + kk;
+      cs[pp][kk] = (p < np && k < half) ? c2[(size_t)p * half + k] : 0.f;
+    }
+    for (int e = tid; e < TO * KS; e += kThreads) {
+      int oo = e / KS, kk = e % KS;
+      int o = o0 + oo, k = k0 + kk;
+      float v = 0.f;
+      if (o < norient && k < half) {
+        float re = Elem<T>::f(bank[(size_t)o * kpad + 2 * k]);
+        float im = Elem<T>::f(bank[(size_t)o * kpad + 2 * k + 1]);
+        v = re * re + im * im;
+      }
+      ts[oo][kk] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < KS; kk++) {
+      float a[4], b[8];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = cs[tp + i][kk];
+#pragma unroll
+      for (int j = 0; j < 8; j++) b[j] = ts[to + j][kk];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      int p = p0 + tp + i, o = o0 + to + j;
+      if (p < np && o < norient) nrm[(size_t)p * norient + o] = acc[i][j];
+    }
+}
+
+// ---------------------------------------------------------------- GEMM
+constexpr int kGemmThreads = 512;  // 8 waves: 4 row groups x 2 column groups
+
+template <typename T>
+__device__ inline void mma_step(const T* As, const T* Bs, floatx4 (&acc)[4][8], int wr, int wc,
                                 int lane);
 
 template <>
 __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
-                                        floatx4 (&acc)[4][8], int wave, int lane) {
+                                        floatx4 (&acc)[4][8], int wr, int wc, int lane) {
   constexpr int P = Elem<__bf16>::kPad;
   const int lr = lane & 15, kq = (lane >> 4) * 8;
   bf16x8 a[4], b[8];
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
-    a[rt] = *reinterpret_cast<const bf16x8*>(As + (wave * 64 + rt * 16 + lr) * P + kq);
+    a[rt] = *reinterpret_cast<const bf16x8*>(As + (wr * 64 + rt * 16 + lr) * P + kq);
 #pragma unroll
   for (int ct = 0; ct < 8; ct++)
-    b[ct] = *reinterpret_cast<const bf16x8*>(Bs + (ct * 16 + lr) * P + kq);
+    b[ct] = *reinterpret_cast<const bf16x8*>(Bs + (wc * 128 + ct * 16 + lr) * P + kq);
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
@@ -157,16 +217,16 @@ __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
 
 template <>
 __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4 (&acc)[4][8],
-                                       int wave, int lane) {
+                                       int wr, int wc, int lane) {
   constexpr int P = Elem<float>::kPad;
   const int lr = lane & 15, kq = lane >> 4;
 #pragma unroll
   for (int kk = 0; kk < kKStep; kk += 4) {
     float a[4], b[8];
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++) a[rt] = As[(wave * 64 + rt * 16 + lr) * P + kk + kq];
+    for (int rt = 0; rt < 4; rt++) a[rt] = As[(wr * 64 + rt * 16 + lr) * P + kk + kq];
 #pragma unroll
-    for (int ct = 0; ct < 8; ct++) b[ct] = Bs[(ct * 16 + lr) * P + kk + kq];
+    for (int ct = 0; ct < 8; ct++) b[ct] = Bs[(wc * 128 + ct * 16 + lr) * P + kk + kq];
 #pragma unroll
     for (int rt = 0; rt < 4; rt++)
 #pragma unroll
@@ -175,20 +235,19 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
   }
 }
 
-// LDS layout of k_search_gemm: two operand buffers (A rows x P, B rows x P),
-// three phase-table buffers, then the normalisation partials.  The epilogue
-// map [kRows][65] floats aliases the start of the region.
+// LDS layout of k_search_gemm: two operand buffers (A rows x P, B cols x P)
+// and three phase-table buffers.  The epilogue map [kRows][65] floats
+// aliases the start of the region.
 template <typename T>
 struct Lds {
   static constexpr size_t a_bytes = (size_t)kRows * Elem<T>::kPad * sizeof(T);
   static constexpr size_t b_bytes = (size_t)kCols * Elem<T>::kPad * sizeof(T);
-  static constexpr size_t op_bytes = a_bytes + b_bytes;                   // one buffer
-  static constexpr size_t tab_one = 2 * 16 * kShift * sizeof(float2) + 16 * sizeof(float);
+  static constexpr size_t op_bytes = a_bytes + b_bytes;  // one buffer
+  static constexpr size_t tab_one = 2 * 16 * kShift * sizeof(float2);
   static constexpr size_t tab_off = 2 * op_bytes;
   static constexpr size_t ep_bytes = (size_t)kRows * 65 * sizeof(float);
   static constexpr size_t end_tab = tab_off + 3 * tab_one;
-  static constexpr size_t nrm_off = ((end_tab > ep_bytes ? end_tab : ep_bytes) + 15) / 16 * 16;
-  static constexpr size_t total = nrm_off + 4 * kCols * sizeof(float) * 2;
+  static constexpr size_t total = end_tab > ep_bytes ? end_tab : ep_bytes;
 };
 
 template <typename T>
@@ -199,16 +258,16 @@ constexpr size_t lds_total() {
 // B chunk transfer: kCols templates x 32 reals, in 16-byte pieces.
 template <typename T>
 struct BPieces {
-  static constexpr int per_row = 32 * sizeof(T) / 16;  // 4 (bf16) or 8 (fp32)
-  static constexpr int total = kCols * per_row;         // 512 or 1024
-  static constexpr int per_thread = total / kThreads;   // 2 or 4
-  static constexpr int elems = 16 / sizeof(T);          // 8 or 4 elements per piece
+  static constexpr int per_row = 32 * sizeof(T) / 16;      // 4 (bf16) or 8 (fp32)
+  static constexpr int total = kCols * per_row;             // 1024 or 2048
+  static constexpr int per_thread = total / kGemmThreads;   // 2 or 4
+  static constexpr int elems = 16 / sizeof(T);              // elements per piece
 };
 
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kGemmThreads)
     k_search_gemm(int norient, int kpad, int np, int ntile, const T* __restrict__ bank,
-                  const float2* __restrict__ ap, const float* __restrict__ c2,
+                  const float2* __restrict__ ap, const float* __restrict__ nrm_po,
                   const float* __restrict__ sum2, const float2* __restrict__ shx_tab,
                   const float2* __restrict__ shy_tab, float* score, int* best_row,
                   float* raw_peak) {
@@ -232,14 +291,12 @@ __global__ void __launch_bounds__(kThreads)
     return reinterpret_cast<float2*>(smem_raw + L::tab_off + buf * L::tab_one);
   };
   auto taby = [&](int buf) { return tabx(buf) + 16 * kShift; };
-  auto c2s = [&](int buf) { return reinterpret_cast<float*>(taby(buf) + 16 * kShift); };
-  float* nrm = reinterpret_cast<float*>(smem_raw + L::nrm_off);  // [pieces-per-row][kCols]
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wr = wave & 3, wc = wave >> 2;
   const int half = kpad / 2;
   const int nchunk = half / 16;
   const float2* apv = ap + (size_t)p * half;
-  const float* c2v = c2 + (size_t)p * half;
 
   floatx4 acc[4][8];
 #pragma unroll
@@ -247,23 +304,17 @@ __global__ void __launch_bounds__(kThreads)
 #pragma unroll
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  // this thread's B pieces: piece q of templates c0 + u * (kCols / per_thread)
+  // this thread's B pieces: piece bq of templates bc0 + u * bstep
   const int bq = tid % BP::per_row;
   const int bc0 = tid / BP::per_row;
-  constexpr int bstep = kThreads / BP::per_row;
-  float my_nrm[BP::per_thread];
-#pragma unroll
-  for (int u = 0; u < BP::per_thread; u++) my_nrm[u] = 0.f;
+  constexpr int bstep = kGemmThreads / BP::per_row;
   uint4 breg[BP::per_thread];
-  // phase-table staging registers: entries e = tid and tid + 256 of 2 x 240 (+16 |c|^2)
-  float2 treg[2];
-  float creg = 0.f;
+  float2 treg;
 
   auto load_b = [&](int chunk) {
 #pragma unroll
     for (int u = 0; u < BP::per_thread; u++) {
-      int c = bc0 + u * bstep;
-      int o = o0 + c;
+      int o = o0 + bc0 + u * bstep;
       if (o < norient)
         breg[u] = *reinterpret_cast<const uint4*>(bank + (size_t)o * kpad + 32 * chunk +
                                                    bq * BP::elems);
@@ -274,68 +325,41 @@ __global__ void __launch_bounds__(kThreads)
   auto store_b = [&](int buf) {
     T* dst = Bs(buf);
 #pragma unroll
-    for (int u = 0; u < BP::per_thread; u++) {
-      int c = bc0 + u * bstep;
-      *reinterpret_cast<uint4*>(dst + c * P + bq * BP::elems) = breg[u];
-    }
+    for (int u = 0; u < BP::per_thread; u++)
+      *reinterpret_cast<uint4*>(dst + (bc0 + u * bstep) * P + bq * BP::elems) = breg[u];
   };
-  // normalisation sum_k |c(k)|^2 |T_o(k)|^2 from the pieces this thread holds
-  auto norm_b = [&](int tbuf) {
-    const float* cc = c2s(tbuf);
-#pragma unroll
-    for (int u = 0; u < BP::per_thread; u++) {
-      const T* e = reinterpret_cast<const T*>(&breg[u]);
-      float acc_n = 0.f;
-#pragma unroll
-      for (int q = 0; q < BP::elems / 2; q++) {
-        float re = Elem<T>::f(e[2 * q]), im = Elem<T>::f(e[2 * q + 1]);
-        acc_n += cc[bq * (BP::elems / 2) + q] * (re * re + im * im);
-      }
-      my_nrm[u] += acc_n;
-    }
-  };
+  // one phase-table entry per thread: 240 X'(k) e^{i I sx} then 240 e^{i J sy}
   auto load_tab = [&](int chunk) {
-    int k0 = 16 * chunk;
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      int e = tid + u * kThreads;  // 0..479 used
-      float2 v = make_float2(0.f, 0.f);
-      if (e < 16 * kShift) {
-        int kk = e / kShift, sx = e % kShift;
-        float2 a = apv[k0 + kk];
-        float2 ex = shx_tab[(size_t)(k0 + kk) * kShift + sx];
-        v = make_float2(a.x * ex.x - a.y * ex.y, a.x * ex.y + a.y * ex.x);
-      } else if (e < 32 * kShift) {
-        int e2 = e - 16 * kShift;
-        v = shy_tab[(size_t)(k0 + e2 / kShift) * kShift + e2 % kShift];
-      }
-      treg[u] = v;
+    const int k0 = 16 * chunk;
+    float2 v = make_float2(0.f, 0.f);
+    if (tid < 16 * kShift) {
+      int kk = tid / kShift, sx = tid % kShift;
+      float2 a = apv[k0 + kk];
+      float2 ex = shx_tab[(size_t)(k0 + kk) * kShift + sx];
+      v = make_float2(a.x * ex.x - a.y * ex.y, a.x * ex.y + a.y * ex.x);
+    } else if (tid < 32 * kShift) {
+      int e2 = tid - 16 * kShift;
+      v = shy_tab[(size_t)(k0 + e2 / kShift) * kShift + e2 % kShift];
     }
-    creg = tid < 16 ? c2v[k0 + tid] : 0.f;
+    treg = v;
   };
   auto store_tab = [&](int tbuf) {
-    float2* tx = tabx(tbuf);
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      int e = tid + u * kThreads;
-      if (e < 32 * kShift) tx[e] = treg[u];  // tabx and taby are contiguous
-    }
-    if (tid < 16) c2s(tbuf)[tid] = creg;
+    if (tid < 32 * kShift) tabx(tbuf)[tid] = treg;  // tabx and taby are contiguous
   };
-  // A chunk: row (sx, sy) = X'(k) e^{i k.s}, written as 16-byte vectors
-  const int row = tid;
+  // A chunk: row (sx, sy) = X'(k) e^{i k.s}; two threads per row, 8 complex each
+  const int row = tid >> 1, hk = (tid & 1) * 8;
   const int rsx = row / kShift, rsy = row % kShift;
   auto gen_a = [&](int buf, int tbuf) {
-    T* dst = As(buf) + row * P;
+    T* dst = As(buf) + row * P + 2 * hk;
     const float2* tx = tabx(tbuf);
     const float2* ty = taby(tbuf);
     constexpr int E = 16 / sizeof(T);  // elements per 16-byte store
 #pragma unroll
-    for (int g = 0; g < 32 / E; g++) {
+    for (int g = 0; g < 16 / E; g++) {
       T v[E];
 #pragma unroll
       for (int q = 0; q < E / 2; q++) {
-        int kk = g * (E / 2) + q;
+        int kk = hk + g * (E / 2) + q;
         float re = 0.f, im = 0.f;
         if (row < kShift * kShift) {
           float2 a = tx[kk * kShift + rsx], e = ty[kk * kShift + rsy];
@@ -360,7 +384,6 @@ __global__ void __launch_bounds__(kThreads)
   store_b(0);
   __syncthreads();
   gen_a(0, 0);
-  norm_b(0);
   __syncthreads();
 
   for (int kc = 0; kc < nchunk; kc++) {
@@ -368,39 +391,36 @@ __global__ void __launch_bounds__(kThreads)
     const bool more = kc + 1 < nchunk;
     if (more) load_b(kc + 1);
     if (kc + 2 < nchunk) load_tab(kc + 2);
-    mma_step<T>(As(cur), Bs(cur), acc, wave, lane);
+    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
     if (more) {
       gen_a(nxt, (kc + 1) % 3);
-      norm_b((kc + 1) % 3);
       store_b(nxt);
     }
     if (kc + 2 < nchunk) store_tab((kc + 2) % 3);
     __syncthreads();
   }
 
-  // normalisation partials: per_row pieces per template
-#pragma unroll
-  for (int u = 0; u < BP::per_thread; u++) nrm[bq * kCols + bc0 + u * bstep] = my_nrm[u];
-
   // ------------------------------------------------------------ epilogue
   float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
   const float s2 = sum2[p];
 #pragma unroll
-  for (int hc = 0; hc < 2; hc++) {
-    __syncthreads();
+  for (int q4 = 0; q4 < 4; q4++) {  // 64-column quarters of the 256 templates
+    if (q4) __syncthreads();
+    if (wc == (q4 >> 1)) {
 #pragma unroll
-    for (int rt = 0; rt < 4; rt++)
+      for (int rt = 0; rt < 4; rt++)
 #pragma unroll
-      for (int ct = 0; ct < 4; ct++)
+        for (int ct = 0; ct < 4; ct++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          int r = wave * 64 + rt * 16 + (lane >> 4) * 4 + i;
-          int c = ct * 16 + (lane & 15);
-          ep[r * 65 + c] = acc[rt][ct + 4 * hc][i];
-        }
+          for (int i = 0; i < 4; i++) {
+            int r = wr * 64 + rt * 16 + (lane >> 4) * 4 + i;
+            int c = ct * 16 + (lane & 15);
+            ep[r * 65 + c] = acc[rt][ct + 4 * (q4 & 1)][i];
+          }
+    }
     __syncthreads();
     if (tid < 64) {
-      const int c = tid, o = o0 + hc * 64 + c;
+      const int c = tid, o = o0 + q4 * 64 + c;
       // ccp.f:229-241: argmax over IX (outer) and IY (inner) in -6..6, first max wins
       float best = -3.0e38f;
       int bx = 0, by = 0;
@@ -414,11 +434,9 @@ __global__ void __launch_bounds__(kThreads)
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
       if (o < norient) {
-        float n = 0.f;
-#pragma unroll
-        for (int q = 0; q < BP::per_row; q++) n += nrm[q * kCols + hc * 64 + c];
-        float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         size_t out = (size_t)p * norient + o;
+        float n = nrm_po[out];
+        float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         score[out] = cc;
         best_row[out] = (bx + 7) * kShift + (by + 7);
         if (raw_peak) raw_peak[out] = best;
@@ -520,8 +538,21 @@ int launch_search_feats(Ctx& c, int nsam, int np, int kc, int kpad, int maxr22, 
   return 0;
 }
 
+int launch_search_norm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
+                       const float* c2, float* nrm) {
+  dim3 grid((norient + 127) / 128, (np + 63) / 64);
+  if (bf16)
+    hipLaunchKernelGGL(k_search_norm<__bf16>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
+                       reinterpret_cast<const __bf16*>(bank), c2, nrm);
+  else
+    hipLaunchKernelGGL(k_search_norm<float>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
+                       reinterpret_cast<const float*>(bank), c2, nrm);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_search_gemm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
-                       const float* ap, const float* c2, const float* sum2, const float* shx_tab,
+                       const float* ap, const float* nrm, const float* sum2, const float* shx_tab,
                        const float* shy_tab, float* score, int* best_row, float* raw_peak) {
   const int ntile = (norient + kCols - 1) / kCols;
   const int ntile8 = (ntile + 7) / 8 * 8;
@@ -536,9 +567,9 @@ int launch_search_gemm(Ctx& c, int norient, int kpad, int np, const void* bank, 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       attr = true;
     }
-    hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kThreads), lds, c.stream,
-                       norient, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank),
-                       reinterpret_cast<const float2*>(ap), c2, sum2,
+    hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
+                       c.stream, norient, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank),
+                       reinterpret_cast<const float2*>(ap), nrm, sum2,
                        reinterpret_cast<const float2*>(shx_tab),
                        reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
   } else {
@@ -548,9 +579,9 @@ int launch_search_gemm(Ctx& c, int norient, int kpad, int np, const void* bank, 
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
       attr = true;
     }
-    hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kThreads), lds, c.stream,
-                       norient, kpad, np, ntile, reinterpret_cast<const float*>(bank),
-                       reinterpret_cast<const float2*>(ap), c2, sum2,
+    hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
+                       c.stream, norient, kpad, np, ntile, reinterpret_cast<const float*>(bank),
+                       reinterpret_cast<const float2*>(ap), nrm, sum2,
                        reinterpret_cast<const float2*>(shx_tab),
                        reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
   }
