@@ -34,6 +34,7 @@ SIGNATURES = {
     "fr_prep_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp], ci),
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
+    "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
     "fr_insert_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp, vp,
                          vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),

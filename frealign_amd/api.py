@@ -161,6 +161,8 @@ class Context:
             raise ValueError("pidx out of range")
         if ctff.numel() != 2 * spec.numel() or fsct.numel() < nsam // 2 + 1:
             raise ValueError("ctff/fsct sizes")
+        if a3df.numel() != nsam * nsam * jc:
+            raise ValueError("a3df must be the NSAM x NSAM x JC half-volume")
         r = np.asarray(ring, dtype=np.float32)
         out = torch.empty(n, dtype=torch.float32, device=a3df.device)
         self._sync_stream()
@@ -168,6 +170,37 @@ class Context:
                                       _p(dm), _p(shift), _p(amag), maxr1, maxr2, float(rbfact),
                                       _p(fsct), _p(r), _p(out)), "fr_cc3m_batch")
         return out
+
+    def presb(self, nsam: int, a3df, spec, ctff, pidx, dm, shift, amag, ring,
+              rbin=None) -> torch.Tensor:
+        """PRESB (presb.f:8-160) for n items; returns the n x NSAM RBIN array,
+        updated in place when `rbin` is given (shells with no point keep their value)."""
+        jc = nsam // 2 + 1
+        for t, nm, dt in ((a3df, "a3df", torch.complex64), (spec, "spec", torch.complex64),
+                          (ctff, "ctff", torch.complex64), (pidx, "pidx", torch.int32),
+                          (dm, "dm", torch.float32), (shift, "shift", torch.float32),
+                          (amag, "amag", torch.float32)):
+            _dev(t, dt, nm)
+        n = pidx.numel()
+        npart = spec.numel() // (nsam * jc)
+        if dm.numel() != 9 * n or shift.numel() != 2 * n or amag.numel() != n:
+            raise ValueError("dm/shift/amag sizes do not match pidx")
+        if n and (int(pidx.max()) >= npart or int(pidx.min()) < 0):
+            raise ValueError("pidx out of range")
+        if ctff.numel() != 2 * spec.numel() or a3df.numel() != nsam * nsam * jc:
+            raise ValueError("ctff/a3df sizes")
+        if rbin is None:
+            rbin = torch.zeros((n, nsam), dtype=torch.float32, device=a3df.device)
+        else:
+            _dev(rbin, torch.float32, "rbin")
+            if rbin.numel() != n * nsam:
+                raise ValueError("rbin must hold n x NSAM floats")
+        r = np.asarray(ring, dtype=np.float32)
+        self._sync_stream()
+        N.check(N.lib().fr_presb_batch(self.h, nsam, n, _p(a3df), _p(spec), _p(ctff), _p(pidx),
+                                       _p(dm), _p(shift), _p(amag), _p(r), _p(rbin)),
+                "fr_presb_batch")
+        return rbin
 
     # ------------------------------------------------------------ insertion
     def insert(self, nsam: int, datc, ctff, recb: np.ndarray, half: np.ndarray, sym, pbc: float,

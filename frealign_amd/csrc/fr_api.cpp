@@ -29,6 +29,9 @@ struct fr_ctx : public fr::Ctx {
   int* d_kidx = nullptr;
   float* d_shx = nullptr;
   float* d_shy = nullptr;
+  // DFT table of the fused CC3M, cached per NSAM
+  int dft_nsam = 0;
+  float* d_dft = nullptr;
 };
 
 namespace fr {
@@ -188,6 +191,7 @@ int fr_destroy(fr_ctx* c) {
   for (int k = 0; k < 8; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
   if (c->d_kidx) (void)hipFree(c->d_kidx);
+  if (c->d_dft) (void)hipFree(c->d_dft);
   if (c->d_shx) (void)hipFree(c->d_shx);
   if (c->d_shy) (void)hipFree(c->d_shy);
   delete c;
@@ -283,6 +287,32 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
            "null argument");
   FR_CHECK(maxr1 >= 0 && maxr2 > maxr1, "bad resolution band %d..%d", maxr1, maxr2);
   if (n == 0) return 0;
+  Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
+  if (fr::cc3m_fused_ok(nsam, maxr2)) {
+    if (c->dft_nsam != nsam) {
+      std::vector<float> tab;
+      fr::cc3m_fused_table(nsam, tab);
+      if (c->d_dft) {
+        FR_HIP(hipStreamSynchronize(c->stream));
+        FR_HIP(hipFree(c->d_dft));
+        c->d_dft = nullptr;
+        c->dft_nsam = 0;
+      }
+      FR_HIP(hipMalloc(&c->d_dft, sizeof(float) * tab.size()));
+      FR_HIP(hipMemcpy(c->d_dft, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+      c->dft_nsam = nsam;
+    }
+    int pslot = c->prof_begin(kProfCc3m);
+    for (int o = 0; o < n; o += 65535) {
+      a.n = n - o < 65535 ? n - o : 65535;
+      if (int r = launch_cc3m_fused(*c, a, d_a3df, d_spec, d_ctff, d_pidx + o,
+                                    d_dm + 9 * (size_t)o, d_shift + 2 * (size_t)o, d_amag + o,
+                                    d_fsct, c->d_dft, d_score + o))
+        return r;
+    }
+    c->prof_end(pslot);
+    return 0;
+  }
   const size_t img = (size_t)nsam * (nsam + 2);
   // process in chunks so the FFT work buffer stays bounded
   const int chunk = (int)((size_t)512 * 1024 * 1024 / (img * sizeof(float)));
@@ -290,7 +320,6 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   int err = 0;
   float* work = (float*)c->slot(0, img * sizeof(float) * (size_t)(n < step ? n : step), &err);
   if (!work) return err;
-  Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
   int pslot = c->prof_begin(kProfCc3m);
   for (int o = 0; o < n; o += step) {
     a.n = n - o < step ? n - o : step;
@@ -299,6 +328,32 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
       return r;
   }
   c->prof_end(pslot);
+  return 0;
+}
+
+int fr_presb_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                   const float* d_ctff, const int* d_pidx, const float* d_dm,
+                   const float* d_shift, const float* d_amag, const float* ring, float* d_rbin) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(n >= 0 && d_a3df && d_spec && d_ctff && d_pidx && d_dm && d_shift && d_amag && ring &&
+               d_rbin,
+           "null argument");
+  FR_CHECK(ring[3] != 0.0f, "HALFW must be non-zero");
+  if (n == 0) return 0;
+  const size_t img = (size_t)nsam * (nsam + 2);
+  const int chunk = (int)((size_t)512 * 1024 * 1024 / (img * sizeof(float)));
+  const int step = chunk < 1 ? 1 : (chunk > 65535 ? 65535 : chunk);
+  int err = 0;
+  float* work = (float*)c->slot(0, img * sizeof(float) * (size_t)(n < step ? n : step), &err);
+  if (!work) return err;
+  for (int o = 0; o < n; o += step) {
+    int m = n - o < step ? n - o : step;
+    if (int r = launch_presb(*c, nsam, m, d_a3df, d_spec, d_ctff, d_pidx + o, d_dm + 9 * (size_t)o,
+                             d_shift + 2 * (size_t)o, d_amag + o, ring[0], ring[1], ring[2],
+                             ring[3], work, d_rbin + (size_t)o * nsam))
+      return r;
+  }
   return 0;
 }
 

@@ -14,6 +14,9 @@
 #include "fr_extract.h"
 #include "fr_kernels.h"
 
+#include <cmath>
+#include <vector>
+
 namespace fr {
 
 namespace {
@@ -141,7 +144,374 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// presb.f:95-123: per-shell Re(SAMP conj(P)) / sqrt(sum|SAMP|^2 sum|P|^2) over
+// 1 <= r^2 < (NSAM/2)^2; shells with no point leave RBIN untouched.
+__global__ void __launch_bounds__(kThreads)
+    k_presb_bins(int nsam, const float2* __restrict__ spec, const int* __restrict__ pidx,
+                 const float* __restrict__ shift, const float* __restrict__ work, float* rbin) {
+  __shared__ float bins[3 * 260];
+  __shared__ int cnt[260];
+  const int jc = nsam / 2 + 1, nsamh = nsam / 2;
+  const size_t ic = (size_t)nsam * jc;
+  const int it = blockIdx.x;
+  const int nb = nsamh + 2;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+    bins[3 * k] = bins[3 * k + 1] = bins[3 * k + 2] = 0.0f;
+    cnt[k] = 0;
+  }
+  __syncthreads();
+  const float2* sp = spec + (size_t)pidx[it] * ic;
+  const float2* pw = reinterpret_cast<const float2*>(work) + (size_t)it * ic;
+  const float shx = shift[2 * it], shy = shift[2 * it + 1];
+  const int r2 = nsamh * nsamh;
+  for (int e = threadIdx.x; e < (int)ic; e += blockDim.x) {
+    int i = e % jc, m = e / jc;
+    int j = m >= jc ? m - nsam : m;
+    int itemp = i * i + j * j;
+    if (itemp < 1 || itemp >= r2) continue;
+    float sn, cs;
+    sincosf(shx * (float)i + shy * (float)j, &sn, &cs);
+    float2 s = sp[e];
+    cf samp = cmul(cf{s.x, s.y}, cf{cs, sn});
+    float2 a = pw[e];
+    int ibin = (int)(sqrtf((float)itemp) + 0.5f);  // 0-based shell
+    atomicAdd(&bins[3 * ibin + 0], samp.re * a.x + samp.im * a.y);
+    atomicAdd(&bins[3 * ibin + 1], samp.re * samp.re + samp.im * samp.im);
+    atomicAdd(&bins[3 * ibin + 2], a.x * a.x + a.y * a.y);
+    atomicAdd(&cnt[ibin], 1);
+  }
+  __syncthreads();
+  float* rb = rbin + (size_t)it * nsam;
+  for (int i = threadIdx.x; i < nsamh; i += blockDim.x) {
+    if (cnt[i] != 0) {
+      rb[2 * i] = bins[3 * i] / sqrtf(bins[3 * i + 1] * bins[3 * i + 2]);
+      rb[2 * i + 1] = 1.0f;
+    }
+  }
+}
+
 }  // namespace
+
+// ------------------------------------------------------------ fused CC3M
+// The same CC3M score with the two full-size transforms pruned to the band
+// they touch.  Only columns I < 32 and rows |J| < 32 of the half-plane are
+// non-zero on the way in (the annulus of cc3m.f:74-112), and only those are
+// read on the way out (cc3m.f:117-150), so with the table
+//   T[n][2i] = cos(2 pi i n/N),  T[n][2i+1] = -sin(2 pi i n/N),  i < 32,
+// every pass is a product with T on the matrix cores (FP32 MFMA):
+//   column c2c   Y  (N x 64) = T   x B(X)       (J pairs folded: K = 64)
+//   row c2r      x^T(16n x 16m) = T x (w Y)^T   (w = 1 at I = 0, else 2)
+//   MASKCOS2     in registers (ring mean from a first pass over ring tiles)
+//   row r2c      Z^T (64 x 16m) = T^T x x'
+//   column r2c   out(64 x 64) = T^T x Z         (C +- i S' gives +-J)
+// One workgroup holds one item: T, Y/Z and the small operands live in LDS.
+// The row-column order and the c2r conventions (Im ignored at I = 0) are
+// those of the FFT path, so the result agrees with it to FP32 rounding.
+namespace {
+
+constexpr int kFT = 512;     // threads: 8 waves
+constexpr int kFK = 64;      // reals per row: 32 frequencies
+constexpr int kFP = 68;      // LDS row pitch (floats): conflict-free MFMA operand reads
+constexpr int kFJ = 31;      // |J| <= 31
+constexpr float kPi = 3.1415926535897f;  // MASKCOS2's PI
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float fx4;
+
+__device__ inline fx4 mfma4(float a, float b, fx4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// squared wrapped distance range of the 16 coordinates v0..v0+15 from the
+// corner origin (MASKCOS2 with origin 1); tiles never straddle NSAM/2
+__device__ inline void tile_r2(int v0, int nsamh, int nsam, int& lo, int& hi) {
+  int a = v0 < nsamh ? v0 : v0 - nsam, b = a + 15;
+  int la = a * a, lb = b * b;
+  lo = (a <= 0 && b >= 0) ? 0 : (la < lb ? la : lb);
+  hi = la > lb ? la : lb;
+}
+
+__global__ void __launch_bounds__(kFT)
+    k_cc3m_fused(int nsam, int maxr12, int maxr22, float rbfact, const float2* __restrict__ a3df,
+                 const float* __restrict__ dm, const int* __restrict__ pidx,
+                 const float2* __restrict__ ctff, const float2* __restrict__ spec,
+                 const float* __restrict__ shift, const float* __restrict__ amag_arr,
+                 const float* __restrict__ fsct, const float* __restrict__ tab, float ri2,
+                 float ri3, float rih, float halfw, float* score) {
+  extern __shared__ __attribute__((aligned(16))) float lds[];
+  float* Ts = lds;                      // [nsam][kFP]
+  float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
+  float* Bs = Ys + nsam * kFP;          // [64][kFP]: B(X)^T, then out
+  float* bins = Bs + 64 * kFP;          // [4][nsam/2 + 2]
+  __shared__ float red_s[kFT / 64];
+  __shared__ int red_c[kFT / 64];
+
+  const int it = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int la = lane & 15, lg = lane >> 4;
+  const int jc = nsam / 2 + 1, nsamh = nsam / 2;
+  const size_t ic = (size_t)nsam * jc;
+  const int p = pidx[it];
+  const float amagp = amag_arr[it];
+  const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
+
+  // ---- stage T, clear the shell bins, fill X (annulus x CTF / NSAM^2) into Ys
+  for (int e = tid; e < nsam * (kFK / 4); e += kFT) {
+    int n = e / (kFK / 4), q = e % (kFK / 4);
+    *reinterpret_cast<float4*>(Ts + n * kFP + 4 * q) =
+        *reinterpret_cast<const float4*>(tab + (size_t)n * kFK + 4 * q);
+  }
+  const int nb = nsam / 2 + 2;
+  for (int k = tid; k < 4 * nb; k += kFT) bins[k] = 0.0f;
+  float2* Xs = reinterpret_cast<float2*>(Ys);  // [32][63]: I, J + 31
+  {
+    const float* d = dm + 9 * (size_t)it;
+    const float2* cp = ctff + (size_t)p * 2 * ic;
+    const float scal = 1.0f / (float)nsam / (float)nsam;
+    for (int e = tid; e < 32 * (2 * kFJ + 1); e += kFT) {
+      int i = e / (2 * kFJ + 1), j = e % (2 * kFJ + 1) - kFJ;
+      int itemp = i * i + j * j;
+      float2 out = make_float2(0.f, 0.f);
+      if (itemp >= maxr12 && itemp < maxr22) {
+        size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
+        cf a = ainterpo_dev(nsam, a3df, d, i, j);
+        float2 cr = cp[id], cl = cp[id + ic];
+        a = cmul(a, cf{cr.x + cl.x, cr.y - cl.y});
+        out = make_float2(a.re * scal, a.im * scal);
+      }
+      Xs[e] = out;
+    }
+  }
+  __syncthreads();
+  // B(X)^T[c][k]: column c = (I, re|im), k = (|J|, cos|sin)
+  for (int e = tid; e < 64 * 64; e += kFT) {
+    int c = e >> 6, k = e & 63;
+    int i = c >> 1, jj = k >> 1;
+    float2 xp = Xs[i * (2 * kFJ + 1) + kFJ + jj];
+    float2 xm = Xs[i * (2 * kFJ + 1) + kFJ - jj];
+    float2 P = jj ? make_float2(xp.x + xm.x, xp.y + xm.y) : xp;
+    float2 Q = jj ? make_float2(xp.x - xm.x, xp.y - xm.y) : make_float2(0.f, 0.f);
+    float v;
+    if (!(c & 1)) v = (k & 1) ? Q.y : P.x;   // Y_re = P_re T_re + Q_im T_im
+    else v = (k & 1) ? -Q.x : P.y;           // Y_im = P_im T_re - Q_re T_im
+    Bs[c * kFP + k] = v;
+  }
+  __syncthreads();
+
+  // ---- column c2c: Y[m][c] = sum_k T[m][k] B[k][c], times the c2r weight
+  const int mt = nsam / 16;
+  for (int t = wave; t < mt * 4; t += kFT / 64) {
+    const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
+    fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < kFK / 4; ks++)
+      acc = mfma4(Ts[(m0 + la) * kFP + 4 * ks + lg], Bs[(c0 + la) * kFP + 4 * ks + lg], acc);
+    const float w = (c0 + la) < 2 ? 1.0f : 2.0f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) Ys[(m0 + 4 * lg + i) * kFP + c0 + la] = acc[i] * w;
+  }
+  __syncthreads();
+
+  // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3' (tiles that can reach it)
+  float rsum = 0.f;
+  int rcnt = 0;
+  for (int rb = wave; rb < mt; rb += kFT / 64) {
+    const int m0 = rb * 16;
+    int mlo, mhi;
+    tile_r2(m0, nsamh, nsam, mlo, mhi);
+    float yv[kFK / 4];
+#pragma unroll
+    for (int ks = 0; ks < kFK / 4; ks++) yv[ks] = Ys[(m0 + la) * kFP + 4 * ks + lg];
+    const int m = m0 + la;
+    const int dy = m < nsamh ? m : m - nsam;
+    for (int nt = 0; nt < mt; nt++) {
+      const int n0 = nt * 16;
+      int nlo, nhi;
+      tile_r2(n0, nsamh, nsam, nlo, nhi);
+      if ((float)(nlo + mlo) > ri3p || (float)(nhi + mhi) <= ri2p) continue;
+      fx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kFK / 4; ks++)
+        acc = mfma4(Ts[(n0 + la) * kFP + 4 * ks + lg], yv[ks], acc);
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
+        float crad2 = (float)(dx * dx + dy * dy);
+        if (crad2 > ri2p && crad2 <= ri3p) {
+          rsum += acc[i];
+          rcnt++;
+        }
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    rsum += __shfl_down(rsum, off, 64);
+    rcnt += __shfl_down(rcnt, off, 64);
+  }
+  if (lane == 0) {
+    red_s[wave] = rsum;
+    red_c[wave] = rcnt;
+  }
+  __syncthreads();
+  float tot = 0.f;
+  int itot = 0;
+#pragma unroll
+  for (int k = 0; k < kFT / 64; k++) {
+    tot += red_s[k];
+    itot += red_c[k];
+  }
+  const float mean = itot ? tot / (float)itot : tot;
+
+  // ---- rows: x = T (wY)^T, mask, Z^T = T^T x'
+  for (int rb = wave; rb < mt; rb += kFT / 64) {
+    const int m0 = rb * 16;
+    int mlo, mhi;
+    tile_r2(m0, nsamh, nsam, mlo, mhi);
+    float yv[kFK / 4];
+#pragma unroll
+    for (int ks = 0; ks < kFK / 4; ks++) yv[ks] = Ys[(m0 + la) * kFP + 4 * ks + lg];
+    const int m = m0 + la;
+    const int dy = m < nsamh ? m : m - nsam;
+    fx4 z[4];
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++) z[kt] = fx4{0.f, 0.f, 0.f, 0.f};
+    for (int nt = 0; nt < mt; nt++) {
+      const int n0 = nt * 16;
+      int nlo, nhi;
+      tile_r2(n0, nsamh, nsam, nlo, nhi);
+      fx4 x;
+      if ((float)(nlo + mlo) > ri3p) {
+        x = fx4{mean, mean, mean, mean};  // wholly outside the mask
+      } else {
+        x = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < kFK / 4; ks++)
+          x = mfma4(Ts[(n0 + la) * kFP + 4 * ks + lg], yv[ks], x);
+        if ((float)(nhi + mhi) > ri2p) {
+#pragma unroll
+          for (int i = 0; i < 4; i++) {
+            int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
+            float crad2 = (float)(dx * dx + dy * dy);
+            if (crad2 > ri2p && crad2 <= ri3p) {
+              float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+              x[i] = x[i] * edge + (1.0f - edge) * mean;
+            } else if (crad2 > ri3p) {
+              x[i] = mean;
+            }
+          }
+        }
+      }
+      // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++)
+          z[kt] = mfma4(Ts[(n0 + 4 * lg + i) * kFP + 16 * kt + la], x[i], z[kt]);
+    }
+    // Z[m][k]: this wave alone read rows m0..m0+15 of Ys
+#pragma unroll
+    for (int kt = 0; kt < 4; kt++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) Ys[(m0 + la) * kFP + 16 * kt + 4 * lg + i] = z[kt][i];
+  }
+  __syncthreads();
+
+  // ---- column r2c: out[r][c] = sum_m T[m][r] Z[m][c]  (into Bs)
+  for (int t = wave; t < 16; t += kFT / 64) {
+    const int r0 = (t >> 2) * 16, c0 = (t & 3) * 16;
+    fx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    for (int m0 = 0; m0 < nsam; m0 += 16) {
+#pragma unroll
+      for (int i = 0; i < 4; i += 2) {
+        const int ma = m0 + 4 * lg + i, mb = ma + 1;
+        acc0 = mfma4(Ts[ma * kFP + r0 + la], Ys[ma * kFP + c0 + la], acc0);
+        acc1 = mfma4(Ts[mb * kFP + r0 + la], Ys[mb * kFP + c0 + la], acc1);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) Bs[(r0 + 4 * lg + i) * kFP + c0 + la] = acc0[i] + acc1[i];
+  }
+  __syncthreads();
+
+  // ---- cc3m.f:117-150: shifted particle x conj(masked projection), shells
+  {
+    const float2* sp = spec + (size_t)p * ic;
+    const float shx = shift[2 * it], shy = shift[2 * it + 1];
+    for (int e = tid; e < 32 * (2 * kFJ + 1); e += kFT) {
+      int i = e / (2 * kFJ + 1), j = e % (2 * kFJ + 1) - kFJ;
+      int itemp = i * i + j * j;
+      if (itemp < maxr12 || itemp >= maxr22) continue;
+      int jj = j < 0 ? -j : j;
+      float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
+      float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
+      float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
+      float sn, cs;
+      sincosf(shx * (float)i + shy * (float)j, &sn, &cs);
+      float2 s = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
+      cf samp = cmul(cf{s.x, s.y}, cf{cs, sn});
+      float q = samp.re * a.x + samp.im * a.y;
+      int ibin = (int)(sqrtf((float)itemp) + 0.5f);
+      atomicAdd(&bins[4 * ibin + 0], q);
+      atomicAdd(&bins[4 * ibin + 1], samp.re * samp.re + samp.im * samp.im);
+      atomicAdd(&bins[4 * ibin + 2], a.x * a.x + a.y * a.y);
+      atomicAdd(&bins[4 * ibin + 3], 1.0f);
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int i = 2; i <= nsam / 2; i++) {
+      const float* b = bins + 4 * (i - 1);
+      if (b[3] != 0.0f) {
+        float r = b[0] / sqrtf(b[2]);
+        if (fsct[i - 1] < 0.95f) r = fabsf(r);
+        float wgt = expf(-rbfact * (float)((i - 1) * (i - 1)));
+        s3 += (double)(r * wgt);
+        s2 += (double)(wgt * wgt * b[1]);
+        s1 += 1.0;
+      }
+    }
+    if (s1 * s2 != 0.0) s3 = s3 / sqrt(s1 * s2);
+    score[it] = (float)s3;
+  }
+}
+
+}  // namespace
+
+bool cc3m_fused_ok(int nsam, int maxr2) {
+  return nsam % 32 == 0 && nsam >= 64 && nsam <= 256 && maxr2 <= 32;
+}
+
+size_t cc3m_fused_lds(int nsam) {
+  return sizeof(float) * ((size_t)2 * nsam * kFP + 64 * kFP + 4 * (nsam / 2 + 2));
+}
+
+// T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N) for i < 32, from
+// the exact residue (i n) mod N in double precision.
+void cc3m_fused_table(int nsam, std::vector<float>& tab) {
+  tab.assign((size_t)nsam * kFK, 0.f);
+  for (int n = 0; n < nsam; n++)
+    for (int i = 0; i < kFK / 2; i++) {
+      double ph = 2.0 * 3.14159265358979323846 * (double)((i * n) % nsam) / (double)nsam;
+      tab[(size_t)n * kFK + 2 * i] = (float)std::cos(ph);
+      tab[(size_t)n * kFK + 2 * i + 1] = (float)-std::sin(ph);
+    }
+}
+
+int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
+                      const float* ctff, const int* pidx, const float* dm, const float* shift,
+                      const float* amag, const float* fsct, const float* tab, float* score) {
+  if (a.n <= 0) return 0;
+  FR_CHECK(cc3m_fused_ok(a.nsam, a.maxr2), "fused CC3M needs NSAM %% 32 == 0, NSAM <= 256, MAXR2 <= 32");
+  const size_t lds = cc3m_fused_lds(a.nsam);
+  FR_HIP(hipFuncSetAttribute((const void*)k_cc3m_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)lds));
+  hipLaunchKernelGGL(k_cc3m_fused, dim3((unsigned)a.n), dim3(kFT), lds, c.stream, a.nsam,
+                     a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
+                     reinterpret_cast<const float2*>(a3df), dm, pidx,
+                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),
+                     shift, amag, fsct, tab, a.ri2, a.ri3, a.rih, a.halfw, score);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
 
 int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec, const float* ctff,
                 const int* pidx, const float* dm, const float* shift, const float* amag,
@@ -167,6 +537,31 @@ int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
   hipLaunchKernelGGL(k_cc3m_score, dim3(a.n), dim3(kThreads), 0, c.stream, nsam,
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
                      reinterpret_cast<const float2*>(spec), pidx, shift, fsct, work, score);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// PRESB (presb.f:8-160) for a batch: the CC3M pipeline over 1 <= r^2 < (NSAM/2)^2
+int launch_presb(Ctx& c, int nsam, int n, const float* a3df, const float* spec, const float* ctff,
+                 const int* pidx, const float* dm, const float* shift, const float* amag,
+                 float ri2, float ri3, float rih, float halfw, float* work, float* rbin) {
+  if (n <= 0) return 0;
+  FR_CHECK(nsam / 2 + 2 <= 260, "box too large for PRESB bins");
+  const int nsamh = nsam / 2;
+  const size_t ic = (size_t)nsam * (nsam / 2 + 1);
+  const size_t img = (size_t)nsam * (nsam + 2);
+  size_t tot = ic * n;
+  hipLaunchKernelGGL(k_cc3m_fill, dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads),
+                     0, c.stream, nsam, n, 1, nsamh * nsamh, reinterpret_cast<const float2*>(a3df),
+                     dm, pidx, reinterpret_cast<const float2*>(ctff), work);
+  FR_HIP(hipGetLastError());
+  if (int r = fft_c2r_2d(c, nsam, n, work, img)) return r;
+  hipLaunchKernelGGL(k_scale_mask, dim3(n), dim3(kThreads), 0, c.stream, nsam, work, img, 1.0f,
+                     amag, 1, ri2, ri3, rih, halfw, 1);
+  FR_HIP(hipGetLastError());
+  if (int r = fft_r2c_2d(c, nsam, n, work, img)) return r;
+  hipLaunchKernelGGL(k_presb_bins, dim3(n), dim3(kThreads), 0, c.stream, nsam,
+                     reinterpret_cast<const float2*>(spec), pidx, shift, work, rbin);
   FR_HIP(hipGetLastError());
   return 0;
 }

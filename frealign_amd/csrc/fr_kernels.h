@@ -1,5 +1,6 @@
 // fr_kernels.h -- launchers shared between the kernel files and the C-ABI glue.
 #pragma once
+#include <vector>
 #include "fr_common.h"
 
 namespace fr {
@@ -51,5 +52,15 @@ __global__ void k_scale_mask(int nsam, float* data, size_t img_stride, float sca
 int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec, const float* ctff,
                 const int* pidx, const float* dm, const float* shift, const float* amag,
                 const float* fsct, float* work, float* score);
+int launch_presb(Ctx& c, int nsam, int n, const float* a3df, const float* spec, const float* ctff,
+                 const int* pidx, const float* dm, const float* shift, const float* amag,
+                 float ri2, float ri3, float rih, float halfw, float* work, float* rbin);
+// CC3M with band-pruned DFT products on the matrix cores; one workgroup per item.
+bool cc3m_fused_ok(int nsam, int maxr2);
+size_t cc3m_fused_lds(int nsam);
+void cc3m_fused_table(int nsam, std::vector<float>& tab);
+int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
+                      const float* ctff, const int* pidx, const float* dm, const float* shift,
+                      const float* amag, const float* fsct, const float* tab, float* score);
 
 }  // namespace fr

@@ -106,6 +106,15 @@ int fr_cc3m_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float
                   const float *d_shift, const float *d_amag, int maxr1, int maxr2, float rbfact,
                   const float *d_fsct, const float *ring, float *d_score);
 
+/* Replaces PRESB (presb.f:8-160; IEWALD=0, non-helical): per-shell
+ * correlations of n (particle, section, shift) items over 1 <= r^2 < (NSAM/2)^2.
+ * Arguments as fr_cc3m_batch.  d_rbin is n x NSAM floats, in/out: for every
+ * shell i = 1..NSAM/2 that holds a point, RBIN(2i-1) = sum Re(SAMP conj P) /
+ * sqrt(sum|SAMP|^2 sum|P|^2) and RBIN(2i) = 1; other entries are untouched. */
+int fr_presb_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
+                   const float *d_ctff, const int *d_pidx, const float *d_dm,
+                   const float *d_shift, const float *d_amag, const float *ring, float *d_rbin);
+
 /* -------------------------------------------------------------- insertion
  * Replaces the A3D3 flush (a3d3.f:107-143) -> A3D3_S -> PINSERT ->
  * PINSERT_TRILIN (pinsert.f:8-195, pinsert_trilin.f:8-244; INTERP=1,

@@ -161,8 +161,11 @@ def test_pextract_vs_oracle(ctx, nsam):
 
 
 # ----------------------------------------------------------------- CC3M
-@pytest.mark.parametrize("nsam", [64])
-def test_cc3m_vs_oracle(ctx, nsam):
+# (64, 24), (128, 32), (256, 32): band-pruned DFT path (MAXR2 <= 32, NSAM % 32 == 0);
+# (96, 40): the full-FFT path (MAXR2 > 32, NSAM = 2^5 x 3)
+@pytest.mark.parametrize("nsam,maxr2,amag_jitter", [(64, 24, 0.0), (128, 32, 0.02), (256, 32, 0.02),
+                                                    (96, 40, 0.02)])
+def test_cc3m_vs_oracle(ctx, nsam, maxr2, amag_jitter):
     jc = nsam // 2 + 1
     L = O.oracle()
     vol = half_volume(nsam, 43)
@@ -173,9 +176,9 @@ def test_cc3m_vs_oracle(ctx, nsam):
     pidx = rng.integers(0, npart, nitem).astype(np.int32)
     eul = rng.uniform(-np.pi, np.pi, (nitem, 3)).astype(np.float32)
     shift = rng.uniform(-0.2, 0.2, (nitem, 2)).astype(np.float32)
-    amag = np.ones(nitem, np.float32)
-    dm = api.rotation(eul, amag=1.0)
-    maxr1, maxr2, rbfact = 2, 24, 0.001
+    amag = (1.0 + amag_jitter * rng.uniform(-1, 1, nitem)).astype(np.float32)
+    dm = np.concatenate([api.rotation(eul[k:k + 1], amag=float(amag[k])) for k in range(nitem)])
+    maxr1, rbfact = 2, 0.001
     fsct = np.zeros(nsam // 2 + 1, np.float32)
     fsct[: nsam // 4] = 1.0
     ri = 0.4 * nsam
@@ -186,15 +189,57 @@ def test_cc3m_vs_oracle(ctx, nsam):
     for k in range(nitem):
         p = pidx[k]
         args = [float(x) for x in eul[k]] + [float(shift[k, 0]), float(shift[k, 1]), rbfact]
-        s32 = L.or32_cc3m(nsam, 1.0, spec[p].reshape(-1), vol.reshape(-1), maxr1, maxr2, *args,
+        s32 = L.or32_cc3m(nsam, float(amag[k]), spec[p].reshape(-1), vol.reshape(-1), maxr1, maxr2, *args,
                           fsct, ctff[p].reshape(-1), *[float(x) for x in ring], O.IDENTITY)
-        s64 = L.or64_cc3m(nsam, 1.0, spec[p].reshape(-1).astype(np.complex128), vol.reshape(-1),
+        s64 = L.or64_cc3m(nsam, float(amag[k]), spec[p].reshape(-1).astype(np.complex128), vol.reshape(-1),
                           maxr1, maxr2, *args, fsct, ctff[p].reshape(-1).astype(np.complex128),
                           *[float(x) for x in ring], O.IDENTITY)
         r32 = abs(got[k] - s32) / abs(s32)
         print(f"item {k}: gpu {got[k]:.7f} or32 {s32:.7f} or64 {s64:.7f} rel32 {r32:.1e} "
               f"rel64 {abs(got[k] - s64) / abs(s64):.1e}")
         assert r32 < 1e-4
+
+
+# ----------------------------------------------------------------- PRESB
+@pytest.mark.parametrize("nsam", [64, 96])
+def test_presb_vs_oracle(ctx, nsam):
+    jc = nsam // 2 + 1
+    L = O.oracle()
+    vol = half_volume(nsam, 47)
+    npart, nitem = 2, 6
+    spec = (rng.standard_normal((npart, nsam, jc)) + 1j * rng.standard_normal((npart, nsam, jc))
+            ).astype(np.complex64)
+    ctff = (0.5 + 0.1 * rng.standard_normal((npart, 2, nsam, jc))).astype(np.complex64)
+    pidx = rng.integers(0, npart, nitem).astype(np.int32)
+    eul = rng.uniform(-np.pi, np.pi, (nitem, 3)).astype(np.float32)
+    shift = rng.uniform(-0.2, 0.2, (nitem, 2)).astype(np.float32)
+    amag = (1.0 + 0.02 * rng.uniform(-1, 1, nitem)).astype(np.float32)
+    dm = np.concatenate([api.rotation(eul[k:k + 1], amag=float(amag[k])) for k in range(nitem)])
+    ri = 0.4 * nsam
+    ring = np.array([(ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0], np.float32)
+    # sentinel: entries PRESB does not write must come back unchanged
+    rbin0 = np.full((nitem, nsam), 7.0, np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
+    got = ctx.presb(nsam, t(vol), t(spec), t(ctff), t(pidx), t(dm), t(shift), t(amag), ring,
+                    rbin=t(rbin0)).cpu().numpy()
+    for k in range(nitem):
+        p = pidx[k]
+        args = [float(x) for x in eul[k]] + [float(shift[k, 0]), float(shift[k, 1])]
+        w32 = rbin0[k].copy()
+        L.or32_presb(nsam, float(amag[k]), spec[p].reshape(-1), vol.reshape(-1), *args, w32,
+                     ctff[p].reshape(-1), *[float(x) for x in ring], O.IDENTITY)
+        w64 = rbin0[k].astype(np.float64)
+        L.or64_presb(nsam, float(amag[k]), spec[p].reshape(-1).astype(np.complex128),
+                     vol.reshape(-1), *args, w64, ctff[p].reshape(-1).astype(np.complex128),
+                     *[float(x) for x in ring], O.IDENTITY)
+        # the written set (count slots = 1) and the untouched sentinels are exact
+        assert np.array_equal(got[k][1::2], w32[1::2])
+        assert np.array_equal(got[k][0::2] == 7.0, w32[0::2] == 7.0)
+        e32 = np.max(np.abs(got[k][0::2] - w32[0::2]))
+        e64 = np.max(np.abs(got[k][0::2] - w64[0::2]))
+        print(f"nsam={nsam} item {k}: max |dRBIN| vs or32 {e32:.1e} vs or64 {e64:.1e}")
+        # shell correlations lie in [-1, 1]; FP32 shell sums over <= 2*pi*r points
+        assert e32 < 2e-5
 
 
 # ------------------------------------------------------------ insertion
