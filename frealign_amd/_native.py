@@ -35,6 +35,18 @@ SIGNATURES = {
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
+    "fr_search_list_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, ci, vp, ci, ci, vp, vp,
+                              vp], ci),
+    "fr_cnn_param_count": ([ci, ci, ci, ci, C.POINTER(C.c_longlong)], ci),
+    "fr_cnn_init": ([vp, ci, ci, ci, ci, C.c_ulonglong, vp, vp], ci),
+    "fr_cnn_forward": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),
+    "fr_cnn_gradients": ([vp, ci, ci, ci, ci, vp, ci, vp, vp, vp, vp], ci),
+    "fr_cnn_adam": ([vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, cf], ci),
+    "fr_cnn_propose": ([vp, ci, ci, vp, ci, vp, vp], ci),
+    "fr_cnn_images": ([vp, ci, ci, ci, vp, vp, vp], ci),
+    "fr_cnn_synth": ([vp, ci, ci, ci, vp, vp, vp, vp, cf, C.c_ulonglong, vp], ci),
+    "fr_orient_nearest": ([vp, ci, vp, ci, vp, ci, vp, vp], ci),
+    "fr_orient_within": ([vp, ci, vp, ci, vp, ci, vp, cf, vp], ci),
     "fr_insert_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp, vp,
                          vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),

@@ -272,6 +272,141 @@ class Context:
                                         _p(top_s)), "fr_search_batch")
         return top_o, top_r, top_s
 
+    def search_list(self, nsam: int, maxr1: int, maxr2: int, rbfact: float, bank: torch.Tensor,
+                    outc, ctff, cand: torch.Tensor, group: int, topk: int = 32):
+        """fr_search_list_batch: particle p against bank rows cand[p] (-1 = empty);
+        returns (bank orientation, shift row, score) of the topk best groups."""
+        _dev(outc, torch.complex64, "outc")
+        _dev(ctff, torch.complex64, "ctff")
+        _dev(cand, torch.int32, "cand")
+        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
+            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        jc = nsam // 2 + 1
+        np_ = outc.numel() // (nsam * jc)
+        if ctff.numel() != 2 * outc.numel():
+            raise ValueError("ctff must hold two half-planes per particle")
+        if cand.dim() != 2 or cand.shape[0] != np_:
+            raise ValueError("cand must be (particles, ncand)")
+        _, kpad = self.search_layout(nsam, maxr1, maxr2)
+        if bank.dim() != 2 or bank.shape[1] != kpad:
+            raise ValueError(f"bank rows must be {kpad} long")
+        if np_ and int(cand.max()) >= bank.shape[0]:
+            raise ValueError("candidate index beyond the bank")
+        dev = outc.device
+        top_o = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_r = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_s = torch.empty((np_, topk), dtype=torch.float32, device=dev)
+        self._sync_stream()
+        N.check(N.lib().fr_search_list_batch(
+            self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0], _p(bank),
+            1 if bank.dtype == torch.bfloat16 else 0, np_, _p(outc), _p(ctff), cand.shape[1],
+            _p(cand), int(group), topk, _p(top_o), _p(top_r), _p(top_s)), "fr_search_list_batch")
+        return top_o, top_r, top_s
+
+    # ------------------------------------------------ proposal network
+    def cnn_init(self, shape, seed: int, master: torch.Tensor, wq: torch.Tensor):
+        _dev(master, torch.float32, "master")
+        _dev(wq, torch.float32, "wq")
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_init(self.h, *shape, C.c_ulonglong(seed), _p(master), _p(wq)),
+                "fr_cnn_init")
+
+    def cnn_forward(self, shape, wq: torch.Tensor, img: torch.Tensor) -> torch.Tensor:
+        _dev(wq, torch.float32, "wq")
+        _dev(img, torch.float32, "img")
+        S, K = shape[0], shape[3]
+        nb = img.numel() // (S * S)
+        if img.numel() != nb * S * S:
+            raise ValueError("img must be nb x S x S")
+        out = torch.empty((nb, K), dtype=torch.float32, device=img.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_forward(self.h, *shape, _p(wq), nb, _p(img), _p(out)),
+                "fr_cnn_forward")
+        return out
+
+    def cnn_gradients(self, shape, wq: torch.Tensor, img: torch.Tensor, label: torch.Tensor,
+                      grad: torch.Tensor) -> torch.Tensor:
+        _dev(wq, torch.float32, "wq")
+        _dev(img, torch.float32, "img")
+        _dev(label, torch.int32, "label")
+        _dev(grad, torch.float32, "grad")
+        S, K = shape[0], shape[3]
+        nb = img.numel() // (S * S)
+        if label.numel() != nb or (nb and (int(label.min()) < 0 or int(label.max()) >= K)):
+            raise ValueError("one label in 0..K-1 per image")
+        loss = torch.empty(nb, dtype=torch.float32, device=img.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_gradients(self.h, *shape, _p(wq), nb, _p(img), _p(label),
+                                         _p(loss), _p(grad)), "fr_cnn_gradients")
+        return loss
+
+    def cnn_adam(self, shape, master, wq, grad, m, v, step: int, lr: float):
+        for t, nm in ((master, "master"), (wq, "wq"), (grad, "grad"), (m, "m"), (v, "v")):
+            _dev(t, torch.float32, nm)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_adam(self.h, *shape, _p(master), _p(wq), _p(grad), _p(m), _p(v),
+                                    int(step), float(lr)), "fr_cnn_adam")
+
+    def cnn_propose(self, logits: torch.Tensor, topk: int):
+        _dev(logits, torch.float32, "logits")
+        nb, K = logits.shape
+        top = torch.empty((nb, topk), dtype=torch.int32, device=logits.device)
+        prob = torch.empty((nb, topk), dtype=torch.float32, device=logits.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_propose(self.h, K, nb, _p(logits), topk, _p(top), _p(prob)),
+                "fr_cnn_propose")
+        return top, prob
+
+    def cnn_images(self, nsam: int, S: int, datc, ctff) -> torch.Tensor:
+        _dev(datc, torch.complex64, "datc")
+        _dev(ctff, torch.complex64, "ctff")
+        jc = nsam // 2 + 1
+        nb = datc.numel() // (nsam * jc)
+        if ctff.numel() != 2 * datc.numel():
+            raise ValueError("ctff must hold two half-planes per particle")
+        img = torch.empty((nb, S, S), dtype=torch.float32, device=datc.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_images(self.h, nsam, S, nb, _p(datc), _p(ctff), _p(img)),
+                "fr_cnn_images")
+        return img
+
+    def cnn_synth(self, nsam: int, S: int, a3df, dm, spar, gpar, disc: float, seed: int):
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(dm, torch.float32, "dm")
+        _dev(spar, torch.float32, "spar")
+        nb = dm.numel() // 9
+        if spar.numel() != 8 * nb or a3df.numel() != nsam * nsam * (nsam // 2 + 1):
+            raise ValueError("spar must be nb x 8 and a3df the half-volume")
+        g = np.ascontiguousarray(gpar, dtype=np.float32)
+        if g.size != 11:
+            raise ValueError("gpar has 11 entries")
+        img = torch.empty((nb, S, S), dtype=torch.float32, device=dm.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_synth(self.h, nsam, S, nb, _p(a3df), _p(dm), _p(spar), _p(g),
+                                     float(disc), C.c_ulonglong(seed), _p(img)), "fr_cnn_synth")
+        return img
+
+    def orient_nearest(self, dm, dmc, sym) -> torch.Tensor:
+        for t, nm in ((dm, "dm"), (dmc, "dmc"), (sym, "sym")):
+            _dev(t, torch.float32, nm)
+        n = dm.numel() // 9
+        lab = torch.empty(n, dtype=torch.int32, device=dm.device)
+        self._sync_stream()
+        N.check(N.lib().fr_orient_nearest(self.h, n, _p(dm), dmc.numel() // 9, _p(dmc),
+                                          sym.numel() // 9, _p(sym), _p(lab)), "fr_orient_nearest")
+        return lab
+
+    def orient_within(self, dma, dmb, sym, max_deg: float) -> torch.Tensor:
+        for t, nm in ((dma, "dma"), (dmb, "dmb"), (sym, "sym")):
+            _dev(t, torch.float32, nm)
+        na, nb = dma.numel() // 9, dmb.numel() // 9
+        w = (na + 31) // 32
+        mask = torch.zeros((nb, w), dtype=torch.int32, device=dma.device)
+        self._sync_stream()
+        N.check(N.lib().fr_orient_within(self.h, na, _p(dma), nb, _p(dmb), sym.numel() // 9,
+                                         _p(sym), float(max_deg), _p(mask)), "fr_orient_within")
+        return mask
+
     # ----------------------------------------------------------- statistics
     def sheltest(self, nsam: int, a, d, s, v, istp: int = 1, fmask: float = 1.0):
         jc = nsam // 2 + 1
@@ -318,6 +453,12 @@ class HalfMaps:
 
 
 # ---------------------------------------------------------------- geometry
+def cnn_param_count(S: int, C_: int, nconv: int, K: int) -> int:
+    n = C.c_longlong(0)
+    N.check(N.lib().fr_cnn_param_count(S, C_, nconv, K, C.byref(n)), "fr_cnn_param_count")
+    return n.value
+
+
 def search_grid(dang: float, asym: str = "C", nasym: int = 1, pmask=(1, 1, 1, 1, 1),
                 rmm: float = 0.1, ri: float = 50.0, asymstore: bool = True):
     """SEARCHANG (searchang.f): returns (sang (n,3) radians float32, iquadmax)."""

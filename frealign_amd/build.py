@@ -24,6 +24,7 @@ SOURCES = [
     "fr_insert.hip",
     "fr_stats.hip",
     "fr_search.hip",
+    "fr_cnn.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

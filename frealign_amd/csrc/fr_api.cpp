@@ -16,12 +16,13 @@
 #include "fr_common.h"
 #include "fr_host.h"
 #include "fr_kernels.h"
+#include "fr_cnn.h"
 #include "fr_search.h"
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  void* slots[8] = {};
-  size_t slot_bytes[8] = {};
+  void* slots[16] = {};
+  size_t slot_bytes[16] = {};
   void* slot(int k, size_t bytes, int* err);
   // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
   int skey[3] = {0, 0, 0};
@@ -188,7 +189,7 @@ int fr_destroy(fr_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 8; k++)
+  for (int k = 0; k < 16; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
   if (c->d_kidx) (void)hipFree(c->d_kidx);
   if (c->d_dft) (void)hipFree(c->d_dft);
@@ -584,11 +585,162 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
     return r;
   if (int r = launch_search_norm(*c, norient, c->kpad, np, d_bank, dtype == 1, c2, nrm)) return r;
   int pslot = c->prof_begin(kProfSearchGemm);
-  if (int r = launch_search_gemm(*c, norient, c->kpad, np, d_bank, dtype == 1, ap, nrm, sum2,
-                                 c->d_shx, c->d_shy, score, brow, nullptr))
+  if (int r = launch_search_gemm(*c, norient, norient, c->kpad, np, d_bank, dtype == 1, nullptr,
+                                 ap, nrm, sum2, c->d_shx, c->d_shy, score, brow, nullptr))
     return r;
   c->prof_end(pslot);
   return launch_topk(*c, np, norient, group, topk, score, brow, d_top_o, d_top_row, d_top_score);
+}
+
+// ------------------------------------------------ learned orientation proposal
+#define CNN_SHAPE(lo)                                    \
+  fr::CnnLayout lo;                                      \
+  if (int r_ = fr::cnn_layout(S, C, nconv, K, &lo)) return r_
+
+int fr_cnn_param_count(int S, int C, int nconv, int K, long long* nparam) {
+  FR_CHECK(nparam, "null argument");
+  CNN_SHAPE(lo);
+  *nparam = lo.total;
+  return 0;
+}
+
+int fr_cnn_init(fr_ctx* c, int S, int C, int nconv, int K, unsigned long long seed,
+                float* d_master, float* d_wq) {
+  CTX_CHECK(c);
+  FR_CHECK(d_master && d_wq, "null argument");
+  CNN_SHAPE(lo);
+  return cnn_init(*c, lo, seed, d_master, d_wq);
+}
+
+int fr_cnn_forward(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
+                   const float* d_img, float* d_logits) {
+  CTX_CHECK(c);
+  FR_CHECK(d_wq && d_img && d_logits && nb >= 0, "null argument");
+  CNN_SHAPE(lo);
+  if (nb == 0) return 0;
+  int err = 0;
+  float* ws = (float*)c->slot(8, sizeof(float) * cnn_workspace(lo, nb, false), &err);
+  if (!ws) return err;
+  return cnn_forward(*c, lo, d_wq, nb, d_img, ws, d_logits);
+}
+
+int fr_cnn_gradients(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
+                     const float* d_img, const int* d_label, float* d_loss, float* d_grad) {
+  CTX_CHECK(c);
+  FR_CHECK(d_wq && d_img && d_label && d_loss && d_grad && nb >= 1, "null argument");
+  CNN_SHAPE(lo);
+  int err = 0;
+  float* ws = (float*)c->slot(9, sizeof(float) * cnn_workspace(lo, nb, true), &err);
+  if (!ws) return err;
+  return cnn_backward(*c, lo, d_wq, nb, d_img, d_label, ws, d_loss, d_grad);
+}
+
+int fr_cnn_adam(fr_ctx* c, int S, int C, int nconv, int K, float* d_master, float* d_wq,
+                const float* d_grad, float* d_m, float* d_v, int step, float lr) {
+  CTX_CHECK(c);
+  FR_CHECK(d_master && d_wq && d_grad && d_m && d_v, "null argument");
+  CNN_SHAPE(lo);
+  return cnn_adam(*c, lo, d_master, d_wq, d_grad, d_m, d_v, step, lr);
+}
+
+int fr_cnn_propose(fr_ctx* c, int K, int nb, const float* d_logits, int topk, int* d_top,
+                   float* d_prob) {
+  CTX_CHECK(c);
+  FR_CHECK(d_logits && d_top && d_prob && nb >= 0, "null argument");
+  if (nb == 0) return 0;
+  return cnn_propose(*c, K, nb, d_logits, topk, d_top, d_prob);
+}
+
+int fr_cnn_images(fr_ctx* c, int nsam, int S, int nb, const float* d_datc, const float* d_ctff,
+                  float* d_img) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_datc && d_ctff && d_img && nb >= 0, "null argument");
+  FR_CHECK(S >= 4 && S <= nsam && S % 2 == 0 && fft_supported(S), "bad network input size %d", S);
+  if (nb == 0) return 0;
+  int err = 0;
+  float* buf = (float*)c->slot(10, sizeof(float) * (size_t)nb * S * (S + 2), &err);
+  if (!buf) return err;
+  return cnn_images(*c, nsam, S, nb, d_datc, d_ctff, buf, d_img);
+}
+
+int fr_cnn_synth(fr_ctx* c, int nsam, int S, int nb, const float* d_a3df, const float* d_dm,
+                 const float* d_spar, const float* gpar, float disc, unsigned long long seed,
+                 float* d_img) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_dm && d_spar && gpar && d_img && nb >= 0, "null argument");
+  FR_CHECK(S >= 4 && S <= nsam && S % 2 == 0 && fft_supported(S), "bad network input size %d", S);
+  FR_CHECK(disc > 0.0f, "disc must be positive");
+  if (nb == 0) return 0;
+  PrepGlobals g{gpar[0], gpar[1], gpar[2], gpar[3], gpar[4], gpar[5], gpar[6],
+                gpar[7], gpar[8], gpar[9], gpar[10], 1};
+  int err = 0;
+  float* buf = (float*)c->slot(10, sizeof(float) * (size_t)nb * S * (S + 2), &err);
+  if (!buf) return err;
+  return cnn_synth(*c, nsam, S, nb, d_a3df, d_dm, d_spar, g, disc, seed, buf, d_img);
+}
+
+int fr_orient_nearest(fr_ctx* c, int n, const float* d_dm, int ncls, const float* d_dmc,
+                      int nsym, const float* d_sym, int* d_label) {
+  CTX_CHECK(c);
+  FR_CHECK(d_dm && d_dmc && d_sym && d_label && n >= 0 && ncls >= 1, "null argument");
+  if (n == 0) return 0;
+  return orient_nearest(*c, n, d_dm, ncls, d_dmc, nsym, d_sym, d_label);
+}
+
+int fr_orient_within(fr_ctx* c, int na, const float* d_dma, int nb, const float* d_dmb,
+                     int nsym, const float* d_sym, float max_deg, unsigned* d_mask) {
+  CTX_CHECK(c);
+  FR_CHECK(d_dma && d_dmb && d_sym && d_mask && na >= 0 && nb >= 0, "null argument");
+  FR_CHECK(max_deg >= 0.0f && max_deg <= 180.0f, "max_deg must be in [0, 180]");
+  if (na == 0 || nb == 0) return 0;
+  const float tmin = (float)(1.0 + 2.0 * cos((double)max_deg * 3.14159265358979323846 / 180.0));
+  return orient_within(*c, na, d_dma, nb, d_dmb, nsym, d_sym, tmin, d_mask);
+}
+
+int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int norient,
+                         const void* d_bank, int dtype, int np, const float* d_outc,
+                         const float* d_ctff, int ncand, const int* d_cand, int group, int topk,
+                         int* d_top_o, int* d_top_row, float* d_top_score) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_bank && d_outc && d_ctff && d_cand && d_top_o && d_top_row && d_top_score,
+           "null argument");
+  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(np >= 0 && norient > 0 && ncand > 0 && topk >= 1 && topk <= 32, "bad sizes");
+  FR_CHECK(group >= 1 && ncand % group == 0, "candidate lists must come in whole groups");
+  if (np == 0) return 0;
+  if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  const int half = c->kpad / 2;
+  int err = 0;
+  float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
+  if (!ap) return err;
+  float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
+  if (!c2) return err;
+  float* sum2 = (float*)c->slot(5, sizeof(float) * (size_t)np, &err);
+  if (!sum2) return err;
+  float* score = (float*)c->slot(6, sizeof(float) * (size_t)np * ncand, &err);
+  if (!score) return err;
+  int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * ncand, &err);
+  if (!brow) return err;
+  float* nrm = (float*)c->slot(2, sizeof(float) * (size_t)np * ncand, &err);
+  if (!nrm) return err;
+  if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
+                                  d_ctff, c->d_kidx, ap, c2, sum2))
+    return r;
+  if (int r = launch_search_norm_list(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand,
+                                      c2, nrm))
+    return r;
+  int pslot = c->prof_begin(kProfSearchGemm);
+  if (int r = launch_search_gemm(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand, ap,
+                                 nrm, sum2, c->d_shx, c->d_shy, score, brow, nullptr))
+    return r;
+  c->prof_end(pslot);
+  if (int r = launch_topk(*c, np, ncand, group, topk, score, brow, d_top_o, d_top_row,
+                          d_top_score))
+    return r;
+  return launch_map_cand(*c, np, ncand, topk, d_cand, d_top_o);
 }
 
 }  // extern "C"

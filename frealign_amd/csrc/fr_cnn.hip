@@ -1,0 +1,869 @@
+// fr_cnn.hip -- the learned orientation proposal, trained and run on the GPU
+// with this file's kernels only.
+//
+// Network: nconv 3x3 / stride 2 / pad 1 convolutions with ReLU, then one
+// fully connected layer to K orientation-class logits.  Activations are
+// channel-major, [C][B][H][W], so every convolution is one GEMM over the
+// batch: out[C_out][B*P] = W[C_out][C_in*9] x cols[C_in*9][B*P] (im2col).
+// The GEMM runs on the matrix cores (v_mfma_f32_16x16x4f32).  Weights are
+// held as BF16-rounded values (wq) with an FP32 master copy for Adam.
+//
+// Data: the network input is the particle spectrum (DATC) phase-flipped by
+// the sign of its CTF, cropped to S x S in Fourier space and brought back to
+// real space (S-point c2r), then normalised.  Training images are made the
+// same way from central sections of the current reference (AINTERPO3DS),
+// times the CTF of ctf.f, a shift ramp and white noise at a given SNR.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "fr_cnn.h"
+#include "fr_ctf.h"
+#include "fr_extract.h"
+
+namespace fr {
+
+namespace {
+
+constexpr int kT = 256;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float fx4;
+
+// ------------------------------------------------------------------ GEMM
+// C[m][n] (+)= sum_k A[m][k] B[k][n] over k in this split's range, with
+// arbitrary element strides (so transposed operands cost nothing), an
+// optional per-row bias and ReLU.  64x64 tiles, K staged 16 at a time.
+struct GemmArgs {
+  int M, N, K;
+  const float* A;
+  long long sam, sak;
+  const float* B;
+  long long sbk, sbn;
+  float* C;
+  long long scm, scn;
+  int kchunk;        // K range per blockIdx.z (multiple of 16)
+  long long csplit;  // C offset between splits
+  const float* bias;
+  int relu;
+  int accumulate;
+};
+
+constexpr int kGP = 80;  // LDS pitch: 16*g + lane -> distinct banks
+
+__global__ void __launch_bounds__(kT) k_gemm(GemmArgs g) {
+  __shared__ float As[16][kGP];
+  __shared__ float Bs[16][kGP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const bool a_kfast = g.sak == 1;
+  const bool b_nfast = g.sbn == 1;
+  fx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int e = tid + kT * q;
+      int mm, ka;
+      if (a_kfast) {
+        ka = e & 15;
+        mm = e >> 4;
+      } else {
+        mm = e & 63;
+        ka = e >> 6;
+      }
+      const int m = m0 + mm, k = k0 + ka;
+      As[ka][mm] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
+      int nn, kb;
+      if (b_nfast) {
+        nn = e & 63;
+        kb = e >> 6;
+      } else {
+        kb = e & 15;
+        nn = e >> 4;
+      }
+      const int n = n0 + nn, k2 = k0 + kb;
+      Bs[kb][nn] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      float a[2], b[2];
+#pragma unroll
+      for (int i = 0; i < 2; i++) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+#pragma unroll
+      for (int j = 0; j < 2; j++) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+#pragma unroll
+      for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  float* C = g.C + blockIdx.z * g.csplit;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + r;
+        const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (m < g.M && n < g.N) {
+          float v = acc[i][j][r];
+          if (g.bias) v += g.bias[m];
+          if (g.relu) v = fmaxf(v, 0.f);
+          float* p = C + m * g.scm + n * g.scn;
+          if (g.accumulate) v += *p;
+          *p = v;
+        }
+      }
+}
+
+// out[i] = sum over s of part[s][i], in split order (deterministic)
+__global__ void __launch_bounds__(kT)
+    k_split_sum(long long n, int nsplit, const float* __restrict__ part, float* out) {
+  const long long i = blockIdx.x * (long long)kT + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int q = 0; q < nsplit; q++) s += part[q * n + i];
+  out[i] = s;
+}
+
+// cols[(ci*9 + ky*3 + kx)][b*P + oy*Wo + ox] = in[ci][b][2oy+ky-1][2ox+kx-1]
+__global__ void __launch_bounds__(kT)
+    k_im2col(int cin, int B, int H, const float* __restrict__ in, float* cols) {
+  const int Ho = H / 2, P = Ho * Ho;
+  const long long NB = (long long)B * P;
+  const long long total = (long long)cin * 9 * NB;
+  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
+  if (idx >= total) return;
+  const long long n = idx % NB;
+  const int r = (int)(idx / NB);
+  const int ci = r / 9, kk = r % 9, ky = kk / 3, kx = kk % 3;
+  const int b = (int)(n / P), p = (int)(n % P), oy = p / Ho, ox = p % Ho;
+  const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
+  float v = 0.f;
+  if (y >= 0 && y < H && x >= 0 && x < H) v = in[(((long long)ci * B + b) * H + y) * H + x];
+  cols[idx] = v;
+}
+
+// gin[ci][b][y][x] = (act[ci][b][y][x] > 0) * sum of the dcols entries that
+// im2col took from (y, x)
+__global__ void __launch_bounds__(kT)
+    k_col2im_relu(int cin, int B, int H, const float* __restrict__ dcols,
+                  const float* __restrict__ act, float* gin) {
+  const int Ho = H / 2, P = Ho * Ho;
+  const long long NB = (long long)B * P;
+  const long long total = (long long)cin * B * H * H;
+  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
+  if (idx >= total) return;
+  const int x = (int)(idx % H);
+  const int y = (int)((idx / H) % H);
+  const int b = (int)((idx / ((long long)H * H)) % B);
+  const int ci = (int)(idx / ((long long)H * H * B));
+  float s = 0.f;
+  if (act[idx] > 0.f) {
+    for (int ky = 0; ky < 3; ky++) {
+      const int ty = y + 1 - ky;
+      if (ty < 0 || (ty & 1) || ty / 2 >= Ho) continue;
+      const int oy = ty / 2;
+      for (int kx = 0; kx < 3; kx++) {
+        const int tx = x + 1 - kx;
+        if (tx < 0 || (tx & 1) || tx / 2 >= Ho) continue;
+        const int ox = tx / 2;
+        s += dcols[(long long)(ci * 9 + ky * 3 + kx) * NB + (long long)b * P + oy * Ho + ox];
+      }
+    }
+  }
+  gin[idx] = s;
+}
+
+// X[c*P + p][b] = act[c][b*P + p]
+__global__ void __launch_bounds__(kT)
+    k_flatten(int C, int B, int P, const float* __restrict__ act, float* X) {
+  const long long total = (long long)C * B * P;
+  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
+  if (idx >= total) return;
+  const int p = (int)(idx % P);
+  const int b = (int)((idx / P) % B);
+  const int c = (int)(idx / ((long long)P * B));
+  X[((long long)c * P + p) * B + b] = act[idx];
+}
+
+// g[c][b*P + p] = (act > 0) * gX[c*P + p][b]
+__global__ void __launch_bounds__(kT)
+    k_unflatten_relu(int C, int B, int P, const float* __restrict__ gX,
+                     const float* __restrict__ act, float* g) {
+  const long long total = (long long)C * B * P;
+  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
+  if (idx >= total) return;
+  const int p = (int)(idx % P);
+  const int b = (int)((idx / P) % B);
+  const int c = (int)(idx / ((long long)P * B));
+  g[idx] = act[idx] > 0.f ? gX[((long long)c * P + p) * B + b] : 0.f;
+}
+
+// out[m] = sum_n g[m*ld + n*sn] over n < N; one block per row, fixed order
+__global__ void __launch_bounds__(kT)
+    k_rowsum(int N, long long ld, long long sn, const float* __restrict__ g, float* out) {
+  __shared__ float red[kT / 64];
+  const float* row = g + blockIdx.x * ld;
+  float s = 0.f;
+  for (int n = threadIdx.x; n < N; n += kT) s += row[n * sn];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+__device__ inline float block_max(float v, float* red) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+
+__device__ inline float block_sum(float v, float* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return red[0] + red[1] + red[2] + red[3];
+}
+
+// softmax cross-entropy of logits[b][*] against label[b]: loss[b] and
+// grad[b][k] = (p_k - [k == label]) / B; one block per sample
+__global__ void __launch_bounds__(kT)
+    k_softmax_xent(int K, int B, const float* __restrict__ logits, const int* __restrict__ label,
+                   float* loss, float* grad) {
+  __shared__ float red[kT / 64];
+  const int b = blockIdx.x;
+  const float* z = logits + (long long)b * K;
+  float mx = -3.0e38f;
+  for (int k = threadIdx.x; k < K; k += kT) mx = fmaxf(mx, z[k]);
+  mx = block_max(mx, red);
+  float s = 0.f;
+  for (int k = threadIdx.x; k < K; k += kT) s += expf(z[k] - mx);
+  s = block_sum(s, red);
+  int y = label[b];
+  y = y < 0 ? 0 : (y >= K ? K - 1 : y);  // labels are validated by the caller; stay in bounds
+  for (int k = threadIdx.x; k < K; k += kT) {
+    float p = expf(z[k] - mx) / s;
+    grad[(long long)b * K + k] = (p - (k == y ? 1.f : 0.f)) / (float)B;
+  }
+  if (threadIdx.x == 0) loss[b] = logf(s) + mx - z[y];
+}
+
+// top-k classes by probability (ties: lower class) and their probabilities
+__global__ void __launch_bounds__(kT)
+    k_propose(int K, const float* __restrict__ logits, int topk, int* top, float* prob) {
+  extern __shared__ float pz[];  // K probabilities
+  __shared__ float red[kT / 64];
+  __shared__ float bv[kT];
+  __shared__ int bi[kT];
+  const int b = blockIdx.x;
+  const float* z = logits + (long long)b * K;
+  float mx = -3.0e38f;
+  for (int k = threadIdx.x; k < K; k += kT) mx = fmaxf(mx, z[k]);
+  mx = block_max(mx, red);
+  float s = 0.f;
+  for (int k = threadIdx.x; k < K; k += kT) {
+    float e = expf(z[k] - mx);
+    pz[k] = e;
+    s += e;
+  }
+  s = block_sum(s, red);
+  for (int k = threadIdx.x; k < K; k += kT) pz[k] /= s;
+  __syncthreads();
+  for (int t = 0; t < topk; t++) {
+    float v = -1.f;
+    int vi = K;
+    for (int k = threadIdx.x; k < K; k += kT)
+      if (pz[k] > v) {
+        v = pz[k];
+        vi = k;
+      }
+    bv[threadIdx.x] = v;
+    bi[threadIdx.x] = vi;
+    __syncthreads();
+    for (int off = kT / 2; off > 0; off >>= 1) {
+      if (threadIdx.x < off) {
+        float v2 = bv[threadIdx.x + off];
+        int i2 = bi[threadIdx.x + off];
+        if (v2 > bv[threadIdx.x] || (v2 == bv[threadIdx.x] && i2 < bi[threadIdx.x])) {
+          bv[threadIdx.x] = v2;
+          bi[threadIdx.x] = i2;
+        }
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      top[(long long)b * topk + t] = bi[0] < K ? bi[0] : -1;
+      prob[(long long)b * topk + t] = bv[0];
+      if (bi[0] < K) pz[bi[0]] = -1.f;
+    }
+    __syncthreads();
+  }
+}
+
+// Adam on one parameter segment; wq gets the BF16-rounded value for weights
+// and the exact value for biases
+__device__ inline float bf16_round(float x) {
+  unsigned u = __float_as_uint(x);
+  u = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+  return __uint_as_float(u);
+}
+
+__global__ void __launch_bounds__(kT)
+    k_adam(long long n, float* master, float* wq, const float* __restrict__ grad, float* m,
+           float* v, float lr, float c1, float c2, int round) {
+  const long long i = blockIdx.x * (long long)kT + threadIdx.x;
+  if (i >= n) return;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float g = grad[i];
+  float mi = b1 * m[i] + (1.f - b1) * g;
+  float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  float w = master[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  master[i] = w;
+  wq[i] = round ? bf16_round(w) : w;
+}
+
+// splitmix64 finaliser: a counter-based stream (seed, counter) -> 64 bits
+__device__ inline unsigned long long mix64(unsigned long long z) {
+  z += 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+// two standard normals (Box-Muller) from counter `ctr` of stream `seed`
+__device__ inline float2 normal2(unsigned long long seed, unsigned long long ctr) {
+  unsigned long long r = mix64(seed ^ mix64(ctr));
+  float u1 = ((float)(r >> 40) + 1.0f) * (1.0f / 16777217.0f);  // (0, 1]
+  float u2 = (float)((r >> 16) & 0xffffffull) * (1.0f / 16777216.0f);
+  float rad = sqrtf(-2.0f * logf(u1));
+  float s, c;
+  sincosf(6.2831853071796f * u2, &s, &c);
+  return make_float2(rad * c, rad * s);
+}
+
+// He-normal weights (std sqrt(2/fan_in)), zero biases
+__global__ void __launch_bounds__(kT)
+    k_init(long long n, unsigned long long seed, long long off, float stdv, float* master,
+           float* wq, int round) {
+  const long long i = blockIdx.x * (long long)kT + threadIdx.x;
+  if (i >= n) return;
+  float w = stdv > 0.f ? normal2(seed, (unsigned long long)(off + i)).x * stdv : 0.f;
+  if (round) w = bf16_round(w);  // the master starts on the BF16 grid
+  master[i] = w;
+  wq[i] = w;
+}
+
+// ------------------------------------------------------------ network input
+// From DATC (origin-centred spectrum) and CTFF: the S x S crop, phase-flipped
+// by sign(Re(CTFR + conj CTFL)) and multiplied by (-1)^(l+j) so that the
+// S-point c2r puts the particle centre at S/2.  The Nyquist row and column of
+// the crop are zero.  Output: in-place c2r layout, pitch S+2 floats.
+__global__ void __launch_bounds__(kT)
+    k_cnn_crop(int nsam, int S, int B, const float2* __restrict__ datc,
+               const float2* __restrict__ ctff, float2* out) {
+  const int jc = nsam / 2 + 1, js = S / 2 + 1;
+  const size_t ic = (size_t)nsam * jc, is = (size_t)S * js;
+  const size_t gid = blockIdx.x * (size_t)kT + threadIdx.x;
+  if (gid >= is * B) return;
+  const int b = (int)(gid / is);
+  const int e = (int)(gid - (size_t)b * is);
+  const int l = e % js, m = e / js;
+  const int j = m >= js ? m - S : m;
+  float2 v = make_float2(0.f, 0.f);
+  if (l < S / 2 && j > -S / 2 && j < S / 2) {
+    const size_t id = (size_t)l + (size_t)jc * wrapn(j, nsam);
+    const float2 d = datc[(size_t)b * ic + id];
+    const float2 cr = ctff[(size_t)b * 2 * ic + id], cl = ctff[(size_t)b * 2 * ic + ic + id];
+    float sg = (cr.x + cl.x) < 0.f ? -1.f : 1.f;
+    if ((l + j) & 1) sg = -sg;
+    v = make_float2(d.x * sg, d.y * sg);
+  }
+  out[gid] = v;
+}
+
+// Training images in the same form: section x CTF x shift ramp + white noise
+// at SNR, cropped and flipped as k_cnn_crop.  One block per image.
+//   spar[b]: SHX, SHY (radians per frequency unit, the ramp is
+//   exp(-i(SHX l + SHY j)) as in a particle shifted by (SHX, SHY)), DF1, DF2,
+//   ANGAST, SNR, 0, 0.
+// The SNR is the signal variance over `disc` pixels of the NSAM^2 image
+// (Parseval on the crop) over the per-pixel noise variance.
+__global__ void __launch_bounds__(kT)
+    k_cnn_synth(int nsam, int S, const float2* __restrict__ a3df, const float* __restrict__ dm,
+                const float* __restrict__ spar, PrepGlobals g, float disc,
+                unsigned long long seed, float2* out) {
+  __shared__ float red[kT / 64];
+  const int js = S / 2 + 1;
+  const size_t is = (size_t)S * js;
+  const int b = blockIdx.x;
+  const float* sp = spar + 8 * (size_t)b;
+  const float shx = sp[0], shy = sp[1], df1 = sp[2], df2 = sp[3], ang = sp[4], snr = sp[5];
+  const float* d = dm + 9 * (size_t)b;
+  float2* o = out + (size_t)b * is;
+  float pw = 0.f, dc = 0.f;
+  for (int e = threadIdx.x; e < (int)is; e += kT) {
+    const int l = e % js, m = e / js;
+    const int j = m >= js ? m - S : m;
+    float2 v = make_float2(0.f, 0.f);
+    if (l < S / 2 && j > -S / 2 && j < S / 2) {
+      cf a = ainterpo_dev(nsam, a3df, d, l, j);
+      cf cr = ctf_dev(g, df1, df2, ang, g.thetatr, l, j);
+      cf cl = ctf_dev(g, df1, df2, ang, g.thetatr, -l, -j);
+      cf c = cf{cr.re + cl.re, cr.im - cl.im};
+      float sn, cs;
+      sincosf(-(shx * (float)l + shy * (float)j), &sn, &cs);
+      cf s = cmul(cmul(a, c), cf{cs, sn});
+      v = make_float2(s.re, s.im);
+      const float w = l > 0 ? 2.f : 1.f;
+      pw += w * (s.re * s.re + s.im * s.im);
+      if (l == 0 && j == 0) dc = s.re;
+    }
+    o[e] = v;
+  }
+  pw = block_sum(pw, red);
+  dc = block_sum(dc, red);
+  const float n2 = (float)nsam * (float)nsam;
+  const float sumsq = pw / n2, mean = dc / disc;
+  float var = sumsq / disc - mean * mean;
+  var = var > 0.f ? var : 0.f;
+  const float sig = sqrtf(n2 * var / snr * 0.5f);  // per component of a coefficient
+  for (int e = threadIdx.x; e < (int)is; e += kT) {
+    const int l = e % js, m = e / js;
+    const int j = m >= js ? m - S : m;
+    if (!(l < S / 2 && j > -S / 2 && j < S / 2)) continue;
+    float2 v = o[e];
+    float2 z = normal2(seed, ((unsigned long long)b << 32) | (unsigned)e);
+    v.x += sig * z.x;
+    v.y += sig * z.y;
+    cf cr = ctf_dev(g, df1, df2, ang, g.thetatr, l, j);
+    cf cl = ctf_dev(g, df1, df2, ang, g.thetatr, -l, -j);
+    float sg = (cr.re + cl.re) < 0.f ? -1.f : 1.f;
+    if ((l + j) & 1) sg = -sg;
+    o[e] = make_float2(v.x * sg, v.y * sg);
+  }
+}
+
+// (x - mean) / std over each S x S image (pitch S+2 in, dense out)
+__global__ void __launch_bounds__(kT)
+    k_cnn_normalise(int S, const float* __restrict__ buf, float* img) {
+  __shared__ float red[kT / 64];
+  const int b = blockIdx.x;
+  const float* x = buf + (size_t)b * S * (S + 2);
+  float s = 0.f;
+  for (int e = threadIdx.x; e < S * S; e += kT) s += x[(e / S) * (S + 2) + e % S];
+  const float mean = block_sum(s, red) / (float)(S * S);
+  float q = 0.f;
+  for (int e = threadIdx.x; e < S * S; e += kT) {
+    float t = x[(e / S) * (S + 2) + e % S] - mean;
+    q += t * t;
+  }
+  const float var = block_sum(q, red) / (float)(S * S);
+  const float inv = var > 0.f ? 1.0f / sqrtf(var) : 1.0f;
+  float* y = img + (size_t)b * S * S;
+  for (int e = threadIdx.x; e < S * S; e += kT) y[e] = (x[(e / S) * (S + 2) + e % S] - mean) * inv;
+}
+
+// ------------------------------------------------------ orientation classes
+// Orientation matrices are A[k][j] = dm[3j + k] (the transpose of the DM
+// rows); two orientations are equivalent under symmetry S when S A = B.  The
+// cosine relation trace((S A)^T B) = 1 + 2 cos(angle) gives the distance.
+__device__ inline float sym_trace(const float* s, const float* a, const float* b) {
+  float t = 0.f;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      float sa = s[3 * i + 0] * a[3 * j + 0] + s[3 * i + 1] * a[3 * j + 1] +
+                 s[3 * i + 2] * a[3 * j + 2];  // (S A)[i][j], A[k][j] = a[3j + k]
+      t += sa * b[3 * j + i];                     // B[i][j] = b[3j + i]
+    }
+  return t;
+}
+
+__global__ void __launch_bounds__(kT)
+    k_orient_nearest(int n, const float* __restrict__ dm, int ncls, const float* __restrict__ dmc,
+                     int nsym, const float* __restrict__ sym, int* label) {
+  __shared__ float cls[kT * 9];
+  __shared__ float ss[60 * 9];
+  for (int e = threadIdx.x; e < nsym * 9; e += kT) ss[e] = sym[e];
+  const int i = blockIdx.x * kT + threadIdx.x;
+  float a[9];
+  if (i < n)
+    for (int q = 0; q < 9; q++) a[q] = dm[9 * (size_t)i + q];
+  float best = -3.0e38f;
+  int bi = -1;
+  for (int c0 = 0; c0 < ncls; c0 += kT) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < kT * 9; e += kT) {
+      int c = c0 + e / 9;
+      cls[e] = c < ncls ? dmc[9 * (size_t)c0 + e] : 0.f;
+    }
+    __syncthreads();
+    if (i < n) {
+      const int cn = min(kT, ncls - c0);
+      for (int c = 0; c < cn; c++)
+        for (int q = 0; q < nsym; q++) {
+          float t = sym_trace(ss + 9 * q, a, cls + 9 * c);
+          if (t > best) {
+            best = t;
+            bi = c0 + c;
+          }
+        }
+    }
+  }
+  if (i < n) label[i] = bi;
+}
+
+// mask[b][w] bit i: orientation a = 32 w + i lies within the angle of
+// orientation b (max over symmetry of the trace >= tmin)
+__global__ void __launch_bounds__(kT)
+    k_orient_within(int na, const float* __restrict__ dma, int nb, const float* __restrict__ dmb,
+                    int nsym, const float* __restrict__ sym, float tmin, unsigned* mask) {
+  __shared__ float ss[60 * 9];
+  for (int e = threadIdx.x; e < nsym * 9; e += kT) ss[e] = sym[e];
+  __syncthreads();
+  const int nw = (na + 31) / 32;
+  const long long gid = blockIdx.x * (long long)kT + threadIdx.x;
+  if (gid >= (long long)nw * nb) return;
+  const int w = (int)(gid % nw), b = (int)(gid / nw);
+  float bm[9];
+  for (int q = 0; q < 9; q++) bm[q] = dmb[9 * (size_t)b + q];
+  unsigned bits = 0;
+  for (int i = 0; i < 32; i++) {
+    const int a = 32 * w + i;
+    if (a >= na) break;
+    float am[9];
+    for (int q = 0; q < 9; q++) am[q] = dma[9 * (size_t)a + q];
+    float t = -3.0e38f;
+    for (int q = 0; q < nsym; q++) t = fmaxf(t, sym_trace(ss + 9 * q, am, bm));
+    if (t >= tmin) bits |= 1u << i;
+  }
+  mask[gid] = bits;
+}
+
+inline unsigned nblk(long long n) { return (unsigned)((n + kT - 1) / kT); }
+
+int gemm(Ctx& c, int M, int N, int K, const float* A, long long sam, long long sak, const float* B,
+         long long sbk, long long sbn, float* C, long long scm, long long scn,
+         const float* bias = nullptr, bool relu = false, bool accumulate = false) {
+  GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, ((K + 15) / 16) * 16, 0, bias,
+             relu ? 1 : 0, accumulate ? 1 : 0};
+  dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
+  FR_CHECK(grid.y < 65536u, "GEMM too tall");
+  hipLaunchKernelGGL(k_gemm, grid, dim3(kT), 0, c.stream, g);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// C[M][N] (row-major, dense) = A B with K split over `nsplit` partial
+// products in `part` (nsplit * M * N floats), summed in order
+int gemm_splitk(Ctx& c, int M, int N, int K, const float* A, long long sam, long long sak,
+                const float* B, long long sbk, long long sbn, float* C, float* part,
+                int nsplit) {
+  int kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
+  nsplit = (K + kchunk - 1) / kchunk;
+  GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, part, N, 1, kchunk, (long long)M * N, nullptr,
+             0, 0};
+  dim3 grid((N + 63) / 64, (M + 63) / 64, nsplit);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(kT), 0, c.stream, g);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_split_sum, dim3(nblk((long long)M * N)), dim3(kT), 0, c.stream,
+                     (long long)M * N, nsplit, part, C);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// workspace offsets (floats) for a batch of B
+struct Ws {
+  size_t act[8];  // act[l]: output of convolution l, [C][B][S>>(l+1)]^2
+  size_t X, logits, cols, dcols, gA, gB, gX, gL, part, total;
+};
+
+constexpr int kMaxSplit = 64;
+
+Ws ws_layout(const CnnLayout& lo, int B, bool backward) {
+  Ws w{};
+  size_t o = 0;
+  for (int l = 0; l < lo.L; l++) {
+    w.act[l] = o;
+    o += (size_t)lo.C * B * lo.hout(l) * lo.hout(l);
+  }
+  w.X = o;
+  o += (size_t)lo.F() * B;
+  w.logits = o;
+  o += (size_t)lo.K * B;
+  size_t cols = 0, gmax = 0;
+  for (int l = 0; l < lo.L; l++) {
+    size_t cl = (size_t)lo.cin(l) * 9 * B * lo.hout(l) * lo.hout(l);
+    cols = cl > cols ? cl : cols;
+    size_t ga = (size_t)lo.C * B * lo.hout(l) * lo.hout(l);
+    gmax = ga > gmax ? ga : gmax;
+  }
+  w.cols = o;
+  o += cols;
+  if (backward) {
+    w.dcols = o;
+    o += cols;
+    w.gA = o;
+    o += gmax;
+    w.gB = o;
+    o += gmax;
+    w.gX = o;
+    o += (size_t)lo.F() * B;
+    w.gL = o;
+    o += (size_t)lo.K * B;
+    w.part = o;
+    size_t pm = (size_t)lo.K * lo.F();
+    size_t pc = (size_t)lo.C * lo.C * 9;
+    o += (size_t)kMaxSplit * (pm > pc ? pm : pc);
+  }
+  w.total = o;
+  return w;
+}
+
+}  // namespace
+
+int cnn_layout(int S, int C, int L, int K, CnnLayout* out) {
+  FR_CHECK(L >= 1 && L <= 7, "nconv must be 1..7 (got %d)", L);
+  FR_CHECK(S >= 2 && S <= 1024 && (S % (1 << L)) == 0, "input size %d must be a multiple of 2^%d",
+           S, L);
+  FR_CHECK(S % 2 == 0 && (S == 0 || fft_supported(S)), "input size %d not supported by the FFT", S);
+  FR_CHECK(C >= 1 && C <= 256, "channels must be 1..256 (got %d)", C);
+  FR_CHECK(K >= 1 && K <= 65536, "classes must be 1..65536 (got %d)", K);
+  CnnLayout lo;
+  lo.S = S;
+  lo.C = C;
+  lo.L = L;
+  lo.K = K;
+  long long o = 0;
+  for (int l = 0; l < L; l++) {
+    lo.w_off[l] = o;
+    o += (long long)C * lo.cin(l) * 9;
+    lo.b_off[l] = o;
+    o += C;
+  }
+  lo.fw_off = o;
+  o += (long long)K * lo.F();
+  lo.fb_off = o;
+  o += K;
+  lo.total = o;
+  *out = lo;
+  return 0;
+}
+
+size_t cnn_workspace(const CnnLayout& lo, int B, bool backward) {
+  return ws_layout(lo, B, backward).total;
+}
+
+int cnn_init(Ctx& c, const CnnLayout& lo, unsigned long long seed, float* master, float* wq) {
+  for (int l = 0; l <= lo.L; l++) {
+    long long wo, bo, nw, nbias;
+    float stdv;
+    if (l < lo.L) {
+      wo = lo.w_off[l];
+      bo = lo.b_off[l];
+      nw = (long long)lo.C * lo.cin(l) * 9;
+      nbias = lo.C;
+      stdv = std::sqrt(2.0f / (float)(lo.cin(l) * 9));
+    } else {
+      wo = lo.fw_off;
+      bo = lo.fb_off;
+      nw = (long long)lo.K * lo.F();
+      nbias = lo.K;
+      stdv = std::sqrt(1.0f / (float)lo.F());
+    }
+    hipLaunchKernelGGL(k_init, dim3(nblk(nw)), dim3(kT), 0, c.stream, nw, seed, wo, stdv,
+                       master + wo, wq + wo, 1);
+    FR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_init, dim3(nblk(nbias)), dim3(kT), 0, c.stream, nbias, seed, bo, 0.0f,
+                       master + bo, wq + bo, 0);
+    FR_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+namespace {
+
+int forward_impl(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
+                 float* ws, const Ws& w) {
+  const float* in = img;
+  for (int l = 0; l < lo.L; l++) {
+    const int H = lo.hin(l), Ho = lo.hout(l), cin = lo.cin(l);
+    const long long NB = (long long)B * Ho * Ho;
+    const long long ncol = (long long)cin * 9 * NB;
+    hipLaunchKernelGGL(k_im2col, dim3(nblk(ncol)), dim3(kT), 0, c.stream, cin, B, H, in,
+                       ws + w.cols);
+    FR_HIP(hipGetLastError());
+    FR_CHECK(NB < (1ll << 31), "batch too large");
+    // out[C][NB] = W[C][cin*9] x cols[cin*9][NB] + b, ReLU
+    if (int r = gemm(c, lo.C, (int)NB, cin * 9, wq + lo.w_off[l], cin * 9, 1, ws + w.cols, NB, 1,
+                     ws + w.act[l], NB, 1, wq + lo.b_off[l], true))
+      return r;
+    in = ws + w.act[l];
+  }
+  const int P = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
+  const long long nx = (long long)lo.C * B * P;
+  hipLaunchKernelGGL(k_flatten, dim3(nblk(nx)), dim3(kT), 0, c.stream, lo.C, B, P, in, ws + w.X);
+  FR_HIP(hipGetLastError());
+  // logits[b][k] = sum_f Wfc[k][f] X[f][b] + bfc[k]  (C element (k, b) at b*K + k)
+  return gemm(c, lo.K, B, lo.F(), wq + lo.fw_off, lo.F(), 1, ws + w.X, B, 1, ws + w.logits, 1,
+              lo.K, wq + lo.fb_off, false);
+}
+
+}  // namespace
+
+int cnn_forward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img, float* ws,
+                float* logits) {
+  Ws w = ws_layout(lo, B, false);
+  if (int r = forward_impl(c, lo, wq, B, img, ws, w)) return r;
+  FR_HIP(hipMemcpyAsync(logits, ws + w.logits, sizeof(float) * (size_t)lo.K * B,
+                        hipMemcpyDeviceToDevice, c.stream));
+  return 0;
+}
+
+int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
+                 const int* label, float* ws, float* loss, float* grad) {
+  Ws w = ws_layout(lo, B, true);
+  if (int r = forward_impl(c, lo, wq, B, img, ws, w)) return r;
+  const int K = lo.K, F = lo.F();
+  hipLaunchKernelGGL(k_softmax_xent, dim3(B), dim3(kT), 0, c.stream, K, B, ws + w.logits, label,
+                     loss, ws + w.gL);
+  FR_HIP(hipGetLastError());
+  // gWfc[k][f] = sum_b gL[b][k] X[f][b]
+  if (int r = gemm(c, K, F, B, ws + w.gL, 1, K, ws + w.X, 1, B, grad + lo.fw_off, F, 1)) return r;
+  // gbfc[k] = sum_b gL[b][k]
+  hipLaunchKernelGGL(k_rowsum, dim3(K), dim3(kT), 0, c.stream, B, 1ll, (long long)K, ws + w.gL,
+                     grad + lo.fb_off);
+  FR_HIP(hipGetLastError());
+  // gX[f][b] = sum_k Wfc[k][f] gL[b][k]
+  if (int r = gemm(c, F, B, K, wq + lo.fw_off, 1, F, ws + w.gL, 1, K, ws + w.gX, B, 1)) return r;
+  const int PL = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
+  const long long nl = (long long)lo.C * B * PL;
+  float* gcur = ws + w.gA;
+  float* gnext = ws + w.gB;
+  hipLaunchKernelGGL(k_unflatten_relu, dim3(nblk(nl)), dim3(kT), 0, c.stream, lo.C, B, PL,
+                     ws + w.gX, ws + w.act[lo.L - 1], gcur);
+  FR_HIP(hipGetLastError());
+  for (int l = lo.L - 1; l >= 0; l--) {
+    const int H = lo.hin(l), Ho = lo.hout(l), cin = lo.cin(l);
+    const long long NB = (long long)B * Ho * Ho;
+    const float* in = l == 0 ? img : ws + w.act[l - 1];
+    hipLaunchKernelGGL(k_im2col, dim3(nblk((long long)cin * 9 * NB)), dim3(kT), 0, c.stream, cin,
+                       B, H, in, ws + w.cols);
+    FR_HIP(hipGetLastError());
+    // gW[C][cin*9] = sum_n g[C][n] cols[cin*9][n]
+    int nsplit = (int)((NB + 4095) / 4096);
+    nsplit = nsplit < 1 ? 1 : (nsplit > kMaxSplit ? kMaxSplit : nsplit);
+    if (int r = gemm_splitk(c, lo.C, cin * 9, (int)NB, gcur, NB, 1, ws + w.cols, 1, NB,
+                            grad + lo.w_off[l], ws + w.part, nsplit))
+      return r;
+    hipLaunchKernelGGL(k_rowsum, dim3(lo.C), dim3(kT), 0, c.stream, (int)NB, NB, 1ll, gcur,
+                       grad + lo.b_off[l]);
+    FR_HIP(hipGetLastError());
+    if (l > 0) {
+      // dcols[cin*9][n] = sum_co W[co][k] g[co][n]
+      if (int r = gemm(c, cin * 9, (int)NB, lo.C, wq + lo.w_off[l], 1, cin * 9, gcur, NB, 1,
+                       ws + w.dcols, NB, 1))
+        return r;
+      const long long ng = (long long)cin * B * H * H;
+      hipLaunchKernelGGL(k_col2im_relu, dim3(nblk(ng)), dim3(kT), 0, c.stream, cin, B, H,
+                         ws + w.dcols, ws + w.act[l - 1], gnext);
+      FR_HIP(hipGetLastError());
+      float* t = gcur;
+      gcur = gnext;
+      gnext = t;
+    }
+  }
+  return 0;
+}
+
+int cnn_adam(Ctx& c, const CnnLayout& lo, float* master, float* wq, const float* grad, float* m,
+             float* v, int step, float lr) {
+  FR_CHECK(step >= 1, "Adam step counts from 1");
+  const float c1 = 1.0f - std::pow(0.9f, (float)step);
+  const float c2 = 1.0f - std::pow(0.999f, (float)step);
+  for (int l = 0; l <= lo.L; l++) {
+    long long wo = l < lo.L ? lo.w_off[l] : lo.fw_off;
+    long long bo = l < lo.L ? lo.b_off[l] : lo.fb_off;
+    long long nw = bo - wo;
+    long long nbias = l < lo.L ? lo.C : lo.K;
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nw)), dim3(kT), 0, c.stream, nw, master + wo, wq + wo,
+                       grad + wo, m + wo, v + wo, lr, c1, c2, 1);
+    FR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_adam, dim3(nblk(nbias)), dim3(kT), 0, c.stream, nbias, master + bo,
+                       wq + bo, grad + bo, m + bo, v + bo, lr, c1, c2, 0);
+    FR_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
+int cnn_propose(Ctx& c, int K, int B, const float* logits, int topk, int* top, float* prob) {
+  FR_CHECK(topk >= 1 && topk <= K && topk <= 64, "topk must be 1..min(K, 64)");
+  FR_CHECK((size_t)K * sizeof(float) <= 48 * 1024, "too many classes for the proposal kernel");
+  hipLaunchKernelGGL(k_propose, dim3(B), dim3(kT), sizeof(float) * K, c.stream, K, logits, topk,
+                     top, prob);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int cnn_images(Ctx& c, int nsam, int S, int B, const float* datc, const float* ctff, float* buf,
+               float* img) {
+  const size_t is = (size_t)S * (S / 2 + 1);
+  hipLaunchKernelGGL(k_cnn_crop, dim3(nblk((long long)(is * B))), dim3(kT), 0, c.stream, nsam, S,
+                     B, reinterpret_cast<const float2*>(datc), reinterpret_cast<const float2*>(ctff),
+                     reinterpret_cast<float2*>(buf));
+  FR_HIP(hipGetLastError());
+  if (int r = fft_c2r_2d(c, S, B, buf, (size_t)S * (S + 2))) return r;
+  hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int cnn_synth(Ctx& c, int nsam, int S, int B, const float* a3df, const float* dm,
+              const float* spar, const PrepGlobals& g, float disc, unsigned long long seed,
+              float* buf, float* img) {
+  hipLaunchKernelGGL(k_cnn_synth, dim3(B), dim3(kT), 0, c.stream, nsam, S,
+                     reinterpret_cast<const float2*>(a3df), dm, spar, g, disc, seed,
+                     reinterpret_cast<float2*>(buf));
+  FR_HIP(hipGetLastError());
+  if (int r = fft_c2r_2d(c, S, B, buf, (size_t)S * (S + 2))) return r;
+  hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int orient_nearest(Ctx& c, int n, const float* dm, int ncls, const float* dmc, int nsym,
+                   const float* sym, int* label) {
+  FR_CHECK(nsym >= 1 && nsym <= 60, "1..60 symmetry operators");
+  hipLaunchKernelGGL(k_orient_nearest, dim3(nblk(n)), dim3(kT), 0, c.stream, n, dm, ncls, dmc,
+                     nsym, sym, label);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int orient_within(Ctx& c, int na, const float* dma, int nb, const float* dmb, int nsym,
+                  const float* sym, float tmin, unsigned* mask) {
+  FR_CHECK(nsym >= 1 && nsym <= 60, "1..60 symmetry operators");
+  const long long nw = (na + 31) / 32;
+  hipLaunchKernelGGL(k_orient_within, dim3(nblk(nw * nb)), dim3(kT), 0, c.stream, na, dma, nb,
+                     dmb, nsym, sym, tmin, mask);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace fr

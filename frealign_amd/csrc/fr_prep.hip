@@ -10,46 +10,14 @@
 
 #include "fr_common.h"
 #include "fr_kernels.h"
+#include "fr_ctf.h"
 
 namespace fr {
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr float kTwoPi = 6.2831853071796f;
 constexpr float kPi = 3.1415926535897f;
-
-// ctf.f:8-64
-__device__ inline cf ctf_dev(const PrepGlobals& g, float df1, float df2, float angast,
-                             float thetatr, int ix, int iy) {
-  float wgh2 = fabsf(g.wgh2);
-  cf out;
-  if (g.wgh2 != -1.0f) {
-    float radf = (float)(ix * ix + iy * iy);
-    if (radf != 0.0f) {
-      float rad = sqrtf(radf);
-      float angle = rad * thetatr;
-      float angspt = atan2f((float)iy, (float)ix);
-      float c1 = kTwoPi * angle * angle / (2.0f * g.wl);
-      float c2 = c1 * g.cs * angle * angle / 2.0f;
-      float ccos = cosf(2.0f * (angspt - angast));
-      float df = -0.5f * (df1 + df2 + ccos * (df1 - df2));
-      float chi = c1 * df + c2;
-      // beam tilt (ctf.f:44)
-      chi = chi + kTwoPi * g.cs * angle * angle * ((float)ix * g.tx + (float)iy * g.ty) * thetatr /
-                      g.wl / 1000.0f;
-      float s, c;
-      sincosf(chi, &s, &c);
-      out = cf{g.wgh1 * s - wgh2 * c, -g.wgh1 * c - wgh2 * s};
-    } else {
-      out = cf{-wgh2, 0.0f};
-    }
-    if (g.wgh2 < 0.0f) out = cf{-out.re, -out.im};
-  } else {
-    out = cf{1.0f, 0.0f};
-  }
-  return out;
-}
 
 // ctfapply.f:64-80: CTF table and shifted spectrum; one thread per half-plane point
 __global__ void __launch_bounds__(kThreads)

@@ -189,6 +189,39 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+// The same normalisation for per-particle candidate lists: one workgroup per
+// (256 candidates, particle), |c_p|^2 staged in LDS.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_search_norm_list(int ncand, int nbank, int kpad, const T* __restrict__ bank,
+                       const int* __restrict__ cand, const float* __restrict__ c2, float* nrm) {
+  extern __shared__ float c2s[];
+  const int half = kpad / 2, p = blockIdx.y;
+  for (int k = threadIdx.x; k < half; k += kThreads) c2s[k] = c2[(size_t)p * half + k];
+  __syncthreads();
+  const int col = blockIdx.x * kThreads + threadIdx.x;
+  if (col >= ncand) return;
+  const int o = cand[(size_t)p * ncand + col];
+  float acc = 0.f;
+  if (o >= 0 && o < nbank) {
+    const T* row = bank + (size_t)o * kpad;
+    for (int k = 0; k < half; k++) {
+      float re = Elem<T>::f(row[2 * k]), im = Elem<T>::f(row[2 * k + 1]);
+      acc += c2s[k] * (re * re + im * im);
+    }
+  }
+  nrm[(size_t)p * ncand + col] = acc;
+}
+
+// candidate positions -> bank orientation indices
+__global__ void __launch_bounds__(kThreads)
+    k_map_cand(int np, int ncand, int topk, const int* __restrict__ cand, int* top_o) {
+  const int i = blockIdx.x * kThreads + threadIdx.x;
+  if (i >= np * topk) return;
+  const int p = i / topk, t = top_o[i];
+  top_o[i] = (t >= 0 && t < ncand) ? cand[(size_t)p * ncand + t] : -1;
+}
+
 // ---------------------------------------------------------------- GEMM
 constexpr int kGemmThreads = 512;  // 8 waves: 4 row groups x 2 column groups
 
@@ -266,11 +299,11 @@ struct BPieces {
 
 template <typename T>
 __global__ void __launch_bounds__(kGemmThreads)
-    k_search_gemm(int norient, int kpad, int np, int ntile, const T* __restrict__ bank,
-                  const float2* __restrict__ ap, const float* __restrict__ nrm_po,
-                  const float* __restrict__ sum2, const float2* __restrict__ shx_tab,
-                  const float2* __restrict__ shy_tab, float* score, int* best_row,
-                  float* raw_peak) {
+    k_search_gemm(int nrow, int nbank, int kpad, int np, int ntile, const T* __restrict__ bank,
+                  const int* __restrict__ cand, const float2* __restrict__ ap,
+                  const float* __restrict__ nrm_po, const float* __restrict__ sum2,
+                  const float2* __restrict__ shx_tab, const float2* __restrict__ shy_tab,
+                  float* score, int* best_row, float* raw_peak) {
   constexpr int P = Elem<T>::kPad;
   using L = Lds<T>;
   using BP = BPieces<T>;
@@ -311,12 +344,20 @@ __global__ void __launch_bounds__(kGemmThreads)
   uint4 breg[BP::per_thread];
   float2 treg;
 
+  // bank rows of this thread's columns: the column itself, or the
+  // particle's candidate list entry (-1: an empty column)
+  int orow[BP::per_thread];
+#pragma unroll
+  for (int u = 0; u < BP::per_thread; u++) {
+    int col = o0 + bc0 + u * bstep;
+    int o = col < nrow ? (cand ? cand[(size_t)p * nrow + col] : col) : -1;
+    orow[u] = (o >= 0 && o < nbank) ? o : -1;
+  }
   auto load_b = [&](int chunk) {
 #pragma unroll
     for (int u = 0; u < BP::per_thread; u++) {
-      int o = o0 + bc0 + u * bstep;
-      if (o < norient)
-        breg[u] = *reinterpret_cast<const uint4*>(bank + (size_t)o * kpad + 32 * chunk +
+      if (orow[u] >= 0)
+        breg[u] = *reinterpret_cast<const uint4*>(bank + (size_t)orow[u] * kpad + 32 * chunk +
                                                    bq * BP::elems);
       else
         breg[u] = make_uint4(0u, 0u, 0u, 0u);
@@ -433,8 +474,8 @@ __global__ void __launch_bounds__(kGemmThreads)
       float box = 0.f;
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
-      if (o < norient) {
-        size_t out = (size_t)p * norient + o;
+      if (o < nrow) {
+        size_t out = (size_t)p * nrow + o;
         float n = nrm_po[out];
         float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         score[out] = cc;
@@ -551,36 +592,52 @@ int launch_search_norm(Ctx& c, int norient, int kpad, int np, const void* bank, 
   return 0;
 }
 
-int launch_search_gemm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
-                       const float* ap, const float* nrm, const float* sum2, const float* shx_tab,
-                       const float* shy_tab, float* score, int* best_row, float* raw_peak) {
-  const int ntile = (norient + kCols - 1) / kCols;
+int launch_search_norm_list(Ctx& c, int ncand, int nbank, int kpad, int np, const void* bank,
+                            bool bf16, const int* cand, const float* c2, float* nrm) {
+  dim3 grid((ncand + kThreads - 1) / kThreads, np);
+  const size_t lds = sizeof(float) * (kpad / 2);
+  FR_CHECK(lds <= 64 * 1024, "annulus too large for the list normalisation");
+  if (bf16)
+    hipLaunchKernelGGL(k_search_norm_list<__bf16>, grid, dim3(kThreads), lds, c.stream, ncand,
+                       nbank, kpad, reinterpret_cast<const __bf16*>(bank), cand, c2, nrm);
+  else
+    hipLaunchKernelGGL(k_search_norm_list<float>, grid, dim3(kThreads), lds, c.stream, ncand,
+                       nbank, kpad, reinterpret_cast<const float*>(bank), cand, c2, nrm);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_map_cand(Ctx& c, int np, int ncand, int topk, const int* cand, int* top_o) {
+  const int n = np * topk;
+  hipLaunchKernelGGL(k_map_cand, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, c.stream,
+                     np, ncand, topk, cand, top_o);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_search_gemm(Ctx& c, int nrow, int nbank, int kpad, int np, const void* bank,
+                       bool bf16, const int* cand, const float* ap, const float* nrm, const float* sum2,
+                       const float* shx_tab, const float* shy_tab, float* score, int* best_row,
+                       float* raw_peak) {
+  const int ntile = (nrow + kCols - 1) / kCols;
   const int ntile8 = (ntile + 7) / 8 * 8;
   const size_t nblk = (size_t)ntile8 * np;
   FR_CHECK(nblk < (1u << 31), "search grid too large");
   const size_t lds = search_gemm_lds(bf16);
   FR_CHECK(lds <= 160 * 1024, "LDS budget exceeded (%zu)", lds);
   if (bf16) {
-    static bool attr = false;
-    if (!attr) {
-      FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<__bf16>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      attr = true;
-    }
+    FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<__bf16>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, norient, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank),
+                       c.stream, nrow, nbank, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank), cand,
                        reinterpret_cast<const float2*>(ap), nrm, sum2,
                        reinterpret_cast<const float2*>(shx_tab),
                        reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
   } else {
-    static bool attr = false;
-    if (!attr) {
-      FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<float>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      attr = true;
-    }
+    FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<float>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, norient, kpad, np, ntile, reinterpret_cast<const float*>(bank),
+                       c.stream, nrow, nbank, kpad, np, ntile, reinterpret_cast<const float*>(bank), cand,
                        reinterpret_cast<const float2*>(ap), nrm, sum2,
                        reinterpret_cast<const float2*>(shx_tab),
                        reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);

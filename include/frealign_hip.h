@@ -175,6 +175,78 @@ int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, i
                     const float *d_ctff, int topk, int *d_top_o, int *d_top_row,
                     float *d_top_score);
 
+/* Candidate-list form of fr_search_batch (no reference counterpart: the
+ * learned proposal prunes PSEARCH's view loop, psearch.f:69-87, to the
+ * neighbourhoods of the proposed classes).  Particle p is scored against the
+ * bank rows d_cand[p*ncand .. p*ncand+ncand-1] (-1: empty slot), which come
+ * in groups of `group` in-plane candidates; each group contributes its first
+ * maximum and the topk best groups are returned as for fr_search_batch, with
+ * d_top_o holding bank orientation indices. */
+int fr_search_list_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, int norient,
+                         const void *d_bank, int dtype, int np, const float *d_outc,
+                         const float *d_ctff, int ncand, const int *d_cand, int group, int topk,
+                         int *d_top_o, int *d_top_row, float *d_top_score);
+
+/* ------------------------------------------- learned orientation proposal
+ * No counterpart in the reference.  A network of nconv 3x3 / stride 2 / pad 1
+ * convolutions with ReLU (1 -> C -> ... -> C channels on an S x S input) and
+ * one fully connected layer to K class logits.  Parameters are one flat
+ * array (per convolution W[C_out][C_in][3][3], b[C_out]; then W_fc[K][F],
+ * b_fc[K], F = C (S >> nconv)^2): an FP32 master copy and d_wq, the values
+ * the kernels use (weights rounded to BF16, biases exact).  Images are
+ * nb x S x S FP32, logits nb x K. */
+int fr_cnn_param_count(int S, int C, int nconv, int K, long long *nparam);
+
+/* He-normal weights from a counter-based stream of `seed`, zero biases. */
+int fr_cnn_init(fr_ctx *ctx, int S, int C, int nconv, int K, unsigned long long seed,
+                float *d_master, float *d_wq);
+
+int fr_cnn_forward(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,
+                   const float *d_img, float *d_logits);
+
+/* Softmax cross-entropy against d_label (class per image): per-image losses
+ * in d_loss[nb] and the gradient of their mean in d_grad (param layout). */
+int fr_cnn_gradients(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,
+                     const float *d_img, const int *d_label, float *d_loss, float *d_grad);
+
+/* One Adam step (beta 0.9/0.999, eps 1e-8, bias-corrected; step >= 1) on the
+ * master copy; d_wq is refreshed from it. */
+int fr_cnn_adam(fr_ctx *ctx, int S, int C, int nconv, int K, float *d_master, float *d_wq,
+                const float *d_grad, float *d_m, float *d_v, int step, float lr);
+
+/* Softmax of each logit row; the topk classes (descending probability, lower
+ * class on ties) and their probabilities, nb x topk each. */
+int fr_cnn_propose(fr_ctx *ctx, int K, int nb, const float *d_logits, int topk, int *d_top,
+                   float *d_prob);
+
+/* Network input of nb prepared particles (DATC and CTFF of fr_prep_batch):
+ * the S x S Fourier crop, phase-flipped by sign(Re(CTFR + conj CTFL)),
+ * brought to real space with the particle centre at S/2 and normalised to
+ * zero mean and unit variance. */
+int fr_cnn_images(fr_ctx *ctx, int nsam, int S, int nb, const float *d_datc, const float *d_ctff,
+                  float *d_img);
+
+/* Training images of the same form from the current reference: central
+ * section d_dm[9b..] (fr_rotation with amag=0) x CTF (ctf.f, gpar as in
+ * fr_prep_batch) x the ramp exp(-i(SHX I + SHY J)) + white noise at SNR.
+ * d_spar is nb x 8: SHX, SHY, DF1, DF2, ANGAST, SNR, 0, 0.  The SNR is the
+ * signal variance over `disc` pixels of the NSAM^2 image over the noise
+ * variance per pixel.  Noise comes from a counter-based stream of `seed`. */
+int fr_cnn_synth(fr_ctx *ctx, int nsam, int S, int nb, const float *d_a3df, const float *d_dm,
+                 const float *d_spar, const float *gpar, float disc, unsigned long long seed,
+                 float *d_img);
+
+/* Orientation classes under a point group.  Orientation matrices are the
+ * DM arrays of fr_rotation (amag=0); d_sym holds nsym row-major 3x3
+ * operators.  fr_orient_nearest: class of each of n orientations (largest
+ * symmetry-maximised trace).  fr_orient_within: bit i of d_mask[b*W + w]
+ * (W = ceil(na/32)) is set when orientation a = 32w+i of set A lies within
+ * max_deg of orientation b of set B. */
+int fr_orient_nearest(fr_ctx *ctx, int n, const float *d_dm, int ncls, const float *d_dmc,
+                      int nsym, const float *d_sym, int *d_label);
+int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const float *d_dmb,
+                     int nsym, const float *d_sym, float max_deg, unsigned *d_mask);
+
 #ifdef __cplusplus
 }
 #endif

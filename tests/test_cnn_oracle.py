@@ -1,0 +1,105 @@
+"""Pins the FP64 network oracle (oracle/cnn_oracle.py) by mathematics:
+central finite differences of its loss, the adjointness of its im2col/col2im
+pair, a closed-form loss, and BF16 rounding on hand-checked values."""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "oracle"))
+import cnn_oracle as CO  # noqa: E402
+
+
+def _case(seed=3, shape=(8, 3, 2, 5), B=3):
+    rng = np.random.default_rng(seed)
+    n = CO.layout(*shape)[-1]
+    p = rng.standard_normal(n) * 0.5
+    img = rng.standard_normal((B, shape[0], shape[0]))
+    lab = rng.integers(0, shape[3], B)
+    return p, img, lab
+
+
+def test_finite_differences():
+    shape = (8, 3, 2, 5)
+    p, img, lab = _case(shape=shape)
+    loss, g = CO.loss_grad(p, shape, img, lab)
+    rng = np.random.default_rng(11)
+    idx = rng.choice(len(p), 60, replace=False)
+    h = 1e-6
+    worst = 0.0
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (CO.loss_grad(pp, shape, img, lab)[0].mean() -
+              CO.loss_grad(pm, shape, img, lab)[0].mean()) / (2 * h)
+        worst = max(worst, abs(fd - g[i]) / max(1e-6, abs(g[i])))
+    assert worst < 1e-5, worst
+
+
+def test_planted_mistake_is_caught():
+    # a gradient with a dropped ReLU mask would fail the same check
+    shape = (8, 3, 2, 5)
+    p, img, lab = _case(shape=shape, seed=5)
+    _, g = CO.loss_grad(p, shape, img, lab)
+    offs = CO.layout(*shape)[0]
+    w0 = offs[0][0]
+    i = w0[0] + 4
+    h = 1e-6
+    pp, pm = p.copy(), p.copy()
+    pp[i] += h
+    pm[i] -= h
+    fd = (CO.loss_grad(pp, shape, img, lab)[0].mean() -
+          CO.loss_grad(pm, shape, img, lab)[0].mean()) / (2 * h)
+    assert abs(fd - g[i]) < 1e-6 * max(1.0, abs(g[i]))
+    assert abs(fd - 1.5 * g[i]) > 1e-4 * abs(g[i]) or abs(g[i]) < 1e-9
+
+
+def test_im2col_adjoint():
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal((2, 3, 8, 8))
+    y = rng.standard_normal((2, 27, 16))
+    lhs = np.sum(CO.im2col(x) * y)
+    rhs = np.sum(x * CO.col2im(y, 3, 8))
+    assert abs(lhs - rhs) < 1e-12 * max(1.0, abs(lhs))
+
+
+def test_im2col_matches_direct_convolution():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((1, 2, 6, 6))
+    w = rng.standard_normal((3, 2, 3, 3))
+    cols = CO.im2col(x)
+    out = np.einsum("ok,bkn->bon", w.reshape(3, 18), cols).reshape(1, 3, 3, 3)
+    ref = np.zeros((1, 3, 3, 3))
+    for co in range(3):
+        for oy in range(3):
+            for ox in range(3):
+                s = 0.0
+                for ci in range(2):
+                    for ky in range(3):
+                        for kx in range(3):
+                            yy, xx = 2 * oy + ky - 1, 2 * ox + kx - 1
+                            if 0 <= yy < 6 and 0 <= xx < 6:
+                                s += w[co, ci, ky, kx] * x[0, ci, yy, xx]
+                ref[0, co, oy, ox] = s
+    assert np.allclose(out, ref, atol=1e-12)
+
+
+def test_zero_network_loss_is_log_k():
+    shape = (8, 2, 2, 7)
+    n = CO.layout(*shape)[-1]
+    img = np.random.default_rng(0).standard_normal((4, 8, 8))
+    lab = np.array([0, 3, 6, 2])
+    loss, g = CO.loss_grad(np.zeros(n), shape, img, lab)
+    assert np.allclose(loss, np.log(7.0), atol=1e-15)
+    fb = CO.layout(*shape)[2]
+    want = np.full(7, 4 / 7.0)
+    for y in lab:
+        want[y] -= 1.0
+    assert np.allclose(g[fb[0]:fb[1]], want / 4, atol=1e-15)
+
+
+def test_bf16_round():
+    v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 0.0], np.float32)
+    want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5, 0.0], np.float32)
+    assert np.array_equal(CO.bf16_round(v), want)
