@@ -35,6 +35,9 @@ SIGNATURES = {
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
+    "fr_powell_minimize": ([ci, vp, vp, cf, ci, ci, vp, vp, vp, vp], ci),
+    "fr_refine_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp, ci, vp],
+                        ci),
     "fr_search_list_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, ci, vp, ci, ci, vp, vp,
                               vp], ci),
     "fr_cnn_param_count": ([ci, ci, ci, ci, C.POINTER(C.c_longlong)], ci),

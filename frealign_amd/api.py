@@ -68,7 +68,7 @@ class Context:
             pass
 
     # ------------------------------------------------------------ profiling
-    PROF = {"search_gemm": 0, "insert": 1, "cc3m": 2, "prep": 3}
+    PROF = {"search_gemm": 0, "insert": 1, "cc3m": 2, "prep": 3, "search_list": 4, "cnn": 5}
 
     def profile(self, on: bool = True):
         N.check(N.lib().fr_profile_enable(self.h, int(on)), "fr_profile_enable")
@@ -303,6 +303,39 @@ class Context:
             _p(cand), int(group), topk, _p(top_o), _p(top_r), _p(top_s)), "fr_search_list_batch")
         return top_o, top_r, top_s
 
+    # -------------------------------------------------------------- PREFINE
+    def refine(self, nsam: int, a3df, spec, ctff, pidx, amag, par, mask, maxr1: int, maxr2: int,
+               rbfact: float, fsct, ring, prior, maxit: int = 50):
+        """PREFINE for n items (fr_refine_batch).  par (n, 6) host float32:
+        PHI THETA PSI DSHX DSHY in; returns (refined par with PRESA in column 5,
+        evaluations per item)."""
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(spec, torch.complex64, "spec")
+        _dev(ctff, torch.complex64, "ctff")
+        _dev(fsct, torch.float32, "fsct")
+        jc = nsam // 2 + 1
+        npart = spec.numel() // (nsam * jc)
+        p = np.array(par, np.float32).reshape(-1, 6).copy()
+        n = p.shape[0]
+        pi = np.ascontiguousarray(pidx, np.int32)
+        am = np.ascontiguousarray(amag, np.float32)
+        pr = np.ascontiguousarray(prior, np.float32).reshape(n, 5)
+        if pi.size != n or am.size != n:
+            raise ValueError("one pidx and amag per item")
+        if n and (pi.max() >= npart or pi.min() < 0):
+            raise ValueError("pidx out of range")
+        if a3df.numel() != nsam * nsam * jc or ctff.numel() != 2 * spec.numel():
+            raise ValueError("a3df/ctff sizes")
+        m = np.ascontiguousarray(mask, np.int32)
+        r = np.asarray(ring, np.float32)
+        nev = np.zeros(n, np.int32)
+        self._sync_stream()
+        N.check(N.lib().fr_refine_batch(self.h, nsam, n, _p(a3df), _p(spec), _p(ctff), _p(pi),
+                                        _p(am), _p(p), _p(m), maxr1, maxr2, float(rbfact),
+                                        _p(fsct), _p(r), _p(pr), int(maxit), _p(nev)),
+                "fr_refine_batch")
+        return p, nev
+
     # ------------------------------------------------ proposal network
     def cnn_init(self, shape, seed: int, master: torch.Tensor, wq: torch.Tensor):
         _dev(master, torch.float32, "master")
@@ -453,6 +486,29 @@ class HalfMaps:
 
 
 # ---------------------------------------------------------------- geometry
+POWELL_FN = C.CFUNCTYPE(C.c_float, C.c_int, C.POINTER(C.c_float), C.c_void_p)
+
+
+def powell_minimize(fn, x, e, escale: float = 100.0, icon: int = 1, maxit: int = 50,
+                    user=None):
+    """VA04A (fr_powell_minimize) on a host objective.  fn: a POWELL_FN, or a
+    Python callable f(x: np.ndarray) -> float.  Returns (x, F, evaluations)."""
+    x = np.array(x, np.float32).copy()
+    e = np.ascontiguousarray(e, np.float32)
+    if not isinstance(fn, (POWELL_FN, C._CFuncPtr)):
+        py = fn
+
+        def tramp(n, px, _u):
+            return float(py(np.ctypeslib.as_array(px, shape=(n,)).copy()))
+        fn = POWELL_FN(tramp)
+    f = C.c_float(0)
+    nev = C.c_int(0)
+    N.check(N.lib().fr_powell_minimize(x.size, _p(x), _p(e), float(escale), int(icon), int(maxit),
+                                       C.cast(fn, C.c_void_p), user, C.byref(f), C.byref(nev)),
+            "fr_powell_minimize")
+    return x, np.float32(f.value), nev.value
+
+
 def cnn_param_count(S: int, C_: int, nconv: int, K: int) -> int:
     n = C.c_longlong(0)
     N.check(N.lib().fr_cnn_param_count(S, C_, nconv, K, C.byref(n)), "fr_cnn_param_count")

@@ -25,6 +25,7 @@ SOURCES = [
     "fr_stats.hip",
     "fr_search.hip",
     "fr_cnn.hip",
+    "fr_refine.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -17,6 +17,7 @@
 #include "fr_host.h"
 #include "fr_kernels.h"
 #include "fr_cnn.h"
+#include "fr_powell.h"
 #include "fr_search.h"
 
 struct fr_ctx : public fr::Ctx {
@@ -621,7 +622,10 @@ int fr_cnn_forward(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq,
   int err = 0;
   float* ws = (float*)c->slot(8, sizeof(float) * cnn_workspace(lo, nb, false), &err);
   if (!ws) return err;
-  return cnn_forward(*c, lo, d_wq, nb, d_img, ws, d_logits);
+  int pslot = c->prof_begin(kProfCnn);
+  int r = cnn_forward(*c, lo, d_wq, nb, d_img, ws, d_logits);
+  c->prof_end(pslot);
+  return r;
 }
 
 int fr_cnn_gradients(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
@@ -732,7 +736,7 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
   if (int r = launch_search_norm_list(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand,
                                       c2, nrm))
     return r;
-  int pslot = c->prof_begin(kProfSearchGemm);
+  int pslot = c->prof_begin(kProfSearchList);
   if (int r = launch_search_gemm(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand, ap,
                                  nrm, sum2, c->d_shx, c->d_shy, score, brow, nullptr))
     return r;
@@ -741,6 +745,72 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
                           d_top_score))
     return r;
   return launch_map_cand(*c, np, ncand, topk, d_cand, d_top_o);
+}
+
+// ------------------------------------------------------------------ PREFINE
+int fr_powell_minimize(int n, float* x, const float* e, float escale, int icon, int maxit,
+                       float (*fn)(int n, const float* x, void* user), void* user, float* f,
+                       int* nevals) {
+  FR_CHECK(x && e && fn && f, "null argument");
+  FR_CHECK(n >= 1 && n <= 6, "VA04A works on 1..6 parameters (got %d)", n);
+  FR_CHECK(maxit >= 1 && escale > 0.0f, "bad MAXIT or ESCALE");
+  fr::Powell p;
+  p.start(x, e, n, escale, icon, maxit);
+  while (p.run()) p.supply(fn(n, p.x(), user));
+  for (int i = 0; i < n; i++) x[i] = p.x()[i];
+  *f = p.value();
+  if (nevals) *nevals = p.evaluations();
+  return 0;
+}
+
+int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                    const float* d_ctff, const int* pidx, const float* amag, float* par,
+                    const int* mask, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+                    const float* ring, const float* prior, int maxit, int* nevals) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_spec && d_ctff && pidx && amag && par && mask && d_fsct && ring && prior,
+           "null argument");
+  FR_CHECK(n >= 0 && maxit >= 1, "bad sizes");
+  FR_CHECK(maxr1 >= 0 && maxr2 > maxr1, "bad resolution band %d..%d", maxr1, maxr2);
+  for (int i = 0; i < n; i++) {
+    FR_CHECK(pidx[i] >= 0, "negative particle index");
+    FR_CHECK(amag[i] > 0.0f, "AMAG must be positive (magnification refinement is not supported)");
+    FR_CHECK(prior[5 * i + 2] != 0.0f && prior[5 * i + 3] != 0.0f, "shift prior SX/SY must be non-zero");
+  }
+  if (n == 0) return 0;
+  fr::RefineArgs a;
+  a.n = n;
+  a.pidx = pidx;
+  a.amag = amag;
+  a.mask = mask;
+  a.prior = prior;
+  a.maxit = maxit;
+  auto cc3m = [&](int m, const int* hp, const float* hdm, const float* hsh, const float* ham,
+                  float* hscore) -> int {
+    int err = 0;
+    int* dp = (int*)c->slot(11, sizeof(int) * (size_t)m, &err);
+    if (!dp) return err;
+    float* ddm = (float*)c->slot(12, sizeof(float) * 9 * (size_t)m, &err);
+    if (!ddm) return err;
+    float* dsh = (float*)c->slot(13, sizeof(float) * 2 * (size_t)m, &err);
+    if (!dsh) return err;
+    float* dam = (float*)c->slot(14, sizeof(float) * (size_t)m, &err);
+    if (!dam) return err;
+    float* dsc = (float*)c->slot(15, sizeof(float) * (size_t)m, &err);
+    if (!dsc) return err;
+    FR_HIP(hipMemcpyAsync(dp, hp, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
+    FR_HIP(hipMemcpyAsync(ddm, hdm, sizeof(float) * 9 * m, hipMemcpyHostToDevice, c->stream));
+    FR_HIP(hipMemcpyAsync(dsh, hsh, sizeof(float) * 2 * m, hipMemcpyHostToDevice, c->stream));
+    FR_HIP(hipMemcpyAsync(dam, ham, sizeof(float) * m, hipMemcpyHostToDevice, c->stream));
+    if (int r = fr_cc3m_batch(c, nsam, m, d_a3df, d_spec, d_ctff, dp, ddm, dsh, dam, maxr1, maxr2,
+                              rbfact, d_fsct, ring, dsc))
+      return r;
+    FR_HIP(hipMemcpyAsync(hscore, dsc, sizeof(float) * m, hipMemcpyDeviceToHost, c->stream));
+    FR_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+  };
+  return fr::refine_batch(*c, a, par, nevals, cc3m);
 }
 
 }  // extern "C"

@@ -62,7 +62,15 @@ __host__ __device__ inline cf cscale(cf a, float s) { return cf{a.re * s, a.im *
 __host__ __device__ inline int wrapn(int j, int n) { return j < 0 ? j + n : j; }
 
 // timed kernel families (fr_profile_read)
-enum ProfName { kProfSearchGemm = 0, kProfInsert = 1, kProfCc3m = 2, kProfPrep = 3, kProfN = 4 };
+enum ProfName {
+  kProfSearchGemm = 0,
+  kProfInsert = 1,
+  kProfCc3m = 2,
+  kProfPrep = 3,
+  kProfSearchList = 4,
+  kProfCnn = 5,
+  kProfN = 6
+};
 
 // ---------------------------------------------------------------- FFT API (fr_fft.hip)
 // Batched 2D real FFTs in the reference's in-place layout: nimg images of

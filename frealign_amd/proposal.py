@@ -268,8 +268,13 @@ class LearnedProposal:
         s_all = np.full((npart, topk), -np.inf, np.float32)
         idx_c = np.nonzero(confident)[0]
         idx_f = np.nonzero(~confident)[0]
+        self.last_valid = 0
+        self.last_slots = 0
         if len(idx_c):
-            cand = torch.from_numpy(self.candidates(top_h[idx_c])).to(self.dev)
+            cand_h = self.candidates(top_h[idx_c])
+            self.last_valid = int((cand_h >= 0).sum())
+            self.last_slots = int(cand_h.size)
+            cand = torch.from_numpy(cand_h).to(self.dev)
             sel = torch.from_numpy(idx_c).to(self.dev)
             o, r, sc = self.ctx.search_list(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.bank,
                                             outc[sel].contiguous(), ctff[sel].contiguous(), cand,

@@ -38,7 +38,8 @@ int fr_set_stream(fr_ctx *ctx, void *hip_stream);
 int fr_synchronize(fr_ctx *ctx);
 
 /* Per-kernel timing with HIP events on the context's stream (off by default).
- * which: 0 search GEMM, 1 insertion, 2 CC3M, 3 particle preparation.
+ * which: 0 search GEMM (exhaustive), 1 insertion, 2 CC3M, 3 particle
+ * preparation, 4 search GEMM (candidate lists), 5 proposal-network forward.
  * fr_profile_read synchronises, returns the summed duration and count of the
  * recorded launches of that family, and forgets them.  No reference
  * counterpart (the reference only prints wall-clock stamps, lmain.f:238-246). */
@@ -174,6 +175,31 @@ int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, i
                     const void *d_bank, int dtype, int group, int np, const float *d_outc,
                     const float *d_ctff, int topk, int *d_top_o, int *d_top_row,
                     float *d_top_score);
+
+/* ---------------------------------------------------------------- PREFINE */
+
+/* Replaces VA04A (va04.f:8-369), Powell's minimiser, for a host objective:
+ * minimises fn over x[0..n-1] (n <= 6) with initial steps e, ESCALE, ICON and
+ * MAXIT as VA04A takes them (PREFINE: ESCALE 100, ICON 1, MAXIT 50).  FP32 in
+ * the reference's order: for the same objective values the iterates are the
+ * reference's.  Returns F in *f and the number of objective calls. */
+int fr_powell_minimize(int n, float *x, const float *e, float escale, int icon, int maxit,
+                       float (*fn)(int n, const float *x, void *user), void *user, float *f,
+                       int *nevals);
+
+/* Replaces PREFINE (prefine.f:8-162; IVFLAG=0, non-helical, AMAG > 0, IEWALD=0)
+ * for n items refined in lock-step: each round evaluates every active item's
+ * CALCFX (calcfx.f:152-191: -CC3M - FSH + range penalties) in one batched CC3M
+ * launch.  Item k uses particle pidx[k] of d_spec/d_ctff (as fr_cc3m_batch)
+ * and magnification amag[k].  par is host n x 6: PHI THETA PSI DSHX DSHY in
+ * (radians, phase units), refined values and PRESA (the final score) out.
+ * mask: the 5 refinement flags of PMASK.  prior is host n x 5 per item: XM-SHX,
+ * YM-SHY, SX, SY, SIG2 of FSH (fsh.f).  nevals (may be NULL) receives each
+ * item's number of CC3M evaluations. */
+int fr_refine_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
+                    const float *d_ctff, const int *pidx, const float *amag, float *par,
+                    const int *mask, int maxr1, int maxr2, float rbfact, const float *d_fsct,
+                    const float *ring, const float *prior, int maxit, int *nevals);
 
 /* Candidate-list form of fr_search_batch (no reference counterpart: the
  * learned proposal prunes PSEARCH's view loop, psearch.f:69-87, to the

@@ -145,6 +145,18 @@ void or_irfft2_d(int nsam, double *data);
                     const int mask[5], float rbfact, const R *ctff,             \
                     float *testpar, int ipmax, int32_t *iran, const float *sm);
 
+/* va04.f:8-369 (VA04A) with the objective as a callback: minimises fx over
+ * x[0..n-1] (n <= 6) with steps e, ESCALE, IPRINT (printing only), ICON and
+ * MAXIT as the reference takes them; returns F and the number of objective
+ * evaluations in *nevals.  FP32 throughout, as the Fortran. */
+typedef float (*or_objective)(int n, const float *x, void *user);
+float or_va04a(float *x, const float *e, int n, float escale, int iprint, int icon, int maxit,
+               or_objective fx, void *user, int *nevals);
+
+/* Closed-form test objectives shared with oracle/ref_va04_obj.f90 (same FP32
+ * operation order): 1 coupled quadratic, 2 Rosenbrock chain, 3 rational bump. */
+float or_va04_testfn(int n, const float *x, void *which);
+
 OR_DECLARE(or32_, float)
 OR_DECLARE(or64_, double)
 

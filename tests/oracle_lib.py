@@ -66,6 +66,10 @@ def _declare_oracle(L):
     L.or_getsymmat.restype = ci
     L.or_searchang.argtypes = [cf, cf, cf, C.c_char, ci, i32p, ci, C.c_void_p, pi_, pf]
     L.or_searchang.restype = ci
+    L.or_va04_testfn.argtypes = [ci, f32p, C.c_void_p]
+    L.or_va04_testfn.restype = cf
+    L.or_va04a.argtypes = [f32p, f32p, ci, cf, ci, ci, ci, C.c_void_p, C.c_void_p, pi_]
+    L.or_va04a.restype = cf
     L.or_rfft2_f.argtypes = [ci, f32p]
     L.or_irfft2_f.argtypes = [ci, f32p]
     L.or_rfft2_d.argtypes = [ci, f64p]
@@ -98,6 +102,8 @@ def _declare_oracle(L):
 def _declare_ref(L):
     L.ref_searchang.argtypes = [pf, pf, pf, pi_, pi_, C.c_char_p, pi_, f32p, pi_, i32p, pi_]
     L.ref_ctf.argtypes = [pf] * 8 + [pi_, pi_, pf, pf, f32p]
+    L.ref_va04a.argtypes = [f32p, f32p, ci, cf, ci, ci, ci, pf, pi_]
+    L.ref_va04a.restype = None
     L.ref_ainterpo3ds.argtypes = [pi_, c64p, pf, pf, pf, f32p]
     L.ref_random.argtypes = [pi_, pf]
     L.ref_maskcos.argtypes = [pi_, f32p, pf, pf, pf, pf, pf, pi_]

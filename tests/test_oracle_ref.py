@@ -250,3 +250,44 @@ def test_sheltest_fsc(ref_lib, nsam, istp):
     assert np.array_equal(npo[:nst], npr[:nst]) and np.array_equal(npto[:nst], nptr[:nst])
     np.testing.assert_allclose(dfo[:nst], dfr[:nst], rtol=1e-9, atol=1e-12)
     np.testing.assert_allclose(pso[:nst], psr[:nst], rtol=1e-5)
+
+
+# ----------------------------------------------------- VA04A (va04.f:8-369)
+def _or_va04a(L, x, e, fn, escale, icon, maxit):
+    x = np.array(x, np.float32)
+    which = C.c_int(fn)
+    nev = C.c_int(0)
+    f = L.or_va04a(x, np.array(e, np.float32), len(x), escale, 0, icon, maxit,
+                   C.cast(L.or_va04_testfn, C.c_void_p), C.cast(C.pointer(which), C.c_void_p),
+                   C.byref(nev))
+    return x, np.float32(f), nev.value
+
+
+@pytest.mark.parametrize("fn", [1, 2, 3])
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 6])
+@pytest.mark.parametrize("icon,maxit,escale", [(1, 50, 100.0), (2, 50, 100.0), (1, 3, 10.0)])
+def test_va04a_bit_exact(ref_lib, fn, n, icon, maxit, escale):
+    """The oracle's Powell minimiser reproduces the reference's iterates bit for
+    bit: final X, F and the number of objective evaluations."""
+    L = O.oracle()
+    r = np.random.default_rng(fn * 100 + n)
+    x0 = r.uniform(-0.6, 0.6, n).astype(np.float32)
+    e = (np.array([0.1, 0.1, 0.1, 0.01, 0.01, 0.001], np.float32)[:n] * r.uniform(0.5, 2.0, n)
+         ).astype(np.float32)
+    xo, fo, no = _or_va04a(L, x0, e, fn, escale, icon, maxit)
+    xr = x0.copy()
+    fr = C.c_float(0)
+    nr = C.c_int(0)
+    ref_lib.ref_va04a(xr, e.copy(), n, escale, icon, maxit, fn, C.byref(fr), C.byref(nr))
+    print(f"fn {fn} n {n}: F {fo:.6g} after {no} evaluations (reference {nr.value})")
+    assert no == nr.value
+    assert np.array_equal(xo.view(np.uint32), xr.view(np.uint32))
+    assert np.float32(fr.value).view(np.uint32) == fo.view(np.uint32)
+
+
+def test_va04a_minimises():
+    """Closed form: the coupled quadratic's minimiser solves a 2x2 linear system."""
+    L = O.oracle()
+    x, f, _ = _or_va04a(L, [0.5, -0.5], [0.1, 0.1], 1, 100.0, 2, 50)
+    # f = 0.7 a^2 + 1.4 b^2 + 0.4 a b with a = x1-0.1, b = x2-0.2: minimum at a = b = 0
+    assert abs(x[0] - 0.1) < 1e-3 and abs(x[1] - 0.2) < 1e-3 and f < 1e-6
