@@ -35,6 +35,7 @@ SIGNATURES = {
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
+    "fr_sigma_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, cf, vp, cf, vp, vp], ci),
     "fr_powell_minimize": ([ci, vp, vp, cf, ci, ci, vp, vp, vp, vp], ci),
     "fr_refine_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp, ci, vp],
                         ci),

@@ -303,6 +303,34 @@ class Context:
             _p(cand), int(group), topk, _p(top_o), _p(top_r), _p(top_s)), "fr_search_list_batch")
         return top_o, top_r, top_s
 
+    # ---------------------------------------------------------------- SIGMA
+    def sigma(self, nsam: int, a3df, datc, ctff, pidx, par, amag, maxr1: int, rclas: float,
+              dmask, sigp: float, signset=None) -> np.ndarray:
+        """SIGMA for n items (fr_sigma_batch): (n, 3) SIGMA2N, SIGMA2, LOGPC."""
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(datc, torch.complex64, "datc")
+        _dev(ctff, torch.complex64, "ctff")
+        jc = nsam // 2 + 1
+        npart = datc.numel() // (nsam * jc)
+        p = np.ascontiguousarray(par, np.float32).reshape(-1, 5)
+        n = p.shape[0]
+        pi = np.ascontiguousarray(pidx, np.int32)
+        am = np.ascontiguousarray(amag, np.float32)
+        if pi.size != n or am.size != n:
+            raise ValueError("one pidx and amag per item")
+        if n and (pi.max() >= npart or pi.min() < 0):
+            raise ValueError("pidx out of range")
+        if a3df.numel() != nsam * nsam * jc or ctff.numel() != 2 * datc.numel():
+            raise ValueError("a3df/ctff sizes")
+        dm = np.ascontiguousarray(dmask, np.float32)
+        ss = None if signset is None else np.ascontiguousarray(signset, np.float32)
+        out = np.zeros((n, 3), np.float32)
+        self._sync_stream()
+        N.check(N.lib().fr_sigma_batch(self.h, nsam, n, _p(a3df), _p(datc), _p(ctff), _p(pi),
+                                       _p(p), _p(am), int(maxr1), float(rclas), _p(dm),
+                                       float(sigp), _p(ss), _p(out)), "fr_sigma_batch")
+        return out
+
     # -------------------------------------------------------------- PREFINE
     def refine(self, nsam: int, a3df, spec, ctff, pidx, amag, par, mask, maxr1: int, maxr2: int,
                rbfact: float, fsct, ring, prior, maxit: int = 50):

@@ -26,6 +26,7 @@ SOURCES = [
     "fr_search.hip",
     "fr_cnn.hip",
     "fr_refine.cpp",
+    "fr_sigma.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

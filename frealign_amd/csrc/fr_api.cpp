@@ -813,4 +813,65 @@ int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float
   return fr::refine_batch(*c, a, par, nevals, cc3m);
 }
 
+// ------------------------------------------------------------------- SIGMA
+int fr_sigma_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_datc,
+                   const float* d_ctff, const int* pidx, const float* par, const float* amag,
+                   int maxr1, float rclas, const float* dmask, float sigp, const float* signset,
+                   float* out) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_datc && d_ctff && pidx && par && amag && dmask && out && n >= 0,
+           "null argument");
+  FR_CHECK(sigp != 0.0f, "SIGP must be non-zero");
+  FR_CHECK(maxr1 >= 0, "bad MAXR1");
+  if (n == 0) return 0;
+  const int jc = nsam / 2 + 1;
+  std::vector<SigmaItem> it(n);
+  for (int k = 0; k < n; k++) {
+    FR_CHECK(pidx[k] >= 0, "negative particle index");
+    FR_CHECK(amag[k] != 0.0f, "AMAG must be non-zero");
+    const float* p = par + 5 * (size_t)k;
+    SigmaItem& s = it[k];
+    s.pidx = pidx[k];
+    rotation(p[0], p[1], p[2], 0.0f, nullptr, s.dm);  // pextract.f:30-51
+    s.shx = p[3];
+    s.shy = p[4];
+    float dm2[9];
+    rotation(-p[2], -p[1], -p[0], amag[k], nullptr, dm2);  // sigma.f:110-111
+    {
+      const float d1 = dmask[0] - (float)jc, d2 = dmask[1] - (float)jc, d3 = dmask[2] - (float)jc;
+      volatile float t;  // keep the reference's FP32 rounding of each product
+      float xm = 0.f, ym = 0.f;
+      t = dm2[0] * d1; xm = t;
+      t = dm2[3] * d2; xm = xm + t;
+      t = dm2[6] * d3; xm = xm + t;
+      s.xm = xm + 1.0f;
+      t = dm2[1] * d1; ym = t;
+      t = dm2[4] * d2; ym = ym + t;
+      t = dm2[7] * d3; ym = ym + t;
+      s.ym = ym + 1.0f;
+    }
+    const float r = dmask[3] / amag[k];
+    s.ri2f = r * r;
+    s.signset = signset ? signset[k] : 0.0f;
+    const int mrclas = (int)((float)nsam * rclas * fabsf(amag[k]));
+    s.maxr12 = maxr1 * maxr1;
+    s.maxr22 = mrclas * mrclas;
+    s.pad_ = 0;
+  }
+  int err = 0;
+  SigmaItem* d_it = (SigmaItem*)c->slot(11, sizeof(SigmaItem) * (size_t)n, &err);
+  if (!d_it) return err;
+  const size_t img = (size_t)nsam * (nsam + 2);
+  float* buf = (float*)c->slot(0, sizeof(float) * img * 2 * (size_t)n, &err);
+  if (!buf) return err;
+  float* d_out = (float*)c->slot(15, sizeof(float) * 3 * (size_t)n, &err);
+  if (!d_out) return err;
+  FR_HIP(hipMemcpyAsync(d_it, it.data(), sizeof(SigmaItem) * n, hipMemcpyHostToDevice, c->stream));
+  if (int r = launch_sigma(*c, nsam, n, d_a3df, d_it, d_datc, d_ctff, sigp, buf, d_out)) return r;
+  FR_HIP(hipMemcpyAsync(out, d_out, sizeof(float) * 3 * n, hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 }  // extern "C"

@@ -55,6 +55,15 @@ int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
 int launch_presb(Ctx& c, int nsam, int n, const float* a3df, const float* spec, const float* ctff,
                  const int* pidx, const float* dm, const float* shift, const float* amag,
                  float ri2, float ri3, float rih, float halfw, float* work, float* rbin);
+// SIGMA (sigma.f:8-221) item: PEXTRACT DM, shift, mask centre/radius^2,
+// SIGNSET, particle and band.
+struct SigmaItem {
+  float dm[9];
+  float shx, shy, xm, ym, ri2f, signset;
+  int pidx, maxr12, maxr22, pad_;
+};
+int launch_sigma(Ctx& c, int nsam, int n, const float* a3df, const SigmaItem* items,
+                 const float* datc, const float* ctff, float sigp, float* buf, float* out);
 // CC3M with band-pruned DFT products on the matrix cores; one workgroup per item.
 bool cc3m_fused_ok(int nsam, int maxr2);
 size_t cc3m_fused_lds(int nsam);

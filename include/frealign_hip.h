@@ -176,6 +176,21 @@ int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, i
                     const float *d_ctff, int topk, int *d_top_o, int *d_top_row,
                     float *d_top_score);
 
+/* ------------------------------------------------------------------ SIGMA */
+
+/* Replaces SIGMA (sigma.f:8-221; non-helical, IEWALD=0, IPAD=1) for n items:
+ * item k compares particle pidx[k] (d_datc: DATC of fr_prep_batch, d_ctff)
+ * with the section at par[5k..5k+4] = PHI, THETA, PSI, SHX, SHY (radians,
+ * phase units) for magnification amag[k], in the band MAXR1^2 <= r^2 <
+ * (INT(NSAM*RCLAS*|AMAG|))^2 and the real-space disc DMASK(4)/AMAG around
+ * the projection of the 3D mask centre DMASK(1..3).  signset (may be NULL):
+ * the previous cycle's sigma per item (0: none).  out is host n x 3:
+ * SIGMA2N, SIGMA2, LOGPC.  pidx, par, amag, dmask, signset are host arrays. */
+int fr_sigma_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_datc,
+                   const float *d_ctff, const int *pidx, const float *par, const float *amag,
+                   int maxr1, float rclas, const float *dmask, float sigp, const float *signset,
+                   float *out);
+
 /* ---------------------------------------------------------------- PREFINE */
 
 /* Replaces VA04A (va04.f:8-369), Powell's minimiser, for a host objective:

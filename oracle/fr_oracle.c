@@ -956,3 +956,136 @@ double P(psearch)(int nsam, float amag, const R *spec, const float *a3df, float 
   free(cbuf);
   return ccmax;
 }
+
+/* ---------------------------------------------------------- phaseflip.f */
+/* SPEC(ID) = SPEC(ID) * CTF / |CTF| over I = 0..JC-1, J = -JC+1..JC-1 (the
+ * Nyquist row is reached from J = +-(JC-1) and so multiplied twice). */
+static void phaseflip_c(int nsam, R *spec, const R *ctff) {
+  int jc = nsam / 2 + 1;
+  size_t ic = (size_t)nsam * jc;
+  for (int i = 0; i <= jc - 1; i++)
+    for (int j = -jc + 1; j <= jc - 1; j++) {
+      size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
+      C c = cadd(cld(ctff, id), cconj(cld(ctff, id + ic)));
+#if OR_FP64
+      R amp = sqrt(c.re * c.re + c.im * c.im);
+#else
+      R amp = hypotf(c.re, c.im);
+#endif
+      if (amp != 0) {
+        C z = cmul(cld(spec, id), c);
+        z.re = z.re / amp;
+        z.im = z.im / amp;
+        cst(spec, id, z);
+      }
+    }
+}
+
+/* --------------------------------------------------------------- sigma.f */
+void P(sigma)(int nsam, float amag, const float *a3df, const R *datc, float phi, float theta,
+              float psi, float shx, float shy, const R *ctff, int maxr1, float rclas,
+              float signset, const float *dmask, float sigp, const float *sm, R *out) {
+  int jc = nsam / 2 + 1, nsamh = nsam / 2;
+  size_t ic = (size_t)nsam * jc;
+  int nsam2 = nsam * nsam;
+  size_t ib1 = (size_t)nsam * (nsam + 2);
+  R *rbuf = (R *)calloc(2 * ib1 + 8, sizeof(R));
+  R *part = rbuf + ib1;
+  float t = dmask[3] / amag;
+  float ri2f = t * t;
+  int mrclas = (int)((float)nsam * rclas * fabsf(amag));
+  int maxr12 = maxr1 * maxr1, maxr22 = mrclas * mrclas;
+  P(pextract)(nsam, rbuf, a3df, phi, theta, psi, ctff, sm);
+  for (int l = 0; l <= jc - 1; l++)
+    for (int m = 0; m <= nsam - 1; m++) {
+      int mm = m >= jc ? m - nsam : m;
+      int itemp = l * l + mm * mm;
+      size_t id = (size_t)l + (size_t)jc * m;
+      if (itemp >= maxr12 && itemp < maxr22) {
+#if OR_FP64
+        double phase = -(double)shx * l - (double)shy * mm;
+#else
+        float phase = -shx * (float)l - shy * (float)mm;
+#endif
+        C pshft = {RCOS(phase), RSIN(phase)};
+        cst(part, id, cmul(pshft, cld(datc, id)));
+      } else {
+        C zero = {0, 0};
+        cst(part, id, zero);
+        cst(rbuf, id, zero);
+      }
+    }
+  phaseflip_c(nsam, part, ctff);
+  phaseflip_c(nsam, rbuf, ctff);
+  IRFFT2(nsam, part);
+  IRFFT2(nsam, rbuf);
+  float dm[9];
+  or_dm(-psi, -theta, -phi, amag, sm, dm);
+  float xm = dm[0] * (dmask[0] - (float)jc) + dm[3] * (dmask[1] - (float)jc) +
+             dm[6] * (dmask[2] - (float)jc) + 1.0f;
+  float ym = dm[1] * (dmask[0] - (float)jc) + dm[4] * (dmask[1] - (float)jc) +
+             dm[7] * (dmask[2] - (float)jc) + 1.0f;
+  double sum1 = 0, sum2 = 0, sum5 = 0, cc = 0, ave1 = 0, ave2 = 0, ave5 = 0;
+  int n1 = 0;
+  for (int L = 1; L <= nsam; L++) {
+    float dx = (float)L - xm;
+    if (L >= nsamh) dx = (float)L - xm - (float)nsam;
+    float x2 = dx * dx;
+    for (int M = 1; M <= nsam; M++) {
+      float dy = (float)M - ym;
+      if (M >= nsamh) dy = (float)M - ym - (float)nsam;
+      float y2 = dy * dy;
+      size_t id = (size_t)(L - 1) + (size_t)(nsam + 2) * (M - 1);
+      R r1 = rbuf[id], r2 = part[id];
+      if (x2 + y2 <= ri2f) {
+        cc = cc + (double)(r1 * r2);
+        sum1 = sum1 + (double)(r1 * r1);
+        sum2 = sum2 + (double)(r2 * r2);
+        ave1 = ave1 + (double)r1;
+        ave2 = ave2 + (double)r2;
+        n1 = n1 + 1;
+      }
+      sum5 = sum5 + (double)(r2 * r2);
+      ave5 = ave5 + (double)r2;
+    }
+  }
+  cc = cc - ave1 * ave2 / n1;
+  sum1 = sum1 - ave1 * ave1 / n1;
+  sum2 = sum2 - ave2 * ave2 / n1;
+  sum5 = sum5 - ave5 * ave5 / nsam2;
+  double a = fabs(cc / sum1);
+  double s2 = fabs(sum2 - a * a * sum1) / n1;
+#if OR_FP64
+  double sigma2 = s2 / nsam2 / nsam2;
+  double s2old = sum5 / nsam2;
+  double sigma2old = s2old / nsam2 / nsam2;
+  if (signset != 0.0f) {
+    double rr = sqrt(sigma2old) / signset;
+    if (rr <= 3.0 && rr >= 0.33) {
+      s2old = (double)signset * signset * nsam2 * nsam2;
+      sigma2old = (double)signset * signset;
+    }
+  }
+  double sigma2n = s2old / a / sqrt(sum1 * sum2);
+  int n1w = (int)(n1 * sigma2old / ((double)sigp * sigp));
+  double logpc = -n1w * sigma2 / sigma2old / 2.0 - n1w * log(2.0 * 3.1415926535897 * sigma2old) / 2.0;
+#else
+  float sigma2 = (float)(s2 / nsam2 / nsam2);
+  float s2old = (float)(sum5 / nsam2);
+  float sigma2old = s2old / (float)nsam2 / (float)nsam2;
+  if (signset != 0.0f) {
+    if (sqrtf(sigma2old) / signset <= 3.0f && sqrtf(sigma2old) / signset >= 0.33f) {
+      s2old = signset * signset * (float)nsam2 * (float)nsam2;
+      sigma2old = signset * signset;
+    }
+  }
+  float sigma2n = (float)((double)s2old / a / sqrt(sum1 * sum2));
+  int n1w = (int)((float)n1 * sigma2old / (sigp * sigp));
+  float logpc = -((float)n1w * sigma2 / sigma2old / 2.0f) -
+                (float)n1w * logf(2.0f * PI_F * sigma2old) / 2.0f;
+#endif
+  out[0] = sigma2n;
+  out[1] = sigma2;
+  out[2] = logpc;
+  free(rbuf);
+}

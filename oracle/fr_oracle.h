@@ -137,6 +137,12 @@ void or_irfft2_d(int nsam, double *data);
                 float dangin, int maxr1, int maxr2, float phi, float theta,     \
                 float *psi, float *shx, float *shy, const int mask[5],          \
                 const R *ctff, int iquadmax, const float *sm);                  \
+  /* sigma.f:8-221 (non-helical, IEWALD=0): out = SIGMA2N, SIGMA2, LOGPC.     \
+   * datc: the particle's DATC; dmask: DMASK(1..4); rclas: RCLAS. */            \
+  void P##sigma(int nsam, float amag, const float *a3df, const R *datc,        \
+                float phi, float theta, float psi, float shx, float shy,        \
+                const R *ctff, int maxr1, float rclas, float signset,           \
+                const float *dmask, float sigp, const float *sm, R *out);       \
   /* psearch.f:8-121.  testpar: 6 x ipmax; returns CCMAX. */                   \
   double P##psearch(int nsam, float amag, const R *spec, const float *a3df,     \
                     float dangin, int maxr1, int maxr2, float *phi,             \

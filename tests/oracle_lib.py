@@ -86,6 +86,8 @@ def _declare_oracle(L):
         g("cc3m").argtypes = [ci, cf, cp, c64p, ci, ci, cf, cf, cf, cf, cf, cf, f32p, cp,
                               cf, cf, cf, cf, f32p]
         g("cc3m").restype = cd
+        g("sigma").argtypes = [ci, cf, c64p, cp, cf, cf, cf, cf, cf, cp, ci, cf, cf, f32p, cf,
+                               f32p, rp]
         g("presb").argtypes = [ci, cf, cp, c64p, cf, cf, cf, cf, cf, rp, cp, cf, cf, cf, cf, f32p]
         g("pinsert").argtypes = [ci, cf, cf, cp, cp, cp, rp, pd, cf, cf, cf, cf, cf, cf, cf, cf,
                                  ci, i32p, f32p, C.POINTER(C.c_int64), cf, cf, f32p, ci]

@@ -378,3 +378,45 @@ def test_storeshift_vs_oracle(ctx):
     bo, co = np.zeros(n, np.complex64), np.zeros(n, np.complex64)
     L.or32_storeshift(nsam, 0.37, 0.41, 14, a, s, d, v, bo, co)
     assert rel_l2(b.cpu().numpy(), bo) < 1e-6 and rel_l2(c.cpu().numpy(), co) < 1e-6
+
+
+# ------------------------------------------------------------------ SIGMA
+@pytest.mark.parametrize("nsam,signset", [(32, 0.0), (64, 0.0), (64, 1.0)])
+def test_sigma_vs_oracle(ctx, nsam, signset):
+    """fr_sigma_batch against the FP32 oracle (sigma.f restated): SIGMA2N and
+    SIGMA2 within 1e-3 relative (SIGMA2 carries the cancellation SUM2 -
+    A^2 SUM1 of FP32 images), LOGPC within 1e-4 relative; errors also printed
+    against the FP64 oracle."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    vol = half_volume(nsam, 53)
+    npart, nitem = 2, 5
+    datc = (rng.standard_normal((npart, nsam, jc)) + 1j * rng.standard_normal((npart, nsam, jc))
+            ).astype(np.complex64)
+    ctff = (0.5 + 0.2 * rng.standard_normal((npart, 2, nsam, jc)) +
+            0.05j * rng.standard_normal((npart, 2, nsam, jc))).astype(np.complex64)
+    pidx = rng.integers(0, npart, nitem).astype(np.int32)
+    par = np.zeros((nitem, 5), np.float32)
+    par[:, 0:3] = rng.uniform(-np.pi, np.pi, (nitem, 3))
+    par[:, 3:5] = rng.uniform(-0.2, 0.2, (nitem, 2))
+    amag = (1.0 + 0.02 * rng.uniform(-1, 1, nitem)).astype(np.float32)
+    dmask = np.array([nsam / 2 + 3, nsam / 2 - 2, nsam / 2 + 1, 0.35 * nsam], np.float32)
+    rclas, maxr1, sigp = 0.3, 1, 0.8
+    ss = np.full(nitem, signset, np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
+    got = ctx.sigma(nsam, t(vol), t(datc), t(ctff), pidx, par, amag, maxr1, rclas, dmask, sigp,
+                    ss if signset else None)
+    for k in range(nitem):
+        p = pidx[k]
+        res = {}
+        for pre, ct, rt in (("or32_", np.complex64, np.float32), ("or64_", np.complex128, np.float64)):
+            out = np.zeros(3, rt)
+            getattr(L, pre + "sigma")(nsam, float(amag[k]), vol.reshape(-1),
+                                      datc[p].reshape(-1).astype(ct), *[float(x) for x in par[k]],
+                                      ctff[p].reshape(-1).astype(ct), maxr1, rclas, float(ss[k]),
+                                      dmask, sigp, O.IDENTITY, out)
+            res[pre] = out
+        r32 = np.abs(got[k] - res["or32_"]) / np.abs(res["or32_"])
+        r64 = np.abs(got[k] - res["or64_"]) / np.abs(res["or64_"])
+        print(f"nsam {nsam} item {k}: gpu {got[k]} or32 {res['or32_']} rel32 {r32} rel64 {r64}")
+        assert r32[0] < 1e-3 and r32[1] < 1e-3 and r32[2] < 1e-4

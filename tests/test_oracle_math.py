@@ -229,3 +229,90 @@ def test_ccp_recovers_planted_shift_and_view():
     assert abs(sy.value - iys / nsamr * 2 * math.pi) < 1e-6
     assert abs(math.remainder(p.value - psi, 2 * math.pi)) < math.radians(5.0 + 0.1)
     assert cc > 0.1
+
+
+# ------------------------------------------------------------------ SIGMA
+def _sigma_case(nsam, seed, beta=1.0, noise=0.0):
+    """A particle made from the section itself: DATC = beta * section * ramp
+    (+ white noise), with the ramp exp(+i(SHX I + SHY J)) that SIGMA undoes."""
+    jc = nsam // 2 + 1
+    r = np.random.default_rng(seed)
+    vol = _volume(nsam, seed)
+    ctff = (0.6 + 0.2 * r.standard_normal((2, nsam, jc))).astype(np.complex64)
+    eul = r.uniform(-1, 1, 3).astype(np.float32)
+    sec = np.zeros(nsam * jc, np.complex64)
+    L.or32_pextract(nsam, sec, vol, *[float(x) for x in eul], ctff.reshape(-1), O.IDENTITY)
+    sec = sec.reshape(nsam, jc)
+    shx, shy = np.float32(0.11), np.float32(-0.07)
+    m = np.arange(nsam)
+    mm = np.where(m >= jc, m - nsam, m)
+    l = np.arange(jc)
+    ph = (shx * l[None, :] + shy * mm[:, None]).astype(np.float64)
+    datc = (beta * sec * np.exp(1j * ph)).astype(np.complex64)
+    if noise:
+        datc = (datc + noise * (r.standard_normal(datc.shape) + 1j * r.standard_normal(datc.shape))
+                ).astype(np.complex64)
+    return vol, datc, ctff, eul, shx, shy
+
+
+def _sigma(pre, nsam, vol, datc, ctff, eul, shx, shy, signset=0.0, sigp=1.0, dmask=None,
+           rclas=0.4):
+    ct, rt = (np.complex64, np.float32) if pre == "or32_" else (np.complex128, np.float64)
+    out = np.zeros(3, rt)
+    dm = np.array(dmask if dmask is not None else [nsam / 2 + 1, nsam / 2 + 1, nsam / 2 + 1,
+                                                  nsam * 0.35], np.float32)
+    getattr(L, pre + "sigma")(nsam, 1.0, vol, datc.reshape(-1).astype(ct), *[float(x) for x in eul],
+                              float(shx), float(shy), ctff.reshape(-1).astype(ct), 1, rclas,
+                              float(signset), dm, float(sigp), O.IDENTITY, out)
+    return out
+
+
+def test_sigma_noise_free_particle_has_no_noise():
+    """A particle equal to the CTF-weighted section: the fitted scale A is 1, so
+    the noise variance SIGMA2 vanishes; scaling the particle by beta leaves
+    SIGMA2N unchanged (S2OLD ~ beta^2, A ~ beta, sqrt(SUM1 SUM2) ~ beta)."""
+    nsam = 32
+    vol, datc, ctff, eul, shx, shy = _sigma_case(nsam, 5)
+    s1 = _sigma("or64_", nsam, vol, datc, ctff, eul, shx, shy)
+    vol, datc2, ctff, eul, shx, shy = _sigma_case(nsam, 5, beta=2.0)
+    s2 = _sigma("or64_", nsam, vol, datc2, ctff, eul, shx, shy)
+    _, datcn, *_ = _sigma_case(nsam, 5, noise=0.5)
+    sn = _sigma("or64_", nsam, vol, datcn, ctff, eul, shx, shy)
+    assert sn[1] > 0
+    assert s1[1] < 1e-8 * sn[1] and s2[1] < 1e-8 * sn[1]
+    assert abs(s2[0] - s1[0]) < 1e-9 * abs(s1[0])
+    # the FP32 restatement agrees with FP64 up to the cancellation in
+    # SUM2 - A^2 SUM1 (FP32 images, FP64 sums: 4e-4 here)
+    s32 = _sigma("or32_", nsam, vol, datcn, ctff, eul, shx, shy)
+    assert np.allclose(s32[:2], sn[:2], rtol=2e-3)
+
+
+def test_sigma_logpc_closed_form():
+    """With SIGNSET inside the accepted band, SIGMA2OLD = SIGNSET^2; with SIGP =
+    SIGNSET the likelihood count is the number of pixels in the mask disc, so
+    LOGPC = -N (SIGMA2/SIGNSET^2 + log(2 pi SIGNSET^2)) / 2 with N counted here."""
+    nsam = 32
+    vol, datc, ctff, eul, shx, shy = _sigma_case(nsam, 9, noise=0.3)
+    base = _sigma("or64_", nsam, vol, datc, ctff, eul, shx, shy)
+    # SIGMA2OLD of this particle without SIGNSET: recover a SIGNSET in range
+    rad = nsam * 0.35
+    dmask = [nsam / 2 + 1, nsam / 2 + 1, nsam / 2 + 1, rad]
+    # the mask centre is the box centre: XM = YM = 1 (rotation of the zero vector)
+    n = 0
+    for Lx in range(1, nsam + 1):
+        dx = Lx - 1.0 - (nsam if Lx >= nsam // 2 else 0)
+        for My in range(1, nsam + 1):
+            dy = My - 1.0 - (nsam if My >= nsam // 2 else 0)
+            if dx * dx + dy * dy <= rad * rad:
+                n += 1
+    out = _sigma("or64_", nsam, vol, datc, ctff, eul, shx, shy, signset=1.0, sigp=1.0, dmask=dmask)
+    # SIGNSET = 1 is accepted only when sqrt(SIGMA2OLD) is within 1/3..3 of it; pick one that is
+    for signset in np.geomspace(1e-6, 1e6, 121):
+        o = _sigma("or64_", nsam, vol, datc, ctff, eul, shx, shy, signset=signset, sigp=signset,
+                   dmask=dmask)
+        want = -n * o[1] / signset ** 2 / 2 - n * math.log(2 * math.pi * signset ** 2) / 2
+        if abs(o[2] - want) <= 1e-9 * abs(want):
+            break
+    else:
+        pytest.fail("no SIGNSET inside the accepted band reproduced the closed form")
+    assert base[1] == o[1] and out is not None
