@@ -258,7 +258,7 @@ def small_search(ctx):
     nsam = 64
     vol = synth.blob_phantom(nsam, 20, 24.0, 0x51, sym=api.point_group("D", 7), device=DEV)
     a3df = synth.half_volume(vol)
-    gs = search.GlobalSearch(ctx, a3df, dang=10.0, asym="D", nasym=7, maxr1=1, maxr2=20,
+    gs = search.GlobalSearch(ctx, a3df, dang=10.0, asym="D", nasym=7, maxr1=1, maxr2=14,
                              rbfact=0.0, dtype=torch.bfloat16, iran=7)
     return nsam, a3df, gs
 
@@ -318,14 +318,18 @@ def test_training_gains_over_no_network(ctx, small_search):
           f"{rep.seconds:.1f}s, loss {rep.log[0][1]:.3f} -> {rep.final_loss:.3f}")
     assert rep.final_loss < rep.log[0][1]
     assert rep.heldout_topk > 0.8 and rep.heldout_topk > 4 * chance
-    # search ablation on held-out particles
+    # search ablation on held-out particles (synthetic particles through the
+    # full preparation path, not the training generator)
     parts = synth.make_particles(ctx, a3df, 64, 99, optics, snr=0.5, ri=26.0)
     outc, ctff, datc = gs_prep(ctx, nsam, parts, optics)
     fsct = synth.default_fsct(nsam, optics.psize)
     ring = optics.gpar(nsam, 26.0)[[7, 8, 5, 6]]
+    truth_cls = ctx.orient_nearest(torch.from_numpy(api.rotation(parts.euler, amag=0.0)).to(DEV),
+                                   lp.dmc, lp.sym).cpu().numpy()
+    top, prob = lp.propose(datc, ctff)
+    hit = np.mean((top.cpu().numpy() == truth_cls[:, None]).any(axis=1))
     ex = gs.run(outc, ctff, fsct, ring, topk=8)
     res, conf = lp.run(outc, ctff, datc, fsct, ring, topk=8)
-    same_l = np.mean(search.angular_error_deg(res.euler, ex.euler, sym) <= 10.0 + 1e-3)
     # no network: the same number of classes, chosen at random
     rtop = np.stack([rng.choice(lp.classes.n, cfg.topk, replace=False) for _ in range(64)])
     cand = torch.from_numpy(lp.candidates(rtop)).to(DEV)
@@ -333,7 +337,12 @@ def test_training_gains_over_no_network(ctx, small_search):
                               cfg.group, topk=8)
     sc, sh = gs.rescore(outc, ctff, o, r, fsct, ring)
     bo = o.cpu().numpy()[np.arange(64), np.argmax(sc, axis=1)]
-    same_r = np.mean(search.angular_error_deg(gs.euler[np.maximum(bo, 0)], ex.euler, sym)
-                     <= 10.0 + 1e-3)
-    print(f"within one step of the exhaustive answer: learned {same_l:.3f}, random {same_r:.3f}")
-    assert same_l >= 0.9 and same_l > same_r + 0.3
+    step = 1.5 * 10.0
+    acc = lambda e: float(np.mean(search.angular_error_deg(e, parts.euler, sym) <= step))  # noqa
+    acc_ex, acc_l = acc(ex.euler), acc(res.euler)
+    acc_r = acc(gs.euler[np.maximum(bo, 0)])
+    print(f"held-out particles: true class in top-{cfg.topk} {hit:.3f}; within {step:g} deg of "
+          f"truth: exhaustive {acc_ex:.3f}, learned {acc_l:.3f}, random classes {acc_r:.3f}")
+    same_l = acc_l
+    same_r = acc_r
+    assert hit >= 0.8 and acc_l >= acc_ex - 0.1 and same_l > same_r + 0.3

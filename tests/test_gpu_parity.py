@@ -381,7 +381,7 @@ def test_storeshift_vs_oracle(ctx):
 
 
 # ------------------------------------------------------------------ SIGMA
-@pytest.mark.parametrize("nsam,signset", [(32, 0.0), (64, 0.0), (64, 1.0)])
+@pytest.mark.parametrize("nsam,signset", [(32, 0.0), (64, 0.0), (64, 0.01)])
 def test_sigma_vs_oracle(ctx, nsam, signset):
     """fr_sigma_batch against the FP32 oracle (sigma.f restated): SIGMA2N and
     SIGMA2 within 1e-3 relative (SIGMA2 carries the cancellation SUM2 -
@@ -401,7 +401,9 @@ def test_sigma_vs_oracle(ctx, nsam, signset):
     par[:, 3:5] = rng.uniform(-0.2, 0.2, (nitem, 2))
     amag = (1.0 + 0.02 * rng.uniform(-1, 1, nitem)).astype(np.float32)
     dmask = np.array([nsam / 2 + 3, nsam / 2 - 2, nsam / 2 + 1, 0.35 * nsam], np.float32)
-    rclas, maxr1, sigp = 0.3, 1, 0.8
+    # SIGP near the noise sigma so that the likelihood count N1*SIGMA2OLD/SIGP^2
+    # is of the order of the disc (SIGNSET 0.01 lands in the accepted band)
+    rclas, maxr1, sigp = 0.3, 1, 0.01
     ss = np.full(nitem, signset, np.float32)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(DEV)  # noqa: E731
     got = ctx.sigma(nsam, t(vol), t(datc), t(ctff), pidx, par, amag, maxr1, rclas, dmask, sigp,
