@@ -36,6 +36,15 @@ SIGNATURES = {
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
     "fr_sigma_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, cf, vp, cf, vp, vp], ci),
+    "fr_plan_create": ([vp, ci, ci, vp, vp, vp, ci, ci, ci, cf, vp, vp, ci, vp, vp, cf, cf, cf,
+                        vp, vp, vp, vp, vp, vp, C.POINTER(vp)], ci),
+    "fr_plan_images": ([vp], vp),
+    "fr_plan_load": ([vp, vp], ci),
+    "fr_plan_run": ([vp, vp, vp, vp, vp], ci),
+    "fr_plan_destroy": ([vp], ci),
+    "fr_set_cc3m_variant": ([vp, ci, ci, ci], ci),
+    "fr_autotune_cc3m": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp,
+                          C.c_char_p, ci], ci),
     "fr_powell_minimize": ([ci, vp, vp, cf, ci, ci, vp, vp, vp, vp], ci),
     "fr_refine_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp, ci, vp],
                         ci),

@@ -68,7 +68,7 @@ class Context:
             pass
 
     # ------------------------------------------------------------ profiling
-    PROF = {"search_gemm": 0, "insert": 1, "cc3m": 2, "prep": 3, "search_list": 4, "cnn": 5}
+    PROF = {"search_gemm": 0, "insert": 1, "cc3m": 2, "prep": 3, "search_list": 4, "cnn": 5, "plan": 6}
 
     def profile(self, on: bool = True):
         N.check(N.lib().fr_profile_enable(self.h, int(on)), "fr_profile_enable")
@@ -331,6 +331,25 @@ class Context:
                                        float(sigp), _p(ss), _p(out)), "fr_sigma_batch")
         return out
 
+    def set_cc3m_variant(self, nsam: int, fused: bool, workgroups: int = 0):
+        N.check(N.lib().fr_set_cc3m_variant(self.h, nsam, int(bool(fused)), int(workgroups)),
+                "fr_set_cc3m_variant")
+
+    def autotune_cc3m(self, nsam: int, a3df, spec, ctff, pidx, dm, shift, amag, maxr1: int,
+                      maxr2: int, rbfact: float, fsct, ring) -> dict:
+        """fr_autotune_cc3m on these items; returns the timings and the choice."""
+        import json
+        n = pidx.numel()
+        out = torch.empty(n, dtype=torch.float32, device=a3df.device)
+        buf = C.create_string_buffer(4096)
+        r = np.asarray(ring, dtype=np.float32)
+        self._sync_stream()
+        N.check(N.lib().fr_autotune_cc3m(self.h, nsam, n, _p(a3df), _p(spec), _p(ctff), _p(pidx),
+                                         _p(dm), _p(shift), _p(amag), maxr1, maxr2, float(rbfact),
+                                         _p(fsct), _p(r), _p(out), buf, 4096),
+                "fr_autotune_cc3m")
+        return json.loads(buf.value.decode())
+
     # -------------------------------------------------------------- PREFINE
     def refine(self, nsam: int, a3df, spec, ctff, pidx, amag, par, mask, maxr1: int, maxr2: int,
                rbfact: float, fsct, ring, prior, maxit: int = 50):
@@ -511,6 +530,60 @@ class HalfMaps:
         ctf2 = tuple(torch.zeros(shp, dtype=torch.float32, device=device) for _ in range(2))
         return HalfMaps(nsam, num, ctf2, torch.zeros(2, dtype=torch.int64, device=device),
                         torch.zeros(2, dtype=torch.float64, device=device))
+
+
+class Plan:
+    """A graph-captured score-and-insert pipeline for batches of one box size
+    (fr_plan_*): CTFAPPLY -> CC3M at each particle's parameters -> insertion
+    into `vols`."""
+
+    def __init__(self, ctx: "Context", nsam: int, nb: int, a3df, pssnr, gpar, maxr1: int,
+                 maxr2: int, rbfact: float, fsct, ring, sym, pbc: float, boff: float,
+                 psize: float, vols: "HalfMaps", ictf: int = 1):
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(pssnr, torch.float32, "pssnr")
+        _dev(fsct, torch.float32, "fsct")
+        if vols.nsam != nsam or a3df.numel() != nsam * nsam * (nsam // 2 + 1):
+            raise ValueError("a3df and half-maps must match NSAM")
+        nsym, symop, jsym, _ = sym
+        self.ctx, self.nsam, self.nb, self.vols = ctx, nsam, nb, vols
+        self._keep = (a3df, pssnr, fsct)
+        g = np.ascontiguousarray(gpar, np.float32)
+        r = np.ascontiguousarray(ring, np.float32)
+        h = C.c_void_p()
+        ctx._sync_stream()
+        N.check(N.lib().fr_plan_create(ctx.h, nsam, nb, _p(a3df), _p(pssnr), _p(g), ictf, maxr1,
+                                       maxr2, float(rbfact), _p(fsct), _p(r), nsym, _p(jsym),
+                                       _p(symop), float(pbc), float(boff), float(psize),
+                                       _p(vols.num[0]), _p(vols.ctf2[0]), _p(vols.num[1]),
+                                       _p(vols.ctf2[1]), _p(vols.nksum), _p(vols.ttd),
+                                       C.byref(h)), "fr_plan_create")
+        self.h = h
+
+    def run(self, images: torch.Tensor, ppar: np.ndarray, recb: np.ndarray,
+            half: np.ndarray) -> np.ndarray:
+        _dev(images, torch.float32, "images")
+        if images.numel() != self.nb * self.nsam * (self.nsam + 2):
+            raise ValueError("images must be nb x NSAM x (NSAM+2)")
+        pp = np.ascontiguousarray(ppar, np.float32).reshape(self.nb, 8)
+        rb = np.ascontiguousarray(recb, np.float32).reshape(self.nb, 10)
+        hf = np.ascontiguousarray(half, np.int32).reshape(self.nb)
+        score = np.zeros(self.nb, np.float32)
+        self.ctx._sync_stream()
+        N.check(N.lib().fr_plan_load(self.h, _p(images)), "fr_plan_load")
+        N.check(N.lib().fr_plan_run(self.h, _p(pp), _p(rb), _p(hf), _p(score)), "fr_plan_run")
+        return score
+
+    def close(self):
+        if self.h:
+            N.lib().fr_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- geometry

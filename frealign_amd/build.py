@@ -27,6 +27,7 @@ SOURCES = [
     "fr_cnn.hip",
     "fr_refine.cpp",
     "fr_sigma.hip",
+    "fr_plan.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

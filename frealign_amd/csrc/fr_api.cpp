@@ -19,22 +19,7 @@
 #include "fr_cnn.h"
 #include "fr_powell.h"
 #include "fr_search.h"
-
-struct fr_ctx : public fr::Ctx {
-  std::vector<fr::InsertTask> host_tasks;
-  void* slots[16] = {};
-  size_t slot_bytes[16] = {};
-  void* slot(int k, size_t bytes, int* err);
-  // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
-  int skey[3] = {0, 0, 0};
-  int kc = 0, kpad = 0;
-  int* d_kidx = nullptr;
-  float* d_shx = nullptr;
-  float* d_shy = nullptr;
-  // DFT table of the fused CC3M, cached per NSAM
-  int dft_nsam = 0;
-  float* d_dft = nullptr;
-};
+#include "fr_ctx.h"
 
 namespace fr {
 
@@ -128,6 +113,27 @@ void* fr_ctx::slot(int k, size_t bytes, int* err) {
   }
   slot_bytes[k] = bytes;
   return slots[k];
+}
+
+int fr_ctx::ensure_dft(int nsam) {
+  if (dft_nsam == nsam) return 0;
+  std::vector<float> tab;
+  fr::cc3m_fused_table(nsam, tab);
+  if (d_dft) {
+    FR_HIP(hipStreamSynchronize(stream));
+    FR_HIP(hipFree(d_dft));
+    d_dft = nullptr;
+    dft_nsam = 0;
+  }
+  FR_HIP(hipMalloc(&d_dft, sizeof(float) * tab.size()));
+  FR_HIP(hipMemcpy(d_dft, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
+  dft_nsam = nsam;
+  return 0;
+}
+
+fr_ctx::Tune fr_ctx::tune_for(int nsam) const {
+  auto it = tune.find(nsam);
+  return it == tune.end() ? Tune{} : it->second;
 }
 
 using namespace fr;
@@ -290,26 +296,15 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   FR_CHECK(maxr1 >= 0 && maxr2 > maxr1, "bad resolution band %d..%d", maxr1, maxr2);
   if (n == 0) return 0;
   Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
-  if (fr::cc3m_fused_ok(nsam, maxr2)) {
-    if (c->dft_nsam != nsam) {
-      std::vector<float> tab;
-      fr::cc3m_fused_table(nsam, tab);
-      if (c->d_dft) {
-        FR_HIP(hipStreamSynchronize(c->stream));
-        FR_HIP(hipFree(c->d_dft));
-        c->d_dft = nullptr;
-        c->dft_nsam = 0;
-      }
-      FR_HIP(hipMalloc(&c->d_dft, sizeof(float) * tab.size()));
-      FR_HIP(hipMemcpy(c->d_dft, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
-      c->dft_nsam = nsam;
-    }
+  const fr_ctx::Tune tn = c->tune_for(nsam);
+  if (fr::cc3m_fused_ok(nsam, maxr2) && tn.cc3m_fused) {
+    if (int r = c->ensure_dft(nsam)) return r;
     int pslot = c->prof_begin(kProfCc3m);
     for (int o = 0; o < n; o += 65535) {
       a.n = n - o < 65535 ? n - o : 65535;
       if (int r = launch_cc3m_fused(*c, a, d_a3df, d_spec, d_ctff, d_pidx + o,
                                     d_dm + 9 * (size_t)o, d_shift + 2 * (size_t)o, d_amag + o,
-                                    d_fsct, c->d_dft, d_score + o))
+                                    d_fsct, c->d_dft, d_score + o, tn.cc3m_blocks))
         return r;
     }
     c->prof_end(pslot);

@@ -69,7 +69,8 @@ enum ProfName {
   kProfPrep = 3,
   kProfSearchList = 4,
   kProfCnn = 5,
-  kProfN = 6
+  kProfPlan = 6,
+  kProfN = 7
 };
 
 // ---------------------------------------------------------------- FFT API (fr_fft.hip)

@@ -231,7 +231,8 @@ __device__ inline void tile_r2(int v0, int nsamh, int nsam, int& lo, int& hi) {
 }
 
 __global__ void __launch_bounds__(kFT)
-    k_cc3m_fused(int nsam, int maxr12, int maxr22, float rbfact, const float2* __restrict__ a3df,
+    k_cc3m_fused(int nsam, int nitem, int maxr12, int maxr22, float rbfact,
+                 const float2* __restrict__ a3df,
                  const float* __restrict__ dm, const int* __restrict__ pidx,
                  const float2* __restrict__ ctff, const float2* __restrict__ spec,
                  const float* __restrict__ shift, const float* __restrict__ amag_arr,
@@ -245,20 +246,23 @@ __global__ void __launch_bounds__(kFT)
   __shared__ float red_s[kFT / 64];
   __shared__ int red_c[kFT / 64];
 
-  const int it = blockIdx.x, tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int la = lane & 15, lg = lane >> 4;
   const int jc = nsam / 2 + 1, nsamh = nsam / 2;
   const size_t ic = (size_t)nsam * jc;
-  const int p = pidx[it];
-  const float amagp = amag_arr[it];
-  const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
 
-  // ---- stage T, clear the shell bins, fill X (annulus x CTF / NSAM^2) into Ys
+  // ---- stage T once; the workgroup then takes items blockIdx.x, +gridDim.x, ...
   for (int e = tid; e < nsam * (kFK / 4); e += kFT) {
     int n = e / (kFK / 4), q = e % (kFK / 4);
     *reinterpret_cast<float4*>(Ts + n * kFP + 4 * q) =
         *reinterpret_cast<const float4*>(tab + (size_t)n * kFK + 4 * q);
   }
+  for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
+  const int p = pidx[it];
+  const float amagp = amag_arr[it];
+  const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
+
+  // ---- clear the shell bins, fill X (annulus x CTF / NSAM^2) into Ys
   const int nb = nsam / 2 + 2;
   for (int k = tid; k < 4 * nb; k += kFT) bins[k] = 0.0f;
   float2* Xs = reinterpret_cast<float2*>(Ys);  // [32][63]: I, J + 31
@@ -472,6 +476,8 @@ __global__ void __launch_bounds__(kFT)
     if (s1 * s2 != 0.0) s3 = s3 / sqrt(s1 * s2);
     score[it] = (float)s3;
   }
+  __syncthreads();  // bins and LDS buffers are reused by the next item
+  }
 }
 
 }  // namespace
@@ -498,13 +504,16 @@ void cc3m_fused_table(int nsam, std::vector<float>& tab) {
 
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
-                      const float* amag, const float* fsct, const float* tab, float* score) {
+                      const float* amag, const float* fsct, const float* tab, float* score,
+                      int blocks) {
   if (a.n <= 0) return 0;
   FR_CHECK(cc3m_fused_ok(a.nsam, a.maxr2), "fused CC3M needs NSAM %% 32 == 0, NSAM <= 256, MAXR2 <= 32");
   const size_t lds = cc3m_fused_lds(a.nsam);
   FR_HIP(hipFuncSetAttribute((const void*)k_cc3m_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
-  hipLaunchKernelGGL(k_cc3m_fused, dim3((unsigned)a.n), dim3(kFT), lds, c.stream, a.nsam,
+  // persistent grid: `blocks` workgroups (0: one per item), each looping over items
+  const int grid = (blocks > 0 && blocks < a.n) ? blocks : a.n;
+  hipLaunchKernelGGL(k_cc3m_fused, dim3((unsigned)grid), dim3(kFT), lds, c.stream, a.nsam, a.n,
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
                      reinterpret_cast<const float2*>(a3df), dm, pidx,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),

@@ -70,6 +70,7 @@ size_t cc3m_fused_lds(int nsam);
 void cc3m_fused_table(int nsam, std::vector<float>& tab);
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
-                      const float* amag, const float* fsct, const float* tab, float* score);
+                      const float* amag, const float* fsct, const float* tab, float* score,
+                      int blocks = 0);
 
 }  // namespace fr

@@ -39,7 +39,8 @@ int fr_synchronize(fr_ctx *ctx);
 
 /* Per-kernel timing with HIP events on the context's stream (off by default).
  * which: 0 search GEMM (exhaustive), 1 insertion, 2 CC3M, 3 particle
- * preparation, 4 search GEMM (candidate lists), 5 proposal-network forward.
+ * preparation, 4 search GEMM (candidate lists), 5 proposal-network forward,
+ * 6 a graph-captured batch (fr_plan_run).
  * fr_profile_read synchronises, returns the summed duration and count of the
  * recorded launches of that family, and forgets them.  No reference
  * counterpart (the reference only prints wall-clock stamps, lmain.f:238-246). */
@@ -287,6 +288,49 @@ int fr_orient_nearest(fr_ctx *ctx, int n, const float *d_dm, int ncls, const flo
                       int nsym, const float *d_sym, int *d_label);
 int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const float *d_dmb,
                      int nsym, const float *d_sym, float max_deg, unsigned *d_mask);
+
+/* ------------------------------------------- batches of uneven box sizes
+ * No counterpart in the reference (its particles go one at a time through
+ * LMAIN, lmain.f:509-885).  A plan serves one box size and batch size: the
+ * score-and-insert pipeline of a batch -- fr_prep_batch, fr_cc3m_batch at
+ * each particle's current parameters, the A3D3 flush of fr_insert_batch into
+ * the bound half-maps -- captured once as a HIP graph and replayed per batch.
+ * Inputs per batch: the images, written into fr_plan_images() (nb x NSAM x
+ * (NSAM+2) floats, device), and the host records ppar (nb x 8, as
+ * fr_prep_batch) and recb/half (as fr_insert_batch; the CC3M uses PHI THETA
+ * PSI DSHX DSHY AMAGP of recb).  score (host, may be NULL) receives the CC3M
+ * of each particle.  Results equal the eager sequence of the three entry
+ * points (scores bit for bit; insertion up to the order of float atomics).
+ * The CC3M launch variant is the context's choice for NSAM at creation
+ * (fr_autotune_cc3m). */
+typedef struct fr_plan fr_plan;
+int fr_plan_create(fr_ctx *ctx, int nsam, int nb, const float *d_a3df, const float *d_pssnr,
+                   const float *gpar, int ictf, int maxr1, int maxr2, float rbfact,
+                   const float *d_fsct, const float *ring, int nsym, const int *jsym,
+                   const float *symop, float pbc, float boff, float psize, float *d_num0,
+                   float *d_ctf2_0, float *d_num1, float *d_ctf2_1, long long *d_nksum,
+                   double *d_ttd, fr_plan **plan);
+float *fr_plan_images(fr_plan *plan);
+/* Copies nb images (device, nb x NSAM x (NSAM+2) floats) into the plan. */
+int fr_plan_load(fr_plan *plan, const float *d_images);
+int fr_plan_run(fr_plan *plan, const float *ppar, const float *recb, const int *half,
+                float *score);
+int fr_plan_destroy(fr_plan *plan);
+
+/* Launch autotuner (no reference counterpart): times the CC3M variants that
+ * apply at NSAM -- the band-pruned fused kernel with one workgroup per item
+ * or a persistent grid of 1/2/4/8 workgroups per CU, and the FFT path -- on
+ * the caller's items (arguments as fr_cc3m_batch), keeps the fastest for
+ * NSAM in the context and writes the timings and the choice as JSON into
+ * log (logcap bytes, may be NULL). */
+/* Sets the CC3M variant for NSAM by hand: fused (1) or FFT path (0), and the
+ * fused kernel's persistent workgroup count (0: one per item). */
+int fr_set_cc3m_variant(fr_ctx *ctx, int nsam, int fused, int workgroups);
+int fr_autotune_cc3m(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
+                     const float *d_ctff, const int *d_pidx, const float *d_dm,
+                     const float *d_shift, const float *d_amag, int maxr1, int maxr2,
+                     float rbfact, const float *d_fsct, const float *ring, float *d_score,
+                     char *log, int logcap);
 
 #ifdef __cplusplus
 }
