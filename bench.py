@@ -4,16 +4,26 @@ Workload (the mode-3 global search the metric is quoted on): a 256^3
 D7-symmetric Gaussian-blob phantom at 1.0 A/px; synthetic 256x256 particles
 at uniform random orientations, +-3 A shifts, CTF at 300 kV / Cs 2.7 mm /
 7% amplitude contrast, defocus U(1,3) um with up to 0.1 um astigmatism, SNR
-0.05.  One step = one batch of B particles already resident in HBM through
-particle preparation (r2c + CTFAPPLY), the exhaustive D7 search (SEARCHANG at
-5 deg: 130 views x 360 in-plane angles = 46,800 orientations, 15x15 NSAMR
-shift hypotheses, 8 A band) as an MFMA GEMM, per-view top-32, and FP32 CC3M
-rescoring of the 32 candidates.  `value` is particles/s of the BF16 path;
-the plain FP32-GEMM path is timed on the same batch and reported beside it.
+0.05.  The search grid is SEARCHANG at 5 deg for D7: 130 views x 360
+in-plane angles = 46,800 orientations, 15x15 NSAMR shift hypotheses, 8 A
+band; the 32 best views are rescored with the FP32 CC3M.
+
+One step = one batch of B particles already resident in HBM through
+particle preparation (r2c + CTFAPPLY), the search and the FP32 rescoring.
+Three searches are timed on the same batches:
+  * learned + BF16 (`value`): the proposal network's 8 most probable
+    15-degree classes select the bank orientations to score (BF16 MFMA
+    GEMM); particles whose top class probability is below 0.3 take the
+    exhaustive search instead;
+  * exhaustive BF16: every orientation, BF16 MFMA GEMM;
+  * plain FP32: every orientation, FP32 MFMA GEMM (the baseline of the metric).
+The network is trained first, on the GPU, on projections of the same
+reference (its time is reported and amortised separately).
 
 Also reported: insertion GB/s (the refined particles inserted into the two
-half-maps with the 14 D7 copies), the roofline of the dominant kernel (the
-search GEMM) from HIP events on its own stream, and a CPU baseline (the
+half-maps with the 14 D7 copies), the FSC(0.143) resolution of the maps
+built from the learned and from the plain FP32 searches, the roofline of the
+dominant kernel from HIP events on its own stream, and a CPU baseline (the
 oracle's port of PSEARCH/CCP on a bounded sample, 1 thread).
 """
 from __future__ import annotations
@@ -38,6 +48,8 @@ def parse():
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-views", type=int, default=16)
+    ap.add_argument("--exh-steps", type=int, default=2, help="timed steps of the exhaustive BF16 search")
+    ap.add_argument("--train-samples", type=int, default=200_000)
     return ap.parse_args()
 
 
@@ -51,6 +63,10 @@ TOPK = 32
 PEAK_BF16 = 2.5e15   # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
+# FP32 MFMA work of one fused CC3M item at 256^2 with MAXR2 = 32 (16x16x4
+# products of the band-pruned transforms: column 1024 + rows 16 x 16 x 32 +
+# column 1024, 2048 flops each); the ring-mean pass is not counted
+CC3M_FLOPS = (1024 + 16 * 16 * 32 + 1024) * 2048.0
 
 
 def log(msg):
@@ -65,7 +81,7 @@ def main():
         sys.exit(2)
     import torch
 
-    from frealign_amd import api, search, synth
+    from frealign_amd import api, proposal, search, synth
 
     dev = "cuda"
     ctx = api.Context(0)
@@ -98,102 +114,167 @@ def main():
         lo = (i * B) % npool
         return parts.images[lo:lo + B], ppar_all[lo:lo + B], lo
 
-    def step(gs, i, rescore=True):
-        img, pp, lo = batch(i)
-        work = img.clone()
-        ctff, outc, datc = ctx.ctfapply(NSAM, work, pp, pssnr, gpar)
-        top_o, top_r, top_s = gs.candidates(outc, ctff, TOPK)
-        res = None
-        if rescore:
-            sc, shifts = gs.rescore(outc, ctff, top_o, top_r, fsct, ring)
-            best = np.argmax(sc, axis=1)
-            o = top_o.cpu().numpy()
-            res = (o[np.arange(B), best], shifts[np.arange(B), best], lo, ctff, datc)
-        return res
+    ar = np.arange(B)
 
-    # ---------------------------------------------------------------- BF16
+    def prep(i):
+        img, pp, lo = batch(i)
+        ctff, outc, datc = ctx.ctfapply(NSAM, img.clone(), pp, pssnr, gpar)
+        return ctff, outc, datc, lo
+
+    def step_exhaustive(gs, i):
+        ctff, outc, datc, lo = prep(i)
+        top_o, top_r, top_s = gs.candidates(outc, ctff, TOPK)
+        sc, shifts = gs.rescore(outc, ctff, top_o, top_r, fsct, ring)
+        best = np.argmax(sc, axis=1)
+        o = top_o.cpu().numpy()
+        return o[ar, best], shifts[ar, best], lo, ctff, datc, None
+
+    def step_learned(i):
+        ctff, outc, datc, lo = prep(i)
+        res, conf = lp.run(outc, ctff, datc, fsct, ring, TOPK)
+        return res.view, res.shift, lo, ctff, datc, conf
+
+    def timed(fn, nsteps, first):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = [fn(first + i) for i in range(nsteps)]
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    # ------------------------------------------------------ proposal network
+    cfg = proposal.ProposalConfig()
+    lp = proposal.LearnedProposal(ctx, gs16, mats, "D", 7, cfg, seed=0xC5)
+    log(f"proposal: {lp.classes.n} classes at {cfg.class_deg:g} deg, "
+        f"{int(lp.cell_groups.sum())} cell groups of {cfg.group}; training on "
+        f"{args.train_samples} synthetic projections")
+    rep = lp.train(optics, RI, nsamples=args.train_samples, batch=256, lr=2e-3, seed=0xC5,
+                   progress=log, log_every=100)
+    log(f"trained in {rep.seconds:.1f}s: held-out top-1 {rep.heldout_top1:.3f}, "
+        f"top-{cfg.topk} {rep.heldout_topk:.3f}")
+
+    # ------------------------------------------------------- learned + BF16
     for i in range(args.warmup):
-        step(gs16, i)
-    torch.cuda.synchronize()
+        step_learned(i)
     ctx.profile(True)
-    ctx.profile_read("search_gemm")
-    results = []
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        results.append(step(gs16, args.warmup + i))
-    torch.cuda.synchronize()
-    dt16 = time.perf_counter() - t1
-    gemm_ms, gemm_n = ctx.profile_read("search_gemm")
+    for k in ("search_gemm", "search_list", "cc3m", "prep", "cnn"):
+        ctx.profile_read(k)
+    valid = []
+
+    def step_learned_counted(i):
+        r = step_learned(i)
+        valid.append(lp.last_valid)
+        return r
+
+    results, dtl = timed(step_learned_counted, args.steps, args.warmup)
+    fb_ms, fb_n = ctx.profile_read("search_gemm")
+    cnn_ms, cnn_n = ctx.profile_read("cnn")
+    gemm_ms, gemm_n = ctx.profile_read("search_list")
     cc3m_ms, cc3m_n = ctx.profile_read("cc3m")
     prep_ms, prep_n = ctx.profile_read("prep")
     ctx.profile(False)
-    pps16 = B * args.steps / dt16
-    log(f"bf16 path: {pps16:.1f} particles/s")
+    ppsl = B * args.steps / dtl
+    conf = np.concatenate([r[5] for r in results])
+    log(f"learned + bf16: {ppsl:.1f} particles/s ({np.mean(~conf):.3f} fell back)")
 
-    # accuracy of the BF16 path against the truth (modulo D7)
-    errs = []
-    for o, _, lo, _, _ in results:
-        errs.append(search.angular_error_deg(gs16.euler[o], parts.euler[lo:lo + B], mats))
-    err = np.concatenate(errs)
+    # ------------------------------------------------------- exhaustive BF16
+    step_exhaustive(gs16, 0)
+    ctx.profile(True)
+    ctx.profile_read("search_gemm")
+    res16, dt16 = timed(lambda i: step_exhaustive(gs16, i), args.exh_steps, args.warmup)
+    gemm16_ms, gemm16_n = ctx.profile_read("search_gemm")
+    ctx.profile(False)
+    pps16 = B * args.exh_steps / dt16
+    log(f"exhaustive bf16: {pps16:.1f} particles/s")
 
-    # ---------------------------------------------------------------- FP32
+    # ---------------------------------------------------------- plain FP32
     gs32 = search.GlobalSearch(ctx, a3df, dang=DANG, asym="D", nasym=7, maxr1=maxr1, maxr2=maxr2,
                                rbfact=0.0, dtype=torch.float32, iran=-0xC3)
-    step(gs32, 0)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    res32 = []
-    for i in range(args.fp32_steps):
-        res32.append(step(gs32, args.warmup + i))
-    torch.cuda.synchronize()
-    dt32 = time.perf_counter() - t2
+    step_exhaustive(gs32, 0)
+    res32, dt32 = timed(lambda i: step_exhaustive(gs32, i), args.fp32_steps, args.warmup)
     pps32 = B * args.fp32_steps / dt32
-    log(f"fp32 path: {pps32:.1f} particles/s")
-    agree = []
-    for (o16, _, _, _, _), (o32, _, _, _, _) in zip(results, res32):
-        agree.append(search.angular_error_deg(gs16.euler[o16], gs32.euler[o32], mats))
-    agree = np.concatenate(agree)
+    log(f"plain fp32: {pps32:.1f} particles/s")
     del gs32
 
-    # ----------------------------------------------------------- insertion
-    vols = api.HalfMaps.zeros(NSAM)
-    ctx.profile(True)
-    ctx.profile_read("insert")
+    # -------------------------------------------------------------- accuracy
+    def within(ra, rb_euler_fn):
+        return np.concatenate([search.angular_error_deg(gs16.euler[np.maximum(o, 0)],
+                                                        rb_euler_fn(k, lo), mats)
+                               for k, (o, _, lo, _, _, _) in enumerate(ra)])
+
+    err_l = within(results, lambda k, lo: parts.euler[lo:lo + B])
+    err_32 = within(res32, lambda k, lo: parts.euler[lo:lo + B])
+    err_l32 = within(results[:len(res32)],
+                     lambda k, lo: gs16.euler[np.maximum(res32[k][0], 0)])
+    err_1632 = within(res16[:len(res32)],
+                      lambda k, lo: gs16.euler[np.maximum(res32[k][0], 0)])
+
+    # ------------------------------------------------- insertion and FSC
     npts = 0
     irrec = NSAM // 2
     for x in range(-irrec, irrec + 1):
         for y in range(-irrec, irrec + 1):
             if x * x + y * y <= irrec * irrec:
                 npts += 1
-    ins_parts = 0
-    for o, sh, lo, ctff, datc in results[:1]:
-        recb = np.zeros((B, 10), np.float32)
-        recb[:, 0:3] = gs16.euler[o]
-        recb[:, 3:5] = sh
-        recb[:, 5] = 1.0
-        recb[:, 6] = 1.0
-        recb[:, 7] = 1.0
-        recb[:, 8] = 0.5
-        recb[:, 9] = 0.5
-        half = (np.arange(B) + lo) % 2
-        ctx.insert(NSAM, datc, ctff, recb, half.astype(np.int32), sym, 5.0, 40.0, PSIZE, vols)
-        ins_parts += B
-    torch.cuda.synchronize()
-    ins_ms, ins_n = ctx.profile_read("insert")
-    ctx.profile(False)
-    ins_bytes = 216.0 * npts * sym[3] * ins_parts
+
+    def reconstruct(res_list, profile=False):
+        vols = api.HalfMaps.zeros(NSAM)
+        if profile:
+            ctx.profile(True)
+            ctx.profile_read("insert")
+        for o, sh, lo, ctff, datc, _ in res_list:
+            recb = np.zeros((B, 10), np.float32)
+            recb[:, 0:3] = gs16.euler[np.maximum(o, 0)]
+            recb[:, 3:5] = sh
+            recb[:, 5] = 1.0
+            recb[:, 6] = 1.0
+            recb[:, 7] = 1.0
+            recb[:, 8] = 0.5
+            recb[:, 9] = 0.5
+            half = (np.arange(B) + lo) % 2
+            ctx.insert(NSAM, datc, ctff, recb, half.astype(np.int32), sym, 5.0, 40.0, PSIZE, vols)
+        torch.cuda.synchronize()
+        ms = None
+        if profile:
+            ms, _ = ctx.profile_read("insert")
+            ctx.profile(False)
+        return vols, ms
+
+    nfsc = min(len(results), len(res32))
+    vols_l, ins_ms = reconstruct(results[:nfsc], profile=True)
+    ins_bytes = 216.0 * npts * sym[3] * B * nfsc
     ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
     log(f"insertion: {ins_gbps:.1f} GB/s")
+    res_l = fsc_resolution(ctx, vols_l, sym)
+    del vols_l
+    vols_32, _ = reconstruct(res32[:nfsc])
+    res_p = fsc_resolution(ctx, vols_32, sym)
+    del vols_32
+    log(f"FSC(0.143): learned {res_l:.2f} A, plain fp32 {res_p:.2f} A ({nfsc * B} particles)")
 
     # ------------------------------------------------------------ roofline
+    # dominant kernel of the learned step: the FP32 CC3M rescoring or the search GEMM
+    kc_flops = 2.0 * 225 * 2 * kc
+    ncand = float(np.mean(valid))
     gemm_avg_s = gemm_ms / 1e3 / max(gemm_n, 1)
-    flops = 2.0 * B * 225 * norient * (2 * kc)
-    achieved = flops / gemm_avg_s
-    roof = {"kernel": "k_search_gemm<bf16>", "bound": "mfma", "achieved": achieved / 1e12,
-            "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_BF16,
-            "traffic": None, "launch_ms": gemm_avg_s * 1e3,
-            "algorithmic_flops_per_launch": flops}
+    cc3m_avg_s = cc3m_ms / 1e3 / max(cc3m_n, 1)
+    if cc3m_avg_s >= gemm_avg_s:
+        flops = B * TOPK * CC3M_FLOPS
+        achieved = flops / cc3m_avg_s
+        roof = {"kernel": "k_cc3m_fused (FP32 MFMA)", "bound": "mfma", "achieved": achieved / 1e12,
+                "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA,
+                "traffic": None, "launch_ms": cc3m_avg_s * 1e3,
+                "algorithmic_flops_per_launch": flops}
+    else:
+        flops = kc_flops * ncand
+        achieved = flops / gemm_avg_s
+        roof = {"kernel": "k_search_gemm<bf16> (candidate lists)", "bound": "mfma",
+                "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_BF16, "traffic": None, "launch_ms": gemm_avg_s * 1e3,
+                "algorithmic_flops_per_launch": flops}
+    g16_avg = gemm16_ms / 1e3 / max(gemm16_n, 1)
+    exh_flops = 2.0 * B * 225 * norient * (2 * kc)
+    exh_roof = {"kernel": "k_search_gemm<bf16> (exhaustive)", "achieved_TFLOPs": exh_flops / g16_avg
+                / 1e12, "frac": exh_flops / g16_avg / PEAK_BF16, "launch_ms": g16_avg * 1e3}
 
     # --------------------------------------------------------- CPU baseline
     cpu = None
@@ -203,14 +284,15 @@ def main():
                            gpar=gpar, maxr1=maxr1, maxr2=maxr2, norient_views=norient // gs16.group,
                            ctx=ctx, pssnr=pssnr)
 
+    amort = B * args.steps / (dtl + rep.seconds * (B * args.steps) / 20000.0)
     out = {
         "metric": "particles/s global-search refine @256² (learned+BF16 vs plain FP32 GPU), same FSC",
-        "value": pps16,
+        "value": ppsl,
         "unit": "particles/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt16 / args.steps * 1e3,
+        "ms_per_step": dtl / args.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -219,21 +301,53 @@ def main():
         "config": {"workload": "mode-3 global search, 256^2, D7, 5 deg (46,800 orientations), "
                                "8 A band, 15x15 shifts, top-32 FP32 CC3M rescoring",
                    "batch_particles": B, "orientations": norient, "annulus_complex": kc,
-                   "proposal": "none (exhaustive; learned proposal not yet built)"},
+                   "proposal": f"CNN {cfg.nconv} conv x {cfg.channels} ch on {cfg.S}^2, "
+                               f"{lp.classes.n} classes at {cfg.class_deg:g} deg, top-{cfg.topk}, "
+                               f"fallback below {cfg.threshold}"},
         "plain_fp32_particles_per_s": pps32,
-        "speedup_vs_plain_fp32": pps16 / pps32,
-        "bf16_within_one_step_of_truth": float(np.mean(err <= DANG)),
-        "bf16_vs_fp32_same_view_within_one_step": float(np.mean(agree <= DANG)),
+        "speedup_vs_plain_fp32": ppsl / pps32,
+        "exhaustive_bf16_particles_per_s": pps16,
+        "learned_within_one_step_of_truth": float(np.mean(err_l <= DANG)),
+        "plain_fp32_within_one_step_of_truth": float(np.mean(err_32 <= DANG)),
+        "learned_vs_plain_fp32_within_one_step": float(np.mean(err_l32 <= DANG)),
+        "exhaustive_bf16_vs_plain_fp32_within_one_step": float(np.mean(err_1632 <= DANG)),
+        "fallback_fraction": float(np.mean(~conf)),
+        "candidates_per_confident_particle": float(np.sum(valid)) / max(1, int(np.sum(conf))),
+        "fsc0143_A": {"learned_bf16": res_l, "plain_fp32": res_p, "particles": nfsc * B},
+        "training": {"samples": rep.samples, "seconds": rep.seconds, "final_loss": rep.final_loss,
+                     "heldout_top1": rep.heldout_top1, f"heldout_top{cfg.topk}": rep.heldout_topk,
+                     "particles_per_s_with_training_amortised_over_20000": amort},
         "insertion_GBps": ins_gbps,
         "insertion_frac_of_hbm_peak": ins_gbps * 1e9 / PEAK_HBM,
-        "kernel_ms_per_step": {"search_gemm": gemm_ms / max(args.steps, 1),
+        "kernel_ms_per_step": {"search_gemm_lists": gemm_ms / max(args.steps, 1),
+                               "search_gemm_fallback": fb_ms / max(args.steps, 1),
+                               "cnn_forward": cnn_ms / max(args.steps, 1),
                                "cc3m_rescoring": cc3m_ms / max(args.steps, 1),
                                "prep": prep_ms / max(args.steps, 1)},
         "roofline": roof,
+        "exhaustive_gemm": exh_roof,
         "cpu_baseline": cpu,
         "setup_s": setup_s,
     }
     print(json.dumps(out))
+
+
+def fsc_resolution(ctx, vols, sym) -> float:
+    """FSC(0.143) resolution (A) of the two half-maps: STD/VTD as in
+    frealign_v9.f:793-798 (TTD / NKSUM * 0.1), STORESHIFT, SHELTEST; the
+    first shell (radius k in Fourier pixels, k >= 2) below 0.143 gives
+    NSAM * PSIZE / k."""
+    nsam = NSAM
+    nk = float(vols.nksum.sum().item())
+    ttd = vols.ttd.cpu().numpy()
+    std, vtd = ttd[0] / nk * 0.1, ttd[1] / nk * 0.1
+    b, c = ctx.storeshift(nsam, std, vtd, sym[3], vols.num[0], vols.ctf2[0], vols.num[1],
+                          vols.ctf2[1])
+    fsc = ctx.sheltest(nsam, b, c, vols.ctf2[0], vols.ctf2[1])["fsc"]
+    for k in range(2, len(fsc)):
+        if fsc[k] < 0.143:
+            return float(nsam * PSIZE / k)
+    return float(2.0 * PSIZE)
 
 
 def cpu_baseline(nviews, parts, a3df, ctff_for_cpu, optics, gpar, maxr1, maxr2, norient_views,

@@ -102,6 +102,54 @@ module frealign_hip
       integer(c_int) :: fr_cc3m_batch
     end function
 
+    ! PRESB (presb.f:8-160)
+    function fr_presb_batch(ctx, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, &
+                            ring, d_rbin) bind(C, name="fr_presb_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, d_rbin
+      integer(c_int), value :: nsam, n
+      real(c_float) :: ring(4)
+      integer(c_int) :: fr_presb_batch
+    end function
+
+    ! SIGMA (sigma.f:8-221)
+    function fr_sigma_batch(ctx, nsam, n, d_a3df, d_datc, d_ctff, pidx, par, amag, maxr1, rclas, &
+                            dmask, sigp, signset, out) bind(C, name="fr_sigma_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_datc, d_ctff
+      integer(c_int), value :: nsam, n, maxr1
+      real(c_float), value :: rclas, sigp
+      integer(c_int) :: pidx(*)
+      real(c_float) :: par(5, *), amag(*), dmask(4), signset(*), out(3, *)
+      integer(c_int) :: fr_sigma_batch
+    end function
+
+    ! VA04A (va04.f:8-369) on a host objective: f = fn(n, x, user)
+    function fr_powell_minimize(n, x, e, escale, icon, maxit, fn, user, f, nevals) &
+        bind(C, name="fr_powell_minimize")
+      import :: c_int, c_float, c_ptr, c_funptr
+      integer(c_int), value :: n, icon, maxit
+      real(c_float) :: x(*), e(*), f
+      real(c_float), value :: escale
+      type(c_funptr), value :: fn
+      type(c_ptr), value :: user
+      integer(c_int) :: nevals
+      integer(c_int) :: fr_powell_minimize
+    end function
+
+    ! PREFINE (prefine.f:8-162) for n items, lock-stepped
+    function fr_refine_batch(ctx, nsam, n, d_a3df, d_spec, d_ctff, pidx, amag, par, mask, maxr1, &
+                             maxr2, rbfact, d_fsct, ring, prior, maxit, nevals) &
+        bind(C, name="fr_refine_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_fsct
+      integer(c_int), value :: nsam, n, maxr1, maxr2, maxit
+      real(c_float), value :: rbfact
+      integer(c_int) :: pidx(*), mask(5), nevals(*)
+      real(c_float) :: amag(*), par(6, *), ring(4), prior(5, *)
+      integer(c_int) :: fr_refine_batch
+    end function
+
     ! A3D3 flush -> PINSERT -> PINSERT_TRILIN (a3d3.f:107-143, pinsert.f, pinsert_trilin.f)
     function fr_insert_batch(ctx, nsam, nb, d_datc, d_ctff, recb, half, nsym, jsym, symop, pbc, &
                              boff, psize, d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd) &

@@ -33,3 +33,26 @@ def test_fortran_host_matches_reference(tmp_path):
     r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "FR_GRID_CHECK mismatches: 0" in r.stdout, r.stdout[-2000:]
+
+
+@pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not installed")
+def test_fortran_host_powell_matches_reference(tmp_path):
+    """The library's VA04A called from Fortran reproduces the reference's
+    flang-compiled VA04A bit for bit (final X, F and evaluation count)."""
+    if not os.path.exists(REF):
+        pytest.skip("reference routines (oracle/_ref) not built")
+    src = os.path.join(ROOT, "frealign_amd", "fortran")
+    mod = tmp_path / "mod.o"
+    r = subprocess.run([FLANG, "-O1", "-c", "-module-dir", str(tmp_path),
+                        os.path.join(src, "frealign_hip_mod.f90"), "-o", str(mod)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    exe = tmp_path / "fr_powell_check"
+    r = subprocess.run([FLANG, "-O1", "-module-dir", str(tmp_path),
+                        os.path.join(src, "fr_powell_check.f90"), str(mod), REF, LIB,
+                        f"-Wl,-rpath,{os.path.dirname(REF)}:{os.path.dirname(LIB)}", "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "FR_POWELL_CHECK mismatches: 0" in r.stdout, r.stdout[-2000:]
