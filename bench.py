@@ -50,6 +50,11 @@ def parse():
     ap.add_argument("--cpu-views", type=int, default=16)
     ap.add_argument("--exh-steps", type=int, default=2, help="timed steps of the exhaustive BF16 search")
     ap.add_argument("--train-samples", type=int, default=200_000)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="one device copy of this many MiB first (PMC calibration of the "
+                         "HBM byte counters against a known size)")
+    ap.add_argument("--uneven-batches", type=int, default=1,
+                    help="batches of 256 per 'unit' of the 3:2:1 uneven workload (0: skip)")
     return ap.parse_args()
 
 
@@ -84,6 +89,14 @@ def main():
     from frealign_amd import api, proposal, search, synth
 
     dev = "cuda"
+    if args.copy_probe_mb > 0:
+        n = args.copy_probe_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written")
+        del src, dst
     ctx = api.Context(0)
     optics = synth.Optics(psize=PSIZE)
     sym = api.point_group("D", 7)
@@ -276,6 +289,11 @@ def main():
     exh_roof = {"kernel": "k_search_gemm<bf16> (exhaustive)", "achieved_TFLOPs": exh_flops / g16_avg
                 / 1e12, "frac": exh_flops / g16_avg / PEAK_BF16, "launch_ms": g16_avg * 1e3}
 
+    # ------------------------------------------------ uneven box sizes
+    uneven = None
+    if args.uneven_batches > 0:
+        uneven = uneven_workload(ctx, args.uneven_batches, log)
+
     # --------------------------------------------------------- CPU baseline
     cpu = None
     if not args.no_cpu:
@@ -327,9 +345,91 @@ def main():
         "roofline": roof,
         "exhaustive_gemm": exh_roof,
         "cpu_baseline": cpu,
+        "uneven_batches": uneven,
         "setup_s": setup_s,
     }
     print(json.dumps(out))
+
+
+def uneven_workload(ctx, units, log):
+    """Three seeded stacks of boxes 96 / 160 / 256 (1.3 / 1.1 / 1.0 A/px) in
+    ratio 3:2:1, defocus U(0.8, 3.5) um and SNR U(0.03, 0.2) per batch, each
+    box size with its own reference, half-maps, autotuned CC3M and
+    graph-captured score-and-insert plan; batches interleaved.  Returns
+    particles/s per box size and in total, and the autotuner's choices."""
+    import torch
+    from frealign_amd import api, synth
+    dev = "cuda"
+    sym = api.point_group("C", 1)
+    B = 256
+    stacks = [(96, 1.3, 3), (160, 1.1, 2), (256, 1.0, 1)]
+    state = {}
+    for nsam, psize, ratio in stacks:
+        optics = synth.Optics(psize=psize)
+        ri = 0.4375 * nsam
+        vol = synth.blob_phantom(nsam, 40, 0.4375 * nsam * 0.9, seed=0xC400 + nsam, device=dev)
+        a3df = synth.half_volume(vol)
+        del vol
+        gpar = optics.gpar(nsam, ri)
+        pssnr = torch.from_numpy(synth.default_pssnr(nsam, ri)).to(dev)
+        fsct = torch.from_numpy(synth.default_fsct(nsam, psize).astype(np.float32)).to(dev)
+        ring = np.array([gpar[7], gpar[8], gpar[5], gpar[6]], np.float32)
+        maxr1 = max(1, int(nsam * psize / 100.0))
+        maxr2 = min(32, int(nsam * psize / 8.0), nsam // 2 - 1)
+        rng = np.random.default_rng(0xC400 + nsam)
+        batches = []
+        for k in range(ratio * units):
+            snr = float(rng.uniform(0.03, 0.2))
+            parts = synth.make_particles(ctx, a3df, B, seed=0xC4000 + 100 * nsam + k,
+                                         optics=optics, snr=snr, ri=ri,
+                                         defocus=(0.8e4, 3.5e4))
+            recb = np.zeros((B, 10), np.float32)
+            recb[:, 0:3] = parts.euler
+            recb[:, 3:5] = parts.shift
+            recb[:, 5:8] = 1.0
+            recb[:, 8] = 0.5
+            recb[:, 9] = 0.5
+            half = (np.arange(B) % 2).astype(np.int32)
+            batches.append((parts.images, parts.ppar().cpu().numpy(), recb, half))
+        # autotune the CC3M launch on the first batch of this box size
+        img, ppar, recb, half = batches[0]
+        ctff, outc, _ = ctx.ctfapply(nsam, img.clone(), torch.from_numpy(ppar).to(dev), pssnr, gpar)
+        t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+        dm = api.rotation(recb[:, 0:3], amag=1.0)
+        tune = ctx.autotune_cc3m(nsam, a3df, outc, ctff, t(np.arange(B, dtype=np.int32)), t(dm),
+                                 t(recb[:, 3:5]), t(recb[:, 5]), maxr1, maxr2, 0.0, fsct, ring)
+        vols = api.HalfMaps.zeros(nsam)
+        plan = api.Plan(ctx, nsam, B, a3df, pssnr, gpar, maxr1, maxr2, 0.0, fsct, ring, sym, 5.0,
+                        40.0, psize, vols)
+        plan.run(*batches[0])  # warm
+        state[nsam] = dict(plan=plan, batches=batches, vols=vols, a3df=a3df, tune=tune,
+                           time=0.0, n=0)
+        log(f"uneven: box {nsam}: {len(batches)} batches, CC3M choice "
+            f"{tune['candidates'][tune['choice']]}")
+    order = []
+    for k in range(6 * units):
+        for nsam, _, ratio in stacks:
+            if k < ratio * units:
+                order.append((nsam, k))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for nsam, k in order:
+        st = state[nsam]
+        t1 = time.perf_counter()
+        st["plan"].run(*st["batches"][k])
+        st["time"] += time.perf_counter() - t1
+        st["n"] += B
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    out = {"workload": "score-and-insert at known parameters, boxes 96/160/256 in ratio 3:2:1, "
+                       "C1, graph-captured plan per box size",
+           "total_particles_per_s": sum(st["n"] for st in state.values()) / total}
+    for nsam, st in state.items():
+        out[f"box{nsam}"] = {"particles_per_s": st["n"] / st["time"], "particles": st["n"],
+                             "cc3m_choice": st["tune"]["candidates"][st["tune"]["choice"]]}
+        st["plan"].close()
+    log(f"uneven: {out['total_particles_per_s']:.1f} particles/s in total")
+    return out
 
 
 def fsc_resolution(ctx, vols, sym) -> float:
