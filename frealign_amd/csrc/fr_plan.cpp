@@ -57,6 +57,10 @@ struct fr_plan {
   InsertTask* h_tasks = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // the plan's own stream (capture is not allowed on the legacy default
+  // stream) and the event that orders it after the caller's stream
+  hipStream_t stream = nullptr;
+  hipEvent_t ready = nullptr;
   int launches = 0;
 };
 
@@ -231,16 +235,24 @@ int fr_plan_create(fr_ctx* c, int nsam, int nb, const float* d_a3df, const float
   for (int k = 0; k < nb * ncopy; k++) p->h_tasks[k].irrec2 = -1;
   // the FFT twiddles of this length are created lazily: make them before capture
   if (!c->twiddles(nsam)) return fail(fr::fail("twiddle table allocation failed"));
+  if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&p->ready, hipEventDisableTiming) != hipSuccess)
+    return fail(fr::fail("plan stream/event creation failed"));
   const bool prof = c->prof_on;
+  hipStream_t caller = c->stream;
+  if (hipStreamSynchronize(caller) != hipSuccess)
+    return fail(fr::fail("synchronisation before capture failed"));
   c->prof_on = false;
-  if (hipStreamSynchronize(c->stream) != hipSuccess ||
-      hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+  c->stream = p->stream;  // the launchers enqueue on the context's stream
+  if (hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    c->stream = caller;
     c->prof_on = prof;
     return fail(fr::fail("stream capture could not start"));
   }
   r = enqueue(p);
   hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  hipError_t e = hipStreamEndCapture(p->stream, &graph);
+  c->stream = caller;
   c->prof_on = prof;
   if (r) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -267,10 +279,17 @@ int fr_plan_run(fr_plan* p, const float* ppar, const float* recb, const int* hal
   FR_CHECK(p && ppar && recb && half, "null argument");
   if (int r = stage(p, ppar, recb, half)) return r;
   fr_ctx* c = p->c;
+  // the images were loaded on the caller's stream: order the graph after it
+  FR_HIP(hipEventRecord(p->ready, c->stream));
+  FR_HIP(hipStreamWaitEvent(p->stream, p->ready, 0));
+  hipStream_t caller = c->stream;
+  c->stream = p->stream;
   int pslot = c->prof_begin(kProfPlan);
-  FR_HIP(hipGraphLaunch(p->exec, c->stream));
+  hipError_t e = hipGraphLaunch(p->exec, p->stream);
   c->prof_end(pslot);
-  FR_HIP(hipStreamSynchronize(c->stream));  // staging and scores are reused by the next batch
+  c->stream = caller;
+  FR_HIP(e);
+  FR_HIP(hipStreamSynchronize(p->stream));  // staging and scores are reused by the next batch
   if (score) memcpy(score, p->h_score, sizeof(float) * p->nb);
   p->launches++;
   return 0;
@@ -278,7 +297,7 @@ int fr_plan_run(fr_plan* p, const float* ppar, const float* recb, const int* hal
 
 int fr_plan_destroy(fr_plan* p) {
   if (!p) return 0;
-  if (p->c) (void)hipStreamSynchronize(p->c->stream);
+  if (p->stream) (void)hipStreamSynchronize(p->stream);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   void* dev[] = {p->d_img, p->d_ppar, p->d_ctff, p->d_outc, p->d_datc, p->d_dm, p->d_shift,
@@ -288,6 +307,8 @@ int fr_plan_destroy(fr_plan* p) {
   void* host[] = {p->h_ppar, p->h_dm, p->h_shift, p->h_amag, p->h_score, p->h_tasks};
   for (void* h : host)
     if (h) (void)hipHostFree(h);
+  if (p->ready) (void)hipEventDestroy(p->ready);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
   delete p;
   return 0;
 }
