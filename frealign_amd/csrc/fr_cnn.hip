@@ -50,8 +50,10 @@ struct GemmArgs {
 constexpr int kGP = 80;  // LDS pitch: 16*g + lane -> distinct banks
 
 __global__ void __launch_bounds__(kT) k_gemm(GemmArgs g) {
-  __shared__ float As[16][kGP];
-  __shared__ float Bs[16][kGP];
+  // two LDS stages: the next K tile is fetched into registers while the
+  // matrix cores work on the current one (one barrier per K tile)
+  __shared__ float As[2][16][kGP];
+  __shared__ float Bs[2][16][kGP];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int wm = wave >> 1, wn = wave & 1;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
@@ -64,46 +66,70 @@ __global__ void __launch_bounds__(kT) k_gemm(GemmArgs g) {
   for (int i = 0; i < 2; i++)
 #pragma unroll
     for (int j = 0; j < 2; j++) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+  // this thread's four A and four B elements of a K tile
+  int am[4], ak[4], bn[4], bk[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int e = tid + kT * q;
+    if (a_kfast) {
+      ak[q] = e & 15;
+      am[q] = e >> 4;
+    } else {
+      am[q] = e & 63;
+      ak[q] = e >> 6;
+    }
+    if (b_nfast) {
+      bn[q] = e & 63;
+      bk[q] = e >> 6;
+    } else {
+      bk[q] = e & 15;
+      bn[q] = e >> 4;
+    }
+  }
+  float ra[4], rb[4];
+  auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int e = tid + kT * q;
-      int mm, ka;
-      if (a_kfast) {
-        ka = e & 15;
-        mm = e >> 4;
-      } else {
-        mm = e & 63;
-        ka = e >> 6;
-      }
-      const int m = m0 + mm, k = k0 + ka;
-      As[ka][mm] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
-      int nn, kb;
-      if (b_nfast) {
-        nn = e & 63;
-        kb = e >> 6;
-      } else {
-        kb = e & 15;
-        nn = e >> 4;
-      }
-      const int n = n0 + nn, k2 = k0 + kb;
-      Bs[kb][nn] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
+      const int m = m0 + am[q], k = k0 + ak[q];
+      ra[q] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
+      const int n = n0 + bn[q], k2 = k0 + bk[q];
+      rb[q] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
     }
-    __syncthreads();
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      As[buf][ak[q]][am[q]] = ra[q];
+      Bs[buf][bk[q]][bn[q]] = rb[q];
+    }
+  };
+  int buf = 0;
+  if (kbeg < kend) {
+    fetch(kbeg);
+    put(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += 16) {
+    const bool more = k0 + 16 < kend;
+    if (more) fetch(k0 + 16);
 #pragma unroll
     for (int kk = 0; kk < 16; kk += 4) {
       float a[2], b[2];
 #pragma unroll
-      for (int i = 0; i < 2; i++) a[i] = As[kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
+      for (int i = 0; i < 2; i++)
+        a[i] = As[buf][kk + (lane >> 4)][wm * 32 + i * 16 + (lane & 15)];
 #pragma unroll
-      for (int j = 0; j < 2; j++) b[j] = Bs[kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
+      for (int j = 0; j < 2; j++)
+        b[j] = Bs[buf][kk + (lane >> 4)][wn * 32 + j * 16 + (lane & 15)];
 #pragma unroll
       for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++)
           acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
     }
+    if (more) put(buf ^ 1);
     __syncthreads();
+    buf ^= 1;
   }
   float* C = g.C + blockIdx.z * g.csplit;
 #pragma unroll
