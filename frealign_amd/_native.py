@@ -35,7 +35,7 @@ SIGNATURES = {
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
-    "fr_sigma_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, cf, vp, cf, vp, vp], ci),
+    "fr_sigma_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, cf, vp, cf, vp, vp, vp], ci),
     "fr_plan_create": ([vp, ci, ci, vp, vp, vp, ci, ci, ci, cf, vp, vp, ci, vp, vp, cf, cf, cf,
                         vp, vp, vp, vp, vp, vp, C.POINTER(vp)], ci),
     "fr_plan_images": ([vp], vp),
@@ -101,4 +101,7 @@ def declared_symbols() -> list[str]:
     """Every function name declared in include/frealign_hip.h."""
     txt = open(HEADER).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(fr_[a-z0-9_]+)\s*\(", txt)))
+    names = set(re.findall(r"\b(fr_[a-z0-9_]+)\s*\(", txt))
+    # the per-call entry points carry the reference's Fortran symbols (name_)
+    names |= set(re.findall(r"\b([a-z][a-z0-9]*_)\s*\(", txt))
+    return sorted(names)

@@ -328,7 +328,7 @@ class Context:
         self._sync_stream()
         N.check(N.lib().fr_sigma_batch(self.h, nsam, n, _p(a3df), _p(datc), _p(ctff), _p(pi),
                                        _p(p), _p(am), int(maxr1), float(rclas), _p(dm),
-                                       float(sigp), _p(ss), _p(out)), "fr_sigma_batch")
+                                       float(sigp), _p(ss), None, _p(out)), "fr_sigma_batch")
         return out
 
     def set_cc3m_variant(self, nsam: int, fused: bool, workgroups: int = 0):

@@ -28,6 +28,7 @@ SOURCES = [
     "fr_refine.cpp",
     "fr_sigma.hip",
     "fr_plan.cpp",
+    "fr_fortran.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

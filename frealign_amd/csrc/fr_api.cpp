@@ -812,7 +812,7 @@ int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float
 int fr_sigma_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_datc,
                    const float* d_ctff, const int* pidx, const float* par, const float* amag,
                    int maxr1, float rclas, const float* dmask, float sigp, const float* signset,
-                   float* out) {
+                   const float* sm, float* out) {
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(d_a3df && d_datc && d_ctff && pidx && par && amag && dmask && out && n >= 0,
@@ -828,11 +828,11 @@ int fr_sigma_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float*
     const float* p = par + 5 * (size_t)k;
     SigmaItem& s = it[k];
     s.pidx = pidx[k];
-    rotation(p[0], p[1], p[2], 0.0f, nullptr, s.dm);  // pextract.f:30-51
+    rotation(p[0], p[1], p[2], 0.0f, sm, s.dm);  // pextract.f:30-51
     s.shx = p[3];
     s.shy = p[4];
     float dm2[9];
-    rotation(-p[2], -p[1], -p[0], amag[k], nullptr, dm2);  // sigma.f:110-111
+    rotation(-p[2], -p[1], -p[0], amag[k], sm, dm2);  // sigma.f:110-111
     {
       const float d1 = dmask[0] - (float)jc, d2 = dmask[1] - (float)jc, d3 = dmask[2] - (float)jc;
       volatile float t;  // keep the reference's FP32 rounding of each product

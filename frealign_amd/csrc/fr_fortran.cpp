@@ -1,0 +1,238 @@
+// fr_fortran.cpp -- per-call entry points with the reference's own symbols and
+// argument lists (Fortran 77 implicit interfaces: every argument by
+// reference, CHARACTER lengths appended), so that a flang- or gfortran-built
+// LMAIN links against this library in place of the reference's routines:
+//
+//   REAL FUNCTION CC3M(...)        cc3m.f:8-169      -> cc3m_
+//   SUBROUTINE PEXTRACT(...)       pextract.f:8-101  -> pextract_
+//   SUBROUTINE PRESB(...)          presb.f:8-160     -> presb_
+//   SUBROUTINE SIGMA(...)          sigma.f:8-221     -> sigma_
+//
+// Each call copies its host arrays to the device, runs the batched kernels on
+// one item and copies the result back; the batched entry points of
+// include/frealign_hip.h are the throughput path.  The reference calls these
+// routines serially per process (a library-wide context on device 0 is used;
+// calls are not thread-safe).  The supported cases are the reference's
+// defaults on the hot path: IPAD = 1, IRAD > 0, IEWALD = 0, non-helical
+// ASYM, CC3M with IBUF = 0.  Anything else, or a device error, is fatal as the
+// reference's own STOPs are: a message on stdout and exit(1).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../../include/frealign_hip.h"
+#include "fr_common.h"
+#include "fr_host.h"
+
+namespace {
+
+struct Dev {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* get(size_t bytes) {
+    if (bytes <= cap) return p;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+    cap = bytes;
+    return p;
+  }
+};
+
+struct State {
+  fr_ctx* ctx = nullptr;
+  Dev vol, spec, ctff, fsct, dm, shift, amag, pidx, out;
+};
+
+State& st() {
+  static State s;
+  return s;
+}
+
+[[noreturn]] void die(const char* routine, const char* what) {
+  printf(" %s (frealign_hip): %s\n", routine, what);
+  fflush(stdout);
+  exit(1);
+}
+
+void need(bool ok, const char* routine, const char* what) {
+  if (!ok) die(routine, what);
+}
+
+void check(int status, const char* routine) {
+  if (status != 0) die(routine, fr_last_error());
+}
+
+fr_ctx* ctx(const char* routine) {
+  State& s = st();
+  if (!s.ctx) check(fr_create(0, &s.ctx), routine);
+  return s.ctx;
+}
+
+void* upload(Dev& d, const void* host, size_t bytes, const char* routine) {
+  void* p = d.get(bytes);
+  need(p != nullptr, routine, "device allocation failed");
+  need(hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) == hipSuccess, routine,
+       "host-to-device copy failed");
+  return p;
+}
+
+void download(void* host, const void* dev, size_t bytes, const char* routine) {
+  need(hipMemcpy(host, dev, bytes, hipMemcpyDeviceToHost) == hipSuccess, routine,
+       "device-to-host copy failed");
+}
+
+void common_checks(const char* routine, int nsam, int irad, int ipad, int iewald) {
+  need(nsam >= 8 && nsam % 2 == 0, routine, "NSAM must be even and >= 8");
+  need(ipad == 1, routine, "only IPAD = 1 is supported");
+  need(irad != 0, routine, "only trilinear interpolation (IRAD > 0) is supported");
+  need(iewald == 0, routine, "Ewald-sphere extraction (IEWALD != 0) is not supported");
+}
+
+size_t half_plane(int nsam) { return (size_t)nsam * (nsam / 2 + 1); }
+size_t half_volume(int nsam) { return (size_t)nsam * nsam * (nsam / 2 + 1); }
+
+}  // namespace
+
+extern "C" {
+
+// cc3m.f:8-169
+float cc3m_(const int* nsam, const int* irad, const float* amag, const float* spec,
+            const float* a3df, const int* maxr1, const int* maxr2, const float* phi,
+            const float* theta, const float* psi, const float* shx, const float* shy,
+            const int* ibuf, float* /*pbuc*/, const float* rbfact, const float* /*sinclut*/,
+            const int* ipad, float* /*rbuf*/, const float* fsct, const int* iewald,
+            const float* /*thetatr*/, const float* ctff, const float* ri2, const float* ri3,
+            const float* rih, const float* halfw, const float* /*pweights*/,
+            const void* /*fftw_plans*/, const float* sm) {
+  const char* R = "CC3M";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  need(*ibuf == 0, R, "only IBUF = 0 is supported");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  void* df = upload(s.fsct, fsct, sizeof(float) * (n / 2 + 1), R);
+  float dmh[9];
+  fr::rotation(*phi, *theta, *psi, *amag, sm, dmh);  // cc3m.f:48-65
+  void* dd = upload(s.dm, dmh, sizeof dmh, R);
+  const float sh[2] = {*shx, *shy};
+  void* dsh = upload(s.shift, sh, sizeof sh, R);
+  const float am = fabsf(*amag);
+  void* da = upload(s.amag, &am, sizeof am, R);
+  const int p0 = 0;
+  void* dp = upload(s.pidx, &p0, sizeof p0, R);
+  void* dout = s.out.get(sizeof(float) * 8);
+  need(dout != nullptr, R, "device allocation failed");
+  const float ring[4] = {*ri2, *ri3, *rih, *halfw};
+  check(fr_cc3m_batch(c, n, 1, (const float*)dv, (const float*)ds, (const float*)dc,
+                      (const int*)dp, (const float*)dd, (const float*)dsh, (const float*)da,
+                      *maxr1, *maxr2, *rbfact, (const float*)df, ring, (float*)dout),
+        R);
+  check(fr_synchronize(c), R);
+  float score = 0.f;
+  download(&score, dout, sizeof score, R);
+  return score;
+}
+
+// pextract.f:8-101
+void pextract_(const int* nsam, const int* irad, float* outc, const float* a3df, const float* phi,
+               const float* theta, const float* psi, const float* /*sinclut*/, const int* ipad,
+               const float* /*thet*/, const int* iewald, const float* ctff, const float* sm) {
+  const char* R = "PEXTRACT";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  float dmh[9];
+  fr::rotation(*phi, *theta, *psi, 0.0f, sm, dmh);  // pextract.f:30-51
+  void* dd = upload(s.dm, dmh, sizeof dmh, R);
+  void* dout = s.spec.get(sizeof(float) * 2 * ic);
+  need(dout != nullptr, R, "device allocation failed");
+  check(fr_pextract_batch(c, n, 1, (const float*)dv, (const float*)dd, nullptr, (const float*)dc,
+                          (float*)dout),
+        R);
+  check(fr_synchronize(c), R);
+  download(outc, dout, sizeof(float) * 2 * ic, R);
+}
+
+// presb.f:8-160
+void presb_(const int* nsam, const int* irad, const float* amag, const float* spec,
+            const float* a3df, const int* /*maxr1*/, const int* /*maxr2*/, const float* phi,
+            const float* theta, const float* psi, const float* shx, const float* shy, float* rbin,
+            const float* /*sinclut*/, const int* ipad, int* /*ns*/, float* /*bufa*/,
+            float* /*bufb*/, float* /*bufab*/, const float* /*cs*/, const float* /*wl*/,
+            const float* /*wgh1*/, const float* /*wgh2*/, const float* /*df1*/,
+            const float* /*df2*/, const float* /*angast*/, const float* /*thet*/,
+            const int* iewald, const float* ctff, const float* ri2, const float* ri3,
+            const float* rih, const float* halfw, const char* asym, const void* /*fftw_plans*/,
+            const float* sm, size_t /*asym_len*/) {
+  const char* R = "PRESB";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  float dmh[9];
+  fr::rotation(*phi, *theta, *psi, *amag, sm, dmh);  // presb.f:47-62
+  void* dd = upload(s.dm, dmh, sizeof dmh, R);
+  const float sh[2] = {*shx, *shy};
+  void* dsh = upload(s.shift, sh, sizeof sh, R);
+  const float am = fabsf(*amag);
+  void* da = upload(s.amag, &am, sizeof am, R);
+  const int p0 = 0;
+  void* dp = upload(s.pidx, &p0, sizeof p0, R);
+  void* dr = upload(s.out, rbin, sizeof(float) * n, R);  // in/out: untouched shells keep values
+  const float ring[4] = {*ri2, *ri3, *rih, *halfw};
+  check(fr_presb_batch(c, n, 1, (const float*)dv, (const float*)ds, (const float*)dc,
+                       (const int*)dp, (const float*)dd, (const float*)dsh, (const float*)da, ring,
+                       (float*)dr),
+        R);
+  check(fr_synchronize(c), R);
+  download(rbin, dr, sizeof(float) * n, R);
+}
+
+// sigma.f:8-221
+void sigma_(const int* nsam, const int* irad, const float* amag, const float* a3df,
+            const float* datc, const float* phi, const float* theta, const float* psi,
+            const float* shx, const float* shy, const float* /*sinclut*/, const int* ipad,
+            const int* iewald, const float* /*thetatr*/, const float* ctff, float* /*rbuf*/,
+            const float* /*ri2*/, const int* maxr1, const int* /*maxr2*/, const float* signset,
+            float* sigma2n, float* sigma2, float* logpc, const float* dmask, const char* asym,
+            const int* /*k*/, const float* sigp, const float* /*pweights*/, const float* rclas,
+            const void* /*fftw_plans*/, const float* sm, size_t /*asym_len*/) {
+  const char* R = "SIGMA";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* ds = upload(s.spec, datc, sizeof(float) * 2 * ic, R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  const int p0 = 0;
+  const float par[5] = {*phi, *theta, *psi, *shx, *shy};
+  float out[3];
+  check(fr_sigma_batch(c, n, 1, (const float*)dv, (const float*)ds, (const float*)dc, &p0, par,
+                       amag, *maxr1, *rclas, dmask, *sigp, signset, sm, out),
+        R);
+  *sigma2n = out[0];
+  *sigma2 = out[1];
+  *logpc = out[2];
+}
+
+}  // extern "C"

@@ -114,9 +114,10 @@ module frealign_hip
 
     ! SIGMA (sigma.f:8-221)
     function fr_sigma_batch(ctx, nsam, n, d_a3df, d_datc, d_ctff, pidx, par, amag, maxr1, rclas, &
-                            dmask, sigp, signset, out) bind(C, name="fr_sigma_batch")
+                            dmask, sigp, signset, sm, out) bind(C, name="fr_sigma_batch")
       import :: c_int, c_float, c_ptr
       type(c_ptr), value :: ctx, d_a3df, d_datc, d_ctff
+      real(c_float) :: sm(9)
       integer(c_int), value :: nsam, n, maxr1
       real(c_float), value :: rclas, sigp
       integer(c_int) :: pidx(*)

@@ -22,6 +22,8 @@
 #ifndef FREALIGN_HIP_H
 #define FREALIGN_HIP_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -185,12 +187,13 @@ int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, i
  * phase units) for magnification amag[k], in the band MAXR1^2 <= r^2 <
  * (INT(NSAM*RCLAS*|AMAG|))^2 and the real-space disc DMASK(4)/AMAG around
  * the projection of the 3D mask centre DMASK(1..3).  signset (may be NULL):
- * the previous cycle's sigma per item (0: none).  out is host n x 3:
+ * the previous cycle's sigma per item (0: none).  sm (may be NULL: identity)
+ * is the symmetry-related matrix SM of the DMs.  out is host n x 3:
  * SIGMA2N, SIGMA2, LOGPC.  pidx, par, amag, dmask, signset are host arrays. */
 int fr_sigma_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_datc,
                    const float *d_ctff, const int *pidx, const float *par, const float *amag,
                    int maxr1, float rclas, const float *dmask, float sigp, const float *signset,
-                   float *out);
+                   const float *sm, float *out);
 
 /* ---------------------------------------------------------------- PREFINE */
 
@@ -288,6 +291,52 @@ int fr_orient_nearest(fr_ctx *ctx, int n, const float *d_dm, int ncls, const flo
                       int nsym, const float *d_sym, int *d_label);
 int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const float *d_dmb,
                      int nsym, const float *d_sym, float max_deg, unsigned *d_mask);
+
+/* --------------------------------- per-call entry points, Fortran symbols
+ * The reference's own routine names and argument lists (Fortran 77 implicit
+ * interfaces: every argument by reference, the CHARACTER length of ASYM
+ * appended), so a Fortran LMAIN links against this library unchanged.  Each
+ * call runs one item through the batched kernels (arrays copied per call) on
+ * a library-wide context on device 0; not thread-safe, as the reference's
+ * own routines are called serially.  Supported: IPAD = 1, IRAD > 0,
+ * IEWALD = 0, non-helical ASYM, IBUF = 0.  Other cases and device errors
+ * stop the program (message on stdout, exit(1)), as the reference's STOPs. */
+
+/* CC3M (cc3m.f:8-169): REAL FUNCTION */
+float cc3m_(const int *nsam, const int *irad, const float *amag, const float *spec,
+            const float *a3df, const int *maxr1, const int *maxr2, const float *phi,
+            const float *theta, const float *psi, const float *shx, const float *shy,
+            const int *ibuf, float *pbuc, const float *rbfact, const float *sinclut,
+            const int *ipad, float *rbuf, const float *fsct, const int *iewald,
+            const float *thetatr, const float *ctff, const float *ri2, const float *ri3,
+            const float *rih, const float *halfw, const float *pweights,
+            const void *fftw_plans, const float *sm);
+
+/* PEXTRACT (pextract.f:8-101) */
+void pextract_(const int *nsam, const int *irad, float *outc, const float *a3df, const float *phi,
+               const float *theta, const float *psi, const float *sinclut, const int *ipad,
+               const float *thet, const int *iewald, const float *ctff, const float *sm);
+
+/* PRESB (presb.f:8-160) */
+void presb_(const int *nsam, const int *irad, const float *amag, const float *spec,
+            const float *a3df, const int *maxr1, const int *maxr2, const float *phi,
+            const float *theta, const float *psi, const float *shx, const float *shy, float *rbin,
+            const float *sinclut, const int *ipad, int *ns, float *bufa, float *bufb,
+            float *bufab, const float *cs, const float *wl, const float *wgh1, const float *wgh2,
+            const float *df1, const float *df2, const float *angast, const float *thet,
+            const int *iewald, const float *ctff, const float *ri2, const float *ri3,
+            const float *rih, const float *halfw, const char *asym, const void *fftw_plans,
+            const float *sm, size_t asym_len);
+
+/* SIGMA (sigma.f:8-221) */
+void sigma_(const int *nsam, const int *irad, const float *amag, const float *a3df,
+            const float *datc, const float *phi, const float *theta, const float *psi,
+            const float *shx, const float *shy, const float *sinclut, const int *ipad,
+            const int *iewald, const float *thetatr, const float *ctff, float *rbuf,
+            const float *ri2, const int *maxr1, const int *maxr2, const float *signset,
+            float *sigma2n, float *sigma2, float *logpc, const float *dmask, const char *asym,
+            const int *k, const float *sigp, const float *pweights, const float *rclas,
+            const void *fftw_plans, const float *sm, size_t asym_len);
 
 /* ------------------------------------------- batches of uneven box sizes
  * No counterpart in the reference (its particles go one at a time through
