@@ -349,7 +349,8 @@ void sigma_(const int *nsam, const int *irad, const float *amag, const float *a3
  * fr_prep_batch) and recb/half (as fr_insert_batch; the CC3M uses PHI THETA
  * PSI DSHX DSHY AMAGP of recb).  score (host, may be NULL) receives the CC3M
  * of each particle.  Results equal the eager sequence of the three entry
- * points (scores bit for bit; insertion up to the order of float atomics).
+ * points (NKSUM exactly; scores and half-maps up to the order of float
+ * atomics in the shell and insertion sums).
  * The CC3M launch variant is the context's choice for NSAM at creation
  * (fr_autotune_cc3m). */
 typedef struct fr_plan fr_plan;

@@ -1,9 +1,10 @@
 """Graph-captured batches (fr_plan_*) and the CC3M autotuner.
 
 A plan's replay must equal the eager sequence fr_prep_batch ->
-fr_cc3m_batch -> fr_insert_batch on the same batch: CC3M scores bit for bit,
-NKSUM exactly, TTD and the half-maps up to the order of float atomics
-(1e-6 relative).  The graph is replayed several times on different batches,
+fr_cc3m_batch -> fr_insert_batch on the same batch: NKSUM exactly; CC3M
+scores, TTD and the half-maps up to the order of float atomics (the shell
+sums of PWEIGHT and CC3M accumulate in LDS with float atomics, so two runs
+may differ in the last bit: scores within 1e-6 relative, maps 1e-6 rel L2).  The graph is replayed several times on different batches,
 with eager work between the replays, and every replay is checked.  The
 autotuner's choice must be one of its measured candidates, and each CC3M
 variant it can choose must give the same scores (bit for bit between
@@ -76,7 +77,7 @@ def test_plan_replays_equal_eager(ctx, nsam, asym, nasym):
                            ring).cpu().numpy()
         ctx.insert(nsam, datc, ctff, recb, half, sym, 5.0, 40.0, psize, vols_e)
         torch.cuda.synchronize()
-        assert np.array_equal(score_p.view(np.uint32), score_e.view(np.uint32)), rep
+        assert np.allclose(score_p, score_e, rtol=1e-6, atol=0), rep
         assert np.array_equal(vols_p.nksum.cpu().numpy(), vols_e.nksum.cpu().numpy()), rep
         tp, te = vols_p.ttd.cpu().numpy(), vols_e.ttd.cpu().numpy()
         assert np.allclose(tp, te, rtol=1e-9), rep
@@ -85,7 +86,8 @@ def test_plan_replays_equal_eager(ctx, nsam, asym, nasym):
                 a, b = a.cpu().numpy().ravel(), b.cpu().numpy().ravel()
                 e = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
                 assert e < 1e-6, (rep, h, e)
-        print(f"nsam {nsam} {asym}{nasym} replay {rep}: scores identical, NKSUM "
+        print(f"nsam {nsam} {asym}{nasym} replay {rep}: max |dscore| "
+              f"{np.max(np.abs(score_p - score_e)):.1e}, NKSUM "
               f"{vols_p.nksum.cpu().numpy().tolist()}")
     plan.close()
 
