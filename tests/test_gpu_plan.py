@@ -7,8 +7,9 @@ sums of PWEIGHT and CC3M accumulate in LDS with float atomics, so two runs
 may differ in the last bit: scores within 1e-6 relative, maps 1e-6 rel L2).  The graph is replayed several times on different batches,
 with eager work between the replays, and every replay is checked.  The
 autotuner's choice must be one of its measured candidates, and each CC3M
-variant it can choose must give the same scores (bit for bit between
-persistent grid sizes of the fused kernel; 1e-4 against the FFT path).
+variant it can choose must give the same scores (1e-6 between persistent
+grid sizes of the fused kernel, whose shell sums use LDS float atomics; 1e-4
+against the FFT path).
 """
 import numpy as np
 import pytest
@@ -117,8 +118,8 @@ def test_autotune_cc3m(ctx):
     assert log["nsam"] == nsam and 0 <= log["choice"] < len(log["candidates"])
     assert all(c["ms"] > 0 for c in log["candidates"])
     chosen = ctx.cc3m(*args).cpu().numpy()
-    # every variant gives the same scores: bit for bit across the fused
-    # kernel's grid sizes, to FP32 rounding against the FFT path
+    # every variant gives the same scores: across the fused kernel's grid sizes
+    # up to the order of its LDS float atomics, to FP32 rounding against the FFT path
     fused = []
     for c in log["candidates"]:
         ctx.set_cc3m_variant(nsam, c["path"] == "fused",
@@ -128,4 +129,4 @@ def test_autotune_cc3m(ctx):
             fused.append(got)
         assert np.allclose(got, chosen, rtol=1e-4, atol=1e-6), c
     for got in fused[1:]:
-        assert np.array_equal(got.view(np.uint32), fused[0].view(np.uint32))
+        assert np.allclose(got, fused[0], rtol=1e-6, atol=0)
