@@ -361,7 +361,13 @@ def test_sheltest_vs_oracle(ctx, nsam, istp):
     err64 = np.max(np.abs(got["fsc"] - res["or64_"][3][:nst]))
     print(f"nsam={nsam} istp={istp}: max |FSC_gpu - FSC_fp32| = {err32:.1e}, vs FP64 {err64:.1e}")
     assert err32 < 1e-6 and err64 < 1e-6
-    np.testing.assert_allclose(got["pssnr"], pso[:nst], rtol=1e-5)
+    # PSSNR = 2|F|/(1-|F|) * npix / sum(S+V): its relative error is the FSC's
+    # absolute error times 1/(|F|(1-|F|)) (large where |F| nears 1), plus the
+    # FP32 summation order of S+V
+    f = np.abs(dfo[:nst])
+    cond = 1.0 / np.maximum(f * (1.0 - f), 1e-12)
+    bound = np.abs(pso[:nst]) * (1e-5 + 2e-6 * cond)
+    assert np.all(np.abs(got["pssnr"] - pso[:nst]) <= bound), (got["pssnr"], pso[:nst])
 
 
 def test_storeshift_vs_oracle(ctx):
