@@ -47,7 +47,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--fp32-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-views", type=int, default=16)
+    ap.add_argument("--cpu-views", type=int, default=48)
     ap.add_argument("--exh-steps", type=int, default=2, help="timed steps of the exhaustive BF16 search")
     ap.add_argument("--train-samples", type=int, default=200_000)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -72,6 +72,23 @@ PEAK_HBM = 8.0e12
 # products of the band-pruned transforms: column 1024 + rows 16 x 16 x 32 +
 # column 1024, 2048 flops each); the ring-mean pass is not counted
 CC3M_FLOPS = (1024 + 16 * 16 * 32 + 1024) * 2048.0
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "hbm_traffic_256_d7_pmc.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per launch of one kernel shape, as two rocprofv3 --pmc passes
+    (FETCH_SIZE, WRITE_SIZE, each scaled by a plain copy of known size)
+    measured them; the summary is a text file under profiles/.  None when that
+    shape was not measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {key}:"):
+                    return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(msg):
@@ -275,7 +292,9 @@ def main():
         achieved = flops / cc3m_avg_s
         roof = {"kernel": "k_cc3m_fused (FP32 MFMA)", "bound": "mfma", "achieved": achieved / 1e12,
                 "peak": PEAK_FP32_MFMA / 1e12, "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_MFMA,
-                "traffic": None, "launch_ms": cc3m_avg_s * 1e3,
+                "traffic": measured_traffic(f"k_cc3m_fused_{B * TOPK}"),
+                "traffic_source": os.path.relpath(TRAFFIC_FILE, os.path.dirname(TRAFFIC_FILE) + "/.."),
+                "launch_ms": cc3m_avg_s * 1e3,
                 "algorithmic_flops_per_launch": flops}
     else:
         flops = kc_flops * ncand
