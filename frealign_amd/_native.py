@@ -50,6 +50,11 @@ SIGNATURES = {
                         ci),
     "fr_search_list_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, ci, vp, ci, ci, vp, vp,
                               vp], ci),
+    "fr_search_ragged_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, vp, vp, vp, ci, ci, vp,
+                                vp, vp], ci),
+    "fr_build_lists": ([vp, ci, ci, vp, vp, cf, ci, vp, vp, ci, ci, ci, ci, vp, vp, vp, vp], ci),
+    "fr_rescore_batch": ([vp, ci, ci, ci, vp, vp, vp, vp, vp, ci, vp, ci, ci, ci, cf, vp, vp, vp,
+                          vp], ci),
     "fr_cnn_param_count": ([ci, ci, ci, ci, C.POINTER(C.c_longlong)], ci),
     "fr_cnn_init": ([vp, ci, ci, ci, ci, C.c_ulonglong, vp, vp], ci),
     "fr_cnn_forward": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),

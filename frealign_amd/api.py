@@ -303,6 +303,97 @@ class Context:
             _p(cand), int(group), topk, _p(top_o), _p(top_r), _p(top_s)), "fr_search_list_batch")
         return top_o, top_r, top_s
 
+    def build_lists(self, top: torch.Tensor, prob: torch.Tensor, threshold: float,
+                    cell_off: torch.Tensor, cells: torch.Tensor, group: int, nbank: int,
+                    bank_group: int, maxlen: int):
+        """fr_build_lists: ragged candidate lists from proposed classes (device
+        tensors in and out, no host synchronisation).  Returns (loff (np+1),
+        grp (np), cand (np*maxlen), info (np, 2): confident, valid entries)."""
+        _dev(top, torch.int32, "top")
+        _dev(prob, torch.float32, "prob")
+        _dev(cell_off, torch.int32, "cell_off")
+        _dev(cells, torch.int32, "cells")
+        if top.dim() != 2 or prob.shape != top.shape:
+            raise ValueError("top and prob must be (particles, ktop)")
+        if cell_off.numel() < 2:
+            raise ValueError("cell_off must hold ncls+1 offsets")
+        np_, ktop = top.shape
+        dev = top.device
+        loff = torch.empty(np_ + 1, dtype=torch.int32, device=dev)
+        grp = torch.empty(np_, dtype=torch.int32, device=dev)
+        cand = torch.empty(max(np_ * maxlen, 1), dtype=torch.int32, device=dev)
+        info = torch.empty((np_, 2), dtype=torch.int32, device=dev)
+        self._sync_stream()
+        N.check(N.lib().fr_build_lists(
+            self.h, np_, ktop, _p(top), _p(prob), float(threshold), cell_off.numel() - 1,
+            _p(cell_off), _p(cells), int(group), int(nbank), int(bank_group), int(maxlen),
+            _p(loff), _p(grp), _p(cand), _p(info)), "fr_build_lists")
+        return loff, grp, cand, info
+
+    def search_ragged(self, nsam: int, maxr1: int, maxr2: int, rbfact: float,
+                      bank: torch.Tensor, outc, ctff, loff: torch.Tensor, grp: torch.Tensor,
+                      cand: torch.Tensor, topk: int = 32):
+        """fr_search_ragged_batch: particle p against bank rows
+        cand[loff[p]:loff[p+1]] in view groups of grp[p]; returns (bank
+        orientation, shift row, score) of the topk best groups."""
+        _dev(outc, torch.complex64, "outc")
+        _dev(ctff, torch.complex64, "ctff")
+        for t, nm in ((loff, "loff"), (grp, "grp"), (cand, "cand")):
+            _dev(t, torch.int32, nm)
+        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
+            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        jc = nsam // 2 + 1
+        np_ = outc.numel() // (nsam * jc)
+        if ctff.numel() != 2 * outc.numel():
+            raise ValueError("ctff must hold two half-planes per particle")
+        if loff.numel() != np_ + 1 or grp.numel() != np_:
+            raise ValueError("loff must hold np+1 offsets and grp np group sizes")
+        _, kpad = self.search_layout(nsam, maxr1, maxr2)
+        if bank.dim() != 2 or bank.shape[1] != kpad:
+            raise ValueError(f"bank rows must be {kpad} long")
+        dev = outc.device
+        top_o = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_r = torch.empty((np_, topk), dtype=torch.int32, device=dev)
+        top_s = torch.empty((np_, topk), dtype=torch.float32, device=dev)
+        self._sync_stream()
+        N.check(N.lib().fr_search_ragged_batch(
+            self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0], _p(bank),
+            1 if bank.dtype == torch.bfloat16 else 0, np_, _p(outc), _p(ctff), _p(loff), _p(grp),
+            _p(cand), cand.numel(), topk, _p(top_o), _p(top_r), _p(top_s)),
+            "fr_search_ragged_batch")
+        return top_o, top_r, top_s
+
+    def rescore(self, nsam: int, a3df, outc, ctff, top_o: torch.Tensor, top_r: torch.Tensor,
+                dm_bank: torch.Tensor, nsamr: int, maxr1: int, maxr2: int, rbfact: float, fsct,
+                ring):
+        """fr_rescore_batch: FP32 CC3M of each particle's candidates and the
+        winner's position; returns (scores (np, k), -inf for empty candidates,
+        best (np))."""
+        for t, nm, dt in ((a3df, "a3df", torch.complex64), (outc, "outc", torch.complex64),
+                          (ctff, "ctff", torch.complex64), (top_o, "top_o", torch.int32),
+                          (top_r, "top_r", torch.int32), (dm_bank, "dm_bank", torch.float32),
+                          (fsct, "fsct", torch.float32)):
+            _dev(t, dt, nm)
+        if top_o.dim() != 2 or top_r.shape != top_o.shape:
+            raise ValueError("top_o and top_r must be (particles, k)")
+        jc = nsam // 2 + 1
+        if a3df.numel() != nsam * nsam * jc:
+            raise ValueError("a3df must be the NSAM x NSAM x JC half-volume")
+        if outc.numel() // (nsam * jc) != top_o.shape[0] or ctff.numel() != 2 * outc.numel():
+            raise ValueError("outc/ctff must hold one particle per candidate row")
+        if dm_bank.dim() != 2 or dm_bank.shape[1] != 9 or fsct.numel() < nsam // 2 + 1:
+            raise ValueError("dm_bank must be (nbank, 9); fsct NSAM/2+1 long")
+        np_, k = top_o.shape
+        r = np.asarray(ring, dtype=np.float32)
+        score = torch.empty((np_, k), dtype=torch.float32, device=outc.device)
+        best = torch.empty(np_, dtype=torch.int32, device=outc.device)
+        self._sync_stream()
+        N.check(N.lib().fr_rescore_batch(
+            self.h, nsam, np_, k, _p(a3df), _p(outc), _p(ctff), _p(top_o), _p(top_r),
+            dm_bank.shape[0], _p(dm_bank), int(nsamr), maxr1, maxr2, float(rbfact), _p(fsct),
+            _p(r), _p(score), _p(best)), "fr_rescore_batch")
+        return score, best
+
     # ---------------------------------------------------------------- SIGMA
     def sigma(self, nsam: int, a3df, datc, ctff, pidx, par, amag, maxr1: int, rclas: float,
               dmask, sigp: float, signset=None) -> np.ndarray:

@@ -24,6 +24,7 @@ SOURCES = [
     "fr_insert.hip",
     "fr_stats.hip",
     "fr_search.hip",
+    "fr_lists.hip",
     "fr_cnn.hip",
     "fr_refine.cpp",
     "fr_sigma.hip",

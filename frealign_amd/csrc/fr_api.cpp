@@ -18,6 +18,7 @@
 #include "fr_kernels.h"
 #include "fr_cnn.h"
 #include "fr_powell.h"
+#include "fr_lists.h"
 #include "fr_search.h"
 #include "fr_ctx.h"
 
@@ -196,7 +197,7 @@ int fr_destroy(fr_ctx* c) {
   if (!c) return 0;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (int k = 0; k < 16; k++)
+  for (int k = 0; k < fr_ctx::kSlots; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
   if (c->d_kidx) (void)hipFree(c->d_kidx);
   if (c->d_dft) (void)hipFree(c->d_dft);
@@ -574,18 +575,16 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
   if (!score) return err;
   int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * norient, &err);
   if (!brow) return err;
-  float* nrm = (float*)c->slot(2, sizeof(float) * (size_t)np * norient, &err);
-  if (!nrm) return err;
   if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
                                   d_ctff, c->d_kidx, ap, c2, sum2))
     return r;
-  if (int r = launch_search_norm(*c, norient, c->kpad, np, d_bank, dtype == 1, c2, nrm)) return r;
   int pslot = c->prof_begin(kProfSearchGemm);
   if (int r = launch_search_gemm(*c, norient, norient, c->kpad, np, d_bank, dtype == 1, nullptr,
-                                 ap, nrm, sum2, c->d_shx, c->d_shy, score, brow, nullptr))
+                                 ap, c2, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
-  return launch_topk(*c, np, norient, group, topk, score, brow, d_top_o, d_top_row, d_top_score);
+  return launch_topk(*c, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
+                     d_top_o, d_top_row, d_top_score);
 }
 
 // ------------------------------------------------ learned orientation proposal
@@ -723,23 +722,114 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
   if (!score) return err;
   int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * ncand, &err);
   if (!brow) return err;
-  float* nrm = (float*)c->slot(2, sizeof(float) * (size_t)np * ncand, &err);
-  if (!nrm) return err;
   if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
                                   d_ctff, c->d_kidx, ap, c2, sum2))
     return r;
-  if (int r = launch_search_norm_list(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand,
-                                      c2, nrm))
-    return r;
   int pslot = c->prof_begin(kProfSearchList);
   if (int r = launch_search_gemm(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand, ap,
-                                 nrm, sum2, c->d_shx, c->d_shy, score, brow, nullptr))
+                                 c2, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
-  if (int r = launch_topk(*c, np, ncand, group, topk, score, brow, d_top_o, d_top_row,
-                          d_top_score))
+  return launch_topk(*c, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
+                     d_top_o, d_top_row, d_top_score);
+}
+
+int fr_build_lists(fr_ctx* c, int np, int ktop, const int* d_top, const float* d_prob,
+                   float threshold, int ncls, const int* d_cell_off, const int* d_cells, int group,
+                   int nbank, int bank_group, int maxlen, int* d_loff, int* d_grp, int* d_cand,
+                   int* d_info) {
+  CTX_CHECK(c);
+  FR_CHECK(d_top && d_prob && d_cell_off && d_cells && d_loff && d_grp && d_cand && d_info,
+           "null argument");
+  FR_CHECK(np >= 0 && ktop >= 1 && ncls >= 1 && nbank >= 1, "bad sizes");
+  FR_CHECK(group >= 1 && bank_group >= 1 && nbank % bank_group == 0,
+           "the bank (%d) must come in whole view groups of %d", nbank, bank_group);
+  FR_CHECK(maxlen >= group && maxlen >= bank_group, "list capacity %d below one group", maxlen);
+  FR_CHECK((long long)np * maxlen < (1LL << 31), "list capacity %d x %d exceeds 2^31 entries",
+           np, maxlen);
+  if (np == 0) return 0;
+  int err = 0;
+  int* len = (int*)c->slot(16, sizeof(int) * (size_t)np, &err);
+  if (!len) return err;
+  return launch_list_build(*c, np, ktop, d_top, d_prob, threshold, ncls, d_cell_off, d_cells, group,
+                           nbank, bank_group, maxlen, len, d_loff, d_grp, d_info, d_cand);
+}
+
+int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int norient,
+                           const void* d_bank, int dtype, int np, const float* d_outc,
+                           const float* d_ctff, const int* d_loff, const int* d_grp,
+                           const int* d_cand, int cap, int topk, int* d_top_o, int* d_top_row,
+                           float* d_top_score) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_bank && d_outc && d_ctff && d_loff && d_grp && d_cand && d_top_o && d_top_row &&
+               d_top_score,
+           "null argument");
+  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(np >= 0 && norient > 0 && cap > 0 && topk >= 1 && topk <= 32, "bad sizes");
+  if (np == 0) return 0;
+  if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  const int half = c->kpad / 2;
+  int err = 0;
+  float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
+  if (!ap) return err;
+  float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
+  if (!c2) return err;
+  float* sum2 = (float*)c->slot(5, sizeof(float) * (size_t)np, &err);
+  if (!sum2) return err;
+  float* score = (float*)c->slot(6, sizeof(float) * (size_t)cap, &err);
+  if (!score) return err;
+  int* brow = (int*)c->slot(7, sizeof(int) * (size_t)cap, &err);
+  if (!brow) return err;
+  int* toff = (int*)c->slot(17, sizeof(int) * ((size_t)np + 1), &err);
+  if (!toff) return err;
+  if (int r = launch_tile_offsets(*c, np, d_loff, toff)) return r;
+  if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
+                                  d_ctff, c->d_kidx, ap, c2, sum2))
     return r;
-  return launch_map_cand(*c, np, ncand, topk, d_cand, d_top_o);
+  // tiles of lists within the capacity: at most cap/256 full tiles plus one
+  // partial tile per particle
+  const long long max_tiles = (long long)cap / 256 + np;
+  int pslot = c->prof_begin(kProfSearchList);
+  if (int r = launch_search_gemm_ragged(*c, np, norient, c->kpad, d_bank, dtype == 1, d_loff, toff,
+                                        d_cand, cap, (int)(max_tiles < (1LL << 30) ? max_tiles
+                                                                                     : (1LL << 30)),
+                                        ap, c2, sum2, c->d_shx, c->d_shy, score, brow))
+    return r;
+  c->prof_end(pslot);
+  return launch_topk(*c, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
+                     d_top_row, d_top_score);
+}
+
+int fr_rescore_batch(fr_ctx* c, int nsam, int np, int k, const float* d_a3df, const float* d_outc,
+                     const float* d_ctff, const int* d_top_o, const int* d_top_row, int nbank,
+                     const float* d_dm_bank, int nsamr, int maxr1, int maxr2, float rbfact,
+                     const float* d_fsct, const float* ring, float* d_score, int* d_best) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_outc && d_ctff && d_top_o && d_top_row && d_dm_bank && d_fsct && ring &&
+               d_score && d_best,
+           "null argument");
+  FR_CHECK(np >= 0 && k >= 1 && nbank >= 1 && nsamr >= 2, "bad sizes");
+  FR_CHECK((long long)np * k < (1LL << 31), "too many candidates");
+  if (np == 0) return 0;
+  const size_t n = (size_t)np * k;
+  int err = 0;
+  int* pidx = (int*)c->slot(18, sizeof(int) * n, &err);
+  if (!pidx) return err;
+  float* dm = (float*)c->slot(19, sizeof(float) * 9 * n, &err);
+  if (!dm) return err;
+  float* sh = (float*)c->slot(20, sizeof(float) * 2 * n, &err);
+  if (!sh) return err;
+  float* am = (float*)c->slot(21, sizeof(float) * n, &err);
+  if (!am) return err;
+  if (int r = launch_cand_items(*c, np, k, d_top_o, d_top_row, nbank, d_dm_bank, nsamr, pidx, dm,
+                                sh, am))
+    return r;
+  if (int r = fr_cc3m_batch(c, nsam, (int)n, d_a3df, d_outc, d_ctff, pidx, dm, sh, am, maxr1,
+                            maxr2, rbfact, d_fsct, ring, d_score))
+    return r;
+  return launch_best_cand(*c, np, k, d_top_o, d_score, d_best);
 }
 
 // ------------------------------------------------------------------ PREFINE

@@ -23,6 +23,7 @@ int fail(const char* fmt, ...);
 // ---------------------------------------------------------------- context
 struct Ctx {
   int device = 0;
+  int cus = 0;  // compute units of the device (filled on first need)
   hipStream_t stream = nullptr;
   // grow-only device scratch owned by the context
   void* scratch = nullptr;

@@ -8,8 +8,9 @@
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  void* slots[16] = {};
-  size_t slot_bytes[16] = {};
+  static constexpr int kSlots = 24;
+  void* slots[kSlots] = {};
+  size_t slot_bytes[kSlots] = {};
   void* slot(int k, size_t bytes, int* err);
   // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
   int skey[3] = {0, 0, 0};

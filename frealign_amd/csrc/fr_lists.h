@@ -1,0 +1,15 @@
+// fr_lists.h -- launchers of the ragged candidate lists and rescoring items.
+#pragma once
+#include "fr_common.h"
+
+namespace fr {
+int launch_list_build(Ctx& c, int np, int ktop, const int* top, const float* prob, float threshold,
+                      int ncls, const int* cell_off, const int* cells, int group, int nbank,
+                      int bank_group, int maxlen, int* len_scratch, int* loff, int* grp, int* info,
+                      int* cand);
+int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff);
+int launch_cand_items(Ctx& c, int np, int k, const int* top_o, const int* top_row, int nbank,
+                      const float* dm_bank, int nsamr, int* pidx, float* dm, float* shift,
+                      float* amag);
+int launch_best_cand(Ctx& c, int np, int k, const int* top_o, float* score, int* best);
+}  // namespace fr

@@ -131,97 +131,6 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) sum2[p] = red[0] + red[1] + red[2] + red[3];
 }
 
-// ---------------------------------------------------------------- norms
-// sum_k |c_p(k)|^2 |T_o(k)|^2 for every (particle, orientation): the
-// normalisation SUM of ccp.f:211-214 with the CTF folded on the particle.
-// Tiles of 64 particles x 128 templates; K staged through LDS.
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    k_search_norm(int norient, int kpad, int np, const T* __restrict__ bank,
-                  const float* __restrict__ c2, float* nrm) {
-  constexpr int TP = 64, TO = 128, KS = 32;  // particles, templates, complex per step
-  __shared__ float cs[TP][KS + 1];
-  __shared__ float ts[TO][KS + 1];
-  const int half = kpad / 2;
-  const int o0 = blockIdx.x * TO, p0 = blockIdx.y * TP;
-  const int tid = threadIdx.x;
-  // thread -> (4 particles) x (8 templates)
-  const int tp = (tid & 15) * 4, to = (tid >> 4) * 8;
-  float acc[4][8] = {};
-  for (int k0 = 0; k0 < half; k0 += KS) {
-    for (int e = tid; e < TP * KS; e += kThreads) {
-      int pp = e / KS, kk = e % KS;
-      int p = p0 + pp, k = k0 + kk;
-      cs[pp][kk] = (p < np && k < half) ? c2[(size_t)p * half + k] : 0.f;
-    }
-    for (int e = tid; e < TO * KS; e += kThreads) {
-      int oo = e / KS, kk = e % KS;
-      int o = o0 + oo, k = k0 + kk;
-      float v = 0.f;
-      if (o < norient && k < half) {
-        float re = Elem<T>::f(bank[(size_t)o * kpad + 2 * k]);
-        float im = Elem<T>::f(bank[(size_t)o * kpad + 2 * k + 1]);
-        v = re * re + im * im;
-      }
-      ts[oo][kk] = v;
-    }
-    __syncthreads();
-#pragma unroll 4
-    for (int kk = 0; kk < KS; kk++) {
-      float a[4], b[8];
-#pragma unroll
-      for (int i = 0; i < 4; i++) a[i] = cs[tp + i][kk];
-#pragma unroll
-      for (int j = 0; j < 8; j++) b[j] = ts[to + j][kk];
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 8; j++) acc[i][j] += a[i] * b[j];
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-#pragma unroll
-    for (int j = 0; j < 8; j++) {
-      int p = p0 + tp + i, o = o0 + to + j;
-      if (p < np && o < norient) nrm[(size_t)p * norient + o] = acc[i][j];
-    }
-}
-
-// The same normalisation for per-particle candidate lists: one workgroup per
-// (256 candidates, particle), |c_p|^2 staged in LDS.
-template <typename T>
-__global__ void __launch_bounds__(kThreads)
-    k_search_norm_list(int ncand, int nbank, int kpad, const T* __restrict__ bank,
-                       const int* __restrict__ cand, const float* __restrict__ c2, float* nrm) {
-  extern __shared__ float c2s[];
-  const int half = kpad / 2, p = blockIdx.y;
-  for (int k = threadIdx.x; k < half; k += kThreads) c2s[k] = c2[(size_t)p * half + k];
-  __syncthreads();
-  const int col = blockIdx.x * kThreads + threadIdx.x;
-  if (col >= ncand) return;
-  const int o = cand[(size_t)p * ncand + col];
-  float acc = 0.f;
-  if (o >= 0 && o < nbank) {
-    const T* row = bank + (size_t)o * kpad;
-    for (int k = 0; k < half; k++) {
-      float re = Elem<T>::f(row[2 * k]), im = Elem<T>::f(row[2 * k + 1]);
-      acc += c2s[k] * (re * re + im * im);
-    }
-  }
-  nrm[(size_t)p * ncand + col] = acc;
-}
-
-// candidate positions -> bank orientation indices
-__global__ void __launch_bounds__(kThreads)
-    k_map_cand(int np, int ncand, int topk, const int* __restrict__ cand, int* top_o) {
-  const int i = blockIdx.x * kThreads + threadIdx.x;
-  if (i >= np * topk) return;
-  const int p = i / topk, t = top_o[i];
-  top_o[i] = (t >= 0 && t < ncand) ? cand[(size_t)p * ncand + t] : -1;
-}
-
 // ---------------------------------------------------------------- GEMM
 constexpr int kGemmThreads = 512;  // 8 waves: 4 row groups x 2 column groups
 
@@ -268,19 +177,24 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
   }
 }
 
-// LDS layout of k_search_gemm: two operand buffers (A rows x P, B cols x P)
-// and three phase-table buffers.  The epilogue map [kRows][65] floats
-// aliases the start of the region.
+// LDS layout of one GEMM tile: two operand buffers (A rows x P, B cols x P)
+// and three table buffers, each holding the chunk's 16x15 X'(k) e^{i I sx}
+// and 16x15 e^{i J sy} phase entries and its 16 weights |c_p(k)|^2.  The
+// epilogue map [kRows][65] floats aliases the operand buffers; the column
+// norms' partial sums alias the table buffers.
 template <typename T>
 struct Lds {
   static constexpr size_t a_bytes = (size_t)kRows * Elem<T>::kPad * sizeof(T);
   static constexpr size_t b_bytes = (size_t)kCols * Elem<T>::kPad * sizeof(T);
   static constexpr size_t op_bytes = a_bytes + b_bytes;  // one buffer
-  static constexpr size_t tab_one = 2 * 16 * kShift * sizeof(float2);
+  static constexpr int tab_entries = 2 * 16 * kShift + 16;
+  static constexpr size_t tab_one = tab_entries * sizeof(float2);
   static constexpr size_t tab_off = 2 * op_bytes;
   static constexpr size_t ep_bytes = (size_t)kRows * 65 * sizeof(float);
   static constexpr size_t end_tab = tab_off + 3 * tab_one;
   static constexpr size_t total = end_tab > ep_bytes ? end_tab : ep_bytes;
+  static_assert(ep_bytes <= tab_off, "epilogue map overlaps the tables");
+  static_assert(3 * tab_one >= kGemmThreads * sizeof(float), "norm partials exceed the tables");
 };
 
 template <typename T>
@@ -297,27 +211,24 @@ struct BPieces {
   static constexpr int elems = 16 / sizeof(T);              // elements per piece
 };
 
+// One tile: the 225 shift rows of particle p against columns o0..o0+255 of
+// its template list (list entry cand_p[o], or o itself when cand_p is null;
+// entries outside 0..nbank-1 are empty columns).  Alongside the MFMA work the
+// tile accumulates each column's normalisation sum_k |c_p(k)|^2 |T_o(k)|^2
+// (SUM of ccp.f:211-214 with the CTF folded on the particle) from the
+// template chunks already in LDS.  Outputs are indexed by list position.
 template <typename T>
-__global__ void __launch_bounds__(kGemmThreads)
-    k_search_gemm(int nrow, int nbank, int kpad, int np, int ntile, const T* __restrict__ bank,
-                  const int* __restrict__ cand, const float2* __restrict__ ap,
-                  const float* __restrict__ nrm_po, const float* __restrict__ sum2,
-                  const float2* __restrict__ shx_tab, const float2* __restrict__ shy_tab,
-                  float* score, int* best_row, float* raw_peak) {
+__device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int ncol, int nbank,
+                                          int kpad, const T* __restrict__ bank,
+                                          const int* __restrict__ cand_p,
+                                          const float2* __restrict__ apv,
+                                          const float* __restrict__ c2p, float s2,
+                                          const float2* __restrict__ shx_tab,
+                                          const float2* __restrict__ shy_tab, float* score_p,
+                                          int* brow_p) {
   constexpr int P = Elem<T>::kPad;
   using L = Lds<T>;
   using BP = BPieces<T>;
-  // XCD-aware mapping: blocks are dealt round-robin to the 8 XCDs; each XCD
-  // walks its own template tiles with the particles innermost, so a template
-  // tile stays resident in that XCD's L2 while every particle uses it.
-  const int b = blockIdx.x;
-  const int xcd = b & 7, j = b >> 3;
-  const int tile = (j / np) * 8 + xcd;
-  const int p = j % np;
-  if (tile >= ntile) return;
-  const int o0 = tile * kCols;
-
-  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
   auto As = [&](int buf) { return reinterpret_cast<T*>(smem_raw + buf * L::op_bytes); };
   auto Bs = [&](int buf) { return reinterpret_cast<T*>(smem_raw + buf * L::op_bytes + L::a_bytes); };
   auto tabx = [&](int buf) {
@@ -329,13 +240,13 @@ __global__ void __launch_bounds__(kGemmThreads)
   const int wr = wave & 3, wc = wave >> 2;
   const int half = kpad / 2;
   const int nchunk = half / 16;
-  const float2* apv = ap + (size_t)p * half;
 
   floatx4 acc[4][8];
 #pragma unroll
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
+  float nacc = 0.f;  // norm partial: column tid/2, complex 8*(tid%2)..+7 of each chunk
 
   // this thread's B pieces: piece bq of templates bc0 + u * bstep
   const int bq = tid % BP::per_row;
@@ -344,13 +255,11 @@ __global__ void __launch_bounds__(kGemmThreads)
   uint4 breg[BP::per_thread];
   float2 treg;
 
-  // bank rows of this thread's columns: the column itself, or the
-  // particle's candidate list entry (-1: an empty column)
   int orow[BP::per_thread];
 #pragma unroll
   for (int u = 0; u < BP::per_thread; u++) {
     int col = o0 + bc0 + u * bstep;
-    int o = col < nrow ? (cand ? cand[(size_t)p * nrow + col] : col) : -1;
+    int o = col < ncol ? (cand_p ? cand_p[col] : col) : -1;
     orow[u] = (o >= 0 && o < nbank) ? o : -1;
   }
   auto load_b = [&](int chunk) {
@@ -369,7 +278,7 @@ __global__ void __launch_bounds__(kGemmThreads)
     for (int u = 0; u < BP::per_thread; u++)
       *reinterpret_cast<uint4*>(dst + (bc0 + u * bstep) * P + bq * BP::elems) = breg[u];
   };
-  // one phase-table entry per thread: 240 X'(k) e^{i I sx} then 240 e^{i J sy}
+  // one table entry per thread: 240 X'(k) e^{i I sx}, 240 e^{i J sy}, 16 |c(k)|^2
   auto load_tab = [&](int chunk) {
     const int k0 = 16 * chunk;
     float2 v = make_float2(0.f, 0.f);
@@ -381,11 +290,13 @@ __global__ void __launch_bounds__(kGemmThreads)
     } else if (tid < 32 * kShift) {
       int e2 = tid - 16 * kShift;
       v = shy_tab[(size_t)(k0 + e2 / kShift) * kShift + e2 % kShift];
+    } else if (tid < L::tab_entries) {
+      v = make_float2(c2p[k0 + tid - 32 * kShift], 0.f);
     }
     treg = v;
   };
   auto store_tab = [&](int tbuf) {
-    if (tid < 32 * kShift) tabx(tbuf)[tid] = treg;  // tabx and taby are contiguous
+    if (tid < L::tab_entries) tabx(tbuf)[tid] = treg;
   };
   // A chunk: row (sx, sy) = X'(k) e^{i k.s}; two threads per row, 8 complex each
   const int row = tid >> 1, hk = (tid & 1) * 8;
@@ -413,6 +324,21 @@ __global__ void __launch_bounds__(kGemmThreads)
       *reinterpret_cast<uint4*>(dst + g * E) = *reinterpret_cast<const uint4*>(v);
     }
   };
+  // norm: |T|^2 of this thread's 8 complex of the chunk in buf, weighted by |c|^2
+  auto norm_step = [&](int buf, int tbuf) {
+    constexpr int E = 16 / sizeof(T);
+    const T* src = Bs(buf) + row * P + 2 * hk;
+    const float2* w = tabx(tbuf) + 32 * kShift + hk;
+    T v[16];
+#pragma unroll
+    for (int g = 0; g < 16 / E; g++)
+      *reinterpret_cast<uint4*>(v + g * E) = *reinterpret_cast<const uint4*>(src + g * E);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float re = Elem<T>::f(v[2 * q]), im = Elem<T>::f(v[2 * q + 1]);
+      nacc += w[q].x * (re * re + im * im);
+    }
+  };
 
   // prologue: tables for chunks 0 and 1, operands for chunk 0
   load_tab(0);
@@ -433,6 +359,7 @@ __global__ void __launch_bounds__(kGemmThreads)
     if (more) load_b(kc + 1);
     if (kc + 2 < nchunk) load_tab(kc + 2);
     mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
+    norm_step(cur, kc % 3);
     if (more) {
       gen_a(nxt, (kc + 1) % 3);
       store_b(nxt);
@@ -443,7 +370,8 @@ __global__ void __launch_bounds__(kGemmThreads)
 
   // ------------------------------------------------------------ epilogue
   float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
-  const float s2 = sum2[p];
+  float* nl = reinterpret_cast<float*>(smem_raw + L::tab_off);
+  nl[tid] = nacc;
 #pragma unroll
   for (int q4 = 0; q4 < 4; q4++) {  // 64-column quarters of the 256 templates
     if (q4) __syncthreads();
@@ -461,7 +389,7 @@ __global__ void __launch_bounds__(kGemmThreads)
     }
     __syncthreads();
     if (tid < 64) {
-      const int c = tid, o = o0 + q4 * 64 + c;
+      const int c = tid, cc = q4 * 64 + c, o = o0 + cc;
       // ccp.f:229-241: argmax over IX (outer) and IY (inner) in -6..6, first max wins
       float best = -3.0e38f;
       int bx = 0, by = 0;
@@ -474,15 +402,73 @@ __global__ void __launch_bounds__(kGemmThreads)
       float box = 0.f;
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
-      if (o < nrow) {
-        size_t out = (size_t)p * nrow + o;
-        float n = nrm_po[out];
-        float cc = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
-        score[out] = cc;
-        best_row[out] = (bx + 7) * kShift + (by + 7);
-        if (raw_peak) raw_peak[out] = best;
+      if (o < ncol) {
+        float n = nl[2 * cc] + nl[2 * cc + 1];
+        float v = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
+        score_p[o] = v;
+        brow_p[o] = (bx + 7) * kShift + (by + 7);
       }
     }
+  }
+  __syncthreads();  // LDS is reused by the next tile
+}
+
+// Uniform lists: nrow columns per particle (the whole bank when cand is
+// null, else the particle's row of cand).  XCD-aware mapping: blocks are
+// dealt round-robin to the 8 XCDs; each XCD walks its own template tiles with
+// the particles innermost, so a template tile stays resident in that XCD's L2
+// while every particle uses it.
+template <typename T>
+__global__ void __launch_bounds__(kGemmThreads)
+    k_search_gemm(int nrow, int nbank, int kpad, int np, int ntile, const T* __restrict__ bank,
+                  const int* __restrict__ cand, const float2* __restrict__ ap,
+                  const float* __restrict__ c2, const float* __restrict__ sum2,
+                  const float2* __restrict__ shx_tab, const float2* __restrict__ shy_tab,
+                  float* score, int* best_row) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int b = blockIdx.x;
+  const int xcd = b & 7, j = b >> 3;
+  const int tile = (j / np) * 8 + xcd;
+  const int p = j % np;
+  if (tile >= ntile) return;
+  const int half = kpad / 2;
+  const size_t base = (size_t)p * nrow;
+  gemm_tile<T>(smem_raw, tile * kCols, nrow, nbank, kpad, bank, cand ? cand + base : nullptr,
+               ap + (size_t)p * half, c2 + (size_t)p * half, sum2[p], shx_tab, shy_tab,
+               score + base, best_row + base);
+}
+
+// Ragged lists: particle p's list is cand[loff[p] .. loff[p+1]) and covers
+// tiles toff[p] .. toff[p+1]) of the launch; list positions at or beyond
+// `cap` (the capacity of cand, score and best_row) are neither read nor
+// written.  Persistent: each workgroup walks tiles blockIdx.x,
+// blockIdx.x + gridDim.x, ... up to toff[np].
+template <typename T>
+__global__ void __launch_bounds__(kGemmThreads)
+    k_search_gemm_ragged(int np, int nbank, int kpad, const T* __restrict__ bank,
+                         const int* __restrict__ loff, const int* __restrict__ toff,
+                         const int* __restrict__ cand, int cap, const float2* __restrict__ ap,
+                         const float* __restrict__ c2, const float* __restrict__ sum2,
+                         const float2* __restrict__ shx_tab, const float2* __restrict__ shy_tab,
+                         float* score, int* best_row) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
+  const int total = toff[np];
+  const int half = kpad / 2;
+  for (int t = blockIdx.x; t < total; t += gridDim.x) {
+    // the particle whose tile range holds t: the largest p with toff[p] <= t
+    int lo = 0, hi = np - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (toff[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int p = lo;
+    const int base = loff[p];
+    const int end = loff[p + 1] < cap ? loff[p + 1] : cap;
+    if (base < 0 || end <= base) continue;  // uniform across the workgroup
+    gemm_tile<T>(smem_raw, (t - toff[p]) * kCols, end - base, nbank, kpad, bank, cand + base,
+                 ap + (size_t)p * half, c2 + (size_t)p * half, sum2[p], shx_tab, shy_tab,
+                 score + base, best_row + base);
   }
 }
 
@@ -490,28 +476,40 @@ __global__ void __launch_bounds__(kGemmThreads)
 // CCP returns one (psi, shift) per view: the first maximum over its in-plane
 // angles in (IPSI outer, IQUAD inner) order (ccp.f:144-264); PSEARCH then
 // keeps the IPMAX best views by strict '>' in grid order (psearch.f:89-102),
-// so ties keep the earlier view.  Orientations come in groups of `group`
-// consecutive entries per view.  One workgroup per particle: each thread
-// keeps a sorted top-K of its views, then a bitonic merge.
+// so ties keep the earlier view.  A particle's list comes in groups of
+// `group` consecutive entries per view.  One workgroup per particle: each
+// thread keeps a sorted top-K of its views, then a bitonic merge.  Uniform
+// lists: n entries per particle, group `group`; ragged lists (loff non-null):
+// particle p's entries loff[p]..min(loff[p+1], cap)) in groups of grp[p].  Output
+// indices are list positions, mapped to bank orientations through cand when
+// it is non-null.
 __global__ void __launch_bounds__(kThreads)
-    k_topk(int norient, int group, int topk, const float* __restrict__ score,
+    k_topk(int n, int group, const int* __restrict__ loff, const int* __restrict__ grp,
+           const int* __restrict__ cand, int cap, int topk, const float* __restrict__ score,
            const int* __restrict__ best_row, int* top_o, int* top_row, float* top_s) {
   constexpr int kK = 32;
   __shared__ float sv[kThreads * kK];
   __shared__ int si[kThreads * kK];
   const int p = blockIdx.x, tid = threadIdx.x;
-  const float* sc = score + (size_t)p * norient;
-  const int ngroup = norient / group;
+  const size_t base = loff ? (size_t)loff[p] : (size_t)p * n;
+  int len = n;
+  if (loff) {
+    const int end = loff[p + 1] < cap ? loff[p + 1] : cap;
+    len = (loff[p] >= 0 && end > loff[p]) ? end - loff[p] : 0;
+  }
+  const int gsz = grp ? grp[p] : group;
+  const float* sc = score + base;
+  const int ngroup = gsz > 0 ? len / gsz : 0;
   float v[kK];
   int ix[kK];
 #pragma unroll
   for (int q = 0; q < kK; q++) { v[q] = -3.4e38f; ix[q] = 0x7fffffff; }
   for (int g = tid; g < ngroup; g += kThreads) {
     float x = -3.4e38f;
-    int xo = g * group;
-    for (int u = 0; u < group; u++) {
-      float y = sc[g * group + u];
-      if (y > x) { x = y; xo = g * group + u; }
+    int xo = g * gsz;
+    for (int u = 0; u < gsz; u++) {
+      float y = sc[g * gsz + u];
+      if (y > x) { x = y; xo = g * gsz + u; }
     }
     if (!(x > v[kK - 1])) continue;
     int q = kK - 1;
@@ -523,10 +521,10 @@ __global__ void __launch_bounds__(kThreads)
   for (int q = 0; q < kK; q++) { sv[tid * kK + q] = v[q]; si[tid * kK + q] = ix[q]; }
   __syncthreads();
   // bitonic sort of kThreads*kK entries, descending score, ascending index on ties
-  const int n = kThreads * kK;
-  for (int size = 2; size <= n; size <<= 1)
+  const int ns = kThreads * kK;
+  for (int size = 2; size <= ns; size <<= 1)
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      for (int e = tid; e < n / 2; e += kThreads) {
+      for (int e = tid; e < ns / 2; e += kThreads) {
         int lo = 2 * e - (e & (stride - 1));
         int hi = lo + stride;
         bool desc = ((lo & size) == 0);
@@ -541,10 +539,11 @@ __global__ void __launch_bounds__(kThreads)
       __syncthreads();
     }
   for (int q = tid; q < topk; q += kThreads) {
-    int o = si[q];
-    bool ok = o >= 0 && o < norient;
-    top_o[(size_t)p * topk + q] = ok ? o : -1;
-    top_row[(size_t)p * topk + q] = ok ? best_row[(size_t)p * norient + o] : -1;
+    int t = si[q];
+    bool ok = t >= 0 && t < len;
+    int o = ok ? (cand ? cand[base + t] : t) : -1;
+    top_o[(size_t)p * topk + q] = o;
+    top_row[(size_t)p * topk + q] = ok ? best_row[base + t] : -1;
     top_s[(size_t)p * topk + q] = sv[q];
   }
 }
@@ -579,80 +578,89 @@ int launch_search_feats(Ctx& c, int nsam, int np, int kc, int kpad, int maxr22, 
   return 0;
 }
 
-int launch_search_norm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
-                       const float* c2, float* nrm) {
-  dim3 grid((norient + 127) / 128, (np + 63) / 64);
-  if (bf16)
-    hipLaunchKernelGGL(k_search_norm<__bf16>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
-                       reinterpret_cast<const __bf16*>(bank), c2, nrm);
-  else
-    hipLaunchKernelGGL(k_search_norm<float>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
-                       reinterpret_cast<const float*>(bank), c2, nrm);
-  FR_HIP(hipGetLastError());
+namespace {
+template <typename T>
+int set_gemm_lds(const void* fn, size_t lds) {
+  FR_CHECK(lds <= 160 * 1024, "LDS budget exceeded (%zu)", lds);
+  FR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   return 0;
 }
-
-int launch_search_norm_list(Ctx& c, int ncand, int nbank, int kpad, int np, const void* bank,
-                            bool bf16, const int* cand, const float* c2, float* nrm) {
-  dim3 grid((ncand + kThreads - 1) / kThreads, np);
-  const size_t lds = sizeof(float) * (kpad / 2);
-  FR_CHECK(lds <= 64 * 1024, "annulus too large for the list normalisation");
-  if (bf16)
-    hipLaunchKernelGGL(k_search_norm_list<__bf16>, grid, dim3(kThreads), lds, c.stream, ncand,
-                       nbank, kpad, reinterpret_cast<const __bf16*>(bank), cand, c2, nrm);
-  else
-    hipLaunchKernelGGL(k_search_norm_list<float>, grid, dim3(kThreads), lds, c.stream, ncand,
-                       nbank, kpad, reinterpret_cast<const float*>(bank), cand, c2, nrm);
-  FR_HIP(hipGetLastError());
-  return 0;
-}
-
-int launch_map_cand(Ctx& c, int np, int ncand, int topk, const int* cand, int* top_o) {
-  const int n = np * topk;
-  hipLaunchKernelGGL(k_map_cand, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, c.stream,
-                     np, ncand, topk, cand, top_o);
-  FR_HIP(hipGetLastError());
-  return 0;
-}
+}  // namespace
 
 int launch_search_gemm(Ctx& c, int nrow, int nbank, int kpad, int np, const void* bank,
-                       bool bf16, const int* cand, const float* ap, const float* nrm, const float* sum2,
-                       const float* shx_tab, const float* shy_tab, float* score, int* best_row,
-                       float* raw_peak) {
+                       bool bf16, const int* cand, const float* ap, const float* c2,
+                       const float* sum2, const float* shx_tab, const float* shy_tab, float* score,
+                       int* best_row) {
   const int ntile = (nrow + kCols - 1) / kCols;
   const int ntile8 = (ntile + 7) / 8 * 8;
   const size_t nblk = (size_t)ntile8 * np;
   FR_CHECK(nblk < (1u << 31), "search grid too large");
   const size_t lds = search_gemm_lds(bf16);
-  FR_CHECK(lds <= 160 * 1024, "LDS budget exceeded (%zu)", lds);
+  const float2* apv = reinterpret_cast<const float2*>(ap);
+  const float2* sx = reinterpret_cast<const float2*>(shx_tab);
+  const float2* sy = reinterpret_cast<const float2*>(shy_tab);
   if (bf16) {
-    FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<__bf16>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (int r = set_gemm_lds<__bf16>((const void*)k_search_gemm<__bf16>, lds)) return r;
     hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, nrow, nbank, kpad, np, ntile, reinterpret_cast<const __bf16*>(bank), cand,
-                       reinterpret_cast<const float2*>(ap), nrm, sum2,
-                       reinterpret_cast<const float2*>(shx_tab),
-                       reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
+                       c.stream, nrow, nbank, kpad, np, ntile,
+                       reinterpret_cast<const __bf16*>(bank), cand, apv, c2, sum2, sx, sy, score,
+                       best_row);
   } else {
-    FR_HIP(hipFuncSetAttribute((const void*)k_search_gemm<float>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (int r = set_gemm_lds<float>((const void*)k_search_gemm<float>, lds)) return r;
     hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, nrow, nbank, kpad, np, ntile, reinterpret_cast<const float*>(bank), cand,
-                       reinterpret_cast<const float2*>(ap), nrm, sum2,
-                       reinterpret_cast<const float2*>(shx_tab),
-                       reinterpret_cast<const float2*>(shy_tab), score, best_row, raw_peak);
+                       c.stream, nrow, nbank, kpad, np, ntile,
+                       reinterpret_cast<const float*>(bank), cand, apv, c2, sum2, sx, sy, score,
+                       best_row);
   }
   FR_HIP(hipGetLastError());
   return 0;
 }
 
-int launch_topk(Ctx& c, int np, int norient, int group, int topk, const float* score,
-                const int* best_row, int* top_o, int* top_row, float* top_s) {
+int launch_search_gemm_ragged(Ctx& c, int np, int nbank, int kpad, const void* bank, bool bf16,
+                              const int* loff, const int* toff, const int* cand, int cap,
+                              int max_tiles, const float* ap, const float* c2, const float* sum2,
+                              const float* shx_tab, const float* shy_tab, float* score,
+                              int* best_row) {
+  const size_t lds = search_gemm_lds(bf16);
+  const void* fn = bf16 ? (const void*)k_search_gemm_ragged<__bf16>
+                        : (const void*)k_search_gemm_ragged<float>;
+  if (int r = bf16 ? set_gemm_lds<__bf16>(fn, lds) : set_gemm_lds<float>(fn, lds)) return r;
+  // persistent grid: as many workgroups as the device holds at once
+  int per_cu = 0;
+  FR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kGemmThreads, lds));
+  if (per_cu < 1) per_cu = 1;
+  if (c.cus <= 0) {
+    hipDeviceProp_t prop;
+    FR_HIP(hipGetDeviceProperties(&prop, c.device));
+    c.cus = prop.multiProcessorCount;
+  }
+  int grid = c.cus * per_cu;
+  if (max_tiles < grid) grid = max_tiles;
+  if (grid < 1) return 0;
+  const float2* apv = reinterpret_cast<const float2*>(ap);
+  const float2* sx = reinterpret_cast<const float2*>(shx_tab);
+  const float2* sy = reinterpret_cast<const float2*>(shy_tab);
+  if (bf16)
+    hipLaunchKernelGGL(k_search_gemm_ragged<__bf16>, dim3(grid), dim3(kGemmThreads), lds,
+                       c.stream, np, nbank, kpad, reinterpret_cast<const __bf16*>(bank), loff, toff,
+                       cand, cap, apv, c2, sum2, sx, sy, score, best_row);
+  else
+    hipLaunchKernelGGL(k_search_gemm_ragged<float>, dim3(grid), dim3(kGemmThreads), lds,
+                       c.stream, np, nbank, kpad, reinterpret_cast<const float*>(bank), loff, toff,
+                       cand, cap, apv, c2, sum2, sx, sy, score, best_row);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_topk(Ctx& c, int np, int n, int group, const int* loff, const int* grp,
+                const int* cand, int cap, int topk, const float* score, const int* best_row,
+                int* top_o, int* top_row, float* top_s) {
   FR_CHECK(topk >= 1 && topk <= 32, "topk must be in 1..32");
-  FR_CHECK(group >= 1 && norient % group == 0, "orientation count %d is not a multiple of the "
-           "per-view group %d", norient, group);
-  hipLaunchKernelGGL(k_topk, dim3(np), dim3(kThreads), 0, c.stream, norient, group, topk, score,
-                     best_row, top_o, top_row, top_s);
+  if (!loff)
+    FR_CHECK(group >= 1 && n % group == 0, "list length %d is not a multiple of the per-view "
+             "group %d", n, group);
+  hipLaunchKernelGGL(k_topk, dim3(np), dim3(kThreads), 0, c.stream, n, group, loff, grp, cand,
+                     cap, topk, score, best_row, top_o, top_row, top_s);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -168,6 +168,16 @@ class LearnedProposal:
             start = end
         self.cells = [np.concatenate(c) if c else np.zeros(0, np.int32) for c in cells]
         self.cell_groups = np.array([len(c) // G for c in self.cells])
+        # device copies for fr_build_lists: cells back to back, class c at
+        # cell_off[c]..cell_off[c+1]
+        lens = np.array([len(c) for c in self.cells], np.int64)
+        off = np.zeros(len(lens) + 1, np.int64)
+        off[1:] = np.cumsum(lens)
+        flat = np.concatenate(self.cells) if off[-1] else np.full(1, -1, np.int32)
+        self.cell_off_dev = torch.from_numpy(off.astype(np.int32)).to(self.dev)
+        self.cells_dev = torch.from_numpy(flat.astype(np.int32)).to(self.dev)
+        # any set of distinct classes spans at most all cells; a fallback list is the bank
+        self.maxlen = int(max(off[-1], s.norient))
 
     def candidates(self, top: np.ndarray) -> np.ndarray:
         """Concatenated cells of each particle's proposed classes, padded with -1
@@ -256,42 +266,37 @@ class LearnedProposal:
         return self.ctx.cnn_propose(self.net.forward(img), self.cfg.topk)
 
     def run(self, outc, ctff, datc, fsct, ring, topk: int = 32):
-        """Learned search of one batch: proposal -> cell search -> FP32 CC3M
-        rescoring, with the exhaustive search for the unconfident particles."""
+        """Learned search of one batch, on the device from images to winners:
+        proposal -> ragged candidate lists (the proposed classes' cells, or the
+        whole bank for a particle below the confidence threshold) -> one GEMM
+        search launch -> FP32 CC3M rescoring of the top-k; the results come
+        back to the host once."""
         s = self.search
         top, prob = self.propose(datc, ctff)
-        top_h, prob_h = top.cpu().numpy(), prob.cpu().numpy()
-        confident = prob_h[:, 0] >= self.cfg.threshold
-        npart = top_h.shape[0]
-        o_all = np.full((npart, topk), -1, np.int32)
-        r_all = np.zeros((npart, topk), np.int32)
-        s_all = np.full((npart, topk), -np.inf, np.float32)
-        idx_c = np.nonzero(confident)[0]
-        idx_f = np.nonzero(~confident)[0]
-        self.last_valid = 0
-        self.last_slots = 0
-        if len(idx_c):
-            cand_h = self.candidates(top_h[idx_c])
-            self.last_valid = int((cand_h >= 0).sum())
-            self.last_slots = int(cand_h.size)
-            cand = torch.from_numpy(cand_h).to(self.dev)
-            sel = torch.from_numpy(idx_c).to(self.dev)
-            o, r, sc = self.ctx.search_list(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.bank,
-                                            outc[sel].contiguous(), ctff[sel].contiguous(), cand,
-                                            self.cfg.group, topk=topk)
-            o_all[idx_c], r_all[idx_c], s_all[idx_c] = o.cpu().numpy(), r.cpu().numpy(), \
-                sc.cpu().numpy()
-        if len(idx_f):
-            sel = torch.from_numpy(idx_f).to(self.dev)
-            o, r, sc = s.candidates(outc[sel].contiguous(), ctff[sel].contiguous(), topk)
-            o_all[idx_f], r_all[idx_f], s_all[idx_f] = o.cpu().numpy(), r.cpu().numpy(), \
-                sc.cpu().numpy()
-        top_o = torch.from_numpy(o_all).to(self.dev)
-        top_r = torch.from_numpy(r_all).to(self.dev)
-        sc, shifts = s.rescore(outc, ctff, top_o, top_r, fsct, ring)
-        best = np.argmax(sc, axis=1)
+        loff, grp, cand, info = self.ctx.build_lists(
+            top, prob, self.cfg.threshold, self.cell_off_dev, self.cells_dev, self.cfg.group,
+            s.norient, s.group, self.maxlen)
+        o, r, sg = self.ctx.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.bank, outc, ctff,
+                                          loff, grp, cand, topk)
+        sc, best = self.ctx.rescore(s.nsam, s.a3df, outc, ctff, o, r, s.dm, s.nsamr, s.maxr1,
+                                    s.maxr2, s.rbfact, s.fsct_device(fsct), ring)
+        npart = o.shape[0]
+        f32 = torch.float32
+        pack = torch.cat([o.view(f32).reshape(-1), r.view(f32).reshape(-1), sg.reshape(-1),
+                          sc.reshape(-1), best.view(f32), info.view(f32).reshape(-1)]).cpu().numpy()
+        n = npart * topk
+        o_h = pack[:n].view(np.int32).reshape(npart, topk)
+        r_h = pack[n:2 * n].view(np.int32).reshape(npart, topk)
+        sg_h = pack[2 * n:3 * n].reshape(npart, topk)
+        sc_h = pack[3 * n:4 * n].reshape(npart, topk)
+        best_h = pack[4 * n:4 * n + npart].view(np.int32)
+        info_h = pack[4 * n + npart:].view(np.int32).reshape(npart, 2)
+        confident = info_h[:, 0] == 1
+        self.last_valid = int(info_h[confident, 1].sum())
         ar = np.arange(npart)
-        bo = o_all[ar, best]
-        res = SearchResult(s.euler[np.maximum(bo, 0)], shifts[ar, best], sc[ar, best], bo, o_all,
-                           s_all)
+        bo = o_h[ar, best_h]
+        shifts = s.row_to_shift(np.where(o_h >= 0, r_h, 7 * 15 + 7).reshape(-1)).reshape(
+            npart, topk, 2)
+        res = SearchResult(s.euler[np.maximum(bo, 0)], shifts[ar, best_h], sc_h[ar, best_h], bo,
+                           o_h, sg_h)
         return res, confident

@@ -103,6 +103,18 @@ class GlobalSearch:
         nsamh = self.nsam // 2
         nsamrh = nsamh // 2 if (nsamh // 2) * 2 == nsamh else (nsamh + 1) // 2
         self.nsamr = 2 * nsamrh
+        self._fsct = (None, None)
+
+    def fsct_device(self, fsct) -> torch.Tensor:
+        """FSCT as a device tensor, uploaded once per distinct host array."""
+        if torch.is_tensor(fsct):
+            return fsct
+        key, dev = self._fsct
+        host = np.ascontiguousarray(fsct, np.float32)
+        if key is None or key.shape != host.shape or not np.array_equal(key, host):
+            dev = torch.from_numpy(host.copy()).to(self.a3df.device)
+            self._fsct = (host.copy(), dev)
+        return dev
 
     @property
     def norient(self) -> int:

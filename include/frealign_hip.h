@@ -232,6 +232,47 @@ int fr_search_list_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfa
                          const float *d_ctff, int ncand, const int *d_cand, int group, int topk,
                          int *d_top_o, int *d_top_row, float *d_top_score);
 
+/* Ragged-list form of fr_search_batch (PSEARCH/CCP, psearch.f:8-121 and
+ * ccp.f:8-275, each particle restricted to its own list): particle p is
+ * scored against bank rows d_cand[d_loff[p] .. d_loff[p+1]-1] (-1: empty
+ * slot), in view groups of d_grp[p] entries; all device arrays.  A list that
+ * is the whole bank in order with the bank's view group gives exactly
+ * fr_search_batch's answer for that particle.  cap is the length of d_cand;
+ * list positions at or beyond it are ignored.  One persistent GEMM launch
+ * covers every list. */
+int fr_search_ragged_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact,
+                           int norient, const void *d_bank, int dtype, int np,
+                           const float *d_outc, const float *d_ctff, const int *d_loff,
+                           const int *d_grp, const int *d_cand, int cap, int topk, int *d_top_o,
+                           int *d_top_row, float *d_top_score);
+
+/* Candidate lists of the learned search (no reference counterpart; the
+ * fallback list is PSEARCH's full view loop, psearch.f:73-102).  d_top and
+ * d_prob are np x ktop classes and probabilities from fr_cnn_propose.  A
+ * particle with d_prob[p*ktop] >= threshold gets the concatenated cells of
+ * its classes (class c's cell: d_cells[d_cell_off[c] .. d_cell_off[c+1]-1],
+ * -1 padded to groups of `group`); any other particle gets the whole bank
+ * 0..nbank-1 in groups of bank_group.  Lists are clipped to maxlen entries
+ * (whole groups) and written to d_cand (np*maxlen ints) at d_loff (np+1);
+ * d_grp (np) receives the group sizes and d_info (np x 2) the confidence flag
+ * and the number of non-padding entries. */
+int fr_build_lists(fr_ctx *ctx, int np, int ktop, const int *d_top, const float *d_prob,
+                   float threshold, int ncls, const int *d_cell_off, const int *d_cells, int group,
+                   int nbank, int bank_group, int maxlen, int *d_loff, int *d_grp, int *d_cand,
+                   int *d_info);
+
+/* FP32 rescoring of each particle's k search candidates (the score LMAIN
+ * refines them with, lmain.f:615-642): CC3M (as fr_cc3m_batch, AMAG = 1) at
+ * bank orientation d_top_o[p*k+q] (rotation d_dm_bank, nbank x 9) and the
+ * CCP shift of map row d_top_row[p*k+q], SX = IXM/NSAMR*2pi (ccp.f:254-256).
+ * d_score (np x k) receives the scores, -inf for empty candidates (-1), and
+ * d_best (np) the first maximum's position (strict '>', lmain.f:634). */
+int fr_rescore_batch(fr_ctx *ctx, int nsam, int np, int k, const float *d_a3df,
+                     const float *d_outc, const float *d_ctff, const int *d_top_o,
+                     const int *d_top_row, int nbank, const float *d_dm_bank, int nsamr,
+                     int maxr1, int maxr2, float rbfact, const float *d_fsct, const float *ring,
+                     float *d_score, int *d_best);
+
 /* ------------------------------------------- learned orientation proposal
  * No counterpart in the reference.  A network of nconv 3x3 / stride 2 / pad 1
  * convolutions with ReLU (1 -> C -> ... -> C channels on an S x S input) and
