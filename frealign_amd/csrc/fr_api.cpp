@@ -116,19 +116,47 @@ void* fr_ctx::slot(int k, size_t bytes, int* err) {
   return slots[k];
 }
 
-int fr_ctx::ensure_dft(int nsam) {
-  if (dft_nsam == nsam) return 0;
-  std::vector<float> tab;
-  fr::cc3m_fused_table(nsam, tab);
-  if (d_dft) {
-    FR_HIP(hipStreamSynchronize(stream));
-    FR_HIP(hipFree(d_dft));
-    d_dft = nullptr;
-    dft_nsam = 0;
+namespace {
+template <typename T>
+int upload_table(const std::vector<T>& h, T** d) {
+  FR_HIP(hipMalloc(d, sizeof(T) * h.size()));
+  FR_HIP(hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+}  // namespace
+
+int fr_ctx::cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t) {
+  std::vector<float> dft;
+  auto di = dft_tabs.find(a.nsam);
+  if (di == dft_tabs.end()) {
+    fr::cc3m_fused_table(a.nsam, dft);
+    float* d = nullptr;
+    if (int r = upload_table(dft, &d)) return r;
+    di = dft_tabs.emplace(a.nsam, d).first;
   }
-  FR_HIP(hipMalloc(&d_dft, sizeof(float) * tab.size()));
-  FR_HIP(hipMemcpy(d_dft, tab.data(), sizeof(float) * tab.size(), hipMemcpyHostToDevice));
-  dft_nsam = nsam;
+  t->dft = di->second;
+  const std::pair<int, int> pk(a.maxr1 * a.maxr1, a.maxr2 * a.maxr2);
+  auto pi = pts_tabs.find(pk);
+  if (pi == pts_tabs.end()) {
+    std::vector<int> pts;
+    fr::cc3m_fused_points(pk.first, pk.second, pts);
+    int* d = nullptr;
+    if (int r = upload_table(pts, &d)) return r;
+    pi = pts_tabs.emplace(pk, d).first;
+  }
+  t->pts = pi->second;
+  const auto rk = std::make_tuple(a.nsam, a.ri2, a.ri3);
+  auto ri = ring_tabs.find(rk);
+  if (ri == ring_tabs.end()) {
+    if (dft.empty()) fr::cc3m_fused_table(a.nsam, dft);
+    std::vector<float> ring;
+    RingTab rt;
+    rt.cnt = fr::cc3m_fused_ring(a.nsam, a.ri2, a.ri3, dft, ring);
+    if (int r = upload_table(ring, &rt.d)) return r;
+    ri = ring_tabs.emplace(rk, rt).first;
+  }
+  t->ring = ri->second.d;
+  t->ring_cnt = ri->second.cnt;
   return 0;
 }
 
@@ -200,7 +228,9 @@ int fr_destroy(fr_ctx* c) {
   for (int k = 0; k < fr_ctx::kSlots; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
   if (c->d_kidx) (void)hipFree(c->d_kidx);
-  if (c->d_dft) (void)hipFree(c->d_dft);
+  for (auto& e : c->dft_tabs) (void)hipFree(e.second);
+  for (auto& e : c->pts_tabs) (void)hipFree(e.second);
+  for (auto& e : c->ring_tabs) (void)hipFree(e.second.d);
   if (c->d_shx) (void)hipFree(c->d_shx);
   if (c->d_shy) (void)hipFree(c->d_shy);
   delete c;
@@ -299,13 +329,14 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
   const fr_ctx::Tune tn = c->tune_for(nsam);
   if (fr::cc3m_fused_ok(nsam, maxr2) && tn.cc3m_fused) {
-    if (int r = c->ensure_dft(nsam)) return r;
+    Cc3mTables tabs;
+    if (int r = c->cc3m_tables(a, &tabs)) return r;
     int pslot = c->prof_begin(kProfCc3m);
     for (int o = 0; o < n; o += 65535) {
       a.n = n - o < 65535 ? n - o : 65535;
       if (int r = launch_cc3m_fused(*c, a, d_a3df, d_spec, d_ctff, d_pidx + o,
                                     d_dm + 9 * (size_t)o, d_shift + 2 * (size_t)o, d_amag + o,
-                                    d_fsct, c->d_dft, d_score + o, tn.cc3m_blocks))
+                                    d_fsct, tabs, d_score + o, tn.cc3m_blocks))
         return r;
     }
     c->prof_end(pslot);

@@ -1,6 +1,8 @@
 // fr_ctx.h -- the library context behind the opaque fr_ctx handle.
 #pragma once
 #include <map>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "fr_common.h"
@@ -18,10 +20,16 @@ struct fr_ctx : public fr::Ctx {
   int* d_kidx = nullptr;
   float* d_shx = nullptr;
   float* d_shy = nullptr;
-  // DFT table of the fused CC3M, cached per NSAM
-  int dft_nsam = 0;
-  float* d_dft = nullptr;
-  int ensure_dft(int nsam);
+  // constant tables of the fused CC3M, built on first use and kept until the
+  // context is destroyed (captured graphs hold their addresses)
+  std::map<int, float*> dft_tabs;                        // per NSAM
+  std::map<std::pair<int, int>, int*> pts_tabs;          // per (MAXR1^2, MAXR2^2)
+  struct RingTab {
+    float* d = nullptr;
+    int cnt = 0;
+  };
+  std::map<std::tuple<int, float, float>, RingTab> ring_tabs;  // per (NSAM, RI2, RI3)
+  int cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t);
   // launch choices per box size (set by fr_autotune_cc3m)
   struct Tune {
     int cc3m_fused = 1;   // band-pruned fused CC3M where it applies

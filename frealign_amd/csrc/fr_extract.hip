@@ -236,8 +236,9 @@ __global__ void __launch_bounds__(kFT)
                  const float* __restrict__ dm, const int* __restrict__ pidx,
                  const float2* __restrict__ ctff, const float2* __restrict__ spec,
                  const float* __restrict__ shift, const float* __restrict__ amag_arr,
-                 const float* __restrict__ fsct, const float* __restrict__ tab, float ri2,
-                 float ri3, float rih, float halfw, float* score) {
+                 const float* __restrict__ fsct, const float* __restrict__ tab,
+                 const int* __restrict__ pts, const float* __restrict__ ring_tab, int ring_cnt,
+                 float ri2, float ri3, float rih, float halfw, float* score) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* Ts = lds;                      // [nsam][kFP]
   float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
@@ -262,9 +263,7 @@ __global__ void __launch_bounds__(kFT)
   const float amagp = amag_arr[it];
   const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
 
-  // ---- clear the shell bins, fill X (annulus x CTF / NSAM^2) into Ys
-  const int nb = nsam / 2 + 2;
-  for (int k = tid; k < 4 * nb; k += kFT) bins[k] = 0.0f;
+  // ---- fill X (annulus x CTF / NSAM^2) into Ys
   float2* Xs = reinterpret_cast<float2*>(Ys);  // [32][63]: I, J + 31
   {
     const float* d = dm + 9 * (size_t)it;
@@ -314,9 +313,16 @@ __global__ void __launch_bounds__(kFT)
   }
   __syncthreads();
 
-  // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3' (tiles that can reach it)
+  // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3'.  x is linear in Y,
+  // so at AMAG = 1 the ring sum is sum_{m,k} R[m][k] Y[m][k] with the
+  // precomputed R; otherwise a pass over the ring tiles on the matrix cores.
   float rsum = 0.f;
   int rcnt = 0;
+  if (ring_tab && amagp == 1.0f) {
+    for (int e = tid; e < nsam * kFK; e += kFT)
+      rsum += ring_tab[e] * Ys[(e / kFK) * kFP + (e % kFK)];
+    rcnt = tid == 0 ? ring_cnt : 0;
+  } else
   for (int rb = wave; rb < mt; rb += kFT / 64) {
     const int m0 = rb * 16;
     int mlo, mhi;
@@ -435,34 +441,53 @@ __global__ void __launch_bounds__(kFT)
   }
   __syncthreads();
 
-  // ---- cc3m.f:117-150: shifted particle x conj(masked projection), shells
+  // ---- cc3m.f:117-150: shifted particle x conj(masked projection), shells.
+  // Each wave owns a fixed set of shells (zig-zag, so the point counts
+  // balance), walks their points from the sorted table and reduces them with
+  // a fixed butterfly: no atomics, the same sums on every run.
   {
     const float2* sp = spec + (size_t)p * ic;
     const float shx = shift[2 * it], shy = shift[2 * it + 1];
-    for (int e = tid; e < 32 * (2 * kFJ + 1); e += kFT) {
-      int i = e / (2 * kFJ + 1), j = e % (2 * kFJ + 1) - kFJ;
-      int itemp = i * i + j * j;
-      if (itemp < maxr12 || itemp >= maxr22) continue;
-      int jj = j < 0 ? -j : j;
-      float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
-      float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
-      float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
-      float sn, cs;
-      sincosf(shx * (float)i + shy * (float)j, &sn, &cs);
-      float2 s = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
-      cf samp = cmul(cf{s.x, s.y}, cf{cs, sn});
-      float q = samp.re * a.x + samp.im * a.y;
-      int ibin = (int)(sqrtf((float)itemp) + 0.5f);
-      atomicAdd(&bins[4 * ibin + 0], q);
-      atomicAdd(&bins[4 * ibin + 1], samp.re * samp.re + samp.im * samp.im);
-      atomicAdd(&bins[4 * ibin + 2], a.x * a.x + a.y * a.y);
-      atomicAdd(&bins[4 * ibin + 3], 1.0f);
+    for (int sh = 0; sh < kCc3mShells; sh++) {
+      const int z = sh & 15;
+      if ((z < 8 ? z : 15 - z) != wave) continue;
+      const int b0 = pts[sh], b1 = pts[sh + 1];
+      float q = 0.f, s2 = 0.f, a2 = 0.f;
+      for (int e = b0 + lane; e < b1; e += 64) {
+        const int pk = pts[kCc3mPtsHdr + e];
+        const int i = pk & 0xff, j = (pk >> 8) - 64;
+        int jj = j < 0 ? -j : j;
+        float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
+        float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
+        float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
+        float sn, cs;
+        sincosf(shx * (float)i + shy * (float)j, &sn, &cs);
+        float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
+        cf samp = cmul(cf{sv.x, sv.y}, cf{cs, sn});
+        q += samp.re * a.x + samp.im * a.y;
+        s2 += samp.re * samp.re + samp.im * samp.im;
+        a2 += a.x * a.x + a.y * a.y;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        q += __shfl_xor(q, off, 64);
+        s2 += __shfl_xor(s2, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+      }
+      if (lane == 0) {
+        bins[4 * sh + 0] = q;
+        bins[4 * sh + 1] = s2;
+        bins[4 * sh + 2] = a2;
+        bins[4 * sh + 3] = (float)(b1 - b0);
+      }
     }
   }
   __syncthreads();
-  if (tid == 0) {
+  // cc3m.f:139-167 over shells 1 .. NSAM/2-1 (the band holds shells below
+  // kCc3mShells): one wave, FP64 sums reduced by a fixed butterfly
+  if (wave == 0) {
     double s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    for (int i = 2; i <= nsam / 2; i++) {
+    const int top = nsam / 2 < kCc3mShells ? nsam / 2 : kCc3mShells;
+    for (int i = 2 + lane; i <= top; i += 64) {
       const float* b = bins + 4 * (i - 1);
       if (b[3] != 0.0f) {
         float r = b[0] / sqrtf(b[2]);
@@ -473,8 +498,15 @@ __global__ void __launch_bounds__(kFT)
         s1 += 1.0;
       }
     }
-    if (s1 * s2 != 0.0) s3 = s3 / sqrt(s1 * s2);
-    score[it] = (float)s3;
+    for (int off = 32; off > 0; off >>= 1) {
+      s1 += __shfl_xor(s1, off, 64);
+      s2 += __shfl_xor(s2, off, 64);
+      s3 += __shfl_xor(s3, off, 64);
+    }
+    if (lane == 0) {
+      if (s1 * s2 != 0.0) s3 = s3 / sqrt(s1 * s2);
+      score[it] = (float)s3;
+    }
   }
   __syncthreads();  // bins and LDS buffers are reused by the next item
   }
@@ -487,7 +519,8 @@ bool cc3m_fused_ok(int nsam, int maxr2) {
 }
 
 size_t cc3m_fused_lds(int nsam) {
-  return sizeof(float) * ((size_t)2 * nsam * kFP + 64 * kFP + 4 * (nsam / 2 + 2));
+  const int nb = nsam / 2 + 2 > kCc3mShells ? nsam / 2 + 2 : kCc3mShells;
+  return sizeof(float) * ((size_t)2 * nsam * kFP + 64 * kFP + 4 * nb);
 }
 
 // T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N) for i < 32, from
@@ -502,12 +535,56 @@ void cc3m_fused_table(int nsam, std::vector<float>& tab) {
     }
 }
 
+// The band points of cc3m.f:117-150 (MAXR12 <= I*I+J*J < MAXR22, I < 32,
+// |J| <= 31) grouped by shell INT(SQRT(FLOAT(ITEMP))+0.5), in the (I, J)
+// loop order within a shell.
+void cc3m_fused_points(int maxr12, int maxr22, std::vector<int>& pts) {
+  std::vector<std::vector<int>> by(kCc3mShells);
+  for (int i = 0; i < 32; i++)
+    for (int j = -kFJ; j <= kFJ; j++) {
+      int itemp = i * i + j * j;
+      if (itemp < maxr12 || itemp >= maxr22) continue;
+      int sh = (int)(sqrtf((float)itemp) + 0.5f);
+      if (sh < kCc3mShells) by[sh].push_back(i | ((j + 64) << 8));
+    }
+  pts.assign(kCc3mPtsHdr, 0);
+  for (int sh = 0; sh < kCc3mShells; sh++) {
+    pts[sh] = (int)(pts.size() - kCc3mPtsHdr);
+    pts.insert(pts.end(), by[sh].begin(), by[sh].end());
+  }
+  pts[kCc3mShells] = (int)(pts.size() - kCc3mPtsHdr);
+}
+
+// R[m][k] = sum of T[n][k] over the pixels (n, m) of the MASKCOS2 ring
+// RI2 < r^2 <= RI3 (wrapped distances from the origin, as the kernel tests
+// them at AMAG = 1), summed in double; returns the ring's pixel count.
+int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& dft,
+                    std::vector<float>& ring) {
+  const int nsamh = nsam / 2;
+  std::vector<double> acc((size_t)nsam * kFK, 0.0);
+  int cnt = 0;
+  for (int m = 0; m < nsam; m++) {
+    const int dy = m < nsamh ? m : m - nsam;
+    for (int n = 0; n < nsam; n++) {
+      const int dx = n < nsamh ? n : n - nsam;
+      const float crad2 = (float)(dx * dx + dy * dy);
+      if (!(crad2 > ri2 && crad2 <= ri3)) continue;
+      cnt++;
+      for (int k = 0; k < kFK; k++) acc[(size_t)m * kFK + k] += dft[(size_t)n * kFK + k];
+    }
+  }
+  ring.resize(acc.size());
+  for (size_t e = 0; e < acc.size(); e++) ring[e] = (float)acc[e];
+  return cnt;
+}
+
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
-                      const float* amag, const float* fsct, const float* tab, float* score,
+                      const float* amag, const float* fsct, const Cc3mTables& t, float* score,
                       int blocks) {
   if (a.n <= 0) return 0;
   FR_CHECK(cc3m_fused_ok(a.nsam, a.maxr2), "fused CC3M needs NSAM %% 32 == 0, NSAM <= 256, MAXR2 <= 32");
+  FR_CHECK(t.dft && t.pts, "fused CC3M tables missing");
   const size_t lds = cc3m_fused_lds(a.nsam);
   FR_HIP(hipFuncSetAttribute((const void*)k_cc3m_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
                              (int)lds));
@@ -517,7 +594,8 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
                      reinterpret_cast<const float2*>(a3df), dm, pidx,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),
-                     shift, amag, fsct, tab, a.ri2, a.ri3, a.rih, a.halfw, score);
+                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, a.ri2, a.ri3, a.rih,
+                     a.halfw, score);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -67,10 +67,27 @@ int launch_sigma(Ctx& c, int nsam, int n, const float* a3df, const SigmaItem* it
 // CC3M with band-pruned DFT products on the matrix cores; one workgroup per item.
 bool cc3m_fused_ok(int nsam, int maxr2);
 size_t cc3m_fused_lds(int nsam);
+// Constant tables of the fused kernel (device pointers, owned by the context):
+//   dft  T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N), i < 32;
+//   pts  kCc3mShells+1 shell offsets, then the band points (I | (J+64) << 8)
+//        sorted by shell INT(SQRT(I*I+J*J)+0.5), (I, J) order within a shell;
+//   ring R[m][k] = sum over MASKCOS2 ring pixels n of row m of T[n][k], for
+//        AMAG = 1 (null: the ring mean is taken by a pass on the matrix cores).
+constexpr int kCc3mShells = 34;
+constexpr int kCc3mPtsHdr = 36;
+struct Cc3mTables {
+  const float* dft = nullptr;
+  const int* pts = nullptr;
+  const float* ring = nullptr;
+  int ring_cnt = 0;
+};
 void cc3m_fused_table(int nsam, std::vector<float>& tab);
+void cc3m_fused_points(int maxr12, int maxr22, std::vector<int>& pts);
+int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& dft,
+                    std::vector<float>& ring);
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
-                      const float* amag, const float* fsct, const float* tab, float* score,
+                      const float* amag, const float* fsct, const Cc3mTables& t, float* score,
                       int blocks = 0);
 
 }  // namespace fr

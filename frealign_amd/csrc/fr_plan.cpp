@@ -38,6 +38,7 @@ struct fr_plan {
   const float* fsct = nullptr;
   PrepGlobals g{};
   Cc3mArgs ca{};
+  Cc3mTables tabs;  // fused CC3M tables (owned by the context)
   int nsym = 0;
   int jsym[8] = {};
   float symop[72] = {};
@@ -84,7 +85,7 @@ int enqueue(fr_plan* p) {
   a.n = nb;
   if (p->fused) {
     if (int r = launch_cc3m_fused(*c, a, p->a3df, p->d_outc, p->d_ctff, p->d_pidx, p->d_dm,
-                                  p->d_shift, p->d_amag, p->fsct, c->d_dft, p->d_score, p->blocks))
+                                  p->d_shift, p->d_amag, p->fsct, p->tabs, p->d_score, p->blocks))
       return r;
   } else {
     if (int r = launch_cc3m(*c, a, p->a3df, p->d_outc, p->d_ctff, p->d_pidx, p->d_dm, p->d_shift,
@@ -221,7 +222,7 @@ int fr_plan_create(fr_ctx* c, int nsam, int nb, const float* d_a3df, const float
       (r = halloc(&p->h_tasks, (size_t)nb * ncopy)))
     return fail(r);
   if (!p->fused && (r = dalloc(&p->d_work, img * nb))) return fail(r);
-  if (p->fused && (r = c->ensure_dft(nsam))) return fail(r);
+  if (p->fused && (r = c->cc3m_tables(p->ca, &p->tabs))) return fail(r);
   std::vector<int> pid(nb);
   for (int b = 0; b < nb; b++) pid[b] = b;
   if (hipMemcpy(p->d_pidx, pid.data(), sizeof(int) * nb, hipMemcpyHostToDevice) != hipSuccess)
