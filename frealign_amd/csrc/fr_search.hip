@@ -134,13 +134,18 @@ __global__ void __launch_bounds__(kThreads)
 // ---------------------------------------------------------------- GEMM
 constexpr int kGemmThreads = 512;  // 8 waves: 4 row groups x 2 column groups
 
+// One K chunk of the tile on the matrix cores.  Alongside, wave (wr, wc)
+// accumulates the norm partials sum |T|^2 w of its columns ct = 2wr, 2wr+1
+// from the B fragments it already holds (w: the chunk's 16 weights |c(k)|^2);
+// the four lane groups of a column are combined after the K loop.
 template <typename T>
 __device__ inline void mma_step(const T* As, const T* Bs, floatx4 (&acc)[4][8], int wr, int wc,
-                                int lane);
+                                int lane, const float* w, float (&nacc)[2]);
 
 template <>
 __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
-                                        floatx4 (&acc)[4][8], int wr, int wc, int lane) {
+                                        floatx4 (&acc)[4][8], int wr, int wc, int lane,
+                                        const float* w, float (&nacc)[2]) {
   constexpr int P = Elem<__bf16>::kPad;
   const int lr = lane & 15, kq = (lane >> 4) * 8;
   bf16x8 a[4], b[8];
@@ -155,11 +160,26 @@ __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
 #pragma unroll
     for (int ct = 0; ct < 8; ct++)
       acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+  // reals kq..kq+7 are complex kq/2..kq/2+3
+  const float4 wv = *reinterpret_cast<const float4*>(w + (kq >> 1));
+  const float wk[4] = {wv.x, wv.y, wv.z, wv.w};
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const bf16x8 v = wr == 0 ? b[u] : wr == 1 ? b[2 + u] : wr == 2 ? b[4 + u] : b[6 + u];
+    float sum = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      float re = (float)v[2 * q], im = (float)v[2 * q + 1];
+      sum += wk[q] * (re * re + im * im);
+    }
+    nacc[u] += sum;
+  }
 }
 
 template <>
 __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4 (&acc)[4][8],
-                                       int wr, int wc, int lane) {
+                                       int wr, int wc, int lane, const float* w,
+                                       float (&nacc)[2]) {
   constexpr int P = Elem<float>::kPad;
   const int lr = lane & 15, kq = lane >> 4;
 #pragma unroll
@@ -174,6 +194,12 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
 #pragma unroll
       for (int ct = 0; ct < 8; ct++)
         acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
+    const float wk = w[(kk + kq) >> 1];  // real kk+kq is part of complex (kk+kq)/2
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const float v = wr == 0 ? b[u] : wr == 1 ? b[2 + u] : wr == 2 ? b[4 + u] : b[6 + u];
+      nacc[u] += wk * (v * v);
+    }
   }
 }
 
@@ -181,7 +207,7 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
 // and three table buffers, each holding the chunk's 16x15 X'(k) e^{i I sx}
 // and 16x15 e^{i J sy} phase entries and its 16 weights |c_p(k)|^2.  The
 // epilogue map [kRows][65] floats aliases the operand buffers; the column
-// norms' partial sums alias the table buffers.
+// norms alias the table buffers.
 template <typename T>
 struct Lds {
   static constexpr size_t a_bytes = (size_t)kRows * Elem<T>::kPad * sizeof(T);
@@ -246,7 +272,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
-  float nacc = 0.f;  // norm partial: column tid/2, complex 8*(tid%2)..+7 of each chunk
+  float nacc[2] = {0.f, 0.f};  // norm partials of columns wc*128 + (2wr+u)*16 + lane%16
 
   // this thread's B pieces: piece bq of templates bc0 + u * bstep
   const int bq = tid % BP::per_row;
@@ -295,8 +321,11 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     }
     treg = v;
   };
+  // phase entries as float2; the 16 weights as plain floats after them
   auto store_tab = [&](int tbuf) {
-    if (tid < L::tab_entries) tabx(tbuf)[tid] = treg;
+    if (tid < 32 * kShift) tabx(tbuf)[tid] = treg;
+    else if (tid < L::tab_entries)
+      reinterpret_cast<float*>(tabx(tbuf) + 32 * kShift)[tid - 32 * kShift] = treg.x;
   };
   // A chunk: row (sx, sy) = X'(k) e^{i k.s}; two threads per row, 8 complex each
   const int row = tid >> 1, hk = (tid & 1) * 8;
@@ -324,22 +353,6 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
       *reinterpret_cast<uint4*>(dst + g * E) = *reinterpret_cast<const uint4*>(v);
     }
   };
-  // norm: |T|^2 of this thread's 8 complex of the chunk in buf, weighted by |c|^2
-  auto norm_step = [&](int buf, int tbuf) {
-    constexpr int E = 16 / sizeof(T);
-    const T* src = Bs(buf) + row * P + 2 * hk;
-    const float2* w = tabx(tbuf) + 32 * kShift + hk;
-    T v[16];
-#pragma unroll
-    for (int g = 0; g < 16 / E; g++)
-      *reinterpret_cast<uint4*>(v + g * E) = *reinterpret_cast<const uint4*>(src + g * E);
-#pragma unroll
-    for (int q = 0; q < 8; q++) {
-      float re = Elem<T>::f(v[2 * q]), im = Elem<T>::f(v[2 * q + 1]);
-      nacc += w[q].x * (re * re + im * im);
-    }
-  };
-
   // prologue: tables for chunks 0 and 1, operands for chunk 0
   load_tab(0);
   store_tab(0);
@@ -358,8 +371,8 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     const bool more = kc + 1 < nchunk;
     if (more) load_b(kc + 1);
     if (kc + 2 < nchunk) load_tab(kc + 2);
-    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
-    norm_step(cur, kc % 3);
+    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane,
+                reinterpret_cast<const float*>(tabx(kc % 3) + 32 * kShift), nacc);
     if (more) {
       gen_a(nxt, (kc + 1) % 3);
       store_b(nxt);
@@ -371,7 +384,12 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
   // ------------------------------------------------------------ epilogue
   float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
   float* nl = reinterpret_cast<float*>(smem_raw + L::tab_off);
-  nl[tid] = nacc;
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    nacc[u] += __shfl_xor(nacc[u], 16, 64);
+    nacc[u] += __shfl_xor(nacc[u], 32, 64);
+    if (lane < 16) nl[wc * 128 + (2 * wr + u) * 16 + lane] = nacc[u];
+  }
 #pragma unroll
   for (int q4 = 0; q4 < 4; q4++) {  // 64-column quarters of the 256 templates
     if (q4) __syncthreads();
@@ -403,7 +421,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
       if (o < ncol) {
-        float n = nl[2 * cc] + nl[2 * cc + 1];
+        float n = nl[cc];
         float v = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         score_p[o] = v;
         brow_p[o] = (bx + 7) * kShift + (by + 7);
