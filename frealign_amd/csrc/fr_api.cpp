@@ -606,12 +606,15 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
   if (!score) return err;
   int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * norient, &err);
   if (!brow) return err;
+  float* nrm = (float*)c->slot(2, sizeof(float) * (size_t)np * norient, &err);
+  if (!nrm) return err;
   if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
                                   d_ctff, c->d_kidx, ap, c2, sum2))
     return r;
+  if (int r = launch_search_norm(*c, norient, c->kpad, np, d_bank, dtype == 1, c2, nrm)) return r;
   int pslot = c->prof_begin(kProfSearchGemm);
   if (int r = launch_search_gemm(*c, norient, norient, c->kpad, np, d_bank, dtype == 1, nullptr,
-                                 ap, c2, sum2, c->d_shx, c->d_shy, score, brow))
+                                 ap, c2, nrm, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
   return launch_topk(*c, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
@@ -758,7 +761,7 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
     return r;
   int pslot = c->prof_begin(kProfSearchList);
   if (int r = launch_search_gemm(*c, ncand, norient, c->kpad, np, d_bank, dtype == 1, d_cand, ap,
-                                 c2, sum2, c->d_shx, c->d_shy, score, brow))
+                                 c2, nullptr, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
   return launch_topk(*c, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,

@@ -131,21 +131,74 @@ __global__ void __launch_bounds__(kThreads)
   if (threadIdx.x == 0) sum2[p] = red[0] + red[1] + red[2] + red[3];
 }
 
+// ---------------------------------------------------------------- norms
+// sum_k |c_p(k)|^2 |T_o(k)|^2 for every (particle, orientation): the
+// normalisation SUM of ccp.f:211-214 with the CTF folded on the particle.
+// Tiles of 64 particles x 128 templates; K staged through LDS.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+    k_search_norm(int norient, int kpad, int np, const T* __restrict__ bank,
+                  const float* __restrict__ c2, float* nrm) {
+  constexpr int TP = 64, TO = 128, KS = 32;  // particles, templates, complex per step
+  __shared__ float cs[TP][KS + 1];
+  __shared__ float ts[TO][KS + 1];
+  const int half = kpad / 2;
+  const int o0 = blockIdx.x * TO, p0 = blockIdx.y * TP;
+  const int tid = threadIdx.x;
+  // thread -> (4 particles) x (8 templates)
+  const int tp = (tid & 15) * 4, to = (tid >> 4) * 8;
+  float acc[4][8] = {};
+  for (int k0 = 0; k0 < half; k0 += KS) {
+    for (int e = tid; e < TP * KS; e += kThreads) {
+      int pp = e / KS, kk = e % KS;
+      int p = p0 + pp, k = k0 + kk;
+      cs[pp][kk] = (p < np && k < half) ? c2[(size_t)p * half + k] : 0.f;
+    }
+    for (int e = tid; e < TO * KS; e += kThreads) {
+      int oo = e / KS, kk = e % KS;
+      int o = o0 + oo, k = k0 + kk;
+      float v = 0.f;
+      if (o < norient && k < half) {
+        float re = Elem<T>::f(bank[(size_t)o * kpad + 2 * k]);
+        float im = Elem<T>::f(bank[(size_t)o * kpad + 2 * k + 1]);
+        v = re * re + im * im;
+      }
+      ts[oo][kk] = v;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < KS; kk++) {
+      float a[4], b[8];
+#pragma unroll
+      for (int i = 0; i < 4; i++) a[i] = cs[tp + i][kk];
+#pragma unroll
+      for (int j = 0; j < 8; j++) b[j] = ts[to + j][kk];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] += a[i] * b[j];
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      int p = p0 + tp + i, o = o0 + to + j;
+      if (p < np && o < norient) nrm[(size_t)p * norient + o] = acc[i][j];
+    }
+}
+
 // ---------------------------------------------------------------- GEMM
 constexpr int kGemmThreads = 512;  // 8 waves: 4 row groups x 2 column groups
 
-// One K chunk of the tile on the matrix cores.  Alongside, wave (wr, wc)
-// accumulates the norm partials sum |T|^2 w of its columns ct = 2wr, 2wr+1
-// from the B fragments it already holds (w: the chunk's 16 weights |c(k)|^2);
-// the four lane groups of a column are combined after the K loop.
 template <typename T>
 __device__ inline void mma_step(const T* As, const T* Bs, floatx4 (&acc)[4][8], int wr, int wc,
-                                int lane, const float* w, float (&nacc)[2]);
+                                int lane);
 
 template <>
 __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
-                                        floatx4 (&acc)[4][8], int wr, int wc, int lane,
-                                        const float* w, float (&nacc)[2]) {
+                                        floatx4 (&acc)[4][8], int wr, int wc, int lane) {
   constexpr int P = Elem<__bf16>::kPad;
   const int lr = lane & 15, kq = (lane >> 4) * 8;
   bf16x8 a[4], b[8];
@@ -160,26 +213,11 @@ __device__ inline void mma_step<__bf16>(const __bf16* As, const __bf16* Bs,
 #pragma unroll
     for (int ct = 0; ct < 8; ct++)
       acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
-  // reals kq..kq+7 are complex kq/2..kq/2+3
-  const float4 wv = *reinterpret_cast<const float4*>(w + (kq >> 1));
-  const float wk[4] = {wv.x, wv.y, wv.z, wv.w};
-#pragma unroll
-  for (int u = 0; u < 2; u++) {
-    const bf16x8 v = wr == 0 ? b[u] : wr == 1 ? b[2 + u] : wr == 2 ? b[4 + u] : b[6 + u];
-    float sum = 0.f;
-#pragma unroll
-    for (int q = 0; q < 4; q++) {
-      float re = (float)v[2 * q], im = (float)v[2 * q + 1];
-      sum += wk[q] * (re * re + im * im);
-    }
-    nacc[u] += sum;
-  }
 }
 
 template <>
 __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4 (&acc)[4][8],
-                                       int wr, int wc, int lane, const float* w,
-                                       float (&nacc)[2]) {
+                                       int wr, int wc, int lane) {
   constexpr int P = Elem<float>::kPad;
   const int lr = lane & 15, kq = lane >> 4;
 #pragma unroll
@@ -194,12 +232,6 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
 #pragma unroll
       for (int ct = 0; ct < 8; ct++)
         acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
-    const float wk = w[(kk + kq) >> 1];  // real kk+kq is part of complex (kk+kq)/2
-#pragma unroll
-    for (int u = 0; u < 2; u++) {
-      const float v = wr == 0 ? b[u] : wr == 1 ? b[2 + u] : wr == 2 ? b[4 + u] : b[6 + u];
-      nacc[u] += wk * (v * v);
-    }
   }
 }
 
@@ -207,7 +239,7 @@ __device__ inline void mma_step<float>(const float* As, const float* Bs, floatx4
 // and three table buffers, each holding the chunk's 16x15 X'(k) e^{i I sx}
 // and 16x15 e^{i J sy} phase entries and its 16 weights |c_p(k)|^2.  The
 // epilogue map [kRows][65] floats aliases the operand buffers; the column
-// norms alias the table buffers.
+// norms' partial sums alias the table buffers.
 template <typename T>
 struct Lds {
   static constexpr size_t a_bytes = (size_t)kRows * Elem<T>::kPad * sizeof(T);
@@ -239,16 +271,21 @@ struct BPieces {
 
 // One tile: the 225 shift rows of particle p against columns o0..o0+255 of
 // its template list (list entry cand_p[o], or o itself when cand_p is null;
-// entries outside 0..nbank-1 are empty columns).  Alongside the MFMA work the
-// tile accumulates each column's normalisation sum_k |c_p(k)|^2 |T_o(k)|^2
-// (SUM of ccp.f:211-214 with the CTF folded on the particle) from the
-// template chunks already in LDS.  Outputs are indexed by list position.
-template <typename T>
+// entries outside 0..nbank-1 are empty columns).  The normalisation
+// sum_k |c_p(k)|^2 |T_o(k)|^2 (SUM of ccp.f:211-214 with the CTF folded on
+// the particle) comes from nrm_p[o], or, with kFused, is accumulated
+// alongside the MFMA work from the template chunks already in LDS: that
+// pays where the template rows are gathered (candidate lists) and the K
+// loop waits on memory, not where it streams the bank (the exhaustive
+// search, whose norms k_search_norm computes).  Outputs are indexed by list
+// position.
+template <typename T, bool kFused>
 __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int ncol, int nbank,
                                           int kpad, const T* __restrict__ bank,
                                           const int* __restrict__ cand_p,
                                           const float2* __restrict__ apv,
-                                          const float* __restrict__ c2p, float s2,
+                                          const float* __restrict__ c2p,
+                                          const float* __restrict__ nrm_p, float s2,
                                           const float2* __restrict__ shx_tab,
                                           const float2* __restrict__ shy_tab, float* score_p,
                                           int* brow_p) {
@@ -272,7 +309,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
   for (int rt = 0; rt < 4; rt++)
 #pragma unroll
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
-  float nacc[2] = {0.f, 0.f};  // norm partials of columns wc*128 + (2wr+u)*16 + lane%16
+  float nacc = 0.f;  // norm partial: column tid/2, complex 8*(tid%2)..+7 of each chunk
 
   // this thread's B pieces: piece bq of templates bc0 + u * bstep
   const int bq = tid % BP::per_row;
@@ -321,11 +358,8 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     }
     treg = v;
   };
-  // phase entries as float2; the 16 weights as plain floats after them
   auto store_tab = [&](int tbuf) {
-    if (tid < 32 * kShift) tabx(tbuf)[tid] = treg;
-    else if (tid < L::tab_entries)
-      reinterpret_cast<float*>(tabx(tbuf) + 32 * kShift)[tid - 32 * kShift] = treg.x;
+    if (tid < L::tab_entries) tabx(tbuf)[tid] = treg;
   };
   // A chunk: row (sx, sy) = X'(k) e^{i k.s}; two threads per row, 8 complex each
   const int row = tid >> 1, hk = (tid & 1) * 8;
@@ -353,6 +387,22 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
       *reinterpret_cast<uint4*>(dst + g * E) = *reinterpret_cast<const uint4*>(v);
     }
   };
+  // norm: |T|^2 of this thread's 8 complex of the chunk in buf, weighted by |c|^2
+  auto norm_step = [&](int buf, int tbuf) {
+    constexpr int E = 16 / sizeof(T);
+    const T* src = Bs(buf) + row * P + 2 * hk;
+    const float2* w = tabx(tbuf) + 32 * kShift + hk;
+    T v[16];
+#pragma unroll
+    for (int g = 0; g < 16 / E; g++)
+      *reinterpret_cast<uint4*>(v + g * E) = *reinterpret_cast<const uint4*>(src + g * E);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      float re = Elem<T>::f(v[2 * q]), im = Elem<T>::f(v[2 * q + 1]);
+      nacc += w[q].x * (re * re + im * im);
+    }
+  };
+
   // prologue: tables for chunks 0 and 1, operands for chunk 0
   load_tab(0);
   store_tab(0);
@@ -371,8 +421,8 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     const bool more = kc + 1 < nchunk;
     if (more) load_b(kc + 1);
     if (kc + 2 < nchunk) load_tab(kc + 2);
-    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane,
-                reinterpret_cast<const float*>(tabx(kc % 3) + 32 * kShift), nacc);
+    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
+    if (kFused) norm_step(cur, kc % 3);
     if (more) {
       gen_a(nxt, (kc + 1) % 3);
       store_b(nxt);
@@ -384,12 +434,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
   // ------------------------------------------------------------ epilogue
   float* ep = reinterpret_cast<float*>(smem_raw);  // [kRows][65] floats, reuses A/B space
   float* nl = reinterpret_cast<float*>(smem_raw + L::tab_off);
-#pragma unroll
-  for (int u = 0; u < 2; u++) {
-    nacc[u] += __shfl_xor(nacc[u], 16, 64);
-    nacc[u] += __shfl_xor(nacc[u], 32, 64);
-    if (lane < 16) nl[wc * 128 + (2 * wr + u) * 16 + lane] = nacc[u];
-  }
+  if (kFused) nl[tid] = nacc;
 #pragma unroll
   for (int q4 = 0; q4 < 4; q4++) {  // 64-column quarters of the 256 templates
     if (q4) __syncthreads();
@@ -421,7 +466,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
       for (int dx = -1; dx <= 1; dx++)
         for (int dy = -1; dy <= 1; dy++) box += ep[((bx + dx + 7) * kShift + (by + dy + 7)) * 65 + c];
       if (o < ncol) {
-        float n = nl[cc];
+        float n = kFused ? nl[2 * cc] + nl[2 * cc + 1] : nrm_p[o];
         float v = (n > 0.f && s2 > 0.f) ? box / sqrtf(n) / sqrtf(s2) / 9.0f : -3.0e38f;
         score_p[o] = v;
         brow_p[o] = (bx + 7) * kShift + (by + 7);
@@ -432,15 +477,17 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
 }
 
 // Uniform lists: nrow columns per particle (the whole bank when cand is
-// null, else the particle's row of cand).  XCD-aware mapping: blocks are
+// null, else the particle's row of cand); norms from nrm (p x nrow) unless
+// kFused.  XCD-aware mapping: blocks are
 // dealt round-robin to the 8 XCDs; each XCD walks its own template tiles with
 // the particles innermost, so a template tile stays resident in that XCD's L2
 // while every particle uses it.
-template <typename T>
+template <typename T, bool kFused>
 __global__ void __launch_bounds__(kGemmThreads)
     k_search_gemm(int nrow, int nbank, int kpad, int np, int ntile, const T* __restrict__ bank,
                   const int* __restrict__ cand, const float2* __restrict__ ap,
-                  const float* __restrict__ c2, const float* __restrict__ sum2,
+                  const float* __restrict__ c2, const float* __restrict__ nrm,
+                  const float* __restrict__ sum2,
                   const float2* __restrict__ shx_tab, const float2* __restrict__ shy_tab,
                   float* score, int* best_row) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem_raw[];
@@ -451,9 +498,10 @@ __global__ void __launch_bounds__(kGemmThreads)
   if (tile >= ntile) return;
   const int half = kpad / 2;
   const size_t base = (size_t)p * nrow;
-  gemm_tile<T>(smem_raw, tile * kCols, nrow, nbank, kpad, bank, cand ? cand + base : nullptr,
-               ap + (size_t)p * half, c2 + (size_t)p * half, sum2[p], shx_tab, shy_tab,
-               score + base, best_row + base);
+  gemm_tile<T, kFused>(smem_raw, tile * kCols, nrow, nbank, kpad, bank,
+                       cand ? cand + base : nullptr, ap + (size_t)p * half, c2 + (size_t)p * half,
+                       kFused ? nullptr : nrm + base, sum2[p], shx_tab, shy_tab, score + base,
+                       best_row + base);
 }
 
 // Ragged lists: particle p's list is cand[loff[p] .. loff[p+1]) and covers
@@ -484,9 +532,9 @@ __global__ void __launch_bounds__(kGemmThreads)
     const int base = loff[p];
     const int end = loff[p + 1] < cap ? loff[p + 1] : cap;
     if (base < 0 || end <= base) continue;  // uniform across the workgroup
-    gemm_tile<T>(smem_raw, (t - toff[p]) * kCols, end - base, nbank, kpad, bank, cand + base,
-                 ap + (size_t)p * half, c2 + (size_t)p * half, sum2[p], shx_tab, shy_tab,
-                 score + base, best_row + base);
+    gemm_tile<T, true>(smem_raw, (t - toff[p]) * kCols, end - base, nbank, kpad, bank,
+                       cand + base, ap + (size_t)p * half, c2 + (size_t)p * half, nullptr,
+                       sum2[p], shx_tab, shy_tab, score + base, best_row + base);
   }
 }
 
@@ -605,10 +653,39 @@ int set_gemm_lds(const void* fn, size_t lds) {
 }
 }  // namespace
 
+int launch_search_norm(Ctx& c, int norient, int kpad, int np, const void* bank, bool bf16,
+                       const float* c2, float* nrm) {
+  dim3 grid((norient + 127) / 128, (np + 63) / 64);
+  if (bf16)
+    hipLaunchKernelGGL(k_search_norm<__bf16>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
+                       reinterpret_cast<const __bf16*>(bank), c2, nrm);
+  else
+    hipLaunchKernelGGL(k_search_norm<float>, grid, dim3(kThreads), 0, c.stream, norient, kpad, np,
+                       reinterpret_cast<const float*>(bank), c2, nrm);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+namespace {
+template <typename T, bool kFused>
+int launch_gemm_t(Ctx& c, size_t nblk, size_t lds, int nrow, int nbank, int kpad, int np,
+                  int ntile, const void* bank, const int* cand, const float2* ap, const float* c2,
+                  const float* nrm, const float* sum2, const float2* sx, const float2* sy,
+                  float* score, int* best_row) {
+  const void* fn = (const void*)k_search_gemm<T, kFused>;
+  if (int r = set_gemm_lds<T>(fn, lds)) return r;
+  hipLaunchKernelGGL((k_search_gemm<T, kFused>), dim3((unsigned)nblk), dim3(kGemmThreads), lds,
+                     c.stream, nrow, nbank, kpad, np, ntile, reinterpret_cast<const T*>(bank), cand,
+                     ap, c2, nrm, sum2, sx, sy, score, best_row);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+}  // namespace
+
 int launch_search_gemm(Ctx& c, int nrow, int nbank, int kpad, int np, const void* bank,
                        bool bf16, const int* cand, const float* ap, const float* c2,
-                       const float* sum2, const float* shx_tab, const float* shy_tab, float* score,
-                       int* best_row) {
+                       const float* nrm, const float* sum2, const float* shx_tab,
+                       const float* shy_tab, float* score, int* best_row) {
   const int ntile = (nrow + kCols - 1) / kCols;
   const int ntile8 = (ntile + 7) / 8 * 8;
   const size_t nblk = (size_t)ntile8 * np;
@@ -617,21 +694,15 @@ int launch_search_gemm(Ctx& c, int nrow, int nbank, int kpad, int np, const void
   const float2* apv = reinterpret_cast<const float2*>(ap);
   const float2* sx = reinterpret_cast<const float2*>(shx_tab);
   const float2* sy = reinterpret_cast<const float2*>(shy_tab);
-  if (bf16) {
-    if (int r = set_gemm_lds<__bf16>((const void*)k_search_gemm<__bf16>, lds)) return r;
-    hipLaunchKernelGGL(k_search_gemm<__bf16>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, nrow, nbank, kpad, np, ntile,
-                       reinterpret_cast<const __bf16*>(bank), cand, apv, c2, sum2, sx, sy, score,
-                       best_row);
-  } else {
-    if (int r = set_gemm_lds<float>((const void*)k_search_gemm<float>, lds)) return r;
-    hipLaunchKernelGGL(k_search_gemm<float>, dim3((unsigned)nblk), dim3(kGemmThreads), lds,
-                       c.stream, nrow, nbank, kpad, np, ntile,
-                       reinterpret_cast<const float*>(bank), cand, apv, c2, sum2, sx, sy, score,
-                       best_row);
-  }
-  FR_HIP(hipGetLastError());
-  return 0;
+  if (bf16)
+    return nrm ? launch_gemm_t<__bf16, false>(c, nblk, lds, nrow, nbank, kpad, np, ntile, bank,
+                                              cand, apv, c2, nrm, sum2, sx, sy, score, best_row)
+               : launch_gemm_t<__bf16, true>(c, nblk, lds, nrow, nbank, kpad, np, ntile, bank,
+                                             cand, apv, c2, nrm, sum2, sx, sy, score, best_row);
+  return nrm ? launch_gemm_t<float, false>(c, nblk, lds, nrow, nbank, kpad, np, ntile, bank, cand,
+                                           apv, c2, nrm, sum2, sx, sy, score, best_row)
+             : launch_gemm_t<float, true>(c, nblk, lds, nrow, nbank, kpad, np, ntile, bank, cand,
+                                          apv, c2, nrm, sum2, sx, sy, score, best_row);
 }
 
 int launch_search_gemm_ragged(Ctx& c, int np, int nbank, int kpad, const void* bank, bool bf16,

@@ -1,9 +1,10 @@
 """Ragged candidate lists of the learned search, on the GPU.
 
-Bars (all bit for bit; the paths run the same kernels on the same inputs):
-- fr_search_ragged_batch gives, for a particle whose list is the whole bank
-  in order with the bank's view group, exactly fr_search_batch's answer, and
-  for any other list exactly fr_search_list_batch's answer on that list;
+Bars (bit for bit where the paths run the same kernels on the same inputs):
+- fr_search_ragged_batch gives exactly fr_search_list_batch's answer on each
+  particle's list; for a list that is the whole bank in order with the
+  bank's view group it gives fr_search_batch's answer up to the summation
+  order of the normalisation (scores to 1e-5, the same best view);
 - fr_build_lists builds the lists a numpy construction builds from the same
   classes, probabilities and cells;
 - fr_rescore_batch gives the scores of CC3M at the host-built candidate
@@ -63,6 +64,12 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
     npart = outc.shape[0]
     topk = 8
     o_d, r_d, s_d = [x.cpu().numpy() for x in gs.candidates(outc, ctff, topk)]
+    bank_rows = torch.arange(gs.norient, dtype=torch.int32, device=DEV).repeat(npart, 1)
+    o_f, r_f, s_f = [x.cpu().numpy() for x in ctx.search_list(
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, bank_rows.contiguous(),
+        gs.group, topk=topk)]
+    assert np.array_equal(o_f[:, 0], o_d[:, 0])
+    assert np.allclose(s_f, s_d, rtol=1e-5, atol=0)
     # every particle: 6 random views (with their whole in-plane groups)
     nview = gs.norient // gs.group
     rows = []
@@ -90,7 +97,7 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
         nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, loff, grp, cand, topk)]
     for p in range(npart):
         if p in full:
-            ref = (o_d[p], r_d[p], s_d[p])
+            ref = (o_f[p], r_f[p], s_f[p])
         elif p == 7:
             assert np.all(o_r[p] == -1) and np.all(r_r[p] == -1)
             continue
@@ -216,7 +223,10 @@ def test_learned_run_equals_composed_host_path(ctx, setup):
                                       outc[one].contiguous(), ctff[one].contiguous(), cand,
                                       cfg.group, topk=8)
         else:
-            o, r, s = gs.candidates(outc[one].contiguous(), ctff[one].contiguous(), 8)
+            cand = torch.arange(gs.norient, dtype=torch.int32, device=DEV)[None, :].contiguous()
+            o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank,
+                                      outc[one].contiguous(), ctff[one].contiguous(), cand,
+                                      gs.group, topk=8)
         o_all[p], r_all[p], s_all[p] = o.cpu().numpy()[0], r.cpu().numpy()[0], s.cpu().numpy()[0]
     assert np.array_equal(res.cand_o, o_all)
     assert np.array_equal(res.cand_score.view(np.uint32), s_all.view(np.uint32))
