@@ -279,7 +279,7 @@ struct BPieces {
 // loop waits on memory, not where it streams the bank (the exhaustive
 // search, whose norms k_search_norm computes).  Outputs are indexed by list
 // position.
-template <typename T, bool kFused>
+template <typename T, bool kFused, bool kLoop>
 __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int ncol, int nbank,
                                           int kpad, const T* __restrict__ bank,
                                           const int* __restrict__ cand_p,
@@ -353,13 +353,13 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     } else if (tid < 32 * kShift) {
       int e2 = tid - 16 * kShift;
       v = shy_tab[(size_t)(k0 + e2 / kShift) * kShift + e2 % kShift];
-    } else if (tid < L::tab_entries) {
+    } else if (kFused && tid < L::tab_entries) {
       v = make_float2(c2p[k0 + tid - 32 * kShift], 0.f);
     }
     treg = v;
   };
   auto store_tab = [&](int tbuf) {
-    if (tid < L::tab_entries) tabx(tbuf)[tid] = treg;
+    if (tid < (kFused ? L::tab_entries : 32 * kShift)) tabx(tbuf)[tid] = treg;
   };
   // A chunk: row (sx, sy) = X'(k) e^{i k.s}; two threads per row, 8 complex each
   const int row = tid >> 1, hk = (tid & 1) * 8;
@@ -473,7 +473,7 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
       }
     }
   }
-  __syncthreads();  // LDS is reused by the next tile
+  if (kLoop) __syncthreads();  // LDS is reused by the next tile
 }
 
 // Uniform lists: nrow columns per particle (the whole bank when cand is
@@ -498,7 +498,7 @@ __global__ void __launch_bounds__(kGemmThreads)
   if (tile >= ntile) return;
   const int half = kpad / 2;
   const size_t base = (size_t)p * nrow;
-  gemm_tile<T, kFused>(smem_raw, tile * kCols, nrow, nbank, kpad, bank,
+  gemm_tile<T, kFused, false>(smem_raw, tile * kCols, nrow, nbank, kpad, bank,
                        cand ? cand + base : nullptr, ap + (size_t)p * half, c2 + (size_t)p * half,
                        kFused ? nullptr : nrm + base, sum2[p], shx_tab, shy_tab, score + base,
                        best_row + base);
@@ -532,7 +532,7 @@ __global__ void __launch_bounds__(kGemmThreads)
     const int base = loff[p];
     const int end = loff[p + 1] < cap ? loff[p + 1] : cap;
     if (base < 0 || end <= base) continue;  // uniform across the workgroup
-    gemm_tile<T, true>(smem_raw, (t - toff[p]) * kCols, end - base, nbank, kpad, bank,
+    gemm_tile<T, true, true>(smem_raw, (t - toff[p]) * kCols, end - base, nbank, kpad, bank,
                        cand + base, ap + (size_t)p * half, c2 + (size_t)p * half, nullptr,
                        sum2[p], shx_tab, shy_tab, score + base, best_row + base);
   }
