@@ -4,7 +4,9 @@
 // Network: nconv 3x3 / stride 2 / pad 1 convolutions with ReLU, then one
 // fully connected layer to K orientation-class logits.  Activations are
 // channel-major, [C][B][H][W], so every convolution is one GEMM over the
-// batch: out[C_out][B*P] = W[C_out][C_in*9] x cols[C_in*9][B*P] (im2col).
+// batch: out[C_out][B*P] = W[C_out][C_in*9] x cols[C_in*9][B*P], with the
+// im2col operand gathered inside the GEMM on the way forward and made
+// explicit (im2col / col2im) for the gradients.
 // The GEMM runs on the matrix cores (v_mfma_f32_16x16x4f32).  Weights are
 // held as BF16-rounded values (wq) with an FP32 master copy for Adam.
 //
@@ -45,11 +47,15 @@ struct GemmArgs {
   const float* bias;
   int relu;
   int accumulate;
+  // implicit convolution (k_gemm_t<true>): B[k][n] = in[ci][b][2oy+ky-1][2ox+kx-1]
+  // for k = ci*9 + ky*3 + kx, n = b*Ho*Ho + oy*Ho + ox, zero outside the image
+  int conv_B, conv_H;
 };
 
 constexpr int kGP = 80;  // LDS pitch: 16*g + lane -> distinct banks
 
-__global__ void __launch_bounds__(kT) k_gemm(GemmArgs g) {
+template <bool kConv>
+__global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
   // two LDS stages: the next K tile is fetched into registers while the
   // matrix cores work on the current one (one barrier per K tile)
   __shared__ float As[2][16][kGP];
@@ -87,13 +93,38 @@ __global__ void __launch_bounds__(kT) k_gemm(GemmArgs g) {
     }
   }
   float ra[4], rb[4];
+  // implicit convolution: this thread's output positions, fixed over K
+  long long cb[4];
+  int cy[4], cx[4];
+  if (kConv) {
+    const int H = g.conv_H, Ho = H / 2, P = Ho * Ho;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int n = n0 + bn[q];
+      const int b = n / P, pp = n - b * P, oy = pp / Ho, ox = pp - oy * Ho;
+      cb[q] = (long long)b * H * H;
+      cy[q] = 2 * oy - 1;
+      cx[q] = 2 * ox - 1;
+    }
+  }
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int m = m0 + am[q], k = k0 + ak[q];
       ra[q] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
       const int n = n0 + bn[q], k2 = k0 + bk[q];
-      rb[q] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
+      if (kConv) {
+        float v = 0.f;
+        if (n < g.N && k2 < kend) {
+          const int ci = k2 / 9, kk = k2 - 9 * ci, ky = kk / 3, kx = kk - 3 * ky;
+          const int y = cy[q] + ky, x = cx[q] + kx, H = g.conv_H;
+          if (y >= 0 && y < H && x >= 0 && x < H)
+            v = g.B[(long long)ci * g.conv_B * H * H + cb[q] + (long long)y * H + x];
+        }
+        rb[q] = v;
+      } else {
+        rb[q] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
+      }
     }
   };
   auto put = [&](int buf) {
@@ -588,10 +619,25 @@ int gemm(Ctx& c, int M, int N, int K, const float* A, long long sam, long long s
          long long sbk, long long sbn, float* C, long long scm, long long scn,
          const float* bias = nullptr, bool relu = false, bool accumulate = false) {
   GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, ((K + 15) / 16) * 16, 0, bias,
-             relu ? 1 : 0, accumulate ? 1 : 0};
+             relu ? 1 : 0, accumulate ? 1 : 0, 0, 0};
   dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
   FR_CHECK(grid.y < 65536u, "GEMM too tall");
-  hipLaunchKernelGGL(k_gemm, grid, dim3(kT), 0, c.stream, g);
+  hipLaunchKernelGGL(k_gemm_t<false>, grid, dim3(kT), 0, c.stream, g);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// out[C][B*Ho*Ho] = W[C][cin*9] x im2col(in) + bias, ReLU, with the im2col
+// operand gathered inside the GEMM (the same K order as the explicit form)
+int conv_forward(Ctx& c, int C, int cin, int B, int H, const float* W, const float* in,
+                 const float* bias, float* out) {
+  const int Ho = H / 2;
+  const long long NB = (long long)B * Ho * Ho;
+  FR_CHECK(NB < (1ll << 31), "batch too large");
+  const int K = cin * 9, N = (int)NB;
+  GemmArgs g{C, N, K, W, K, 1, in, NB, 1, out, NB, 1, ((K + 15) / 16) * 16, 0, bias, 1, 0, B, H};
+  dim3 grid((N + 63) / 64, (C + 63) / 64, 1);
+  hipLaunchKernelGGL(k_gemm_t<true>, grid, dim3(kT), 0, c.stream, g);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -604,9 +650,9 @@ int gemm_splitk(Ctx& c, int M, int N, int K, const float* A, long long sam, long
   int kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
   nsplit = (K + kchunk - 1) / kchunk;
   GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, part, N, 1, kchunk, (long long)M * N, nullptr,
-             0, 0};
+             0, 0, 0, 0};
   dim3 grid((N + 63) / 64, (M + 63) / 64, nsplit);
-  hipLaunchKernelGGL(k_gemm, grid, dim3(kT), 0, c.stream, g);
+  hipLaunchKernelGGL(k_gemm_t<false>, grid, dim3(kT), 0, c.stream, g);
   FR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_split_sum, dim3(nblk((long long)M * N)), dim3(kT), 0, c.stream,
                      (long long)M * N, nsplit, part, C);
@@ -640,9 +686,11 @@ Ws ws_layout(const CnnLayout& lo, int B, bool backward) {
     size_t ga = (size_t)lo.C * B * lo.hout(l) * lo.hout(l);
     gmax = ga > gmax ? ga : gmax;
   }
-  w.cols = o;
-  o += cols;
+  // the forward gathers its convolution operands in place; only the
+  // backward's weight and input gradients use the explicit im2col buffers
   if (backward) {
+    w.cols = o;
+    o += cols;
     w.dcols = o;
     o += cols;
     w.gA = o;
@@ -729,16 +777,10 @@ int forward_impl(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
                  float* ws, const Ws& w) {
   const float* in = img;
   for (int l = 0; l < lo.L; l++) {
-    const int H = lo.hin(l), Ho = lo.hout(l), cin = lo.cin(l);
-    const long long NB = (long long)B * Ho * Ho;
-    const long long ncol = (long long)cin * 9 * NB;
-    hipLaunchKernelGGL(k_im2col, dim3(nblk(ncol)), dim3(kT), 0, c.stream, cin, B, H, in,
-                       ws + w.cols);
-    FR_HIP(hipGetLastError());
-    FR_CHECK(NB < (1ll << 31), "batch too large");
+    const int H = lo.hin(l), cin = lo.cin(l);
     // out[C][NB] = W[C][cin*9] x cols[cin*9][NB] + b, ReLU
-    if (int r = gemm(c, lo.C, (int)NB, cin * 9, wq + lo.w_off[l], cin * 9, 1, ws + w.cols, NB, 1,
-                     ws + w.act[l], NB, 1, wq + lo.b_off[l], true))
+    if (int r = conv_forward(c, lo.C, cin, B, H, wq + lo.w_off[l], in, wq + lo.b_off[l],
+                             ws + w.act[l]))
       return r;
     in = ws + w.act[l];
   }
