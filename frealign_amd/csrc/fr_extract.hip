@@ -370,7 +370,12 @@ __global__ void __launch_bounds__(kFT)
   }
   const float mean = itot ? tot / (float)itot : tot;
 
-  // ---- rows: x = T (wY)^T, mask, Z^T = T^T x'
+  // ---- rows: x = T (wY)^T, MASKCOS2, Z^T = T^T x'.  The mask gives
+  // x' = mean + (x - mean) w, with w = 1 inside RI2', the cosine edge in the
+  // ring and 0 outside RI3'; since sum_n T[n][k] is N for the DC term (k = 0)
+  // and 0 for every other k, Z^T = N mean e_0 + T^T ((x - mean) w), and tiles
+  // wholly outside the mask add nothing.  The active tiles of a row block go
+  // through the matrix cores four at a time, their chains interleaved.
   for (int rb = wave; rb < mt; rb += kFT / 64) {
     const int m0 = rb * 16;
     int mlo, mhi;
@@ -383,39 +388,79 @@ __global__ void __launch_bounds__(kFT)
     fx4 z[4];
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) z[kt] = fx4{0.f, 0.f, 0.f, 0.f};
+    // active tiles of this row block as a bit set (wave-uniform)
+    unsigned act = 0u;
     for (int nt = 0; nt < mt; nt++) {
-      const int n0 = nt * 16;
+      int nlo, nhi;
+      tile_r2(nt * 16, nsamh, nsam, nlo, nhi);
+      if (!((float)(nlo + mlo) > ri3p)) act |= 1u << nt;
+    }
+    // (x - mean) w of one tile in place
+    auto maskv = [&](int n0, fx4& x) {
       int nlo, nhi;
       tile_r2(n0, nsamh, nsam, nlo, nhi);
-      fx4 x;
-      if ((float)(nlo + mlo) > ri3p) {
-        x = fx4{mean, mean, mean, mean};  // wholly outside the mask
-      } else {
-        x = fx4{0.f, 0.f, 0.f, 0.f};
+      const bool inner = !((float)(nhi + mhi) > ri2p);
 #pragma unroll
-        for (int ks = 0; ks < kFK / 4; ks++)
-          x = mfma4(Ts[(n0 + la) * kFP + 4 * ks + lg], yv[ks], x);
-        if ((float)(nhi + mhi) > ri2p) {
-#pragma unroll
-          for (int i = 0; i < 4; i++) {
-            int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
-            float crad2 = (float)(dx * dx + dy * dy);
-            if (crad2 > ri2p && crad2 <= ri3p) {
-              float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
-              x[i] = x[i] * edge + (1.0f - edge) * mean;
-            } else if (crad2 > ri3p) {
-              x[i] = mean;
-            }
+      for (int i = 0; i < 4; i++) {
+        float v = x[i] - mean;
+        if (!inner) {
+          int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
+          float crad2 = (float)(dx * dx + dy * dy);
+          if (crad2 > ri3p) {
+            v = 0.f;
+          } else if (crad2 > ri2p) {
+            float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+            v = v * edge;
           }
         }
+        x[i] = v;
       }
-      // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
+    };
+    // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
+    auto zacc = [&](int n0, const fx4& x) {
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int kt = 0; kt < 4; kt++)
           z[kt] = mfma4(Ts[(n0 + 4 * lg + i) * kFP + 16 * kt + la], x[i], z[kt]);
+    };
+    while (__builtin_popcount(act) >= 4) {
+      const int t0 = __builtin_ctz(act);
+      act &= act - 1u;
+      const int t1 = __builtin_ctz(act);
+      act &= act - 1u;
+      const int t2 = __builtin_ctz(act);
+      act &= act - 1u;
+      const int t3 = __builtin_ctz(act);
+      act &= act - 1u;
+      fx4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = x0, x2 = x0, x3 = x0;
+#pragma unroll
+      for (int ks = 0; ks < kFK / 4; ks++) {
+        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
+        x1 = mfma4(Ts[(16 * t1 + la) * kFP + 4 * ks + lg], yv[ks], x1);
+        x2 = mfma4(Ts[(16 * t2 + la) * kFP + 4 * ks + lg], yv[ks], x2);
+        x3 = mfma4(Ts[(16 * t3 + la) * kFP + 4 * ks + lg], yv[ks], x3);
+      }
+      maskv(16 * t0, x0);
+      maskv(16 * t1, x1);
+      maskv(16 * t2, x2);
+      maskv(16 * t3, x3);
+      zacc(16 * t0, x0);
+      zacc(16 * t1, x1);
+      zacc(16 * t2, x2);
+      zacc(16 * t3, x3);
     }
+    while (act) {
+      const int t0 = __builtin_ctz(act);
+      act &= act - 1u;
+      fx4 x0 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int ks = 0; ks < kFK / 4; ks++)
+        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
+      maskv(16 * t0, x0);
+      zacc(16 * t0, x0);
+    }
+    if (lg == 0) z[0][0] += mean * (float)nsam;  // row k = 0, the DC term
     // Z[m][k]: this wave alone read rows m0..m0+15 of Ys
 #pragma unroll
     for (int kt = 0; kt < 4; kt++)
