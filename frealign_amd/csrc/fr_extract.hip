@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(kFT)
                  const float* __restrict__ shift, const float* __restrict__ amag_arr,
                  const float* __restrict__ fsct, const float* __restrict__ tab,
                  const int* __restrict__ pts, const float* __restrict__ ring_tab, int ring_cnt,
-                 float ri2, float ri3, float rih, float halfw, float* score) {
+                 const float* __restrict__ edge_tab, int edge_c0, int edge_n, float ri2,
+                 float ri3, float rih, float halfw, float* score) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* Ts = lds;                      // [nsam][kFP]
   float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
@@ -303,16 +304,10 @@ __global__ void __launch_bounds__(kFT)
   const int mt = nsam / 16;
   for (int t = wave; t < mt * 4; t += kFT / 64) {
     const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
-    float ta[kFK / 4], tb[kFK / 4];
-#pragma unroll
-    for (int ks = 0; ks < kFK / 4; ks++) {
-      ta[ks] = Ts[(m0 + la) * kFP + 4 * ks + lg];
-      tb[ks] = Bs[(c0 + la) * kFP + 4 * ks + lg];
-    }
-    __builtin_amdgcn_sched_barrier(0);
     fx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int ks = 0; ks < kFK / 4; ks++) acc = mfma4(ta[ks], tb[ks], acc);
+    for (int ks = 0; ks < kFK / 4; ks++)
+      acc = mfma4(Ts[(m0 + la) * kFP + 4 * ks + lg], Bs[(c0 + la) * kFP + 4 * ks + lg], acc);
     const float w = (c0 + la) < 2 ? 1.0f : 2.0f;
 #pragma unroll
     for (int i = 0; i < 4; i++) Ys[(m0 + 4 * lg + i) * kFP + c0 + la] = acc[i] * w;
@@ -402,6 +397,7 @@ __global__ void __launch_bounds__(kFT)
       if (!((float)(nlo + mlo) > ri3p)) act |= 1u << nt;
     }
     // (x - mean) w of one tile in place
+    const bool use_edge = edge_tab != nullptr && edge_n > 0 && amagp == 1.0f;
     auto maskv = [&](int n0, fx4& x) {
       int nlo, nhi;
       tile_r2(n0, nsamh, nsam, nlo, nhi);
@@ -415,7 +411,13 @@ __global__ void __launch_bounds__(kFT)
           if (crad2 > ri3p) {
             v = 0.f;
           } else if (crad2 > ri2p) {
-            float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+            float edge;
+            if (use_edge) {
+              int e = (int)crad2 - edge_c0;
+              edge = edge_tab[e < 0 ? 0 : (e >= edge_n ? edge_n - 1 : e)];
+            } else {
+              edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+            }
             v = v * edge;
           }
         }
@@ -423,23 +425,12 @@ __global__ void __launch_bounds__(kFT)
       }
     };
     // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
-    // T operands of a tile, read from LDS ahead of the matrix-core work
-    // (a scheduling barrier keeps the reads from being sunk next to each use)
-    auto load_x = [&](int n0, float (&tx)[kFK / 4]) {
-#pragma unroll
-      for (int ks = 0; ks < kFK / 4; ks++) tx[ks] = Ts[(n0 + la) * kFP + 4 * ks + lg];
-    };
-    auto load_z = [&](int n0, float (&tz)[16]) {
+    auto zacc = [&](int n0, const fx4& x) {
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int kt = 0; kt < 4; kt++) tz[4 * i + kt] = Ts[(n0 + 4 * lg + i) * kFP + 16 * kt + la];
-    };
-    auto zacc_r = [&](const float (&tz)[16], const fx4& x) {
-#pragma unroll
-      for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int kt = 0; kt < 4; kt++) z[kt] = mfma4(tz[4 * i + kt], x[i], z[kt]);
+        for (int kt = 0; kt < 4; kt++)
+          z[kt] = mfma4(Ts[(n0 + 4 * lg + i) * kFP + 16 * kt + la], x[i], z[kt]);
     };
     while (__builtin_popcount(act) >= 4) {
       const int t0 = __builtin_ctz(act);
@@ -450,46 +441,32 @@ __global__ void __launch_bounds__(kFT)
       act &= act - 1u;
       const int t3 = __builtin_ctz(act);
       act &= act - 1u;
-      float ta0[kFK / 4], ta1[kFK / 4], ta2[kFK / 4], ta3[kFK / 4];
-      float tb0[16], tb1[16], tb2[16], tb3[16];
-      load_x(16 * t0, ta0);
-      load_x(16 * t1, ta1);
-      load_x(16 * t2, ta2);
-      load_x(16 * t3, ta3);
-      load_z(16 * t0, tb0);
-      load_z(16 * t1, tb1);
-      load_z(16 * t2, tb2);
-      load_z(16 * t3, tb3);
-      __builtin_amdgcn_sched_barrier(0);
       fx4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = x0, x2 = x0, x3 = x0;
 #pragma unroll
       for (int ks = 0; ks < kFK / 4; ks++) {
-        x0 = mfma4(ta0[ks], yv[ks], x0);
-        x1 = mfma4(ta1[ks], yv[ks], x1);
-        x2 = mfma4(ta2[ks], yv[ks], x2);
-        x3 = mfma4(ta3[ks], yv[ks], x3);
+        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
+        x1 = mfma4(Ts[(16 * t1 + la) * kFP + 4 * ks + lg], yv[ks], x1);
+        x2 = mfma4(Ts[(16 * t2 + la) * kFP + 4 * ks + lg], yv[ks], x2);
+        x3 = mfma4(Ts[(16 * t3 + la) * kFP + 4 * ks + lg], yv[ks], x3);
       }
       maskv(16 * t0, x0);
       maskv(16 * t1, x1);
       maskv(16 * t2, x2);
       maskv(16 * t3, x3);
-      zacc_r(tb0, x0);
-      zacc_r(tb1, x1);
-      zacc_r(tb2, x2);
-      zacc_r(tb3, x3);
+      zacc(16 * t0, x0);
+      zacc(16 * t1, x1);
+      zacc(16 * t2, x2);
+      zacc(16 * t3, x3);
     }
     while (act) {
       const int t0 = __builtin_ctz(act);
       act &= act - 1u;
-      float ta0[kFK / 4], tb0[16];
-      load_x(16 * t0, ta0);
-      load_z(16 * t0, tb0);
-      __builtin_amdgcn_sched_barrier(0);
       fx4 x0 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int ks = 0; ks < kFK / 4; ks++) x0 = mfma4(ta0[ks], yv[ks], x0);
+      for (int ks = 0; ks < kFK / 4; ks++)
+        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
       maskv(16 * t0, x0);
-      zacc_r(tb0, x0);
+      zacc(16 * t0, x0);
     }
     if (lg == 0) z[0][0] += mean * (float)nsam;  // row k = 0, the DC term
     // Z[m][k]: this wave alone read rows m0..m0+15 of Ys
@@ -504,23 +481,12 @@ __global__ void __launch_bounds__(kFT)
   for (int t = wave; t < 16; t += kFT / 64) {
     const int r0 = (t >> 2) * 16, c0 = (t & 3) * 16;
     fx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    // four 16-row steps of operands read ahead of their eight MFMA pairs
-    for (int m0 = 0; m0 < nsam; m0 += 64) {
-      const int nstep = (nsam - m0) / 16;  // NSAM % 32 == 0: 2 or >= 4
-      float ta[16], tb[16];
+    for (int m0 = 0; m0 < nsam; m0 += 16) {
 #pragma unroll
-      for (int u = 0; u < 4; u++)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int ma = m0 + 16 * u + 4 * lg + i;
-          ta[4 * u + i] = u < nstep ? Ts[ma * kFP + r0 + la] : 0.f;
-          tb[4 * u + i] = u < nstep ? Ys[ma * kFP + c0 + la] : 0.f;
-        }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int e = 0; e < 16; e += 2) {
-        acc0 = mfma4(ta[e], tb[e], acc0);
-        acc1 = mfma4(ta[e + 1], tb[e + 1], acc1);
+      for (int i = 0; i < 4; i += 2) {
+        const int ma = m0 + 4 * lg + i, mb = ma + 1;
+        acc0 = mfma4(Ts[ma * kFP + r0 + la], Ys[ma * kFP + c0 + la], acc0);
+        acc1 = mfma4(Ts[mb * kFP + r0 + la], Ys[mb * kFP + c0 + la], acc1);
       }
     }
 #pragma unroll
@@ -665,6 +631,19 @@ int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& df
   return cnt;
 }
 
+// MASKCOS2's edge weight for the integer squared radii of the ring
+// RI2 < r2 <= RI3 (maskcos2.f: (1 + COS(PI (SQRT(r2) - RIH) / HALFW)) / 2).
+void cc3m_fused_edge(float ri2, float ri3, float rih, float halfw, std::vector<float>& edge,
+                     int* c0) {
+  const int lo = (int)std::floor(ri2) + 1, hi = (int)std::floor(ri3);
+  edge.clear();
+  *c0 = lo;
+  for (int r2 = lo; r2 <= hi; r2++) {
+    const float c = (float)r2;
+    edge.push_back((1.0f + cosf(kPi * (sqrtf(c) - rih) / halfw)) / 2.0f);
+  }
+}
+
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
                       const float* amag, const float* fsct, const Cc3mTables& t, float* score,
@@ -681,8 +660,8 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
                      reinterpret_cast<const float2*>(a3df), dm, pidx,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),
-                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, a.ri2, a.ri3, a.rih,
-                     a.halfw, score);
+                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, t.edge, t.edge_c0,
+                     t.edge_n, a.ri2, a.ri3, a.rih, a.halfw, score);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -161,10 +161,12 @@ def test_pextract_vs_oracle(ctx, nsam):
 
 
 # ----------------------------------------------------------------- CC3M
-# (64, 24), (128, 32), (256, 32): band-pruned DFT path (MAXR2 <= 32, NSAM % 32 == 0);
-# (96, 40): the full-FFT path (MAXR2 > 32, NSAM = 2^5 x 3)
+# (64, 24), (128, 32), (256, 32), (160, 24): band-pruned DFT path (MAXR2 <= 32,
+# NSAM % 32 == 0), with AMAG = 1 (ring mean from the precomputed table) or
+# jittered (ring pass on the matrix cores); (96, 40): the full-FFT path
+# (MAXR2 > 32, NSAM = 2^5 x 3)
 @pytest.mark.parametrize("nsam,maxr2,amag_jitter", [(64, 24, 0.0), (128, 32, 0.02), (256, 32, 0.02),
-                                                    (96, 40, 0.02)])
+                                                    (160, 24, 0.0), (96, 40, 0.02)])
 def test_cc3m_vs_oracle(ctx, nsam, maxr2, amag_jitter):
     jc = nsam // 2 + 1
     L = O.oracle()
@@ -197,7 +199,9 @@ def test_cc3m_vs_oracle(ctx, nsam, maxr2, amag_jitter):
         r32 = abs(got[k] - s32) / abs(s32)
         print(f"item {k}: gpu {got[k]:.7f} or32 {s32:.7f} or64 {s64:.7f} rel32 {r32:.1e} "
               f"rel64 {abs(got[k] - s64) / abs(s64):.1e}")
-        assert r32 < 1e-4
+        # 1e-4 relative; a score near zero is a difference of shell terms of
+        # order 1e-2, so its error is bounded by 1e-4 of 1e-3 instead
+        assert abs(got[k] - s32) <= 1e-4 * max(abs(s32), 1e-3)
 
 
 # ----------------------------------------------------------------- PRESB
