@@ -157,20 +157,16 @@ int fr_ctx::cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t) {
   }
   t->ring = ri->second.d;
   t->ring_cnt = ri->second.cnt;
-  const auto ek = std::make_tuple(a.ri2, a.ri3, a.rih, a.halfw);
-  auto ei = edge_tabs.find(ek);
-  if (ei == edge_tabs.end()) {
-    std::vector<float> edge;
-    EdgeTab et;
-    fr::cc3m_fused_edge(a.ri2, a.ri3, a.rih, a.halfw, edge, &et.c0);
-    et.n = (int)edge.size();
-    if (et.n > 0)
-      if (int r = upload_table(edge, &et.d)) return r;
-    ei = edge_tabs.emplace(ek, et).first;
+  const auto wk = std::make_tuple(a.nsam, a.ri2, a.ri3, a.rih, a.halfw);
+  auto wi = wmap_tabs.find(wk);
+  if (wi == wmap_tabs.end()) {
+    std::vector<float> w;
+    fr::cc3m_fused_wmap(a.nsam, a.ri2, a.ri3, a.rih, a.halfw, w);
+    float* d = nullptr;
+    if (int r = upload_table(w, &d)) return r;
+    wi = wmap_tabs.emplace(wk, d).first;
   }
-  t->edge = ei->second.d;
-  t->edge_c0 = ei->second.c0;
-  t->edge_n = ei->second.n;
+  t->wmap = wi->second;
   return 0;
 }
 
@@ -245,8 +241,7 @@ int fr_destroy(fr_ctx* c) {
   for (auto& e : c->dft_tabs) (void)hipFree(e.second);
   for (auto& e : c->pts_tabs) (void)hipFree(e.second);
   for (auto& e : c->ring_tabs) (void)hipFree(e.second.d);
-  for (auto& e : c->edge_tabs)
-    if (e.second.d) (void)hipFree(e.second.d);
+  for (auto& e : c->wmap_tabs) (void)hipFree(e.second);
   if (c->d_shx) (void)hipFree(c->d_shx);
   if (c->d_shy) (void)hipFree(c->d_shy);
   delete c;

@@ -29,11 +29,8 @@ struct fr_ctx : public fr::Ctx {
     int cnt = 0;
   };
   std::map<std::tuple<int, float, float>, RingTab> ring_tabs;  // per (NSAM, RI2, RI3)
-  struct EdgeTab {
-    float* d = nullptr;
-    int c0 = 0, n = 0;
-  };
-  std::map<std::tuple<float, float, float, float>, EdgeTab> edge_tabs;  // per (RI2, RI3, RIH, HALFW)
+  // per (NSAM, RI2, RI3, RIH, HALFW)
+  std::map<std::tuple<int, float, float, float, float>, float*> wmap_tabs;
   int cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t);
   // launch choices per box size (set by fr_autotune_cc3m)
   struct Tune {

@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(kFT)
                  const float* __restrict__ shift, const float* __restrict__ amag_arr,
                  const float* __restrict__ fsct, const float* __restrict__ tab,
                  const int* __restrict__ pts, const float* __restrict__ ring_tab, int ring_cnt,
-                 const float* __restrict__ edge_tab, int edge_c0, int edge_n, float ri2,
+                 const float* __restrict__ wmap, float ri2,
                  float ri3, float rih, float halfw, float* score) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* Ts = lds;                      // [nsam][kFP]
@@ -314,6 +314,7 @@ __global__ void __launch_bounds__(kFT)
   }
   __syncthreads();
 
+  const bool use_wmap = wmap != nullptr && amagp == 1.0f;
   // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3'.  x is linear in Y,
   // so at AMAG = 1 the ring sum is sum_{m,k} R[m][k] Y[m][k] with the
   // precomputed R; otherwise a pass over the ring tiles on the matrix cores.
@@ -396,33 +397,28 @@ __global__ void __launch_bounds__(kFT)
       tile_r2(nt * 16, nsamh, nsam, nlo, nhi);
       if (!((float)(nlo + mlo) > ri3p)) act |= 1u << nt;
     }
-    // (x - mean) w of one tile in place
-    const bool use_edge = edge_tab != nullptr && edge_n > 0 && amagp == 1.0f;
-    auto maskv = [&](int n0, fx4& x) {
+    // MASKCOS2 weights w of this lane's pixels (n0 + 4 lg + i, m) of a tile:
+    // 1 inside RI2', from the precomputed map at AMAG = 1, else per pixel
+    auto weights = [&](int n0) -> fx4 {
       int nlo, nhi;
       tile_r2(n0, nsamh, nsam, nlo, nhi);
-      const bool inner = !((float)(nhi + mhi) > ri2p);
+      if (!((float)(nhi + mhi) > ri2p)) return fx4{1.f, 1.f, 1.f, 1.f};
+      if (use_wmap) return *reinterpret_cast<const fx4*>(wmap + (size_t)m * nsam + n0 + 4 * lg);
+      fx4 w;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        float v = x[i] - mean;
-        if (!inner) {
-          int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
-          float crad2 = (float)(dx * dx + dy * dy);
-          if (crad2 > ri3p) {
-            v = 0.f;
-          } else if (crad2 > ri2p) {
-            float edge;
-            if (use_edge) {
-              int e = (int)crad2 - edge_c0;
-              edge = edge_tab[e < 0 ? 0 : (e >= edge_n ? edge_n - 1 : e)];
-            } else {
-              edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
-            }
-            v = v * edge;
-          }
-        }
-        x[i] = v;
+        int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
+        float crad2 = (float)(dx * dx + dy * dy);
+        float e = 1.0f;
+        if (crad2 > ri3p) e = 0.0f;
+        else if (crad2 > ri2p) e = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+        w[i] = e;
       }
+      return w;
+    };
+    auto maskv = [&](const fx4& w, fx4& x) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) x[i] = (x[i] - mean) * w[i];
     };
     // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
     auto zacc = [&](int n0, const fx4& x) {
@@ -441,6 +437,8 @@ __global__ void __launch_bounds__(kFT)
       act &= act - 1u;
       const int t3 = __builtin_ctz(act);
       act &= act - 1u;
+      const fx4 w0 = weights(16 * t0), w1 = weights(16 * t1), w2 = weights(16 * t2),
+                w3 = weights(16 * t3);
       fx4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = x0, x2 = x0, x3 = x0;
 #pragma unroll
       for (int ks = 0; ks < kFK / 4; ks++) {
@@ -449,10 +447,10 @@ __global__ void __launch_bounds__(kFT)
         x2 = mfma4(Ts[(16 * t2 + la) * kFP + 4 * ks + lg], yv[ks], x2);
         x3 = mfma4(Ts[(16 * t3 + la) * kFP + 4 * ks + lg], yv[ks], x3);
       }
-      maskv(16 * t0, x0);
-      maskv(16 * t1, x1);
-      maskv(16 * t2, x2);
-      maskv(16 * t3, x3);
+      maskv(w0, x0);
+      maskv(w1, x1);
+      maskv(w2, x2);
+      maskv(w3, x3);
       zacc(16 * t0, x0);
       zacc(16 * t1, x1);
       zacc(16 * t2, x2);
@@ -461,11 +459,12 @@ __global__ void __launch_bounds__(kFT)
     while (act) {
       const int t0 = __builtin_ctz(act);
       act &= act - 1u;
+      const fx4 w0 = weights(16 * t0);
       fx4 x0 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < kFK / 4; ks++)
         x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
-      maskv(16 * t0, x0);
+      maskv(w0, x0);
       zacc(16 * t0, x0);
     }
     if (lg == 0) z[0][0] += mean * (float)nsam;  // row k = 0, the DC term
@@ -631,16 +630,24 @@ int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& df
   return cnt;
 }
 
-// MASKCOS2's edge weight for the integer squared radii of the ring
-// RI2 < r2 <= RI3 (maskcos2.f: (1 + COS(PI (SQRT(r2) - RIH) / HALFW)) / 2).
-void cc3m_fused_edge(float ri2, float ri3, float rih, float halfw, std::vector<float>& edge,
-                     int* c0) {
-  const int lo = (int)std::floor(ri2) + 1, hi = (int)std::floor(ri3);
-  edge.clear();
-  *c0 = lo;
-  for (int r2 = lo; r2 <= hi; r2++) {
-    const float c = (float)r2;
-    edge.push_back((1.0f + cosf(kPi * (sqrtf(c) - rih) / halfw)) / 2.0f);
+// MASKCOS2 (maskcos2.f) weights at AMAG = 1 for every pixel (n, m) of the
+// NSAM x NSAM image, indexed [m][n] with wrapped distances from the origin:
+// 1 for r2 <= RI2, (1 + COS(PI (SQRT(r2) - RIH) / HALFW)) / 2 for
+// RI2 < r2 <= RI3, 0 beyond.
+void cc3m_fused_wmap(int nsam, float ri2, float ri3, float rih, float halfw,
+                     std::vector<float>& w) {
+  const int nsamh = nsam / 2;
+  w.assign((size_t)nsam * nsam, 0.f);
+  for (int m = 0; m < nsam; m++) {
+    const int dy = m < nsamh ? m : m - nsam;
+    for (int n = 0; n < nsam; n++) {
+      const int dx = n < nsamh ? n : n - nsam;
+      const float crad2 = (float)(dx * dx + dy * dy);
+      float e = 1.0f;
+      if (crad2 > ri3) e = 0.0f;
+      else if (crad2 > ri2) e = (1.0f + cosf(kPi * (sqrtf(crad2) - rih) / halfw)) / 2.0f;
+      w[(size_t)m * nsam + n] = e;
+    }
   }
 }
 
@@ -660,8 +667,8 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
                      reinterpret_cast<const float2*>(a3df), dm, pidx,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),
-                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, t.edge, t.edge_c0,
-                     t.edge_n, a.ri2, a.ri3, a.rih, a.halfw, score);
+                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3,
+                     a.rih, a.halfw, score);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -73,9 +73,8 @@ size_t cc3m_fused_lds(int nsam);
 //        sorted by shell INT(SQRT(I*I+J*J)+0.5), (I, J) order within a shell;
 //   ring R[m][k] = sum over MASKCOS2 ring pixels n of row m of T[n][k], for
 //        AMAG = 1 (null: the ring mean is taken by a pass on the matrix cores);
-//   edge the MASKCOS2 cosine edge (1 + COS(PI (SQRT(r2) - RIH) / HALFW)) / 2
-//        for the integer r2 = edge_c0 .. edge_c0 + edge_n - 1 of the ring, at
-//        AMAG = 1 (null: evaluated per pixel).
+//   wmap the MASKCOS2 weight of every pixel [m][n] at AMAG = 1 (null:
+//        evaluated per pixel).
 constexpr int kCc3mShells = 34;
 constexpr int kCc3mPtsHdr = 36;
 struct Cc3mTables {
@@ -83,15 +82,14 @@ struct Cc3mTables {
   const int* pts = nullptr;
   const float* ring = nullptr;
   int ring_cnt = 0;
-  const float* edge = nullptr;
-  int edge_c0 = 0, edge_n = 0;
+  const float* wmap = nullptr;
 };
 void cc3m_fused_table(int nsam, std::vector<float>& tab);
 void cc3m_fused_points(int maxr12, int maxr22, std::vector<int>& pts);
 int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& dft,
                     std::vector<float>& ring);
-void cc3m_fused_edge(float ri2, float ri3, float rih, float halfw, std::vector<float>& edge,
-                     int* c0);
+void cc3m_fused_wmap(int nsam, float ri2, float ri3, float rih, float halfw,
+                     std::vector<float>& w);
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
                       const float* amag, const float* fsct, const Cc3mTables& t, float* score,
