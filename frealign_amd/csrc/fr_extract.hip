@@ -690,7 +690,7 @@ int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, nsam, a.n, work, img)) return r;
   // cc3m.f:114: MASKCOS2 with AMAG of each item (scal 1: already scaled)
-  hipLaunchKernelGGL(k_scale_mask, dim3(a.n), dim3(kThreads), 0, c.stream, nsam, work, img, 1.0f,
+  hipLaunchKernelGGL(k_scale_mask, dim3(a.n), dim3(kMaskThreads), 0, c.stream, nsam, work, img, 1.0f,
                      amag, 1, a.ri2, a.ri3, a.rih, a.halfw, 1);
   FR_HIP(hipGetLastError());
   if (int r = fft_r2c_2d(c, nsam, a.n, work, img)) return r;
@@ -716,7 +716,7 @@ int launch_presb(Ctx& c, int nsam, int n, const float* a3df, const float* spec, 
                      dm, pidx, reinterpret_cast<const float2*>(ctff), work);
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, nsam, n, work, img)) return r;
-  hipLaunchKernelGGL(k_scale_mask, dim3(n), dim3(kThreads), 0, c.stream, nsam, work, img, 1.0f,
+  hipLaunchKernelGGL(k_scale_mask, dim3(n), dim3(kMaskThreads), 0, c.stream, nsam, work, img, 1.0f,
                      amag, 1, ri2, ri3, rih, halfw, 1);
   FR_HIP(hipGetLastError());
   if (int r = fft_r2c_2d(c, nsam, n, work, img)) return r;

@@ -46,6 +46,7 @@ struct Cc3mArgs {
   float rbfact, ri2, ri3, rih, halfw;
 };
 // Scale by `scal`, then MASKCOS (origin=0) / MASKCOS2 (origin=1); one workgroup per image.
+constexpr int kMaskThreads = 1024;
 __global__ void k_scale_mask(int nsam, float* data, size_t img_stride, float scal,
                              const float* amag_arr, int amag_stride, float ri2, float ri3,
                              float rih, float halfw, int origin);

@@ -62,20 +62,23 @@ __device__ inline float block_sum(float v, float* red) {
 
 // ctfapply.f:107-122 and maskcos.f / maskcos2.f: scale, then cosine-edged
 // real-space mask.  origin=0 centres the disc at NSAM/2+1 (MASKCOS), origin=1
-// at the FFT origin (MASKCOS2).  One workgroup per image.
-__global__ void __launch_bounds__(kThreads)
+// at the FFT origin (MASKCOS2).  One workgroup per image: a read-only pass for
+// the mean of the scaled ring, then one pass that scales, masks and writes.
+// Threads take fixed columns and step over rows (no per-pixel division).
+__global__ void __launch_bounds__(kMaskThreads)
     k_scale_mask(int nsam, float* data, size_t img_stride, float scal, const float* amag_arr,
                  int amag_stride, float ri2, float ri3, float rih, float halfw, int origin) {
-  __shared__ float red[kThreads / 64];
-  __shared__ int redi[kThreads / 64];
+  __shared__ float red[kMaskThreads / 64];
+  __shared__ int redi[kMaskThreads / 64];
   float* img = data + blockIdx.x * img_stride;
   const float amagp = amag_arr[(size_t)blockIdx.x * amag_stride];
   const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
   const int nmid = origin ? 1 : nsam / 2 + 1, nsamh = nsam / 2, pitch = nsam + 2;
-  float sum = 0.0f;
-  int isum = 0;
-  for (int e = threadIdx.x; e < nsam * nsam; e += blockDim.x) {
-    int i = e / nsam + 1, jx = e % nsam + 1;
+  const int nt = blockDim.x;
+  const int cols = nsam < nt ? nsam : nt, rstep = nt / cols;
+  const int c0 = threadIdx.x % cols, r0 = threadIdx.x / cols;
+  const bool act = r0 < rstep;
+  auto rad2 = [&](int i, int jx) {  // 1-based pixel (i, jx)
     int ii = i, jj = jx;
     if (origin) {
       if (ii > nsamh) ii -= nsam;
@@ -87,16 +90,20 @@ __global__ void __launch_bounds__(kThreads)
 #pragma clang fp contract(off)
       crad2 = d2 * d2 + d1 * d1;
     }
-    float* px = img + (size_t)(i - 1) * pitch + (jx - 1);
-    float v = *px * scal;
-    *px = v;
-    if (crad2 > ri2p && crad2 <= ri3p) {
-      sum += v;
-      isum++;
-    }
-  }
+    return crad2;
+  };
+  float sum = 0.0f;
+  int isum = 0;
+  if (act)
+    for (int i0 = r0; i0 < nsam; i0 += rstep)
+      for (int j0 = c0; j0 < nsam; j0 += cols) {
+        float crad2 = rad2(i0 + 1, j0 + 1);
+        if (crad2 > ri2p && crad2 <= ri3p) {
+          sum += img[(size_t)i0 * pitch + j0] * scal;
+          isum++;
+        }
+      }
   float tot = block_sum(sum, red);
-  // integer count reduction
   int ci = isum;
   for (int off = 32; off > 0; off >>= 1) ci += __shfl_down(ci, off, 64);
   if ((threadIdx.x & 63) == 0) redi[threadIdx.x >> 6] = ci;
@@ -104,34 +111,28 @@ __global__ void __launch_bounds__(kThreads)
   int itot = 0;
   for (int k = 0; k < (int)(blockDim.x >> 6); k++) itot += redi[k];
   const float mean = itot ? tot / (float)itot : tot;
-  for (int e = threadIdx.x; e < nsam * nsam; e += blockDim.x) {
-    int i = e / nsam + 1, jx = e % nsam + 1;
-    int ii = i, jj = jx;
-    if (origin) {
-      if (ii > nsamh) ii -= nsam;
-      if (jj > nsamh) jj -= nsam;
-    }
-    float d1 = (float)(ii - nmid), d2 = (float)(jj - nmid);
-    float crad2;
-    {
-#pragma clang fp contract(off)
-      crad2 = d2 * d2 + d1 * d1;
-    }
-    float* px = img + (size_t)(i - 1) * pitch + (jx - 1);
-    if (crad2 > ri2p && crad2 <= ri3p) {
-      float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
-      *px = *px * edge + (1.0f - edge) * mean;
-    } else if (crad2 > ri3p) {
-      *px = mean;
-    }
-  }
+  if (act)
+    for (int i0 = r0; i0 < nsam; i0 += rstep)
+      for (int j0 = c0; j0 < nsam; j0 += cols) {
+        float crad2 = rad2(i0 + 1, j0 + 1);
+        float* px = img + (size_t)i0 * pitch + j0;
+        float v = *px * scal;
+        if (crad2 > ri2p && crad2 <= ri3p) {
+          float edge = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+          v = v * edge + (1.0f - edge) * mean;
+        } else if (crad2 > ri3p) {
+          v = mean;
+        }
+        *px = v;
+      }
 }
 
 namespace {
 
 // ctfapply.f:144-158: origin shift of DATC and OUTC, then PWEIGHT on OUTC.
-// One workgroup per particle.  Shell bins are 1-based as in pweight.f.
-__global__ void __launch_bounds__(kThreads)
+// One workgroup per particle; threads take fixed columns L and step over
+// rows M.  Shell bins are 1-based as in pweight.f.
+__global__ void __launch_bounds__(kMaskThreads)
     k_origin_pweight(int nsam, const float* __restrict__ pssnr, int ictf,
                      const float2* __restrict__ ctff, float2* outc, float2* datc) {
   __shared__ float rbuf[2 * 520 + 4];
@@ -140,28 +141,34 @@ __global__ void __launch_bounds__(kThreads)
   float2* oc = outc + blockIdx.x * ic;
   float2* dc = datc + blockIdx.x * ic;
   const float2* cf_ = ctff + blockIdx.x * 2 * ic;
+  const int nt = blockDim.x;
+  const int cols = jc < nt ? jc : nt, rstep = nt / cols;
+  const int l0 = threadIdx.x % cols, r0 = threadIdx.x / cols;
+  const bool act = r0 < rstep;
   for (int k = threadIdx.x; k < 2 * jc + 2; k += blockDim.x) rbuf[k] = 0.0f;
   __syncthreads();
   // origin shift, and the shell sums of pweight.f:31-42 over J=-JC+1..JC-1
   // (the Nyquist row is addressed by both J=-(JC-1) and J=+(JC-1))
-  for (int e = threadIdx.x; e < (int)ic; e += blockDim.x) {
-    int l = e % jc, m = e / jc;
-    int mm = m >= jc ? m - nsam : m;
-    float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
-    float2 o = oc[e], d = dc[e];
-    o.x *= sg; o.y *= sg;
-    d.x *= sg; d.y *= sg;
-    oc[e] = o;
-    dc[e] = d;
-    int itemp = l * l + mm * mm;
-    if (itemp < jc * jc) {
-      int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
-      float h = hypotf(o.x, o.y);
-      int visits = (2 * m == nsam) ? 2 : 1;
-      atomicAdd(&rbuf[ibin], visits * (h * h));
-      atomicAdd(&rbuf[ibin + jc], (float)visits);
-    }
-  }
+  if (act)
+    for (int m = r0; m < nsam; m += rstep)
+      for (int l = l0; l < jc; l += cols) {
+        const int e = m * jc + l;
+        int mm = m >= jc ? m - nsam : m;
+        float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
+        float2 o = oc[e], d = dc[e];
+        o.x *= sg; o.y *= sg;
+        d.x *= sg; d.y *= sg;
+        oc[e] = o;
+        dc[e] = d;
+        int itemp = l * l + mm * mm;
+        if (itemp < jc * jc) {
+          int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+          float h = hypotf(o.x, o.y);
+          int visits = (2 * m == nsam) ? 2 : 1;
+          atomicAdd(&rbuf[ibin], visits * (h * h));
+          atomicAdd(&rbuf[ibin + jc], (float)visits);
+        }
+      }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i <= jc; i++) {
@@ -170,31 +177,33 @@ __global__ void __launch_bounds__(kThreads)
     }
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < (int)ic; e += blockDim.x) {
-    int l = e % jc, m = e / jc;
-    int mm = m >= jc ? m - nsam : m;
-    int itemp = l * l + mm * mm;
-    float2 o = oc[e];
-    if (itemp < jc * jc) {
-      int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
-      float snr;
-      if (ictf == 1) {
-        float2 cr = cf_[e], cl = cf_[e + ic];
-        float re = cr.x + cl.x, im = cr.y - cl.y;
-        float h = hypotf(re, im);
-        snr = pssnr[ibin - 1] * (h * h);
-      } else {
-        snr = pssnr[ibin - 1];
+  if (act)
+    for (int m = r0; m < nsam; m += rstep)
+      for (int l = l0; l < jc; l += cols) {
+        const int e = m * jc + l;
+        int mm = m >= jc ? m - nsam : m;
+        int itemp = l * l + mm * mm;
+        float2 o = oc[e];
+        if (itemp < jc * jc) {
+          int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+          float snr;
+          if (ictf == 1) {
+            float2 cr = cf_[e], cl = cf_[e + ic];
+            float re = cr.x + cl.x, im = cr.y - cl.y;
+            float h = hypotf(re, im);
+            snr = pssnr[ibin - 1] * (h * h);
+          } else {
+            snr = pssnr[ibin - 1];
+          }
+          float wgt = snr / (snr + 1.0f) / rbuf[ibin];
+          o.x *= wgt;
+          o.y *= wgt;
+          if (2 * m == nsam) { o.x *= wgt; o.y *= wgt; }
+        } else {
+          o = make_float2(0.0f, 0.0f);
+        }
+        oc[e] = o;
       }
-      float wgt = snr / (snr + 1.0f) / rbuf[ibin];
-      o.x *= wgt;
-      o.y *= wgt;
-      if (2 * m == nsam) { o.x *= wgt; o.y *= wgt; }
-    } else {
-      o = make_float2(0.0f, 0.0f);
-    }
-    oc[e] = o;
-  }
 }
 
 }  // namespace
@@ -216,12 +225,12 @@ int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const 
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, nsam, nb, outc, img)) return r;
   // ctfapply.f:107-122; per-particle AMAGP is column 5 of ppar
-  hipLaunchKernelGGL(k_scale_mask, dim3(nb), dim3(kThreads), 0, c.stream, nsam, outc, img,
+  hipLaunchKernelGGL(k_scale_mask, dim3(nb), dim3(kMaskThreads), 0, c.stream, nsam, outc, img,
                      1.0f / (float)nsam / (float)nsam, ppar + 5, 8, g.ri2, g.ri3, g.rih, g.halfw,
                      0);
   FR_HIP(hipGetLastError());
   if (int r = fft_r2c_2d(c, nsam, nb, outc, img)) return r;
-  hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kThreads), 0, c.stream, nsam, pssnr, g.ictf,
+  hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kMaskThreads), 0, c.stream, nsam, pssnr, g.ictf,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<float2*>(outc),
                      reinterpret_cast<float2*>(datc));
   FR_HIP(hipGetLastError());

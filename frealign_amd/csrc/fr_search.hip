@@ -566,6 +566,42 @@ __global__ void __launch_bounds__(kThreads)
   const int gsz = grp ? grp[p] : group;
   const float* sc = score + base;
   const int ngroup = gsz > 0 ? len / gsz : 0;
+  // few groups (candidate lists, a bank of up to 2048 views): sort the group
+  // maxima themselves, padded to a power of two
+  if (ngroup <= 2048) {
+    int np2 = 1;
+    while (np2 < ngroup || np2 < topk) np2 <<= 1;
+    for (int g = tid; g < np2; g += kThreads) {
+      float x = -3.4e38f;
+      int xo = 0x7fffffff;
+      if (g < ngroup) {
+        xo = g * gsz;
+        for (int u = 0; u < gsz; u++) {
+          float y = sc[g * gsz + u];
+          if (y > x) { x = y; xo = g * gsz + u; }
+        }
+      }
+      sv[g] = x;
+      si[g] = xo;
+    }
+    __syncthreads();
+    for (int size = 2; size <= np2; size <<= 1)
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int e = tid; e < np2 / 2; e += kThreads) {
+          int lo = 2 * e - (e & (stride - 1));
+          int hi = lo + stride;
+          bool desc = ((lo & size) == 0);
+          float a = sv[lo], bb = sv[hi];
+          int ia = si[lo], ib = si[hi];
+          bool a_first = (a > bb) || (a == bb && ia < ib);
+          if (a_first != desc) {
+            sv[lo] = bb; sv[hi] = a;
+            si[lo] = ib; si[hi] = ia;
+          }
+        }
+        __syncthreads();
+      }
+  } else {
   float v[kK];
   int ix[kK];
 #pragma unroll
@@ -604,6 +640,7 @@ __global__ void __launch_bounds__(kThreads)
       }
       __syncthreads();
     }
+  }
   for (int q = tid; q < topk; q += kThreads) {
     int t = si[q];
     bool ok = t >= 0 && t < len;
