@@ -213,6 +213,66 @@ module frealign_hip
       real(c_float), value :: rbfact
       integer(c_int) :: fr_search_batch
     end function
+
+    ! PSEARCH over per-particle orientation lists d_cand(d_loff(p)+1 : d_loff(p+1))
+    function fr_search_ragged_batch(ctx, nsam, maxr1, maxr2, rbfact, norient, d_bank, dtype, &
+                                    np, d_outc, d_ctff, d_loff, d_grp, d_cand, cap, topk, &
+                                    d_top_o, d_top_row, d_top_score) &
+        bind(C, name="fr_search_ragged_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_bank, d_outc, d_ctff, d_loff, d_grp, d_cand
+      type(c_ptr), value :: d_top_o, d_top_row, d_top_score
+      integer(c_int), value :: nsam, maxr1, maxr2, norient, dtype, np, cap, topk
+      real(c_float), value :: rbfact
+      integer(c_int) :: fr_search_ragged_batch
+    end function
+
+    ! learned proposal: network input, forward pass, top classes, candidate lists
+    function fr_cnn_images(ctx, nsam, s, nb, d_datc, d_ctff, d_img) bind(C, name="fr_cnn_images")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_datc, d_ctff, d_img
+      integer(c_int), value :: nsam, s, nb
+      integer(c_int) :: fr_cnn_images
+    end function
+
+    function fr_cnn_forward(ctx, s, c, nconv, k, d_wq, nb, d_img, d_logits) &
+        bind(C, name="fr_cnn_forward")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_wq, d_img, d_logits
+      integer(c_int), value :: s, c, nconv, k, nb
+      integer(c_int) :: fr_cnn_forward
+    end function
+
+    function fr_cnn_propose(ctx, k, nb, d_logits, topk, d_top, d_prob) &
+        bind(C, name="fr_cnn_propose")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_logits, d_top, d_prob
+      integer(c_int), value :: k, nb, topk
+      integer(c_int) :: fr_cnn_propose
+    end function
+
+    function fr_build_lists(ctx, np, ktop, d_top, d_prob, threshold, ncls, d_cell_off, d_cells, &
+                            group, nbank, bank_group, maxlen, d_loff, d_grp, d_cand, d_info) &
+        bind(C, name="fr_build_lists")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_top, d_prob, d_cell_off, d_cells, d_loff, d_grp, d_cand, d_info
+      integer(c_int), value :: np, ktop, ncls, group, nbank, bank_group, maxlen
+      real(c_float), value :: threshold
+      integer(c_int) :: fr_build_lists
+    end function
+
+    ! CC3M of every candidate (lmain.f:615-642) and each particle's winner
+    function fr_rescore_batch(ctx, nsam, np, k, d_a3df, d_outc, d_ctff, d_top_o, d_top_row, &
+                              nbank, d_dm_bank, nsamr, maxr1, maxr2, rbfact, d_fsct, ring, &
+                              d_score, d_best) bind(C, name="fr_rescore_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_outc, d_ctff, d_top_o, d_top_row, d_dm_bank, d_fsct
+      type(c_ptr), value :: d_score, d_best
+      integer(c_int), value :: nsam, np, k, nbank, nsamr, maxr1, maxr2
+      real(c_float), value :: rbfact
+      real(c_float), intent(in) :: ring(4)
+      integer(c_int) :: fr_rescore_batch
+    end function
   end interface
 
 contains
