@@ -265,12 +265,14 @@ class LearnedProposal:
         img = self.ctx.cnn_images(self.search.nsam, self.cfg.S, datc, ctff)
         return self.ctx.cnn_propose(self.net.forward(img), self.cfg.topk)
 
-    def run(self, outc, ctff, datc, fsct, ring, topk: int = 32):
-        """Learned search of one batch, on the device from images to winners:
-        proposal -> ragged candidate lists (the proposed classes' cells, or the
-        whole bank for a particle below the confidence threshold) -> one GEMM
-        search launch -> FP32 CC3M rescoring of the top-k; the results come
-        back to the host once."""
+    def launch(self, outc, ctff, datc, fsct, ring, topk: int = 32):
+        """Enqueue the learned search of one batch on the current stream, from
+        images to winners: proposal -> ragged candidate lists (the proposed
+        classes' cells, or the whole bank for a particle below the confidence
+        threshold) -> one GEMM search launch -> FP32 CC3M rescoring of the
+        top-k.  The results are copied into a pinned host buffer
+        asynchronously; `finish` waits for them.  A caller may launch the next
+        batch before finishing this one (two batches can be in flight)."""
         s = self.search
         top, prob = self.propose(datc, ctff)
         loff, grp, cand, info = self.ctx.build_lists(
@@ -283,7 +285,26 @@ class LearnedProposal:
         npart = o.shape[0]
         f32 = torch.float32
         pack = torch.cat([o.view(f32).reshape(-1), r.view(f32).reshape(-1), sg.reshape(-1),
-                          sc.reshape(-1), best.view(f32), info.view(f32).reshape(-1)]).cpu().numpy()
+                          sc.reshape(-1), best.view(f32), info.view(f32).reshape(-1)])
+        # two pinned buffers, used in turn, so that one batch can be unpacked
+        # while the next one is in flight
+        self._flip = getattr(self, "_flip", 0) ^ 1
+        bufs = getattr(self, "_pinned", None)
+        if bufs is None or bufs[0].numel() < pack.numel():
+            bufs = [torch.empty(pack.numel(), dtype=f32, pin_memory=True) for _ in range(2)]
+            self._pinned = bufs
+        host = bufs[self._flip][:pack.numel()]
+        host.copy_(pack, non_blocking=True)
+        done = torch.cuda.Event()
+        done.record()
+        return (host, done, npart, topk, pack)
+
+    def finish(self, pending):
+        """Wait for a launched batch and return (SearchResult, confident)."""
+        host, done, npart, topk, _ = pending
+        done.synchronize()
+        s = self.search
+        pack = host.numpy().copy()
         n = npart * topk
         o_h = pack[:n].view(np.int32).reshape(npart, topk)
         r_h = pack[n:2 * n].view(np.int32).reshape(npart, topk)
@@ -300,3 +321,7 @@ class LearnedProposal:
         res = SearchResult(s.euler[np.maximum(bo, 0)], shifts[ar, best_h], sc_h[ar, best_h], bo,
                            o_h, sg_h)
         return res, confident
+
+    def run(self, outc, ctff, datc, fsct, ring, topk: int = 32):
+        """Learned search of one batch (launch + finish)."""
+        return self.finish(self.launch(outc, ctff, datc, fsct, ring, topk))
