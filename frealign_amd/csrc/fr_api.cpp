@@ -458,9 +458,12 @@ int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const floa
   if (!d_tasks) return err;
   FR_HIP(hipMemcpyAsync(d_tasks, tasks.data(), sizeof(InsertTask) * tasks.size(),
                         hipMemcpyHostToDevice, c->stream));
+  void* scratch = c->slot(22, insert_scratch_bytes(nsam, nb, (int)tasks.size()), &err);
+  if (!scratch) return err;
   int pslot = c->prof_begin(kProfInsert);
-  int r = launch_insert(*c, nsam, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0, d_ctf2_0,
-                        d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum), d_ttd);
+  int r = launch_insert(*c, nsam, nb, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0,
+                        d_ctf2_0, d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum),
+                        d_ttd, scratch);
   c->prof_end(pslot);
   return r;
 }

@@ -1,14 +1,29 @@
-// fr_insert.hip -- CTF-weighted trilinear Fourier insertion.
+// fr_insert.hip -- CTF-weighted trilinear Fourier insertion, gathered per
+// voxel tile.
 //
 // Replaces PINSERT_TRILIN (pinsert_trilin.f:8-244) as driven by PINSERT
 // (pinsert.f:8-195) for a buffer of particles, the A3D3/A3D3_S flush
-// (a3d3.f:107-143).  One task = (particle, symmetry copy); one thread = one
-// point (I,J) of the full central plane with I^2+J^2 <= IRREC^2.  The plane
-// coordinates X3 = DM*(I,J) and the voxel range follow the reference's FP32
-// operation order exactly (no FMA contraction), so the set of voxels touched
-// and NKSUM are bit-identical to the reference; the numerator and CTF^2 sums
-// use hardware FP32 atomics (order-dependent rounding only).
+// (a3d3.f:107-143).  A task is (particle, symmetry copy); its plane points
+// are the (I,J) of the full central plane with I^2+J^2 <= IRREC^2, at
+// X3 = DM*(I,J), each adding to the voxels L in [max(0,FLOOR(X3)),
+// min(NSAM/2,FLOOR(X3+1))] (and likewise M, N within +-NSAM/2) with the
+// trilinear weight (1-|X3-L|)(1-|Y3-M|)(1-|Z3-N|).
+//
+// Instead of scattering every point with float atomics, each workgroup owns
+// one 8x8x8 tile of the half-volume and each thread one voxel: the tile first
+// lists (in task order) the tasks whose plane passes within reach of it, and
+// each voxel then visits, per listed task, the few plane points (I,J) that can
+// reach it -- those near its projection onto the plane, found through the
+// inverse Gram matrix of the plane's axes -- and re-derives for each the
+// reference's FP32 coordinates and voxel range (same operation order, no FMA
+// contraction), so exactly the scatter's (point, voxel) pairs contribute,
+// with bit-identical terms.  Sums run in a fixed order: the volumes, NKSUM
+// and TTD are deterministic, and no atomics are needed.  The per-point terms
+// (sample x phase ramp x CTF x weight, and CTF^2 x weight) depend only on the
+// particle, so they are formed once per particle.
 #include <hip/hip_runtime.h>
+
+#include <math.h>
 
 #include "fr_common.h"
 #include "fr_kernels.h"
@@ -18,126 +33,458 @@ namespace fr {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kTile = 8;            // voxels per tile edge
+constexpr int kGatherThreads = 512; // one per voxel of a tile
+constexpr int kChunk = 4096;        // tasks listed per pass
+constexpr int kReduceThreads = 1024;
 
-__device__ inline void wave_reduce(double& d, unsigned long long& n) {
-  for (int off = 32; off > 0; off >>= 1) {
-    d += __shfl_down(d, off, 64);
-    n += __shfl_down(n, off, 64);
-  }
+// per task, for the candidate search only (never in the sums)
+struct InsertGeo {
+  float nx, ny, nz, nl1;    // plane normal a x b and its L1 norm
+  float h1x, h1y, h1z, r1;  // I = h1 . P for P on the plane; r1 = |h1|_1
+  float h2x, h2y, h2z, r2;  // J = h2 . P; r2 = |h2|_1
+  float rmax2;              // squared reach of the plane's points (+ margin)
+  int valid;
+  float pad_[2];
+};
+
+__host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct Scratch {
+  int* first;        // [nb] first task of each particle (or -1)
+  InsertGeo* geo;    // [ntask]
+  float4* pts;       // [nb][w*w] (samp.re, samp.im, ctfv2, 0)
+  unsigned long long* pcount;  // [ntiles][2]
+  double* pttd;      // [ntiles][2]
+};
+
+Scratch scratch_layout(void* base, int nsam, int nb, int ntask, int ntiles) {
+  const int jc = nsam / 2 + 1, w = 2 * jc - 1;
+  char* p = static_cast<char*>(base);
+  Scratch s;
+  size_t o = 0;
+  s.first = reinterpret_cast<int*>(p + o);
+  o += align256(sizeof(int) * (size_t)nb);
+  s.geo = reinterpret_cast<InsertGeo*>(p + o);
+  o += align256(sizeof(InsertGeo) * (size_t)ntask);
+  s.pts = reinterpret_cast<float4*>(p + o);
+  o += align256(sizeof(float4) * (size_t)nb * w * w);
+  s.pcount = reinterpret_cast<unsigned long long*>(p + o);
+  o += align256(sizeof(unsigned long long) * 2 * (size_t)ntiles);
+  s.pttd = reinterpret_cast<double*>(p + o);
+  return s;
 }
 
-__global__ void __launch_bounds__(kThreads)
-    k_insert(const float2* __restrict__ datc, const float2* __restrict__ ctff,
-             const InsertTask* __restrict__ tasks, int nsam, float2* num0, float* ctf2_0,
-             float2* num1, float* ctf2_1, unsigned long long* nksum, double* ttd) {
-  const InsertTask t = tasks[blockIdx.y];
-  const int jc = nsam / 2 + 1, nsamh = nsam / 2, w = 2 * jc - 1;
-  const size_t ic = (size_t)nsam * jc;
+void tile_counts(int nsam, int* tx, int* ty, int* tz) {
+  const int jc = nsam / 2 + 1;
+  *tx = (jc + kTile - 1) / kTile;
+  *ty = (nsam + kTile - 1) / kTile;
+  *tz = *ty;
+}
+
+__global__ void __launch_bounds__(kThreads) k_first_clear(int nb, int* first) {
   const int p = blockIdx.x * kThreads + threadIdx.x;
-  double my_ttd = 0.0;
-  unsigned long long my_n = 0;
-  if (p < w * w) {
-    const int i = p % w - (jc - 1);
-    const int j = p / w - (jc - 1);
-    float x3, y3, z3, g2;
-    int ls, lt, ms, mt, ns, nt;
-    bool inside;
-    {
+  if (p < nb) first[p] = -1;
+}
+
+// the first task of each run of tasks of one particle (tasks come grouped by
+// particle; a particle's copies share its shift, weight and radius)
+__global__ void __launch_bounds__(kThreads)
+    k_first_mark(int ntask, int nb, const InsertTask* __restrict__ tasks, int* first) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= ntask) return;
+  const int part = tasks[k].part;
+  if (part < 0 || part >= nb) return;
+  if (k == 0 || tasks[k - 1].part != part) first[part] = k;
+}
+
+// pinsert_trilin.f:60-87: the weighted sample and CTF^2 of every plane point
+// of a particle, exactly as the scatter formed them
+__global__ void __launch_bounds__(kThreads)
+    k_insert_points(int nsam, const int* __restrict__ first,
+                    const InsertTask* __restrict__ tasks, const float2* __restrict__ datc,
+                    const float2* __restrict__ ctff, float4* pts) {
+  const int b = blockIdx.y;
+  const int k = first[b];
+  if (k < 0) return;
+  const InsertTask t = tasks[k];
+  const int jc = nsam / 2 + 1, w = 2 * jc - 1;
+  const size_t ic = (size_t)nsam * jc;
+  const int e = blockIdx.x * kThreads + threadIdx.x;
+  if (e >= w * w) return;
+  const int i = e % w - (jc - 1);
+  const int j = e / w - (jc - 1);
+  float4 out = make_float4(0.f, 0.f, 0.f, 0.f);
+  float g2;
+  bool inside;
+  {
 #pragma clang fp contract(off)
-      g2 = (float)(i * i + j * j);
-      inside = g2 <= (float)t.irrec2;
-      float fi = (float)i, fj = (float)j;
-      float dm1 = t.dm[0] * fi, dm2 = t.dm[1] * fi, dm3 = t.dm[2] * fi;
-      x3 = dm1 + t.dm[3] * fj;
-      y3 = dm2 + t.dm[4] * fj;
-      z3 = dm3 + t.dm[5] * fj;
-      // pinsert_trilin.f:88-111: A=X3+0.5-RI12 (RI12=0.5), A=X3+RI2 (RI2=1)
-      float a;
-      a = (x3 + 0.5f) - 0.5f; ls = (int)a; if (a < 0.0f) ls--; if (ls < 0) ls = 0;
-      a = x3 + 1.0f;          lt = (int)a; if (a < 0.0f) lt--; if (lt > nsamh) lt = nsamh;
-      a = (y3 + 0.5f) - 0.5f; ms = (int)a; if (a < 0.0f) ms--; if (ms < -nsamh) ms = -nsamh;
-      a = y3 + 1.0f;          mt = (int)a; if (a < 0.0f) mt--; if (mt > nsamh) mt = nsamh;
-      a = (z3 + 0.5f) - 0.5f; ns = (int)a; if (a < 0.0f) ns--; if (ns < -nsamh) ns = -nsamh;
-      a = z3 + 1.0f;          nt = (int)a; if (a < 0.0f) nt--; if (nt > nsamh) nt = nsamh;
+    g2 = (float)(i * i + j * j);
+    inside = g2 <= (float)t.irrec2;
+  }
+  if (inside) {
+    const float2* sp = datc + (size_t)t.part * ic;
+    const float2* cp = ctff + (size_t)t.part * 2 * ic;
+    size_t id;
+    bool mate = i < 0;
+    if (!mate) id = (size_t)i + (size_t)jc * wrapn(j, nsam);
+    else id = (size_t)(-i) + (size_t)jc * wrapn(-j, nsam);
+    float2 s = sp[id];
+    if (mate) s.y = -s.y;
+    float2 cr = cp[id], cl = cp[id + ic];
+    cf ctfs = cf{cr.x + cl.x, cr.y - cl.y};  // CTFR + CONJG(CTFL)
+    float phase = t.shx * (float)i + t.shy * (float)j;
+    float sn, cs;
+    sincosf(phase, &sn, &cs);
+    float wgt = expf(t.tmp * g2) * t.occ;
+    cf samp = cmul(cmul(cf{s.x, s.y}, cf{cs, sn}), ctfs);
+    float ctfv2 = 0.25f * (ctfs.re * ctfs.re + ctfs.im * ctfs.im) * wgt;
+    samp = cscale(samp, wgt);
+    out = make_float4(samp.re, samp.im, ctfv2, 0.f);
+  }
+  pts[(size_t)b * w * w + e] = out;
+}
+
+// per-task plane geometry for the candidate search (double, then rounded;
+// the margins below cover the rounding)
+__global__ void __launch_bounds__(kThreads)
+    k_insert_geo(int ntask, int nsam, const InsertTask* __restrict__ tasks, InsertGeo* geo) {
+  const int k = blockIdx.x * kThreads + threadIdx.x;
+  if (k >= ntask) return;
+  const InsertTask t = tasks[k];
+  const double ax = t.dm[0], ay = t.dm[1], az = t.dm[2];
+  const double bx = t.dm[3], by = t.dm[4], bz = t.dm[5];
+  InsertGeo g;
+  const double nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const double aa = ax * ax + ay * ay + az * az, bb = bx * bx + by * by + bz * bz;
+  const double ab = ax * bx + ay * by + az * bz;
+  const double det = aa * bb - ab * ab;
+  g.valid = (t.irrec2 >= 0 && det > 1e-12) ? 1 : 0;
+  const double id = g.valid ? 1.0 / det : 0.0;
+  const double h1x = (bb * ax - ab * bx) * id, h1y = (bb * ay - ab * by) * id,
+               h1z = (bb * az - ab * bz) * id;
+  const double h2x = (aa * bx - ab * ax) * id, h2y = (aa * by - ab * ay) * id,
+               h2z = (aa * bz - ab * az) * id;
+  g.nx = (float)nx;
+  g.ny = (float)ny;
+  g.nz = (float)nz;
+  g.nl1 = (float)(fabs(nx) + fabs(ny) + fabs(nz));
+  g.h1x = (float)h1x;
+  g.h1y = (float)h1y;
+  g.h1z = (float)h1z;
+  g.r1 = (float)(fabs(h1x) + fabs(h1y) + fabs(h1z));
+  g.h2x = (float)h2x;
+  g.h2y = (float)h2y;
+  g.h2z = (float)h2z;
+  g.r2 = (float)(fabs(h2x) + fabs(h2y) + fabs(h2z));
+  // points reach |P| <= sqrt(IRREC^2 * lambda_max(G)) within the square
+  // |I|,|J| <= NSAM/2; a voxel they touch lies within sqrt(3) more
+  const double lmax = 0.5 * (aa + bb + sqrt((aa - bb) * (aa - bb) + 4.0 * ab * ab));
+  const double jcm = (double)(nsam / 2);
+  const double r2pts = t.irrec2 < 2.0 * jcm * jcm ? (double)t.irrec2 : 2.0 * jcm * jcm;
+  const double reach = sqrt(r2pts * lmax) + 1.7320508 + 0.01;
+  g.rmax2 = (float)(reach * reach);
+  g.pad_[0] = g.pad_[1] = 0.f;
+  geo[k] = g;
+}
+
+// the logical coordinates stored at index s of a wrapped axis: s itself for
+// s < NSAM/2, s - NSAM for s > NSAM/2, and both +NSAM/2 and -NSAM/2 at s = NSAM/2
+__device__ inline int logical_axis(int s, int nsam, int* v) {
+  const int h = nsam / 2;
+  if (s < h) { v[0] = s; return 1; }
+  if (s > h) { v[0] = s - nsam; return 1; }
+  v[0] = h;
+  v[1] = -h;
+  return 2;
+}
+
+// logical intervals covered by the storage range [s0, s1] of a wrapped axis
+__device__ inline int logical_ranges(int s0, int s1, int nsam, int* lo, int* hi) {
+  const int h = nsam / 2;
+  int n = 0;
+  if (s0 <= h) {  // the non-negative part s0 .. min(s1, h)
+    lo[n] = s0;
+    hi[n] = s1 < h ? s1 : h;
+    n++;
+  }
+  if (s1 >= h) {  // the negative part max(s0, h) - nsam .. s1 - nsam
+    lo[n] = (s0 > h ? s0 : h) - nsam;
+    hi[n] = s1 - nsam;
+    n++;
+  }
+  return n;
+}
+
+// does the plane through the origin with normal g reach the box [lo, hi]
+// grown by one voxel (the trilinear reach), within the plane's disc?
+__device__ inline bool plane_reaches(const InsertGeo& g, float xl, float xh, float yl, float yh,
+                                     float zl, float zh) {
+  xl -= 1.01f; yl -= 1.01f; zl -= 1.01f;
+  xh += 1.01f; yh += 1.01f; zh += 1.01f;
+  const float mn = (g.nx > 0.f ? g.nx * xl : g.nx * xh) + (g.ny > 0.f ? g.ny * yl : g.ny * yh) +
+                   (g.nz > 0.f ? g.nz * zl : g.nz * zh);
+  const float mx = (g.nx > 0.f ? g.nx * xh : g.nx * xl) + (g.ny > 0.f ? g.ny * yh : g.ny * yl) +
+                   (g.nz > 0.f ? g.nz * zh : g.nz * zl);
+  const float slack = 1e-3f * g.nl1 + 1e-3f;
+  if (mn > slack || mx < -slack) return false;
+  // nearest point of the box to the origin
+  const float cx = xl > 0.f ? xl : (xh < 0.f ? xh : 0.f);
+  const float cy = yl > 0.f ? yl : (yh < 0.f ? yh : 0.f);
+  const float cz = zl > 0.f ? zl : (zh < 0.f ? zh : 0.f);
+  return cx * cx + cy * cy + cz * cz <= g.rmax2;
+}
+
+__global__ void __launch_bounds__(kGatherThreads)
+    k_insert_gather(int nsam, int nb, int ntask, int tx, int ty, const InsertTask* __restrict__ tasks,
+                    const InsertGeo* __restrict__ geo, const float4* __restrict__ pts,
+                    float2* num0, float* ctf2_0, float2* num1, float* ctf2_1,
+                    unsigned long long* pcount, double* pttd) {
+  __shared__ int list[kChunk];
+  __shared__ int wave_tot[kGatherThreads / 64];
+  const int jc = nsam / 2 + 1, nsamh = nsam / 2, w = 2 * jc - 1;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int b = blockIdx.x;
+  const int l0 = (b % tx) * kTile, m0 = ((b / tx) % ty) * kTile, n0 = (b / (tx * ty)) * kTile;
+  // the tile's logical boxes (a wrapped axis may split in two)
+  const int l1 = min(l0 + kTile - 1, jc - 1);
+  int mlo[2], mhi[2], nlo[2], nhi[2];
+  const int nmr = logical_ranges(m0, min(m0 + kTile - 1, nsam - 1), nsam, mlo, mhi);
+  const int nnr = logical_ranges(n0, min(n0 + kTile - 1, nsam - 1), nsam, nlo, nhi);
+  // this thread's voxel
+  const int l = l0 + (tid % kTile), ms_ = m0 + (tid / kTile) % kTile, ns_ = n0 + tid / (kTile * kTile);
+  const bool mine = l < jc && ms_ < nsam && ns_ < nsam;
+  int mv[2], nv[2];
+  const int nm = mine ? logical_axis(ms_, nsam, mv) : 0;
+  const int nn = mine ? logical_axis(ns_, nsam, nv) : 0;
+  float a0r = 0.f, a0i = 0.f, a0c = 0.f, a1r = 0.f, a1i = 0.f, a1c = 0.f;
+  unsigned c0 = 0u, c1 = 0u;
+  double t0 = 0.0, t1 = 0.0;
+  for (int base = 0; base < ntask; base += kChunk) {
+    __syncthreads();  // the previous chunk's list is no longer read
+    const int cnt = min(kChunk, ntask - base);
+    // list the tasks reaching this tile, in task order (ballot compaction)
+    int nlist = 0;
+    for (int k0 = 0; k0 < cnt; k0 += kGatherThreads) {
+      const int k = k0 + tid;
+      bool hit = false;
+      if (k < cnt) {
+        const InsertGeo g = geo[base + k];
+        if (g.valid)
+          for (int a = 0; a < nmr && !hit; a++)
+            for (int c = 0; c < nnr && !hit; c++)
+              hit = plane_reaches(g, (float)l0, (float)l1, (float)mlo[a], (float)mhi[a],
+                                  (float)nlo[c], (float)nhi[c]);
+      }
+      const unsigned long long bal = __ballot(hit);
+      const int before = __popcll(bal & ((1ull << lane) - 1ull));
+      if (lane == 0) wave_tot[wave] = __popcll(bal);
+      __syncthreads();
+      int off = nlist;
+      for (int q = 0; q < wave; q++) off += wave_tot[q];
+      if (hit) list[off + before] = base + k;
+      int tot = 0;
+      for (int q = 0; q < kGatherThreads / 64; q++) tot += wave_tot[q];
+      __syncthreads();
+      nlist += tot;
     }
-    if (inside) {
-      const float2* sp = datc + (size_t)t.part * ic;
-      const float2* cp = ctff + (size_t)t.part * 2 * ic;
-      size_t id;
-      bool mate = i < 0;
-      if (!mate) id = (size_t)i + (size_t)jc * wrapn(j, nsam);
-      else id = (size_t)(-i) + (size_t)jc * wrapn(-j, nsam);
-      float2 s = sp[id];
-      if (mate) s.y = -s.y;
-      float2 cr = cp[id], cl = cp[id + ic];
-      cf ctfs = cf{cr.x + cl.x, cr.y - cl.y};  // CTFR + CONJG(CTFL)
-      float phase = t.shx * (float)i + t.shy * (float)j;
-      float sn, cs;
-      sincosf(phase, &sn, &cs);
-      float wgt = expf(t.tmp * g2) * t.occ;
-      cf samp = cmul(cmul(cf{s.x, s.y}, cf{cs, sn}), ctfs);
-      float ctfv2 = 0.25f * (ctfs.re * ctfs.re + ctfs.im * ctfs.im) * wgt;
-      samp = cscale(samp, wgt);
-      float2* num = t.half ? num1 : num0;
-      float* c2 = t.half ? ctf2_1 : ctf2_0;
-      for (int n = ns; n <= nt; n++) {
-        float b3 = 1.0f - fabsf(z3 - (float)n);
-        size_t n1 = (size_t)nsam * wrapn(n, nsam);
-        for (int m = ms; m <= mt; m++) {
-          float b2 = (1.0f - fabsf(y3 - (float)m));
-          size_t m1 = (size_t)jc * (wrapn(m, nsam) + n1);
-          for (int l = ls; l <= lt; l++) {
-            float box = (1.0f - fabsf(x3 - (float)l)) * b2 * b3;
-            size_t v = (size_t)l + m1;
-            float cv = ctfv2 * box;
-            unsafeAtomicAdd(&num[v].x, samp.re * box);
-            unsafeAtomicAdd(&num[v].y, samp.im * box);
-            unsafeAtomicAdd(&c2[v], cv);
-            my_n++;
-            my_ttd += (double)cv;
+    // gather this voxel's contributions from the listed tasks
+    if (mine) {
+      for (int q = 0; q < nlist; q++) {
+        const int k = list[q];
+        const InsertGeo g = geo[k];
+        const InsertTask t = tasks[k];
+        if (t.part < 0 || t.part >= nb) continue;
+        const float4* tp = pts + (size_t)t.part * w * w;
+        for (int am = 0; am < nm; am++)
+          for (int an = 0; an < nn; an++) {
+            const int m = mv[am], n = nv[an];
+            const float fl = (float)l, fm = (float)m, fn = (float)n;
+            const float d = g.nx * fl + g.ny * fm + g.nz * fn;
+            if (fabsf(d) > 1.01f * g.nl1 + 1e-3f) continue;
+            const float ci = g.h1x * fl + g.h1y * fm + g.h1z * fn;
+            const float cj = g.h2x * fl + g.h2y * fm + g.h2z * fn;
+            int ilo = (int)ceilf(ci - g.r1 - 0.02f), ihi = (int)floorf(ci + g.r1 + 0.02f);
+            int jlo = (int)ceilf(cj - g.r2 - 0.02f), jhi = (int)floorf(cj + g.r2 + 0.02f);
+            ilo = max(ilo, -(jc - 1));
+            ihi = min(ihi, jc - 1);
+            jlo = max(jlo, -(jc - 1));
+            jhi = min(jhi, jc - 1);
+            for (int j = jlo; j <= jhi; j++)
+              for (int i = ilo; i <= ihi; i++) {
+                float x3, y3, z3;
+                int ls, lt, ms, mt, ns, nt;
+                bool inside;
+                {
+#pragma clang fp contract(off)
+                  float g2 = (float)(i * i + j * j);
+                  inside = g2 <= (float)t.irrec2;
+                  float fi = (float)i, fj = (float)j;
+                  float dm1 = t.dm[0] * fi, dm2 = t.dm[1] * fi, dm3 = t.dm[2] * fi;
+                  x3 = dm1 + t.dm[3] * fj;
+                  y3 = dm2 + t.dm[4] * fj;
+                  z3 = dm3 + t.dm[5] * fj;
+                  // pinsert_trilin.f:88-111: A=X3+0.5-RI12 (RI12=0.5), A=X3+RI2 (RI2=1)
+                  float a;
+                  a = (x3 + 0.5f) - 0.5f; ls = (int)a; if (a < 0.0f) ls--; if (ls < 0) ls = 0;
+                  a = x3 + 1.0f;          lt = (int)a; if (a < 0.0f) lt--; if (lt > nsamh) lt = nsamh;
+                  a = (y3 + 0.5f) - 0.5f; ms = (int)a; if (a < 0.0f) ms--; if (ms < -nsamh) ms = -nsamh;
+                  a = y3 + 1.0f;          mt = (int)a; if (a < 0.0f) mt--; if (mt > nsamh) mt = nsamh;
+                  a = (z3 + 0.5f) - 0.5f; ns = (int)a; if (a < 0.0f) ns--; if (ns < -nsamh) ns = -nsamh;
+                  a = z3 + 1.0f;          nt = (int)a; if (a < 0.0f) nt--; if (nt > nsamh) nt = nsamh;
+                }
+                if (!inside || l < ls || l > lt || m < ms || m > mt || n < ns || n > nt) continue;
+                const float b3 = 1.0f - fabsf(z3 - fn);
+                const float b2 = (1.0f - fabsf(y3 - fm));
+                const float box = (1.0f - fabsf(x3 - fl)) * b2 * b3;
+                const float4 pv = tp[(size_t)(j + jc - 1) * w + (i + jc - 1)];
+                const float cv = pv.z * box;
+                if (t.half) {
+                  a1r += pv.x * box;
+                  a1i += pv.y * box;
+                  a1c += cv;
+                  c1++;
+                  t1 += (double)cv;
+                } else {
+                  a0r += pv.x * box;
+                  a0i += pv.y * box;
+                  a0c += cv;
+                  c0++;
+                  t0 += (double)cv;
+                }
+              }
           }
-        }
       }
     }
   }
-  // block reduction of NKSUM / TTD, one atomic per block
-  __shared__ double sd[kThreads / 64];
-  __shared__ unsigned long long sn[kThreads / 64];
-  wave_reduce(my_ttd, my_n);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) { sd[wid] = my_ttd; sn[wid] = my_n; }
+  if (mine) {
+    const size_t v = (size_t)l + (size_t)jc * ((size_t)ms_ + (size_t)nsam * ns_);
+    if (c0) {
+      float2 o = num0[v];
+      num0[v] = make_float2(o.x + a0r, o.y + a0i);
+      ctf2_0[v] += a0c;
+    }
+    if (c1) {
+      float2 o = num1[v];
+      num1[v] = make_float2(o.x + a1r, o.y + a1i);
+      ctf2_1[v] += a1c;
+    }
+  }
+  // the tile's NKSUM and TTD: a fixed reduction order
+  __shared__ unsigned long long rc[kGatherThreads / 64][2];
+  __shared__ double rt[kGatherThreads / 64][2];
+  unsigned long long u0 = c0, u1 = c1;
+  for (int off = 32; off > 0; off >>= 1) {
+    u0 += __shfl_xor(u0, off, 64);
+    u1 += __shfl_xor(u1, off, 64);
+    t0 += __shfl_xor(t0, off, 64);
+    t1 += __shfl_xor(t1, off, 64);
+  }
+  if (lane == 0) {
+    rc[wave][0] = u0;
+    rc[wave][1] = u1;
+    rt[wave][0] = t0;
+    rt[wave][1] = t1;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double d = 0.0;
-    unsigned long long n = 0;
-    for (int k = 0; k < kThreads / 64; k++) { d += sd[k]; n += sn[k]; }
-    if (n) {
-      atomicAdd(&nksum[t.half], n);
-      unsafeAtomicAdd(&ttd[t.half], d);
+  if (tid < 2) {
+    unsigned long long cc = 0;
+    double tt = 0.0;
+    for (int q = 0; q < kGatherThreads / 64; q++) {
+      cc += rc[q][tid];
+      tt += rt[q][tid];
+    }
+    pcount[2 * (size_t)b + tid] = cc;
+    pttd[2 * (size_t)b + tid] = tt;
+  }
+}
+
+// NKSUM and TTD of the launch: tiles summed in a fixed order
+__global__ void __launch_bounds__(kReduceThreads)
+    k_insert_reduce(int ntiles, const unsigned long long* __restrict__ pcount,
+                    const double* __restrict__ pttd, unsigned long long* nksum, double* ttd) {
+  __shared__ unsigned long long rc[kReduceThreads / 64][2];
+  __shared__ double rt[kReduceThreads / 64][2];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  unsigned long long u0 = 0, u1 = 0;
+  double t0 = 0.0, t1 = 0.0;
+  for (int b = tid; b < ntiles; b += kReduceThreads) {
+    u0 += pcount[2 * (size_t)b];
+    u1 += pcount[2 * (size_t)b + 1];
+    t0 += pttd[2 * (size_t)b];
+    t1 += pttd[2 * (size_t)b + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    u0 += __shfl_xor(u0, off, 64);
+    u1 += __shfl_xor(u1, off, 64);
+    t0 += __shfl_xor(t0, off, 64);
+    t1 += __shfl_xor(t1, off, 64);
+  }
+  if (lane == 0) {
+    rc[wave][0] = u0;
+    rc[wave][1] = u1;
+    rt[wave][0] = t0;
+    rt[wave][1] = t1;
+  }
+  __syncthreads();
+  if (tid < 2) {
+    unsigned long long cc = 0;
+    double tt = 0.0;
+    for (int q = 0; q < kReduceThreads / 64; q++) {
+      cc += rc[q][tid];
+      tt += rt[q][tid];
+    }
+    if (cc) {
+      nksum[tid] += cc;
+      ttd[tid] += tt;
     }
   }
 }
 
 }  // namespace
 
-int launch_insert(Ctx& c, int nsam, int ntask, const InsertTask* d_tasks, const float* d_datc,
-                  const float* d_ctff, float* num0, float* ctf2_0, float* num1, float* ctf2_1,
-                  unsigned long long* d_nksum, double* d_ttd) {
-  if (ntask <= 0) return 0;
+size_t insert_scratch_bytes(int nsam, int nb, int ntask) {
   const int jc = nsam / 2 + 1, w = 2 * jc - 1;
-  FR_CHECK(ntask <= 65535 * 64, "too many insertion tasks (%d)", ntask);
-  int done = 0;
-  while (done < ntask) {
-    int chunk = ntask - done < 65535 ? ntask - done : 65535;
-    dim3 grid((w * w + kThreads - 1) / kThreads, chunk);
-    hipLaunchKernelGGL(k_insert, grid, dim3(kThreads), 0, c.stream,
-                       reinterpret_cast<const float2*>(d_datc),
-                       reinterpret_cast<const float2*>(d_ctff), d_tasks + done, nsam,
-                       reinterpret_cast<float2*>(num0), ctf2_0, reinterpret_cast<float2*>(num1),
-                       ctf2_1, d_nksum, d_ttd);
-    FR_HIP(hipGetLastError());
-    done += chunk;
-  }
+  int tx, ty, tz;
+  tile_counts(nsam, &tx, &ty, &tz);
+  const size_t ntiles = (size_t)tx * ty * tz;
+  return align256(sizeof(int) * (size_t)nb) + align256(sizeof(InsertGeo) * (size_t)ntask) +
+         align256(sizeof(float4) * (size_t)nb * w * w) +
+         align256(sizeof(unsigned long long) * 2 * ntiles) + align256(sizeof(double) * 2 * ntiles);
+}
+
+int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks,
+                  const float* d_datc, const float* d_ctff, float* num0, float* ctf2_0,
+                  float* num1, float* ctf2_1, unsigned long long* d_nksum, double* d_ttd,
+                  void* scratch) {
+  if (ntask <= 0 || nb <= 0) return 0;
+  FR_CHECK(scratch != nullptr, "insertion scratch missing");
+  const int jc = nsam / 2 + 1, w = 2 * jc - 1;
+  int tx, ty, tz;
+  tile_counts(nsam, &tx, &ty, &tz);
+  const int ntiles = tx * ty * tz;
+  Scratch s = scratch_layout(scratch, nsam, nb, ntask, ntiles);
+  hipLaunchKernelGGL(k_first_clear, dim3((nb + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     c.stream, nb, s.first);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_first_mark, dim3((ntask + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     c.stream, ntask, nb, d_tasks, s.first);
+  FR_HIP(hipGetLastError());
+  FR_CHECK(nb <= 65535, "too many particles in one insertion (%d)", nb);
+  hipLaunchKernelGGL(k_insert_points, dim3((w * w + kThreads - 1) / kThreads, nb), dim3(kThreads),
+                     0, c.stream, nsam, s.first, d_tasks, reinterpret_cast<const float2*>(d_datc),
+                     reinterpret_cast<const float2*>(d_ctff), s.pts);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_insert_geo, dim3((ntask + kThreads - 1) / kThreads), dim3(kThreads), 0,
+                     c.stream, ntask, nsam, d_tasks, s.geo);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_insert_gather, dim3(ntiles), dim3(kGatherThreads), 0, c.stream, nsam, nb,
+                     ntask, tx, ty, d_tasks, s.geo, s.pts, reinterpret_cast<float2*>(num0),
+                     ctf2_0, reinterpret_cast<float2*>(num1), ctf2_1, s.pcount, s.pttd);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_insert_reduce, dim3(1), dim3(kReduceThreads), 0, c.stream, ntiles,
+                     s.pcount, s.pttd, d_nksum, d_ttd);
+  FR_HIP(hipGetLastError());
   return 0;
 }
 

@@ -17,9 +17,12 @@ struct InsertTask {
   int pad_;
 };
 
-int launch_insert(Ctx& c, int nsam, int ntask, const InsertTask* d_tasks, const float* d_datc,
-                  const float* d_ctff, float* num0, float* ctf2_0, float* num1, float* ctf2_1,
-                  unsigned long long* d_nksum, double* d_ttd);
+// scratch of launch_insert for nb particles and ntask tasks (grouped by particle)
+size_t insert_scratch_bytes(int nsam, int nb, int ntask);
+int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks,
+                  const float* d_datc, const float* d_ctff, float* num0, float* ctf2_0,
+                  float* num1, float* ctf2_1, unsigned long long* d_nksum, double* d_ttd,
+                  void* scratch);
 
 // FP64 shell sums of sheltest.f:48-88 (istp-wide shells); acc is nstp x 6 doubles:
 // DFSC, MEAN12, MEAN22, PSSNR-sum, NPIX, NPIXT.

@@ -52,6 +52,7 @@ struct fr_plan {
         *d_score = nullptr, *d_work = nullptr;
   int* d_pidx = nullptr;
   InsertTask* d_tasks = nullptr;
+  unsigned char* d_ins = nullptr;  // insertion scratch
   // pinned staging
   float *h_ppar = nullptr, *h_dm = nullptr, *h_shift = nullptr, *h_amag = nullptr,
         *h_score = nullptr;
@@ -97,8 +98,9 @@ int enqueue(fr_plan* p) {
   const int ntask = nb * p->ncopy;
   FR_HIP(hipMemcpyAsync(p->d_tasks, p->h_tasks, sizeof(InsertTask) * ntask, hipMemcpyHostToDevice,
                         c->stream));
-  return launch_insert(*c, nsam, ntask, p->d_tasks, p->d_datc, p->d_ctff, p->num0, p->ctf20,
-                       p->num1, p->ctf21, reinterpret_cast<unsigned long long*>(p->nksum), p->ttd);
+  return launch_insert(*c, nsam, nb, ntask, p->d_tasks, p->d_datc, p->d_ctff, p->num0, p->ctf20,
+                       p->num1, p->ctf21, reinterpret_cast<unsigned long long*>(p->nksum), p->ttd,
+                       p->d_ins);
 }
 
 // Host staging for one batch: the CC3M DM (cc3m.f:48-65) and the insertion
@@ -222,6 +224,7 @@ int fr_plan_create(fr_ctx* c, int nsam, int nb, const float* d_a3df, const float
       (r = halloc(&p->h_tasks, (size_t)nb * ncopy)))
     return fail(r);
   if (!p->fused && (r = dalloc(&p->d_work, img * nb))) return fail(r);
+  if ((r = dalloc(&p->d_ins, insert_scratch_bytes(nsam, nb, nb * ncopy)))) return fail(r);
   if (p->fused && (r = c->cc3m_tables(p->ca, &p->tabs))) return fail(r);
   std::vector<int> pid(nb);
   for (int b = 0; b < nb; b++) pid[b] = b;
@@ -302,7 +305,7 @@ int fr_plan_destroy(fr_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   void* dev[] = {p->d_img, p->d_ppar, p->d_ctff, p->d_outc, p->d_datc, p->d_dm, p->d_shift,
-                 p->d_amag, p->d_score, p->d_work, p->d_pidx, p->d_tasks};
+                 p->d_amag, p->d_score, p->d_work, p->d_pidx, p->d_tasks, p->d_ins};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   void* host[] = {p->h_ppar, p->h_dm, p->h_shift, p->h_amag, p->h_score, p->h_tasks};
