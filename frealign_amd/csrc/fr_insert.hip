@@ -291,7 +291,8 @@ __global__ void __launch_bounds__(kGatherThreads)
     // gather this voxel's contributions from the listed tasks
     if (mine) {
       for (int q = 0; q < nlist; q++) {
-        const int k = list[q];
+        // the same task for the whole workgroup: scalar loads of its records
+        const int k = __builtin_amdgcn_readfirstlane(list[q]);
         const InsertGeo g = geo[k];
         const InsertTask t = tasks[k];
         if (t.part < 0 || t.part >= nb) continue;
