@@ -46,6 +46,7 @@ struct InsertGeo {
   float rmax2;              // squared reach of the plane's points (+ margin)
   int valid;
   float pad_[2];
+  float ibx, iby, ibz, pad2_;  // 1/b per axis (0 where |b| is tiny)
 };
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -185,6 +186,10 @@ __global__ void __launch_bounds__(kThreads)
   const double reach = sqrt(r2pts * lmax) + 1.7320508 + 0.01;
   g.rmax2 = (float)(reach * reach);
   g.pad_[0] = g.pad_[1] = 0.f;
+  g.ibx = fabs(bx) > 1e-6 ? (float)(1.0 / bx) : 0.f;
+  g.iby = fabs(by) > 1e-6 ? (float)(1.0 / by) : 0.f;
+  g.ibz = fabs(bz) > 1e-6 ? (float)(1.0 / bz) : 0.f;
+  g.pad2_ = 0.f;
   geo[k] = g;
 }
 
@@ -311,8 +316,29 @@ __global__ void __launch_bounds__(kGatherThreads)
             ihi = min(ihi, jc - 1);
             jlo = max(jlo, -(jc - 1));
             jhi = min(jhi, jc - 1);
-            for (int j = jlo; j <= jhi; j++)
-              for (int i = ilo; i <= ihi; i++) {
+            for (int i = ilo; i <= ihi; i++) {
+              // the J of this I whose point lies in the voxel's reach
+              // |I a + J b - v| <= 1 on every axis (with margin)
+              float jl = (float)jlo, jh = (float)jhi;
+              bool none = false;
+              const float fi = (float)i;
+              auto axis = [&](float vk, float ak, float bk, float ibk) {
+                const float o = vk - fi * ak;
+                if (ibk != 0.f) {
+                  float e1 = (o - 1.02f) * ibk, e2 = (o + 1.02f) * ibk;
+                  if (e1 > e2) { float tt = e1; e1 = e2; e2 = tt; }
+                  jl = fmaxf(jl, e1);
+                  jh = fminf(jh, e2);
+                } else if (fabsf(o) > 1.02f + 1e-3f * fabsf(bk) * (float)(jc)) {
+                  none = true;
+                }
+              };
+              axis(fl, t.dm[0], t.dm[3], g.ibx);
+              axis(fm, t.dm[1], t.dm[4], g.iby);
+              axis(fn, t.dm[2], t.dm[5], g.ibz);
+              if (none) continue;
+              const int j0 = (int)ceilf(jl - 0.01f), j1 = (int)floorf(jh + 0.01f);
+              for (int j = j0; j <= j1; j++) {
                 float x3, y3, z3;
                 int ls, lt, ms, mt, ns, nt;
                 bool inside;
@@ -335,6 +361,7 @@ __global__ void __launch_bounds__(kGatherThreads)
                   a = z3 + 1.0f;          nt = (int)a; if (a < 0.0f) nt--; if (nt > nsamh) nt = nsamh;
                 }
                 if (!inside || l < ls || l > lt || m < ms || m > mt || n < ns || n > nt) continue;
+                if (j < jlo || j > jhi) continue;
                 const float b3 = 1.0f - fabsf(z3 - fn);
                 const float b2 = (1.0f - fabsf(y3 - fm));
                 const float box = (1.0f - fabsf(x3 - fl)) * b2 * b3;
@@ -354,6 +381,7 @@ __global__ void __launch_bounds__(kGatherThreads)
                   t0 += (double)cv;
                 }
               }
+            }
           }
       }
     }
