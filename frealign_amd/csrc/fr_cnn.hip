@@ -279,6 +279,30 @@ __global__ void __launch_bounds__(kT)
   if (threadIdx.x == 0) out[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
+// part[m * nsplit + s] = sum of g[m*ld + n] over split s of 0..N-1 (splits of
+// `chunk` elements); one block per (row, split), fixed order per thread
+__global__ void __launch_bounds__(kT)
+    k_rowsum_part(int N, long long ld, int chunk, const float* __restrict__ g, float* part) {
+  __shared__ float red[kT / 64];
+  const int m = blockIdx.x, sp = blockIdx.y;
+  const float* row = g + m * ld;
+  const int n0 = sp * chunk, n1 = min(N, n0 + chunk);
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  int n = n0 + threadIdx.x;
+  for (; n + 3 * kT < n1; n += 4 * kT) {
+    s0 += row[n];
+    s1 += row[n + kT];
+    s2 += row[n + 2 * kT];
+    s3 += row[n + 3 * kT];
+  }
+  for (; n < n1; n += kT) s0 += row[n];
+  float s = (s0 + s1) + (s2 + s3);
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) part[m * gridDim.y + sp] = red[0] + red[1] + red[2] + red[3];
+}
+
 __device__ inline float block_max(float v, float* red) {
   for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
   __syncthreads();
@@ -555,35 +579,54 @@ __device__ inline float sym_trace(const float* s, const float* a, const float* b
 __global__ void __launch_bounds__(kT)
     k_orient_nearest(int n, const float* __restrict__ dm, int ncls, const float* __restrict__ dmc,
                      int nsym, const float* __restrict__ sym, int* label) {
-  __shared__ float cls[kT * 9];
+  // one block per orientation; thread t scans classes t, t + kT, ...; the
+  // block keeps the largest trace, ties going to the lowest class (the first
+  // maximum of a scan over classes in order)
   __shared__ float ss[60 * 9];
+  __shared__ float rv[kT / 64];
+  __shared__ int ri[kT / 64];
   for (int e = threadIdx.x; e < nsym * 9; e += kT) ss[e] = sym[e];
-  const int i = blockIdx.x * kT + threadIdx.x;
+  __syncthreads();
+  const int i = blockIdx.x;
   float a[9];
-  if (i < n)
-    for (int q = 0; q < 9; q++) a[q] = dm[9 * (size_t)i + q];
+  for (int q = 0; q < 9; q++) a[q] = dm[9 * (size_t)i + q];
   float best = -3.0e38f;
   int bi = -1;
-  for (int c0 = 0; c0 < ncls; c0 += kT) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < kT * 9; e += kT) {
-      int c = c0 + e / 9;
-      cls[e] = c < ncls ? dmc[9 * (size_t)c0 + e] : 0.f;
-    }
-    __syncthreads();
-    if (i < n) {
-      const int cn = min(kT, ncls - c0);
-      for (int c = 0; c < cn; c++)
-        for (int q = 0; q < nsym; q++) {
-          float t = sym_trace(ss + 9 * q, a, cls + 9 * c);
-          if (t > best) {
-            best = t;
-            bi = c0 + c;
-          }
-        }
+  for (int c = threadIdx.x; c < ncls; c += kT) {
+    float b[9];
+    for (int q = 0; q < 9; q++) b[q] = dmc[9 * (size_t)c + q];
+    for (int q = 0; q < nsym; q++) {
+      float t = sym_trace(ss + 9 * q, a, b);
+      if (t > best) {
+        best = t;
+        bi = c;
+      }
     }
   }
-  if (i < n) label[i] = bi;
+  auto better = [](float v, int k, float bv, int bk) {
+    return k >= 0 && (bk < 0 || v > bv || (v == bv && k < bk));
+  };
+  for (int off = 32; off > 0; off >>= 1) {
+    float v = __shfl_down(best, off, 64);
+    int k = __shfl_down(bi, off, 64);
+    if (better(v, k, best, bi)) {
+      best = v;
+      bi = k;
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rv[threadIdx.x >> 6] = best;
+    ri[threadIdx.x >> 6] = bi;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kT / 64; w++)
+      if (better(rv[w], ri[w], best, bi)) {
+        best = rv[w];
+        bi = ri[w];
+      }
+    label[i] = bi;
+  }
 }
 
 // mask[b][w] bit i: orientation a = 32 w + i lies within the angle of
@@ -840,8 +883,16 @@ int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
     if (int r = gemm_splitk(c, lo.C, cin * 9, (int)NB, gcur, NB, 1, ws + w.cols, 1, NB,
                             grad + lo.w_off[l], ws + w.part, nsplit))
       return r;
-    hipLaunchKernelGGL(k_rowsum, dim3(lo.C), dim3(kT), 0, c.stream, (int)NB, NB, 1ll, gcur,
-                       grad + lo.b_off[l]);
+    // gb[C] = sum_n g[C][n]: per-split partials in ws.part (free again once
+    // the split-K sum above has run), then the splits in order
+    const int bchunk = 16384;
+    const int bsplit = (int)((NB + bchunk - 1) / bchunk);
+    FR_CHECK(bsplit <= kMaxSplit * 9, "batch too large for the bias reduction");
+    hipLaunchKernelGGL(k_rowsum_part, dim3(lo.C, bsplit), dim3(kT), 0, c.stream, (int)NB, NB,
+                       bchunk, gcur, ws + w.part);
+    FR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_rowsum, dim3(lo.C), dim3(kT), 0, c.stream, bsplit, (long long)bsplit, 1ll,
+                       ws + w.part, grad + lo.b_off[l]);
     FR_HIP(hipGetLastError());
     if (l > 0) {
       // dcols[cin*9][n] = sum_co W[co][k] g[co][n]
@@ -918,8 +969,9 @@ int cnn_synth(Ctx& c, int nsam, int S, int B, const float* a3df, const float* dm
 int orient_nearest(Ctx& c, int n, const float* dm, int ncls, const float* dmc, int nsym,
                    const float* sym, int* label) {
   FR_CHECK(nsym >= 1 && nsym <= 60, "1..60 symmetry operators");
-  hipLaunchKernelGGL(k_orient_nearest, dim3(nblk(n)), dim3(kT), 0, c.stream, n, dm, ncls, dmc,
-                     nsym, sym, label);
+  if (n == 0) return 0;
+  hipLaunchKernelGGL(k_orient_nearest, dim3(n), dim3(kT), 0, c.stream, n, dm, ncls, dmc, nsym,
+                     sym, label);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -130,45 +130,92 @@ __global__ void __launch_bounds__(kMaskThreads)
 namespace {
 
 // ctfapply.f:144-158: origin shift of DATC and OUTC, then PWEIGHT on OUTC.
-// One workgroup per particle; threads take fixed columns L and step over
-// rows M.  Shell bins are 1-based as in pweight.f.
+// One workgroup per particle.  Shell bins are 1-based as in pweight.f.  The
+// shell sums run in a fixed order: wave w takes the 64-point runs w, w + 16,
+// ... of the half-plane (row-major), each run reduces its points per shell
+// with a segmented scan (a row's shells never decrease along L, so a shell's
+// points in a run are contiguous), the run's last point of each shell adds
+// the run's sum into the wave's own bins, and the waves' bins are added in
+// wave order.
+constexpr int kPwWaves = kMaskThreads / 64;
+
+// LDS: kPwWaves wave bin sets and the combined bins, 2 JC + 2 floats each
+size_t pweight_lds(int nsam) { return sizeof(float) * (kPwWaves + 1) * (size_t)(nsam + 4); }
+
 __global__ void __launch_bounds__(kMaskThreads)
     k_origin_pweight(int nsam, const float* __restrict__ pssnr, int ictf,
                      const float2* __restrict__ ctff, float2* outc, float2* datc) {
-  __shared__ float rbuf[2 * 520 + 4];
+  extern __shared__ float pw_lds[];
   const int jc = nsam / 2 + 1;
-  const size_t ic = (size_t)nsam * jc;
-  float2* oc = outc + blockIdx.x * ic;
-  float2* dc = datc + blockIdx.x * ic;
-  const float2* cf_ = ctff + blockIdx.x * 2 * ic;
-  const int nt = blockDim.x;
-  const int cols = jc < nt ? jc : nt, rstep = nt / cols;
-  const int l0 = threadIdx.x % cols, r0 = threadIdx.x / cols;
-  const bool act = r0 < rstep;
-  for (int k = threadIdx.x; k < 2 * jc + 2; k += blockDim.x) rbuf[k] = 0.0f;
-  __syncthreads();
+  const int ic = nsam * jc;
+  const int nbin = 2 * jc + 2;
+  float* rbuf = pw_lds + kPwWaves * nbin;
+  float2* oc = outc + blockIdx.x * (size_t)ic;
+  float2* dc = datc + blockIdx.x * (size_t)ic;
+  const float2* cf_ = ctff + blockIdx.x * 2 * (size_t)ic;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float* wb = pw_lds + wave * nbin;
+  for (int k = lane; k < nbin; k += 64) wb[k] = 0.0f;
+  // segment keys: row * (jc + 3) + shell, shell jc + 2 marking points outside
+  // the disc (they close each row), so keys never decrease within a run
+  const int kstride = jc + 3;
   // origin shift, and the shell sums of pweight.f:31-42 over J=-JC+1..JC-1
   // (the Nyquist row is addressed by both J=-(JC-1) and J=+(JC-1))
-  if (act)
-    for (int m = r0; m < nsam; m += rstep)
-      for (int l = l0; l < jc; l += cols) {
-        const int e = m * jc + l;
-        int mm = m >= jc ? m - nsam : m;
-        float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
-        float2 o = oc[e], d = dc[e];
-        o.x *= sg; o.y *= sg;
-        d.x *= sg; d.y *= sg;
-        oc[e] = o;
-        dc[e] = d;
-        int itemp = l * l + mm * mm;
-        if (itemp < jc * jc) {
-          int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
-          float h = hypotf(o.x, o.y);
-          int visits = (2 * m == nsam) ? 2 : 1;
-          atomicAdd(&rbuf[ibin], visits * (h * h));
-          atomicAdd(&rbuf[ibin + jc], (float)visits);
-        }
+  for (int r0 = wave * 64; r0 < ic; r0 += kMaskThreads) {
+    const int e = r0 + lane;
+    int key = 0x7fffffff;  // past the end
+    float v = 0.0f, cnt = 0.0f;
+    if (e < ic) {
+      const int m = e / jc, l = e - m * jc;
+      int mm = m >= jc ? m - nsam : m;
+      float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
+      float2 o = oc[e], d = dc[e];
+      o.x *= sg; o.y *= sg;
+      d.x *= sg; d.y *= sg;
+      oc[e] = o;
+      dc[e] = d;
+      int itemp = l * l + mm * mm;
+      int ibin = jc + 2;
+      if (itemp < jc * jc) {
+        ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+        float h = hypotf(o.x, o.y);
+        float visits = (2 * m == nsam) ? 2.0f : 1.0f;
+        v = visits * (h * h);
+        cnt = visits;
       }
+      key = m * kstride + ibin;
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+      float vo = __shfl_up(v, off, 64), co = __shfl_up(cnt, off, 64);
+      int ko = __shfl_up(key, off, 64);
+      if (lane >= off && ko == key) {
+        v += vo;
+        cnt += co;
+      }
+    }
+    const int kn = __shfl_down(key, 1, 64);
+    const bool tail = e < ic && (lane == 63 || kn != key);
+    const int row = key / kstride, ibin = key - row * kstride;
+    // one row of the run at a time: the shells closing in one row are
+    // distinct, so no two lanes touch one address in a statement (bin JC+1's
+    // sum aliases bin 1's count, but bin 1 lies only in row 0 and bin JC+1
+    // never does); the empty asm keeps the compiler from moving LDS accesses
+    // across statements, since lanes may alias one another
+    const int mlo = r0 / jc, mhi = min(ic - 1, r0 + 63) / jc;
+    for (int mr = mlo; mr <= mhi; mr++) {
+      const bool w = tail && row == mr && ibin <= jc + 1;
+      if (w) wb[ibin] += v;
+      asm volatile("" ::: "memory");
+      if (w) wb[ibin + jc] += cnt;
+      asm volatile("" ::: "memory");
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nbin; k += kMaskThreads) {
+    float t = 0.0f;
+    for (int w = 0; w < kPwWaves; w++) t += pw_lds[w * nbin + k];
+    rbuf[k] = t;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int i = 1; i <= jc; i++) {
@@ -177,33 +224,31 @@ __global__ void __launch_bounds__(kMaskThreads)
     }
   }
   __syncthreads();
-  if (act)
-    for (int m = r0; m < nsam; m += rstep)
-      for (int l = l0; l < jc; l += cols) {
-        const int e = m * jc + l;
-        int mm = m >= jc ? m - nsam : m;
-        int itemp = l * l + mm * mm;
-        float2 o = oc[e];
-        if (itemp < jc * jc) {
-          int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
-          float snr;
-          if (ictf == 1) {
-            float2 cr = cf_[e], cl = cf_[e + ic];
-            float re = cr.x + cl.x, im = cr.y - cl.y;
-            float h = hypotf(re, im);
-            snr = pssnr[ibin - 1] * (h * h);
-          } else {
-            snr = pssnr[ibin - 1];
-          }
-          float wgt = snr / (snr + 1.0f) / rbuf[ibin];
-          o.x *= wgt;
-          o.y *= wgt;
-          if (2 * m == nsam) { o.x *= wgt; o.y *= wgt; }
-        } else {
-          o = make_float2(0.0f, 0.0f);
-        }
-        oc[e] = o;
+  for (int e = threadIdx.x; e < ic; e += kMaskThreads) {
+    const int m = e / jc, l = e - m * jc;
+    int mm = m >= jc ? m - nsam : m;
+    int itemp = l * l + mm * mm;
+    float2 o = oc[e];
+    if (itemp < jc * jc) {
+      int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
+      float snr;
+      if (ictf == 1) {
+        float2 cr = cf_[e], cl = cf_[e + ic];
+        float re = cr.x + cl.x, im = cr.y - cl.y;
+        float h = hypotf(re, im);
+        snr = pssnr[ibin - 1] * (h * h);
+      } else {
+        snr = pssnr[ibin - 1];
       }
+      float wgt = snr / (snr + 1.0f) / rbuf[ibin];
+      o.x *= wgt;
+      o.y *= wgt;
+      if (2 * m == nsam) { o.x *= wgt; o.y *= wgt; }
+    } else {
+      o = make_float2(0.0f, 0.0f);
+    }
+    oc[e] = o;
+  }
 }
 
 }  // namespace
@@ -230,7 +275,10 @@ int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const 
                      0);
   FR_HIP(hipGetLastError());
   if (int r = fft_r2c_2d(c, nsam, nb, outc, img)) return r;
-  hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kMaskThreads), 0, c.stream, nsam, pssnr, g.ictf,
+  const size_t lds = pweight_lds(nsam);
+  FR_HIP(hipFuncSetAttribute((const void*)k_origin_pweight,
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kMaskThreads), lds, c.stream, nsam, pssnr, g.ictf,
                      reinterpret_cast<const float2*>(ctff), reinterpret_cast<float2*>(outc),
                      reinterpret_cast<float2*>(datc));
   FR_HIP(hipGetLastError());

@@ -38,6 +38,10 @@ constexpr int kCols = 256;     // templates per block
 constexpr int kKStep = 32;     // reals per K step (16 complex coefficients)
 constexpr int kShift = 15;     // shift grid edge (-7..7 on the NSAMR grid)
 constexpr int kInner = 6;      // argmax window (-6..6), ccp.f:229-241
+#ifndef FR_SEARCH_BDEPTH
+#define FR_SEARCH_BDEPTH 1
+#endif
+constexpr int kBDepth = FR_SEARCH_BDEPTH;  // template K chunks in flight per thread
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float floatx4;
@@ -311,11 +315,12 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     for (int ct = 0; ct < 8; ct++) acc[rt][ct] = floatx4{0.f, 0.f, 0.f, 0.f};
   float nacc = 0.f;  // norm partial: column tid/2, complex 8*(tid%2)..+7 of each chunk
 
-  // this thread's B pieces: piece bq of templates bc0 + u * bstep
+  // this thread's B pieces: piece bq of templates bc0 + u * bstep, for the
+  // kBDepth chunks in flight (chunk c in register slot c % kBDepth)
   const int bq = tid % BP::per_row;
   const int bc0 = tid / BP::per_row;
   constexpr int bstep = kGemmThreads / BP::per_row;
-  uint4 breg[BP::per_thread];
+  uint4 breg[kBDepth][BP::per_thread];
   float2 treg;
 
   int orow[BP::per_thread];
@@ -325,21 +330,21 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     int o = col < ncol ? (cand_p ? cand_p[col] : col) : -1;
     orow[u] = (o >= 0 && o < nbank) ? o : -1;
   }
-  auto load_b = [&](int chunk) {
+  auto load_b = [&](uint4 (&r)[BP::per_thread], int chunk) {
 #pragma unroll
     for (int u = 0; u < BP::per_thread; u++) {
       if (orow[u] >= 0)
-        breg[u] = *reinterpret_cast<const uint4*>(bank + (size_t)orow[u] * kpad + 32 * chunk +
-                                                   bq * BP::elems);
+        r[u] = *reinterpret_cast<const uint4*>(bank + (size_t)orow[u] * kpad + 32 * chunk +
+                                                bq * BP::elems);
       else
-        breg[u] = make_uint4(0u, 0u, 0u, 0u);
+        r[u] = make_uint4(0u, 0u, 0u, 0u);
     }
   };
-  auto store_b = [&](int buf) {
+  auto store_b = [&](int buf, const uint4 (&r)[BP::per_thread]) {
     T* dst = Bs(buf);
 #pragma unroll
     for (int u = 0; u < BP::per_thread; u++)
-      *reinterpret_cast<uint4*>(dst + (bc0 + u * bstep) * P + bq * BP::elems) = breg[u];
+      *reinterpret_cast<uint4*>(dst + (bc0 + u * bstep) * P + bq * BP::elems) = r[u];
   };
   // one table entry per thread: 240 X'(k) e^{i I sx}, 240 e^{i J sy}, 16 |c(k)|^2
   auto load_tab = [&](int chunk) {
@@ -403,32 +408,43 @@ __device__ __forceinline__ void gemm_tile(unsigned char* smem_raw, int o0, int n
     }
   };
 
-  // prologue: tables for chunks 0 and 1, operands for chunk 0
+  // prologue: tables for chunks 0 and 1, template loads for chunks
+  // 0..kBDepth-1 in flight, operands for chunk 0 in LDS
   load_tab(0);
   store_tab(0);
   if (nchunk > 1) {
     load_tab(1);
     store_tab(1);
   }
-  load_b(0);
-  store_b(0);
+#pragma unroll
+  for (int d = 0; d < kBDepth; d++)
+    if (d < nchunk) load_b(breg[d], d);
+  store_b(0, breg[0]);
+  if (kBDepth < nchunk) load_b(breg[0], kBDepth);
   __syncthreads();
   gen_a(0, 0);
   __syncthreads();
 
-  for (int kc = 0; kc < nchunk; kc++) {
-    const int cur = kc & 1, nxt = cur ^ 1;
-    const bool more = kc + 1 < nchunk;
-    if (more) load_b(kc + 1);
-    if (kc + 2 < nchunk) load_tab(kc + 2);
-    mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
-    if (kFused) norm_step(cur, kc % 3);
-    if (more) {
-      gen_a(nxt, (kc + 1) % 3);
-      store_b(nxt);
+  // chunk kc's templates were loaded kBDepth chunks earlier; the loop is
+  // unrolled by kBDepth so that every register slot index is static
+  for (int kc0 = 0; kc0 < nchunk; kc0 += kBDepth) {
+#pragma unroll
+    for (int d = 0; d < kBDepth; d++) {
+      const int kc = kc0 + d;
+      if (kc >= nchunk) break;
+      const int cur = kc & 1, nxt = cur ^ 1;
+      const bool more = kc + 1 < nchunk;
+      if (kc + 2 < nchunk) load_tab(kc + 2);
+      mma_step<T>(As(cur), Bs(cur), acc, wr, wc, lane);
+      if (kFused) norm_step(cur, kc % 3);
+      if (more) {
+        gen_a(nxt, (kc + 1) % 3);
+        store_b(nxt, breg[(d + 1) % kBDepth]);
+        if (kc + 1 + kBDepth < nchunk) load_b(breg[(d + 1) % kBDepth], kc + 1 + kBDepth);
+      }
+      if (kc + 2 < nchunk) store_tab((kc + 2) % 3);
+      __syncthreads();
     }
-    if (kc + 2 < nchunk) store_tab((kc + 2) % 3);
-    __syncthreads();
   }
 
   // ------------------------------------------------------------ epilogue

@@ -86,7 +86,7 @@ def test_fft_r2c_c2r(ctx, nsam):
 
 
 # ------------------------------------------------------------- CTFAPPLY
-@pytest.mark.parametrize("nsam", [64, 96])
+@pytest.mark.parametrize("nsam", [64, 96, 256])
 def test_ctfapply_vs_oracle(ctx, nsam):
     nb = 3
     jc = nsam // 2 + 1
@@ -106,6 +106,11 @@ def test_ctfapply_vs_oracle(ctx, nsam):
                                     torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV),
                                     g)
     ctff, outc, datc = ctff.cpu().numpy(), outc.cpu().numpy(), datc.cpu().numpy()
+    # PWEIGHT's shell sums run in a fixed order: a rerun gives the same bits
+    again = ctx.ctfapply(nsam, torch.from_numpy(buf.copy()).to(DEV),
+                         torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV), g)
+    for x, y in zip((ctff, outc, datc), again):
+        assert np.array_equal(x.view(np.uint32), y.cpu().numpy().view(np.uint32))
     for b in range(nb):
         spec = buf[b].copy().reshape(-1)
         L.or_rfft2_f(nsam, spec)
