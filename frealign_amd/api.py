@@ -229,21 +229,43 @@ class Context:
                 "fr_search_layout")
         return kc.value, kpad.value
 
+    SEP_ROW = 4096      # bf16 values per template of the separable bank
+    SEP_MAXR2 = 32      # the separable bank holds MAXR2 <= 32
+
     def search_bank(self, nsam: int, maxr1: int, maxr2: int, a3df, dm: torch.Tensor,
-                    dtype=torch.bfloat16) -> torch.Tensor:
-        """CTF-free template bank (norient, kpad) for the orientations dm (norient, 9)."""
+                    dtype=torch.bfloat16, separable: bool = False) -> torch.Tensor:
+        """CTF-free template bank for the orientations dm (norient, 9): the
+        annulus in order (norient, kpad), or with `separable` (bf16, MAXR2 <= 32)
+        the layout of the two-stage search (norient, 4096)."""
         _dev(a3df, torch.complex64, "a3df")
         _dev(dm, torch.float32, "dm")
         if dtype not in (torch.bfloat16, torch.float32):
             raise ValueError("bank dtype must be bfloat16 or float32")
         _, kpad = self.search_layout(nsam, maxr1, maxr2)
         n = dm.numel() // 9
-        bank = torch.empty((n, kpad), dtype=dtype, device=a3df.device)
+        if separable:
+            if dtype != torch.bfloat16 or maxr2 > self.SEP_MAXR2:
+                raise ValueError("the separable bank is bfloat16 with MAXR2 <= 32")
+            row, code = self.SEP_ROW, 2
+        else:
+            row, code = kpad, 1 if dtype == torch.bfloat16 else 0
+        bank = torch.empty((n, row), dtype=dtype, device=a3df.device)
         self._sync_stream()
-        N.check(N.lib().fr_search_bank(self.h, nsam, maxr1, maxr2, _p(a3df), n, _p(dm),
-                                       1 if dtype == torch.bfloat16 else 0, _p(bank)),
-                "fr_search_bank")
+        N.check(N.lib().fr_search_bank(self.h, nsam, maxr1, maxr2, _p(a3df), n, _p(dm), code,
+                                       _p(bank)), "fr_search_bank")
         return bank
+
+    def _bank_code(self, nsam: int, maxr1: int, maxr2: int, bank: torch.Tensor) -> int:
+        """0: FP32 annulus bank, 1: BF16 annulus bank, 2: BF16 separable bank."""
+        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
+            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        _, kpad = self.search_layout(nsam, maxr1, maxr2)
+        if bank.dim() == 2 and bank.shape[1] == kpad:
+            return 1 if bank.dtype == torch.bfloat16 else 0
+        if (bank.dim() == 2 and bank.dtype == torch.bfloat16 and bank.shape[1] == self.SEP_ROW
+                and maxr2 <= self.SEP_MAXR2):
+            return 2
+        raise ValueError(f"bank rows must be {kpad} long (or {self.SEP_ROW}, separable bf16)")
 
     def search(self, nsam: int, maxr1: int, maxr2: int, rbfact: float, bank: torch.Tensor,
                outc, ctff, topk: int = 32, group: int = 1):
@@ -251,23 +273,18 @@ class Context:
         `group` consecutive orientations form one view (its in-plane angles)."""
         _dev(outc, torch.complex64, "outc")
         _dev(ctff, torch.complex64, "ctff")
-        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
-            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        code = self._bank_code(nsam, maxr1, maxr2, bank)
         jc = nsam // 2 + 1
         np_ = outc.numel() // (nsam * jc)
         if ctff.numel() != 2 * outc.numel():
             raise ValueError("ctff must hold two half-planes per particle")
-        _, kpad = self.search_layout(nsam, maxr1, maxr2)
-        if bank.dim() != 2 or bank.shape[1] != kpad:
-            raise ValueError(f"bank rows must be {kpad} long")
         dev = outc.device
         top_o = torch.empty((np_, topk), dtype=torch.int32, device=dev)
         top_r = torch.empty((np_, topk), dtype=torch.int32, device=dev)
         top_s = torch.empty((np_, topk), dtype=torch.float32, device=dev)
         self._sync_stream()
         N.check(N.lib().fr_search_batch(self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0],
-                                        _p(bank), 1 if bank.dtype == torch.bfloat16 else 0,
-                                        int(group), np_,
+                                        _p(bank), code, int(group), np_,
                                         _p(outc), _p(ctff), topk, _p(top_o), _p(top_r),
                                         _p(top_s)), "fr_search_batch")
         return top_o, top_r, top_s
@@ -279,17 +296,13 @@ class Context:
         _dev(outc, torch.complex64, "outc")
         _dev(ctff, torch.complex64, "ctff")
         _dev(cand, torch.int32, "cand")
-        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
-            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        code = self._bank_code(nsam, maxr1, maxr2, bank)
         jc = nsam // 2 + 1
         np_ = outc.numel() // (nsam * jc)
         if ctff.numel() != 2 * outc.numel():
             raise ValueError("ctff must hold two half-planes per particle")
         if cand.dim() != 2 or cand.shape[0] != np_:
             raise ValueError("cand must be (particles, ncand)")
-        _, kpad = self.search_layout(nsam, maxr1, maxr2)
-        if bank.dim() != 2 or bank.shape[1] != kpad:
-            raise ValueError(f"bank rows must be {kpad} long")
         if np_ and int(cand.max()) >= bank.shape[0]:
             raise ValueError("candidate index beyond the bank")
         dev = outc.device
@@ -299,7 +312,7 @@ class Context:
         self._sync_stream()
         N.check(N.lib().fr_search_list_batch(
             self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0], _p(bank),
-            1 if bank.dtype == torch.bfloat16 else 0, np_, _p(outc), _p(ctff), cand.shape[1],
+            code, np_, _p(outc), _p(ctff), cand.shape[1],
             _p(cand), int(group), topk, _p(top_o), _p(top_r), _p(top_s)), "fr_search_list_batch")
         return top_o, top_r, top_s
 
@@ -340,17 +353,13 @@ class Context:
         _dev(ctff, torch.complex64, "ctff")
         for t, nm in ((loff, "loff"), (grp, "grp"), (cand, "cand")):
             _dev(t, torch.int32, nm)
-        if bank.dtype not in (torch.bfloat16, torch.float32) or not bank.is_cuda:
-            raise ValueError("bank must be a device bfloat16/float32 tensor")
+        code = self._bank_code(nsam, maxr1, maxr2, bank)
         jc = nsam // 2 + 1
         np_ = outc.numel() // (nsam * jc)
         if ctff.numel() != 2 * outc.numel():
             raise ValueError("ctff must hold two half-planes per particle")
         if loff.numel() != np_ + 1 or grp.numel() != np_:
             raise ValueError("loff must hold np+1 offsets and grp np group sizes")
-        _, kpad = self.search_layout(nsam, maxr1, maxr2)
-        if bank.dim() != 2 or bank.shape[1] != kpad:
-            raise ValueError(f"bank rows must be {kpad} long")
         dev = outc.device
         top_o = torch.empty((np_, topk), dtype=torch.int32, device=dev)
         top_r = torch.empty((np_, topk), dtype=torch.int32, device=dev)
@@ -358,7 +367,7 @@ class Context:
         self._sync_stream()
         N.check(N.lib().fr_search_ragged_batch(
             self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0], _p(bank),
-            1 if bank.dtype == torch.bfloat16 else 0, np_, _p(outc), _p(ctff), _p(loff), _p(grp),
+            code, np_, _p(outc), _p(ctff), _p(loff), _p(grp),
             _p(cand), cand.numel(), topk, _p(top_o), _p(top_r), _p(top_s)),
             "fr_search_ragged_batch")
         return top_o, top_r, top_s

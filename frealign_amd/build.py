@@ -25,6 +25,7 @@ SOURCES = [
     "fr_stats.hip",
     "fr_search.hip",
     "fr_lists.hip",
+    "fr_search_sep.hip",
     "fr_cnn.hip",
     "fr_refine.cpp",
     "fr_sigma.hip",

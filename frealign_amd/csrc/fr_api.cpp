@@ -569,6 +569,7 @@ static int search_tables(fr_ctx* c, int nsam, int maxr1, int maxr2) {
   FR_HIP(hipMemcpy(c->d_shy, shy.data(), sizeof(float) * shy.size(), hipMemcpyHostToDevice));
   c->kc = kc;
   c->kpad = 2 * half;
+  c->nsamr = nsamr;
   c->skey[0] = nsam;
   c->skey[1] = maxr1;
   c->skey[2] = maxr2;
@@ -590,10 +591,40 @@ int fr_search_bank(fr_ctx* c, int nsam, int maxr1, int maxr2, const float* d_a3d
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(d_a3df && d_dm && d_bank && norient >= 0, "null argument");
-  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(dtype >= 0 && dtype <= 2, "dtype must be 0 (fp32), 1 (bf16) or 2 (bf16 separable)");
   if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
+  if (dtype == 2) {
+    FR_CHECK(maxr2 <= kSepMaxR2, "the separable bank holds MAXR2 <= %d (got %d)", kSepMaxR2,
+             maxr2);
+    return launch_bank_build_sep(*c, nsam, norient, maxr1 * maxr1 < 1 ? 1 : maxr1 * maxr1,
+                                 maxr2 * maxr2, d_a3df, d_dm, d_bank);
+  }
   return launch_bank_build(*c, nsam, norient, c->kc, c->kpad, d_a3df, d_dm, c->d_kidx, d_bank,
                            dtype == 1);
+}
+
+// Particle terms of the separable search (slots 23-25) and SUM2 (slot 5).
+static int sep_terms(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int np,
+                     const float* d_outc, const float* d_ctff, void** xa, void** xb, float** c2s,
+                     float** sum2) {
+  FR_CHECK(maxr2 <= kSepMaxR2, "the separable search holds MAXR2 <= %d (got %d)", kSepMaxR2,
+           maxr2);
+  const int half = c->kpad / 2;
+  const size_t per = sizeof(unsigned) * 2048 * (size_t)np;
+  int err = 0;
+  float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
+  if (!ap) return err;
+  float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
+  if (!c2) return err;
+  if (!(*sum2 = (float*)c->slot(5, sizeof(float) * (size_t)np, &err))) return err;
+  if (!(*xa = c->slot(23, per, &err))) return err;
+  if (!(*xb = c->slot(24, per, &err))) return err;
+  if (!(*c2s = (float*)c->slot(25, per, &err))) return err;
+  if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
+                                  d_ctff, c->d_kidx, ap, c2, *sum2))
+    return r;
+  return launch_sep_feats(*c, nsam, np, maxr1 * maxr1 < 1 ? 1 : maxr1 * maxr1, maxr2 * maxr2,
+                          rbfact, d_outc, d_ctff, *xa, *xb, *c2s);
 }
 
 int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int norient,
@@ -603,13 +634,30 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(d_bank && d_outc && d_ctff && d_top_o && d_top_row && d_top_score, "null argument");
-  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(dtype >= 0 && dtype <= 2, "dtype must be 0 (fp32), 1 (bf16) or 2 (bf16 separable)");
   FR_CHECK(np >= 0 && norient > 0 && topk >= 1 && topk <= 32, "bad sizes");
   FR_CHECK(group >= 1 && norient % group == 0, "orientations must come in whole view groups");
   if (np == 0) return 0;
   if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
   const int half = c->kpad / 2;
   int err = 0;
+  if (dtype == 2) {
+    void *xa, *xb;
+    float *c2s, *sum2;
+    if (int r = sep_terms(c, nsam, maxr1, maxr2, rbfact, np, d_outc, d_ctff, &xa, &xb, &c2s, &sum2))
+      return r;
+    float* score = (float*)c->slot(6, sizeof(float) * (size_t)np * norient, &err);
+    if (!score) return err;
+    int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * norient, &err);
+    if (!brow) return err;
+    int pslot = c->prof_begin(kProfSearchGemm);
+    if (int r = launch_search_sep(*c, np, norient, c->nsamr, norient, nullptr, nullptr, 0, 0,
+                                  nullptr, d_bank, xa, xb, c2s, sum2, score, brow))
+      return r;
+    c->prof_end(pslot);
+    return launch_topk(*c, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
+                       d_top_o, d_top_row, d_top_score);
+  }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
   if (!ap) return err;
   float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
@@ -753,13 +801,30 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
   SIZE_CHECK(nsam);
   FR_CHECK(d_bank && d_outc && d_ctff && d_cand && d_top_o && d_top_row && d_top_score,
            "null argument");
-  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(dtype >= 0 && dtype <= 2, "dtype must be 0 (fp32), 1 (bf16) or 2 (bf16 separable)");
   FR_CHECK(np >= 0 && norient > 0 && ncand > 0 && topk >= 1 && topk <= 32, "bad sizes");
   FR_CHECK(group >= 1 && ncand % group == 0, "candidate lists must come in whole groups");
   if (np == 0) return 0;
   if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
   const int half = c->kpad / 2;
   int err = 0;
+  if (dtype == 2) {
+    void *xa, *xb;
+    float *c2s, *sum2;
+    if (int r = sep_terms(c, nsam, maxr1, maxr2, rbfact, np, d_outc, d_ctff, &xa, &xb, &c2s, &sum2))
+      return r;
+    float* score = (float*)c->slot(6, sizeof(float) * (size_t)np * ncand, &err);
+    if (!score) return err;
+    int* brow = (int*)c->slot(7, sizeof(int) * (size_t)np * ncand, &err);
+    if (!brow) return err;
+    int pslot = c->prof_begin(kProfSearchList);
+    if (int r = launch_search_sep(*c, np, norient, c->nsamr, ncand, nullptr, nullptr, 0, 0,
+                                  d_cand, d_bank, xa, xb, c2s, sum2, score, brow))
+      return r;
+    c->prof_end(pslot);
+    return launch_topk(*c, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
+                       d_top_o, d_top_row, d_top_score);
+  }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
   if (!ap) return err;
   float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
@@ -813,12 +878,36 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
   FR_CHECK(d_bank && d_outc && d_ctff && d_loff && d_grp && d_cand && d_top_o && d_top_row &&
                d_top_score,
            "null argument");
-  FR_CHECK(dtype == 0 || dtype == 1, "dtype must be 0 (fp32) or 1 (bf16)");
+  FR_CHECK(dtype >= 0 && dtype <= 2, "dtype must be 0 (fp32), 1 (bf16) or 2 (bf16 separable)");
   FR_CHECK(np >= 0 && norient > 0 && cap > 0 && topk >= 1 && topk <= 32, "bad sizes");
   if (np == 0) return 0;
   if (int r = search_tables(c, nsam, maxr1, maxr2)) return r;
   const int half = c->kpad / 2;
   int err = 0;
+  // tiles of lists within the capacity: at most cap/256 full tiles plus one
+  // partial tile per particle
+  const long long max_tiles = (long long)cap / 256 + np;
+  if (dtype == 2) {
+    void *xa, *xb;
+    float *c2s, *sum2;
+    if (int r = sep_terms(c, nsam, maxr1, maxr2, rbfact, np, d_outc, d_ctff, &xa, &xb, &c2s, &sum2))
+      return r;
+    float* score = (float*)c->slot(6, sizeof(float) * (size_t)cap, &err);
+    if (!score) return err;
+    int* brow = (int*)c->slot(7, sizeof(int) * (size_t)cap, &err);
+    if (!brow) return err;
+    int* toff = (int*)c->slot(17, sizeof(int) * ((size_t)np + 1), &err);
+    if (!toff) return err;
+    if (int r = launch_tile_offsets(*c, np, d_loff, toff)) return r;
+    int pslot = c->prof_begin(kProfSearchList);
+    if (int r = launch_search_sep(*c, np, norient, c->nsamr, 0, d_loff, toff, cap,
+                                  (int)(max_tiles < (1LL << 30) ? max_tiles : (1LL << 30)), d_cand,
+                                  d_bank, xa, xb, c2s, sum2, score, brow))
+      return r;
+    c->prof_end(pslot);
+    return launch_topk(*c, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
+                       d_top_row, d_top_score);
+  }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
   if (!ap) return err;
   float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
@@ -835,9 +924,6 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
   if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
                                   d_ctff, c->d_kidx, ap, c2, sum2))
     return r;
-  // tiles of lists within the capacity: at most cap/256 full tiles plus one
-  // partial tile per particle
-  const long long max_tiles = (long long)cap / 256 + np;
   int pslot = c->prof_begin(kProfSearchList);
   if (int r = launch_search_gemm_ragged(*c, np, norient, c->kpad, d_bank, dtype == 1, d_loff, toff,
                                         d_cand, cap, (int)(max_tiles < (1LL << 30) ? max_tiles

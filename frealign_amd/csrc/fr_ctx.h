@@ -10,13 +10,13 @@
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  static constexpr int kSlots = 24;
+  static constexpr int kSlots = 27;
   void* slots[kSlots] = {};
   size_t slot_bytes[kSlots] = {};
   void* slot(int k, size_t bytes, int* err);
   // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
   int skey[3] = {0, 0, 0};
-  int kc = 0, kpad = 0;
+  int kc = 0, kpad = 0, nsamr = 0;
   int* d_kidx = nullptr;
   float* d_shx = nullptr;
   float* d_shy = nullptr;

@@ -278,8 +278,8 @@ class LearnedProposal:
         loff, grp, cand, info = self.ctx.build_lists(
             top, prob, self.cfg.threshold, self.cell_off_dev, self.cells_dev, self.cfg.group,
             s.norient, s.group, self.maxlen)
-        o, r, sg = self.ctx.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.bank, outc, ctff,
-                                          loff, grp, cand, topk)
+        o, r, sg = self.ctx.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.search_bank,
+                                          outc, ctff, loff, grp, cand, topk)
         sc, best = self.ctx.rescore(s.nsam, s.a3df, outc, ctff, o, r, s.dm, s.nsamr, s.maxr1,
                                     s.maxr2, s.rbfact, s.fsct_device(fsct), ring)
         npart = o.shape[0]

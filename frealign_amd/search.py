@@ -100,6 +100,12 @@ class GlobalSearch:
         self.dm = torch.from_numpy(dm).to(a3df.device)
         self.dtype = dtype
         self.bank = ctx.search_bank(self.nsam, maxr1, maxr2, a3df, self.dm, dtype=dtype)
+        # BF16 searches run on the separable bank (two-stage translations)
+        # where the band fits it; the annulus bank stays for the GEMM form
+        self.bank_sep = None
+        if dtype == torch.bfloat16 and maxr2 <= ctx.SEP_MAXR2:
+            self.bank_sep = ctx.search_bank(self.nsam, maxr1, maxr2, a3df, self.dm, dtype=dtype,
+                                            separable=True)
         nsamh = self.nsam // 2
         nsamrh = nsamh // 2 if (nsamh // 2) * 2 == nsamh else (nsamh + 1) // 2
         self.nsamr = 2 * nsamrh
@@ -120,9 +126,14 @@ class GlobalSearch:
     def norient(self) -> int:
         return self.euler.shape[0]
 
+    @property
+    def search_bank(self) -> torch.Tensor:
+        """The bank the searches read: separable where built, else the annulus bank."""
+        return self.bank_sep if self.bank_sep is not None else self.bank
+
     def candidates(self, outc: torch.Tensor, ctff: torch.Tensor, topk: int = 32):
-        return self.ctx.search(self.nsam, self.maxr1, self.maxr2, self.rbfact, self.bank, outc,
-                               ctff, topk=topk, group=self.group)
+        return self.ctx.search(self.nsam, self.maxr1, self.maxr2, self.rbfact, self.search_bank,
+                               outc, ctff, topk=topk, group=self.group)
 
     def row_to_shift(self, row: np.ndarray) -> np.ndarray:
         """ccp.f:254-256: SX = IXM/NSAMR*2pi on the NSAMR grid."""

@@ -157,14 +157,21 @@ int fr_storeshift(fr_ctx *ctx, int nsam, double std_, double vtd, int iredun, co
  * orientation) is scored as CCP scores a view: argmax over -6..6 (x outer,
  * first max wins), 3x3 box sum /9, normalised by sqrt(sum|cT|^2 sum|X|^2).
  * dtype 1 runs the correlation as a BF16 MFMA GEMM with FP32 accumulation,
- * dtype 0 as an FP32 MFMA GEMM. */
+ * dtype 0 as an FP32 MFMA GEMM.  dtype 2 (MAXR2 <= 32) evaluates the same
+ * 15x15 map on the BF16 matrix cores in two separable stages (the J phase,
+ * then the I phase) from a bank in the separable layout. */
 
 /* Annulus size kc and the padded real row length kpad of a bank row. */
 int fr_search_layout(fr_ctx *ctx, int nsam, int maxr1, int maxr2, int *kc, int *kpad);
 
 /* Bank of CTF-free central sections T_o (AINTERPO3DS of the plane
  * d_dm[9o..]*(I,J), dm from fr_rotation with amag=AMAG): norient x kpad
- * elements of 4 (dtype 0) or 2 (dtype 1) bytes. */
+ * elements of 4 (dtype 0) or 2 (dtype 1) bytes, the annulus in order; or
+ * (dtype 2) norient x 4096 bf16 in the separable layout: grid point
+ * (I, J), I in 0..31, J in -32..31, is complex element
+ * ((q * 64 + lane) * 4 + t) of the row with q = (I / 16) * 4 + (J + 32) / 16,
+ * lane = ((J + 32) % 16 / 4) * 16 + I % 16, t = (J + 32) % 4; zero off the
+ * annulus.  The bf16 values equal dtype 1's. */
 int fr_search_bank(fr_ctx *ctx, int nsam, int maxr1, int maxr2, const float *d_a3df, int norient,
                    const float *d_dm, int dtype, void *d_bank);
 

@@ -272,7 +272,7 @@ def test_list_search_equals_exhaustive(ctx, small_search):
     o_d, r_d, s_d = gs.candidates(outc, ctff, topk=8)
     # candidate list = every orientation, in bank order, grouped by view
     cand = torch.arange(gs.norient, dtype=torch.int32, device=DEV).repeat(8, 1).contiguous()
-    o_l, r_l, s_l = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff,
+    o_l, r_l, s_l = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.search_bank, outc, ctff,
                                     cand, gs.group, topk=8)
     assert np.array_equal(o_d.cpu().numpy()[:, 0], o_l.cpu().numpy()[:, 0])
     assert np.array_equal(r_d.cpu().numpy()[:, 0], r_l.cpu().numpy()[:, 0])
@@ -286,7 +286,7 @@ def test_list_search_equals_exhaustive(ctx, small_search):
         rng.shuffle(views)
         rows.append(np.concatenate([np.arange(v * gs.group, (v + 1) * gs.group) for v in views]))
     cand2 = torch.from_numpy(np.array(rows, np.int32)).to(DEV)
-    o2, r2, s2 = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, cand2,
+    o2, r2, s2 = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.search_bank, outc, ctff, cand2,
                                  gs.group, topk=4)
     assert np.array_equal(o2.cpu().numpy()[:, 0], o_d.cpu().numpy()[:, 0])
     assert np.allclose(s2.cpu().numpy()[:, 0], s_d.cpu().numpy()[:, 0], rtol=1e-5)
@@ -334,7 +334,7 @@ def test_training_gains_over_no_network(ctx, small_search):
     # no network: the same number of classes, chosen at random
     rtop = np.stack([rng.choice(lp.classes.n, cfg.topk, replace=False) for _ in range(64)])
     cand = torch.from_numpy(lp.candidates(rtop)).to(DEV)
-    o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, cand,
+    o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.search_bank, outc, ctff, cand,
                               cfg.group, topk=8)
     sc, sh = gs.rescore(outc, ctff, o, r, fsct, ring)
     bo = o.cpu().numpy()[np.arange(64), np.argmax(sc, axis=1)]

@@ -59,15 +59,22 @@ def ragged(lists, groups):
     return t(loff), t(np.asarray(groups, np.int32)), t(cand if len(cand) else np.full(1, -1))
 
 
-def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
+@pytest.mark.parametrize("kind", ["annulus", "separable"])
+def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup, kind):
     nsam, a3df, gs, optics, parts, outc, ctff, datc = setup
+    bank = gs.bank if kind == "annulus" else gs.bank_sep
     npart = outc.shape[0]
     topk = 8
-    o_d, r_d, s_d = [x.cpu().numpy() for x in gs.candidates(outc, ctff, topk)]
+    o_d, r_d, s_d = [x.cpu().numpy() for x in ctx.search(
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff, topk=topk, group=gs.group)]
     bank_rows = torch.arange(gs.norient, dtype=torch.int32, device=DEV).repeat(npart, 1)
     o_f, r_f, s_f = [x.cpu().numpy() for x in ctx.search_list(
-        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, bank_rows.contiguous(),
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff, bank_rows.contiguous(),
         gs.group, topk=topk)]
+    if kind == "separable":
+        # one kernel, the norm inside it either way: the same bits
+        assert np.array_equal(o_f, o_d) and np.array_equal(r_f, r_d)
+        assert np.array_equal(s_f.view(np.uint32), s_d.view(np.uint32))
     assert np.array_equal(o_f[:, 0], o_d[:, 0])
     assert np.allclose(s_f, s_d, rtol=1e-5, atol=0)
     # every particle: 6 random views (with their whole in-plane groups)
@@ -78,7 +85,7 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
         rows.append(np.concatenate([np.arange(v * gs.group, (v + 1) * gs.group) for v in views]))
     rows = np.array(rows, np.int32)
     o_u, r_u, s_u = [x.cpu().numpy() for x in ctx.search_list(
-        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff,
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff,
         torch.from_numpy(rows).to(DEV), gs.group, topk=topk)]
     # ragged: particles 0, 5, 9 search the whole bank, 7 has an empty list,
     # the others their uniform-list row (some cut to fewer views)
@@ -94,7 +101,7 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
         groups.append(gs.group)
     loff, grp, cand = ragged(lists, groups)
     o_r, r_r, s_r = [x.cpu().numpy() for x in ctx.search_ragged(
-        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, outc, ctff, loff, grp, cand, topk)]
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff, loff, grp, cand, topk)]
     for p in range(npart):
         if p in full:
             ref = (o_f[p], r_f[p], s_f[p])
@@ -108,20 +115,22 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup):
         assert np.array_equal(s_r[p].view(np.uint32), ref[2].view(np.uint32)), p
 
 
-def test_ragged_capacity_is_respected(ctx, setup):
+@pytest.mark.parametrize("kind", ["annulus", "separable"])
+def test_ragged_capacity_is_respected(ctx, setup, kind):
     """Entries at or beyond the capacity are ignored: a list cut by the
     capacity gives the answer of the shorter list."""
     nsam, a3df, gs, optics, parts, outc, ctff, datc = setup
+    bank = gs.bank if kind == "annulus" else gs.bank_sep
     o1, c1 = outc[:2].contiguous(), ctff[:2].contiguous()
     lists = [np.arange(3 * gs.group, dtype=np.int32),
              np.arange(5 * gs.group, dtype=np.int32)]
     loff, grp, cand = ragged(lists, [gs.group, gs.group])
     short = cand[:3 * gs.group + 2 * gs.group].clone()  # particle 1 keeps 2 views
     o_c, r_c, s_c = [x.cpu().numpy() for x in ctx.search_ragged(
-        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, o1, c1, loff, grp, short, 4)]
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, o1, c1, loff, grp, short, 4)]
     loff2, grp2, cand2 = ragged([lists[0], lists[1][:2 * gs.group]], [gs.group, gs.group])
     o_s, r_s, s_s = [x.cpu().numpy() for x in ctx.search_ragged(
-        nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank, o1, c1, loff2, grp2, cand2, 4)]
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, o1, c1, loff2, grp2, cand2, 4)]
     assert np.array_equal(o_c, o_s) and np.array_equal(r_c, r_s)
     assert np.array_equal(s_c.view(np.uint32), s_s.view(np.uint32))
 
@@ -219,12 +228,12 @@ def test_learned_run_equals_composed_host_path(ctx, setup):
         one = slice(p, p + 1)
         if conf_h[p]:
             cand = torch.from_numpy(lists[p][None, :].copy()).to(DEV)
-            o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank,
+            o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.search_bank,
                                       outc[one].contiguous(), ctff[one].contiguous(), cand,
                                       cfg.group, topk=8)
         else:
             cand = torch.arange(gs.norient, dtype=torch.int32, device=DEV)[None, :].contiguous()
-            o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.bank,
+            o, r, s = ctx.search_list(nsam, gs.maxr1, gs.maxr2, gs.rbfact, gs.search_bank,
                                       outc[one].contiguous(), ctff[one].contiguous(), cand,
                                       gs.group, topk=8)
         o_all[p], r_all[p], s_all[p] = o.cpu().numpy()[0], r.cpu().numpy()[0], s.cpu().numpy()[0]

@@ -102,9 +102,47 @@ def test_search_bank_matches_oracle(ctx):
     assert np.all(bank[:, 2 * kc:] == 0)
 
 
-@pytest.mark.parametrize("dtype,tol", [(torch.float32, 1e-4), (torch.bfloat16, 2e-2)])
-def test_search_scores_vs_numpy(ctx, dtype, tol):
-    nsam, maxr1, maxr2, rbfact = 64, 2, 14, 0.0005
+def sep_element(i, j):
+    """Complex element of grid point (I, J) in a row of the separable bank."""
+    jg = j + 32
+    q = (i // 16) * 4 + jg // 16
+    lane = ((jg % 16) // 4) * 16 + i % 16
+    return (q * 64 + lane) * 4 + jg % 4
+
+
+@pytest.mark.parametrize("maxr1,maxr2", [(2, 14), (1, 32)])
+def test_separable_bank_equals_annulus_bank(ctx, maxr1, maxr2):
+    """The separable layout holds the annulus bank's BF16 values, bit for bit,
+    at the documented positions, and zeros elsewhere."""
+    nsam = 128
+    rng = np.random.default_rng(6)
+    v = rng.standard_normal((nsam,) * 3).astype(np.float32)
+    vol = torch.from_numpy(np.ascontiguousarray(np.fft.rfftn(v).astype(np.complex64))).to(DEV)
+    eul = rng.uniform(-np.pi, np.pi, (21, 3)).astype(np.float32)
+    dm = torch.from_numpy(api.rotation(eul, amag=1.0)).to(DEV)
+    ann = ctx.search_bank(nsam, maxr1, maxr2, vol, dm, dtype=torch.bfloat16)
+    sep = ctx.search_bank(nsam, maxr1, maxr2, vol, dm, dtype=torch.bfloat16, separable=True)
+    assert sep.shape == (21, 4096)
+    ann = ann.view(torch.int16).cpu().numpy().astype(np.int32)
+    sep = sep.view(torch.int16).cpu().numpy().astype(np.int32)
+    kidx = annulus(nsam, maxr1, maxr2)
+    e = np.array([sep_element(i, j) for i, j in kidx])
+    assert len(set(e.tolist())) == len(e)
+    kc = len(kidx)
+    assert np.array_equal(sep[:, 2 * e], ann[:, 0:2 * kc:2])
+    assert np.array_equal(sep[:, 2 * e + 1], ann[:, 1:2 * kc:2])
+    rest = np.ones(2048, bool)
+    rest[e] = False
+    assert np.all(sep[:, 0::2][:, rest] == 0) and np.all(sep[:, 1::2][:, rest] == 0)
+
+
+# BF16 runs the separable two-stage search (the whole I < 32, |J| < 32 grid
+# at 128, MAXR2 = 32)
+@pytest.mark.parametrize("dtype,tol,nsam,maxr2", [(torch.float32, 1e-4, 64, 14),
+                                                  (torch.bfloat16, 2e-2, 64, 14),
+                                                  (torch.bfloat16, 2e-2, 128, 32)])
+def test_search_scores_vs_numpy(ctx, dtype, tol, nsam, maxr2):
+    maxr1, rbfact = 2, 0.0005
     jc = nsam // 2 + 1
     rng = np.random.default_rng(11)
     v = rng.standard_normal((nsam,) * 3).astype(np.float32)
