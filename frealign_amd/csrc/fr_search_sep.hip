@@ -65,6 +65,35 @@ __device__ inline unsigned pack_bf16(float lo, float hi) {
 
 __device__ inline bf16x2 as_bf2(unsigned u) { return *reinterpret_cast<bf16x2*>(&u); }
 
+// Wave-wide maximum and sum through DPP row operations (fixed order), the
+// result read from lane 63: a uniform value.
+template <int kCtrl, int kRowMask>
+__device__ inline float dpp_f(float old, float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), kCtrl,
+                                                    kRowMask, 0xF, false));
+}
+
+__device__ inline float wave_max(float v) {
+  const float lo = -3.4e38f;
+  v = fmaxf(v, dpp_f<0xB1, 0xF>(lo, v));   // quad_perm [1,0,3,2]
+  v = fmaxf(v, dpp_f<0x4E, 0xF>(lo, v));   // quad_perm [2,3,0,1]
+  v = fmaxf(v, dpp_f<0x141, 0xF>(lo, v));  // row_half_mirror
+  v = fmaxf(v, dpp_f<0x140, 0xF>(lo, v));  // row_mirror
+  v = fmaxf(v, dpp_f<0x142, 0xA>(lo, v));  // row_bcast15
+  v = fmaxf(v, dpp_f<0x143, 0xC>(lo, v));  // row_bcast31
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
+__device__ inline float wave_sum(float v) {
+  v += dpp_f<0xB1, 0xF>(0.f, v);
+  v += dpp_f<0x4E, 0xF>(0.f, v);
+  v += dpp_f<0x141, 0xF>(0.f, v);
+  v += dpp_f<0x140, 0xF>(0.f, v);
+  v += dpp_f<0x142, 0xA>(0.f, v);
+  v += dpp_f<0x143, 0xC>(0.f, v);
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+
 // cos and sin of 2 pi (a mod n) / n: the argument reduced exactly first
 __device__ inline void phase(int a, int n, float& c, float& s) {
   int r = a % n;
@@ -221,8 +250,11 @@ __global__ void __launch_bounds__(kSepThreads, 2)
 #pragma unroll
       for (int b = 0; b < 2; b++) acc1[a][b] = floatx4{0.f, 0.f, 0.f, 0.f};
     float nacc = 0.f;
+    // fragments in (K step, row block) order: four independent accumulator
+    // chains between dependent matrix-core operations
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int qq = 0; qq < 8; qq++) {
+      const int q = (qq & 1) * 4 + (qq >> 1);
       const unsigned tw[4] = {tv[q].x, tv[q].y, tv[q].z, tv[q].w};
       const unsigned aw[4] = {pa[q].x, pa[q].y, pa[q].z, pa[q].w};
       const unsigned bw[4] = {pb[q].x, pb[q].y, pb[q].z, pb[q].w};
@@ -242,11 +274,10 @@ __global__ void __launch_bounds__(kSepThreads, 2)
       acc1[q >> 2][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(y, ey[q & 3][1], acc1[q >> 2][1],
                                                                 0, 0, 0);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) nacc += __shfl_xor(nacc, off, 64);
+    nacc = wave_sum(nacc);
     // stage 2: this lane's accumulators (rows I = 16 s + 4 g + i, column sy)
-    // are its B operand for K step s
-    floatx4 acc2 = floatx4{0.f, 0.f, 0.f, 0.f};
+    // are its B operand for K step s (two chains, added)
+    floatx4 acc2h[2];
 #pragma unroll
     for (int s = 0; s < 2; s++) {
       bf16x8 z;
@@ -255,8 +286,12 @@ __global__ void __launch_bounds__(kSepThreads, 2)
         z[i] = (__bf16)acc1[s][0][i];
         z[4 + i] = (__bf16)acc1[s][1][i];
       }
-      acc2 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ex[s], z, acc2, 0, 0, 0);
+      acc2h[s] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ex[s], z, floatx4{0.f, 0.f, 0.f, 0.f},
+                                                         0, 0, 0);
     }
+    floatx4 acc2;
+#pragma unroll
+    for (int i = 0; i < 4; i++) acc2[i] = acc2h[0][i] + acc2h[1][i];
     // lane (g, n) holds map[sx = 4 g + i][sy = n]; ccp.f:229-241: argmax over
     // the inner 13 x 13, x outer, first maximum: the maximum value across the
     // wave, then the first (sx, sy) holding it from the lanes' ballots
@@ -268,8 +303,7 @@ __global__ void __launch_bounds__(kSepThreads, 2)
       ok[i] = sx >= 1 && sx <= 13 && n >= 1 && n <= 13;
       if (ok[i]) m = fmaxf(m, acc2[i]);
     }
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    m = wave_max(m);
     unsigned long long hit[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) hit[i] = __ballot(ok[i] && acc2[i] == m);
