@@ -58,6 +58,7 @@ SIGNATURES = {
     "fr_cnn_param_count": ([ci, ci, ci, ci, C.POINTER(C.c_longlong)], ci),
     "fr_cnn_init": ([vp, ci, ci, ci, ci, C.c_ulonglong, vp, vp], ci),
     "fr_cnn_forward": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),
+    "fr_cnn_forward_bf16": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),
     "fr_cnn_gradients": ([vp, ci, ci, ci, ci, vp, ci, vp, vp, vp, vp], ci),
     "fr_cnn_adam": ([vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, cf], ci),
     "fr_cnn_propose": ([vp, ci, ci, vp, ci, vp, vp], ci),

@@ -504,6 +504,20 @@ class Context:
                 "fr_cnn_forward")
         return out
 
+    def cnn_forward_bf16(self, shape, wq: torch.Tensor, img: torch.Tensor) -> torch.Tensor:
+        """fr_cnn_forward_bf16: the logits with BF16 activations (C = 64)."""
+        _dev(wq, torch.float32, "wq")
+        _dev(img, torch.float32, "img")
+        S, K = shape[0], shape[3]
+        nb = img.numel() // (S * S)
+        if img.numel() != nb * S * S:
+            raise ValueError("img must be nb x S x S")
+        out = torch.empty((nb, K), dtype=torch.float32, device=img.device)
+        self._sync_stream()
+        N.check(N.lib().fr_cnn_forward_bf16(self.h, *shape, _p(wq), nb, _p(img), _p(out)),
+                "fr_cnn_forward_bf16")
+        return out
+
     def cnn_gradients(self, shape, wq: torch.Tensor, img: torch.Tensor, label: torch.Tensor,
                       grad: torch.Tensor) -> torch.Tensor:
         _dev(wq, torch.float32, "wq")

@@ -718,6 +718,22 @@ int fr_cnn_forward(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq,
   return r;
 }
 
+int fr_cnn_forward_bf16(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
+                        const float* d_img, float* d_logits) {
+  CTX_CHECK(c);
+  FR_CHECK(d_wq && d_img && d_logits && nb >= 0, "null argument");
+  CNN_SHAPE(lo);
+  FR_CHECK(cnn_bf16_ok(lo), "BF16 inference needs C = 64 channels and 2..8 convolutions");
+  if (nb == 0) return 0;
+  int err = 0;
+  void* ws = c->slot(26, cnn_workspace_bf16(lo, nb), &err);
+  if (!ws) return err;
+  int pslot = c->prof_begin(kProfCnn);
+  int r = cnn_forward_bf16(*c, lo, d_wq, nb, d_img, ws, d_logits);
+  c->prof_end(pslot);
+  return r;
+}
+
 int fr_cnn_gradients(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
                      const float* d_img, const int* d_label, float* d_loss, float* d_grad) {
   CTX_CHECK(c);

@@ -30,6 +30,11 @@ size_t cnn_workspace(const CnnLayout& lo, int B, bool backward);
 int cnn_init(Ctx& c, const CnnLayout& lo, unsigned long long seed, float* master, float* wq);
 int cnn_forward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img, float* ws,
                 float* logits);
+// inference with BF16 activations (C = 64): workspace in bytes
+bool cnn_bf16_ok(const CnnLayout& lo);
+size_t cnn_workspace_bf16(const CnnLayout& lo, int B);
+int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
+                     void* ws, float* logits);
 int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
                  const int* label, float* ws, float* loss, float* grad);
 int cnn_adam(Ctx& c, const CnnLayout& lo, float* master, float* wq, const float* grad, float* m,

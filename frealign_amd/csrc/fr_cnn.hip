@@ -15,9 +15,11 @@
 // real space (S-point c2r), then normalised.  Training images are made the
 // same way from central sections of the current reference (AINTERPO3DS),
 // times the CTF of ctf.f, a shift ramp and white noise at a given SNR.
+#include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 
 #include "fr_cnn.h"
 #include "fr_ctf.h"
@@ -984,6 +986,187 @@ int orient_within(Ctx& c, int na, const float* dma, int nb, const float* dmb, in
                      dmb, nsym, sym, tmin, mask);
   FR_HIP(hipGetLastError());
   return 0;
+}
+
+
+// ------------------------------------------------------------------------
+// Inference with BF16 activations (the proposal's forward).  The weights are
+// the same BF16 values; activations are stored as BF16, channels innermost
+// ([B][H][W][C]), and the convolutions after the first run on the BF16
+// matrix cores with FP32 accumulation.  K is ordered (tap, channel) so that
+// a fragment's 8 consecutive K values are 8 channels of one input pixel: one
+// 16-byte load.  Bias and ReLU are applied in FP32, then rounded to BF16.
+// The fully connected layer is the FP32 GEMM of the training forward.
+namespace {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
+constexpr int kInferC = 64;  // output channels per tile (the whole layer)
+
+struct WOff {
+  long long o[8];
+};
+
+// weights of convolutions 1..L-1 as BF16 [co][tap][ci]
+__global__ void __launch_bounds__(kT)
+    k_cnn_wbf16(int C, int L, const float* __restrict__ wq, WOff woff, __bf16* wb) {
+  const long long per = (long long)C * 9 * C;
+  const long long gid = blockIdx.x * (long long)kT + threadIdx.x;
+  if (gid >= per * (L - 1)) return;
+  const int l = 1 + (int)(gid / per);
+  const long long e = gid % per;
+  const int ci = (int)(e % C), t = (int)((e / C) % 9), co = (int)(e / (9LL * C));
+  wb[gid] = (__bf16)wq[woff.o[l] + ((long long)co * C + ci) * 9 + t];
+}
+
+// convolution 0 (one input channel): one thread per output pixel, all C
+// channels, FP32 arithmetic in the im2col order of the training forward
+__global__ void __launch_bounds__(kT)
+    k_cnn_conv0_bf16(int B, int H, int C, const float* __restrict__ img,
+                     const float* __restrict__ w0, const float* __restrict__ b0, __bf16* out) {
+  __shared__ float ws[kInferC * 9 + kInferC];
+  for (int e = threadIdx.x; e < C * 9; e += kT) ws[e] = w0[e];
+  for (int e = threadIdx.x; e < C; e += kT) ws[C * 9 + e] = b0[e];
+  __syncthreads();
+  const int Ho = H / 2;
+  const long long pos = blockIdx.x * (long long)kT + threadIdx.x;
+  if (pos >= (long long)B * Ho * Ho) return;
+  const int b = (int)(pos / (Ho * Ho)), r = (int)(pos % (Ho * Ho)), oy = r / Ho, ox = r % Ho;
+  float x[9];
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int y = 2 * oy + t / 3 - 1, xx = 2 * ox + t % 3 - 1;
+    x[t] = (y >= 0 && y < H && xx >= 0 && xx < H) ? img[((long long)b * H + y) * H + xx] : 0.f;
+  }
+  uint4* dst = reinterpret_cast<uint4*>(out + pos * C);
+  for (int c0 = 0; c0 < C; c0 += 8) {
+    bf16x8_t v;
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+      float z = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; t++) z += ws[(c0 + j) * 9 + t] * x[t];
+      z += ws[C * 9 + c0 + j];
+      v[j] = (__bf16)(z > 0.f ? z : 0.f);
+    }
+    dst[c0 / 8] = *reinterpret_cast<uint4*>(&v);
+  }
+}
+
+// convolution l >= 1 with C = 64: a workgroup computes all 64 output
+// channels of 64 output pixels; wave w takes pixels 16 w .. 16 w + 15
+__global__ void __launch_bounds__(kT)
+    k_cnn_conv_bf16(int B, int H, const __bf16* __restrict__ in, const __bf16* __restrict__ wb,
+                    const float* __restrict__ bias, __bf16* out) {
+  constexpr int C = kInferC;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, n = lane & 15;
+  const int Ho = H / 2;
+  const long long npos = (long long)B * Ho * Ho;
+  const long long pos = blockIdx.x * 64LL + wave * 16 + n;
+  const bool live = pos < npos;
+  int b = 0, oy = 0, ox = 0;
+  if (live) {
+    b = (int)(pos / (Ho * Ho));
+    const int r = (int)(pos % (Ho * Ho));
+    oy = r / Ho;
+    ox = r % Ho;
+  }
+  fx4 acc[4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) acc[rb] = fx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+  for (int s = 0; s < 18; s++) {
+    const int tap = s >> 1, ci0 = (s & 1) * 32 + 8 * g;
+    const int y = 2 * oy + tap / 3 - 1, x = 2 * ox + tap % 3 - 1;
+    uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+    if (live && y >= 0 && y < H && x >= 0 && x < H)
+      bv = *reinterpret_cast<const uint4*>(in + (((long long)b * H + y) * H + x) * C + ci0);
+    const bf16x8_t bf = *reinterpret_cast<bf16x8_t*>(&bv);
+#pragma unroll
+    for (int rb = 0; rb < 4; rb++) {
+      const int co = rb * 16 + n;
+      const uint4 av = *reinterpret_cast<const uint4*>(wb + ((long long)co * 9 + tap) * C + ci0);
+      acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(&av),
+                                                        bf, acc[rb], 0, 0, 0);
+    }
+  }
+  // lane holds output channels rb*16 + 4g + i of pixel n
+  if (!live) return;
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++) {
+    const int co = rb * 16 + 4 * g;
+    __bf16 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const float z = acc[rb][i] + bias[co + i];
+      v[i] = (__bf16)(z > 0.f ? z : 0.f);
+    }
+    *reinterpret_cast<uint2*>(out + pos * C + co) = *reinterpret_cast<uint2*>(v);
+  }
+}
+
+// last activation [B][P][C] (BF16) -> X[F][B] (FP32), f = c P + p
+__global__ void __launch_bounds__(kT)
+    k_cnn_flatten_bf16(int C, int B, int P, const __bf16* __restrict__ in, float* X) {
+  const long long gid = blockIdx.x * (long long)kT + threadIdx.x;
+  if (gid >= (long long)C * P * B) return;
+  const int b = (int)(gid % B);
+  const long long f = gid / B;
+  const int c = (int)(f / P), p = (int)(f % P);
+  X[gid] = (float)in[((long long)b * P + p) * C + c];
+}
+
+}  // namespace
+
+bool cnn_bf16_ok(const CnnLayout& lo) { return lo.C == kInferC && lo.L >= 2 && lo.L <= 8; }
+
+size_t cnn_workspace_bf16(const CnnLayout& lo, int B) {
+  const size_t act = (size_t)B * lo.hout(0) * lo.hout(0) * lo.C;  // largest activation
+  const size_t wb = (size_t)(lo.L - 1) * lo.C * 9 * lo.C;
+  // bytes: two BF16 activation buffers, BF16 weights, the FP32 flatten, offsets
+  return 2 * act * 2 + wb * 2 + (size_t)lo.F() * B * 4 + 256;
+}
+
+int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
+                     void* ws, float* logits) {
+  FR_CHECK(cnn_bf16_ok(lo), "BF16 inference needs C = %d channels and 2..8 convolutions",
+           kInferC);
+  unsigned char* p = reinterpret_cast<unsigned char*>(ws);
+  const size_t act = (size_t)B * lo.hout(0) * lo.hout(0) * lo.C;
+  __bf16* a0 = reinterpret_cast<__bf16*>(p);
+  __bf16* a1 = a0 + act;
+  __bf16* wb = a1 + act;
+  const size_t nwb = (size_t)(lo.L - 1) * lo.C * 9 * lo.C;
+  float* X = reinterpret_cast<float*>(p + (2 * act + nwb) * 2);
+  X = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(X) + 15) & ~(uintptr_t)15);
+  WOff woff;
+  for (int l = 0; l < 8; l++) woff.o[l] = lo.w_off[l];
+  hipLaunchKernelGGL(k_cnn_wbf16, dim3(nblk((long long)nwb)), dim3(kT), 0, c.stream, lo.C, lo.L,
+                     wq, woff, wb);
+  FR_HIP(hipGetLastError());
+  const int H0 = lo.hin(0), Ho0 = lo.hout(0);
+  hipLaunchKernelGGL(k_cnn_conv0_bf16, dim3(nblk((long long)B * Ho0 * Ho0)), dim3(kT), 0,
+                     c.stream, B, H0, lo.C, img, wq + lo.w_off[0], wq + lo.b_off[0], a0);
+  FR_HIP(hipGetLastError());
+  __bf16* cur = a0;
+  __bf16* nxt = a1;
+  for (int l = 1; l < lo.L; l++) {
+    const int H = lo.hin(l), Ho = lo.hout(l);
+    const long long npos = (long long)B * Ho * Ho;
+    hipLaunchKernelGGL(k_cnn_conv_bf16, dim3((unsigned)((npos + 63) / 64)), dim3(kT), 0, c.stream,
+                       B, H, cur, wb + (size_t)(l - 1) * lo.C * 9 * lo.C, wq + lo.b_off[l], nxt);
+    FR_HIP(hipGetLastError());
+    __bf16* t = cur;
+    cur = nxt;
+    nxt = t;
+  }
+  const int P = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
+  hipLaunchKernelGGL(k_cnn_flatten_bf16, dim3(nblk((long long)lo.C * P * B)), dim3(kT), 0,
+                     c.stream, lo.C, B, P, cur, X);
+  FR_HIP(hipGetLastError());
+  // logits[b][k] = sum_f Wfc[k][f] X[f][b] + bfc[k]
+  return gemm(c, lo.K, B, lo.F(), wq + lo.fw_off, lo.F(), 1, X, B, 1, logits, 1, lo.K,
+              wq + lo.fb_off, false);
 }
 
 }  // namespace fr

@@ -103,6 +103,13 @@ class Network:
     def forward(self, img: torch.Tensor) -> torch.Tensor:
         return self.ctx.cnn_forward(self.shape, self.wq, img)
 
+    def infer(self, img: torch.Tensor) -> torch.Tensor:
+        """Logits for proposing: BF16 activations where the shape allows (64
+        channels), else the FP32 forward."""
+        if self.shape[1] == 64 and self.shape[2] >= 2:
+            return self.ctx.cnn_forward_bf16(self.shape, self.wq, img)
+        return self.forward(img)
+
     def train_step(self, img: torch.Tensor, label: torch.Tensor, lr: float) -> torch.Tensor:
         loss = self.ctx.cnn_gradients(self.shape, self.wq, img, label, self.grad)
         self.step += 1
@@ -253,7 +260,7 @@ class LearnedProposal:
         for b0 in range(0, n, 256):
             m = min(256, n - b0)
             img, label, _ = self.make_batch(rng, m, optics, ri, seed * 7 + b0, snr_range)
-            top, _ = self.ctx.cnn_propose(self.net.forward(img), self.cfg.topk)
+            top, _ = self.ctx.cnn_propose(self.net.infer(img), self.cfg.topk)
             t = top.cpu().numpy()
             y = label.cpu().numpy()
             hit1 += int((t[:, 0] == y).sum())
@@ -263,7 +270,7 @@ class LearnedProposal:
     # ------------------------------------------------------------ search
     def propose(self, datc: torch.Tensor, ctff: torch.Tensor):
         img = self.ctx.cnn_images(self.search.nsam, self.cfg.S, datc, ctff)
-        return self.ctx.cnn_propose(self.net.forward(img), self.cfg.topk)
+        return self.ctx.cnn_propose(self.net.infer(img), self.cfg.topk)
 
     def launch(self, outc, ctff, datc, fsct, ring, topk: int = 32):
         """Enqueue the learned search of one batch on the current stream, from

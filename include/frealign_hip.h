@@ -297,6 +297,12 @@ int fr_cnn_init(fr_ctx *ctx, int S, int C, int nconv, int K, unsigned long long 
 int fr_cnn_forward(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,
                    const float *d_img, float *d_logits);
 
+/* The same logits with the activations stored as BF16 and the convolutions
+ * after the first on the BF16 matrix cores (FP32 accumulation); C must be
+ * 64.  The proposal's inference; no counterpart in the reference. */
+int fr_cnn_forward_bf16(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,
+                        const float *d_img, float *d_logits);
+
 /* Softmax cross-entropy against d_label (class per image): per-image losses
  * in d_loss[nb] and the gradient of their mean in d_grad (param layout). */
 int fr_cnn_gradients(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,

@@ -5,6 +5,10 @@ import this module; the library never does.  The network has no counterpart
 in Frealign: it is pinned by central finite differences of its own loss
 (tests/test_cnn_oracle.py), and the GPU kernels are checked against it.
 
+forward_bf16_act (activations rounded to BF16, the reference for
+fr_cnn_forward_bf16) is pinned by tests/test_cnn_oracle.py: it equals forward
+where every activation is exact in BF16 and differs from it by rounding only.
+
 Shapes follow include/frealign_hip.h: nconv 3x3 / stride 2 / pad 1
 convolutions with ReLU (1 -> C -> ... -> C channels), then one fully
 connected layer to K logits.  Flat parameters: per convolution W[Co][Ci][3][3]
@@ -79,6 +83,25 @@ def forward(p: np.ndarray, shape, img: np.ndarray):
     Wf = p[fw[0]:fw[1]].reshape(K, F)
     logits = X @ Wf.T + p[fb[0]:fb[1]][None, :]
     return logits, (cache, X)
+
+
+def forward_bf16_act(p: np.ndarray, shape, img: np.ndarray):
+    """Logits (B, K) with every convolution's output (after bias and ReLU)
+    rounded to BF16, as the BF16-activation inference stores them; the
+    arithmetic itself in FP64."""
+    S, C, L, K = shape
+    offs, fw, fb, F, _ = layout(S, C, L, K)
+    p = np.asarray(p, np.float64)
+    x = np.asarray(img, np.float64).reshape(-1, 1, S, S)
+    B = x.shape[0]
+    for (w, b, ci) in offs:
+        W = p[w[0]:w[1]].reshape(C, ci * 9)
+        z = np.einsum("ok,bkn->bon", W, im2col(x)) + p[b[0]:b[1]][None, :, None]
+        a = bf16_round(np.maximum(z, 0.0))
+        Ho = x.shape[2] // 2
+        x = a.reshape(B, C, Ho, Ho)
+    Wf = p[fw[0]:fw[1]].reshape(K, F)
+    return x.reshape(B, F) @ Wf.T + p[fb[0]:fb[1]][None, :]
 
 
 def loss_grad(p: np.ndarray, shape, img: np.ndarray, label: np.ndarray):

@@ -103,3 +103,28 @@ def test_bf16_round():
     v = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, 1.0 + 2 ** -9, -2.5, 0.0], np.float32)
     want = np.array([1.0, 1.0, 1.0 + 2 ** -6, 1.0, -2.5, 0.0], np.float32)
     assert np.array_equal(CO.bf16_round(v), want)
+
+
+def test_forward_bf16_act_rounds_only_the_activations():
+    """With integer weights, biases and images small enough that every
+    activation is an integer below 256 (exact in BF16), the BF16-activation
+    forward is the forward; on random values it differs from it by BF16
+    rounding, not more."""
+    shape = (16, 4, 2, 5)
+    offs, fw, fb, F, n = CO.layout(*shape)
+    rng = np.random.default_rng(3)
+    p = np.zeros(n)
+    for (w, b, ci) in offs:
+        p[w[0]:w[1]] = rng.integers(-1, 2, w[1] - w[0])
+        p[b[0]:b[1]] = rng.integers(0, 2, b[1] - b[0])
+    p[fw[0]:fw[1]] = rng.integers(-2, 3, fw[1] - fw[0])
+    img = rng.integers(0, 3, (3, 16, 16)).astype(np.float64)
+    exact, _ = CO.forward(p, shape, img)
+    act = CO.forward_bf16_act(p, shape, img)
+    assert np.array_equal(exact, act)
+    p2 = rng.standard_normal(n) * 0.3
+    img2 = rng.standard_normal((3, 16, 16))
+    a, _ = CO.forward(p2, shape, img2)
+    b = CO.forward_bf16_act(p2, shape, img2)
+    rel = np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert 1e-6 < rel < 2e-2, rel

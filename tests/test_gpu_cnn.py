@@ -347,3 +347,21 @@ def test_training_gains_over_no_network(ctx, small_search):
     same_l = acc_l
     same_r = acc_r
     assert hit >= 0.8 and acc_l >= acc_ex - 0.1 and same_l > same_r + 0.3
+
+
+@pytest.mark.parametrize("shape,B", [((32, 64, 3, 10), 6), ((128, 64, 5, 424), 12)])
+def test_forward_bf16_vs_oracle(ctx, shape, B):
+    """BF16-activation inference against the oracle that rounds the same
+    activations (FP64 arithmetic), and its top class against the FP32 forward."""
+    S = shape[0]
+    net = proposal.Network(ctx, shape, seed=11, device=DEV)
+    img = torch.from_numpy(rng.standard_normal((B, S, S)).astype(np.float32)).to(DEV)
+    got = ctx.cnn_forward_bf16(shape, net.wq, img).cpu().numpy()
+    ref = CO.forward_bf16_act(net.wq.cpu().numpy(), shape, img.cpu().numpy())
+    e = rel_l2(got, ref)
+    f32 = net.forward(img).cpu().numpy()
+    agree = np.mean(np.argmax(got, 1) == np.argmax(f32, 1))
+    print(f"{shape}: BF16-activation logits rel L2 {e:.1e} vs oracle, "
+          f"{rel_l2(got, f32):.1e} vs FP32; top-1 agreement {agree:.2f}")
+    assert e < 1e-3
+    assert rel_l2(got, f32) < 3e-2
