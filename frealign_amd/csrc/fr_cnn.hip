@@ -19,7 +19,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
-#include <cstdint>
 
 #include "fr_cnn.h"
 #include "fr_ctf.h"
@@ -542,6 +541,44 @@ __global__ void __launch_bounds__(kT)
 }
 
 // (x - mean) / std over each S x S image (pitch S+2 in, dense out)
+// the same with the image held in registers (S^2 <= 16 x 1024): one read
+__global__ void __launch_bounds__(1024)
+    k_cnn_normalise_reg(int S, const float* __restrict__ buf, float* img) {
+  __shared__ float red[16];
+  const int b = blockIdx.x, t = threadIdx.x, n = S * S;
+  const float* x = buf + (size_t)b * S * (S + 2);
+  float v[16];
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int e = t + 1024 * j;
+    v[j] = e < n ? x[(e / S) * (S + 2) + e % S] : 0.f;
+    s += v[j];
+  }
+  auto bsum = [&](float a) {
+    for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+    __syncthreads();
+    if ((t & 63) == 0) red[t >> 6] = a;
+    __syncthreads();
+    float r = 0.f;
+    for (int k = 0; k < 16; k++) r += red[k];
+    return r;
+  };
+  const float mean = bsum(s) / (float)n;
+  float q = 0.f;
+#pragma unroll
+  for (int j = 0; j < 16; j++)
+    if (t + 1024 * j < n) q += (v[j] - mean) * (v[j] - mean);
+  const float var = bsum(q) / (float)n;
+  const float inv = var > 0.f ? 1.0f / sqrtf(var) : 1.0f;
+  float* y = img + (size_t)b * n;
+#pragma unroll
+  for (int j = 0; j < 16; j++) {
+    const int e = t + 1024 * j;
+    if (e < n) y[e] = (v[j] - mean) * inv;
+  }
+}
+
 __global__ void __launch_bounds__(kT)
     k_cnn_normalise(int S, const float* __restrict__ buf, float* img) {
   __shared__ float red[kT / 64];
@@ -950,7 +987,10 @@ int cnn_images(Ctx& c, int nsam, int S, int B, const float* datc, const float* c
                      reinterpret_cast<float2*>(buf));
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, S, B, buf, (size_t)S * (S + 2))) return r;
-  hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
+  if (S * S <= 16 * 1024)
+    hipLaunchKernelGGL(k_cnn_normalise_reg, dim3(B), dim3(1024), 0, c.stream, S, buf, img);
+  else
+    hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -963,7 +1003,10 @@ int cnn_synth(Ctx& c, int nsam, int S, int B, const float* a3df, const float* dm
                      reinterpret_cast<float2*>(buf));
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, S, B, buf, (size_t)S * (S + 2))) return r;
-  hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
+  if (S * S <= 16 * 1024)
+    hipLaunchKernelGGL(k_cnn_normalise_reg, dim3(B), dim3(1024), 0, c.stream, S, buf, img);
+  else
+    hipLaunchKernelGGL(k_cnn_normalise, dim3(B), dim3(kT), 0, c.stream, S, buf, img);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -996,7 +1039,7 @@ int orient_within(Ctx& c, int na, const float* dma, int nb, const float* dmb, in
 // matrix cores with FP32 accumulation.  K is ordered (tap, channel) so that
 // a fragment's 8 consecutive K values are 8 channels of one input pixel: one
 // 16-byte load.  Bias and ReLU are applied in FP32, then rounded to BF16.
-// The fully connected layer is the FP32 GEMM of the training forward.
+// The fully connected layer reads the last BF16 activation directly.
 namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8_t;
@@ -1053,7 +1096,8 @@ __global__ void __launch_bounds__(kT)
 }
 
 // convolution l >= 1 with C = 64: a workgroup computes all 64 output
-// channels of 64 output pixels; wave w takes pixels 16 w .. 16 w + 15
+// channels of 256 output pixels (the weights are read once per 256 pixels);
+// wave w takes pixels 64 w .. 64 w + 63 as four 16-pixel blocks
 __global__ void __launch_bounds__(kT)
     k_cnn_conv_bf16(int B, int H, const __bf16* __restrict__ in, const __bf16* __restrict__ wb,
                     const float* __restrict__ bias, __bf16* out) {
@@ -1062,58 +1106,96 @@ __global__ void __launch_bounds__(kT)
   const int g = lane >> 4, n = lane & 15;
   const int Ho = H / 2;
   const long long npos = (long long)B * Ho * Ho;
-  const long long pos = blockIdx.x * 64LL + wave * 16 + n;
-  const bool live = pos < npos;
-  int b = 0, oy = 0, ox = 0;
-  if (live) {
-    b = (int)(pos / (Ho * Ho));
-    const int r = (int)(pos % (Ho * Ho));
-    oy = r / Ho;
-    ox = r % Ho;
-  }
-  fx4 acc[4];
+  long long pos[4];
+  int pb[4], py[4], px[4];
 #pragma unroll
-  for (int rb = 0; rb < 4; rb++) acc[rb] = fx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
+  for (int nb = 0; nb < 4; nb++) {
+    pos[nb] = blockIdx.x * 256LL + wave * 64 + nb * 16 + n;
+    pb[nb] = py[nb] = px[nb] = -1;
+    if (pos[nb] < npos) {
+      pb[nb] = (int)(pos[nb] / (Ho * Ho));
+      const int r = (int)(pos[nb] % (Ho * Ho));
+      py[nb] = 2 * (r / Ho) - 1;
+      px[nb] = 2 * (r % Ho) - 1;
+    }
+  }
+  fx4 acc[4][4];
+#pragma unroll
+  for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) acc[rb][nb] = fx4{0.f, 0.f, 0.f, 0.f};
   for (int s = 0; s < 18; s++) {
     const int tap = s >> 1, ci0 = (s & 1) * 32 + 8 * g;
-    const int y = 2 * oy + tap / 3 - 1, x = 2 * ox + tap % 3 - 1;
-    uint4 bv = make_uint4(0u, 0u, 0u, 0u);
-    if (live && y >= 0 && y < H && x >= 0 && x < H)
-      bv = *reinterpret_cast<const uint4*>(in + (((long long)b * H + y) * H + x) * C + ci0);
-    const bf16x8_t bf = *reinterpret_cast<bf16x8_t*>(&bv);
+    const int ky = tap / 3, kx = tap % 3;
+    bf16x8_t bf[4];
+#pragma unroll
+    for (int nb = 0; nb < 4; nb++) {
+      const int y = py[nb] + ky, x = px[nb] + kx;
+      uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+      if (pb[nb] >= 0 && y >= 0 && y < H && x >= 0 && x < H)
+        bv = *reinterpret_cast<const uint4*>(in + (((long long)pb[nb] * H + y) * H + x) * C + ci0);
+      bf[nb] = *reinterpret_cast<bf16x8_t*>(&bv);
+    }
 #pragma unroll
     for (int rb = 0; rb < 4; rb++) {
-      const int co = rb * 16 + n;
-      const uint4 av = *reinterpret_cast<const uint4*>(wb + ((long long)co * 9 + tap) * C + ci0);
-      acc[rb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8_t*>(&av),
-                                                        bf, acc[rb], 0, 0, 0);
+      const uint4 av =
+          *reinterpret_cast<const uint4*>(wb + ((long long)(rb * 16 + n) * 9 + tap) * C + ci0);
+      const bf16x8_t af = *reinterpret_cast<const bf16x8_t*>(&av);
+#pragma unroll
+      for (int nb = 0; nb < 4; nb++)
+        acc[rb][nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bf[nb], acc[rb][nb], 0, 0, 0);
     }
   }
-  // lane holds output channels rb*16 + 4g + i of pixel n
-  if (!live) return;
+  // lane holds output channels rb*16 + 4g + i of pixel block nb, pixel n
 #pragma unroll
-  for (int rb = 0; rb < 4; rb++) {
-    const int co = rb * 16 + 4 * g;
-    __bf16 v[4];
+  for (int nb = 0; nb < 4; nb++) {
+    if (pos[nb] >= npos) continue;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const float z = acc[rb][i] + bias[co + i];
-      v[i] = (__bf16)(z > 0.f ? z : 0.f);
+    for (int rb = 0; rb < 4; rb++) {
+      const int co = rb * 16 + 4 * g;
+      __bf16 v[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float z = acc[rb][nb][i] + bias[co + i];
+        v[i] = (__bf16)(z > 0.f ? z : 0.f);
+      }
+      *reinterpret_cast<uint2*>(out + pos[nb] * C + co) = *reinterpret_cast<uint2*>(v);
     }
-    *reinterpret_cast<uint2*>(out + pos * C + co) = *reinterpret_cast<uint2*>(v);
   }
 }
 
-// last activation [B][P][C] (BF16) -> X[F][B] (FP32), f = c P + p
+// logits[b][k] = sum_f Wfc[k][f] X_f(b) + bfc[k] from the last activation
+// [B][P][C] (BF16), f = c P + p; K ordered (p, c) so that a fragment is 8
+// channels of one pixel.  A workgroup: 16 classes x 64 images, wave w the
+// images 16 w .. 16 w + 15.
 __global__ void __launch_bounds__(kT)
-    k_cnn_flatten_bf16(int C, int B, int P, const __bf16* __restrict__ in, float* X) {
-  const long long gid = blockIdx.x * (long long)kT + threadIdx.x;
-  if (gid >= (long long)C * P * B) return;
-  const int b = (int)(gid % B);
-  const long long f = gid / B;
-  const int c = (int)(f / P), p = (int)(f % P);
-  X[gid] = (float)in[((long long)b * P + p) * C + c];
+    k_cnn_fc_bf16(int K, int B, int P, int C, const float* __restrict__ W,
+                  const float* __restrict__ bias, const __bf16* __restrict__ in, float* logits) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int g = lane >> 4, n = lane & 15;
+  const int F = C * P;
+  const int k0 = blockIdx.x * 16, kr = k0 + n;
+  const int b = blockIdx.y * 64 + wave * 16 + n;
+  fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
+  for (int p = 0; p < P; p++)
+    for (int c0 = 0; c0 < C; c0 += 32) {
+      const int c = c0 + 8 * g;
+      bf16x8_t af, bfv;
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+        af[j] = (__bf16)(kr < K ? W[(long long)kr * F + (long long)(c + j) * P + p] : 0.f);
+      uint4 bv = make_uint4(0u, 0u, 0u, 0u);
+      if (b < B) bv = *reinterpret_cast<const uint4*>(in + ((long long)b * P + p) * C + c);
+      bfv = *reinterpret_cast<bf16x8_t*>(&bv);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af, bfv, acc, 0, 0, 0);
+    }
+  // lane holds classes k0 + 4g + i of image b
+  if (b >= B) return;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int k = k0 + 4 * g + i;
+    if (k < K) logits[(long long)b * K + k] = acc[i] + bias[k];
+  }
 }
 
 }  // namespace
@@ -1123,8 +1205,8 @@ bool cnn_bf16_ok(const CnnLayout& lo) { return lo.C == kInferC && lo.L >= 2 && l
 size_t cnn_workspace_bf16(const CnnLayout& lo, int B) {
   const size_t act = (size_t)B * lo.hout(0) * lo.hout(0) * lo.C;  // largest activation
   const size_t wb = (size_t)(lo.L - 1) * lo.C * 9 * lo.C;
-  // bytes: two BF16 activation buffers, BF16 weights, the FP32 flatten, offsets
-  return 2 * act * 2 + wb * 2 + (size_t)lo.F() * B * 4 + 256;
+  // bytes: two BF16 activation buffers and the BF16 weights
+  return 2 * act * 2 + wb * 2 + 256;
 }
 
 int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
@@ -1137,8 +1219,6 @@ int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const 
   __bf16* a1 = a0 + act;
   __bf16* wb = a1 + act;
   const size_t nwb = (size_t)(lo.L - 1) * lo.C * 9 * lo.C;
-  float* X = reinterpret_cast<float*>(p + (2 * act + nwb) * 2);
-  X = reinterpret_cast<float*>((reinterpret_cast<uintptr_t>(X) + 15) & ~(uintptr_t)15);
   WOff woff;
   for (int l = 0; l < 8; l++) woff.o[l] = lo.w_off[l];
   hipLaunchKernelGGL(k_cnn_wbf16, dim3(nblk((long long)nwb)), dim3(kT), 0, c.stream, lo.C, lo.L,
@@ -1153,7 +1233,7 @@ int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const 
   for (int l = 1; l < lo.L; l++) {
     const int H = lo.hin(l), Ho = lo.hout(l);
     const long long npos = (long long)B * Ho * Ho;
-    hipLaunchKernelGGL(k_cnn_conv_bf16, dim3((unsigned)((npos + 63) / 64)), dim3(kT), 0, c.stream,
+    hipLaunchKernelGGL(k_cnn_conv_bf16, dim3((unsigned)((npos + 255) / 256)), dim3(kT), 0, c.stream,
                        B, H, cur, wb + (size_t)(l - 1) * lo.C * 9 * lo.C, wq + lo.b_off[l], nxt);
     FR_HIP(hipGetLastError());
     __bf16* t = cur;
@@ -1161,12 +1241,10 @@ int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const 
     nxt = t;
   }
   const int P = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
-  hipLaunchKernelGGL(k_cnn_flatten_bf16, dim3(nblk((long long)lo.C * P * B)), dim3(kT), 0,
-                     c.stream, lo.C, B, P, cur, X);
+  hipLaunchKernelGGL(k_cnn_fc_bf16, dim3((lo.K + 15) / 16, (B + 63) / 64), dim3(kT), 0, c.stream,
+                     lo.K, B, P, lo.C, wq + lo.fw_off, wq + lo.fb_off, cur, logits);
   FR_HIP(hipGetLastError());
-  // logits[b][k] = sum_f Wfc[k][f] X[f][b] + bfc[k]
-  return gemm(c, lo.K, B, lo.F(), wq + lo.fw_off, lo.F(), 1, X, B, 1, logits, 1, lo.K,
-              wq + lo.fb_off, false);
+  return 0;
 }
 
 }  // namespace fr

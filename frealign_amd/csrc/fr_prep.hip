@@ -42,7 +42,10 @@ __global__ void __launch_bounds__(kThreads)
   float2 s = spec[(size_t)b * ic + id];
   cf v = cmul(cf{s.x, s.y}, cf{cs, sn});
   outc[(size_t)b * ic + id] = make_float2(v.re, v.im);
-  datc[(size_t)b * ic + id] = make_float2(v.re, v.im);
+  // DATC takes the (-1)^(l+m) origin shift of ctfapply.f:144-158 here (it
+  // is not transformed again); OUTC takes it in PWEIGHT's weighting pass
+  const float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
+  datc[(size_t)b * ic + id] = make_float2(v.re * sg, v.im * sg);
   ctff[(size_t)b * 2 * ic + id] = make_float2(cr.re, cr.im);
   ctff[(size_t)b * 2 * ic + ic + id] = make_float2(cl.re, cl.im);
 }
@@ -129,7 +132,8 @@ __global__ void __launch_bounds__(kMaskThreads)
 
 namespace {
 
-// ctfapply.f:144-158: origin shift of DATC and OUTC, then PWEIGHT on OUTC.
+// ctfapply.f:144-158: PWEIGHT on OUTC, with OUTC's origin shift applied in
+// the weighting pass (DATC's is applied in k_ctf_shift).
 // One workgroup per particle.  Shell bins are 1-based as in pweight.f.  The
 // shell sums run in a fixed order: wave w takes the 64-point runs w, w + 16,
 // ... of the half-plane (row-major), each run reduces its points per shell
@@ -144,14 +148,13 @@ size_t pweight_lds(int nsam) { return sizeof(float) * (kPwWaves + 1) * (size_t)(
 
 __global__ void __launch_bounds__(kMaskThreads)
     k_origin_pweight(int nsam, const float* __restrict__ pssnr, int ictf,
-                     const float2* __restrict__ ctff, float2* outc, float2* datc) {
+                     const float2* __restrict__ ctff, float2* outc) {
   extern __shared__ float pw_lds[];
   const int jc = nsam / 2 + 1;
   const int ic = nsam * jc;
   const int nbin = 2 * jc + 2;
   float* rbuf = pw_lds + kPwWaves * nbin;
   float2* oc = outc + blockIdx.x * (size_t)ic;
-  float2* dc = datc + blockIdx.x * (size_t)ic;
   const float2* cf_ = ctff + blockIdx.x * 2 * (size_t)ic;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float* wb = pw_lds + wave * nbin;
@@ -159,8 +162,9 @@ __global__ void __launch_bounds__(kMaskThreads)
   // segment keys: row * (jc + 3) + shell, shell jc + 2 marking points outside
   // the disc (they close each row), so keys never decrease within a run
   const int kstride = jc + 3;
-  // origin shift, and the shell sums of pweight.f:31-42 over J=-JC+1..JC-1
-  // (the Nyquist row is addressed by both J=-(JC-1) and J=+(JC-1))
+  // the shell sums of pweight.f:31-42 over J=-JC+1..JC-1 (the Nyquist row
+  // is addressed by both J=-(JC-1) and J=+(JC-1)); |X| is the same before
+  // and after the origin shift, which the weighting pass applies
   for (int r0 = wave * 64; r0 < ic; r0 += kMaskThreads) {
     const int e = r0 + lane;
     int key = 0x7fffffff;  // past the end
@@ -168,12 +172,7 @@ __global__ void __launch_bounds__(kMaskThreads)
     if (e < ic) {
       const int m = e / jc, l = e - m * jc;
       int mm = m >= jc ? m - nsam : m;
-      float sg = ((l + mm) & 1) ? -1.0f : 1.0f;
-      float2 o = oc[e], d = dc[e];
-      o.x *= sg; o.y *= sg;
-      d.x *= sg; d.y *= sg;
-      oc[e] = o;
-      dc[e] = d;
+      const float2 o = oc[e];
       int itemp = l * l + mm * mm;
       int ibin = jc + 2;
       if (itemp < jc * jc) {
@@ -229,6 +228,9 @@ __global__ void __launch_bounds__(kMaskThreads)
     int mm = m >= jc ? m - nsam : m;
     int itemp = l * l + mm * mm;
     float2 o = oc[e];
+    const float sg = ((l + mm) & 1) ? -1.0f : 1.0f;  // origin shift, ctfapply.f:144-158
+    o.x *= sg;
+    o.y *= sg;
     if (itemp < jc * jc) {
       int ibin = (int)(sqrtf((float)itemp) + 0.5f) + 1;
       float snr;
@@ -279,8 +281,7 @@ int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const 
   FR_HIP(hipFuncSetAttribute((const void*)k_origin_pweight,
                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   hipLaunchKernelGGL(k_origin_pweight, dim3(nb), dim3(kMaskThreads), lds, c.stream, nsam, pssnr, g.ictf,
-                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<float2*>(outc),
-                     reinterpret_cast<float2*>(datc));
+                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<float2*>(outc));
   FR_HIP(hipGetLastError());
   return 0;
 }
