@@ -247,6 +247,188 @@ int row_seqs(int n) {
   return s < 1 ? 1 : s;
 }
 
+// ---- the same three kernels with the length known at compile time (the
+// box sizes of the bench and the proposal: 64, 128, 256): every index
+// division and modulus becomes a shift or a constant multiply.
+// Radix of pass i in make_plan's order (0 past the last pass).
+constexpr int radix_at(int n, int i) {
+  int m = n, k = 0;
+  while (m % 4 == 0) { if (k == i) return 4; m /= 4; k++; }
+  while (m % 2 == 0) { if (k == i) return 2; m /= 2; k++; }
+  while (m % 3 == 0) { if (k == i) return 3; m /= 3; k++; }
+  while (m % 5 == 0) { if (k == i) return 5; m /= 5; k++; }
+  return 0;
+}
+
+template <int R, int N, int P, int NSEQ, int LD, int SIGN>
+__device__ inline void stockham_pass_t(const float2* in, float2* out,
+                                       const float2* __restrict__ tw) {
+  constexpr int T = N / R;
+  constexpr int stride = N / (P * R);
+  for (int b = threadIdx.x; b < NSEQ * T; b += kThreads) {
+    const int s = b / T, i = b - s * T;
+    const int k = i % P;
+    const float2* x = in + s * LD;
+    float2 u[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) u[r] = x[i + r * T];
+#pragma unroll
+    for (int r = 1; r < R; r++) {
+      float2 w = tw[(r * k * stride) % N];
+      if (SIGN > 0) w.y = -w.y;
+      u[r] = mul2(u[r], w);
+    }
+    butterfly<R>(u, SIGN);
+    float2* y = out + s * LD;
+    const int base = (i - k) * R + k;
+#pragma unroll
+    for (int q = 0; q < R; q++) y[base + q * P] = u[q];
+  }
+}
+
+template <int N, int NSEQ, int LD, int SIGN, int I = 0, int P = 1>
+__device__ inline float2* lds_fft_t(float2* buf, float2* tmp, const float2* __restrict__ tw) {
+  constexpr int R = radix_at(N, I);
+  if constexpr (R == 0) {
+    return buf;
+  } else {
+    stockham_pass_t<R, N, P, NSEQ, LD, SIGN>(buf, tmp, tw);
+    __syncthreads();
+    return lds_fft_t<N, NSEQ, LD, SIGN, I + 1, P * R>(tmp, buf, tw);
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_rows_r2c_t(float* data, size_t img_stride,
+                                                         int npairs, const float2* __restrict__ tw) {
+  constexpr int NSEQ = 2048 / N, LD = N + 1, JC = N / 2 + 1, PITCH = N + 2;
+  extern __shared__ float2 smem[];
+  float2* buf = smem;
+  float2* tmp = smem + NSEQ * LD;
+  const int pair0 = blockIdx.x * NSEQ;
+  for (int e = threadIdx.x; e < NSEQ * N; e += kThreads) {
+    const int s = e / N, x = e - s * N;
+    const int pr = pair0 + s;
+    float2 v = f2(0.f, 0.f);
+    if (pr < npairs) {
+      const int img = (2 * pr) / N, row = (2 * pr) % N;
+      const float* ra = data + img * img_stride + (size_t)row * PITCH;
+      v = f2(ra[x], ra[x + PITCH]);
+    }
+    buf[s * LD + x] = v;
+  }
+  __syncthreads();
+  float2* res = lds_fft_t<N, NSEQ, LD, -1>(buf, tmp, tw);
+  for (int e = threadIdx.x; e < NSEQ * JC; e += kThreads) {
+    const int s = e / JC, k = e - s * JC;
+    const int pr = pair0 + s;
+    if (pr >= npairs) continue;
+    const float2 zk = res[s * LD + k];
+    const float2 zm = res[s * LD + ((N - k) % N)];
+    const float2 xa = f2(0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y));
+    const float2 xb = f2(0.5f * (zk.y + zm.y), -0.5f * (zk.x - zm.x));
+    const int img = (2 * pr) / N, row = (2 * pr) % N;
+    float* ra = data + img * img_stride + (size_t)row * PITCH;
+    ra[2 * k] = xa.x;
+    ra[2 * k + 1] = xa.y;
+    ra[PITCH + 2 * k] = xb.x;
+    ra[PITCH + 2 * k + 1] = xb.y;
+  }
+}
+
+template <int N>
+__global__ void __launch_bounds__(kThreads) k_rows_c2r_t(float* data, size_t img_stride,
+                                                         int npairs, const float2* __restrict__ tw) {
+  constexpr int NSEQ = 2048 / N, LD = N + 1, JC = N / 2 + 1, PITCH = N + 2;
+  extern __shared__ float2 smem[];
+  float2* buf = smem;
+  float2* tmp = smem + NSEQ * LD;
+  const int pair0 = blockIdx.x * NSEQ;
+  for (int e = threadIdx.x; e < NSEQ * JC; e += kThreads) {
+    const int s = e / JC, k = e - s * JC;
+    const int pr = pair0 + s;
+    float2 a = f2(0.f, 0.f), b = f2(0.f, 0.f);
+    if (pr < npairs) {
+      const int img = (2 * pr) / N, row = (2 * pr) % N;
+      const float* ra = data + img * img_stride + (size_t)row * PITCH;
+      a = f2(ra[2 * k], ra[2 * k + 1]);
+      b = f2(ra[PITCH + 2 * k], ra[PITCH + 2 * k + 1]);
+    }
+    if (k == 0 || 2 * k == N) { a.y = 0.f; b.y = 0.f; }
+    buf[s * LD + k] = f2(a.x - b.y, a.y + b.x);
+    if (k > 0 && 2 * k < N) buf[s * LD + (N - k)] = f2(a.x + b.y, -a.y + b.x);
+  }
+  __syncthreads();
+  float2* res = lds_fft_t<N, NSEQ, LD, +1>(buf, tmp, tw);
+  for (int e = threadIdx.x; e < NSEQ * PITCH; e += kThreads) {
+    const int s = e / PITCH, x = e - s * PITCH;
+    const int pr = pair0 + s;
+    if (pr >= npairs) continue;
+    const int img = (2 * pr) / N, row = (2 * pr) % N;
+    float* ra = data + img * img_stride + (size_t)row * PITCH;
+    const float2 z = x < N ? res[s * LD + x] : f2(0.f, 0.f);
+    ra[x] = z.x;
+    ra[PITCH + x] = z.y;
+  }
+}
+
+template <int N, int SIGN>
+__global__ void __launch_bounds__(kThreads) k_cols_t(float* data, size_t img_stride,
+                                                     const float2* __restrict__ tw) {
+  constexpr int JC = N / 2 + 1, LD = N + 1;
+  constexpr int CW = (2048 / N) < JC ? (2048 / N) : JC;
+  extern __shared__ float2 smem[];
+  float2* buf = smem;
+  float2* tmp = smem + CW * LD;
+  const int x0 = blockIdx.x * CW;
+  const int ncol = min(CW, JC - x0);
+  float2* base = reinterpret_cast<float2*>(data + blockIdx.y * img_stride);
+  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
+    const int y = e / CW, c = e - y * CW;
+    float2 v = f2(0.f, 0.f);
+    if (c < ncol) v = base[(size_t)y * JC + x0 + c];
+    buf[c * LD + y] = v;
+  }
+  __syncthreads();
+  float2* res = lds_fft_t<N, CW, LD, SIGN>(buf, tmp, tw);
+  for (int e = threadIdx.x; e < N * CW; e += kThreads) {
+    const int y = e / CW, c = e - y * CW;
+    if (c < ncol) base[(size_t)y * JC + x0 + c] = res[c * LD + y];
+  }
+}
+
+template <int N>
+int rows_pass_t(Ctx& c, int nimg, float* data, size_t img_stride, bool fwd, const float2* tw) {
+  constexpr int NSEQ = 2048 / N;
+  const int npairs = nimg * (N / 2);
+  const size_t lds = sizeof(float2) * 2 * NSEQ * (N + 1);
+  dim3 grid((npairs + NSEQ - 1) / NSEQ);
+  if (fwd)
+    hipLaunchKernelGGL(k_rows_r2c_t<N>, grid, dim3(kThreads), lds, c.stream, data, img_stride,
+                       npairs, tw);
+  else
+    hipLaunchKernelGGL(k_rows_c2r_t<N>, grid, dim3(kThreads), lds, c.stream, data, img_stride,
+                       npairs, tw);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+template <int N>
+int cols_pass_t(Ctx& c, int nimg, float* data, size_t img_stride, int sign, const float2* tw) {
+  constexpr int JC = N / 2 + 1;
+  constexpr int CW = (2048 / N) < JC ? (2048 / N) : JC;
+  const size_t lds = sizeof(float2) * 2 * CW * (N + 1);
+  dim3 grid((JC + CW - 1) / CW, nimg);
+  if (sign < 0)
+    hipLaunchKernelGGL((k_cols_t<N, -1>), grid, dim3(kThreads), lds, c.stream, data, img_stride,
+                       tw);
+  else
+    hipLaunchKernelGGL((k_cols_t<N, +1>), grid, dim3(kThreads), lds, c.stream, data, img_stride,
+                       tw);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace
 
 bool fft_supported(int nsam) {
@@ -259,6 +441,12 @@ static int rows_pass(Ctx& c, int nsam, int nimg, float* data, size_t img_stride,
   FR_CHECK(make_plan(nsam, &pl) && nsam % 2 == 0, "FFT length %d not supported", nsam);
   const float2* tw = c.twiddles(nsam);
   FR_CHECK(tw != nullptr, "twiddle table allocation failed");
+  switch (nsam) {
+    case 64: return rows_pass_t<64>(c, nimg, data, img_stride, fwd, tw);
+    case 128: return rows_pass_t<128>(c, nimg, data, img_stride, fwd, tw);
+    case 256: return rows_pass_t<256>(c, nimg, data, img_stride, fwd, tw);
+    default: break;
+  }
   int npairs = nimg * (nsam / 2);
   int nseq = row_seqs(nsam);
   size_t lds = sizeof(float2) * 2 * nseq * (nsam + 1);
@@ -278,6 +466,13 @@ static int cols_pass(Ctx& c, int nsam, int nimg, float* data, size_t img_stride,
   FR_CHECK(make_plan(nsam, &pl), "FFT length %d not supported", nsam);
   const float2* tw = c.twiddles(nsam);
   FR_CHECK(tw != nullptr, "twiddle table allocation failed");
+  FR_CHECK(nimg <= 65535, "too many images in one FFT batch (%d)", nimg);
+  switch (nsam) {
+    case 64: return cols_pass_t<64>(c, nimg, data, img_stride, sign, tw);
+    case 128: return cols_pass_t<128>(c, nimg, data, img_stride, sign, tw);
+    case 256: return cols_pass_t<256>(c, nimg, data, img_stride, sign, tw);
+    default: break;
+  }
   int jc = nsam / 2 + 1;
   int cw = 2048 / nsam;
   if (cw < 1) cw = 1;
