@@ -1164,16 +1164,25 @@ __global__ void __launch_bounds__(kT)
   }
 }
 
+// fully connected weights W[k][c P + p] as BF16 [k][p][c]
+__global__ void __launch_bounds__(kT)
+    k_cnn_fcw_bf16(int K, int P, int C, const float* __restrict__ W, __bf16* wf) {
+  const long long gid = blockIdx.x * (long long)kT + threadIdx.x;
+  if (gid >= (long long)K * P * C) return;
+  const int c = (int)(gid % C), p = (int)((gid / C) % P);
+  const long long k = gid / ((long long)C * P);
+  wf[gid] = (__bf16)W[k * C * P + (long long)c * P + p];
+}
+
 // logits[b][k] = sum_f Wfc[k][f] X_f(b) + bfc[k] from the last activation
 // [B][P][C] (BF16), f = c P + p; K ordered (p, c) so that a fragment is 8
-// channels of one pixel.  A workgroup: 16 classes x 64 images, wave w the
-// images 16 w .. 16 w + 15.
+// channels of one pixel (weights from k_cnn_fcw_bf16).  A workgroup: 16
+// classes x 64 images, wave w the images 16 w .. 16 w + 15.
 __global__ void __launch_bounds__(kT)
-    k_cnn_fc_bf16(int K, int B, int P, int C, const float* __restrict__ W,
+    k_cnn_fc_bf16(int K, int B, int P, int C, const __bf16* __restrict__ wf,
                   const float* __restrict__ bias, const __bf16* __restrict__ in, float* logits) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
-  const int F = C * P;
   const int k0 = blockIdx.x * 16, kr = k0 + n;
   const int b = blockIdx.y * 64 + wave * 16 + n;
   fx4 acc = fx4{0.f, 0.f, 0.f, 0.f};
@@ -1181,9 +1190,9 @@ __global__ void __launch_bounds__(kT)
     for (int c0 = 0; c0 < C; c0 += 32) {
       const int c = c0 + 8 * g;
       bf16x8_t af, bfv;
-#pragma unroll
-      for (int j = 0; j < 8; j++)
-        af[j] = (__bf16)(kr < K ? W[(long long)kr * F + (long long)(c + j) * P + p] : 0.f);
+      uint4 av = make_uint4(0u, 0u, 0u, 0u);
+      if (kr < K) av = *reinterpret_cast<const uint4*>(wf + ((long long)kr * P + p) * C + c);
+      af = *reinterpret_cast<bf16x8_t*>(&av);
       uint4 bv = make_uint4(0u, 0u, 0u, 0u);
       if (b < B) bv = *reinterpret_cast<const uint4*>(in + ((long long)b * P + p) * C + c);
       bfv = *reinterpret_cast<bf16x8_t*>(&bv);
@@ -1205,8 +1214,9 @@ bool cnn_bf16_ok(const CnnLayout& lo) { return lo.C == kInferC && lo.L >= 2 && l
 size_t cnn_workspace_bf16(const CnnLayout& lo, int B) {
   const size_t act = (size_t)B * lo.hout(0) * lo.hout(0) * lo.C;  // largest activation
   const size_t wb = (size_t)(lo.L - 1) * lo.C * 9 * lo.C;
-  // bytes: two BF16 activation buffers and the BF16 weights
-  return 2 * act * 2 + wb * 2 + 256;
+  // bytes: two BF16 activation buffers, the BF16 convolution and fully
+  // connected weights
+  return 2 * act * 2 + wb * 2 + (size_t)lo.K * lo.F() * 2 + 256;
 }
 
 int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
@@ -1241,8 +1251,12 @@ int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const 
     nxt = t;
   }
   const int P = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
+  __bf16* wf = wb + nwb;
+  hipLaunchKernelGGL(k_cnn_fcw_bf16, dim3(nblk((long long)lo.K * lo.F())), dim3(kT), 0, c.stream,
+                     lo.K, P, lo.C, wq + lo.fw_off, wf);
+  FR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_cnn_fc_bf16, dim3((lo.K + 15) / 16, (B + 63) / 64), dim3(kT), 0, c.stream,
-                     lo.K, B, P, lo.C, wq + lo.fw_off, wq + lo.fb_off, cur, logits);
+                     lo.K, B, P, lo.C, wf, wq + lo.fb_off, cur, logits);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -150,24 +150,24 @@ __global__ void __launch_bounds__(kFill)
 }
 
 // Winner per particle: the first maximum of the rescored CC3M over its valid
-// candidates (scores of empty candidates become -inf).
-__global__ void __launch_bounds__(kFill)
+// candidates (scores of empty candidates become -inf).  One wave per
+// particle, a lane per candidate (k <= 64): the wave maximum, then the lowest
+// lane holding it (lane 0 when every score is -inf).
+__global__ void __launch_bounds__(256)
     k_best_cand(int np, int k, const int* __restrict__ top_o, float* score, int* best) {
-  const int p = blockIdx.x * kFill + threadIdx.x;
-  if (p >= np) return;
+  const int p = blockIdx.x * 4 + (threadIdx.x >> 6), q = threadIdx.x & 63;
+  if (p >= np) return;  // uniform across the wave
   const float ninf = -__builtin_inff();
-  int b = 0;
-  float bv = ninf;
-  for (int q = 0; q < k; q++) {
+  float v = ninf;
+  if (q < k) {
     const size_t i = (size_t)p * k + q;
-    float v = top_o[i] >= 0 ? score[i] : ninf;
+    v = top_o[i] >= 0 ? score[i] : ninf;
     score[i] = v;
-    if (q == 0 || v > bv) {
-      bv = v;
-      b = q;
-    }
   }
-  best[p] = b;
+  float m = v;
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  const unsigned long long hit = __ballot(q < k && v == m);
+  if (q == 0) best[p] = hit ? __ffsll((unsigned long long)hit) - 1 : 0;
 }
 
 }  // namespace
@@ -203,8 +203,9 @@ int launch_cand_items(Ctx& c, int np, int k, const int* top_o, const int* top_ro
 }
 
 int launch_best_cand(Ctx& c, int np, int k, const int* top_o, float* score, int* best) {
-  hipLaunchKernelGGL(k_best_cand, dim3((np + kFill - 1) / kFill), dim3(kFill), 0, c.stream, np, k,
-                     top_o, score, best);
+  FR_CHECK(k >= 1 && k <= 64, "at most 64 candidates per particle (got %d)", k);
+  hipLaunchKernelGGL(k_best_cand, dim3((np + 3) / 4), dim3(256), 0, c.stream, np, k, top_o, score,
+                     best);
   FR_HIP(hipGetLastError());
   return 0;
 }
