@@ -35,6 +35,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kTile = 8;            // voxels per tile edge
 constexpr int kGatherThreads = 512; // one per voxel of a tile
+static_assert(kTile == 8 && kGatherThreads == 512, "the gather maps 8 waves to 4x4x4 sub-cubes");
 constexpr int kChunk = 4096;        // tasks listed per pass
 constexpr int kReduceThreads = 1024;
 
@@ -256,8 +257,12 @@ __global__ void __launch_bounds__(kGatherThreads)
   int mlo[2], mhi[2], nlo[2], nhi[2];
   const int nmr = logical_ranges(m0, min(m0 + kTile - 1, nsam - 1), nsam, mlo, mhi);
   const int nnr = logical_ranges(n0, min(n0 + kTile - 1, nsam - 1), nsam, nlo, nhi);
-  // this thread's voxel
-  const int l = l0 + (tid % kTile), ms_ = m0 + (tid / kTile) % kTile, ns_ = n0 + tid / (kTile * kTile);
+  // this thread's voxel: wave w takes the 4x4x4 sub-cube w of the tile, so
+  // that a plane leaves whole waves idle (their exec mask empty) rather than
+  // most lanes of every wave
+  const int l = l0 + 4 * (wave & 1) + (lane & 3);
+  const int ms_ = m0 + 4 * ((wave >> 1) & 1) + ((lane >> 2) & 3);
+  const int ns_ = n0 + 4 * (wave >> 2) + (lane >> 4);
   const bool mine = l < jc && ms_ < nsam && ns_ < nsam;
   int mv[2], nv[2];
   const int nm = mine ? logical_axis(ms_, nsam, mv) : 0;
