@@ -243,6 +243,14 @@ module frealign_hip
       integer(c_int) :: fr_cnn_forward
     end function
 
+    function fr_cnn_forward_bf16(ctx, s, c, nconv, k, d_wq, nb, d_img, d_logits) &
+        bind(C, name="fr_cnn_forward_bf16")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_wq, d_img, d_logits
+      integer(c_int), value :: s, c, nconv, k, nb
+      integer(c_int) :: fr_cnn_forward_bf16
+    end function
+
     function fr_cnn_propose(ctx, k, nb, d_logits, topk, d_top, d_prob) &
         bind(C, name="fr_cnn_propose")
       import :: c_int, c_ptr
