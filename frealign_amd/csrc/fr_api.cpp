@@ -609,19 +609,14 @@ static int sep_terms(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, in
                      float** sum2) {
   FR_CHECK(maxr2 <= kSepMaxR2, "the separable search holds MAXR2 <= %d (got %d)", kSepMaxR2,
            maxr2);
-  const int half = c->kpad / 2;
   const size_t per = sizeof(unsigned) * 2048 * (size_t)np;
   int err = 0;
-  float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
-  if (!ap) return err;
-  float* c2 = (float*)c->slot(4, sizeof(float) * (size_t)np * half, &err);
-  if (!c2) return err;
   if (!(*sum2 = (float*)c->slot(5, sizeof(float) * (size_t)np, &err))) return err;
   if (!(*xa = c->slot(23, per, &err))) return err;
   if (!(*xb = c->slot(24, per, &err))) return err;
   if (!(*c2s = (float*)c->slot(25, per, &err))) return err;
   if (int r = launch_search_feats(*c, nsam, np, c->kc, c->kpad, maxr2 * maxr2, rbfact, d_outc,
-                                  d_ctff, c->d_kidx, ap, c2, *sum2))
+                                  d_ctff, c->d_kidx, nullptr, nullptr, *sum2))
     return r;
   return launch_sep_feats(*c, nsam, np, maxr1 * maxr1 < 1 ? 1 : maxr1 * maxr1, maxr2 * maxr2,
                           rbfact, d_outc, d_ctff, *xa, *xb, *c2s);
@@ -655,7 +650,7 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
                                   nullptr, d_bank, xa, xb, c2s, sum2, score, brow))
       return r;
     c->prof_end(pslot);
-    return launch_topk(*c, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
+    return launch_topk(*c, norient, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
                        d_top_o, d_top_row, d_top_score);
   }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
@@ -679,7 +674,7 @@ int fr_search_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int
                                  ap, c2, nrm, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
-  return launch_topk(*c, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
+  return launch_topk(*c, norient, np, norient, group, nullptr, nullptr, nullptr, 0, topk, score, brow,
                      d_top_o, d_top_row, d_top_score);
 }
 
@@ -838,7 +833,7 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
                                   d_cand, d_bank, xa, xb, c2s, sum2, score, brow))
       return r;
     c->prof_end(pslot);
-    return launch_topk(*c, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
+    return launch_topk(*c, norient, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
                        d_top_o, d_top_row, d_top_score);
   }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
@@ -859,7 +854,7 @@ int fr_search_list_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact
                                  c2, nullptr, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
-  return launch_topk(*c, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
+  return launch_topk(*c, norient, np, ncand, group, nullptr, nullptr, d_cand, 0, topk, score, brow,
                      d_top_o, d_top_row, d_top_score);
 }
 
@@ -921,7 +916,7 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
                                   d_bank, xa, xb, c2s, sum2, score, brow))
       return r;
     c->prof_end(pslot);
-    return launch_topk(*c, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
+    return launch_topk(*c, norient, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
                        d_top_row, d_top_score);
   }
   float* ap = (float*)c->slot(3, sizeof(float) * 2 * (size_t)np * half, &err);
@@ -947,7 +942,7 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
                                         ap, c2, sum2, c->d_shx, c->d_shy, score, brow))
     return r;
   c->prof_end(pslot);
-  return launch_topk(*c, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
+  return launch_topk(*c, norient, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,
                      d_top_row, d_top_score);
 }
 

@@ -34,7 +34,7 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
                       const int* toff, int cap, int max_tiles, const int* cand, const void* bank,
                       const void* xa, const void* xb, const float* c2s, const float* sum2,
                       float* score, int* best_row);
-int launch_topk(Ctx& c, int np, int n, int group, const int* loff, const int* grp,
+int launch_topk(Ctx& c, int nbank, int np, int n, int group, const int* loff, const int* grp,
                 const int* cand, int cap, int topk, const float* score, const int* best_row,
                 int* top_o, int* top_row, float* top_s);
 }  // namespace fr

@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(kThreads)
   const size_t ic = (size_t)nsam * jc;
   const float2* sp = outc + (size_t)p * ic;
   const float2* cp = ctff + (size_t)p * 2 * ic;
-  for (int k = threadIdx.x; k < half; k += blockDim.x) {
+  // ap and c2 null: SUM2 only (the separable search forms its own terms)
+  for (int k = threadIdx.x; ap && k < half; k += blockDim.x) {
     float2 a = make_float2(0.f, 0.f);
     float cc = 0.f;
     if (k < kc) {
@@ -564,9 +565,10 @@ __global__ void __launch_bounds__(kGemmThreads)
 // lists: n entries per particle, group `group`; ragged lists (loff non-null):
 // particle p's entries loff[p]..min(loff[p+1], cap)) in groups of grp[p].  Output
 // indices are list positions, mapped to bank orientations through cand when
-// it is non-null.
+// it is non-null; an orientation outside 0..nbank-1 is reported as -1 (an
+// empty column, as its -3e38 score says).
 __global__ void __launch_bounds__(kThreads)
-    k_topk(int n, int group, const int* __restrict__ loff, const int* __restrict__ grp,
+    k_topk(int nbank, int n, int group, const int* __restrict__ loff, const int* __restrict__ grp,
            const int* __restrict__ cand, int cap, int topk, const float* __restrict__ score,
            const int* __restrict__ best_row, int* top_o, int* top_row, float* top_s) {
   constexpr int kK = 32;
@@ -661,8 +663,9 @@ __global__ void __launch_bounds__(kThreads)
     int t = si[q];
     bool ok = t >= 0 && t < len;
     int o = ok ? (cand ? cand[base + t] : t) : -1;
+    if (o >= nbank) o = -1;  // a list entry outside the bank is an empty column
     top_o[(size_t)p * topk + q] = o;
-    top_row[(size_t)p * topk + q] = ok ? best_row[base + t] : -1;
+    top_row[(size_t)p * topk + q] = o >= 0 ? best_row[base + t] : -1;
     top_s[(size_t)p * topk + q] = sv[q];
   }
 }
@@ -794,15 +797,15 @@ int launch_search_gemm_ragged(Ctx& c, int np, int nbank, int kpad, const void* b
   return 0;
 }
 
-int launch_topk(Ctx& c, int np, int n, int group, const int* loff, const int* grp,
+int launch_topk(Ctx& c, int nbank, int np, int n, int group, const int* loff, const int* grp,
                 const int* cand, int cap, int topk, const float* score, const int* best_row,
                 int* top_o, int* top_row, float* top_s) {
   FR_CHECK(topk >= 1 && topk <= 32, "topk must be in 1..32");
   if (!loff)
     FR_CHECK(group >= 1 && n % group == 0, "list length %d is not a multiple of the per-view "
              "group %d", n, group);
-  hipLaunchKernelGGL(k_topk, dim3(np), dim3(kThreads), 0, c.stream, n, group, loff, grp, cand,
-                     cap, topk, score, best_row, top_o, top_row, top_s);
+  hipLaunchKernelGGL(k_topk, dim3(np), dim3(kThreads), 0, c.stream, nbank, n, group, loff, grp,
+                     cand, cap, topk, score, best_row, top_o, top_row, top_s);
   FR_HIP(hipGetLastError());
   return 0;
 }

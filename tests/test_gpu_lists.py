@@ -247,3 +247,28 @@ def test_learned_run_equals_composed_host_path(ctx, setup):
     assert np.array_equal(res.score.view(np.uint32), sc[ar, best].astype(np.float32).view(np.uint32))
     assert np.array_equal(res.shift.view(np.uint32), shifts[ar, best].view(np.uint32))
     assert np.array_equal(res.euler, gs.euler[np.maximum(o_all[ar, best], 0)])
+
+
+@pytest.mark.parametrize("kind", ["annulus", "separable"])
+def test_invalid_entries_score_as_empty_columns(ctx, setup, kind):
+    """List entries outside the bank (-1, >= norient) are empty columns: they
+    never reach the top-K, and the valid entries keep their scores."""
+    nsam, a3df, gs, optics, parts, outc, ctff, datc = setup
+    bank = gs.bank if kind == "annulus" else gs.bank_sep
+    o1, c1 = outc[:3].contiguous(), ctff[:3].contiguous()
+    g = gs.group
+    clean = [np.arange(v * g, (v + 1) * g, dtype=np.int32) for v in (2, 5, 9)]
+    dirty = []
+    for p, rows in enumerate(clean):
+        bad = np.full(g, -1 if p != 1 else gs.norient + 7, np.int32)
+        dirty.append(np.concatenate([bad, rows, bad]))
+    loff, grp, cand = ragged(clean, [g] * 3)
+    loff2, grp2, cand2 = ragged(dirty, [g] * 3)
+    o_c, r_c, s_c = [x.cpu().numpy() for x in ctx.search_ragged(
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, o1, c1, loff, grp, cand, 4)]
+    o_d, r_d, s_d = [x.cpu().numpy() for x in ctx.search_ragged(
+        nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, o1, c1, loff2, grp2, cand2, 4)]
+    # one view per particle is valid: the best entry is that view's, same bits
+    assert np.array_equal(o_d[:, 0], o_c[:, 0]) and np.array_equal(r_d[:, 0], r_c[:, 0])
+    assert np.array_equal(s_d[:, 0].view(np.uint32), s_c[:, 0].view(np.uint32))
+    assert np.all((o_d[:, 1:] == -1) | (s_d[:, 1:] < -1e37))
