@@ -448,7 +448,8 @@ int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const floa
       t.part = b;
       t.half = half[b];
       t.irrec2 = irrec * irrec;
-      t.pad_ = 0;
+      t.ilo = (short)-nsamh;
+      t.ihi = (short)nsamh;
       tasks.push_back(t);
     }
   }

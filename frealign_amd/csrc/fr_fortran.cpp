@@ -3,10 +3,12 @@
 // reference, CHARACTER lengths appended), so that a flang- or gfortran-built
 // LMAIN links against this library in place of the reference's routines:
 //
+//   SUBROUTINE CTFAPPLY(...)       ctfapply.f:8-161  -> ctfapply_
 //   REAL FUNCTION CC3M(...)        cc3m.f:8-169      -> cc3m_
 //   SUBROUTINE PEXTRACT(...)       pextract.f:8-101  -> pextract_
 //   SUBROUTINE PRESB(...)          presb.f:8-160     -> presb_
 //   SUBROUTINE SIGMA(...)          sigma.f:8-221     -> sigma_
+//   SUBROUTINE PINSERT_TRILIN(...) pinsert_trilin.f:8-244 -> pinsert_trilin_
 //
 // Each call copies its host arrays to the device, runs the batched kernels on
 // one item and copies the result back; the batched entry points of
@@ -24,6 +26,7 @@
 
 #include "../../include/frealign_hip.h"
 #include "fr_common.h"
+#include "fr_ctx.h"
 #include "fr_host.h"
 
 namespace {
@@ -45,6 +48,7 @@ struct Dev {
 struct State {
   fr_ctx* ctx = nullptr;
   Dev vol, spec, ctff, fsct, dm, shift, amag, pidx, out;
+  Dev vol2, outc, datc, ppar, pssnr, task, sums, work;
 };
 
 State& st() {
@@ -233,6 +237,125 @@ void sigma_(const int* nsam, const int* irad, const float* amag, const float* a3
   *sigma2n = out[0];
   *sigma2 = out[1];
   *logpc = out[2];
+}
+
+// ctfapply.f:8-161 (MASKCOS, ASYM non-helical, XSTD >= 0).  OUTD and OUTC
+// are one array (the driver passes OUTD for both, frealign_v9.f:679), the
+// padded real image and its half-plane transform; the result is written
+// through OUTD.  CTFF is recomputed on every call: the reference skips the
+// CTF evaluation when DFMID1, DFMID2 and ANGAST equal the previous call's
+// (ctfapply.f:54-58) and then reads the CTFF the caller passes back, which
+// holds the same values.
+void ctfapply_(const int* nsam, const float* spec, const float* shx, const float* shy,
+               const float* cs, const float* wl, const float* wgh1, const float* wgh2,
+               const float* dfmid1, const float* dfmid2, const float* angast,
+               const float* thetatr, float* ctff, float* outd, float* /*outc*/,
+               const float* amagp, const float* rih, const float* halfw, const float* ri2,
+               const float* ri3, const float* /*ri4*/, float* /*datd*/, float* datc,
+               const float* /*b3dv*/, const float* /*phi*/, const float* /*theta*/,
+               const float* /*psi*/, float* /*w*/, const float* xstd, const int* /*ibuf*/,
+               const float* tx, const float* ty, const char* asym, const float* pssnr,
+               float* /*rbuf*/, float* /*pweights*/, const int* ictf,
+               const void* /*fftw_plans*/, size_t /*asym_len*/) {
+  const char* R = "CTFAPPLY";
+  const int n = *nsam;
+  need(n >= 8 && n % 2 == 0 && n <= 1024, R, "NSAM must be even, 8 .. 1024");
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  need(*xstd >= 0.0f, R, "the MASKENV envelope (XSTD < 0) is not supported");
+  need(*amagp != 0.0f, R, "AMAGP must be non-zero");
+  need(*halfw != 0.0f, R, "HALFW must be non-zero");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const int jc = n / 2 + 1;
+  const size_t ic = half_plane(n);
+  void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
+  // PWEIGHT reads PSSNR(1..JC+1) (pweight.f)
+  void* dps = upload(s.pssnr, pssnr, sizeof(float) * (jc + 1), R);
+  const float pp[8] = {*shx, *shy, *dfmid1, *dfmid2, *angast, *amagp, 0.f, 0.f};
+  void* dpp = upload(s.ppar, pp, sizeof pp, R);
+  void* dc = s.ctff.get(sizeof(float) * 4 * ic);
+  void* doc = s.outc.get(sizeof(float) * 2 * ic);
+  void* ddc = s.datc.get(sizeof(float) * 2 * ic);
+  need(dc && doc && ddc, R, "device allocation failed");
+  const fr::PrepGlobals g{*cs, *wl, *wgh1, *wgh2, *thetatr, *rih, *halfw, *ri2, *ri3, *tx, *ty,
+                          *ictf};
+  check(fr::launch_ctfapply(*c, n, 1, (const float*)ds, (const float*)dpp, (const float*)dps, g,
+                            (float*)dc, (float*)doc, (float*)ddc),
+        R);
+  check(fr_synchronize(c), R);
+  download(ctff, dc, sizeof(float) * 4 * ic, R);
+  download(outd, doc, sizeof(float) * 2 * ic, R);
+  download(datc, ddc, sizeof(float) * 2 * ic, R);
+}
+
+// pinsert_trilin.f:8-244 (IRAD = 2 trilinear weights, IEWALD = 0, NNSTAT = 0):
+// the stripe IP of width ST, I = -JC+1+NINT((IP-1)*ST) .. -JC+NINT(IP*ST)
+// (pinsert_trilin.f:44-45), added into B3DF, T3DF, NKSUM and TTD.
+void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, const float* spec,
+                     const float* ctff, const float* occ, float* b3df, float* t3df, double* ttd,
+                     const float* shx, const float* shy, float* /*asum*/, float* /*vsum*/,
+                     float* /*psum*/, float* /*c3df*/, int* /*ksum*/, const float* /*sinclut*/,
+                     const int* /*irada*/, const int* ipad, const float* /*thet*/,
+                     const int* iewald, const int* nnstat, long long* nksum, const int* jc,
+                     const float* dm, const float* tmp, const int* nsamh, const int* ic,
+                     const float* dx, const float* dy, const float* stw /* ST */, const int* ip,
+                     int* /*ns1*/) {
+  const char* R = "PINSERT_TRILIN";
+  const int n = *nsam;
+  need(n >= 8 && n % 2 == 0 && n <= 1024, R, "NSAM must be even, 8 .. 1024");
+  need(*ipad == 1, R, "only IPAD = 1 is supported");
+  need(*iewald == 0, R, "Ewald-sphere insertion (IEWALD != 0) is not supported");
+  need(*nnstat == 0, R, "the NNSTAT statistics are not supported");
+  const int j = n / 2 + 1;
+  need(*jc == j && *nsamh == n / 2 && *ic == n * j, R, "JC, NSAMH or IC inconsistent with NSAM");
+  // pinsert_trilin.f:44-45 (NINT: nearest, halves away from zero)
+  int i1 = -j + 1 + (int)lroundf((float)(*ip - 1) * *stw);
+  int i3 = -j + (int)lroundf((float)*ip * *stw);
+  if (i1 < -(j - 1)) i1 = -(j - 1);
+  if (i3 > j - 1) i3 = j - 1;
+  if (i1 > i3 || *irrec2 < 0) return;  // an empty stripe adds nothing
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t hv = half_volume(n);
+  void* ds = upload(s.spec, spec, sizeof(float) * 2 * half_plane(n), R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * half_plane(n), R);
+  void* db = upload(s.vol, b3df, sizeof(float) * 2 * hv, R);
+  void* dt = upload(s.vol2, t3df, sizeof(float) * hv, R);
+  fr::InsertTask t;
+  memcpy(t.dm, dm, sizeof t.dm);  // DM(1..6)
+  t.shx = *shx + *dx;             // pinsert_trilin.f:55,63: (SHX+DX)*I + (SHY+DY)*J
+  t.shy = *shy + *dy;
+  t.tmp = *tmp;
+  t.occ = *occ;
+  t.part = 0;
+  t.half = 0;
+  t.irrec2 = *irrec2;
+  t.ilo = (short)i1;
+  t.ihi = (short)i3;
+  void* dtask = upload(s.task, &t, sizeof t, R);
+  // NKSUM and TTD of this call (two halves' worth, only the first is used)
+  void* dsum = s.sums.get(2 * sizeof(unsigned long long) + 2 * sizeof(double));
+  need(dsum != nullptr, R, "device allocation failed");
+  need(hipMemset(dsum, 0, 2 * sizeof(unsigned long long) + 2 * sizeof(double)) == hipSuccess, R,
+       "device memset failed");
+  unsigned long long* dk = (unsigned long long*)dsum;
+  double* dd = (double*)(dk + 2);
+  void* work = s.work.get(fr::insert_scratch_bytes(n, 1, 1));
+  need(work != nullptr, R, "device allocation failed");
+  // the second half-map is never addressed (t.half = 0): B3DF/T3DF stand in
+  check(fr::launch_insert(*c, n, 1, 1, (const fr::InsertTask*)dtask, (const float*)ds,
+                          (const float*)dc, (float*)db, (float*)dt, (float*)db, (float*)dt, dk,
+                          dd, work),
+        R);
+  check(fr_synchronize(c), R);
+  download(b3df, db, sizeof(float) * 2 * hv, R);
+  download(t3df, dt, sizeof(float) * hv, R);
+  unsigned long long k = 0;
+  double td = 0.0;
+  download(&k, dk, sizeof k, R);
+  download(&td, dd, sizeof td, R);
+  *nksum += (long long)k;
+  *ttd += td;
 }
 
 }  // extern "C"

@@ -317,8 +317,8 @@ __global__ void __launch_bounds__(kGatherThreads)
             const float cj = g.h2x * fl + g.h2y * fm + g.h2z * fn;
             int ilo = (int)ceilf(ci - g.r1 - 0.02f), ihi = (int)floorf(ci + g.r1 + 0.02f);
             int jlo = (int)ceilf(cj - g.r2 - 0.02f), jhi = (int)floorf(cj + g.r2 + 0.02f);
-            ilo = max(ilo, -(jc - 1));
-            ihi = min(ihi, jc - 1);
+            ilo = max(ilo, (int)t.ilo);  // the task's I stripe, within -(JC-1)..JC-1
+            ihi = min(ihi, (int)t.ihi);
             jlo = max(jlo, -(jc - 1));
             jhi = min(jhi, jc - 1);
             for (int i = ilo; i <= ihi; i++) {

@@ -14,7 +14,7 @@ struct InsertTask {
   int part;      // particle slot in the batch
   int half;      // 0: first half-map (A3DF/S3DF), 1: second (D3DF/V3DF)
   int irrec2;    // INT(NSAM*RREC)**2
-  int pad_;
+  short ilo, ihi;  // I stripe of pinsert_trilin.f:44-45 (whole plane: -NSAM/2, NSAM/2)
 };
 
 // scratch of launch_insert for nb particles and ntask tasks (grouped by particle)
@@ -38,6 +38,10 @@ struct PrepGlobals {
 };
 int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const float* pssnr,
                 const PrepGlobals& g, float* ctff, float* outc, float* datc);
+// ctfapply.f:8-161 alone, from nb spectra (NSAM x JC complex each, as the r2c leaves them)
+int launch_ctfapply(Ctx& c, int nsam, int nb, const float* spec, const float* ppar,
+                    const float* pssnr, const PrepGlobals& g, float* ctff, float* outc,
+                    float* datc);
 
 // Central-section extraction (pextract.f:8-101), one output plane per item.
 int launch_pextract(Ctx& c, int nsam, int n, const float* a3df, const float* dm, const int* pidx,

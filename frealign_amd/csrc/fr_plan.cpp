@@ -139,7 +139,8 @@ int stage(fr_plan* p, const float* ppar, const float* recb, const int* half) {
       t.part = b;
       t.half = half[b];
       t.irrec2 = on == 0.0f ? -1 : irrec * irrec;  // pinsert.f:40: ON = 0 inserts nothing
-      t.pad_ = 0;
+      t.ilo = (short)-nsamh;
+      t.ihi = (short)nsamh;
     }
   }
   return 0;

@@ -258,15 +258,22 @@ __global__ void __launch_bounds__(kMaskThreads)
 int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const float* pssnr,
                 const PrepGlobals& g, float* ctff, float* outc, float* datc) {
   if (nb <= 0) return 0;
+  // lmain.f:459: r2c of the particle image, in place
+  if (int r = fft_r2c_2d(c, nsam, nb, data, (size_t)nsam * (nsam + 2))) return r;
+  return launch_ctfapply(c, nsam, nb, data, ppar, pssnr, g, ctff, outc, datc);
+}
+
+int launch_ctfapply(Ctx& c, int nsam, int nb, const float* spec, const float* ppar,
+                    const float* pssnr, const PrepGlobals& g, float* ctff, float* outc,
+                    float* datc) {
+  if (nb <= 0) return 0;
   FR_CHECK(nsam / 2 + 1 <= 520, "box too large for PWEIGHT bins");
   const int jc = nsam / 2 + 1;
   const size_t ic = (size_t)nsam * jc;
   const size_t img = (size_t)nsam * (nsam + 2);
-  // lmain.f:459: r2c of the particle image, in place
-  if (int r = fft_r2c_2d(c, nsam, nb, data, img)) return r;
   size_t tot = ic * nb;
   hipLaunchKernelGGL(k_ctf_shift, dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads),
-                     0, c.stream, nsam, nb, reinterpret_cast<const float2*>(data), ppar, g,
+                     0, c.stream, nsam, nb, reinterpret_cast<const float2*>(spec), ppar, g,
                      reinterpret_cast<float2*>(ctff), reinterpret_cast<float2*>(outc),
                      reinterpret_cast<float2*>(datc));
   FR_HIP(hipGetLastError());

@@ -356,6 +356,20 @@ int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const floa
  * IEWALD = 0, non-helical ASYM, IBUF = 0.  Other cases and device errors
  * stop the program (message on stdout, exit(1)), as the reference's STOPs. */
 
+/* CTFAPPLY (ctfapply.f:8-161; XSTD >= 0, non-helical): CTFF, DATC and the
+ * masked, SNR-weighted transform in OUTD (OUTD and OUTC are one array,
+ * frealign_v9.f:679).  CTFF is recomputed on every call. */
+void ctfapply_(const int *nsam, const float *spec, const float *shx, const float *shy,
+               const float *cs, const float *wl, const float *wgh1, const float *wgh2,
+               const float *dfmid1, const float *dfmid2, const float *angast,
+               const float *thetatr, float *ctff, float *outd, float *outc,
+               const float *amagp, const float *rih, const float *halfw, const float *ri2,
+               const float *ri3, const float *ri4, float *datd, float *datc, const float *b3dv,
+               const float *phi, const float *theta, const float *psi, float *w,
+               const float *xstd, const int *ibuf, const float *tx, const float *ty,
+               const char *asym, const float *pssnr, float *rbuf, float *pweights,
+               const int *ictf, const void *fftw_plans, size_t asym_len);
+
 /* CC3M (cc3m.f:8-169): REAL FUNCTION */
 float cc3m_(const int *nsam, const int *irad, const float *amag, const float *spec,
             const float *a3df, const int *maxr1, const int *maxr2, const float *phi,
@@ -391,6 +405,18 @@ void sigma_(const int *nsam, const int *irad, const float *amag, const float *a3
             float *sigma2n, float *sigma2, float *logpc, const float *dmask, const char *asym,
             const int *k, const float *sigp, const float *pweights, const float *rclas,
             const void *fftw_plans, const float *sm, size_t asym_len);
+
+/* PINSERT_TRILIN (pinsert_trilin.f:8-244; IEWALD = 0, NNSTAT = 0): the
+ * stripe IP of width ST of the plane DM(1..6), phase shift SHX+DX, SHY+DY,
+ * added into B3DF, T3DF, NKSUM and TTD (volumes in the batched layout). */
+void pinsert_trilin_(const int *nsam, const float *rii, const int *irrec2, const float *spec,
+                     const float *ctff, const float *occ, float *b3df, float *t3df, double *ttd,
+                     const float *shx, const float *shy, float *asum, float *vsum, float *psum,
+                     float *c3df, int *ksum, const float *sinclut, const int *irada,
+                     const int *ipad, const float *thet, const int *iewald, const int *nnstat,
+                     long long *nksum, const int *jc, const float *dm, const float *tmp,
+                     const int *nsamh, const int *ic, const float *dx, const float *dy,
+                     const float *st, const int *ip, int *ns1);
 
 /* ------------------------------------------- batches of uneven box sizes
  * No counterpart in the reference (its particles go one at a time through

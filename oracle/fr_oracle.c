@@ -507,15 +507,18 @@ void P(presb)(int nsam, float amag, const R *spec, const float *a3df, float phi,
 }
 
 /* ------------------------------------------------- pinsert_trilin.f */
+/* pinsert_trilin.f:44-244 for the stripe I = i1..i3 (loops 30 and 31 cover it
+ * in order); the phase uses SHX+DX, SHY+DY (pinsert_trilin.f:55,63) */
 static void pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff, R occ, R *b3df,
                            R *t3df, double *ttd, float shx, float shy, int64_t *nksum, int jc,
-                           const float dm[9], float tmp, int nsamh, size_t ic) {
+                           const float dm[9], float tmp, int nsamh, size_t ic, int i1, int i3,
+                           float dx, float dy) {
   const float ri12 = 0.5f, ri2 = 1.0f; /* IRAD=2: (IRAD-1)/2, IRAD/2 */
-  /* IP=1, IMP=1: I1=-JC+1, I3=JC-1, loops 30 and 31 cover I=I1..I3 in order */
-  for (int i = -jc + 1; i <= jc - 1; i++) {
+  const float sx = shx + dx, sy = shy + dy;
+  for (int i = i1; i <= i3; i++) {
     float dm1 = dm[0] * (float)i, dm2 = dm[1] * (float)i, dm3 = dm[2] * (float)i;
     int iq = i * i;
-    float px = (shx + 0.0f) * (float)i;
+    float px = sx * (float)i;
     for (int j = -jc + 1; j <= jc - 1; j++) {
       float g2 = (float)(iq + j * j);
       if (!(g2 <= (float)irrec2)) continue;
@@ -524,10 +527,10 @@ static void pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff, R
       float z3 = dm3 + dm[5] * (float)j;
 #if OR_FP64
       double wgt = exp((double)tmp * (double)g2) * occ;
-      double phase = (double)shx * i + (double)shy * j;
+      double phase = (double)sx * i + (double)sy * j;
 #else
       float wgt = expf(tmp * g2) * occ;
-      float phase = px + (shy + 0.0f) * (float)j;
+      float phase = px + sy * (float)j;
 #endif
       C pshft = {RCOS(phase), RSIN(phase)};
       C samp, cr, cl;
@@ -612,8 +615,9 @@ void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, 
         if (isym[i] != jsym[i])
           for (int j = 0; j < isym[i]; j++) or_matmul_t(dm, symop + 9 * i, dm);
     }
+    /* IP=1, IMP=1 (pinsert.f): the whole plane, I = -JC+1 .. JC-1, DX = DY = 0 */
     pinsert_trilin(nsam, irrec2, spec, ctff, (R)occ, b3df, t3df, ttd, shx, shy, nksum, jc, dm,
-                   tmp, nsamh, ic);
+                   tmp, nsamh, ic, -jc + 1, jc - 1, 0.0f, 0.0f);
     if (nonsym) break;
     int i = 0, done = 0;
     for (;;) {
@@ -628,6 +632,18 @@ void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, 
     }
     if (done) break;
   }
+}
+
+/* pinsert_trilin.f:8-244 as one call: stripe IP of width ST,
+ * I1 = -JC+1+NINT((IP-1)*ST), I3 = -JC+NINT(IP*ST) (pinsert_trilin.f:44-45) */
+void P(pinsert_trilin)(int nsam, int irrec2, const R *spec, const R *ctff, float occ, R *b3df,
+                       R *t3df, double *ttd, float shx, float shy, int64_t *nksum,
+                       const float *dm, float tmp, float dx, float dy, float st, int ip) {
+  int jc = nsam / 2 + 1;
+  int i1 = -jc + 1 + (int)lroundf((float)(ip - 1) * st);
+  int i3 = -jc + (int)lroundf((float)ip * st);
+  pinsert_trilin(nsam, irrec2, spec, ctff, (R)occ, b3df, t3df, ttd, shx, shy, nksum, jc, dm, tmp,
+                 nsam / 2, (size_t)nsam * jc, i1, i3, dx, dy);
 }
 
 /* ------------------------------------------------------- storeshift.f */

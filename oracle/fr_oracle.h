@@ -122,6 +122,12 @@ void or_irfft2_d(int nsam, double *data);
                   float pbc, float boff, int nsym, const int *jsym,             \
                   const float *symop, int64_t *nksum, float psize, float occ,   \
                   const float *sm, int nonsym);                                 \
+  /* pinsert_trilin.f:8-244 as one call (IRAD=2, IEWALD=0, NNSTAT=0): the      \
+   * stripe IP of width ST, phase shift SHX+DX, SHY+DY, rotation DM(1..6) */    \
+  void P##pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff,    \
+                         float occ, R *b3df, R *t3df, double *ttd, float shx,   \
+                         float shy, int64_t *nksum, const float *dm, float tmp, \
+                         float dx, float dy, float st, int ip);                 \
   /* storeshift.f:43-102 (IFSC=0, NNSTAT=0): half-maps A,D -> B,C */            \
   void P##storeshift(int nsam, double std, double vtd, int iredun,              \
                      const R *a3df, const R *s3df, const R *d3df,               \

@@ -91,6 +91,8 @@ def _declare_oracle(L):
         g("presb").argtypes = [ci, cf, cp, c64p, cf, cf, cf, cf, cf, rp, cp, cf, cf, cf, cf, f32p]
         g("pinsert").argtypes = [ci, cf, cf, cp, cp, cp, rp, pd, cf, cf, cf, cf, cf, cf, cf, cf,
                                  ci, i32p, f32p, C.POINTER(C.c_int64), cf, cf, f32p, ci]
+        g("pinsert_trilin").argtypes = [ci, ci, cp, cp, cf, cp, rp, pd, cf, cf,
+                                        C.POINTER(C.c_int64), f32p, cf, cf, cf, cf, ci]
         g("storeshift").argtypes = [ci, cd, cd, ci, cp, rp, cp, rp, cp, cp]
         g("sheltest").argtypes = [ci, ci, cp, cp, rp, rp, cf, i32p, i32p, f64p, rp]
         g("sheltest").restype = ci

@@ -1,10 +1,14 @@
-"""A Fortran host calling PEXTRACT, CC3M, PRESB and SIGMA with the reference's
-own symbols and argument lists, linked against libfrealign_hip.so in place of
-the reference routines, against the FP32 oracle on the same inputs.
+"""A Fortran host calling CTFAPPLY, PEXTRACT, CC3M, PRESB, SIGMA and
+PINSERT_TRILIN with the reference's own symbols and argument lists, linked
+against libfrealign_hip.so in place of the reference routines, against the
+FP32 oracle on the same inputs.
 
-Bars as the batched parity tests: PEXTRACT 1e-5 relative L2, CC3M 1e-4
-relative, PRESB 2e-5 absolute per shell (unwritten shells untouched), SIGMA
-1e-3 relative (SIGMA2 carries a cancellation) and LOGPC 1e-4."""
+Bars as the batched parity tests: CTFAPPLY 1e-5 relative L2 per output,
+PEXTRACT 1e-5 relative L2, CC3M 1e-4 relative, PRESB 2e-5 absolute per shell
+(unwritten shells untouched), SIGMA 1e-3 relative (SIGMA2 carries a
+cancellation) and LOGPC 1e-4, PINSERT_TRILIN (one stripe, accumulated onto
+non-zero volumes) NKSUM exact and the added volumes 1e-5 relative L2."""
+import ctypes as C
 import math
 import os
 import subprocess
@@ -44,6 +48,24 @@ def test_fortran_percall_vs_oracle(tmp_path):
     rbfact, rclas, sigp, signset = 0.001, 0.3, 0.01, 0.0
     dmask = np.array([nsam / 2 + 2, nsam / 2 - 1, nsam / 2 + 1, 0.35 * nsam], np.float32)
     sm = O.IDENTITY.astype(np.float32)
+    # CTFAPPLY: 300 kV, CS 2.7 mm, 7% amplitude contrast, 1.5 A pixels
+    wl = 12.26 / np.sqrt(300000.0 + 0.9785 * 300000.0 ** 2 / 1e6)
+    cpar = np.array([2.7e7, wl, np.sqrt(1 - 0.07 ** 2), 0.07, 15000.0, 15600.0, 0.4,
+                     wl / (1.5 * nsam), 1.0, rih, halfw, rih * rih, (ri + 3) ** 2], np.float32)
+    pssnr = np.exp(-((np.arange(jc + 1) / nsam) * 4.0) ** 2).astype(np.float32)
+    # PINSERT_TRILIN: stripe 2 of 3 (ST = (2JC-1)/3 as pinsert.f:95), onto
+    # non-zero volumes and counters.  The starting volumes are of the size of
+    # one call's increments: the library adds a voxel's terms first and the
+    # sum once, the reference each term in turn, so on a much larger base the
+    # two differ by the base's rounding, not by the insertion.
+    irrec2, ip = int(0.4 * nsam) ** 2, 2
+    occ, tmp, dx, dy = 0.8, -0.004, 0.02, 0.05
+    stw = np.float32(2 * jc - 1) / np.float32(3)
+    dmi = np.zeros(9, np.float32)
+    L.or_dm(0.7, 1.9, -2.3, 1.0, None, dmi)
+    b3 = (0.002 * vol).astype(np.complex64).reshape(-1)
+    t3 = r.uniform(0, 1, vol.size).astype(np.float32)
+    nk0, ttd0 = 5, 0.25
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     with open(fin, "wb") as f:
         np.array([nsam, maxr1, maxr2], np.int32).tofile(f)
@@ -53,6 +75,15 @@ def test_fortran_percall_vs_oracle(tmp_path):
         sm.tofile(f)
         for a in (vol, spec, datc, ctff, fsct, rbin0):
             a.tofile(f)
+        cpar.tofile(f)
+        pssnr.tofile(f)
+        np.array([irrec2, ip], np.int32).tofile(f)
+        np.array([occ, tmp, dx, dy, stw], np.float32).tofile(f)
+        dmi.tofile(f)
+        b3.tofile(f)
+        t3.tofile(f)
+        np.array([nk0], np.int64).tofile(f)
+        np.array([ttd0], np.float64).tofile(f)
     exe = tmp_path / "fr_percall_check"
     rr = subprocess.run([FLANG, "-O1", os.path.join(ROOT, "frealign_amd", "fortran",
                                                    "fr_percall_check.f90"), LIB,
@@ -73,6 +104,41 @@ def test_fortran_percall_vs_oracle(tmp_path):
     rbin = raw[off:off + 4 * nsam].view(np.float32)
     off += 4 * nsam
     s2n, s2, lpc = raw[off:off + 12].view(np.float32)
+    off += 12
+    nv = vol.size
+    views = {}
+    for name, nbytes, dt in (("ctff", 16 * ic, np.complex64), ("outd", 8 * ic, np.complex64),
+                             ("datc", 8 * ic, np.complex64), ("b3df", 8 * nv, np.complex64),
+                             ("t3df", 4 * nv, np.float32), ("nksum", 8, np.int64),
+                             ("ttd", 8, np.float64)):
+        views[name] = raw[off:off + nbytes].view(dt)
+        off += nbytes
+    assert off == raw.size
+    # CTFAPPLY vs the oracle
+    cf_ = np.zeros(2 * ic, np.complex64)
+    oc = np.zeros(ic, np.complex64)
+    dc = np.zeros(ic, np.complex64)
+    L.or32_ctfapply(nsam, spec.reshape(-1), shx, shy, *[float(x) for x in cpar], pssnr, 1, cf_,
+                    oc, dc)
+    for name, want in (("ctff", cf_), ("outd", oc), ("datc", dc)):
+        e = np.linalg.norm(views[name] - want) / np.linalg.norm(want)
+        print(f"CTFAPPLY {name} rel L2 {e:.1e}")
+        assert e < 1e-5, name
+    # PINSERT_TRILIN vs the stripe oracle, on the same starting volumes
+    bo, to = b3.copy(), t3.copy()
+    ttd = C.c_double(ttd0)
+    nk = C.c_int64(nk0)
+    L.or32_pinsert_trilin(nsam, irrec2, spec.reshape(-1), ctff.reshape(-1), occ, bo, to,
+                          C.byref(ttd), shx, shy, C.byref(nk), dmi, tmp, dx, dy, float(stw), ip)
+    assert nk.value > nk0
+    assert int(views["nksum"][0]) == nk.value
+    db_gpu, db_or = views["b3df"] - b3, bo - b3
+    dt_gpu, dt_or = views["t3df"] - t3, to - t3
+    eb = np.linalg.norm(db_gpu - db_or) / np.linalg.norm(db_or)
+    et = np.linalg.norm(dt_gpu - dt_or) / np.linalg.norm(dt_or)
+    ett = abs(views["ttd"][0] - ttd.value) / abs(ttd.value - ttd0)
+    print(f"PINSERT_TRILIN NKSUM {nk.value - nk0} B3DF {eb:.1e} T3DF {et:.1e} TTD {ett:.1e}")
+    assert eb < 1e-5 and et < 1e-5 and ett < 1e-5
     # oracle
     w = np.zeros(ic, np.complex64)
     L.or32_pextract(nsam, w, vol.reshape(-1), phi, theta, psi, ctff.reshape(-1), sm)

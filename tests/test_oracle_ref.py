@@ -211,6 +211,74 @@ def test_pinsert_integer_angles_exact(ref_lib):
         assert np.array_equal(bo.view(np.int32), br.view(np.int32))
 
 
+def ref_pinsert_trilin(ref_lib, nsam, irrec2, spec, ctff, occ, b3, t3, ttd, shx, shy, nk, dm,
+                       tmp, dx, dy, st, ip):
+    """The reference's own PINSERT_TRILIN (pinsert_trilin.f:8-244), every
+    argument by reference; ASUM..KSUM, SINCLUT, THET and NS1 are unused with
+    IEWALD = 0, NNSTAT = 0."""
+    jc = nsam // 2 + 1
+    f, i = O.byref_f, O.byref_i
+    dummy = np.zeros(4, np.float32)
+    ns1 = np.zeros(4 * nsam, np.int32)
+    ref_lib.pinsert_trilin_(i(nsam), f(1.0), i(irrec2), spec.ctypes, ctff.ctypes, f(occ),
+                            b3.ctypes, t3.ctypes, C.byref(ttd), f(shx), f(shy), dummy.ctypes,
+                            dummy.ctypes, dummy.ctypes, dummy.ctypes, ns1.ctypes, dummy.ctypes,
+                            i(1), i(1), f(0.0), i(0), i(0), C.byref(nk), i(jc), dm.ctypes, f(tmp),
+                            i(nsam // 2), i(nsam * jc), f(dx), f(dy), f(st), i(ip), ns1.ctypes)
+
+
+@pytest.mark.parametrize("nsam,imp", [(16, 1), (32, 3), (64, 4)])
+def test_pinsert_trilin_stripes(ref_lib, nsam, imp):
+    """One PINSERT_TRILIN call per stripe IP = 1..IMP (ST = (2JC-1)/IMP, as
+    pinsert.f:95 sets it) with a DX, DY phase offset: the stripe oracle is bit
+    exact to the reference routine, the stripes partition the plane (their sum
+    equals the whole-plane call), and the last stripe's I range is checked
+    against the reference formula by an empty-stripe case."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    nv = nsam * nsam * jc
+    spec = half_plane(nsam, 41)
+    ctff = np.concatenate([half_plane(nsam, 42), half_plane(nsam, 43)]) * np.complex64(0.5)
+    dm = np.zeros(9, np.float32)
+    L.or_dm(0.7, 1.9, -2.3, 1.0, None, dm)
+    irrec2 = (nsam * 2 // 5) ** 2
+    tmp, occ, shx, shy, dx, dy = -0.004, 0.8, 0.11, -0.07, 0.02, 0.05
+    st = np.float32(2 * jc - 1) / np.float32(imp)
+    tot_b = np.zeros(nv, np.complex64)
+    tot_k = 0
+    for ip in range(1, imp + 1):
+        bo, br = np.zeros(nv, np.complex64), np.zeros(nv, np.complex64)
+        to, tr = np.zeros(nv, np.float32), np.zeros(nv, np.float32)
+        tdo, tdr = C.c_double(0.0), C.c_double(0.0)
+        nko, nkr = C.c_int64(0), C.c_int64(0)
+        L.or32_pinsert_trilin(nsam, irrec2, spec, ctff, occ, bo, to, C.byref(tdo), shx, shy,
+                              C.byref(nko), dm, tmp, dx, dy, float(st), ip)
+        ref_pinsert_trilin(ref_lib, nsam, irrec2, spec, ctff, occ, br, tr, tdr, shx, shy, nkr,
+                           dm, tmp, dx, dy, float(st), ip)
+        assert nko.value == nkr.value
+        assert np.array_equal(bo.view(np.int32), br.view(np.int32))
+        assert np.array_equal(to.view(np.int32), tr.view(np.int32))
+        assert tdo.value == tdr.value
+        tot_b += bo
+        tot_k += nko.value
+    whole = np.zeros(nv, np.complex64)
+    tw = np.zeros(nv, np.float32)
+    td, nk = C.c_double(0.0), C.c_int64(0)
+    L.or32_pinsert_trilin(nsam, irrec2, spec, ctff, occ, whole, tw, C.byref(td), shx, shy,
+                          C.byref(nk), dm, tmp, dx, dy, float(2 * jc - 1), 1)
+    assert nk.value == tot_k > 0
+    assert np.allclose(tot_b, whole, rtol=1e-5, atol=1e-5)
+    # a stripe past the plane (IP = IMP + 1) inserts nothing in both
+    bo, br = np.zeros(nv, np.complex64), np.zeros(nv, np.complex64)
+    to, tr = np.zeros(nv, np.float32), np.zeros(nv, np.float32)
+    nko, nkr = C.c_int64(0), C.c_int64(0)
+    L.or32_pinsert_trilin(nsam, irrec2, spec, ctff, occ, bo, to, C.byref(C.c_double()), shx, shy,
+                          C.byref(nko), dm, tmp, dx, dy, float(st), imp + 1)
+    ref_pinsert_trilin(ref_lib, nsam, irrec2, spec, ctff, occ, br, tr, C.c_double(), shx, shy,
+                       nkr, dm, tmp, dx, dy, float(st), imp + 1)
+    assert nko.value == nkr.value == 0
+
+
 @pytest.mark.parametrize("nsam", [16, 32])
 def test_storeshift(ref_lib, nsam):
     L = O.oracle()
