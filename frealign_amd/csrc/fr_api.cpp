@@ -998,6 +998,17 @@ int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float
                     const float* d_ctff, const int* pidx, const float* amag, float* par,
                     const int* mask, int maxr1, int maxr2, float rbfact, const float* d_fsct,
                     const float* ring, const float* prior, int maxit, int* nevals) {
+  return fr::refine_items(c, nsam, n, d_a3df, d_spec, d_ctff, pidx, amag, par, mask, maxr1, maxr2,
+                          rbfact, d_fsct, ring, prior, maxit, nullptr, nevals);
+}
+}  // extern "C"
+
+namespace fr {
+int refine_items(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                 const float* d_ctff, const int* pidx, const float* amag, float* par,
+                 const int* mask, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+                 const float* ring, const float* prior, int maxit, const float* sm,
+                 int* nevals) {
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(d_a3df && d_spec && d_ctff && pidx && amag && par && mask && d_fsct && ring && prior,
@@ -1017,6 +1028,7 @@ int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float
   a.mask = mask;
   a.prior = prior;
   a.maxit = maxit;
+  a.sm = sm;
   auto cc3m = [&](int m, const int* hp, const float* hdm, const float* hsh, const float* ham,
                   float* hscore) -> int {
     int err = 0;
@@ -1043,6 +1055,9 @@ int fr_refine_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float
   };
   return fr::refine_batch(*c, a, par, nevals, cc3m);
 }
+}  // namespace fr
+
+extern "C" {
 
 // ------------------------------------------------------------------- SIGMA
 int fr_sigma_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_datc,

@@ -40,3 +40,12 @@ struct fr_ctx : public fr::Ctx {
   std::map<int, Tune> tune;
   Tune tune_for(int nsam) const;
 };
+
+namespace fr {
+// fr_refine_batch with the symmetry matrix SM applied to every DM (null: identity)
+int refine_items(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                 const float* d_ctff, const int* pidx, const float* amag, float* par,
+                 const int* mask, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+                 const float* ring, const float* prior, int maxit, const float* sm,
+                 int* nevals);
+}  // namespace fr

@@ -8,6 +8,7 @@
 //   SUBROUTINE PEXTRACT(...)       pextract.f:8-101  -> pextract_
 //   SUBROUTINE PRESB(...)          presb.f:8-160     -> presb_
 //   SUBROUTINE SIGMA(...)          sigma.f:8-221     -> sigma_
+//   SUBROUTINE PREFINE(...)        prefine.f:8-162   -> prefine_
 //   SUBROUTINE PINSERT_TRILIN(...) pinsert_trilin.f:8-244 -> pinsert_trilin_
 //
 // Each call copies its host arrays to the device, runs the batched kernels on
@@ -237,6 +238,64 @@ void sigma_(const int* nsam, const int* irad, const float* amag, const float* a3
   *sigma2n = out[0];
   *sigma2 = out[1];
   *logpc = out[2];
+}
+
+// prefine.f:8-162 (IVFLAG = 0, AMAG > 0, non-helical): VA04A over the
+// MASK-selected PHI, THETA, PSI, DSHX, DSHY minimising CALCFX (-CC3M - FSH and
+// the range penalties, calcfx.f:152-191), the CC3M evaluations on the device.
+// Returns the refined values, PRESA (the final score) and NPAR.
+void prefine_(const int* nsam, const int* irad, float* amag, const int* maxr1, const int* maxr2,
+              float* phi, float* theta, float* psi, float* dshx, float* dshy, float* presa,
+              const int* mask, const float* rbfact, int* npar, const float* c3df,
+              float* /*pbuf*/, const float* shx, const float* shy, const int* ipad,
+              const float* /*cs*/, const float* /*wl*/, const float* /*wgh1*/,
+              const float* /*wgh2*/, const float* /*thetatr*/, const float* ctff,
+              const float* /*amagp*/, const float* rih, const float* halfw, const float* ri2,
+              const float* ri3, const float* /*ri4*/, const float* /*rmax1*/,
+              const float* /*rmax2*/, const float* /*xstd*/, float* /*mbuf*/,
+              const int* /*ilst*/, const float* /*datc*/, const int* /*ibuf*/,
+              const float* /*b3dv*/, float* /*datd*/, const float* /*sinclut*/,
+              const int* ivflag, float* /*outd*/, const float* outc, float* /*qbuc*/,
+              float* /*rbuf*/, const float* fsct, const int* iewald, const float* xm,
+              const float* ym, const float* sx, const float* sy, const float* sig2,
+              const char* asym, const float* /*thetam*/, const float* /*stheta*/,
+              const float* /*psim*/, const float* /*spsi*/, const float* /*stif*/,
+              const float* /*pweights*/, const void* /*fftw_plans*/, const float* sm,
+              size_t /*asym_len*/) {
+  const char* R = "PREFINE";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  need(n <= 1024, R, "NSAM above 1024 is not supported");
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  need(*ivflag == 0, R, "only IVFLAG = 0 is supported");
+  need(*amag > 0.0f, R, "magnification refinement (AMAG < 0) is not supported");
+  need(*sx != 0.0f && *sy != 0.0f, R, "the shift prior SX, SY must be non-zero");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = upload(s.vol, c3df, sizeof(float) * 2 * half_volume(n), R);
+  void* ds = upload(s.spec, outc, sizeof(float) * 2 * ic, R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  void* df = upload(s.fsct, fsct, sizeof(float) * (n / 2 + 1), R);
+  const int p0 = 0;
+  const float am = *amag;
+  float par[6] = {*phi, *theta, *psi, *dshx, *dshy, 0.f};
+  const float ring[4] = {*ri2, *ri3, *rih, *halfw};
+  // FSH(SIG2, DSHX, DSHY, XM-SHX, YM-SHY, SX, SY) (prefine.f:74-75)
+  const float prior[5] = {*xm - *shx, *ym - *shy, *sx, *sy, *sig2};
+  check(fr::refine_items(c, n, 1, (const float*)dv, (const float*)ds, (const float*)dc, &p0, &am,
+                         par, mask, *maxr1, *maxr2, *rbfact, (const float*)df, ring, prior, 50, sm,
+                         nullptr),
+        R);
+  int np = 0;
+  for (int j = 0; j < 5; j++) np += mask[j] == 1 ? 1 : 0;
+  *npar = np;
+  *phi = par[0];
+  *theta = par[1];
+  *psi = par[2];
+  *dshx = par[3];
+  *dshy = par[4];
+  *presa = par[5];
 }
 
 // ctfapply.f:8-161 (MASKCOS, ASYM non-helical, XSTD >= 0).  OUTD and OUTC

@@ -54,6 +54,7 @@ struct RefineArgs {
   const int* mask = nullptr;    // 5 flags: refine PHI, THETA, PSI, DSHX, DSHY
   const float* prior = nullptr; // host n x 5: XM-SHX, YM-SHY, SX, SY, SIG2 (FSH of fsh.f)
   int maxit = 50;               // NCYCLS (prefine.f:29)
+  const float* sm = nullptr;    // SM of the DMs (cc3m.f:66, MATMUL_T); null: identity
 };
 
 // Objective values in one launch: score[q] = CC3M of particle pidx[q] at

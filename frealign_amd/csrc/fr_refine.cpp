@@ -415,7 +415,7 @@ int refine_batch(Ctx& c, const RefineArgs& a, float* par, int* nevals, const Cc3
       }
       x[5] = a.amag[i];
       pidx[q] = a.pidx[i];
-      rotation(x[0], x[1], x[2], x[5], nullptr, &dm[9 * (size_t)q]);
+      rotation(x[0], x[1], x[2], x[5], a.sm, &dm[9 * (size_t)q]);
       shift[2 * q] = x[3];
       shift[2 * q + 1] = x[4];
       amag[q] = x[5];

@@ -406,6 +406,24 @@ void sigma_(const int *nsam, const int *irad, const float *amag, const float *a3
             const int *k, const float *sigp, const float *pweights, const float *rclas,
             const void *fftw_plans, const float *sm, size_t asym_len);
 
+/* PREFINE (prefine.f:8-162; IVFLAG = 0, AMAG > 0, non-helical): refined
+ * PHI, THETA, PSI, DSHX, DSHY (as MASK selects), PRESA and NPAR out. */
+void prefine_(const int *nsam, const int *irad, float *amag, const int *maxr1, const int *maxr2,
+              float *phi, float *theta, float *psi, float *dshx, float *dshy, float *presa,
+              const int *mask, const float *rbfact, int *npar, const float *c3df, float *pbuf,
+              const float *shx, const float *shy, const int *ipad, const float *cs,
+              const float *wl, const float *wgh1, const float *wgh2, const float *thetatr,
+              const float *ctff, const float *amagp, const float *rih, const float *halfw,
+              const float *ri2, const float *ri3, const float *ri4, const float *rmax1,
+              const float *rmax2, const float *xstd, float *mbuf, const int *ilst,
+              const float *datc, const int *ibuf, const float *b3dv, float *datd,
+              const float *sinclut, const int *ivflag, float *outd, const float *outc,
+              float *qbuc, float *rbuf, const float *fsct, const int *iewald, const float *xm,
+              const float *ym, const float *sx, const float *sy, const float *sig2,
+              const char *asym, const float *thetam, const float *stheta, const float *psim,
+              const float *spsi, const float *stif, const float *pweights,
+              const void *fftw_plans, const float *sm, size_t asym_len);
+
 /* PINSERT_TRILIN (pinsert_trilin.f:8-244; IEWALD = 0, NNSTAT = 0): the
  * stripe IP of width ST of the plane DM(1..6), phase shift SHX+DX, SHY+DY,
  * added into B3DF, T3DF, NKSUM and TTD (volumes in the batched layout). */
