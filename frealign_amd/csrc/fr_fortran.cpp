@@ -10,6 +10,7 @@
 //   SUBROUTINE SIGMA(...)          sigma.f:8-221     -> sigma_
 //   SUBROUTINE PREFINE(...)        prefine.f:8-162   -> prefine_
 //   SUBROUTINE PINSERT_TRILIN(...) pinsert_trilin.f:8-244 -> pinsert_trilin_
+//   SUBROUTINE A3D3(...)           a3d3.f:8-146      -> a3d3_
 //
 // Each call copies its host arrays to the device, runs the batched kernels on
 // one item and copies the result back; the batched entry points of
@@ -415,6 +416,108 @@ void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, c
   download(&td, dd, sizeof td, R);
   *nksum += (long long)k;
   *ttd += td;
+}
+
+// a3d3.f:8-146 with NVOL = 1 (IMP = 1, the reference's serial build): the
+// particle is recorded in RECBF/CTFBF/RECB/ISW as a3d3.f:39-85 does (ISWITCH
+// alternating the half-maps), and the one-slot buffer is flushed at once
+// (a3d3.f:107-143 -> A3D3_S -> PINSERT -> PINSERT_TRILIN; INTERP = 1,
+// IEWALD = 0, NNSTAT = 0, NONSYM false) into A3DF/S3DF (ISW 0) or D3DF/V3DF
+// (ISW 1), adding the voxel count to NKSUM1(1) and the CTF^2 sum to TTD1(1)
+// or TTD2(1).  PBC is declared INTEGER in a3d3.f and a3d3_s.f, which only pass
+// it on; PINSERT reads the same storage as REAL (pinsert.f:31,43), as the
+// driver's PBC is, so it is taken as REAL here.
+void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, const float* /*ri*/,
+           const float* rrec, const int* k, const int* ilast, const float* amagp,
+           const float* datc, const float* ctff, float* /*asum*/, float* /*vsum*/,
+           float* /*psum*/, float* /*c3df*/, int* /*ksum*/, const float* phi, const float* theta,
+           const float* psi, const float* dshx, const float* dshy, const float* presa,
+           const float* pbc, const float* boff, float* a3df, float* s3df, double* /*std*/,
+           float* d3df, float* v3df, double* /*vtd*/, const int* nn1, const int* /*maxset*/,
+           const float* /*sinclut*/, const int* /*irada*/, const int* ipad,
+           const float* /*thetatr*/, const int* iewald, const int* nsym, const int* /*isymax*/,
+           const int* jsym, const float* symop, const int* nnstat, long long* /*nksum*/,
+           const int* ifsc, const int* imp, const int* nvol, const int* /*nvol1*/,
+           const int* /*nvol2*/, const float* occ, const float* sig2n, const float* psize,
+           const char* asym, const float* /*alpha*/, const float* /*rise*/, const int* /*nu*/,
+           const int* /*hstart*/, long long* nksum1, double* ttd1, double* ttd2, int* /*ns*/,
+           int* /*ns1*/, const int* interp, float* recbf, float* recb, int* isw, int* ibuf,
+           float* ctfbf, const int* /*nn2*/, const int* /*nn3*/, const float* /*sm*/,
+           const int* nonsym, const int* ism, size_t /*asym_len*/) {
+  const char* R = "A3D3";
+  const int n = *nsam;
+  need(n >= 8 && n % 2 == 0 && n <= 1024 && n <= *nn1, R, "NSAM must be even, 8 .. min(NN1, 1024)");
+  need(*nvol == 1 && *imp == 1, R, "only NVOL = 1, IMP = 1 is supported");
+  need(*irad != 0 && *interp == 1, R, "only trilinear insertion (INTERP = 1) is supported");
+  need(*ipad == 1 && *iewald == 0 && *nnstat == 0, R,
+       "only IPAD = 1, IEWALD = 0, NNSTAT = 0 are supported");
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  need(*nonsym == 0, R, "the NONSYM insertion is not supported");
+  const size_t ic = half_plane(n);
+  const size_t slot_c = (size_t)(*nn1 / 2) * *nn1;  // RECBF(NN1/2*NN1, NVOL), complex
+  const size_t slot_f = (size_t)*nn1 * *nn1;        // CTFBF(NN1*NN1, NVOL), complex
+  auto record = [&](int which) {                    // a3d3.f:41-56, 63-78
+    const int b = (*ibuf)++;
+    isw[b] = which;
+    memcpy(recbf + 2 * slot_c * b, datc, sizeof(float) * 2 * ic);
+    memcpy(ctfbf + 2 * slot_f * b, ctff, sizeof(float) * 2 * (size_t)n * (n + 2));
+    float* r = recb + 10 * (size_t)b;
+    r[0] = *phi;
+    r[1] = *theta;
+    r[2] = *psi;
+    r[3] = *dshx;
+    r[4] = *dshy;
+    r[5] = *amagp;
+    r[6] = *occ;
+    r[7] = *sig2n;
+    r[8] = *presa;
+    r[9] = rrec[*nset - 1];
+  };
+  if (*ibuf < *nvol) {
+    if (*iswitch == 0) {
+      if (*occ != 0.0f) record(0);
+      if (*ifsc <= 2 && *ism == 1) *iswitch = 1;
+    } else {
+      if (*ifsc == 0 && *occ != 0.0f) record(1);
+      if (*ism == 1) *iswitch = 0;
+    }
+  }
+  if (!(*ibuf == *nvol || *k == *ilast)) return;
+  if (*ibuf >= 1) {
+    const int h = isw[0];
+    need(recb[7] != 0.0f, R, "SIG2N is zero");
+    fr_ctx* c = ctx(R);
+    State& s = st();
+    const size_t hv = half_volume(n);
+    float* num = h == 0 ? a3df : d3df;
+    float* ctf2 = h == 0 ? s3df : v3df;
+    void* ds = upload(s.spec, recbf, sizeof(float) * 2 * ic, R);
+    void* dc = upload(s.ctff, ctfbf, sizeof(float) * 4 * ic, R);
+    void* dv = upload(s.vol, num, sizeof(float) * 2 * hv, R);
+    void* dt = upload(s.vol2, ctf2, sizeof(float) * hv, R);
+    void* dsum = s.sums.get(2 * sizeof(long long) + 2 * sizeof(double));
+    need(dsum != nullptr, R, "device allocation failed");
+    need(hipMemset(dsum, 0, 2 * sizeof(long long) + 2 * sizeof(double)) == hipSuccess, R,
+         "device memset failed");
+    long long* dk = (long long*)dsum;
+    double* dd = (double*)(dk + 2);
+    // one half is addressed (ISW): its arrays stand in for the other's
+    check(fr_insert_batch(c, n, 1, (const float*)ds, (const float*)dc, recb, &h, *nsym, jsym,
+                          symop, *pbc, *boff, *psize, (float*)dv, (float*)dt, (float*)dv,
+                          (float*)dt, dk, dd),
+          R);
+    check(fr_synchronize(c), R);
+    download(num, dv, sizeof(float) * 2 * hv, R);
+    download(ctf2, dt, sizeof(float) * hv, R);
+    long long kc[2];
+    double td[2];
+    download(kc, dk, sizeof kc, R);
+    download(td, dd, sizeof td, R);
+    nksum1[0] += kc[h];  // PINSERT_TRILIN's NKSUM is NKSUM1(I), I = 1
+    if (h == 0) ttd1[0] += td[0];
+    else ttd2[0] += td[1];
+  }
+  *ibuf = 0;
 }
 
 }  // extern "C"

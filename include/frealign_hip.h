@@ -436,6 +436,27 @@ void pinsert_trilin_(const int *nsam, const float *rii, const int *irrec2, const
                      const int *nsamh, const int *ic, const float *dx, const float *dy,
                      const float *st, const int *ip, int *ns1);
 
+/* A3D3 (a3d3.f:8-146; NVOL = 1, IMP = 1, INTERP = 1, non-helical, NONSYM
+ * false): records the particle in RECBF/CTFBF/RECB/ISW, alternates ISWITCH,
+ * and flushes the one-slot buffer into A3DF/S3DF or D3DF/V3DF, NKSUM1(1) and
+ * TTD1(1)/TTD2(1).  PBC is REAL storage (pinsert.f:31 reads it so). */
+void a3d3_(const int *nset, int *iswitch, const int *nsam, const int *irad, const float *ri,
+           const float *rrec, const int *k, const int *ilast, const float *amagp,
+           const float *datc, const float *ctff, float *asum, float *vsum, float *psum,
+           float *c3df, int *ksum, const float *phi, const float *theta, const float *psi,
+           const float *dshx, const float *dshy, const float *presa, const float *pbc,
+           const float *boff, float *a3df, float *s3df, double *std_, float *d3df, float *v3df,
+           double *vtd, const int *nn1, const int *maxset, const float *sinclut,
+           const int *irada, const int *ipad, const float *thetatr, const int *iewald,
+           const int *nsym, const int *isymax, const int *jsym, const float *symop,
+           const int *nnstat, long long *nksum, const int *ifsc, const int *imp,
+           const int *nvol, const int *nvol1, const int *nvol2, const float *occ,
+           const float *sig2n, const float *psize, const char *asym, const float *alpha,
+           const float *rise, const int *nu, const int *hstart, long long *nksum1,
+           double *ttd1, double *ttd2, int *ns, int *ns1, const int *interp, float *recbf,
+           float *recb, int *isw, int *ibuf, float *ctfbf, const int *nn2, const int *nn3,
+           const float *sm, const int *nonsym, const int *ism, size_t asym_len);
+
 /* ------------------------------------------- batches of uneven box sizes
  * No counterpart in the reference (its particles go one at a time through
  * LMAIN, lmain.f:509-885).  A plan serves one box size and batch size: the

@@ -162,3 +162,98 @@ def test_fortran_percall_vs_oracle(tmp_path):
     print(f"SIGMA {got} oracle {out} rel {rel}")
     assert rel[0] < 1e-3 and rel[1] < 1e-3 and rel[2] < 1e-4
     assert math.isfinite(ccv)
+
+
+def test_a3d3_percall_vs_oracle():
+    """a3d3_ (the reference's A3D3 symbol and argument list, NVOL = 1) called
+    for four particles in turn, IFSC = 0 and ISM = 1 as the reference's FSC
+    run: the half-maps alternate with ISWITCH, a particle with OCC = 0 is not
+    recorded but still switches, and every call flushes.  Against the FP32
+    oracle's PINSERT into the same halves: NKSUM1 exact, volumes 1e-5 relative
+    L2, TTD1/TTD2 1e-6 relative; the bookkeeping (RECB, ISW, IBUF, ISWITCH)
+    as a3d3.f:39-85,145 leave it."""
+    from frealign_amd import api
+
+    L = O.oracle()
+    lib = C.CDLL(LIB)
+    nsam, nn1 = 32, 40
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    nv = nsam * nsam * jc
+    nn3 = nn1 * nn1 * (nn1 // 2 + 1)
+    r = np.random.default_rng(23)
+    npart = 4
+    datc = (r.standard_normal((npart, ic)) + 1j * r.standard_normal((npart, ic))).astype(np.complex64)
+    ctff = (0.6 + 0.2 * (r.standard_normal((npart, 2 * ic))
+                         + 1j * r.standard_normal((npart, 2 * ic)))).astype(np.complex64)
+    eul = r.uniform(-np.pi, np.pi, (npart, 3)).astype(np.float32)
+    shf = r.uniform(-0.3, 0.3, (npart, 2)).astype(np.float32)
+    occ = np.array([1.0, 0.7, 0.0, 0.9], np.float32)
+    sig2n = r.uniform(0.5, 2.0, npart).astype(np.float32)
+    presa = r.uniform(0.2, 0.8, npart).astype(np.float32)
+    rrec = np.array([0.0, 0.45], np.float32)  # RREC(NSET), NSET = 2
+    nsym, symop, jsym, _ = api.point_group("D", 7)
+    pbc, boff, psize = 5.0, 40.0, 1.5
+    a3df, d3df = np.zeros(nn3, np.complex64), np.zeros(nn3, np.complex64)
+    s3df, v3df = np.zeros(nn3, np.float32), np.zeros(nn3, np.float32)
+    recbf = np.zeros(nn1 // 2 * nn1 + ic, np.complex64)  # the copy runs past NN1/2*NN1
+    ctfbf = np.zeros(nn1 * nn1 + 2 * ic, np.complex64)
+    recb = np.zeros(10, np.float32)
+    isw = np.full(2, -1, np.int32)
+    nksum1 = np.zeros(2, np.int64)
+    ttd1, ttd2 = np.zeros(2, np.float64), np.zeros(2, np.float64)
+    iswitch, ibuf = C.c_int(0), C.c_int(0)
+    dummy = np.zeros(16, np.float32)
+    idummy = np.zeros(64, np.int32)
+    f, i = O.byref_f, O.byref_i
+    expect_half = []
+    for k in range(npart):
+        before = iswitch.value
+        args = [i(2), C.byref(iswitch), i(nsam), i(1), f(0.4 * nsam), rrec.ctypes, i(k + 1),
+                i(npart), f(1.0), np.ascontiguousarray(datc[k]).ctypes,
+                np.ascontiguousarray(ctff[k]).ctypes, dummy.ctypes, dummy.ctypes, dummy.ctypes,
+                dummy.ctypes, idummy.ctypes, f(float(eul[k, 0])), f(float(eul[k, 1])),
+                f(float(eul[k, 2])), f(float(shf[k, 0])), f(float(shf[k, 1])),
+                f(float(presa[k])), f(pbc), f(boff), a3df.ctypes, s3df.ctypes,
+                C.byref(C.c_double(0.0)), d3df.ctypes, v3df.ctypes, C.byref(C.c_double(0.0)),
+                i(nn1), i(1), dummy.ctypes, i(1), i(1), f(0.0), i(0), i(nsym), i(8),
+                jsym.ctypes, symop.ctypes, i(0), C.byref(C.c_int64(0)), i(0), i(1), i(1), i(1),
+                i(1), f(float(occ[k])), f(float(sig2n[k])), f(psize), C.c_char_p(b"D7 "),
+                f(0.0), f(0.0), i(0), i(0), nksum1.ctypes, ttd1.ctypes, ttd2.ctypes,
+                idummy.ctypes, idummy.ctypes, i(1), recbf.ctypes, recb.ctypes, isw.ctypes,
+                C.byref(ibuf), ctfbf.ctypes, i(nn1), i(nn3), O.IDENTITY.ctypes, i(0), i(1),
+                C.c_size_t(3)]
+        assert len(args) == 73
+        print(f"A3D3 call K={k + 1} ISWITCH={before} OCC={occ[k]}", flush=True)
+        lib.a3d3_(*args)
+        assert ibuf.value == 0
+        assert iswitch.value == 1 - before
+        expect_half.append(before if occ[k] != 0.0 else -1)
+        if occ[k] != 0.0:
+            assert isw[0] == before
+            want = [*eul[k], *shf[k], 1.0, occ[k], sig2n[k], presa[k], rrec[1]]
+            assert np.array_equal(recb, np.array(want, np.float32))
+            assert np.array_equal(recbf[:ic], datc[k])
+            assert np.array_equal(ctfbf[:2 * ic], ctff[k])
+    assert expect_half == [0, 1, -1, 1]
+    # the oracle's PINSERT of the recorded particles into their halves
+    num = [np.zeros(nv, np.complex64) for _ in range(2)]
+    c2 = [np.zeros(nv, np.float32) for _ in range(2)]
+    ttd = [C.c_double(0.0), C.c_double(0.0)]
+    nk = [C.c_int64(0), C.c_int64(0)]
+    for k, h in enumerate(expect_half):
+        if h < 0:
+            continue
+        on = np.float32(occ[k]) / np.float32(sig2n[k])
+        L.or32_pinsert(nsam, float(rrec[1]), 1.0, datc[k], ctff[k], num[h], c2[h],
+                       C.byref(ttd[h]), float(eul[k, 0]), float(eul[k, 1]), float(eul[k, 2]),
+                       float(shf[k, 0]), float(shf[k, 1]), float(presa[k]), pbc, boff, nsym,
+                       jsym, symop, C.byref(nk[h]), psize, float(on), O.IDENTITY, 0)
+    assert int(nksum1[0]) == nk[0].value + nk[1].value and nksum1[1] == 0
+    for h, (gn, gc, gt) in enumerate(((a3df, s3df, ttd1), (d3df, v3df, ttd2))):
+        e_num = np.linalg.norm(gn[:nv] - num[h]) / np.linalg.norm(num[h])
+        e_c2 = np.linalg.norm(gc[:nv] - c2[h]) / np.linalg.norm(c2[h])
+        e_t = abs(gt[0] - ttd[h].value) / ttd[h].value
+        print(f"A3D3 half {h}: numerator {e_num:.1e} CTF^2 {e_c2:.1e} TTD {e_t:.1e}")
+        assert e_num < 1e-5 and e_c2 < 1e-5 and e_t < 1e-6
+        assert not gn[nv:].any() and not gc[nv:].any() and gt[1] == 0.0
