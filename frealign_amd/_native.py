@@ -70,6 +70,11 @@ SIGNATURES = {
                          vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_storeshift": ([vp, ci, cd, cd, ci, vp, vp, vp, vp, vp, vp], ci),
+    "fr_combine_arrays": ([vp, ci, C.c_longlong, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp,
+                           C.POINTER(C.c_longlong), pd, pd], ci),
+    "fr_shiftvol": ([vp, ci, pd, cd, cf, ci, vp, vp, vp, vp, ci, vp, cf, vp, cf, ci], ci),
+    "fr_correct3d": ([vp, ci, vp, ci, ci, vp, ci], ci),
+    "fr_fpart_from_mw": ([ci, cf, cf, cf, vp, C.POINTER(C.c_float)], ci),
     "fr_profile_enable": ([vp, ci], ci),
     "fr_profile_read": ([vp, ci, pd, C.POINTER(C.c_longlong)], ci),
     "fr_search_layout": ([vp, ci, ci, ci, pi_, pi_], ci),

@@ -625,6 +625,40 @@ class Context:
                                       _p(s), _p(d), _p(v), _p(b), _p(c)), "fr_storeshift")
         return b, c
 
+    def combine_arrays(self, nsam: int, nn3: int, nvol1: int, nvol2: int, ifsc: int, a, s, d, v,
+                       imp: int, nksum1, ttd1, ttd2, nksum: int, std: float, vtd: float):
+        """COMBINE_ARRAYS (combine_arrays.f): copies summed into copy 1 in place
+        (a, s, d, v hold the copies nn3 elements apart); returns the updated
+        (NKSUM, STD, VTD)."""
+        k1 = np.ascontiguousarray(nksum1, dtype=np.int64)
+        t1 = np.ascontiguousarray(ttd1, dtype=np.float64)
+        t2 = np.ascontiguousarray(ttd2, dtype=np.float64)
+        nk, sd, vd = C.c_longlong(int(nksum)), C.c_double(std), C.c_double(vtd)
+        self._sync_stream()
+        N.check(N.lib().fr_combine_arrays(self.h, nsam, int(nn3), nvol1, nvol2, ifsc, _p(a), _p(s),
+                                          _p(d), _p(v), imp, _p(k1), _p(t1), _p(t2), C.byref(nk),
+                                          C.byref(sd), C.byref(vd)), "fr_combine_arrays")
+        return nk.value, sd.value, vd.value
+
+    def shiftvol(self, nsam: int, std: float, vtd: float, rlim: float, iredun: int, a, d, s, v,
+                 ifsc: int, pssnr, fpart: float, fsc, fmask: float):
+        """SHIFTVOL (shiftvol.f) in place on a (and s for ifsc 0); returns STD+VTD."""
+        ps = np.ascontiguousarray(pssnr, dtype=np.float32)
+        fs = np.ascontiguousarray(fsc, dtype=np.float32)
+        sd = C.c_double(std)
+        self._sync_stream()
+        N.check(N.lib().fr_shiftvol(self.h, nsam, C.byref(sd), float(vtd), float(rlim), int(iredun),
+                                    _p(a), _p(d), _p(s), _p(v), ifsc, _p(ps), float(fpart),
+                                    _p(fs), float(fmask), min(ps.size, fs.size)), "fr_shiftvol")
+        return sd.value
+
+    def correct3d(self, nsam: int, a3dv, interp: int, ipad: int, sinclut):
+        """CORRECT3D (correct3d.f) in place on the padded real volume."""
+        lut = np.ascontiguousarray(sinclut, dtype=np.float32)
+        self._sync_stream()
+        N.check(N.lib().fr_correct3d(self.h, nsam, _p(a3dv), interp, ipad, _p(lut), lut.size),
+                "fr_correct3d")
+
 
 @dataclass
 class HalfMaps:

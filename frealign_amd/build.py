@@ -22,6 +22,7 @@ SOURCES = [
     "fr_prep.hip",
     "fr_extract.hip",
     "fr_insert.hip",
+    "fr_final.hip",
     "fr_stats.hip",
     "fr_search.hip",
     "fr_lists.hip",

@@ -516,6 +516,90 @@ int fr_storeshift(fr_ctx* c, int nsam, double std_, double vtd, int iredun, cons
   return launch_storeshift(*c, nsam, std_, vtd, iredun, d_a, d_s, d_d, d_v, d_b, d_c);
 }
 
+// ------------------------------------------------------ volume finalisation
+int fr_combine_arrays(fr_ctx* c, int nsam, long long nn3, int nvol1, int nvol2, int ifsc,
+                      float* d_a3df, float* d_s3df, float* d_d3df, float* d_v3df, int imp,
+                      const long long* nksum1, const double* ttd1, const double* ttd2,
+                      long long* nksum, double* std_, double* vtd) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3df && d_s3df && nksum1 && ttd1 && nksum && std_, "null argument");
+  FR_CHECK(ifsc != 0 || (d_d3df && d_v3df && ttd2 && vtd), "null second half-map argument");
+  FR_CHECK(nvol1 >= 1 && nvol2 >= 1 && imp >= 1, "bad NVOL1/NVOL2/IMP");
+  FR_CHECK(nn3 >= (long long)(nsam / 2 + 1) * nsam * nsam, "NN3 smaller than the half-volume");
+  if (int r = launch_combine(*c, nsam, (size_t)nn3, nvol1, d_a3df, d_s3df)) return r;
+  if (ifsc == 0)
+    if (int r = launch_combine(*c, nsam, (size_t)nn3, nvol2, d_d3df, d_v3df)) return r;
+  // combine_arrays.f:53-67
+  for (int i = 0; i < imp; i++) *nksum += nksum1[i];
+  const int n = nvol1 != 1 ? nvol1 : imp;
+  for (int i = 0; i < n; i++) {
+    *std_ = *std_ + ttd1[i];
+    if (ifsc == 0) *vtd = *vtd + ttd2[i];
+  }
+  return 0;
+}
+
+int fr_shiftvol(fr_ctx* c, int nsam, double* std_, double vtd, float rlim, int iredun,
+                float* d_a3df, float* d_d3df, float* d_s3df, const float* d_v3df, int ifsc,
+                const float* pssnr, float fpart, const float* fsc, float fmask, int nshell) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(std_ && d_a3df && d_s3df, "null argument");
+  FR_CHECK(ifsc != 0 || (d_d3df && d_v3df), "null second half-map argument");
+  const int ir = (int)((float)nsam * rlim);
+  FR_CHECK(ir >= 0, "negative RLIM");
+  if (fpart != 0.0f) {
+    FR_CHECK(pssnr && fsc, "null PSSNR/FSC with FPART != 0");
+    FR_CHECK(fmask != 0.0f, "FMASK must be non-zero with FPART != 0");
+    // shell index INT(SQRT(JRAD2)+0.5)+1 <= IR+1 inside the RLIM sphere
+    FR_CHECK(nshell >= ir + 1, "PSSNR/FSC need %d shells (got %d)", ir + 1, nshell);
+  }
+  *std_ = *std_ + vtd;  // shiftvol.f:23
+  int err = 0;
+  float* tabs = nullptr;
+  if (fpart != 0.0f) {
+    tabs = (float*)c->slot(27, sizeof(float) * 2 * (size_t)(ir + 1), &err);
+    if (!tabs) return err;
+    FR_HIP(hipMemcpyAsync(tabs, pssnr, sizeof(float) * (ir + 1), hipMemcpyHostToDevice,
+                          c->stream));
+    FR_HIP(hipMemcpyAsync(tabs + ir + 1, fsc, sizeof(float) * (ir + 1), hipMemcpyHostToDevice,
+                          c->stream));
+  }
+  int r = launch_shiftvol(*c, nsam, *std_ * iredun, ir * ir, ifsc, fpart, fmask, d_a3df, d_d3df,
+                          d_s3df, d_v3df, tabs, tabs ? tabs + ir + 1 : nullptr);
+  if (r) return r;
+  // the host tables are read by the copies above: finish before returning
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int fr_correct3d(fr_ctx* c, int nsam, float* d_a3dv, int interp, int ipad, const float* sinclut,
+                 int nlut) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3dv && sinclut, "null argument");
+  FR_CHECK(nlut >= kSincLut, "SINCLUT needs %d entries (got %d)", kSincLut, nlut);
+  float scal = 1.0f / (float)nsam;  // correct3d.f:23-24
+  if (ipad != 0) scal = scal / (float)ipad;
+  return launch_correct3d(*c, nsam, scal, interp, sinclut, d_a3dv);
+}
+
+int fr_fpart_from_mw(int nsam, float mw, float dalt, float psize, float* ssnr, float* fpart) {
+  FR_CHECK(fpart && ssnr, "null argument");
+  FR_CHECK(nsam >= 2, "bad NSAM");
+  FR_CHECK(mw != 0.0f, "MW = 0 needs the iterative FPART search, which is not provided");
+  FR_CHECK(dalt != 0.0f && psize != 0.0f, "DALT and PSIZE must be non-zero");
+  // find_fpart.f:141-149
+  const float box = psize * (float)nsam;
+  float f = 1000.0f * mw / dalt / (box * box * box);
+  f = fabsf(f);
+  if (f != 0.0f)
+    for (int i = 0; i < nsam / 2 + 1; i++) ssnr[i] = ssnr[i] / f;
+  *fpart = f;
+  return 0;
+}
+
 // ------------------------------------------------------------------ search
 // The correlation annulus of ccp.f:110-131 and 194-216 on the half-plane:
 // I >= 0, max(MAXR1^2, 1) <= I^2+J^2 < MAXR2^2 (DC is outside the NSAMR

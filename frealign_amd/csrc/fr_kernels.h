@@ -31,6 +31,18 @@ int launch_shell_sums(Ctx& c, int nsam, int istp, int nstp, const float* a, cons
 int launch_storeshift(Ctx& c, int nsam, double std, double vtd, int iredun, const float* a,
                       const float* s, const float* d, const float* v, float* b, float* cc);
 
+// Volume finalisation (fr_final.hip).  COMBINE_ARRAYS: copy 1 += copies
+// 2..ncopy (stride elements apart), complex a and real s.
+int launch_combine(Ctx& c, int nsam, size_t stride, int ncopy, float* a, float* s);
+// SHIFTVOL (NNSTAT = 0); stdi = STD*IREDUN after STD += VTD; pssnr/fsc on the device
+int launch_shiftvol(Ctx& c, int nsam, double stdi, int irrec2, int ifsc, float fpart, float fmask,
+                    float* a, const float* d, float* s, const float* v, const float* pssnr,
+                    const float* fsc);
+// CORRECT3D on the padded real volume; sinclut: SINCLUT(1..kSincLut) on the host
+constexpr int kSincLut = 90;
+int launch_correct3d(Ctx& c, int nsam, float scal, int interp, const float* sinclut,
+                     float* a3dv);
+
 // Particle preparation (lmain.f:459 r2c + ctfapply.f:8-161)
 struct PrepGlobals {
   float cs, wl, wgh1, wgh2, thetatr, rih, halfw, ri2, ri3, tx, ty;

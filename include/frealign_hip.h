@@ -137,6 +137,38 @@ int fr_insert_batch(fr_ctx *ctx, int nsam, int nb, const float *d_datc, const fl
 
 /* ------------------------------------------------------------- statistics */
 
+/* Replaces COMBINE_ARRAYS (combine_arrays.f:8-69, NNSTAT=0): copies 2..nvol1
+ * of d_a3df/d_s3df (nn3 elements apart, the A3DF(NN3,NVOL1) layout) are added
+ * into copy 1 in copy order, and for ifsc == 0 copies 2..nvol2 of
+ * d_d3df/d_v3df.  Host counters: *nksum += nksum1[0..imp-1]; *std_ (and *vtd
+ * for ifsc == 0) += ttd1 (ttd2) over nvol1 entries when nvol1 != 1, else imp. */
+int fr_combine_arrays(fr_ctx *ctx, int nsam, long long nn3, int nvol1, int nvol2, int ifsc,
+                      float *d_a3df, float *d_s3df, float *d_d3df, float *d_v3df, int imp,
+                      const long long *nksum1, const double *ttd1, const double *ttd2,
+                      long long *nksum, double *std_, double *vtd);
+
+/* Replaces SHIFTVOL (shiftvol.f:8-77, NNSTAT=0), in place: *std_ += vtd; for
+ * ifsc == 0 A += D and S += V; then within INT(NSAM*RLIM) A = A(-1)^(l+m+n) /
+ * (S + STD*IREDUN) (fpart == 0) or that / (1 + FPART/A1/FMASK) with
+ * A1 = 2|FSC|/(1-|FSC|) (1000 at |FSC| >= 1), zero where PSSNR or S is zero;
+ * zero outside.  pssnr/fsc: host, nshell >= INT(NSAM*RLIM)+1 entries (read
+ * only when fpart != 0). */
+int fr_shiftvol(fr_ctx *ctx, int nsam, double *std_, double vtd, float rlim, int iredun,
+                float *d_a3df, float *d_d3df, float *d_s3df, const float *d_v3df, int ifsc,
+                const float *pssnr, float fpart, const float *fsc, float fmask, int nshell);
+
+/* Replaces CORRECT3D (correct3d.f:19-47) with BOXFT_LUT (boxft_lut.f:26-37):
+ * the real volume d_a3dv (NSAM+2 row pitch) divided by the sinc product
+ * (interp == 0) or its square (interp == 1).  sinclut: host SINCLUT of
+ * CALCSINC (calcsinc.f:17-19), nlut >= 90 entries. */
+int fr_correct3d(fr_ctx *ctx, int nsam, float *d_a3dv, int interp, int ipad, const float *sinclut,
+                 int nlut);
+
+/* FIND_FPART's closed form for MW != 0 (find_fpart.f:141-149): FPART =
+ * |1000 MW / DALT / (PSIZE NSAM)^3| and SSNR(1..JC) /= FPART (host).  The MW = 0
+ * search (3D FFTs, MASK3D, VOLMEASURE, FPART_CC) is not provided. */
+int fr_fpart_from_mw(int nsam, float mw, float dalt, float psize, float *ssnr, float *fpart);
+
 /* Replaces SHELTEST's FSC/PSSNR part (sheltest.f:8-122, NNSTAT=0) with FP64
  * shell sums on the GPU.  Host outputs per shell: npix, npixt, dfsc, pssnr. */
 int fr_sheltest(fr_ctx *ctx, int nsam, int istp, const float *d_a, const float *d_d,

@@ -672,6 +672,104 @@ void P(storeshift)(int nsam, double std, double vtd, int iredun, const R *a3df, 
   }
 }
 
+/* --------------------------------------------------- combine_arrays.f */
+/* combine_arrays.f:33-52 (NNSTAT=0): copies 2..NVOL1 of A3DF/S3DF (stride
+ * nn3 elements) added into copy 1 in order; for IFSC=0 likewise D3DF/V3DF
+ * over NVOL2.  The counters (combine_arrays.f:53-67) are host arithmetic. */
+void P(combine_arrays)(int nsam, long long nn3, int nvol1, int nvol2, int ifsc, R *a3df, R *s3df,
+                       R *d3df, R *v3df) {
+  size_t nv = (size_t)nsam * nsam * (nsam / 2 + 1);
+  for (size_t i = 0; i < nv; i++) {
+    for (int is = 2; is <= nvol1; is++) {
+      size_t o = (size_t)(is - 1) * (size_t)nn3;
+      a3df[2 * i] = a3df[2 * i] + a3df[2 * (o + i)];
+      a3df[2 * i + 1] = a3df[2 * i + 1] + a3df[2 * (o + i) + 1];
+      s3df[i] = s3df[i] + s3df[o + i];
+    }
+    if (ifsc == 0)
+      for (int is = 2; is <= nvol2; is++) {
+        size_t o = (size_t)(is - 1) * (size_t)nn3;
+        d3df[2 * i] = d3df[2 * i] + d3df[2 * (o + i)];
+        d3df[2 * i + 1] = d3df[2 * i + 1] + d3df[2 * (o + i) + 1];
+        v3df[i] = v3df[i] + v3df[o + i];
+      }
+  }
+}
+
+/* --------------------------------------------------------- shiftvol.f */
+/* shiftvol.f:8-77 (NNSTAT=0).  *std is STD in/out (STD+VTD on return).  The
+ * divisions by S3DF+STD*IREDUN (a DOUBLE) run in double per component, as
+ * COMPLEX / DOUBLE does (storeshift above follows the same rule). */
+void P(shiftvol)(int nsam, double *std, double vtd, float rlim, int iredun, R *a3df, R *d3df,
+                 R *s3df, const R *v3df, int ifsc, const float *pssnr, float fpart,
+                 const float *fsc, float fmask) {
+  int jc = nsam / 2 + 1;
+  *std = *std + vtd;
+  int ir = (int)((float)nsam * rlim);
+  int irrec2 = ir * ir;
+  for (int l = 1; l <= jc; l++) {
+    int ll = l - 1;
+    for (int m = 1; m <= nsam; m++) {
+      int mm = m - 1;
+      if (mm >= jc) mm -= nsam;
+      for (int n = 1; n <= nsam; n++) {
+        int nn = n - 1;
+        if (nn >= jc) nn -= nsam;
+        int jrad2 = ll * ll + mm * mm + nn * nn;
+        R sg = ((ll + mm + nn) % 2 != 0) ? (R)-1.0 : (R)1.0;
+        int i = (int)(sqrtf((float)jrad2) + 0.5f) + 1;
+        size_t id = (size_t)(l - 1) + (size_t)jc * ((size_t)(m - 1) + (size_t)nsam * (n - 1));
+        if (ifsc == 0) {
+          a3df[2 * id] = a3df[2 * id] + d3df[2 * id];
+          a3df[2 * id + 1] = a3df[2 * id + 1] + d3df[2 * id + 1];
+          s3df[id] = s3df[id] + v3df[id];
+        }
+        R re = 0, im = 0;
+        if (jrad2 <= irrec2) {
+          double den = (double)s3df[id] + *std * iredun;
+          if (fpart == 0.0f) {
+            re = (R)((double)(R)(a3df[2 * id] * sg) / den);
+            im = (R)((double)(R)(a3df[2 * id + 1] * sg) / den);
+          } else if (pssnr[i - 1] != 0.0f && s3df[id] != (R)0) {
+            float af = fabsf(fsc[i - 1]);
+            float a1 = af < 1.0f ? 2.0f * af / (1.0f - af) : 1000.0f;
+            float w = 1.0f + fpart / a1 / fmask;
+            re = (R)((double)(R)(a3df[2 * id] * sg) / den / (double)w);
+            im = (R)((double)(R)(a3df[2 * id + 1] * sg) / den / (double)w);
+          }
+        }
+        a3df[2 * id] = re;
+        a3df[2 * id + 1] = im;
+      }
+    }
+  }
+}
+
+/* ------------------------------------------- correct3d.f, boxft_lut.f */
+/* correct3d.f:19-47: A3DV (NSAM+2 row pitch) divided by BOXFT_LUT (INTERP=0)
+ * or its square (INTERP=1); BOXFT_LUT (boxft_lut.f:26-37) is the product of
+ * SINCLUT(NINT(180|a|)) over the three axes, 1 where |a| < 0.02. */
+void P(correct3d)(int nsam, R *a3dv, int interp, int ipad, const float *sinclut) {
+  int nmid = nsam / 2 + 1;
+  float scal = 1.0f / (float)nsam;
+  if (ipad != 0) scal = scal / (float)ipad;
+  for (int i = 1; i <= nsam; i++)
+    for (int j = 1; j <= nsam; j++)
+      for (int k = 1; k <= nsam; k++) {
+        float a[3] = {(float)(i - nmid) * scal, (float)(j - nmid) * scal,
+                      (float)(k - nmid) * scal};
+        float prod = 1.0f;
+        for (int q = 0; q < 3; q++) {
+          float x = fabsf(a[q]);
+          float sn = x < 2.0e-2f ? 1.0f : sinclut[lroundf(180.0f * x) - 1];
+          prod = prod * sn;
+        }
+        size_t id = (size_t)(i - 1) + (size_t)(nsam + 2) * ((size_t)(j - 1) + (size_t)nsam * (k - 1));
+        if (interp == 0) a3dv[id] = a3dv[id] / (R)prod;
+        else a3dv[id] = a3dv[id] / (R)(prod * prod);
+      }
+}
+
 /* -------------------------------------------------------- sheltest.f */
 int P(sheltest)(int nsam, int istp, const R *a3df, const R *d3df, const R *s3df, const R *v3df,
                 float fmask, int *npix, int *npixt, double *dfsc, R *pssnr) {

@@ -50,6 +50,11 @@ void or_dm(float phi, float theta, float psi, float amag, const float *sm, float
 
 /* matmul_t.f:8-26: C <- (A^T B)^T computed in the reference order. */
 void or_matmul_t(const float a[9], const float b[9], float c[9]);
+/* calcsinc.f:17-19: SINCLUT(1..n) */
+void or_calcsinc(float *sinclut, int n);
+/* combine_arrays.f:53-67: the particle-count and CTF^2-sum counters */
+void or_combine_counters(int ifsc, int imp, int nvol1, const int64_t *nksum1, const double *ttd1,
+                         const double *ttd2, int64_t *nksum, double *std, double *vtd);
 
 /* getsymmat.f:8-97 for C<n> and D<n> (and T/O/I tables).  Returns nsym;
  * fills symop (3x3 column-major per operator), jsym (rotational order per
@@ -132,6 +137,18 @@ void or_irfft2_d(int nsam, double *data);
   void P##storeshift(int nsam, double std, double vtd, int iredun,              \
                      const R *a3df, const R *s3df, const R *d3df,               \
                      const R *v3df, R *b3df, R *c3df);                          \
+  /* combine_arrays.f:33-52 (NNSTAT=0): the private copies (stride nn3)       \
+   * added into copy 1 in order */                                             \
+  void P##combine_arrays(int nsam, long long nn3, int nvol1, int nvol2,         \
+                         int ifsc, R *a3df, R *s3df, R *d3df, R *v3df);         \
+  /* shiftvol.f:8-77 (NNSTAT=0), in place; *std += vtd first */                 \
+  void P##shiftvol(int nsam, double *std, double vtd, float rlim, int iredun,   \
+                   R *a3df, R *d3df, R *s3df, const R *v3df, int ifsc,          \
+                   const float *pssnr, float fpart, const float *fsc,           \
+                   float fmask);                                                \
+  /* correct3d.f:19-47 with boxft_lut.f:26-37; a3dv has NSAM+2 row pitch */     \
+  void P##correct3d(int nsam, R *a3dv, int interp, int ipad,                    \
+                    const float *sinclut);                                      \
   /* sheltest.f:8-122 (FSC part, NNSTAT=0); returns nstp */                     \
   int P##sheltest(int nsam, int istp, const R *a3df, const R *d3df,             \
                   const R *s3df, const R *v3df, float fmask, int *npix,         \

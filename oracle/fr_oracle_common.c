@@ -374,3 +374,24 @@ void or_irfft2_f(int n, float *data) {
   for (size_t i = 0; i < sz; i++) data[i] = (float)d[i];
   free(d);
 }
+
+/* calcsinc.f:17-19: SINCLUT(J) = SIN(J*PI/180)/(J*PI/180), J = 1..n (FP32) */
+void or_calcsinc(float *sinclut, int n) {
+  const float pi = 3.1415926535897f;
+  for (int j = 1; j <= n; j++) {
+    float x = (float)j * pi / 180.0f;
+    sinclut[j - 1] = sinf(x) / x;
+  }
+}
+
+/* combine_arrays.f:53-67: NKSUM += NKSUM1(1..IMP); STD (and VTD for IFSC=0)
+ * += TTD1 (TTD2) over NVOL1 entries when NVOL1 != 1, else over IMP */
+void or_combine_counters(int ifsc, int imp, int nvol1, const int64_t *nksum1, const double *ttd1,
+                         const double *ttd2, int64_t *nksum, double *std, double *vtd) {
+  for (int i = 0; i < imp; i++) *nksum += nksum1[i];
+  int n = nvol1 != 1 ? nvol1 : imp;
+  for (int i = 0; i < n; i++) {
+    *std = *std + ttd1[i];
+    if (ifsc == 0) *vtd = *vtd + ttd2[i];
+  }
+}

@@ -70,6 +70,9 @@ def _declare_oracle(L):
     L.or_va04_testfn.restype = cf
     L.or_va04a.argtypes = [f32p, f32p, ci, cf, ci, ci, ci, C.c_void_p, C.c_void_p, pi_]
     L.or_va04a.restype = cf
+    L.or_calcsinc.argtypes = [f32p, ci]
+    L.or_combine_counters.argtypes = [ci, ci, ci, C.POINTER(C.c_int64), f64p, f64p,
+                                      C.POINTER(C.c_int64), pd, pd]
     L.or_rfft2_f.argtypes = [ci, f32p]
     L.or_irfft2_f.argtypes = [ci, f32p]
     L.or_rfft2_d.argtypes = [ci, f64p]
@@ -94,6 +97,9 @@ def _declare_oracle(L):
         g("pinsert_trilin").argtypes = [ci, ci, cp, cp, cf, cp, rp, pd, cf, cf,
                                         C.POINTER(C.c_int64), f32p, cf, cf, cf, cf, ci]
         g("storeshift").argtypes = [ci, cd, cd, ci, cp, rp, cp, rp, cp, cp]
+        g("combine_arrays").argtypes = [ci, C.c_longlong, ci, ci, ci, cp, rp, cp, rp]
+        g("shiftvol").argtypes = [ci, pd, cd, cf, ci, cp, cp, rp, rp, ci, f32p, cf, f32p, cf]
+        g("correct3d").argtypes = [ci, rp, ci, ci, f32p]
         g("sheltest").argtypes = [ci, ci, cp, cp, rp, rp, cf, i32p, i32p, f64p, rp]
         g("sheltest").restype = ci
         g("ccp").argtypes = [ci, cf, cp, c64p, cf, ci, ci, cf, cf, pf, pf, pf, i32p, cp, ci, f32p]

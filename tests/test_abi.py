@@ -60,3 +60,36 @@ def test_lcg_matches_oracle():
         a = O.oracle().or_random(C.byref(s))
         b, v = api.lcg_random(v)
         assert v == s.value and np.float32(a) == np.float32(b)
+
+
+def test_fpart_from_mw_closed_form():
+    """FIND_FPART's MW != 0 branch (find_fpart.f:141-149), a host function.
+    find_fpart.f needs FFTW and is not built here, so the checks are what the
+    quantity is: the particle's share of the box volume.  A particle of
+    MW kDa at DALT Da/A^3 whose volume equals the box gives 1; FPART is linear
+    in MW and inversely proportional to the box volume; SSNR(1..JC) is divided
+    by it; MW = 0 (the iterative search) is refused."""
+    import ctypes as C
+
+    L = N.lib()
+
+    def run(nsam, mw, dalt, psize):
+        ssnr = np.arange(1, nsam // 2 + 3, dtype=np.float32)
+        f = C.c_float(0.0)
+        N.check(L.fr_fpart_from_mw(nsam, mw, dalt, psize, ssnr.ctypes.data, C.byref(f)), "fpart")
+        return f.value, ssnr
+
+    f, ssnr = run(10, 0.81, 0.81, 1.0)  # 810 Da at 0.81 Da/A^3 fill 1000 A^3 = 10^3 A^3
+    assert abs(f - 1.0) < 1e-6
+    f1, _ = run(64, 50.0, 0.81, 1.5)
+    f2, s2 = run(64, 100.0, 0.81, 1.5)
+    f3, _ = run(128, 100.0, 0.81, 1.5)
+    assert abs(f2 / f1 - 2.0) < 1e-5 and abs(f2 / f3 - 8.0) < 1e-5
+    assert abs(f2 - 100e3 / 0.81 / 96.0 ** 3) < 1e-6 * f2
+    jc = 64 // 2 + 1
+    assert np.allclose(s2[:jc], np.arange(1, jc + 1) / np.float32(f2), rtol=1e-6)
+    assert np.array_equal(s2[jc:], np.arange(jc + 1, 64 // 2 + 3, dtype=np.float32))
+    fneg, _ = run(64, -50.0, 0.81, 1.5)
+    assert fneg == f1  # FPART = ABS(FPART)
+    with pytest.raises(N.NativeError):
+        run(64, 0.0, 0.81, 1.5)

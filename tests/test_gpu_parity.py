@@ -440,3 +440,92 @@ def test_sigma_vs_oracle(ctx, nsam, signset):
         r64 = np.abs(got[k] - res["or64_"]) / np.abs(res["or64_"])
         print(f"nsam {nsam} item {k}: gpu {got[k]} or32 {res['or32_']} rel32 {r32} rel64 {r64}")
         assert r32[0] < 1e-3 and r32[1] < 1e-3 and r32[2] < 1e-4
+
+
+# ------------------------------------------------- volume finalisation
+@pytest.mark.parametrize("ifsc,imp,nvol1", [(0, 4, 2), (1, 3, 3), (0, 1, 1)])
+def test_combine_arrays_vs_oracle(ctx, ifsc, imp, nvol1):
+    """COMBINE_ARRAYS: the copies add in copy order, so the sums are the
+    oracle's bit for bit (the oracle is bit-exact to the reference's
+    combine_arrays.f); the counters as combine_arrays.f:53-67."""
+    L = O.oracle()
+    nsam, nn1 = 64, 72
+    nn3 = nn1 * nn1 * (nn1 // 2 + 1)
+    nvol2 = nvol1
+    a = (rng.standard_normal(nn3 * nvol1) + 1j * rng.standard_normal(nn3 * nvol1)).astype(np.complex64)
+    d = (rng.standard_normal(nn3 * nvol2) + 1j * rng.standard_normal(nn3 * nvol2)).astype(np.complex64)
+    s = rng.uniform(0, 3, nn3 * nvol1).astype(np.float32)
+    v = rng.uniform(0, 3, nn3 * nvol2).astype(np.float32)
+    m = max(imp, nvol1)
+    nksum1 = rng.integers(0, 10 ** 9, m).astype(np.int64)
+    ttd1, ttd2 = rng.uniform(0, 10, m), rng.uniform(0, 10, m)
+    ga, gs, gd, gv = (torch.from_numpy(x.copy()).to(DEV) for x in (a, s, d, v))
+    nk, sd, vd = ctx.combine_arrays(nsam, nn3, nvol1, nvol2, ifsc, ga, gs, gd, gv, imp, nksum1,
+                                    ttd1, ttd2, 7, 0.5, 0.25)
+    torch.cuda.synchronize()
+    L.or32_combine_arrays(nsam, nn3, nvol1, nvol2, ifsc, a, s, d, v)
+    onk, osd, ovd = C.c_int64(7), C.c_double(0.5), C.c_double(0.25)
+    L.or_combine_counters(ifsc, imp, nvol1, nksum1.ctypes.data_as(C.POINTER(C.c_int64)), ttd1,
+                          ttd2, C.byref(onk), C.byref(osd), C.byref(ovd))
+    for g, o in ((ga, a), (gs, s), (gd, d), (gv, v)):
+        assert np.array_equal(g.cpu().numpy().view(np.int32), o.view(np.int32))
+    assert (nk, sd, vd) == (onk.value, osd.value, ovd.value)
+
+
+@pytest.mark.parametrize("nsam,fpart,ifsc", [(64, 0.0, 0), (64, 0.3, 0), (128, 0.3, 1)])
+def test_shiftvol_vs_oracle(ctx, nsam, fpart, ifsc):
+    """SHIFTVOL (plain and Wiener-filtered final map): 1e-6 relative L2 against
+    the FP32 oracle (bit-exact to the reference's shiftvol.f); the support
+    (zeros outside RLIM and on empty shells) identical."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    s[::97] = 0.0
+    v[::97] = 0.0
+    nsh = nsam
+    pssnr = rng.uniform(0.1, 2.0, nsh).astype(np.float32)
+    pssnr[5] = 0.0
+    fsc = rng.uniform(-0.2, 0.99, nsh).astype(np.float32)
+    fsc[3], fsc[4] = 1.0, -1.0
+    rlim, iredun, fmask = 0.4, 14, 0.3
+    ga, gd, gs, gv = (torch.from_numpy(x.copy()).to(DEV) for x in (a, d, s, v))
+    std_g = ctx.shiftvol(nsam, 0.37, 0.41, rlim, iredun, ga, gd, gs, gv, ifsc, pssnr, fpart, fsc,
+                         fmask)
+    torch.cuda.synchronize()
+    std_o = C.c_double(0.37)
+    L.or32_shiftvol(nsam, C.byref(std_o), 0.41, rlim, iredun, a, d, s, v, ifsc, pssnr, fpart, fsc,
+                    fmask)
+    assert std_g == std_o.value
+    got = ga.cpu().numpy()
+    assert np.array_equal(got != 0, a != 0)
+    e = rel_l2(got, a)
+    exact = np.mean(got.view(np.int32) == a.view(np.int32))
+    print(f"SHIFTVOL nsam={nsam} FPART={fpart} IFSC={ifsc}: rel L2 {e:.1e}, bit-equal {exact:.4f}")
+    assert e < 1e-6
+    assert np.array_equal(gs.cpu().numpy().view(np.int32), s.view(np.int32))
+
+
+@pytest.mark.parametrize("nsam,interp,ipad", [(64, 1, 1), (64, 0, 1), (128, 1, 2)])
+def test_correct3d_vs_oracle(ctx, nsam, interp, ipad):
+    """CORRECT3D: 1e-6 relative L2 against the FP32 oracle (bit-exact to the
+    reference's correct3d.f and boxft_lut.f); padding columns untouched."""
+    L = O.oracle()
+    lut = np.zeros(180, np.float32)
+    L.or_calcsinc(lut, 180)
+    vol = rng.standard_normal(nsam * nsam * (nsam + 2)).astype(np.float32)
+    g = torch.from_numpy(vol.copy()).to(DEV)
+    ctx.correct3d(nsam, g, interp, ipad, lut)
+    torch.cuda.synchronize()
+    want = vol.copy()
+    L.or32_correct3d(nsam, want, interp, ipad, lut)
+    got = g.cpu().numpy()
+    e = rel_l2(got, want)
+    exact = np.mean(got.view(np.int32) == want.view(np.int32))
+    print(f"CORRECT3D nsam={nsam} INTERP={interp} IPAD={ipad}: rel L2 {e:.1e}, bit-equal {exact:.4f}")
+    assert e < 1e-6
+    pad = got.reshape(nsam, nsam, nsam + 2)[:, :, nsam:]
+    assert np.array_equal(pad, vol.reshape(nsam, nsam, nsam + 2)[:, :, nsam:])

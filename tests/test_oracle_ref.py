@@ -359,3 +359,107 @@ def test_va04a_minimises():
     x, f, _ = _or_va04a(L, [0.5, -0.5], [0.1, 0.1], 1, 100.0, 2, 50)
     # f = 0.7 a^2 + 1.4 b^2 + 0.4 a b with a = x1-0.1, b = x2-0.2: minimum at a = b = 0
     assert abs(x[0] - 0.1) < 1e-3 and abs(x[1] - 0.2) < 1e-3 and f < 1e-6
+
+
+# ------------------------------------------------- the volume finalisation
+def test_calcsinc_bitexact(ref_lib):
+    L = O.oracle()
+    a = np.zeros(180, np.float32)
+    b = np.zeros(180, np.float32)
+    L.or_calcsinc(a, 180)
+    ref_lib.calcsinc_(b.ctypes, O.byref_i(180))
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
+@pytest.mark.parametrize("nsam,interp,ipad", [(16, 1, 1), (32, 1, 1), (32, 0, 1), (32, 1, 2)])
+def test_correct3d_bitexact(ref_lib, nsam, interp, ipad):
+    L = O.oracle()
+    lut = np.zeros(180, np.float32)
+    L.or_calcsinc(lut, 180)
+    v = rng.standard_normal(nsam * nsam * (nsam + 2)).astype(np.float32)
+    a, b = v.copy(), v.copy()
+    L.or32_correct3d(nsam, a, interp, ipad, lut)
+    ref_lib.correct3d_(O.byref_i(nsam), lut.ctypes, b.ctypes, O.byref_i(interp), O.byref_i(ipad))
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    # the padding columns are untouched, the centre voxel (all |a| < 0.02) too
+    assert np.array_equal(a.reshape(nsam, nsam, nsam + 2)[:, :, nsam:],
+                          v.reshape(nsam, nsam, nsam + 2)[:, :, nsam:])
+    c = nsam // 2
+    assert a.reshape(nsam, nsam, nsam + 2)[c, c, c] == v.reshape(nsam, nsam, nsam + 2)[c, c, c]
+
+
+@pytest.mark.parametrize("nsam,fpart,ifsc", [(16, 0.0, 0), (32, 0.0, 0), (32, 0.3, 0), (32, 0.3, 1)])
+def test_shiftvol_bitexact(ref_lib, nsam, fpart, ifsc):
+    """Both branches of the final-map filter (plain and Wiener), with the
+    half-map sum (IFSC = 0) or without; FSC entries at +-1 take A1 = 1000 and
+    zero PSSNR entries zero their shells."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    a = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    d = (rng.standard_normal(n) + 1j * rng.standard_normal(n)).astype(np.complex64)
+    s = rng.uniform(0, 3, n).astype(np.float32)
+    v = rng.uniform(0, 3, n).astype(np.float32)
+    s[::97] = 0.0
+    v[::97] = 0.0
+    nsh = 2 * nsam
+    pssnr = rng.uniform(0.1, 2.0, nsh).astype(np.float32)
+    pssnr[5] = 0.0
+    fsc = rng.uniform(-0.2, 0.99, nsh).astype(np.float32)
+    fsc[3], fsc[4] = 1.0, -1.0
+    rlim, iredun, fmask = 0.4, 14, 0.3
+    ao, do, so = a.copy(), d.copy(), s.copy()
+    ar, dr, sr = a.copy(), d.copy(), s.copy()
+    std_o, std_r = C.c_double(0.37), C.c_double(0.37)
+    L.or32_shiftvol(nsam, C.byref(std_o), 0.41, rlim, iredun, ao, do, so, v, ifsc, pssnr, fpart,
+                    fsc, fmask)
+    asum = np.zeros(4, np.float32)
+    ref_lib.shiftvol_(O.byref_i(nsam), C.byref(std_r), C.byref(C.c_double(0.41)), O.byref_i(jc),
+                      O.byref_f(rlim), O.byref_i(nsam // 2), O.byref_i(iredun), ar.ctypes,
+                      dr.ctypes, sr.ctypes, v.ctypes, asum.ctypes, asum.ctypes, O.byref_i(0),
+                      O.byref_i(ifsc), pssnr.ctypes, O.byref_f(fpart), fsc.ctypes,
+                      O.byref_f(fmask))
+    assert std_o.value == std_r.value
+    assert np.array_equal(ao.view(np.int32), ar.view(np.int32))
+    assert np.array_equal(so.view(np.int32), sr.view(np.int32))
+    assert np.array_equal(do.view(np.int32), dr.view(np.int32))
+    assert np.count_nonzero(ao) > n // 10
+
+
+@pytest.mark.parametrize("ifsc,imp,nvol1", [(0, 1, 1), (0, 4, 2), (1, 3, 3)])
+def test_combine_arrays_bitexact(ref_lib, ifsc, imp, nvol1):
+    L = O.oracle()
+    nsam, nn1 = 16, 20
+    jc = nsam // 2 + 1
+    nn3 = nn1 * nn1 * (nn1 // 2 + 1)
+    nvol2 = nvol1
+    a = (rng.standard_normal(nn3 * nvol1) + 1j * rng.standard_normal(nn3 * nvol1)).astype(np.complex64)
+    d = (rng.standard_normal(nn3 * nvol2) + 1j * rng.standard_normal(nn3 * nvol2)).astype(np.complex64)
+    s = rng.uniform(0, 3, nn3 * nvol1).astype(np.float32)
+    v = rng.uniform(0, 3, nn3 * nvol2).astype(np.float32)
+    nksum1 = rng.integers(0, 10 ** 9, max(imp, nvol1)).astype(np.int64)
+    ttd1 = rng.uniform(0, 10, max(imp, nvol1))
+    ttd2 = rng.uniform(0, 10, max(imp, nvol1))
+    out = []
+    for which in ("oracle", "ref"):
+        aa, dd, ss, vv = a.copy(), d.copy(), s.copy(), v.copy()
+        nk, sd, vd = C.c_int64(7), C.c_double(0.5), C.c_double(0.25)
+        if which == "oracle":
+            L.or32_combine_arrays(nsam, nn3, nvol1, nvol2, ifsc, aa, ss, dd, vv)
+            L.or_combine_counters(ifsc, imp, nvol1, nksum1.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  ttd1, ttd2, C.byref(nk), C.byref(sd), C.byref(vd))
+        else:
+            z = np.zeros(4, np.float32)
+            iz = np.zeros(4, np.int32)
+            i = O.byref_i
+            ref_lib.combine_arrays_(i(nsam), z.ctypes, z.ctypes, z.ctypes, iz.ctypes, aa.ctypes,
+                                    ss.ctypes, C.byref(sd), dd.ctypes, vv.ctypes, C.byref(vd),
+                                    i(nn1), i(0), C.byref(nk), i(ifsc), i(imp), i(max(imp, nvol1)),
+                                    i(nvol1), i(nvol2), nksum1.ctypes, ttd1.ctypes, ttd2.ctypes,
+                                    i(nn1), i(nn3))
+        out.append((aa, ss, dd, vv, nk.value, sd.value, vd.value))
+    o, r = out
+    for x, y in zip(o[:4], r[:4]):
+        assert np.array_equal(x.view(np.int32), y.view(np.int32))
+    assert o[4:] == r[4:]
+    del jc
