@@ -93,3 +93,73 @@ def test_fpart_from_mw_closed_form():
     assert fneg == f1  # FPART = ABS(FPART)
     with pytest.raises(N.NativeError):
         run(64, 0.0, 0.81, 1.5)
+
+
+@pytest.mark.parametrize("call,message", [
+    ("a3d3", "A3D3 (frealign_hip): only NVOL = 1, IMP = 1 is supported"),
+    ("ctfapply", "CTFAPPLY (frealign_hip): the MASKENV envelope (XSTD < 0) is not supported"),
+    ("pinsert_trilin", "PINSERT_TRILIN (frealign_hip): Ewald-sphere insertion (IEWALD != 0) is not supported"),
+    ("prefine", "PREFINE (frealign_hip): only IVFLAG = 0 is supported"),
+])
+def test_percall_unsupported_cases_stop(tmp_path, call, message):
+    """The per-call symbols stop an unsupported case as the reference's STOPs
+    do: the message on stdout and exit status 1, before any device work (so
+    this runs without a GPU)."""
+    import os
+    import subprocess
+    import sys
+
+    script = f"""
+import ctypes as C
+import numpy as np
+from frealign_amd import _native as N
+lib = N.lib()
+i = lambda v: C.byref(C.c_int(v))
+f = lambda v: C.byref(C.c_float(v))
+buf = np.zeros(1 << 16, np.float32)
+d = np.zeros(64, np.float64)
+k = np.zeros(8, np.int64)
+which = {call!r}
+if which == "a3d3":
+    args = [i(1), i(0), i(32), i(1), f(12.0), buf.ctypes, i(1), i(1), f(1.0), buf.ctypes,
+            buf.ctypes] + [buf.ctypes] * 5 + [f(0.0)] * 6 + [f(5.0), f(40.0), buf.ctypes,
+            buf.ctypes, d.ctypes, buf.ctypes, buf.ctypes, d.ctypes, i(40), i(1), buf.ctypes, i(1),
+            i(1), f(0.0), i(0), i(1), i(8), buf.ctypes, buf.ctypes, i(0), k.ctypes, i(0), i(2),
+            i(2), i(1), i(1), f(1.0), f(1.0), f(1.5), C.c_char_p(b"C1 "), f(0.0), f(0.0), i(0),
+            i(0), k.ctypes, d.ctypes, d.ctypes, buf.ctypes, buf.ctypes, i(1), buf.ctypes,
+            buf.ctypes, buf.ctypes, C.byref(C.c_int(0)), buf.ctypes, i(40), i(40), buf.ctypes,
+            i(0), i(1), C.c_size_t(3)]
+    assert len(args) == 73
+    lib.a3d3_(*args)
+elif which == "ctfapply":
+    args = [i(32), buf.ctypes] + [f(0.0)] * 10 + [buf.ctypes, buf.ctypes, buf.ctypes, f(1.0),
+            f(10.0), f(6.0), f(49.0), f(169.0), f(0.0), buf.ctypes, buf.ctypes, buf.ctypes,
+            f(0.0), f(0.0), f(0.0), buf.ctypes, f(-1.0), i(0), f(0.0), f(0.0),
+            C.c_char_p(b"C1 "), buf.ctypes, buf.ctypes, buf.ctypes, i(1), buf.ctypes,
+            C.c_size_t(3)]
+    assert len(args) == 39
+    lib.ctfapply_(*args)
+elif which == "pinsert_trilin":
+    args = [i(32), f(1.0), i(100), buf.ctypes, buf.ctypes, f(1.0), buf.ctypes, buf.ctypes,
+            d.ctypes, f(0.0), f(0.0), buf.ctypes, buf.ctypes, buf.ctypes, buf.ctypes, buf.ctypes,
+            buf.ctypes, i(1), i(1), f(0.0), i(1), i(0), k.ctypes, i(17), buf.ctypes, f(0.0),
+            i(16), i(32 * 17), f(0.0), f(0.0), f(33.0), i(1), buf.ctypes]
+    assert len(args) == 33
+    lib.pinsert_trilin_(*args)
+else:
+    args = [i(32), i(1), f(1.0), i(1), i(8)] + [f(0.0)] * 6 + [buf.ctypes, f(0.0), i(0),
+            buf.ctypes, buf.ctypes, f(0.0), f(0.0), i(1)] + [f(0.0)] * 5 + [buf.ctypes, f(1.0)] + \\
+           [f(1.0)] * 8 + [buf.ctypes, buf.ctypes, buf.ctypes, i(0), buf.ctypes, buf.ctypes,
+            buf.ctypes, i(1), buf.ctypes, buf.ctypes, buf.ctypes, buf.ctypes, buf.ctypes, i(0),
+            f(0.0), f(0.0), f(0.3), f(0.3), f(0.01), C.c_char_p(b"C1 ")] + [f(0.0)] * 5 + \\
+           [buf.ctypes, buf.ctypes, buf.ctypes, C.c_size_t(3)]
+    assert len(args) == 63
+    lib.prefine_(*args)
+print("returned")
+"""
+    root = os.path.dirname(os.path.dirname(os.path.abspath(N.__file__)))
+    r = subprocess.run([sys.executable, "-c", script], capture_output=True, text=True, timeout=120,
+                       cwd=str(tmp_path), env={**os.environ, "PYTHONPATH": root})
+    assert r.returncode == 1, r.stdout + r.stderr
+    assert message in r.stdout
+    assert "returned" not in r.stdout
