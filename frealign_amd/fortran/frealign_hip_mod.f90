@@ -187,6 +187,53 @@ module frealign_hip
       integer(c_int) :: fr_storeshift
     end function
 
+    ! COMBINE_ARRAYS (combine_arrays.f:8-69): copies nn3 elements apart
+    function fr_combine_arrays(ctx, nsam, nn3, nvol1, nvol2, ifsc, d_a3df, d_s3df, d_d3df, &
+                               d_v3df, imp, nksum1, ttd1, ttd2, nksum, std, vtd) &
+        bind(C, name="fr_combine_arrays")
+      import :: c_int, c_long_long, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_s3df, d_d3df, d_v3df
+      integer(c_int), value :: nsam, nvol1, nvol2, ifsc, imp
+      integer(c_long_long), value :: nn3
+      integer(c_long_long), intent(in) :: nksum1(*)
+      real(c_double), intent(in) :: ttd1(*), ttd2(*)
+      integer(c_long_long) :: nksum
+      real(c_double) :: std, vtd
+      integer(c_int) :: fr_combine_arrays
+    end function
+
+    ! SHIFTVOL (shiftvol.f:8-77), in place; std is STD in/out
+    function fr_shiftvol(ctx, nsam, std, vtd, rlim, iredun, d_a3df, d_d3df, d_s3df, d_v3df, &
+                         ifsc, pssnr, fpart, fsc, fmask, nshell) bind(C, name="fr_shiftvol")
+      import :: c_int, c_float, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_d3df, d_s3df, d_v3df
+      integer(c_int), value :: nsam, iredun, ifsc, nshell
+      real(c_double) :: std
+      real(c_double), value :: vtd
+      real(c_float), value :: rlim, fpart, fmask
+      real(c_float), intent(in) :: pssnr(*), fsc(*)
+      integer(c_int) :: fr_shiftvol
+    end function
+
+    ! CORRECT3D (correct3d.f:19-47) on the padded real volume
+    function fr_correct3d(ctx, nsam, d_a3dv, interp, ipad, sinclut, nlut) &
+        bind(C, name="fr_correct3d")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3dv
+      integer(c_int), value :: nsam, interp, ipad, nlut
+      real(c_float), intent(in) :: sinclut(*)
+      integer(c_int) :: fr_correct3d
+    end function
+
+    ! FIND_FPART closed form for MW /= 0 (find_fpart.f:141-149)
+    function fr_fpart_from_mw(nsam, mw, dalt, psize, ssnr, fpart) bind(C, name="fr_fpart_from_mw")
+      import :: c_int, c_float
+      integer(c_int), value :: nsam
+      real(c_float), value :: mw, dalt, psize
+      real(c_float) :: ssnr(*), fpart
+      integer(c_int) :: fr_fpart_from_mw
+    end function
+
     ! PSEARCH -> CCP (psearch.f:8-121, ccp.f:8-275)
     function fr_search_layout(ctx, nsam, maxr1, maxr2, kc, kpad) bind(C, name="fr_search_layout")
       import :: c_int, c_ptr
