@@ -659,6 +659,15 @@ class Context:
         N.check(N.lib().fr_correct3d(self.h, nsam, _p(a3dv), interp, ipad, _p(lut), lut.size),
                 "fr_correct3d")
 
+    def mask3d(self, nsam: int, a3dv, rio: float, ric: float, halfw: float):
+        """MASK3D (mask3d.f) in place on the padded real volume; returns
+        (FMASK, AVE, STD)."""
+        fm, av, sd = C.c_float(), C.c_float(), C.c_float()
+        self._sync_stream()
+        N.check(N.lib().fr_mask3d(self.h, nsam, _p(a3dv), float(rio), float(ric), float(halfw),
+                                  C.byref(fm), C.byref(av), C.byref(sd)), "fr_mask3d")
+        return fm.value, av.value, sd.value
+
 
 @dataclass
 class HalfMaps:

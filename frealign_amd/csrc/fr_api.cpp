@@ -585,6 +585,65 @@ int fr_correct3d(fr_ctx* c, int nsam, float* d_a3dv, int interp, int ipad, const
   return launch_correct3d(*c, nsam, scal, interp, sinclut, d_a3dv);
 }
 
+int fr_mask3d(fr_ctx* c, int nsam, float* d_a3dv, float rio, float ric, float halfw,
+              float* fmask, float* ave, float* std_) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_a3dv && fmask && ave && std_, "null argument");
+  FR_CHECK(halfw != 0.0f, "HALFW must be non-zero");
+  // mask3d.f:24-36: radii in FP32 as the reference forms them
+  float ri = rio, ri4 = rio * rio, ri5 = ric * ric;
+  float rm = fminf(ri, (float)(nsam / 2) - 1.0f - halfw / 2.0f);
+  float rii = rm + halfw / 2.0f, rih = rm - halfw / 2.0f;
+  if (rih < 0.0f) rih = 0.0f;
+  float ri2 = rih * rih, ri3 = rii * rii;
+  const float xc = 1.0f + 0.5f * (float)nsam;
+  int err = 0;
+  double* part = (double*)c->slot(28, sizeof(double) * (5 * (size_t)kMask3dBlocks + 8), &err);
+  if (!part) return err;
+  double* out = part + 5 * (size_t)kMask3dBlocks;
+  double h[5];
+  if (int r = launch_mask3d_sums(*c, nsam, d_a3dv, xc, ri2, ri3, ri4, ri5, 1, part, out)) return r;
+  FR_HIP(hipMemcpyAsync(h, out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  // mask3d.f:49-55 (counts are exact in double)
+  double bg = h[1] != 0.0 ? h[0] / h[1] : h[0];
+  double sum1 = h[2], sum2 = h[3];
+  if (h[4] != 0.0) {
+    sum1 = sum1 / h[4];
+    sum2 = sum2 / h[4];
+  }
+  *ave = (float)sum1;
+  *std_ = (float)sqrt(fabs(sum2 - sum1 * sum1));
+  double sum3 = 0.0;
+  if (int r = launch_mask3d_apply(*c, nsam, d_a3dv, xc, ri2, ri3, rih, rii, halfw, bg, 0, part, out))
+    return r;
+  FR_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  sum3 = h[0];
+  if (ric > 0.0f) {  // mask3d.f:74-118: the inner mask, on the masked volume
+    ri = ric;
+    rm = fmaxf(ri, halfw / 2.0f);
+    rii = rm + halfw / 2.0f;
+    rih = rm - halfw / 2.0f;
+    ri2 = rih * rih;
+    ri3 = rii * rii;
+    if (int r = launch_mask3d_sums(*c, nsam, d_a3dv, xc, ri2, ri3, 0.f, 0.f, 0, part, out))
+      return r;
+    FR_HIP(hipMemcpyAsync(h, out, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FR_HIP(hipStreamSynchronize(c->stream));
+    bg = h[1] != 0.0 ? h[0] / h[1] : h[0];
+    if (int r = launch_mask3d_apply(*c, nsam, d_a3dv, xc, ri2, ri3, rih, rii, halfw, bg, 1, part,
+                                    out))
+      return r;
+    FR_HIP(hipMemcpyAsync(h, out, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    FR_HIP(hipStreamSynchronize(c->stream));
+    sum3 = sum3 + h[0];
+  }
+  *fmask = (float)(sum3 / nsam / nsam / nsam);  // mask3d.f:119
+  return 0;
+}
+
 int fr_fpart_from_mw(int nsam, float mw, float dalt, float psize, float* ssnr, float* fpart) {
   FR_CHECK(fpart && ssnr, "null argument");
   FR_CHECK(nsam >= 2, "bad NSAM");

@@ -10,7 +10,7 @@
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  static constexpr int kSlots = 28;
+  static constexpr int kSlots = 29;
   void* slots[kSlots] = {};
   size_t slot_bytes[kSlots] = {};
   void* slot(int k, size_t bytes, int* err);

@@ -1,10 +1,11 @@
 // fr_final.hip -- the volume finalisation after the particle loop: the sum of
 // the private insertion copies (COMBINE_ARRAYS, combine_arrays.f:33-52), the
-// final map filter (SHIFTVOL, shiftvol.f:8-77) and the real-space
-// interpolation correction (CORRECT3D, correct3d.f:19-47 with BOXFT_LUT,
-// boxft_lut.f:26-37).  All three are one pass over a half-volume (or the
-// padded real volume) with no reuse: HBM-bound elementwise kernels, one
-// thread per voxel, grid-strided.
+// final map filter (SHIFTVOL, shiftvol.f:8-77), the real-space interpolation
+// correction (CORRECT3D, correct3d.f:19-47 with BOXFT_LUT, boxft_lut.f:26-37)
+// and the spherical cosine-edge mask (MASK3D, mask3d.f:8-142).  Each is one
+// pass (MASK3D: two or four) over a half-volume or the padded real volume
+// with no reuse: HBM-bound kernels, one thread per voxel, grid-strided.  The
+// FP64 sums of MASK3D are block partials reduced in a fixed order.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -120,7 +121,122 @@ __global__ void __launch_bounds__(kThreads)
   }
 }
 
+// mask3d.f: the voxel's (X, Y, Z) - XC squared radius, in the reference's
+// FP32 order (no contraction)
+__device__ inline float mask3d_rad2(int nsam, size_t e, float xc) {
+#pragma clang fp contract(off)
+  const int l = (int)(e % nsam);
+  const size_t r = e / nsam;
+  const int m = (int)(r % nsam), n = (int)(r / nsam);
+  const float xl = (float)(l + 1) - xc, ym = (float)(m + 1) - xc, zn = (float)(n + 1) - xc;
+  return xl * xl + ym * ym + zn * zn;
+}
+
+// block sums in a fixed order (lane butterflies, then waves in order)
+template <int K>
+__device__ inline void block_sums(double (&v)[K], double* out) {
+  __shared__ double red[kThreads / 64][K];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < K; k++) red[wave][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x < K) {
+    double t = 0.0;
+    for (int w = 0; w < kThreads / 64; w++) t += red[w][threadIdx.x];
+    out[threadIdx.x] = t;
+  }
+}
+
+// mask3d.f:32-48 (and 83-95 with stats = false): the edge shell's sum and
+// count, and the RIC^2 <= r^2 <= RIO^2 sums of A and A^2 (A^2 in FP32)
+__global__ void __launch_bounds__(kThreads)
+    k_mask3d_sums(int nsam, const float* __restrict__ a, float xc, float ri2, float ri3, float ri4,
+                  float ri5, int stats, double* part) {
+  const size_t nv = (size_t)nsam * nsam * nsam;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (size_t e = blockIdx.x * (size_t)kThreads + threadIdx.x; e < nv;
+       e += (size_t)gridDim.x * kThreads) {
+    const float rad2 = mask3d_rad2(nsam, e, xc);
+    const float x = a[e % nsam + (size_t)(nsam + 2) * (e / nsam)];
+    if (rad2 > ri2 && rad2 < ri3) {
+      v[0] += (double)x;
+      v[1] += 1.0;
+    }
+    if (stats && rad2 <= ri4 && rad2 >= ri5) {
+      v[2] += (double)x;
+      v[3] += (double)(x * x);
+      v[4] += 1.0;
+    }
+  }
+  block_sums<5>(v, part + 5 * (size_t)blockIdx.x);
+}
+
+// mask3d.f:50-63 (outer, inner = 0) and 96-110 (inner = 1): the cosine edge,
+// A = A*EDGE + BG*(1-EDGE) with BG in double; FMASK's SUM3 terms per block
+__global__ void __launch_bounds__(kThreads)
+    k_mask3d_apply(int nsam, float* a, float xc, float ri2, float ri3, float rih, float rii,
+                   float halfw, double bg, int inner, double* part) {
+  const float pi = 3.1415926535897f;
+  const size_t nv = (size_t)nsam * nsam * nsam;
+  double v[1] = {0.0};
+  for (size_t e = blockIdx.x * (size_t)kThreads + threadIdx.x; e < nv;
+       e += (size_t)gridDim.x * kThreads) {
+#pragma clang fp contract(off)
+    const float rad2 = mask3d_rad2(nsam, e, xc);
+    const size_t id = e % nsam + (size_t)(nsam + 2) * (e / nsam);
+    float edge = 1.0f;
+    const bool in = inner ? rad2 < ri3 : rad2 > ri2;
+    if (in) {
+      const float rad = sqrtf(rad2);
+      if (rad > rih && rad < rii)
+        edge = (1.0f + cosf(pi * (inner ? rii - rad : rad - rih) / halfw)) / 2.0f;
+      else
+        edge = 0.0f;
+      const double t = bg * (double)(1.0f - edge);
+      a[id] = (float)((double)(a[id] * edge) + t);
+    }
+    if (!inner) v[0] += (double)(edge * edge);
+    else if (edge != 1.0f) v[0] += -1.0 + (double)(edge * edge);
+  }
+  block_sums<1>(v, part + blockIdx.x);
+}
+
+// the partial sums of nblk blocks (k values each), added in block order
+__global__ void __launch_bounds__(kThreads)
+    k_sum_parts(int nblk, int k, const double* __restrict__ part, double* out) {
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int b = threadIdx.x; b < nblk; b += kThreads)
+    for (int q = 0; q < k; q++) v[q] += part[(size_t)b * k + q];
+  block_sums<5>(v, out);
+}
+
 }  // namespace
+
+int launch_mask3d_sums(Ctx& c, int nsam, const float* a, float xc, float ri2, float ri3, float ri4,
+                       float ri5, int stats, double* part, double* out) {
+  hipLaunchKernelGGL(k_mask3d_sums, dim3(kMask3dBlocks), dim3(kThreads), 0, c.stream, nsam, a, xc,
+                     ri2, ri3, ri4, ri5, stats, part);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, c.stream, kMask3dBlocks, 5, part,
+                     out);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_mask3d_apply(Ctx& c, int nsam, float* a, float xc, float ri2, float ri3, float rih,
+                        float rii, float halfw, double bg, int inner, double* part, double* out) {
+  hipLaunchKernelGGL(k_mask3d_apply, dim3(kMask3dBlocks), dim3(kThreads), 0, c.stream, nsam, a, xc,
+                     ri2, ri3, rih, rii, halfw, bg, inner, part);
+  FR_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kThreads), 0, c.stream, kMask3dBlocks, 1, part,
+                     out);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
 
 int launch_combine(Ctx& c, int nsam, size_t stride, int ncopy, float* a, float* s) {
   if (ncopy <= 1) return 0;

@@ -43,6 +43,16 @@ constexpr int kSincLut = 90;
 int launch_correct3d(Ctx& c, int nsam, float scal, int interp, const float* sinclut,
                      float* a3dv);
 
+// MASK3D passes (mask3d.f) over the padded real volume: block partial sums
+// (kMask3dBlocks x 5 doubles in part) reduced in a fixed order into out[0..4]:
+// edge-shell sum and count, then (stats) the RIC..RIO sums of A, A^2, count.
+constexpr int kMask3dBlocks = 1024;
+int launch_mask3d_sums(Ctx& c, int nsam, const float* a, float xc, float ri2, float ri3, float ri4,
+                       float ri5, int stats, double* part, double* out);
+// the cosine edge with background bg; out[0] = this pass's SUM3 terms
+int launch_mask3d_apply(Ctx& c, int nsam, float* a, float xc, float ri2, float ri3, float rih,
+                        float rii, float halfw, double bg, int inner, double* part, double* out);
+
 // Particle preparation (lmain.f:459 r2c + ctfapply.f:8-161)
 struct PrepGlobals {
   float cs, wl, wgh1, wgh2, thetatr, rih, halfw, ri2, ri3, tx, ty;

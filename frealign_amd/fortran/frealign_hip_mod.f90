@@ -225,6 +225,17 @@ module frealign_hip
       integer(c_int) :: fr_correct3d
     end function
 
+    ! MASK3D (mask3d.f:8-142) on the padded real volume
+    function fr_mask3d(ctx, nsam, d_a3dv, rio, ric, halfw, fmask, ave, std) &
+        bind(C, name="fr_mask3d")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3dv
+      integer(c_int), value :: nsam
+      real(c_float), value :: rio, ric, halfw
+      real(c_float) :: fmask, ave, std
+      integer(c_int) :: fr_mask3d
+    end function
+
     ! FIND_FPART closed form for MW /= 0 (find_fpart.f:141-149)
     function fr_fpart_from_mw(nsam, mw, dalt, psize, ssnr, fpart) bind(C, name="fr_fpart_from_mw")
       import :: c_int, c_float

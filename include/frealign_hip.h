@@ -164,6 +164,14 @@ int fr_shiftvol(fr_ctx *ctx, int nsam, double *std_, double vtd, float rlim, int
 int fr_correct3d(fr_ctx *ctx, int nsam, float *d_a3dv, int interp, int ipad, const float *sinclut,
                  int nlut);
 
+/* Replaces MASK3D (mask3d.f:8-142), in place on the real volume d_a3dv (NSAM+2
+ * row pitch): cosine edge of width HALFW at RIO (clamped to NSAM/2-1-HALFW/2)
+ * about (1+NSAM/2)^3 with the edge shell's mean outside, and for ric > 0 the
+ * inner mask at RIC likewise.  Host outputs: FMASK (the mask's mean square),
+ * AVE and STD of the volume over RIC^2 <= r^2 <= RIO^2 (before masking). */
+int fr_mask3d(fr_ctx *ctx, int nsam, float *d_a3dv, float rio, float ric, float halfw,
+              float *fmask, float *ave, float *std_);
+
 /* FIND_FPART's closed form for MW != 0 (find_fpart.f:141-149): FPART =
  * |1000 MW / DALT / (PSIZE NSAM)^3| and SSNR(1..JC) /= FPART (host).  The MW = 0
  * search (3D FFTs, MASK3D, VOLMEASURE, FPART_CC) is not provided. */

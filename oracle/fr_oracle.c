@@ -770,6 +770,121 @@ void P(correct3d)(int nsam, R *a3dv, int interp, int ipad, const float *sinclut)
       }
 }
 
+/* ----------------------------------------------------------- mask3d.f */
+/* mask3d.f:8-142: cosine-edged spherical mask of the real volume (NSAM+2 row
+ * pitch) about (1+NSAM/2)^3, the outside set to the mean BG of the edge
+ * shell; with RIC > 0 an inner mask of radius RIC likewise.  BG and the
+ * SUM1/SUM2 statistics of RIC^2 <= r^2 <= RIO^2 are DOUBLE PRECISION sums
+ * in the reference's loop order (L outer, N inner). */
+static void mask3d_bg(int nsam, const R *a, float xc, float ri2, float ri3, double *bg) {
+  int nbg = 0;
+  *bg = 0.0;
+  for (int l = 1; l <= nsam; l++) {
+    float xl = (float)l - xc;
+    for (int m = 1; m <= nsam; m++) {
+      float ym = (float)m - xc;
+      for (int n = 1; n <= nsam; n++) {
+        float zn = (float)n - xc;
+        size_t id = (size_t)(l - 1) + (size_t)(nsam + 2) * ((size_t)(m - 1) + (size_t)nsam * (n - 1));
+        float rad2 = xl * xl + ym * ym + zn * zn;
+        if (rad2 > ri2 && rad2 < ri3) {
+          *bg = *bg + (double)a[id];
+          nbg++;
+        }
+      }
+    }
+  }
+  if (nbg != 0) *bg = *bg / nbg;
+}
+
+void P(mask3d)(int nsam, R *a, float rio, float ric, float halfw, float *fmask, float *ave,
+               float *std) {
+  const float pi = 3.1415926535897f;
+  float ri = rio, ri4 = rio * rio, ri5 = ric * ric;
+  float rm = fminf(ri, (float)(nsam / 2) - 1.0f - halfw / 2.0f);
+  float rii = rm + halfw / 2.0f, rih = rm - halfw / 2.0f;
+  if (rih < 0.0f) rih = 0.0f;
+  float ri2 = rih * rih, ri3 = rii * rii;
+  float xc = 1.0f + 0.5f * (float)nsam;
+  double bg = 0.0, sum1 = 0.0, sum2 = 0.0, sum3 = 0.0;
+  int nbg = 0, na = 0;
+  for (int l = 1; l <= nsam; l++) {
+    float xl = (float)l - xc;
+    for (int m = 1; m <= nsam; m++) {
+      float ym = (float)m - xc;
+      for (int n = 1; n <= nsam; n++) {
+        float zn = (float)n - xc;
+        size_t id = (size_t)(l - 1) + (size_t)(nsam + 2) * ((size_t)(m - 1) + (size_t)nsam * (n - 1));
+        float rad2 = xl * xl + ym * ym + zn * zn;
+        if (rad2 > ri2 && rad2 < ri3) {
+          bg = bg + (double)a[id];
+          nbg++;
+        }
+        if (rad2 <= ri4 && rad2 >= ri5) {
+          sum1 = sum1 + (double)a[id];
+          sum2 = sum2 + (double)(a[id] * a[id]);
+          na++;
+        }
+      }
+    }
+  }
+  if (nbg != 0) bg = bg / nbg;
+  if (na != 0) {
+    sum1 = sum1 / na;
+    sum2 = sum2 / na;
+  }
+  *ave = (float)sum1;
+  *std = (float)sqrt(fabs(sum2 - sum1 * sum1));
+  for (int l = 1; l <= nsam; l++) {
+    float xl = (float)l - xc;
+    for (int m = 1; m <= nsam; m++) {
+      float ym = (float)m - xc;
+      for (int n = 1; n <= nsam; n++) {
+        float zn = (float)n - xc;
+        size_t id = (size_t)(l - 1) + (size_t)(nsam + 2) * ((size_t)(m - 1) + (size_t)nsam * (n - 1));
+        float rad2 = xl * xl + ym * ym + zn * zn;
+        float edge = 1.0f;
+        if (rad2 > ri2) {
+          float rad = sqrtf(rad2);
+          if (rad > rih && rad < rii) edge = (1.0f + cosf(pi * (rad - rih) / halfw)) / 2.0f;
+          else edge = 0.0f;
+          a[id] = (R)((double)(a[id] * (R)edge) + bg * (double)(1.0f - edge));
+        }
+        sum3 = sum3 + (double)(edge * edge);
+      }
+    }
+  }
+  if (ric > 0.0f) {
+    ri = ric;
+    rm = fmaxf(ri, halfw / 2.0f);
+    rii = rm + halfw / 2.0f;
+    rih = rm - halfw / 2.0f;
+    ri2 = rih * rih;
+    ri3 = rii * rii;
+    mask3d_bg(nsam, a, xc, ri2, ri3, &bg);
+    for (int l = 1; l <= nsam; l++) {
+      float xl = (float)l - xc;
+      for (int m = 1; m <= nsam; m++) {
+        float ym = (float)m - xc;
+        for (int n = 1; n <= nsam; n++) {
+          float zn = (float)n - xc;
+          size_t id = (size_t)(l - 1) + (size_t)(nsam + 2) * ((size_t)(m - 1) + (size_t)nsam * (n - 1));
+          float rad2 = xl * xl + ym * ym + zn * zn;
+          float edge = 1.0f;
+          if (rad2 < ri3) {
+            float rad = sqrtf(rad2);
+            if (rad > rih && rad < rii) edge = (1.0f + cosf(pi * (rii - rad) / halfw)) / 2.0f;
+            else edge = 0.0f;
+            a[id] = (R)((double)(a[id] * (R)edge) + bg * (double)(1.0f - edge));
+          }
+          if (edge != 1.0f) sum3 = sum3 - 1.0 + (double)(edge * edge);
+        }
+      }
+    }
+  }
+  *fmask = (float)(sum3 / nsam / nsam / nsam);
+}
+
 /* -------------------------------------------------------- sheltest.f */
 int P(sheltest)(int nsam, int istp, const R *a3df, const R *d3df, const R *s3df, const R *v3df,
                 float fmask, int *npix, int *npixt, double *dfsc, R *pssnr) {

@@ -149,6 +149,9 @@ void or_irfft2_d(int nsam, double *data);
   /* correct3d.f:19-47 with boxft_lut.f:26-37; a3dv has NSAM+2 row pitch */     \
   void P##correct3d(int nsam, R *a3dv, int interp, int ipad,                    \
                     const float *sinclut);                                      \
+  /* mask3d.f:8-142, in place on the padded real volume; FMASK, AVE, STD */    \
+  void P##mask3d(int nsam, R *a3dv, float rio, float ric, float halfw,          \
+                 float *fmask, float *ave, float *std);                         \
   /* sheltest.f:8-122 (FSC part, NNSTAT=0); returns nstp */                     \
   int P##sheltest(int nsam, int istp, const R *a3df, const R *d3df,             \
                   const R *s3df, const R *v3df, float fmask, int *npix,         \

@@ -100,6 +100,7 @@ def _declare_oracle(L):
         g("combine_arrays").argtypes = [ci, C.c_longlong, ci, ci, ci, cp, rp, cp, rp]
         g("shiftvol").argtypes = [ci, pd, cd, cf, ci, cp, cp, rp, rp, ci, f32p, cf, f32p, cf]
         g("correct3d").argtypes = [ci, rp, ci, ci, f32p]
+        g("mask3d").argtypes = [ci, rp, cf, cf, cf, pf, pf, pf]
         g("sheltest").argtypes = [ci, ci, cp, cp, rp, rp, cf, i32p, i32p, f64p, rp]
         g("sheltest").restype = ci
         g("ccp").argtypes = [ci, cf, cp, c64p, cf, ci, ci, cf, cf, pf, pf, pf, i32p, cp, ci, f32p]

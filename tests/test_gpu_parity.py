@@ -529,3 +529,29 @@ def test_correct3d_vs_oracle(ctx, nsam, interp, ipad):
     assert e < 1e-6
     pad = got.reshape(nsam, nsam, nsam + 2)[:, :, nsam:]
     assert np.array_equal(pad, vol.reshape(nsam, nsam, nsam + 2)[:, :, nsam:])
+
+
+@pytest.mark.parametrize("nsam,rio,ric,halfw", [(64, 24.0, 0.0, 6.0), (64, 40.0, 6.0, 6.0),
+                                                (128, 50.0, 0.0, 6.0)])
+def test_mask3d_vs_oracle(ctx, nsam, rio, ric, halfw):
+    """MASK3D: the masked volume 1e-6 relative L2 against the FP32 oracle
+    (bit-exact to the reference's mask3d.f); FMASK, AVE, STD to 1e-6 relative
+    (FP64 sums in a different order); padding columns untouched."""
+    L = O.oracle()
+    vol = (rng.standard_normal(nsam * nsam * (nsam + 2)) + 0.3).astype(np.float32)
+    g = torch.from_numpy(vol.copy()).to(DEV)
+    fm, av, sd = ctx.mask3d(nsam, g, rio, ric, halfw)
+    torch.cuda.synchronize()
+    want = vol.copy()
+    fo, ao, so = C.c_float(), C.c_float(), C.c_float()
+    L.or32_mask3d(nsam, want, rio, ric, halfw, C.byref(fo), C.byref(ao), C.byref(so))
+    got = g.cpu().numpy()
+    e = rel_l2(got, want)
+    exact = np.mean(got.view(np.int32) == want.view(np.int32))
+    print(f"MASK3D nsam={nsam} RIO={rio} RIC={ric}: rel L2 {e:.1e}, bit-equal {exact:.4f}; "
+          f"FMASK {fm} / {fo.value}, AVE {av} / {ao.value}, STD {sd} / {so.value}")
+    assert e < 1e-6
+    for x, y in ((fm, fo.value), (av, ao.value), (sd, so.value)):
+        assert abs(x - y) <= 1e-6 * abs(y)
+    pad = got.reshape(nsam, nsam, nsam + 2)[:, :, nsam:]
+    assert np.array_equal(pad, vol.reshape(nsam, nsam, nsam + 2)[:, :, nsam:])

@@ -463,3 +463,21 @@ def test_combine_arrays_bitexact(ref_lib, ifsc, imp, nvol1):
         assert np.array_equal(x.view(np.int32), y.view(np.int32))
     assert o[4:] == r[4:]
     del jc
+
+
+@pytest.mark.parametrize("nsam,rio,ric,halfw", [(16, 6.0, 0.0, 3.0), (32, 12.0, 0.0, 6.0),
+                                                (32, 20.0, 4.0, 6.0), (32, 9.0, 2.0, 1.0)])
+def test_mask3d_bitexact(ref_lib, nsam, rio, ric, halfw):
+    """MASK3D: outer cosine edge (and the inner mask for RIC > 0), the edge
+    shell's mean outside, FMASK/AVE/STD; RIO beyond NSAM/2 is clamped."""
+    L = O.oracle()
+    v = (rng.standard_normal(nsam * nsam * (nsam + 2)) + 0.3).astype(np.float32)
+    a, b = v.copy(), v.copy()
+    fo, ao, so = C.c_float(), C.c_float(), C.c_float()
+    fr, ar, sr = C.c_float(), C.c_float(), C.c_float()
+    L.or32_mask3d(nsam, a, rio, ric, halfw, C.byref(fo), C.byref(ao), C.byref(so))
+    ref_lib.mask3d_(O.byref_i(nsam), b.ctypes, O.byref_f(rio), O.byref_f(ric), O.byref_f(halfw),
+                    C.byref(fr), C.byref(ar), C.byref(sr))
+    assert np.array_equal(a.view(np.int32), b.view(np.int32))
+    assert (fo.value, ao.value, so.value) == (fr.value, ar.value, sr.value)
+    assert 0.0 < fo.value < 1.0 and not np.array_equal(a, v)
