@@ -74,6 +74,8 @@ SIGNATURES = {
                            C.POINTER(C.c_longlong), pd, pd], ci),
     "fr_shiftvol": ([vp, ci, pd, cd, cf, ci, vp, vp, vp, vp, ci, vp, cf, vp, cf, ci], ci),
     "fr_correct3d": ([vp, ci, vp, ci, ci, vp, ci], ci),
+    "fr_fft3d_r2c": ([vp, ci, vp], ci),
+    "fr_fft3d_c2r": ([vp, ci, vp], ci),
     "fr_mask3d": ([vp, ci, vp, cf, cf, cf, C.POINTER(C.c_float), C.POINTER(C.c_float),
                    C.POINTER(C.c_float)], ci),
     "fr_fpart_from_mw": ([ci, cf, cf, cf, vp, C.POINTER(C.c_float)], ci),

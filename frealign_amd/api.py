@@ -659,6 +659,12 @@ class Context:
         N.check(N.lib().fr_correct3d(self.h, nsam, _p(a3dv), interp, ipad, _p(lut), lut.size),
                 "fr_correct3d")
 
+    def fft3d(self, nsam: int, vol, forward: bool):
+        """3D r2c (forward) or c2r transform of one padded volume, in place."""
+        self._sync_stream()
+        fn = N.lib().fr_fft3d_r2c if forward else N.lib().fr_fft3d_c2r
+        N.check(fn(self.h, nsam, _p(vol)), "fr_fft3d")
+
     def mask3d(self, nsam: int, a3dv, rio: float, ric: float, halfw: float):
         """MASK3D (mask3d.f) in place on the padded real volume; returns
         (FMASK, AVE, STD)."""

@@ -299,6 +299,20 @@ int fr_fft2d_c2r(fr_ctx* c, int nsam, int nimg, float* d_data) {
   return fft_c2r_2d(*c, nsam, nimg, d_data, (size_t)nsam * (nsam + 2));
 }
 
+int fr_fft3d_r2c(fr_ctx* c, int nsam, float* d_data) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_data != nullptr, "null argument");
+  return fft_r2c_3d(*c, nsam, d_data);
+}
+
+int fr_fft3d_c2r(fr_ctx* c, int nsam, float* d_data) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(d_data != nullptr, "null argument");
+  return fft_c2r_3d(*c, nsam, d_data);
+}
+
 // ------------------------------------------------------- particle preparation
 int fr_prep_batch(fr_ctx* c, int nsam, int nb, float* d_data, const float* d_ppar,
                   const float* d_pssnr, const float* gpar, int ictf, float* d_ctff,

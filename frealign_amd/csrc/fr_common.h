@@ -82,6 +82,9 @@ enum ProfName {
 // columns ignored after the column pass).
 int fft_r2c_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride);
 int fft_c2r_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride);
+// one padded volume (NSAM+2 row pitch), in place, unnormalised
+int fft_r2c_3d(Ctx& c, int nsam, float* data);
+int fft_c2r_3d(Ctx& c, int nsam, float* data);
 bool fft_supported(int nsam);
 
 }  // namespace fr

@@ -242,6 +242,32 @@ __global__ void __launch_bounds__(kThreads) k_cols(float* data, Plan pl, size_t 
   }
 }
 
+// the third axis of a 3D transform: ncols columns of length n, element y of
+// column q at complex index y * rs + q (rs = JC*NSAM for a padded volume)
+__global__ void __launch_bounds__(kThreads) k_cols_z(float* data, Plan pl, int ncols, size_t rs,
+                                                     int cw, int sign,
+                                                     const float2* __restrict__ tw) {
+  extern __shared__ float2 smem[];
+  const int n = pl.n, ld = n + 1;
+  float2* buf = smem;
+  float2* tmp = smem + cw * ld;
+  const int x0 = blockIdx.x * cw;
+  const int ncol = min(cw, ncols - x0);
+  float2* base = reinterpret_cast<float2*>(data);
+  for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
+    int y = e / cw, c = e - y * cw;
+    float2 v = f2(0.f, 0.f);
+    if (c < ncol) v = base[(size_t)y * rs + x0 + c];
+    buf[c * ld + y] = v;
+  }
+  __syncthreads();
+  float2* res = lds_fft(buf, tmp, pl, ld, cw, sign, tw);
+  for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
+    int y = e / cw, c = e - y * cw;
+    if (c < ncol) base[(size_t)y * rs + x0 + c] = res[c * ld + y];
+  }
+}
+
 int row_seqs(int n) {
   int s = 2048 / n;
   return s < 1 ? 1 : s;
@@ -502,6 +528,37 @@ int fft_c2r_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride) {
   if (nimg <= 0) return 0;
   if (int r = cols_pass(c, nsam, nimg, data, img_stride, +1)) return r;
   return rows_pass(c, nsam, nimg, data, img_stride, false);
+}
+
+static int z_pass(Ctx& c, int nsam, float* data, int sign) {
+  Plan pl;
+  FR_CHECK(make_plan(nsam, &pl), "FFT length %d not supported", nsam);
+  const float2* tw = c.twiddles(nsam);
+  FR_CHECK(tw != nullptr, "twiddle table allocation failed");
+  const int jc = nsam / 2 + 1;
+  const int ncols = jc * nsam;
+  int cw = 2048 / nsam;
+  if (cw < 1) cw = 1;
+  const size_t lds = sizeof(float2) * 2 * cw * (nsam + 1);
+  dim3 grid((ncols + cw - 1) / cw);
+  hipLaunchKernelGGL(k_cols_z, grid, dim3(kThreads), lds, c.stream, data, pl, ncols,
+                     (size_t)ncols, cw, sign, tw);
+  FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// 3D transforms of a padded volume (NSAM+2 row pitch, NSAM^2 rows): the 2D
+// transform of every z-slice, then the z axis
+int fft_r2c_3d(Ctx& c, int nsam, float* data) {
+  const size_t img = (size_t)nsam * (nsam + 2);
+  if (int r = fft_r2c_2d(c, nsam, nsam, data, img)) return r;
+  return z_pass(c, nsam, data, -1);
+}
+
+int fft_c2r_3d(Ctx& c, int nsam, float* data) {
+  const size_t img = (size_t)nsam * (nsam + 2);
+  if (int r = z_pass(c, nsam, data, +1)) return r;
+  return fft_c2r_2d(c, nsam, nsam, data, img);
 }
 
 }  // namespace fr

@@ -225,6 +225,21 @@ module frealign_hip
       integer(c_int) :: fr_correct3d
     end function
 
+    ! 3D FFTs of one padded volume (FFTW_PLANS(3), (4)), in place
+    function fr_fft3d_r2c(ctx, nsam, d_data) bind(C, name="fr_fft3d_r2c")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_data
+      integer(c_int), value :: nsam
+      integer(c_int) :: fr_fft3d_r2c
+    end function
+
+    function fr_fft3d_c2r(ctx, nsam, d_data) bind(C, name="fr_fft3d_c2r")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_data
+      integer(c_int), value :: nsam
+      integer(c_int) :: fr_fft3d_c2r
+    end function
+
     ! MASK3D (mask3d.f:8-142) on the padded real volume
     function fr_mask3d(ctx, nsam, d_a3dv, rio, ric, halfw, fmask, ave, std) &
         bind(C, name="fr_mask3d")

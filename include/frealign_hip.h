@@ -77,6 +77,13 @@ float fr_lcg_random(int *iran);
 int fr_fft2d_r2c(fr_ctx *ctx, int nsam, int nimg, float *d_data);
 int fr_fft2d_c2r(fr_ctx *ctx, int nsam, int nimg, float *d_data);
 
+/* Replaces the 3D plans FFTW_PLANS(3), (4) (frealign_v9.f:91,490; used
+ * through FFTW_FWD / FFTW_BWD on the maps, e.g. frealign_v9.f:825,839): one
+ * padded volume (NSAM+2 row pitch, NSAM^2 rows), in
+ * place, unnormalised. */
+int fr_fft3d_r2c(fr_ctx *ctx, int nsam, float *d_data);
+int fr_fft3d_c2r(fr_ctx *ctx, int nsam, float *d_data);
+
 /* ---------------------------------------------------- particle preparation
  * Replaces, per particle, the r2c FFT in LMAIN (lmain.f:459) and CTFAPPLY
  * (ctfapply.f:8-161; called at lmain.f:509-514) with MASKCOS and PWEIGHT.
