@@ -576,3 +576,66 @@ def test_fft3d_r2c_c2r(ctx, nsam):
     torch.cuda.synchronize()
     back = d.cpu().numpy()[:, :, :nsam] / float(nsam) ** 3
     assert rel_l2(back, vol) < 2e-6
+
+
+def test_finalisation_chain_fsc(ctx):
+    """The half-map finalisation of frealign_v9.f:809-869 end to end on the
+    device -- STORESHIFT, 3D c2r, CORRECT3D, MASK3D, 3D r2c, SHELTEST -- against
+    the same chain through the FP32 oracle (numpy's FP64 transforms standing
+    in for FFTW).  The per-shell FSC agrees to 1e-5 and so does FSC = 0.143's
+    crossing shell."""
+    L = O.oracle()
+    nsam = 64
+    jc = nsam // 2 + 1
+    n = nsam * nsam * jc
+    # two half-maps sharing a signal that falls off with resolution
+    zz, yy, xx = np.meshgrid(np.fft.fftfreq(nsam), np.fft.fftfreq(nsam), np.fft.rfftfreq(nsam),
+                             indexing="ij")
+    env = np.exp(-((xx ** 2 + yy ** 2 + zz ** 2) * nsam ** 2) / 300.0).reshape(-1)
+    sig = env * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    a = (sig + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    d = (sig + 0.3 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))).astype(np.complex64)
+    s = rng.uniform(0.5, 3, n).astype(np.float32)
+    v = rng.uniform(0.5, 3, n).astype(np.float32)
+    std, vtd, iredun = 0.37, 0.41, 1
+    lut = np.zeros(180, np.float32)
+    L.or_calcsinc(lut, 180)
+    rio, ric, halfw = 24.0, 0.0, 6.0
+    # device chain
+    t = lambda x: torch.from_numpy(x.copy()).to(DEV)  # noqa: E731
+    b, c = ctx.storeshift(nsam, std, vtd, iredun, t(a), t(s), t(d), t(v))
+    for vol in (b, c):
+        ctx.fft3d(nsam, vol, False)
+        ctx.correct3d(nsam, vol, 1, 1, lut)
+        ctx.mask3d(nsam, vol, rio, ric, halfw)
+        ctx.fft3d(nsam, vol, True)
+    got = ctx.sheltest(nsam, b, c, t(s), t(v))
+    torch.cuda.synchronize()
+    # oracle chain
+    bo, co = np.zeros(n, np.complex64), np.zeros(n, np.complex64)
+    L.or32_storeshift(nsam, std, vtd, iredun, a, s, d, v, bo, co)
+    outs = []
+    for h in (bo, co):
+        real = np.fft.irfftn(h.reshape(nsam, nsam, jc).astype(np.complex128), s=(nsam,) * 3,
+                             axes=(0, 1, 2))
+        pad = np.zeros((nsam, nsam, nsam + 2), np.float32)
+        pad[:, :, :nsam] = (real * float(nsam) ** 3).astype(np.float32)  # FFTW_BWD: unnormalised
+        pad = pad.reshape(-1)
+        L.or32_correct3d(nsam, pad, 1, 1, lut)
+        fm, av, sd = C.c_float(), C.c_float(), C.c_float()
+        L.or32_mask3d(nsam, pad, rio, ric, halfw, C.byref(fm), C.byref(av), C.byref(sd))
+        spec = np.fft.rfftn(pad.reshape(nsam, nsam, nsam + 2)[:, :, :nsam].astype(np.float64))
+        outs.append(spec.astype(np.complex64).reshape(-1))
+    npo, npto = np.zeros(jc + 2, np.int32), np.zeros(jc + 2, np.int32)
+    dfo, pso = np.zeros(jc + 2), np.zeros(jc + 2, np.float32)
+    nst = L.or32_sheltest(nsam, 1, outs[0], outs[1], s, v, 1.0, npo, npto, dfo, pso)
+    fsc_g, fsc_o = got["fsc"], dfo[:nst]
+    err = np.max(np.abs(fsc_g - fsc_o))
+    # the first shell past DC (entry 0) whose FSC falls below 0.143
+    cross = lambda f: 1 + (int(np.argmax(f[1:] < 0.143)) if np.any(f[1:] < 0.143)  # noqa: E731
+                           else len(f) - 1)
+    print(f"finalisation chain: max |FSC_gpu - FSC_oracle| = {err:.1e}; FSC(0.143) shell "
+          f"{cross(fsc_g)} / {cross(fsc_o)}; FSC[1:6] {np.round(fsc_g[1:6], 4)}")
+    assert err < 1e-5
+    assert cross(fsc_g) == cross(fsc_o)
+    assert fsc_g[1] > 0.5 and 0 < cross(fsc_g) < nst
