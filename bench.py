@@ -24,7 +24,9 @@ reference (its time is reported and amortised separately).
 
 Also reported: insertion GB/s (the refined particles inserted into the two
 half-maps with the 14 D7 copies), the FSC(0.143) resolution of the maps
-built from the learned and from the plain FP32 searches, the roofline of the
+built from the learned and from the plain FP32 searches, the wall time of the
+device finalisation of the learned half-maps (STORESHIFT, 3D FFTs,
+CORRECT3D, MASK3D, SHELTEST at 256^3) with its masked FSC, the roofline of the
 dominant kernel from HIP events on its own stream, and a CPU baseline (the
 oracle's port of PSEARCH/CCP on a bounded sample, 1 thread).
 """
@@ -298,6 +300,8 @@ def main():
     ins_gbps = ins_bytes / (ins_ms / 1e3) / 1e9
     log(f"insertion: {ins_gbps:.1f} GB/s")
     res_l = fsc_resolution(ctx, vols_l, sym)
+    fin_ms, res_l_masked = finalisation(ctx, vols_l, sym)
+    log(f"finalisation (STORESHIFT .. SHELTEST, 256^3): {fin_ms:.1f} ms")
     del vols_l
     vols_32, _ = reconstruct(res32[:nfsc])
     res_p = fsc_resolution(ctx, vols_32, sym)
@@ -381,6 +385,9 @@ def main():
         "training": {"samples": rep.samples, "seconds": rep.seconds, "final_loss": rep.final_loss,
                      "heldout_top1": rep.heldout_top1, f"heldout_top{cfg.topk}": rep.heldout_topk,
                      "particles_per_s_with_training_amortised_over_20000": amort},
+        "finalisation_256": {"ms": fin_ms, "fsc0143_A_masked_learned": res_l_masked,
+                             "steps": "STORESHIFT, 3D c2r, CORRECT3D, MASK3D, 3D r2c per half-map, "
+                                      "SHELTEST"},
         "insertion_GBps": ins_gbps,
         "insertion_frac_of_hbm_peak": ins_gbps * 1e9 / PEAK_HBM,
         "kernel_ms_per_step": {"search_gemm_lists": gemm_ms / max(args.steps, 1),
@@ -494,6 +501,41 @@ def fsc_resolution(ctx, vols, sym) -> float:
         if fsc[k] < 0.143:
             return float(nsam * PSIZE / k)
     return float(2.0 * PSIZE)
+
+
+def finalisation(ctx, vols, sym):
+    """The half-map finalisation of frealign_v9.f:809-869 on the device:
+    STORESHIFT, then per half-map the 3D c2r, CORRECT3D (trilinear sinc^2),
+    MASK3D at the particle radius, the 3D r2c, and SHELTEST.  Returns its
+    wall time (ms, host syncs included) and the FSC(0.143) resolution (A) of
+    the masked half-maps."""
+    import torch
+
+    nsam = NSAM
+    nk = float(vols.nksum.sum().item())
+    ttd = vols.ttd.cpu().numpy()
+    std, vtd = ttd[0] / nk * 0.1, ttd[1] / nk * 0.1
+    j = np.arange(1, 181, dtype=np.float32)
+    x = j * np.float32(3.1415926535897) / np.float32(180.0)
+    lut = (np.sin(x) / x).astype(np.float32)  # CALCSINC (calcsinc.f:17-19)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b, c = ctx.storeshift(nsam, std, vtd, sym[3], vols.num[0], vols.ctf2[0], vols.num[1],
+                          vols.ctf2[1])
+    for vol in (b, c):
+        ctx.fft3d(nsam, vol, False)
+        ctx.correct3d(nsam, vol, 1, 1, lut)
+        ctx.mask3d(nsam, vol, RI, 0.0, 6.0)
+        ctx.fft3d(nsam, vol, True)
+    fsc = ctx.sheltest(nsam, b, c, vols.ctf2[0], vols.ctf2[1])["fsc"]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3
+    res = float(2.0 * PSIZE)
+    for k in range(2, len(fsc)):
+        if fsc[k] < 0.143:
+            res = float(nsam * PSIZE / k)
+            break
+    return ms, res
 
 
 def cpu_baseline(nviews, parts, a3df, ctff_for_cpu, optics, gpar, maxr1, maxr2, norient_views,
