@@ -323,7 +323,7 @@ def main():
                 "launch_ms": cc3m_avg_s * 1e3,
                 "algorithmic_flops_per_launch": flops}
     else:
-        flops = SEP_FLOPS * ncand * B
+        flops = SEP_FLOPS * ncand  # ncand: listed candidates summed over the batch
         achieved = flops / gemm_avg_s
         roof = {"kernel": "k_search_sep (candidate lists)", "bound": "mfma",
                 "achieved": achieved / 1e12, "peak": PEAK_BF16 / 1e12, "unit": "TFLOP/s",

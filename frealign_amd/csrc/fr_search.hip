@@ -659,14 +659,18 @@ __global__ void __launch_bounds__(kThreads)
       __syncthreads();
     }
   }
+  // PSEARCH starts every TESTPAR(6,K) at 0.0 and admits a view only when
+  // CC > TESTPAR(6,K) (psearch.f:42-45,89-90): a view scoring <= 0 never
+  // enters the list.  Such slots, like empty columns, are reported as
+  // orientation -1, row -1 and TESTPAR's start value 0.0.
   for (int q = tid; q < topk; q += kThreads) {
     int t = si[q];
-    bool ok = t >= 0 && t < len;
+    bool ok = t >= 0 && t < len && sv[q] > 0.0f;
     int o = ok ? (cand ? cand[base + t] : t) : -1;
     if (o >= nbank) o = -1;  // a list entry outside the bank is an empty column
     top_o[(size_t)p * topk + q] = o;
     top_row[(size_t)p * topk + q] = o >= 0 ? best_row[base + t] : -1;
-    top_s[(size_t)p * topk + q] = sv[q];
+    top_s[(size_t)p * topk + q] = o >= 0 ? sv[q] : 0.0f;
   }
 }
 

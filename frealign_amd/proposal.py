@@ -234,6 +234,7 @@ class LearnedProposal:
         rng = np.random.default_rng(seed)
         steps = max(1, nsamples // batch)
         t0 = time.perf_counter()
+        t_last = t0
         log = []
         loss_v = float("nan")
         for st in range(steps):
@@ -242,7 +243,11 @@ class LearnedProposal:
             img, label, _ = self.make_batch(rng, batch, optics, ri, seed * 1_000_003 + st,
                                             snr_range)
             loss = self.net.train_step(img, label, lr_t)
-            if (st + 1) % log_every == 0 or st + 1 == steps:
+            # a line every log_every steps and at least every 10 s, so that a
+            # slowed-down run (e.g. under hardware counters) is never silent
+            now = time.perf_counter()
+            if (st + 1) % log_every == 0 or st + 1 == steps or now - t_last >= 10.0:
+                t_last = now
                 loss_v = float(loss.mean())
                 log.append((st + 1, loss_v))
                 if progress:

@@ -12,6 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FLANG = "/opt/rocm/lib/llvm/bin/flang"
 REF = os.path.join(ROOT, "oracle", "_ref", "libfrealign_ref.so")
 LIB = os.path.join(ROOT, "frealign_amd", "libfrealign_hip.so")
+HOSTS = os.path.join(ROOT, "tests", "fortran")
 
 
 @pytest.mark.skipif(not os.path.exists(FLANG), reason="flang not installed")
@@ -26,7 +27,7 @@ def test_fortran_host_matches_reference(tmp_path):
     assert r.returncode == 0, r.stderr
     exe = tmp_path / "fr_grid_check"
     r = subprocess.run([FLANG, "-O1", "-module-dir", str(tmp_path),
-                        os.path.join(src, "fr_grid_check.f90"), str(mod), REF, LIB,
+                        os.path.join(HOSTS, "fr_grid_check.f90"), str(mod), REF, LIB,
                         f"-Wl,-rpath,{os.path.dirname(REF)}:{os.path.dirname(LIB)}", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
@@ -49,7 +50,7 @@ def test_fortran_host_powell_matches_reference(tmp_path):
     assert r.returncode == 0, r.stderr
     exe = tmp_path / "fr_powell_check"
     r = subprocess.run([FLANG, "-O1", "-module-dir", str(tmp_path),
-                        os.path.join(src, "fr_powell_check.f90"), str(mod), REF, LIB,
+                        os.path.join(HOSTS, "fr_powell_check.f90"), str(mod), REF, LIB,
                         f"-Wl,-rpath,{os.path.dirname(REF)}:{os.path.dirname(LIB)}", "-o", str(exe)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr

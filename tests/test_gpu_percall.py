@@ -85,7 +85,7 @@ def test_fortran_percall_vs_oracle(tmp_path):
         np.array([nk0], np.int64).tofile(f)
         np.array([ttd0], np.float64).tofile(f)
     exe = tmp_path / "fr_percall_check"
-    rr = subprocess.run([FLANG, "-O1", os.path.join(ROOT, "frealign_amd", "fortran",
+    rr = subprocess.run([FLANG, "-O1", os.path.join(ROOT, "tests", "fortran",
                                                    "fr_percall_check.f90"), LIB,
                          f"-Wl,-rpath,{os.path.dirname(LIB)}", "-o", str(exe)],
                         capture_output=True, text=True)

@@ -316,3 +316,191 @@ def test_sigma_logpc_closed_form():
     else:
         pytest.fail("no SIGNSET inside the accepted band reproduced the closed form")
     assert base[1] == o[1] and out is not None
+
+
+# ------------------------------------------- planted per-shell patterns
+def _shells(it):
+    """Shell of each point: INT(SQRT(REAL(r^2)) + 0.5), the bin PRESB and CC3M
+    file a point under less one (presb.f:120, cc3m.f:128)."""
+    return (np.sqrt(it.astype(np.float32)) + np.float32(0.5)).astype(int)
+
+
+def test_presb_planted_shell_signs():
+    """A particle equal to the masked projection times a per-shell sign s_k and
+    scale a_k: PRESB gives RBIN(2k+1) = s_k (the scale cancels) in exactly the
+    shells that hold points, RBIN(2k+2) = 1, and leaves the rest untouched.
+    A shell index off by one, or a dropped conjugate, breaks the signs."""
+    nsam = 32
+    jc = nsam // 2 + 1
+    vol = _volume(nsam, 19)
+    ctff = (0.8 + 0.1 * rng.standard_normal(2 * nsam * jc)).astype(np.complex128)
+    eul = (0.7, 1.3, -0.4)
+    ri = 12.0
+    ring = ((ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0)
+    ainpo, it, keep = _masked_projection(nsam, vol, eul, ctff, 1, nsam // 2, ring)
+    kb = _shells(it)
+    r = np.random.default_rng(23)
+    sign = r.choice([-1.0, 1.0], nsam)
+    scale = r.uniform(0.2, 5.0, nsam)
+    spec = ainpo * sign[kb] * scale[kb]
+    rbin = np.full(2 * nsam, 9.0)
+    L.or64_presb(nsam, 1.0, spec, vol, *eul, 0.0, 0.0, rbin, ctff, *ring, O.IDENTITY)
+    held = set(kb[keep].tolist())
+    assert len(held) >= nsam // 2 - 1 and (sign[sorted(held)] < 0).any()
+    for k in range(nsam // 2):
+        if k in held:
+            assert abs(rbin[2 * k] - sign[k]) < 1e-10 and rbin[2 * k + 1] == 1.0, k
+        else:
+            assert rbin[2 * k] == 9.0 and rbin[2 * k + 1] == 9.0, k
+
+
+def test_cc3m_planted_signs_weights_and_fsct_branch():
+    """SPEC = s_k a_k P in shell k (P the masked projection): per shell
+    R_k = s_k b_k with b_k = a_k sqrt(sum |P|^2)_k, or b_k where FSCT < 0.95
+    (the |R| branch, cc3m.f:140-145), weighted by w_k = exp(-RBFACT (k-1)^2)
+    for the shell I = k+1 of cc3m.f:136-150, so
+    CC3M = sum sigma_k w b_k / sqrt(N sum w^2 b_k^2)."""
+    nsam = 32
+    jc = nsam // 2 + 1
+    vol = _volume(nsam, 29)
+    ctff = (0.7 + 0.3 * (rng.standard_normal(2 * nsam * jc)
+                         + 1j * rng.standard_normal(2 * nsam * jc))).astype(np.complex128)
+    eul = (-0.3, 0.8, 1.7)
+    maxr1, maxr2 = 2, nsam // 2 - 3
+    ri = 0.4 * nsam
+    ring = ((ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0)
+    ainpo, it, keep = _masked_projection(nsam, vol, eul, ctff, maxr1, maxr2, ring)
+    kb = _shells(it)
+    r = np.random.default_rng(31)
+    sign = r.choice([-1.0, 1.0], nsam)
+    scale = r.uniform(0.3, 3.0, nsam)
+    fsct = np.where(np.arange(jc) % 3 == 0, 0.6, 0.97).astype(np.float32)
+    rbfact = np.float32(0.004)
+    spec = ainpo * sign[kb] * scale[kb]
+    got = L.or64_cc3m(nsam, 1.0, spec, vol, maxr1, maxr2, *eul, 0.0, 0.0, float(rbfact), fsct,
+                      ctff, *ring, O.IDENTITY)
+    num = den = 0.0
+    nsh = 0
+    for I in range(2, nsam // 2 + 1):
+        sel = keep & (kb == I - 1)
+        if not sel.any():
+            continue
+        b = scale[I - 1] * math.sqrt(np.sum(np.abs(ainpo[sel]) ** 2))
+        sg = sign[I - 1] if fsct[I - 1] >= 0.95 else 1.0
+        w = math.exp(-float(rbfact) * (I - 1) ** 2)
+        num += sg * w * b
+        den += w * w * b * b
+        nsh += 1
+    want = num / math.sqrt(nsh * den)
+    # 1e-7: the weight's argument -RBFACT*(I-1)^2 is a REAL (FP32) product
+    assert abs(got - want) < 1e-7 * abs(want), (got, want)
+    assert (fsct < 0.95).any() and (fsct >= 0.95).any()
+
+
+@pytest.mark.parametrize("amag", [1.07, 0.93])
+def test_cc3m_magnification_closed_form(amag):
+    """With AMAG != 1 CC3M extracts at DM/|AMAG| and masks with the radii
+    scaled by AMAG (cc3m.f:93-103,114): a particle built that way is matched
+    exactly (the self-correlation closed form), and the AMAG = 1 section is not."""
+    nsam = 32
+    jc = nsam // 2 + 1
+    vol = _volume(nsam, 41)
+    ctff = (0.7 + 0.2 * rng.standard_normal(2 * nsam * jc)).astype(np.complex128)
+    eul = (0.5, 1.0, -2.2)
+    maxr1, maxr2 = 2, 12
+    ri = 0.35 * nsam
+    ring = ((ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0)
+    dm = np.zeros(9, np.float32)
+    L.or_dm(*eul, amag, None, dm)
+    i = np.arange(jc)[None, :]
+    m = np.arange(nsam)[:, None]
+    j = np.where(m >= jc, m - nsam, m)
+    it = (i * i + j * j).reshape(-1)
+    ii = np.broadcast_to(i, (nsam, jc)).reshape(-1)
+    jj = np.broadcast_to(j, (nsam, jc)).reshape(-1)
+    keep = (it >= maxr1 * maxr1) & (it < maxr2 * maxr2)
+    c = ctff[:nsam * jc] + np.conj(ctff[nsam * jc:])
+    sec = np.zeros(nsam * jc, np.complex128)
+    out = np.zeros(2)
+    f = np.float32
+    for k in np.nonzero(keep)[0]:
+        x3 = f(dm[0] * f(ii[k])) + f(dm[3] * f(jj[k]))
+        y3 = f(dm[1] * f(ii[k])) + f(dm[4] * f(jj[k]))
+        z3 = f(dm[2] * f(ii[k])) + f(dm[5] * f(jj[k]))
+        L.or64_ainterpo3ds(nsam, vol, float(x3), float(y3), float(z3), out)
+        sec[k] = complex(out[0], out[1]) * c[k]
+    d = np.zeros(nsam * (nsam + 2))
+    d.view(np.complex128)[:] = sec / nsam / nsam
+    L.or_irfft2_d(nsam, d)
+    L.or64_maskcos2(nsam, d, *ring, amag)
+    L.or_rfft2_d(nsam, d)
+    ainpo = d.view(np.complex128).copy()
+    fsct = np.ones(jc, np.float32)
+    got = L.or64_cc3m(nsam, amag, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, fsct, ctff,
+                      *ring, O.IDENTITY)
+    kb = _shells(it)
+    a = [math.sqrt(np.sum(np.abs(ainpo[keep & (kb == k)]) ** 2)) for k in range(1, nsam // 2)
+         if (keep & (kb == k)).any()]
+    a = np.array(a)
+    want = a.sum() / math.sqrt(len(a) * (a ** 2).sum())
+    assert abs(got - want) < 1e-9, (got, want)
+    plain = L.or64_cc3m(nsam, 1.0, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, fsct, ctff,
+                        *ring, O.IDENTITY)
+    assert plain < want - 1e-3
+
+
+def test_sigma_phaseflip_planted_sign_pattern():
+    """CTFR + conj(CTFL) = s(I,J) = +-1 (CTFR and CTFL complex, so a missing
+    conjugate shows), section = P s (P the CTF-free projection), particle =
+    beta P s: after PHASEFLIP (multiply by CTF/|CTF|, phaseflip.f:22-31) both
+    are P and beta P, except on the Nyquist row, which the loop reaches twice
+    (J = +-NSAM/2) and so keeps P s.  Then SIGMA2 = 0 and SIGMA2N = V / (NSAM^2
+    SUM1), with V the particle's variance over the whole image and SUM1 the
+    section's over the mask disc, both computed here from P in numpy."""
+    nsam = 32
+    jc = nsam // 2 + 1
+    vol = _volume(nsam, 43)
+    r = np.random.default_rng(47)
+    i = np.arange(jc)[None, :]
+    m = np.arange(nsam)[:, None]
+    j = np.where(m >= jc, m - nsam, m)
+    it = i * i + j * j
+    s = np.where(((np.sqrt(it) + 0.5).astype(int) // 2) % 2 == 0, 1.0, -1.0)  # sign per shell pair
+    g = r.standard_normal((nsam, jc))
+    ctfr = 0.7 * s + 0.2j * g
+    ctfl = 0.3 * s + 0.2j * g
+    ctff = np.concatenate([ctfr.reshape(-1), ctfl.reshape(-1)]).astype(np.complex128)
+    one = np.concatenate([np.ones(nsam * jc), np.zeros(nsam * jc)]).astype(np.complex128)
+    eul = (0.4, 1.1, 0.9)
+    p = np.zeros(nsam * jc, np.complex128)
+    L.or64_pextract(nsam, p, vol, *eul, one, O.IDENTITY)
+    p = p.reshape(nsam, jc)
+    beta, shx, shy = 1.7, 0.13, -0.21
+    ph = shx * i + shy * j
+    datc = beta * p * s * np.exp(1j * ph)
+    rclas, maxr1 = 0.7, 1
+    mrclas = int(nsam * rclas)
+    band = (it >= maxr1 * maxr1) & (it < mrclas * mrclas)
+    assert band[nsam // 2, :].any()  # the band reaches the Nyquist row
+    flip = np.where(band, p, 0)
+    flip[nsam // 2, :] *= s[nsam // 2, :]
+    x = np.fft.irfft2(flip, s=(nsam, nsam)) * nsam * nsam  # unnormalised c2r, [y][x]
+    rad = nsam * 0.3
+    dmask = np.array([jc, jc, jc, rad], np.float32)
+    lx = np.arange(1, nsam + 1)
+    dx = np.where(lx >= nsam // 2, lx - 1.0 - nsam, lx - 1.0)
+    disc = (dx[None, :] ** 2 + dx[:, None] ** 2) <= rad * rad
+    n1 = disc.sum()
+    sum1 = np.sum(x[disc] ** 2) - np.sum(x[disc]) ** 2 / n1
+    v = np.sum((beta * x) ** 2) - np.sum(beta * x) ** 2 / nsam ** 2
+    want = v / nsam ** 2 / (beta * beta * sum1)
+    out = np.zeros(3)
+    L.or64_sigma(nsam, 1.0, vol, datc.reshape(-1), *eul, shx, shy, ctff, maxr1, rclas, 0.0,
+                 dmask, 1.0, O.IDENTITY, out)
+    assert abs(out[0] - want) < 1e-9 * abs(want), (out[0], want)
+    # proportional after the flip: no noise left (relative to the image power)
+    assert abs(out[1]) < 1e-12 * v / nsam ** 4
+    # the same closed form without the flip differs, so the check has teeth
+    x0 = np.fft.irfft2(np.where(band, p * s, 0), s=(nsam, nsam)) * nsam * nsam
+    s10 = np.sum(x0[disc] ** 2) - np.sum(x0[disc]) ** 2 / n1
+    assert abs(s10 - sum1) > 1e-3 * sum1
