@@ -84,6 +84,9 @@ SIGNATURES = {
     "fr_search_layout": ([vp, ci, ci, ci, pi_, pi_], ci),
     "fr_search_bank": ([vp, ci, ci, ci, vp, ci, vp, ci, vp], ci),
     "fr_search_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, ci, vp, vp, ci, vp, vp, vp], ci),
+    "fr_ccp_view_batch": ([vp, ci, ci, ci, vp, vp, vp, vp, ci, ci, cf, cf, ci, vp, vp, vp], ci),
+    "fr_psearch": ([vp, ci, ci, vp, vp, vp, vp, cf, ci, ci, vp, vp, ci, ci, vp, cf, vp, ci, pi_,
+                    vp, C.c_char], ci),
 }
 
 

@@ -267,6 +267,59 @@ class Context:
             return 2
         raise ValueError(f"bank rows must be {kpad} long (or {self.SEP_ROW}, separable bf16)")
 
+    def psearch(self, nsam: int, a3df, outc, ctff, *, dangin: float, maxr1: int, maxr2: int,
+                rbfact: float, sang: np.ndarray, iquadmax: int, par=None, amag=None,
+                mask=(1, 1, 1, 1, 1), ipmax: int = 32, iran: int = -1, sm=None, asym: str = "C",
+                testpar=None):
+        """PSEARCH (psearch.f:8-121) over CCP with PAD and ROTATE2D, reference-exact,
+        for every particle in call order.  par: (np, 5) PHI THETA PSI DSHX DSHY (the
+        starting values; default zero).  Returns (par out = the best view, testpar
+        (np, ipmax, 6), iran after the calls)."""
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(outc, torch.complex64, "outc")
+        _dev(ctff, torch.complex64, "ctff")
+        jc = nsam // 2 + 1
+        np_ = outc.numel() // (nsam * jc)
+        if ctff.numel() != 2 * outc.numel():
+            raise ValueError("ctff must hold two half-planes per particle")
+        p = np.zeros((np_, 5), np.float32) if par is None else \
+            np.ascontiguousarray(np.asarray(par, np.float32).reshape(np_, 5)).copy()
+        am = np.ones(np_, np.float32) if amag is None else \
+            np.ascontiguousarray(np.asarray(amag, np.float32).reshape(np_))
+        sg = np.ascontiguousarray(np.asarray(sang, np.float32).reshape(-1, 3))
+        tp = np.zeros((np_, ipmax, 6), np.float32) if testpar is None else \
+            np.ascontiguousarray(np.asarray(testpar, np.float32).reshape(np_, ipmax, 6)).copy()
+        m = np.ascontiguousarray(np.asarray(mask, np.int32))
+        smv = None if sm is None else np.ascontiguousarray(sm, dtype=np.float32)
+        ir = C.c_int(int(iran))
+        self._sync_stream()
+        N.check(N.lib().fr_psearch(self.h, nsam, np_, _p(am), _p(a3df), _p(outc), _p(ctff),
+                                   float(dangin), int(maxr1), int(maxr2), _p(p), _p(sg),
+                                   sg.shape[0], int(iquadmax), _p(m), float(rbfact), _p(tp),
+                                   int(ipmax), C.byref(ir), _p(smv), asym.encode()[:1]),
+                "fr_psearch")
+        return p, tp, ir.value
+
+    def ccp_views(self, nsam: int, a3df, outc, ctff, dm, *, maxr1: int, maxr2: int,
+                  rbfact: float, dang: float, iquadmax: int, mask=(1, 1, 1, 1, 1)):
+        """fr_ccp_view_batch: CCP of every (particle, view) item; dm (np, nview, 9)
+        device.  Returns (score, code) device tensors (np, nview)."""
+        _dev(dm, torch.float32, "dm")
+        jc = nsam // 2 + 1
+        np_ = outc.numel() // (nsam * jc)
+        nview = dm.numel() // (9 * max(np_, 1))
+        if dm.numel() != 9 * np_ * nview:
+            raise ValueError("dm must be (particles, views, 9)")
+        score = torch.empty((np_, nview), dtype=torch.float32, device=outc.device)
+        code = torch.empty((np_, nview), dtype=torch.int32, device=outc.device)
+        m = np.ascontiguousarray(np.asarray(mask, np.int32))
+        self._sync_stream()
+        N.check(N.lib().fr_ccp_view_batch(self.h, nsam, np_, nview, _p(a3df), _p(outc), _p(ctff),
+                                          _p(dm), int(maxr1), int(maxr2), float(rbfact),
+                                          float(dang), int(iquadmax), _p(m), _p(score), _p(code)),
+                "fr_ccp_view_batch")
+        return score, code
+
     def search(self, nsam: int, maxr1: int, maxr2: int, rbfact: float, bank: torch.Tensor,
                outc, ctff, topk: int = 32, group: int = 1):
         """Top-k views (orientation, shift row, score) per particle against the bank;

@@ -32,6 +32,8 @@ SOURCES = [
     "fr_sigma.hip",
     "fr_plan.cpp",
     "fr_fortran.cpp",
+    "fr_ccp.hip",
+    "fr_psearch.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -242,6 +242,14 @@ int fr_destroy(fr_ctx* c) {
   for (auto& e : c->pts_tabs) (void)hipFree(e.second);
   for (auto& e : c->ring_tabs) (void)hipFree(e.second.d);
   for (auto& e : c->wmap_tabs) (void)hipFree(e.second);
+  for (auto& e : c->ccp_tabs) {
+    (void)hipFree((void*)e.second.d);
+    (void)hipFree((void*)e.second.k);
+    (void)hipFree((void*)e.second.w);
+    (void)hipFree((void*)e.second.e);
+    (void)hipFree((void*)e.second.e1);
+    (void)hipFree((void*)e.second.region);
+  }
   if (c->d_shx) (void)hipFree(c->d_shx);
   if (c->d_shy) (void)hipFree(c->d_shy);
   delete c;

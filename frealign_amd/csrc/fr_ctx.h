@@ -5,12 +5,13 @@
 #include <utility>
 #include <vector>
 
+#include "fr_ccp.h"
 #include "fr_common.h"
 #include "fr_kernels.h"
 
 struct fr_ctx : public fr::Ctx {
   std::vector<fr::InsertTask> host_tasks;
-  static constexpr int kSlots = 29;
+  static constexpr int kSlots = 36;
   void* slots[kSlots] = {};
   size_t slot_bytes[kSlots] = {};
   void* slot(int k, size_t bytes, int* err);
@@ -39,6 +40,10 @@ struct fr_ctx : public fr::Ctx {
   };
   std::map<int, Tune> tune;
   Tune tune_for(int nsam) const;
+  // constant tables of the exact CCP search, per (NSAM, MAXR2), kept until
+  // the context is destroyed
+  std::map<std::pair<int, int>, fr::CcpTablesDev> ccp_tabs;
+  int ccp_tables(int nsam, int maxr2, fr::CcpTablesDev* t);
 };
 
 namespace fr {

@@ -233,6 +233,40 @@ int fr_search_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact, i
                     const float *d_ctff, int topk, int *d_top_o, int *d_top_row,
                     float *d_top_score);
 
+/* Replaces PSEARCH -> CCP (psearch.f:8-121, ccp.f:8-275) with ROTATE2D and
+ * PAD (rotate2d.f:8-77, pad.f:8-68) as the reference computes them, for np
+ * particles (IRAD > 0, IPAD = 1, IEWALD = 0, non-helical ASYM): each view's
+ * CTF-multiplied section is padded to 4 NSAM with its edge mean, rotated
+ * through the NPSI in-plane steps by nearest-neighbour lookup, turned through
+ * the IQUADMAX quadrants by index permutation and correlated with the
+ * B-factor-weighted particle on the NSAMR grid (argmax over -6..6, first
+ * maximum, 3x3 box / 9, normalised by sqrt(SUM) sqrt(SUM2)).  Call order and
+ * arithmetic of the reference's bookkeeping are kept: particle k draws its
+ * jitter as the k-th PSEARCH call (RANDOM, *iran advanced), XPAR, TESTPAR by
+ * strict '>' with CC > 0 admission (columns of unadmitted slots keep their
+ * contents, CC = 0), and the running best.
+ *   amag     host np: AMAGP of each particle (the DM is divided by |AMAG|)
+ *   par      host np x 5, in/out: PHI THETA PSI DSHX DSHY -> the best view
+ *   sang     host nsang x 3 from fr_search_grid; dangin as given to SEARCHANG
+ *   mask     host PMASK(1..5); sm: SM (NULL: identity)
+ *   testpar  host np x ipmax x 6 (TESTPAR(6,IPMAX) per particle), in/out
+ * MAXR2 <= min(32, NSAMR/2). */
+int fr_psearch(fr_ctx *ctx, int nsam, int np, const float *amag, const float *d_a3df,
+               const float *d_spec, const float *d_ctff, float dangin, int maxr1, int maxr2,
+               float *par, const float *sang, int nsang, int iquadmax, const int *mask,
+               float rbfact, float *testpar, int ipmax, int *iran, const float *sm, char asym);
+
+/* The device part of fr_psearch: CCP (ccp.f:8-275 with rotate2d.f, pad.f) for
+ * np x nview (particle, view) items.  d_dm holds each item's DM (np x nview x
+ * 9, fr_rotation with amag = AMAG and SM at the view's XPAR, psearch.f:69-80);
+ * dang is CCP's DANGIN in degrees (NPSI = NINT(90/DANG), ccp.f:62-64).  Per
+ * item: d_score the CCMAX of the view and d_code its argument,
+ * ((IPSI-1)*4 + IQUAD-1)*225 + (IXM+7)*15 + (IYM+7). */
+int fr_ccp_view_batch(fr_ctx *ctx, int nsam, int np, int nview, const float *d_a3df,
+                      const float *d_spec, const float *d_ctff, const float *d_dm, int maxr1,
+                      int maxr2, float rbfact, float dang, int iquadmax, const int *mask,
+                      float *d_score, int *d_code);
+
 /* ------------------------------------------------------------------ SIGMA */
 
 /* Replaces SIGMA (sigma.f:8-221; non-helical, IEWALD=0, IPAD=1) for n items:
