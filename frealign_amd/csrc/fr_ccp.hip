@@ -354,6 +354,8 @@ __global__ void __launch_bounds__(kThreads)
         const int oo = __shfl_xor(bo, off, 64);
         if (ov > bv || (ov == bv && oo < bo)) { bv = ov; bo = oo; }
       }
+      // a map without a finite value leaves bo unset: take the first shift
+      if (bo >= nx * ny) bo = 0;
       const int ixm = bo / ny - 6 * a.mask4, iym = bo % ny - 6 * a.mask5;
       // 3x3 box in IX-outer order, then the normalisation (ccp.f:242-252)
       float box = 0.f;
@@ -506,6 +508,10 @@ int launch_ccp(Ctx& c, const CcpArgs& a, int item0, int nitems, const float* a3d
            kCcpMaxR);
   const size_t l1 = pad_lds(a.maxr2), l2 = scan_lds(a.maxr2);
   FR_CHECK(l1 <= 160 * 1024 && l2 <= 160 * 1024, "exact CCP search: LDS");
+  FR_HIP(hipFuncSetAttribute((const void*)k_ccp_pad, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)l1));
+  FR_HIP(hipFuncSetAttribute((const void*)k_ccp_scan, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)l2));
   hipLaunchKernelGGL(k_ccp_pad, dim3(nitems), dim3(kThreads), l1, c.stream, a,
                      reinterpret_cast<const float2*>(a3df), reinterpret_cast<const float2*>(ctff),
                      dm, item0, reinterpret_cast<const float2*>(t.d),
