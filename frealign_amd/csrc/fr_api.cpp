@@ -117,6 +117,9 @@ void* fr_ctx::slot(int k, size_t bytes, int* err) {
 }
 
 namespace {
+void sheltest_finish(const double* h, int ns, float fmask, int* npix, int* npixt, double* dfsc,
+                     float* pssnr);
+
 template <typename T>
 int upload_table(const std::vector<T>& h, T** d) {
   FR_HIP(hipMalloc(d, sizeof(T) * h.size()));
@@ -224,6 +227,8 @@ int fr_create(int device, fr_ctx** out) {
   FR_HIP(hipGetDeviceCount(&n));
   FR_CHECK(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
   FR_HIP(hipSetDevice(device));
+  // the no-allocation entry points' device scratch, once per device
+  if (int r = fr::noalloc_scratch_init(device)) return r;
   fr_ctx* c = new fr_ctx();
   c->device = device;
   c->stream = nullptr;
@@ -510,8 +515,43 @@ int fr_sheltest(fr_ctx* c, int nsam, int istp, const float* d_a, const float* d_
   FR_HIP(hipMemcpyAsync(h.data(), acc, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
                         c->stream));
   FR_HIP(hipStreamSynchronize(c->stream));
-  for (int i = 0; i < ns; i++) {  // sheltest.f:90-112
-    const double* a = h.data() + 6 * i;
+  sheltest_finish(h.data(), ns, fmask, npix, npixt, dfsc, pssnr);
+  *nstp = ns;
+  return 0;
+}
+
+int fr_sheltest_noalloc(fr_ctx* c, int nsam, int istp, const float* d_a, const float* d_d,
+                        const float* d_s, const float* d_v, float fmask, int* npix, int* npixt,
+                        double* dfsc, float* pssnr, int* nstp) {
+  CTX_CHECK(c);
+  SIZE_CHECK(nsam);
+  FR_CHECK(istp >= 1 && d_a && d_d && d_s && d_v && npix && npixt && dfsc && pssnr && nstp,
+           "null argument");
+  const int jc = nsam / 2 + 1;
+  int ns = jc / istp;
+  if (ns * istp < jc) ns++;
+  FR_CHECK(ns <= fr::kNoAllocShells,
+           "fr_sheltest_noalloc: %d shells exceed its fixed scratch of %d shells "
+           "(NSAM/2+1 over ISTP)", ns, fr::kNoAllocShells);
+  double* acc = fr::noalloc_scratch(c->device);
+  FR_CHECK(acc != nullptr, "fr_sheltest_noalloc: no scratch address for device %d", c->device);
+  if (int r = launch_shell_sums(*c, nsam, istp, ns, d_a, d_d, d_s, d_v, acc, true)) return r;
+  double h[6 * fr::kNoAllocShells];
+  FR_HIP(hipMemcpyAsync(h, acc, sizeof(double) * 6 * ns, hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  sheltest_finish(h, ns, fmask, npix, npixt, dfsc, pssnr);
+  *nstp = ns;
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+// sheltest.f:90-112: FSC, NPIX, NPIXT and PSSNR of each shell from its FP64 sums
+void sheltest_finish(const double* h, int ns, float fmask, int* npix, int* npixt, double* dfsc,
+                     float* pssnr) {
+  for (int i = 0; i < ns; i++) {
+    const double* a = h + 6 * i;
     npix[i] = (int)a[4];
     npixt[i] = (int)a[5];
     double d = a[0];
@@ -525,9 +565,10 @@ int fr_sheltest(fr_ctx* c, int nsam, int istp, const float* d_a, const float* d_
     dfsc[i] = d;
     pssnr[i] = ps;
   }
-  *nstp = ns;
-  return 0;
 }
+}  // namespace
+
+extern "C" {
 
 int fr_storeshift(fr_ctx* c, int nsam, double std_, double vtd, int iredun, const float* d_a,
                   const float* d_s, const float* d_d, const float* d_v, float* d_b, float* d_c) {

@@ -25,9 +25,17 @@ int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks
                   void* scratch);
 
 // FP64 shell sums of sheltest.f:48-88 (istp-wide shells); acc is nstp x 6 doubles:
-// DFSC, MEAN12, MEAN22, PSSNR-sum, NPIX, NPIXT.
+// DFSC, MEAN12, MEAN22, PSSNR-sum, NPIX, NPIXT.  acc is cleared first, by a fill
+// kernel when fill_kernel is set (stream-capturable), else by hipMemsetAsync.
 int launch_shell_sums(Ctx& c, int nsam, int istp, int nstp, const float* a, const float* d,
-                      const float* s, const float* v, double* acc);
+                      const float* s, const float* v, double* acc, bool fill_kernel = false);
+// Shells held by the file-scope accumulator array of the no-allocation shell
+// sums (fr_stats.hip), and its per-device address: noalloc_scratch_init(dev)
+// with dev current takes the address once; noalloc_scratch(dev) returns it
+// (nullptr before the init).
+constexpr int kNoAllocShells = 520;
+int noalloc_scratch_init(int device);
+double* noalloc_scratch(int device);
 int launch_storeshift(Ctx& c, int nsam, double std, double vtd, int iredun, const float* a,
                       const float* s, const float* d, const float* v, float* b, float* cc);
 

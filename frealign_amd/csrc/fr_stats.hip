@@ -6,7 +6,15 @@
 // launch_storeshift: the half-map normalisation of STORESHIFT
 //   (storeshift.f:43-66, IFSC=0, NNSTAT=0), computed in FP64 as the reference
 //   does (COMPLEX / DOUBLE promotes to double precision).
+//
+// The no-allocation form of the shell sums keeps its accumulators in
+// `scratch`, a file-scope device array of kNoAllocShells x 6 doubles (24,960
+// bytes) that every device the library runs on holds until the process ends.
+// Its device address is taken once per device, at context creation, with
+// that device current; kernels and copies use that address only.
 #include <hip/hip_runtime.h>
+
+#include <mutex>
 
 #include "fr_common.h"
 #include "fr_kernels.h"
@@ -16,7 +24,18 @@ namespace fr {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxShell = 520;
+constexpr int kMaxShell = kNoAllocShells;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[6 * kNoAllocShells];
+
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+__global__ void k_zero_f64(double* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
 
 __device__ inline int isqrt_floor(int v) {
   int r = (int)sqrtf((float)v);
@@ -88,10 +107,32 @@ __global__ void __launch_bounds__(kThreads)
 
 }  // namespace
 
+int noalloc_scratch_init(int device) {
+  FR_CHECK(device >= 0 && device < kMaxDevices, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  FR_CHECK(e == hipSuccess && p != nullptr, "hipGetSymbolAddress(scratch) failed on device %d: %s",
+           device, hipGetErrorString(e));
+  g_scratch_dev[device] = static_cast<double*>(p);
+  return 0;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_dev[device];
+}
+
 int launch_shell_sums(Ctx& c, int nsam, int istp, int nstp, const float* a, const float* d,
-                      const float* s, const float* v, double* acc) {
+                      const float* s, const float* v, double* acc, bool fill_kernel) {
   FR_CHECK(nstp <= kMaxShell, "too many shells (%d)", nstp);
-  FR_HIP(hipMemsetAsync(acc, 0, sizeof(double) * 6 * nstp, c.stream));
+  if (fill_kernel)
+    hipLaunchKernelGGL(k_zero_f64, dim3((6 * nstp + 255) / 256), dim3(256), 0, c.stream, acc,
+                       6 * nstp);
+  else
+    FR_HIP(hipMemsetAsync(acc, 0, sizeof(double) * 6 * nstp, c.stream));
   const size_t nvox = (size_t)(nsam / 2 + 1) * nsam * nsam;
   int blocks = (int)((nvox + kThreads - 1) / kThreads);
   if (blocks > 2048) blocks = 2048;

@@ -190,6 +190,18 @@ int fr_sheltest(fr_ctx *ctx, int nsam, int istp, const float *d_a, const float *
                 const float *d_s, const float *d_v, float fmask, int *npix, int *npixt,
                 double *dfsc, float *pssnr, int *nstp);
 
+/* No counterpart in the reference.  fr_sheltest for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same arguments and results, with the FP64 shell
+ * accumulators in a fixed device array of 520 shells x 6 doubles (24,960 bytes
+ * per device, held until the process ends) instead of context scratch.  It
+ * calls no hipMalloc/hipFree; the accumulators are cleared by a fill kernel.
+ * Returns an error when NSAM/2+1 over ISTP exceeds 520 shells.  Every call on
+ * one device shares that array: calls that overlap on one device race. */
+int fr_sheltest_noalloc(fr_ctx *ctx, int nsam, int istp, const float *d_a, const float *d_d,
+                        const float *d_s, const float *d_v, float fmask, int *npix, int *npixt,
+                        double *dfsc, float *pssnr, int *nstp);
+
 /* Replaces STORESHIFT's half-map normalisation (storeshift.f:43-66, IFSC=0,
  * NNSTAT=0): B = A*(-1)^(l+m+n)/(S+STD*IREDUN)/NSAM^3, C likewise from D, V. */
 int fr_storeshift(fr_ctx *ctx, int nsam, double std_, double vtd, int iredun, const float *d_a,
