@@ -11,15 +11,26 @@
 //   SUBROUTINE PREFINE(...)        prefine.f:8-162   -> prefine_
 //   SUBROUTINE PINSERT_TRILIN(...) pinsert_trilin.f:8-244 -> pinsert_trilin_
 //   SUBROUTINE A3D3(...)           a3d3.f:8-146      -> a3d3_
+//   SUBROUTINE PSEARCH(...)        psearch.f:8-121   -> psearch_
 //
-// Each call copies its host arrays to the device, runs the batched kernels on
-// one item and copies the result back; the batched entry points of
-// include/frealign_hip.h are the throughput path.  The reference calls these
-// routines serially per process (a library-wide context on device 0 is used;
-// calls are not thread-safe).  The supported cases are the reference's
-// defaults on the hot path: IPAD = 1, IRAD > 0, IEWALD = 0, non-helical
-// ASYM, CC3M with IBUF = 0.  Anything else, or a device error, is fatal as the
-// reference's own STOPs are: a message on stdout and exit(1).
+// Each call copies its per-particle host arrays to the device, runs the
+// batched kernels on one item and copies the result back; the batched entry
+// points of include/frealign_hip.h are the throughput path.  Two arrays stay
+// resident instead of crossing PCIe on every call:
+//   * the reference transform (C3DF / A3DF of CC3M, PEXTRACT, PRESB, SIGMA,
+//     PREFINE and PSEARCH), uploaded when a call names a host array other than
+//     the resident one, or the same array with a different fingerprint (1,024
+//     words sampled across it), or after fr_percall_invalidate;
+//   * A3D3's half-maps (A3DF/S3DF, D3DF/V3DF, NVOL1/NVOL2 private volumes),
+//     accumulated on the device between the flushes and copied back to the
+//     host arrays at the flush of the last particle (K = ILAST), which is when
+//     LMAIN's loop ends, or by fr_percall_flush.
+// The reference calls these routines serially per process (a library-wide
+// context on device 0 is used; calls are not thread-safe).  The supported
+// cases are the reference's defaults on the hot path: IPAD = 1, IRAD > 0,
+// IEWALD = 0, non-helical ASYM, CC3M with IBUF = 0.  Anything else, or a
+// device error, is fatal as the reference's own STOPs are: a message on
+// stdout and exit(1).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -47,10 +58,22 @@ struct Dev {
   }
 };
 
+// A host array mirrored on the device.  `dirty`: the device copy holds
+// accumulations the host array does not have yet.
+struct Resident {
+  const void* host = nullptr;
+  size_t bytes = 0;
+  unsigned long long fp = 0;
+  bool dirty = false;
+  Dev dev;
+};
+
 struct State {
   fr_ctx* ctx = nullptr;
-  Dev vol, spec, ctff, fsct, dm, shift, amag, pidx, out;
-  Dev vol2, outc, datc, ppar, pssnr, task, sums, work;
+  Dev spec, ctff, fsct, dm, shift, amag, pidx, out;
+  Dev vol, vol2, outc, datc, ppar, pssnr, task, sums, work;
+  Resident ref;                   // the reference transform
+  Resident maps[4];               // A3DF, S3DF, D3DF, V3DF of A3D3
 };
 
 State& st() {
@@ -91,6 +114,45 @@ void download(void* host, const void* dev, size_t bytes, const char* routine) {
        "device-to-host copy failed");
 }
 
+// FNV-1a over 1,024 64-bit words spread evenly across the array, plus its last
+// word: enough to notice a host array that was rewritten or reloaded
+unsigned long long fingerprint(const void* host, size_t bytes) {
+  const unsigned char* b = static_cast<const unsigned char*>(host);
+  unsigned long long h = 1469598103934665603ull ^ bytes;
+  const size_t nw = bytes / 8;
+  auto mix = [&](size_t off) {
+    unsigned long long w = 0;
+    memcpy(&w, b + off, 8);
+    h = (h ^ w) * 1099511628211ull;
+  };
+  if (nw > 0) {
+    const size_t step = nw > 1024 ? nw / 1024 : 1;
+    for (size_t k = 0; k < nw; k += step) mix(8 * k);
+    mix(8 * (nw - 1));
+  }
+  return h;
+}
+
+// the device copy of a read-only host array, uploaded only when it changed
+void* resident(Resident& r, const void* host, size_t bytes, const char* routine) {
+  const unsigned long long fp = fingerprint(host, bytes);
+  if (r.host == host && r.bytes == bytes && r.fp == fp && r.dev.p) return r.dev.p;
+  void* p = upload(r.dev, host, bytes, routine);
+  r.host = host;
+  r.bytes = bytes;
+  r.fp = fp;
+  r.dirty = false;
+  return p;
+}
+
+// copies an accumulated device map back into its host array
+void write_back(Resident& r, const char* routine) {
+  if (!r.dirty || !r.host) return;
+  download(const_cast<void*>(r.host), r.dev.p, r.bytes, routine);
+  r.fp = fingerprint(r.host, r.bytes);
+  r.dirty = false;
+}
+
 void common_checks(const char* routine, int nsam, int irad, int ipad, int iewald) {
   need(nsam >= 8 && nsam % 2 == 0, routine, "NSAM must be even and >= 8");
   need(ipad == 1, routine, "only IPAD = 1 is supported");
@@ -121,7 +183,7 @@ float cc3m_(const int* nsam, const int* irad, const float* amag, const float* sp
   fr_ctx* c = ctx(R);
   State& s = st();
   const size_t ic = half_plane(n);
-  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dv = resident(s.ref, a3df, sizeof(float) * 2 * half_volume(n), R);
   void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
   void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
   void* df = upload(s.fsct, fsct, sizeof(float) * (n / 2 + 1), R);
@@ -157,7 +219,7 @@ void pextract_(const int* nsam, const int* irad, float* outc, const float* a3df,
   fr_ctx* c = ctx(R);
   State& s = st();
   const size_t ic = half_plane(n);
-  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dv = resident(s.ref, a3df, sizeof(float) * 2 * half_volume(n), R);
   void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
   float dmh[9];
   fr::rotation(*phi, *theta, *psi, 0.0f, sm, dmh);  // pextract.f:30-51
@@ -189,7 +251,7 @@ void presb_(const int* nsam, const int* irad, const float* amag, const float* sp
   fr_ctx* c = ctx(R);
   State& s = st();
   const size_t ic = half_plane(n);
-  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dv = resident(s.ref, a3df, sizeof(float) * 2 * half_volume(n), R);
   void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
   void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
   float dmh[9];
@@ -227,7 +289,7 @@ void sigma_(const int* nsam, const int* irad, const float* amag, const float* a3
   fr_ctx* c = ctx(R);
   State& s = st();
   const size_t ic = half_plane(n);
-  void* dv = upload(s.vol, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dv = resident(s.ref, a3df, sizeof(float) * 2 * half_volume(n), R);
   void* ds = upload(s.spec, datc, sizeof(float) * 2 * ic, R);
   void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
   const int p0 = 0;
@@ -274,7 +336,7 @@ void prefine_(const int* nsam, const int* irad, float* amag, const int* maxr1, c
   fr_ctx* c = ctx(R);
   State& s = st();
   const size_t ic = half_plane(n);
-  void* dv = upload(s.vol, c3df, sizeof(float) * 2 * half_volume(n), R);
+  void* dv = resident(s.ref, c3df, sizeof(float) * 2 * half_volume(n), R);
   void* ds = upload(s.spec, outc, sizeof(float) * 2 * ic, R);
   void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
   void* df = upload(s.fsct, fsct, sizeof(float) * (n / 2 + 1), R);
@@ -418,15 +480,17 @@ void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, c
   *ttd += td;
 }
 
-// a3d3.f:8-146 with NVOL = 1 (IMP = 1, the reference's serial build): the
+// a3d3.f:8-146 (INTERP = 1, IEWALD = 0, NNSTAT = 0, NONSYM false): the
 // particle is recorded in RECBF/CTFBF/RECB/ISW as a3d3.f:39-85 does (ISWITCH
-// alternating the half-maps), and the one-slot buffer is flushed at once
-// (a3d3.f:107-143 -> A3D3_S -> PINSERT -> PINSERT_TRILIN; INTERP = 1,
-// IEWALD = 0, NNSTAT = 0, NONSYM false) into A3DF/S3DF (ISW 0) or D3DF/V3DF
-// (ISW 1), adding the voxel count to NKSUM1(1) and the CTF^2 sum to TTD1(1)
-// or TTD2(1).  PBC is declared INTEGER in a3d3.f and a3d3_s.f, which only pass
-// it on; PINSERT reads the same storage as REAL (pinsert.f:31,43), as the
-// driver's PBC is, so it is taken as REAL here.
+// alternating the half-maps); when the buffer holds NVOL particles, or at the
+// last particle, slot I is inserted (a3d3.f:107-137 -> A3D3_S -> PINSERT ->
+// PINSERT_TRILIN) into private volume IS of A3DF/S3DF (ISW 0) or D3DF/V3DF
+// (ISW 1), IS = I, or (I-1)/2+1 when IFSC = 0 (a3d3_s.f:39-40), adding its
+// voxel count to NKSUM1(I) and its CTF^2 sum to TTD1(IS) or TTD2(IS).  The
+// slots go in order, one after another, where the reference runs them in an
+// OpenMP loop.  PBC is declared INTEGER in a3d3.f and a3d3_s.f, which only
+// pass it on; PINSERT reads the same storage as REAL (pinsert.f:31,43), as
+// the driver's PBC is, so it is taken as REAL here.
 void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, const float* /*ri*/,
            const float* rrec, const int* k, const int* ilast, const float* amagp,
            const float* datc, const float* ctff, float* /*asum*/, float* /*vsum*/,
@@ -437,23 +501,26 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
            const float* /*sinclut*/, const int* /*irada*/, const int* ipad,
            const float* /*thetatr*/, const int* iewald, const int* nsym, const int* /*isymax*/,
            const int* jsym, const float* symop, const int* nnstat, long long* /*nksum*/,
-           const int* ifsc, const int* imp, const int* nvol, const int* /*nvol1*/,
-           const int* /*nvol2*/, const float* occ, const float* sig2n, const float* psize,
+           const int* ifsc, const int* /*imp*/, const int* nvol, const int* nvol1,
+           const int* nvol2, const float* occ, const float* sig2n, const float* psize,
            const char* asym, const float* /*alpha*/, const float* /*rise*/, const int* /*nu*/,
            const int* /*hstart*/, long long* nksum1, double* ttd1, double* ttd2, int* /*ns*/,
            int* /*ns1*/, const int* interp, float* recbf, float* recb, int* isw, int* ibuf,
-           float* ctfbf, const int* /*nn2*/, const int* /*nn3*/, const float* /*sm*/,
+           float* ctfbf, const int* /*nn2*/, const int* nn3, const float* /*sm*/,
            const int* nonsym, const int* ism, size_t /*asym_len*/) {
   const char* R = "A3D3";
   const int n = *nsam;
   need(n >= 8 && n % 2 == 0 && n <= 1024 && n <= *nn1, R, "NSAM must be even, 8 .. min(NN1, 1024)");
-  need(*nvol == 1 && *imp == 1, R, "only NVOL = 1, IMP = 1 is supported");
+  need(*nvol >= 1 && *nvol1 >= 1 && *nvol2 >= 1, R, "NVOL, NVOL1 and NVOL2 must be >= 1");
   need(*irad != 0 && *interp == 1, R, "only trilinear insertion (INTERP = 1) is supported");
   need(*ipad == 1 && *iewald == 0 && *nnstat == 0, R,
        "only IPAD = 1, IEWALD = 0, NNSTAT = 0 are supported");
   need(asym[0] != 'H', R, "helical symmetry is not supported");
   need(*nonsym == 0, R, "the NONSYM insertion is not supported");
   const size_t ic = half_plane(n);
+  const size_t hv = half_volume(n);
+  const size_t nvox = (size_t)*nn3;  // A3DF(NN3, NVOL1): one private volume per column
+  need(nvox >= hv, R, "NN3 is smaller than the NSAM half-volume");
   const size_t slot_c = (size_t)(*nn1 / 2) * *nn1;  // RECBF(NN1/2*NN1, NVOL), complex
   const size_t slot_f = (size_t)*nn1 * *nn1;        // CTFBF(NN1*NN1, NVOL), complex
   auto record = [&](int which) {                    // a3d3.f:41-56, 63-78
@@ -484,40 +551,114 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
   }
   if (!(*ibuf == *nvol || *k == *ilast)) return;
   if (*ibuf >= 1) {
-    const int h = isw[0];
-    need(recb[7] != 0.0f, R, "SIG2N is zero");
     fr_ctx* c = ctx(R);
     State& s = st();
-    const size_t hv = half_volume(n);
-    float* num = h == 0 ? a3df : d3df;
-    float* ctf2 = h == 0 ? s3df : v3df;
-    void* ds = upload(s.spec, recbf, sizeof(float) * 2 * ic, R);
-    void* dc = upload(s.ctff, ctfbf, sizeof(float) * 4 * ic, R);
-    void* dv = upload(s.vol, num, sizeof(float) * 2 * hv, R);
-    void* dt = upload(s.vol2, ctf2, sizeof(float) * hv, R);
+    // the private volumes stay on the device between flushes
+    float* host[4] = {a3df, s3df, d3df, v3df};
+    const size_t bytes[4] = {sizeof(float) * 2 * nvox * *nvol1, sizeof(float) * nvox * *nvol1,
+                             sizeof(float) * 2 * nvox * *nvol2, sizeof(float) * nvox * *nvol2};
+    float* dev[4];
+    for (int q = 0; q < 4; q++) {
+      Resident& r = s.maps[q];
+      if (r.dirty && r.host == host[q] && r.bytes == bytes[q])
+        dev[q] = (float*)r.dev.p;
+      else
+        dev[q] = (float*)resident(r, host[q], bytes[q], R);
+    }
     void* dsum = s.sums.get(2 * sizeof(long long) + 2 * sizeof(double));
     need(dsum != nullptr, R, "device allocation failed");
-    need(hipMemset(dsum, 0, 2 * sizeof(long long) + 2 * sizeof(double)) == hipSuccess, R,
-         "device memset failed");
     long long* dk = (long long*)dsum;
     double* dd = (double*)(dk + 2);
-    // one half is addressed (ISW): its arrays stand in for the other's
-    check(fr_insert_batch(c, n, 1, (const float*)ds, (const float*)dc, recb, &h, *nsym, jsym,
-                          symop, *pbc, *boff, *psize, (float*)dv, (float*)dt, (float*)dv,
-                          (float*)dt, dk, dd),
-          R);
-    check(fr_synchronize(c), R);
-    download(num, dv, sizeof(float) * 2 * hv, R);
-    download(ctf2, dt, sizeof(float) * hv, R);
-    long long kc[2];
-    double td[2];
-    download(kc, dk, sizeof kc, R);
-    download(td, dd, sizeof td, R);
-    nksum1[0] += kc[h];  // PINSERT_TRILIN's NKSUM is NKSUM1(I), I = 1
-    if (h == 0) ttd1[0] += td[0];
-    else ttd2[0] += td[1];
+    // a3d3.f:107-137 -> A3D3_S: slot I into private volume IS of its half
+    // (IS = I, or (I-1)/2+1 when IFSC = 0), NKSUM1(I) and TTD1/TTD2(IS)
+    for (int i = 0; i < *ibuf; i++) {
+      const int h = isw[i];
+      const int is = *ifsc == 0 ? i / 2 : i;
+      need(is < (h == 0 ? *nvol1 : *nvol2), R, "buffer slot beyond NVOL1/NVOL2 private volumes");
+      need(recb[10 * (size_t)i + 7] != 0.0f, R, "SIG2N is zero");
+      void* ds = upload(s.spec, recbf + 2 * slot_c * i, sizeof(float) * 2 * ic, R);
+      void* dc = upload(s.ctff, ctfbf + 2 * slot_f * i, sizeof(float) * 4 * ic, R);
+      float* num = dev[2 * h] + 2 * nvox * is;
+      float* ctf2 = dev[2 * h + 1] + nvox * is;
+      need(hipMemset(dsum, 0, 2 * sizeof(long long) + 2 * sizeof(double)) == hipSuccess, R,
+           "device memset failed");
+      // one half is addressed (ISW): its arrays stand in for the other's
+      check(fr_insert_batch(c, n, 1, (const float*)ds, (const float*)dc, recb + 10 * (size_t)i,
+                            &h, *nsym, jsym, symop, *pbc, *boff, *psize, num, ctf2, num, ctf2, dk,
+                            dd),
+            R);
+      check(fr_synchronize(c), R);
+      long long kc[2];
+      double td[2];
+      download(kc, dk, sizeof kc, R);
+      download(td, dd, sizeof td, R);
+      nksum1[i] += kc[h];  // PINSERT_TRILIN's NKSUM is NKSUM1(I)
+      if (h == 0) ttd1[is] += td[0];
+      else ttd2[is] += td[1];
+      s.maps[2 * h].dirty = s.maps[2 * h + 1].dirty = true;
+    }
   }
   *ibuf = 0;
+  // the particle loop ends here: the host's half-maps become current
+  if (*k == *ilast) {
+    State& s = st();
+    for (int q = 0; q < 4; q++) write_back(s.maps[q], R);
+  }
 }
+
+// psearch.f:8-121 (non-helical ASYM, IPAD = 1, IEWALD = 0): CCP with PAD and
+// ROTATE2D for every SANG view on the device, the RANDOM jitter, TESTPAR's
+// strict '>' insertion with CC > 0 admission and the running best on the
+// host.  Returns PHI, THETA, PSI, DSHX, DSHY of the best view, TESTPAR and
+// IRAN; PRESA is not touched (as psearch.f leaves it).
+void psearch_(const int* nsam, const int* irad, const float* amag, const float* spec,
+              const float* a3df, const float* dangin, const int* maxr1, const int* maxr2,
+              float* phi, float* theta, float* psi, float* dshx, float* dshy,
+              float* /*presa*/, const float* sang, const int* nsang, float* /*ccd*/,
+              float* /*ccc*/, float* /*cbuf*/, const int* iquadmax, const int* mask,
+              const float* rbfact, const float* /*sinclut*/, const int* ipad, float* /*rbuf*/,
+              const int* iewald, const float* /*thetatr*/, const float* ctff,
+              const float* /*ri2*/, const float* /*ri3*/, const float* /*rih*/,
+              const float* /*halfw*/, float* testpar, const int* ipmax, const char* asym,
+              int* iran, const void* /*fftw_plans*/, const float* sm, size_t /*asym_len*/) {
+  const char* R = "PSEARCH";
+  const int n = *nsam;
+  common_checks(R, n, *irad, *ipad, *iewald);
+  need(asym[0] != 'H', R, "helical symmetry is not supported");
+  need(*nsang >= 1 && *ipmax >= 1, R, "NSANG and IPMAX must be >= 1");
+  fr_ctx* c = ctx(R);
+  State& s = st();
+  const size_t ic = half_plane(n);
+  void* dv = resident(s.ref, a3df, sizeof(float) * 2 * half_volume(n), R);
+  void* ds = upload(s.spec, spec, sizeof(float) * 2 * ic, R);
+  void* dc = upload(s.ctff, ctff, sizeof(float) * 4 * ic, R);
+  float par[5] = {*phi, *theta, *psi, *dshx, *dshy};
+  check(fr_psearch(c, n, 1, amag, (const float*)dv, (const float*)ds, (const float*)dc, *dangin,
+                   *maxr1, *maxr2, par, sang, *nsang, *iquadmax, mask, *rbfact, testpar, *ipmax,
+                   iran, sm, asym[0]),
+        R);
+  *phi = par[0];
+  *theta = par[1];
+  *psi = par[2];
+  *dshx = par[3];
+  *dshy = par[4];
+}
+
+// Not in the reference.  The host changed the reference transform in place:
+// the next per-call routine uploads it again.  (Fortran: CALL FR_PERCALL_INVALIDATE)
+void fr_percall_invalidate(void) {
+  State& s = st();
+  s.ref.host = nullptr;
+  s.ref.fp = 0;
+}
+void fr_percall_invalidate_(void) { fr_percall_invalidate(); }
+
+// Not in the reference.  Copies A3D3's device-resident half-maps back into the
+// host arrays now (A3D3 itself does so at K = ILAST).  (Fortran: CALL FR_PERCALL_FLUSH)
+void fr_percall_flush(void) {
+  State& s = st();
+  for (int q = 0; q < 4; q++) write_back(s.maps[q], "FR_PERCALL_FLUSH");
+}
+void fr_percall_flush_(void) { fr_percall_flush(); }
 
 }  // extern "C"

@@ -443,11 +443,17 @@ int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const floa
  * The reference's own routine names and argument lists (Fortran 77 implicit
  * interfaces: every argument by reference, the CHARACTER length of ASYM
  * appended), so a Fortran LMAIN links against this library unchanged.  Each
- * call runs one item through the batched kernels (arrays copied per call) on
- * a library-wide context on device 0; not thread-safe, as the reference's
- * own routines are called serially.  Supported: IPAD = 1, IRAD > 0,
- * IEWALD = 0, non-helical ASYM, IBUF = 0.  Other cases and device errors
- * stop the program (message on stdout, exit(1)), as the reference's STOPs. */
+ * call runs one item through the batched kernels on a library-wide context
+ * on device 0; not thread-safe, as the reference's own routines are called
+ * serially.  Per-particle arrays are copied per call.  The reference
+ * transform (C3DF/A3DF) stays resident on the device and is uploaded again
+ * only when a call names another host array or the array's fingerprint
+ * (1,024 sampled words) changed, or after fr_percall_invalidate.  A3D3's
+ * half-maps accumulate on the device and are copied back into the host
+ * arrays at the flush of the last particle (K = ILAST) or by
+ * fr_percall_flush.  Supported: IPAD = 1, IRAD > 0, IEWALD = 0, non-helical
+ * ASYM, IBUF = 0.  Other cases and device errors stop the program (message
+ * on stdout, exit(1)), as the reference's STOPs. */
 
 /* CTFAPPLY (ctfapply.f:8-161; XSTD >= 0, non-helical): CTFF, DATC and the
  * masked, SNR-weighted transform in OUTD (OUTD and OUTC are one array,
@@ -529,10 +535,12 @@ void pinsert_trilin_(const int *nsam, const float *rii, const int *irrec2, const
                      const int *nsamh, const int *ic, const float *dx, const float *dy,
                      const float *st, const int *ip, int *ns1);
 
-/* A3D3 (a3d3.f:8-146; NVOL = 1, IMP = 1, INTERP = 1, non-helical, NONSYM
- * false): records the particle in RECBF/CTFBF/RECB/ISW, alternates ISWITCH,
- * and flushes the one-slot buffer into A3DF/S3DF or D3DF/V3DF, NKSUM1(1) and
- * TTD1(1)/TTD2(1).  PBC is REAL storage (pinsert.f:31 reads it so). */
+/* A3D3 (a3d3.f:8-146; INTERP = 1, non-helical, NONSYM false): records the
+ * particle in RECBF/CTFBF/RECB/ISW, alternates ISWITCH, and when NVOL
+ * particles are buffered or K = ILAST flushes slot I into private volume IS
+ * (column IS of A3DF/S3DF(NN3, NVOL1) or D3DF/V3DF(NN3, NVOL2); IS = I, or
+ * (I-1)/2+1 when IFSC = 0, a3d3_s.f:39-40), NKSUM1(I) and TTD1/TTD2(IS).
+ * PBC is REAL storage (pinsert.f:31 reads it so). */
 void a3d3_(const int *nset, int *iswitch, const int *nsam, const int *irad, const float *ri,
            const float *rrec, const int *k, const int *ilast, const float *amagp,
            const float *datc, const float *ctff, float *asum, float *vsum, float *psum,
@@ -549,6 +557,31 @@ void a3d3_(const int *nset, int *iswitch, const int *nsam, const int *irad, cons
            double *ttd1, double *ttd2, int *ns, int *ns1, const int *interp, float *recbf,
            float *recb, int *isw, int *ibuf, float *ctfbf, const int *nn2, const int *nn3,
            const float *sm, const int *nonsym, const int *ism, size_t asym_len);
+
+/* PSEARCH (psearch.f:8-121; non-helical, IPAD = 1, IEWALD = 0): the
+ * reference-exact global search of fr_psearch for one particle.  Returns the
+ * best view in PHI, THETA, PSI, DSHX, DSHY, the IPMAX best views in TESTPAR
+ * (strict '>', CC > 0 only) and IRAN after its three RANDOM draws. */
+void psearch_(const int *nsam, const int *irad, const float *amag, const float *spec,
+              const float *a3df, const float *dangin, const int *maxr1, const int *maxr2,
+              float *phi, float *theta, float *psi, float *dshx, float *dshy, float *presa,
+              const float *sang, const int *nsang, float *ccd, float *ccc, float *cbuf,
+              const int *iquadmax, const int *mask, const float *rbfact, const float *sinclut,
+              const int *ipad, float *rbuf, const int *iewald, const float *thetatr,
+              const float *ctff, const float *ri2, const float *ri3, const float *rih,
+              const float *halfw, float *testpar, const int *ipmax, const char *asym, int *iran,
+              const void *fftw_plans, const float *sm, size_t asym_len);
+
+/* No counterpart in the reference: the residency controls of the per-call
+ * symbols, for a host that changes the reference transform in place
+ * (invalidate: the next call uploads it again) or reads A3D3's half-maps
+ * before the last particle (flush: copy them back now).  The trailing-
+ * underscore forms are CALL FR_PERCALL_INVALIDATE / CALL FR_PERCALL_FLUSH
+ * from Fortran. */
+void fr_percall_invalidate(void);
+void fr_percall_invalidate_(void);
+void fr_percall_flush(void);
+void fr_percall_flush_(void);
 
 /* ------------------------------------------- batches of uneven box sizes
  * No counterpart in the reference (its particles go one at a time through

@@ -96,7 +96,7 @@ def test_fpart_from_mw_closed_form():
 
 
 @pytest.mark.parametrize("call,message", [
-    ("a3d3", "A3D3 (frealign_hip): only NVOL = 1, IMP = 1 is supported"),
+    ("a3d3", "A3D3 (frealign_hip): the NONSYM insertion is not supported"),
     ("ctfapply", "CTFAPPLY (frealign_hip): the MASKENV envelope (XSTD < 0) is not supported"),
     ("pinsert_trilin", "PINSERT_TRILIN (frealign_hip): Ewald-sphere insertion (IEWALD != 0) is not supported"),
     ("prefine", "PREFINE (frealign_hip): only IVFLAG = 0 is supported"),
@@ -128,7 +128,7 @@ if which == "a3d3":
             i(2), i(1), i(1), f(1.0), f(1.0), f(1.5), C.c_char_p(b"C1 "), f(0.0), f(0.0), i(0),
             i(0), k.ctypes, d.ctypes, d.ctypes, buf.ctypes, buf.ctypes, i(1), buf.ctypes,
             buf.ctypes, buf.ctypes, C.byref(C.c_int(0)), buf.ctypes, i(40), i(40), buf.ctypes,
-            i(0), i(1), C.c_size_t(3)]
+            i(1), i(1), C.c_size_t(3)]
     assert len(args) == 73
     lib.a3d3_(*args)
 elif which == "ctfapply":

@@ -257,3 +257,220 @@ def test_a3d3_percall_vs_oracle():
         print(f"A3D3 half {h}: numerator {e_num:.1e} CTF^2 {e_c2:.1e} TTD {e_t:.1e}")
         assert e_num < 1e-5 and e_c2 < 1e-5 and e_t < 1e-6
         assert not gn[nv:].any() and not gc[nv:].any() and gt[1] == 0.0
+
+
+def _a3d3_args(k, npart, nsam, nn1, nn3, iswitch, ibuf, datc, ctff, eul, shf, presa, occ, sig2n,
+               rrec, maps, sums, bufs, sym, nvol, ifsc, pbc=5.0, boff=40.0, psize=1.5):
+    f, i = O.byref_f, O.byref_i
+    a3df, s3df, d3df, v3df = maps
+    nksum1, ttd1, ttd2 = sums
+    recbf, recb, isw, ctfbf = bufs
+    nsym, symop, jsym = sym
+    dummy = np.zeros(16, np.float32)
+    idummy = np.zeros(64, np.int32)
+    nv1 = nv2 = max(1, (nvol + 1) // 2) if ifsc == 0 else nvol
+    args = [i(2), C.byref(iswitch), i(nsam), i(1), f(0.4 * nsam), rrec.ctypes, i(k + 1),
+            i(npart), f(1.0), np.ascontiguousarray(datc).ctypes,
+            np.ascontiguousarray(ctff).ctypes, dummy.ctypes, dummy.ctypes, dummy.ctypes,
+            dummy.ctypes, idummy.ctypes, f(float(eul[0])), f(float(eul[1])), f(float(eul[2])),
+            f(float(shf[0])), f(float(shf[1])), f(float(presa)), f(pbc), f(boff), a3df.ctypes,
+            s3df.ctypes, C.byref(C.c_double(0.0)), d3df.ctypes, v3df.ctypes,
+            C.byref(C.c_double(0.0)), i(nn1), i(1), dummy.ctypes, i(1), i(1), f(0.0), i(0),
+            i(nsym), i(8), jsym.ctypes, symop.ctypes, i(0), C.byref(C.c_int64(0)), i(ifsc), i(1),
+            i(nvol), i(nv1), i(nv2), f(float(occ)), f(float(sig2n)), f(psize),
+            C.c_char_p(b"D7 "), f(0.0), f(0.0), i(0), i(0), nksum1.ctypes, ttd1.ctypes,
+            ttd2.ctypes, idummy.ctypes, idummy.ctypes, i(1), recbf.ctypes, recb.ctypes,
+            isw.ctypes, C.byref(ibuf), ctfbf.ctypes, i(nn1), i(nn3), O.IDENTITY.ctypes, i(0),
+            i(1), C.c_size_t(3)]
+    assert len(args) == 73
+    return args
+
+
+def test_a3d3_private_volumes_vs_oracle():
+    """a3d3_ with NVOL = 4 (IFSC = 0): six particles, a flush of four slots at
+    K = 4 and of two at K = ILAST = 6.  Slot I goes to private volume
+    IS = (I-1)/2+1 of its half (a3d3_s.f:39-40), NKSUM1(I) and TTD1/TTD2(IS)
+    per slot and volume.  The half-maps stay on the device until the last
+    flush: the host arrays are unchanged after K = 4 until fr_percall_flush,
+    which then shows the first flush's volumes.  Bars as the NVOL = 1 test."""
+    from frealign_amd import api
+
+    L = O.oracle()
+    lib = C.CDLL(LIB)
+    nsam, nn1, nvol = 32, 36, 4
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    nv = nsam * nsam * jc
+    nn3 = nn1 * nn1 * (nn1 // 2 + 1)
+    r = np.random.default_rng(29)
+    npart = 6
+    datc = (r.standard_normal((npart, ic)) + 1j * r.standard_normal((npart, ic))).astype(np.complex64)
+    ctff = (0.6 + 0.2 * (r.standard_normal((npart, 2 * ic))
+                         + 1j * r.standard_normal((npart, 2 * ic)))).astype(np.complex64)
+    eul = r.uniform(-np.pi, np.pi, (npart, 3)).astype(np.float32)
+    shf = r.uniform(-0.3, 0.3, (npart, 2)).astype(np.float32)
+    occ = r.uniform(0.5, 1.0, npart).astype(np.float32)
+    sig2n = r.uniform(0.5, 2.0, npart).astype(np.float32)
+    presa = r.uniform(0.2, 0.8, npart).astype(np.float32)
+    rrec = np.array([0.0, 0.45], np.float32)
+    nsym, symop, jsym, _ = api.point_group("D", 7)
+    nvh = 2  # NVOL1 = NVOL2 = NVOL/2 private volumes per half
+    maps = (np.zeros(nn3 * nvh, np.complex64), np.zeros(nn3 * nvh, np.float32),
+            np.zeros(nn3 * nvh, np.complex64), np.zeros(nn3 * nvh, np.float32))
+    sums = (np.zeros(nvol, np.int64), np.zeros(nvh, np.float64), np.zeros(nvh, np.float64))
+    bufs = (np.zeros(nn1 // 2 * nn1 * nvol + ic, np.complex64), np.zeros(10 * nvol, np.float32),
+            np.full(nvol, -1, np.int32), np.zeros(nn1 * nn1 * nvol + 2 * ic, np.complex64))
+    iswitch, ibuf = C.c_int(0), C.c_int(0)
+    slot_of = []  # (half, IS, slot) of each particle
+    for k in range(npart):
+        print(f"A3D3 NVOL={nvol} call K={k + 1}", flush=True)
+        h = iswitch.value
+        slot = ibuf.value
+        slot_of.append((h, slot // 2, slot))
+        lib.a3d3_(*_a3d3_args(k, npart, nsam, nn1, nn3, iswitch, ibuf, datc[k], ctff[k], eul[k],
+                              shf[k], presa[k], occ[k], sig2n[k], rrec, maps, sums, bufs,
+                              (nsym, symop, jsym), nvol, 0))
+        if k == 3:
+            assert ibuf.value == 0
+            assert not maps[0].any() and not maps[2].any()  # still on the device
+            lib.fr_percall_flush()
+            snap = [m.copy() for m in maps]
+    assert ibuf.value == 0
+    assert [s[:2] for s in slot_of] == [(0, 0), (1, 0), (0, 1), (1, 1), (0, 0), (1, 0)]
+
+    def oracle_into(parts):
+        num = [[np.zeros(nv, np.complex64) for _ in range(nvh)] for _ in range(2)]
+        c2 = [[np.zeros(nv, np.float32) for _ in range(nvh)] for _ in range(2)]
+        ttd = [[C.c_double(0.0) for _ in range(nvh)] for _ in range(2)]
+        nk = [C.c_int64(0) for _ in range(nvol)]
+        for k in parts:
+            h, is_, slot = slot_of[k]
+            on = np.float32(occ[k]) / np.float32(sig2n[k])
+            L.or32_pinsert(nsam, float(rrec[1]), 1.0, datc[k], ctff[k], num[h][is_], c2[h][is_],
+                           C.byref(ttd[h][is_]), float(eul[k, 0]), float(eul[k, 1]),
+                           float(eul[k, 2]), float(shf[k, 0]), float(shf[k, 1]), float(presa[k]),
+                           5.0, 40.0, nsym, jsym, symop, C.byref(nk[slot]), 1.5, float(on),
+                           O.IDENTITY, 0)
+        return num, c2, ttd, nk
+
+    def compare(got_maps, want, label):
+        num, c2, _, _ = want
+        for h in range(2):
+            gn, gc = got_maps[2 * h], got_maps[2 * h + 1]
+            for is_ in range(nvh):
+                a = gn[is_ * nn3:is_ * nn3 + nv]
+                b = gc[is_ * nn3:is_ * nn3 + nv]
+                e_num = np.linalg.norm(a - num[h][is_]) / np.linalg.norm(num[h][is_])
+                e_c2 = np.linalg.norm(b - c2[h][is_]) / np.linalg.norm(c2[h][is_])
+                print(f"A3D3 {label} half {h} volume {is_ + 1}: numerator {e_num:.1e} "
+                      f"CTF^2 {e_c2:.1e}")
+                assert e_num < 1e-5 and e_c2 < 1e-5
+                assert not gn[is_ * nn3 + nv:(is_ + 1) * nn3].any()
+
+    compare(snap, oracle_into(range(4)), "after the first flush")
+    want = oracle_into(range(npart))
+    compare(maps, want, "at K = ILAST")
+    num, c2, ttd, nk = want
+    assert [int(v) for v in sums[0]] == [v.value for v in nk]
+    for h, got in ((0, sums[1]), (1, sums[2])):
+        for is_ in range(nvh):
+            assert abs(got[is_] - ttd[h][is_].value) <= 1e-6 * ttd[h][is_].value
+
+
+def test_psearch_and_resident_reference_vs_oracle():
+    """psearch_ (the reference's PSEARCH symbol and argument list) for three
+    particles in turn, IRAN carried from call to call, against or32_psearch:
+    the best view bit for bit, TESTPAR as fr_psearch's test.  Then cc3m_ on
+    the resident reference: the same value on a repeat call, the new
+    reference's value after the host array is rewritten in place (its
+    fingerprint changes) and after fr_percall_invalidate."""
+    from frealign_amd import api, synth
+
+    L = O.oracle()
+    lib = C.CDLL(LIB)
+    ctx = api.Context(0)
+    nsam, psize, ri = 64, 2.0, 26.0
+    optics = synth.Optics(psize=psize)
+    sym = api.point_group("D", 7)
+    vol = synth.blob_phantom(nsam, 20, 24.0, seed=51, sym=sym, device="cuda")
+    a3df_t = synth.half_volume(vol)
+    npart = 3
+    parts = synth.make_particles(ctx, a3df_t, npart, seed=52, optics=optics, snr=2.0, ri=ri,
+                                 shift_a=4.0)
+    gpar = optics.gpar(nsam, ri)
+    pssnr = torch.from_numpy(synth.default_pssnr(nsam, ri)).to("cuda")
+    ctff_t, outc_t, _ = ctx.ctfapply(nsam, parts.images.clone(), parts.ppar(), pssnr, gpar)
+    a3df = np.ascontiguousarray(a3df_t.cpu().numpy().reshape(-1))
+    outc = outc_t.cpu().numpy().reshape(npart, -1)
+    ctff = ctff_t.cpu().numpy().reshape(npart, -1)
+    ctx.close()
+    dang, maxr1, maxr2, rbfact, ipmax = 15.0, 1, 14, 0.0005, 8
+    sang, iq = api.search_grid(dang, "D", 7)
+    flat = np.ascontiguousarray(sang.reshape(-1))
+    mask = np.ones(5, np.int32)
+    f, i = O.byref_f, O.byref_i
+    iran = C.c_int(-23)
+    iran_o = C.c_int(-23)
+    dummy = np.zeros(64, np.float32)
+    slots = agree = 0
+    for p in range(npart):
+        print(f"[psearch_] particle {p + 1}", flush=True)
+        par = [C.c_float(0.0) for _ in range(6)]
+        tp = np.zeros(6 * ipmax, np.float32)
+        lib.psearch_(i(nsam), i(1), f(1.0), outc[p].ctypes, a3df.ctypes, f(dang), i(maxr1),
+                     i(maxr2), *[C.byref(v) for v in par], flat.ctypes, i(len(sang)),
+                     dummy.ctypes, dummy.ctypes, dummy.ctypes, i(iq), mask.ctypes, f(rbfact),
+                     dummy.ctypes, i(1), dummy.ctypes, i(0), f(0.0), ctff[p].ctypes, f(0.0),
+                     f(0.0), f(0.0), f(0.0), tp.ctypes, i(ipmax), C.c_char_p(b"D7 "),
+                     C.byref(iran), dummy.ctypes, O.IDENTITY.ctypes, C.c_size_t(3))
+        ot = np.zeros(6 * ipmax, np.float32)
+        ph, th, ps, dx, dy = (C.c_float(0.0) for _ in range(5))
+        L.or32_psearch(nsam, 1.0, outc[p], a3df, dang, maxr1, maxr2, C.byref(ph), C.byref(th),
+                       C.byref(ps), C.byref(dx), C.byref(dy), flat, len(sang), iq, mask, rbfact,
+                       ctff[p], ot, ipmax, C.byref(iran_o), O.IDENTITY)
+        assert iran.value == iran_o.value
+        assert [par[j].value for j in range(3)] == [ph.value, th.value, ps.value]
+        tp, ot = tp.reshape(ipmax, 6), ot.reshape(ipmax, 6)
+        for q in range(ipmax):
+            if ot[q, 5] == 0.0:
+                assert tp[q, 5] == 0.0
+                continue
+            slots += 1
+            near = [abs(ot[q, 5] - ot[s, 5]) <= 1e-4 * abs(ot[q, 5]) for s in (q - 1, q + 1)
+                    if 0 <= s < ipmax]
+            if any(near):
+                continue
+            assert abs(tp[q, 5] - ot[q, 5]) < 1e-4 * abs(ot[q, 5]), (p, q, tp[q], ot[q])
+            agree += np.array_equal(tp[q, :5], ot[q, :5])
+    print(f"[psearch_] TESTPAR slots with identical angles and shifts: {agree}/{slots}")
+    assert agree >= 0.95 * slots
+
+    # cc3m_ on the resident reference
+    jc = nsam // 2 + 1
+    fsct = np.ones(jc, np.float32)
+    ri2, ri3, rih, halfw = (ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0
+    ang = (0.4, 1.2, -0.5, 0.03, -0.02)
+
+    def cc3m_gpu():
+        return lib.cc3m_(i(nsam), i(1), f(1.0), outc[0].ctypes, a3df.ctypes, i(maxr1), i(maxr2),
+                         *[f(v) for v in ang], i(0), dummy.ctypes, f(rbfact), dummy.ctypes, i(1),
+                         dummy.ctypes, fsct.ctypes, i(0), f(0.0), ctff[0].ctypes, f(ri2), f(ri3),
+                         f(rih), f(halfw), dummy.ctypes, dummy.ctypes, O.IDENTITY.ctypes)
+
+    lib.cc3m_.restype = C.c_float
+
+    def cc3m_oracle():
+        return L.or32_cc3m(nsam, 1.0, outc[0], a3df, maxr1, maxr2, *ang, rbfact, fsct, ctff[0],
+                           ri2, ri3, rih, halfw, O.IDENTITY)
+
+    first, again = cc3m_gpu(), cc3m_gpu()
+    want = cc3m_oracle()
+    assert first == again and abs(first - want) <= 1e-4 * abs(want)
+    a3df *= np.complex64(0.5 + 0.25j)  # rewritten in place: the fingerprint changes
+    a3df[: nsam * jc] *= np.complex64(1.5)
+    changed, want2 = cc3m_gpu(), cc3m_oracle()
+    print(f"[cc3m_] resident {first:.6f} / {again:.6f} (oracle {want:.6f}); "
+          f"after the rewrite {changed:.6f} (oracle {want2:.6f})")
+    assert abs(changed - want2) <= 1e-4 * abs(want2) and abs(changed - first) > 1e-3
+    lib.fr_percall_invalidate()
+    assert abs(cc3m_gpu() - want2) <= 1e-4 * abs(want2)
