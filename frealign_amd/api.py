@@ -536,6 +536,48 @@ class Context:
                 "fr_refine_batch")
         return p, nev
 
+    def local_search(self, nsam: int, a3df, spec, ctff, par, *, nang: int, dang: float,
+                     nsh: int, dsh: float, maxr1: int, maxr2: int, rbfact: float, fsct, ring,
+                     amag=None, sm=None, all_scores: bool = False):
+        """The local grid search before PREFINE (fr_local_search_batch): per
+        particle, (2 nang + 1)^3 orientations DANG apart (radians) around par's
+        PHI THETA PSI, each scored by CC3M at the (2 nsh + 1)^2 shifts DSH apart
+        (phase units) around its SHX SHY.  par (np, 5 or 6) host.  Returns the
+        (np, 6) best grid points (PHI THETA PSI SHX SHY CC3M) and, with
+        all_scores, the device (np, norient) best score and shift index of
+        every orientation."""
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(spec, torch.complex64, "spec")
+        _dev(ctff, torch.complex64, "ctff")
+        _dev(fsct, torch.float32, "fsct")
+        jc = nsam // 2 + 1
+        np_ = spec.numel() // (nsam * jc)
+        pin = np.asarray(par, np.float32).reshape(np_, -1)
+        p = np.zeros((np_, 6), np.float32)
+        p[:, :5] = pin[:, :5]
+        if a3df.numel() != nsam * nsam * jc or ctff.numel() != 2 * spec.numel():
+            raise ValueError("a3df/ctff sizes")
+        if fsct.numel() < jc:
+            raise ValueError("fsct needs NSAM/2+1 values")
+        am = np.ones(np_, np.float32) if amag is None else \
+            np.ascontiguousarray(np.asarray(amag, np.float32).reshape(np_))
+        r = np.asarray(ring, np.float32)
+        smv = None if sm is None else np.ascontiguousarray(sm, dtype=np.float32)
+        norient = (2 * nang + 1) ** 3
+        sc = cd = None
+        if all_scores:
+            sc = torch.empty((np_, norient), dtype=torch.float32, device=spec.device)
+            cd = torch.empty((np_, norient), dtype=torch.int32, device=spec.device)
+        self._sync_stream()
+        N.check(N.lib().fr_local_search_batch(self.h, nsam, np_, _p(a3df), _p(spec), _p(ctff),
+                                              _p(am), _p(p), int(nang), float(dang), int(nsh),
+                                              float(dsh), int(maxr1), int(maxr2), float(rbfact),
+                                              _p(fsct), _p(r), _p(smv), _p(sc), _p(cd)),
+                "fr_local_search_batch")
+        if all_scores:
+            return p, sc, cd
+        return p
+
     # ------------------------------------------------ proposal network
     def cnn_init(self, shape, seed: int, master: torch.Tensor, wq: torch.Tensor):
         _dev(master, torch.float32, "master")

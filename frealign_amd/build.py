@@ -34,6 +34,7 @@ SOURCES = [
     "fr_fortran.cpp",
     "fr_ccp.hip",
     "fr_psearch.cpp",
+    "fr_local.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -230,6 +230,9 @@ __device__ inline void tile_r2(int v0, int nsamh, int nsam, int& lo, int& hi) {
   hi = la > lb ? la : lb;
 }
 
+// kWin: the score at every shift of a window around the item's shift (the
+// best of them and its index), instead of at the item's shift alone
+template <bool kWin>
 __global__ void __launch_bounds__(kFT)
     k_cc3m_fused(int nsam, int nitem, int maxr12, int maxr22, float rbfact,
                  const float2* __restrict__ a3df,
@@ -239,7 +242,8 @@ __global__ void __launch_bounds__(kFT)
                  const float* __restrict__ fsct, const float* __restrict__ tab,
                  const int* __restrict__ pts, const float* __restrict__ ring_tab, int ring_cnt,
                  const float* __restrict__ wmap, float ri2,
-                 float ri3, float rih, float halfw, float* score) {
+                 float ri3, float rih, float halfw, float* score, Cc3mWindow win,
+                 int* __restrict__ wcode) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* Ts = lds;                      // [nsam][kFP]
   float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
@@ -493,6 +497,7 @@ __global__ void __launch_bounds__(kFT)
   }
   __syncthreads();
 
+  if constexpr (!kWin) {
   // ---- cc3m.f:117-150: shifted particle x conj(masked projection), shells.
   // Each wave owns a fixed set of shells (zig-zag, so the point counts
   // balance), walks their points from the sorted table and reduces them with
@@ -559,6 +564,152 @@ __global__ void __launch_bounds__(kFT)
       if (s1 * s2 != 0.0) s3 = s3 / sqrt(s1 * s2);
       score[it] = (float)s3;
     }
+  }
+  } else {
+  // ---- cc3m.f:117-167 at every shift s of the window.  The shell sums of
+  // SAMP conj(AINPO) are Q_k(s) = Re sum C e^{i s.(I,J)} with
+  // C = SPEC conj(AINPO); the other sums do not depend on s (|SAMP| = |SPEC|).
+  // C goes to LDS (the Y/Z buffer is free), the shell sums of |SPEC|^2 and
+  // |AINPO|^2 to bins; then each wave takes its shells with one lane per shift
+  // (two for windows above 64 shifts) and the phase e^{i sx I} e^{i sy J} from
+  // two small tables, and adds R_k WGT into its FP64 partial per shift.
+  float2* Cs = reinterpret_cast<float2*>(Ys);
+  {
+    const float2* sp = spec + (size_t)p * ic;
+    for (int sh = 0; sh < kCc3mShells; sh++) {
+      const int z = sh & 15;
+      if ((z < 8 ? z : 15 - z) != wave) continue;
+      const int b0 = pts[sh], b1 = pts[sh + 1];
+      float s2 = 0.f, a2 = 0.f;
+      for (int e = b0 + lane; e < b1; e += 64) {
+        const int pk = pts[kCc3mPtsHdr + e];
+        const int i = pk & 0xff, j = (pk >> 8) - 64;
+        int jj = j < 0 ? -j : j;
+        float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
+        float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
+        float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
+        float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
+        Cs[e] = make_float2(sv.x * a.x + sv.y * a.y, sv.y * a.x - sv.x * a.y);
+        s2 += sv.x * sv.x + sv.y * sv.y;
+        a2 += a.x * a.x + a.y * a.y;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        s2 += __shfl_xor(s2, off, 64);
+        a2 += __shfl_xor(a2, off, 64);
+      }
+      if (lane == 0) {
+        bins[4 * sh + 1] = s2;
+        bins[4 * sh + 2] = a2;
+        bins[4 * sh + 3] = (float)(b1 - b0);
+      }
+    }
+  }
+  __syncthreads();
+  const int nsx = win.nsx, nsy = win.nsy, ns = nsx * nsy;
+  float2* Ex = reinterpret_cast<float2*>(Bs);          // [I][ix]
+  float2* Ey = Ex + 32 * kCc3mWinMax;                   // [J + 31][iy]
+  double* part = reinterpret_cast<double*>(Ey + (2 * kFJ + 1) * kCc3mWinMax);  // [wave][s]
+  {
+    const float shx0 = shift[2 * it], shy0 = shift[2 * it + 1];
+    for (int e = tid; e < 32 * nsx; e += kFT) {
+      const int i = e / nsx, ix = e % nsx;
+      const float sx = shx0 + (float)(ix - nsx / 2) * win.dsx;
+      float sn, cs;
+      sincosf(sx * (float)i, &sn, &cs);
+      Ex[i * kCc3mWinMax + ix] = make_float2(cs, sn);
+    }
+    for (int e = tid; e < (2 * kFJ + 1) * nsy; e += kFT) {
+      const int jj = e / nsy, iy = e % nsy;
+      const float sy = shy0 + (float)(iy - nsy / 2) * win.dsy;
+      float sn, cs;
+      sincosf(sy * (float)(jj - kFJ), &sn, &cs);
+      Ey[jj * kCc3mWinMax + iy] = make_float2(cs, sn);
+    }
+  }
+  __syncthreads();
+  const int top = nsam / 2 < kCc3mShells ? nsam / 2 : kCc3mShells;
+  {
+    const int s0 = lane, s1 = lane + 64;
+    const bool v0 = s0 < ns, v1 = s1 < ns;
+    const int ix0 = v0 ? s0 / nsy : 0, iy0 = v0 ? s0 % nsy : 0;
+    const int ix1 = v1 ? s1 / nsy : 0, iy1 = v1 ? s1 % nsy : 0;
+    double acc0 = 0.0, acc1 = 0.0;
+    for (int sh = 1; sh < top; sh++) {
+      const int z = sh & 15;
+      if ((z < 8 ? z : 15 - z) != wave) continue;
+      const float cnt = bins[4 * sh + 3];
+      if (cnt == 0.0f) continue;
+      const int b0 = pts[sh], b1 = pts[sh + 1];
+      float q0 = 0.f, q1 = 0.f;
+      for (int e = b0; e < b1; e++) {
+        const int pk = pts[kCc3mPtsHdr + e];
+        const int i = pk & 0xff, jj = (pk >> 8) - 64 + kFJ;
+        const float2 c = Cs[e];
+        {
+          const float2 ex = Ex[i * kCc3mWinMax + ix0], ey = Ey[jj * kCc3mWinMax + iy0];
+          const float er = ex.x * ey.x - ex.y * ey.y, ei = ex.x * ey.y + ex.y * ey.x;
+          q0 += c.x * er - c.y * ei;
+        }
+        if (v1) {
+          const float2 ex = Ex[i * kCc3mWinMax + ix1], ey = Ey[jj * kCc3mWinMax + iy1];
+          const float er = ex.x * ey.x - ex.y * ey.y, ei = ex.x * ey.y + ex.y * ey.x;
+          q1 += c.x * er - c.y * ei;
+        }
+      }
+      const float ra = sqrtf(bins[4 * sh + 2]);
+      float r0 = q0 / ra, r1 = q1 / ra;
+      if (fsct[sh] < 0.95f) {
+        r0 = fabsf(r0);
+        r1 = fabsf(r1);
+      }
+      const float wgt = expf(-rbfact * (float)(sh * sh));
+      acc0 += (double)(r0 * wgt);
+      acc1 += (double)(r1 * wgt);
+    }
+    if (v0) part[wave * 128 + s0] = acc0;
+    if (v1) part[wave * 128 + s1] = acc1;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    double t1 = 0.0, t2 = 0.0;
+    for (int i = 2 + lane; i <= top; i += 64) {
+      const float* b = bins + 4 * (i - 1);
+      if (b[3] != 0.0f) {
+        const float wgt = expf(-rbfact * (float)((i - 1) * (i - 1)));
+        t2 += (double)(wgt * wgt * b[1]);
+        t1 += 1.0;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      t1 += __shfl_xor(t1, off, 64);
+      t2 += __shfl_xor(t2, off, 64);
+    }
+    float bv = -3.0e38f;
+    int bs = 0x7fffffff;
+    for (int s = lane; s < ns; s += 64) {
+      double s3 = 0.0;
+      for (int w = 0; w < kFT / 64; w++) s3 += part[w * 128 + s];
+      if (t1 * t2 != 0.0) s3 = s3 / sqrt(t1 * t2);
+      const float v = (float)s3;
+      if (v > bv) {
+        bv = v;
+        bs = s;
+      }
+    }
+    // the first maximum in window order (IX outer, IY inner)
+    for (int off = 32; off > 0; off >>= 1) {
+      const float ov = __shfl_xor(bv, off, 64);
+      const int os = __shfl_xor(bs, off, 64);
+      if (ov > bv || (ov == bv && os < bs)) {
+        bv = ov;
+        bs = os;
+      }
+    }
+    if (lane == 0) {
+      score[it] = bv;
+      wcode[it] = bs < ns ? bs : 0;
+    }
+  }
   }
   __syncthreads();  // bins and LDS buffers are reused by the next item
   }
@@ -654,21 +805,38 @@ void cc3m_fused_wmap(int nsam, float ri2, float ri3, float rih, float halfw,
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
                       const float* amag, const float* fsct, const Cc3mTables& t, float* score,
-                      int blocks) {
+                      int blocks, const Cc3mWindow* win, int* wcode) {
   if (a.n <= 0) return 0;
   FR_CHECK(cc3m_fused_ok(a.nsam, a.maxr2), "fused CC3M needs NSAM %% 32 == 0, NSAM <= 256, MAXR2 <= 32");
   FR_CHECK(t.dft && t.pts, "fused CC3M tables missing");
   const size_t lds = cc3m_fused_lds(a.nsam);
-  FR_HIP(hipFuncSetAttribute((const void*)k_cc3m_fused, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)lds));
+  Cc3mWindow w{1, 1, 0.f, 0.f};
+  if (win) {
+    FR_CHECK(wcode != nullptr, "shift window without an index output");
+    FR_CHECK(win->nsx >= 1 && win->nsy >= 1 && win->nsx <= kCc3mWinMax &&
+                 win->nsy <= kCc3mWinMax && win->nsx % 2 == 1 && win->nsy % 2 == 1,
+             "shift window %d x %d: odd sizes up to %d", win->nsx, win->nsy, kCc3mWinMax);
+    w = *win;
+  }
+  const void* fn = win ? (const void*)k_cc3m_fused<true> : (const void*)k_cc3m_fused<false>;
+  FR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
   // persistent grid: `blocks` workgroups (0: one per item), each looping over items
   const int grid = (blocks > 0 && blocks < a.n) ? blocks : a.n;
-  hipLaunchKernelGGL(k_cc3m_fused, dim3((unsigned)grid), dim3(kFT), lds, c.stream, a.nsam, a.n,
-                     a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
-                     reinterpret_cast<const float2*>(a3df), dm, pidx,
-                     reinterpret_cast<const float2*>(ctff), reinterpret_cast<const float2*>(spec),
-                     shift, amag, fsct, t.dft, t.pts, t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3,
-                     a.rih, a.halfw, score);
+  if (win)
+    hipLaunchKernelGGL(k_cc3m_fused<true>, dim3((unsigned)grid), dim3(kFT), lds, c.stream, a.nsam,
+                       a.n, a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
+                       reinterpret_cast<const float2*>(a3df), dm, pidx,
+                       reinterpret_cast<const float2*>(ctff),
+                       reinterpret_cast<const float2*>(spec), shift, amag, fsct, t.dft, t.pts,
+                       t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3, a.rih, a.halfw, score, w, wcode);
+  else
+    hipLaunchKernelGGL(k_cc3m_fused<false>, dim3((unsigned)grid), dim3(kFT), lds, c.stream,
+                       a.nsam, a.n, a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
+                       reinterpret_cast<const float2*>(a3df), dm, pidx,
+                       reinterpret_cast<const float2*>(ctff),
+                       reinterpret_cast<const float2*>(spec), shift, amag, fsct, t.dft, t.pts,
+                       t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3, a.rih, a.halfw, score, w,
+                       (int*)nullptr);
   FR_HIP(hipGetLastError());
   return 0;
 }

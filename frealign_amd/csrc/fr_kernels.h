@@ -128,9 +128,17 @@ int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& df
                     std::vector<float>& ring);
 void cc3m_fused_wmap(int nsam, float ri2, float ri3, float rih, float halfw,
                      std::vector<float>& w);
+// A window of nsx x nsy shifts (odd sizes up to kCc3mWinMax) centred on each
+// item's shift, dsx/dsy apart (phase units): the fused kernel then writes the
+// best score of the window and its index (ix * nsy + iy, the first maximum).
+constexpr int kCc3mWinMax = 11;
+struct Cc3mWindow {
+  int nsx, nsy;
+  float dsx, dsy;
+};
 int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
                       const float* ctff, const int* pidx, const float* dm, const float* shift,
                       const float* amag, const float* fsct, const Cc3mTables& t, float* score,
-                      int blocks = 0);
+                      int blocks = 0, const Cc3mWindow* win = nullptr, int* wcode = nullptr);
 
 }  // namespace fr

@@ -306,6 +306,26 @@ int fr_powell_minimize(int n, float *x, const float *e, float escale, int icon, 
                        float (*fn)(int n, const float *x, void *user), void *user, float *f,
                        int *nevals);
 
+/* The local grid search of a mode-1 refinement, before PREFINE (no routine of
+ * the reference scans a grid here: LMAIN calls PREFINE from the particle's
+ * parameters, lmain.f:743-828).  For each of np particles (pidx 0..np-1 of
+ * d_spec/d_ctff) the (2 nang + 1)^3 orientations PHI + a DANG, THETA + b DANG,
+ * PSI + c DANG (a, b, c = -nang..nang; angles summed in FP32, radians) are
+ * scored by CC3M (cc3m.f:8-169, the score PREFINE maximises) at every shift
+ * of the (2 nsh + 1)^2 window SHX + ix DSH, SHY + iy DSH (phase units,
+ * ix, iy = -nsh..nsh), and the first maximum in grid order (PHI outer, then
+ * THETA, PSI, then the shift with SHX outer) is kept.  par: host np x 6, in:
+ * PHI THETA PSI SHX SHY (the centre), out: the best grid point and its CC3M.
+ * amag: host np (DM divided by |AMAG|, cc3m.f:60-64); sm: host 9 or NULL
+ * (DM <- DM^T SM, matmul_t.f).  NSAM a multiple of 32 in 64..256, MAXR2 <=
+ * 32, nang <= 12, nsh <= 5.  d_scores / d_codes (device, np x (2 nang + 1)^3,
+ * may be NULL) receive every orientation's best score and shift index. */
+int fr_local_search_batch(fr_ctx *ctx, int nsam, int np, const float *d_a3df,
+                          const float *d_spec, const float *d_ctff, const float *amag, float *par,
+                          int nang, float dang, int nsh, float dsh, int maxr1, int maxr2,
+                          float rbfact, const float *d_fsct, const float *ring, const float *sm,
+                          float *d_scores, int *d_codes);
+
 /* Replaces PREFINE (prefine.f:8-162; IVFLAG=0, non-helical, AMAG > 0, IEWALD=0)
  * for n items refined in lock-step: each round evaluates every active item's
  * CALCFX (calcfx.f:152-191: -CC3M - FSH + range penalties) in one batched CC3M

@@ -1,0 +1,187 @@
+"""The local grid search of a mode-1 refinement (fr_local_search_batch)
+against the oracle's CC3M (cc3m.f:8-169) evaluated at every grid point.
+
+Each orientation of the grid reports the best CC3M over its shift window and
+the shift's index; the oracle evaluates or32_cc3m at every (orientation,
+shift) of the same FP32 grid.  Bars: every reported score within 1e-4
+relative of the oracle's window maximum; the same shift index and, per
+particle, the same best orientation wherever the oracle's first and second
+values are not tied within that tolerance.  FSCT below 0.95 on the outer
+shells and RBFACT > 0 exercise CC3M's |R| branch and its shell weights.
+"""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+import oracle_lib as O  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+if not torch.cuda.is_available():  # pragma: no cover
+    pytest.skip("no HIP device", allow_module_level=True)
+
+from frealign_amd import api, synth  # noqa: E402
+
+DEV = "cuda"
+F = np.float32
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = api.Context(0)
+    yield c
+    c.close()
+
+
+def _case(ctx, nsam, psize, npart, seed, snr=2.0):
+    ri = 0.4 * nsam
+    optics = synth.Optics(psize=psize)
+    sym = api.point_group("C", 1)
+    vol = synth.blob_phantom(nsam, 40, 0.375 * nsam, seed=seed, sym=sym, device=DEV)
+    a3df = synth.half_volume(vol)
+    parts = synth.make_particles(ctx, a3df, npart, seed=seed + 1, optics=optics, snr=snr, ri=ri,
+                                 shift_a=2.0 * psize)
+    gpar = optics.gpar(nsam, ri)
+    pssnr = torch.from_numpy(synth.default_pssnr(nsam, ri)).to(DEV)
+    ctff, outc, _ = ctx.ctfapply(nsam, parts.images.clone(), parts.ppar(), pssnr, gpar)
+    ring = [(ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0]
+    return a3df, outc, ctff, parts, ring
+
+
+def _grid_value(x0, k, n, step):
+    return F(F(x0) + F(F(k - n) * F(step)))
+
+
+def _oracle_window(L, nsam, oc, vol, cf, fs, ring, maxr1, maxr2, rbfact, ang, centre, nsh, dsh):
+    """or32_cc3m at every shift of the window; (values in window order)."""
+    vals = []
+    for ix in range(2 * nsh + 1):
+        sx = _grid_value(centre[0], ix, nsh, dsh)
+        for iy in range(2 * nsh + 1):
+            sy = _grid_value(centre[1], iy, nsh, dsh)
+            vals.append(L.or32_cc3m(nsam, 1.0, oc, vol, maxr1, maxr2, float(ang[0]), float(ang[1]),
+                                    float(ang[2]), float(sx), float(sy), rbfact, fs, cf, *ring,
+                                    O.IDENTITY))
+    return np.array(vals, np.float64)
+
+
+def _check_window(got_s, got_c, want, tol=1e-4):
+    best = float(np.max(want))
+    assert abs(got_s - best) <= tol * abs(best), (got_s, best)
+    order = np.argsort(-want, kind="stable")
+    tied = len(want) > 1 and abs(want[order[0]] - want[order[1]]) <= tol * abs(best)
+    if not tied:
+        assert got_c == int(np.argmax(want)), (got_c, int(np.argmax(want)))
+    return tied
+
+
+@pytest.mark.parametrize("nsam,psize,maxr2,nang,nsh", [(64, 2.0, 14, 2, 2), (96, 1.5, 20, 1, 3)])
+def test_local_search_full_grid_vs_oracle(ctx, nsam, psize, maxr2, nang, nsh):
+    L = O.oracle()
+    npart, maxr1, rbfact = 3, 1, 0.002
+    dang = F(np.deg2rad(1.0))
+    dsh = F(2 * np.pi / nsam)  # one pixel
+    a3df, outc, ctff, parts, ring = _case(ctx, nsam, psize, npart, seed=71)
+    jc = nsam // 2 + 1
+    fsct = np.ones(jc, np.float32)
+    fsct[jc // 2:] = 0.5
+    rng = np.random.default_rng(5)
+    start = np.zeros((npart, 5), np.float32)
+    start[:, :3] = parts.euler + rng.normal(0, np.deg2rad(1.5), (npart, 3)).astype(np.float32)
+    start[:, 3:5] = parts.shift + rng.normal(0, 0.5 * float(dsh), (npart, 2)).astype(np.float32)
+    print(f"[local] nsam={nsam} grid {2 * nang + 1}^3 x {2 * nsh + 1}^2 shifts", flush=True)
+    best, sc, cd = ctx.local_search(nsam, a3df, outc, ctff, start, nang=nang, dang=float(dang),
+                                    nsh=nsh, dsh=float(dsh), maxr1=maxr1, maxr2=maxr2,
+                                    rbfact=rbfact, fsct=torch.from_numpy(fsct).to(DEV), ring=ring,
+                                    all_scores=True)
+    sc, cd = sc.cpu().numpy(), cd.cpu().numpy()
+    oc, cf = outc.cpu().numpy().reshape(npart, -1), ctff.cpu().numpy().reshape(npart, -1)
+    vol = a3df.cpu().numpy().reshape(-1)
+    n = 2 * nang + 1
+    grid = [(a, b, c) for a in range(n) for b in range(n) for c in range(n)]
+
+    def one(job):
+        p, o = job
+        a, b, c = grid[o]
+        ang = (_grid_value(start[p, 0], a, nang, dang), _grid_value(start[p, 1], b, nang, dang),
+               _grid_value(start[p, 2], c, nang, dang))
+        return _oracle_window(L, nsam, oc[p], vol, cf[p], fsct, ring, maxr1, maxr2, rbfact, ang,
+                              start[p, 3:5], nsh, dsh)
+
+    jobs = [(p, o) for p in range(npart) for o in range(len(grid))]
+    print(f"[local] oracle CC3M at {len(jobs) * (2 * nsh + 1) ** 2} points", flush=True)
+    with ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(one, jobs))
+    ties = 0
+    worst = 0.0
+    for (p, o), want in zip(jobs, res):
+        worst = max(worst, abs(sc[p, o] - want.max()) / abs(want.max()))
+        ties += _check_window(sc[p, o], cd[p, o], want)
+    print(f"[local] worst relative score error {worst:.1e}; windows tied within 1e-4: {ties}")
+    for p in range(npart):
+        wmax = np.array([res[p * len(grid) + o].max() for o in range(len(grid))])
+        o = int(np.argmax(wmax))
+        srt = np.sort(wmax)[::-1]
+        a, b, c = grid[o]
+        if abs(srt[0] - srt[1]) > 1e-4 * abs(srt[0]):
+            want_ang = [_grid_value(start[p, 0], a, nang, dang),
+                        _grid_value(start[p, 1], b, nang, dang),
+                        _grid_value(start[p, 2], c, nang, dang)]
+            assert np.array_equal(best[p, :3], np.array(want_ang, np.float32)), (p, best[p])
+        assert abs(best[p, 5] - srt[0]) <= 1e-4 * abs(srt[0])
+
+
+def test_local_search_config2_size_vs_oracle(ctx):
+    """128^2, the mode-1 configuration's grid: +-6 steps of 1 deg (2,197
+    orientations) and +-4 one-pixel shifts (+-4.8 A at 1.2 A/px); 25 sampled
+    orientations per particle and the reported best against the oracle."""
+    L = O.oracle()
+    nsam, psize, maxr1, maxr2, rbfact, nang, nsh = 128, 1.2, 1, 25, 0.0005, 6, 4
+    npart = 2
+    dang = F(np.deg2rad(1.0))
+    dsh = F(2 * np.pi / nsam)
+    a3df, outc, ctff, parts, ring = _case(ctx, nsam, psize, npart, seed=81, snr=0.5)
+    jc = nsam // 2 + 1
+    fsct = synth.default_fsct(nsam, psize).astype(np.float32)
+    rng = np.random.default_rng(9)
+    start = np.zeros((npart, 5), np.float32)
+    start[:, :3] = parts.euler + rng.normal(0, np.deg2rad(3.0), (npart, 3)).astype(np.float32)
+    start[:, 3:5] = parts.shift + rng.normal(0, 2.0 / psize * float(dsh), (npart, 2)).astype(np.float32)
+    print(f"[local] nsam={nsam} grid 13^3 x 9^2", flush=True)
+    best, sc, cd = ctx.local_search(nsam, a3df, outc, ctff, start, nang=nang, dang=float(dang),
+                                    nsh=nsh, dsh=float(dsh), maxr1=maxr1, maxr2=maxr2,
+                                    rbfact=rbfact, fsct=torch.from_numpy(fsct).to(DEV), ring=ring,
+                                    all_scores=True)
+    sc, cd = sc.cpu().numpy(), cd.cpu().numpy()
+    n = 2 * nang + 1
+    oc, cf = outc.cpu().numpy().reshape(npart, -1), ctff.cpu().numpy().reshape(npart, -1)
+    vol = a3df.cpu().numpy().reshape(-1)
+    jobs = []
+    for p in range(npart):
+        picks = set(rng.choice(n ** 3, 24, replace=False).tolist())
+        picks.add(int(np.argmax(sc[p])))
+        jobs += [(p, o) for o in sorted(picks)]
+
+    def one(job):
+        p, o = job
+        a, b, c = o // (n * n), (o // n) % n, o % n
+        ang = (_grid_value(start[p, 0], a, nang, dang), _grid_value(start[p, 1], b, nang, dang),
+               _grid_value(start[p, 2], c, nang, dang))
+        return _oracle_window(L, nsam, oc[p], vol, cf[p], fsct, ring, maxr1, maxr2, rbfact, ang,
+                              start[p, 3:5], nsh, dsh)
+
+    print(f"[local] oracle CC3M at {len(jobs) * (2 * nsh + 1) ** 2} points", flush=True)
+    with ThreadPoolExecutor(8) as ex:
+        res = list(ex.map(one, jobs))
+    worst = 0.0
+    for (p, o), want in zip(jobs, res):
+        worst = max(worst, abs(sc[p, o] - want.max()) / abs(want.max()))
+        _check_window(sc[p, o], cd[p, o], want)
+    print(f"[local] worst relative score error {worst:.1e}")
+    for p in range(npart):
+        o = int(np.argmax(sc[p]))
+        assert best[p, 5] == sc[p, o]
+        # the reported best beats every sampled orientation's oracle maximum
+        mx = max(w.max() for (q, _), w in zip(jobs, res) if q == p)
+        assert best[p, 5] >= mx * (1 - 1e-4)
