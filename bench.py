@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="one device copy of this many MiB first (PMC calibration of the "
                          "HBM byte counters against a known size)")
+    ap.add_argument("--c2-particles", type=int, default=10000,
+                    help="particles of the mode-1 local refinement at 128^2 (0: skip)")
     ap.add_argument("--uneven-batches", type=int, default=1,
                     help="batches of 256 per 'unit' of the 3:2:1 uneven workload (0: skip)")
     return ap.parse_args()
@@ -339,6 +341,13 @@ def main():
                 # the 225-row GEMM form's flops for the same scores
                 "gemm_equivalent_TFLOPs": 2.0 * 225 * (2 * kc) * exh_pairs / g16_avg / 1e12}
 
+    # --------------------------------------- mode-1 local refinement, 128^2
+    c2 = None
+    if args.c2_particles > 0:
+        c2 = config2_workload(ctx, args.c2_particles, log)
+        log(f"config 2: {c2['particles_per_s']:.1f} particles/s, median error "
+            f"{c2['median_angular_error_deg']['refined']:.2f} deg")
+
     # ------------------------------------------------ uneven box sizes
     uneven = None
     if args.uneven_batches > 0:
@@ -398,6 +407,7 @@ def main():
         "roofline": roof,
         "exhaustive_gemm": exh_roof,
         "cpu_baseline": cpu,
+        "local_refinement_128": c2,
         "uneven_batches": uneven,
         "setup_s": setup_s,
     }
@@ -485,12 +495,11 @@ def uneven_workload(ctx, units, log):
     return out
 
 
-def fsc_resolution(ctx, vols, sym) -> float:
+def fsc_resolution(ctx, vols, sym, nsam=NSAM, psize=PSIZE) -> float:
     """FSC(0.143) resolution (A) of the two half-maps: STD/VTD as in
     frealign_v9.f:793-798 (TTD / NKSUM * 0.1), STORESHIFT, SHELTEST; the
     first shell (radius k in Fourier pixels, k >= 2) below 0.143 gives
     NSAM * PSIZE / k."""
-    nsam = NSAM
     nk = float(vols.nksum.sum().item())
     ttd = vols.ttd.cpu().numpy()
     std, vtd = ttd[0] / nk * 0.1, ttd[1] / nk * 0.1
@@ -499,8 +508,122 @@ def fsc_resolution(ctx, vols, sym) -> float:
     fsc = ctx.sheltest(nsam, b, c, vols.ctf2[0], vols.ctf2[1])["fsc"]
     for k in range(2, len(fsc)):
         if fsc[k] < 0.143:
-            return float(nsam * PSIZE / k)
-    return float(2.0 * PSIZE)
+            return float(nsam * psize / k)
+    return float(2.0 * psize)
+
+
+def config2_workload(ctx, npart, log):
+    """Mode-1 local refinement at 128^2 (the FP32 configuration): a 128^3 C1
+    blob phantom (40 blobs in a radius-56 px ball, 1.2 A/px), particles at
+    SNR 0.1 with +-3 A shifts and the CTF as above, started N(0, 3 deg) in
+    each Euler angle and N(0, 2 A) in each shift from the truth.  Per batch of
+    256: CTFAPPLY, the local grid search (+-6 steps of 1 deg = 2,197
+    orientations, +-4 one-pixel shifts = +-4.8 A, CC3M on a 6 A band), PREFINE
+    from its best point, and trilinear insertion into the half-maps.  Returns
+    particles/s with every stage inside the timed region, the stage times,
+    the angular and shift errors against the truth before and after, and the
+    FSC(0.143) of the refined half-maps next to that at the true parameters."""
+    import torch
+    from frealign_amd import api, search, synth
+
+    dev = "cuda"
+    nsam, psize, ri, B = 128, 1.2, 56.0, 256
+    nb = max(1, (npart + B - 1) // B)
+    sym = api.point_group("C", 1)
+    optics = synth.Optics(psize=psize)
+    vol = synth.blob_phantom(nsam, 40, ri, seed=0xC2, sym=sym, device=dev)
+    a3df = synth.half_volume(vol)
+    del vol
+    parts = synth.make_particles(ctx, a3df, nb * B, seed=0xC2 + 1, optics=optics, snr=0.1,
+                                 ri=ri, shift_a=3.0)
+    gpar = optics.gpar(nsam, ri)
+    pssnr = torch.from_numpy(synth.default_pssnr(nsam, ri)).to(dev)
+    fsct = torch.from_numpy(synth.default_fsct(nsam, psize).astype(np.float32)).to(dev)
+    ring = np.array([gpar[7], gpar[8], gpar[5], gpar[6]], np.float32)
+    maxr1, maxr2 = max(1, int(nsam * psize / 100.0)), int(nsam * psize / 6.0)
+    rng = np.random.default_rng(0xC2)
+    dph = np.float32(2 * math.pi / nsam / psize)  # phase units per A
+    start = np.zeros((nb * B, 5), np.float32)
+    start[:, :3] = parts.euler + np.radians(rng.normal(0, 3.0, (nb * B, 3))).astype(np.float32)
+    start[:, 3:5] = parts.shift + (rng.normal(0, 2.0, (nb * B, 2)) * dph).astype(np.float32)
+    sx = np.float32(2.0 * dph)
+    lr = search.LocalRefinement(ctx, a3df, maxr1=maxr1, maxr2=maxr2, rbfact=0.0,
+                                prior=(0.0, 0.0, sx, sx, 0.01))
+    ppar = parts.ppar(dev)
+
+    def recb_of(par):
+        r = np.zeros((len(par), 10), np.float32)
+        r[:, 0:5] = par[:, 0:5]
+        r[:, 5:8] = 1.0
+        r[:, 8] = 0.5
+        r[:, 9] = 0.5
+        return r
+
+    half = (np.arange(B) % 2).astype(np.int32)
+
+    def one(k, vols, stage):
+        lo = k * B
+        t0 = time.perf_counter()
+        ctff, outc, datc = ctx.ctfapply(nsam, parts.images[lo:lo + B].clone(), ppar[lo:lo + B],
+                                        pssnr, gpar)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        grid, ref, nev = lr.run(outc, ctff, start[lo:lo + B], fsct, ring)
+        t2 = time.perf_counter()
+        ctx.insert(nsam, datc, ctff, recb_of(ref), half, sym, 5.0, 40.0, psize, vols)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        stage["prep"] += t1 - t0
+        stage["search_and_refine"] += t2 - t1
+        stage["insert"] += t3 - t2
+        return grid, ref, nev
+
+    stage = {"prep": 0.0, "search_and_refine": 0.0, "insert": 0.0}
+    one(0, api.HalfMaps.zeros(nsam), dict(stage))  # warm
+    vols = api.HalfMaps.zeros(nsam)
+    grids, refs, nevs = [], [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(nb):
+        g, r, n = one(k, vols, stage)
+        grids.append(g)
+        refs.append(r)
+        nevs.append(n)
+        if (k + 1) % 8 == 0 or k + 1 == nb:
+            log(f"config 2: {(k + 1) * B} particles, {time.perf_counter() - t0:.1f} s")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    grid, ref = np.concatenate(grids), np.concatenate(refs)
+    e_start = search.angular_error_deg(start[:, :3], parts.euler)
+    e_grid = search.angular_error_deg(grid[:, :3], parts.euler)
+    e_ref = search.angular_error_deg(ref[:, :3], parts.euler)
+    sh_err = lambda p: np.hypot(*(p[:, 3:5] - parts.shift).T) / dph  # noqa: E731
+    res_ref = fsc_resolution(ctx, vols, sym, nsam, psize)
+    del vols
+    # the same particles inserted at their true parameters
+    vt = api.HalfMaps.zeros(nsam)
+    truth = np.concatenate([parts.euler, parts.shift], axis=1)
+    for k in range(nb):
+        lo = k * B
+        ctff, outc, datc = ctx.ctfapply(nsam, parts.images[lo:lo + B].clone(), ppar[lo:lo + B],
+                                        pssnr, gpar)
+        ctx.insert(nsam, datc, ctff, recb_of(truth[lo:lo + B]), half, sym, 5.0, 40.0, psize, vt)
+    res_true = fsc_resolution(ctx, vt, sym, nsam, psize)
+    del vt
+    n = nb * B
+    return {"workload": "mode-1 local refinement, 128^2 C1, 1.2 A/px, SNR 0.1: +-6 deg at 1 deg "
+                        "(2,197 orientations) x +-4.8 A at one pixel (81 shifts) scored by CC3M "
+                        "on a 6 A band, then PREFINE, then trilinear insertion",
+            "particles": n, "particles_per_s": n / dt,
+            "stage_s": stage, "cc3m_evaluations_per_particle_prefine": float(np.mean(np.concatenate(nevs))),
+            "grid_cc3m_evaluations_per_particle": lr.norient * (2 * lr.nsh + 1) ** 2,
+            "median_angular_error_deg": {"start": float(np.median(e_start)),
+                                         "grid": float(np.median(e_grid)),
+                                         "refined": float(np.median(e_ref))},
+            "within_1deg_of_truth_refined": float(np.mean(e_ref <= 1.0)),
+            "median_shift_error_A": {"start": float(np.median(sh_err(start))),
+                                     "refined": float(np.median(sh_err(ref)))},
+            "fsc0143_A": {"refined": res_ref, "true_parameters": res_true}}
 
 
 def finalisation(ctx, vols, sym):

@@ -188,3 +188,51 @@ def angular_error_deg(e1: np.ndarray, e2: np.ndarray, sym_mats=None) -> np.ndarr
         tr = np.clip((np.trace(rel, axis1=1, axis2=2) - 1.0) / 2.0, -1.0, 1.0)
         best = np.minimum(best, np.degrees(np.arccos(tr)))
     return best
+
+
+class LocalRefinement:
+    """Mode-1 refinement (lmain.f:743-828) of particles near known parameters:
+    the local grid search (every orientation of a (2 nang + 1)^3 grid DANG
+    apart around the start, each at every shift of a (2 nsh + 1)^2 window DSH
+    apart, scored by CC3M: fr_local_search_batch), then PREFINE (prefine.f,
+    VA04A over CALCFX = -CC3M - FSH; fr_refine_batch) from the best grid
+    point.  Shifts are in phase units (SHX = 2 pi dx / NSAM for dx pixels).
+
+    prior: (XM - SHX, YM - SHY, SX, SY, SIG2) of FSH (fsh.f), one for all
+    particles (the dataset's shift statistics, card10.f:541-566)."""
+
+    def __init__(self, ctx: api.Context, a3df: torch.Tensor, *, maxr1: int, maxr2: int,
+                 rbfact: float, nang: int = 6, dang_deg: float = 1.0, nsh: int = 4,
+                 dsh: float | None = None, prior=(0.0, 0.0, 0.1, 0.1, 0.01),
+                 mask=(1, 1, 1, 1, 1), maxit: int = 50):
+        self.ctx = ctx
+        self.a3df = a3df
+        self.nsam = a3df.shape[0]
+        self.maxr1, self.maxr2, self.rbfact = maxr1, maxr2, rbfact
+        self.nang, self.nsh = nang, nsh
+        self.dang = float(np.float32(np.deg2rad(dang_deg)))
+        self.dsh = float(np.float32(2 * math.pi / self.nsam)) if dsh is None else float(dsh)
+        self.prior = np.asarray(prior, np.float32)
+        self.mask = np.asarray(mask, np.int32)
+        self.maxit = maxit
+
+    @property
+    def norient(self) -> int:
+        return (2 * self.nang + 1) ** 3
+
+    def run(self, outc, ctff, start, fsct, ring):
+        """start (np, 5) host PHI THETA PSI SHX SHY.  Returns (grid (np, 6):
+        the best grid point and its CC3M, refined (np, 6): PREFINE's
+        parameters and PRESA, CC3M evaluations per particle)."""
+        fs = fsct if torch.is_tensor(fsct) else torch.as_tensor(
+            np.ascontiguousarray(fsct, np.float32), device=outc.device)
+        grid = self.ctx.local_search(self.nsam, self.a3df, outc, ctff, start, nang=self.nang,
+                                     dang=self.dang, nsh=self.nsh, dsh=self.dsh,
+                                     maxr1=self.maxr1, maxr2=self.maxr2, rbfact=self.rbfact,
+                                     fsct=fs, ring=ring)
+        n = grid.shape[0]
+        prior = np.repeat(self.prior[None], n, axis=0)
+        ref, nev = self.ctx.refine(self.nsam, self.a3df, outc, ctff, np.arange(n, dtype=np.int32),
+                                   np.ones(n, np.float32), grid, self.mask, self.maxr1,
+                                   self.maxr2, self.rbfact, fs, ring, prior, maxit=self.maxit)
+        return grid, ref, nev

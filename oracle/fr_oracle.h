@@ -185,6 +185,15 @@ typedef float (*or_objective)(int n, const float *x, void *user);
 float or_va04a(float *x, const float *e, int n, float escale, int iprint, int icon, int maxit,
                or_objective fx, void *user, int *nevals);
 
+/* prefine.f:51-145 (non-helical, AMAG > 0, IVFLAG = 0): VA04A over the
+ * MASK-selected PHI THETA PSI DSHX DSHY of par[5] (in/out) minimising CALCFX
+ * (calcfx.f:152-191: -or32_cc3m - FSH + the range penalties); prior =
+ * (XM-SHX, YM-SHY, SX, SY, SIG2) of FSH (fsh.f).  Returns PRESA. */
+float or32_prefine(int nsam, float amag, const float *spec, const float *a3df, int maxr1,
+                   int maxr2, float *par, const int mask[5], float rbfact, const float *fsct,
+                   const float *ctff, float ri2, float ri3, float rih, float halfw,
+                   const float prior[5], const float *sm, int *nevals);
+
 /* Closed-form test objectives shared with oracle/ref_va04_obj.f90 (same FP32
  * operation order): 1 coupled quadratic, 2 Rosenbrock chain, 3 rational bump. */
 float or_va04_testfn(int n, const float *x, void *which);

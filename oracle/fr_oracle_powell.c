@@ -416,3 +416,76 @@ float or_va04_testfn(int n, const float *x, void *which) {
   }
   return rf;
 }
+
+/* ------------------------------------------------------------ prefine.f */
+/* PREFINE (prefine.f:51-145; non-helical, AMAG > 0, IVFLAG = 0) over the
+ * MASK-selected PHI THETA PSI DSHX DSHY: VA04A with steps EPAR (prefine.f:48),
+ * ESCALE 100, ICON 1, MAXIT NCYCLS = 50, minimising CALCFX
+ * (calcfx.f:152-191): -CC3M(|AMAG|) - FSH(SIG2, DSHX, DSHY, XM-SHX, YM-SHY,
+ * SX, SY) (fsh.f) plus the out-of-range penalties.  par[5] in/out; prior =
+ * (XM-SHX, YM-SHY, SX, SY, SIG2).  Returns PRESA = -F; *nevals the CC3M
+ * evaluations. */
+struct prefine_ctx {
+  int nsam, maxr1, maxr2;
+  float amag, rbfact, ri2, ri3, rih, halfw;
+  const float *spec, *a3df, *fsct, *ctff, *sm, *prior;
+  const int *mask;
+  float xpar[5];
+};
+
+static float fsh(float sig2, float x, float y, float xm, float ym, float sx, float sy) {
+  const float dx = x - xm, dy = y - ym;
+  return sig2 * (-(dx * dx) / 2.0f / (sx * sx) - (dy * dy) / 2.0f / (sy * sy));
+}
+
+static float calcfx(int n, const float *xmap, void *user) {
+  struct prefine_ctx *c = (struct prefine_ctx *)user;
+  const float pi = 3.1415926535897f;
+  float x[5];
+  int ip = 0;
+  for (int j = 0; j < 5; j++) x[j] = c->xpar[j];
+  for (int j = 0; j < 5; j++)
+    if (c->mask[j] == 1) x[j] = xmap[ip++];
+  (void)n;
+  float cc = (float)or32_cc3m(c->nsam, c->amag, c->spec, c->a3df, c->maxr1, c->maxr2, x[0], x[1],
+                              x[2], x[3], x[4], c->rbfact, c->fsct, c->ctff, c->ri2, c->ri3,
+                              c->rih, c->halfw, c->sm);
+  float rft = -cc - fsh(c->prior[4], x[3], x[4], c->prior[0], c->prior[1], c->prior[2],
+                        c->prior[3]);
+  for (int j = 0; j < 3; j++)
+    if (fabsf(x[j]) > pi) rft = rft + (pi - fabsf(x[j])) * (pi - fabsf(x[j]));
+  for (int j = 3; j < 5; j++)
+    if (fabsf(x[j]) > 1.0f) rft = rft + (1.0f - fabsf(x[j])) * (1.0f - fabsf(x[j]));
+  return rft;
+}
+
+float or32_prefine(int nsam, float amag, const float *spec, const float *a3df, int maxr1,
+                   int maxr2, float *par, const int mask[5], float rbfact, const float *fsct,
+                   const float *ctff, float ri2, float ri3, float rih, float halfw,
+                   const float prior[5], const float *sm, int *nevals) {
+  static const float epar[5] = {0.1f, 0.1f, 0.1f, 0.01f, 0.01f};
+  struct prefine_ctx c = {nsam, maxr1, maxr2, fabsf(amag), rbfact, ri2, ri3, rih, halfw,
+                          spec, a3df, fsct, ctff, sm, prior, mask, {0}};
+  float xmap[5], emap[5];
+  int npar = 0;
+  for (int j = 0; j < 5; j++) c.xpar[j] = par[j];
+  for (int j = 0; j < 5; j++)
+    if (mask[j] == 1) {
+      xmap[npar] = par[j];
+      emap[npar] = epar[j];
+      npar++;
+    }
+  float f;
+  int ne = 0;
+  if (npar != 0) {
+    f = or_va04a(xmap, emap, npar, 100.0f, 0, 1, 50, calcfx, &c, &ne);
+  } else {
+    f = calcfx(0, xmap, &c);
+    ne = 1;
+  }
+  int ip = 0;
+  for (int j = 0; j < 5; j++)
+    if (mask[j] == 1) par[j] = xmap[ip++];
+  if (nevals) *nevals = ne;
+  return -f;
+}

@@ -71,6 +71,9 @@ def _declare_oracle(L):
     L.or_va04a.argtypes = [f32p, f32p, ci, cf, ci, ci, ci, C.c_void_p, C.c_void_p, pi_]
     L.or_va04a.restype = cf
     L.or_calcsinc.argtypes = [f32p, ci]
+    L.or32_prefine.argtypes = [ci, cf, c64p, c64p, ci, ci, f32p, i32p, cf, f32p, c64p, cf, cf, cf,
+                               cf, f32p, C.c_void_p, pi_]
+    L.or32_prefine.restype = cf
     L.or_combine_counters.argtypes = [ci, ci, ci, C.POINTER(C.c_int64), f64p, f64p,
                                       C.POINTER(C.c_int64), pd, pd]
     L.or_rfft2_f.argtypes = [ci, f32p]

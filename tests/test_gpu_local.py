@@ -185,3 +185,38 @@ def test_local_search_config2_size_vs_oracle(ctx):
         # the reported best beats every sampled orientation's oracle maximum
         mx = max(w.max() for (q, _), w in zip(jobs, res) if q == p)
         assert best[p, 5] >= mx * (1 - 1e-4)
+
+
+def test_local_refinement_pipeline_128(ctx):
+    """Config 2's mode-1 step at 128^2 (1.2 A/px, 6 A band) on 16 particles
+    started N(0, 3 deg) / N(0, 2 A) from the truth: the grid search and then
+    PREFINE.  PREFINE starts from the grid's best point and never ends below
+    it (PRESA >= its CC3M + FSH there); the refined orientations are closer
+    to the truth than the starts."""
+    from frealign_amd import search
+
+    nsam, psize, npart = 128, 1.2, 16
+    a3df, outc, ctff, parts, ring = _case(ctx, nsam, psize, npart, seed=91, snr=0.3)
+    fsct = synth.default_fsct(nsam, psize).astype(np.float32)
+    maxr1, maxr2 = 1, int(nsam * psize / 6.0)
+    rng = np.random.default_rng(13)
+    start = np.zeros((npart, 5), np.float32)
+    start[:, :3] = parts.euler + rng.normal(0, np.deg2rad(3.0), (npart, 3)).astype(np.float32)
+    dph = 2 * np.pi / nsam / psize  # phase units per A
+    start[:, 3:5] = parts.shift + rng.normal(0, 2.0 * dph, (npart, 2)).astype(np.float32)
+    sx = np.float32(2.0 * dph)
+    lr = search.LocalRefinement(ctx, a3df, maxr1=maxr1, maxr2=maxr2, rbfact=0.0,
+                                prior=(0.0, 0.0, sx, sx, 0.01))
+    grid, ref, nev = lr.run(outc, ctff, start, torch.from_numpy(fsct).to(DEV), ring)
+    f = np.float32
+    for p in range(npart):
+        fs = f(0.01) * (-(grid[p, 3] * grid[p, 3]) / f(2) / (sx * sx)
+                        - (grid[p, 4] * grid[p, 4]) / f(2) / (sx * sx))
+        # FSH centres on XM - SHX = 0 here: the prior pulls towards zero shift
+        assert ref[p, 5] >= grid[p, 5] + fs - 1e-5, (p, ref[p, 5], grid[p, 5], fs)
+    e0 = search.angular_error_deg(start[:, :3], parts.euler)
+    e1 = search.angular_error_deg(grid[:, :3], parts.euler)
+    e2 = search.angular_error_deg(ref[:, :3], parts.euler)
+    print(f"[local] angular error to truth: start {np.median(e0):.2f}, grid {np.median(e1):.2f}, "
+          f"refined {np.median(e2):.2f} deg (median); CC3M evaluations {np.mean(nev):.0f}")
+    assert np.median(e2) < np.median(e0) and np.median(e1) < np.median(e0)

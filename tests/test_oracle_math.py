@@ -504,3 +504,76 @@ def test_sigma_phaseflip_planted_sign_pattern():
     x0 = np.fft.irfft2(np.where(band, p * s, 0), s=(nsam, nsam)) * nsam * nsam
     s10 = np.sum(x0[disc] ** 2) - np.sum(x0[disc]) ** 2 / n1
     assert abs(s10 - sum1) > 1e-3 * sum1
+
+
+def _prefine_case(nsam=32, seed=21):
+    jc = nsam // 2 + 1
+    vol = _volume(nsam, seed)
+    r = np.random.default_rng(seed)
+    ctff = (0.6 + 0.1 * r.standard_normal(2 * nsam * jc)).astype(np.complex128)
+    eul = (0.5, 1.2, -0.8)
+    maxr1, maxr2 = 1, 13
+    ri = 0.4 * nsam
+    ring = ((ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0)
+    ainpo, it, keep = _masked_projection(nsam, vol, eul, ctff, maxr1, maxr2, ring)
+    i = np.tile(np.arange(jc), nsam)
+    m = np.repeat(np.arange(nsam), jc)
+    j = np.where(m >= jc, m - nsam, m)
+    sx, sy = 0.11, -0.07
+    moved = (ainpo * np.exp(-1j * (sx * i + sy * j))).astype(np.complex64)
+    fsct = np.ones(jc, np.float32)
+    return nsam, vol, ctff.astype(np.complex64), eul, (sx, sy), moved, fsct, maxr1, maxr2, ring
+
+
+def _prefine(case, start, mask, prior):
+    nsam, vol, ctff, eul, sh, spec, fsct, maxr1, maxr2, ring = case
+    par = np.asarray(start, np.float32).copy()
+    ne = C.c_int(0)
+    presa = L.or32_prefine(nsam, 1.0, spec, vol, maxr1, maxr2, par, np.asarray(mask, np.int32),
+                           0.0, fsct, ctff, *[float(v) for v in ring],
+                           np.asarray(prior, np.float32), None, C.byref(ne))
+    return par, presa, ne.value
+
+
+def _cc3m32(case, x):
+    nsam, vol, ctff, eul, sh, spec, fsct, maxr1, maxr2, ring = case
+    return np.float32(L.or32_cc3m(nsam, 1.0, spec, vol, maxr1, maxr2, *[float(v) for v in x],
+                                  0.0, fsct, ctff, *ring, O.IDENTITY))
+
+
+def test_prefine_recovers_planted_shift_and_orientation():
+    """PREFINE (prefine.f + calcfx.f over VA04A, which is pinned to va04.f) on a
+    noise-free particle: the masked CTF projection at a known view, moved by a
+    known shift.  From zero shift (shifts only) and from a start 2 degrees
+    off (all five), it returns the planted values, and PRESA is the self-match
+    CC3M there (no prior: SIG2 = 0)."""
+    case = _prefine_case()
+    eul, (sx, sy) = case[3], case[4]
+    best = _cc3m32(case, (*eul, sx, sy))
+    par, presa, ne = _prefine(case, (*eul, 0.0, 0.0), (0, 0, 0, 1, 1), (0, 0, 0.1, 0.1, 0.0))
+    assert abs(par[3] - sx) < 2e-3 and abs(par[4] - sy) < 2e-3, par
+    assert abs(presa - best) < 1e-4 and ne > 5
+    d = np.radians(2.0)
+    par, presa, ne = _prefine(case, (eul[0] + d, eul[1] - d, eul[2] + d, 0.0, 0.0), (1, 1, 1, 1, 1),
+                              (0, 0, 0.1, 0.1, 0.0))
+    assert np.max(np.abs(par[:3] - np.asarray(eul, np.float32))) < np.radians(0.3), par
+    assert abs(par[3] - sx) < 3e-3 and abs(par[4] - sy) < 3e-3
+    assert abs(presa - best) < 1e-3
+
+
+def test_prefine_shift_prior_sign_and_presa():
+    """FSH (fsh.f) enters CALCFX with a minus sign, so a strong prior centred
+    on zero pulls the refined shift towards zero and one centred on the planted
+    shift keeps it there; PRESA equals CC3M + FSH at the returned point (the
+    range penalties are zero inside +-pi, +-1)."""
+    case = _prefine_case()
+    eul, (sx, sy) = case[3], case[4]
+    free, _, _ = _prefine(case, (*eul, 0.0, 0.0), (0, 0, 0, 1, 1), (0, 0, 0.05, 0.05, 0.0))
+    pull, presa, _ = _prefine(case, (*eul, 0.0, 0.0), (0, 0, 0, 1, 1), (0, 0, 0.05, 0.05, 1.0))
+    held, _, _ = _prefine(case, (*eul, 0.0, 0.0), (0, 0, 0, 1, 1), (sx, sy, 0.05, 0.05, 1.0))
+    assert np.hypot(pull[3], pull[4]) < np.hypot(free[3], free[4]) - 1e-3
+    assert np.hypot(held[3] - sx, held[4] - sy) < 3e-3
+    f = np.float32
+    fsh = f(1.0) * (-(pull[3] * pull[3]) / f(2) / (f(0.05) * f(0.05))
+                    - (pull[4] * pull[4]) / f(2) / (f(0.05) * f(0.05)))
+    assert abs(presa - (_cc3m32(case, pull) + fsh)) < 1e-6
