@@ -34,6 +34,7 @@ SIGNATURES = {
     "fr_prep_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp], ci),
     "fr_pextract_batch": ([vp, ci, ci, vp, vp, vp, vp, vp], ci),
     "fr_cc3m_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
+    "fr_cc3_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, ci, ci, cf, vp, vp, vp], ci),
     "fr_presb_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, vp, vp, vp], ci),
     "fr_sigma_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, cf, vp, cf, vp, vp, vp], ci),
     "fr_plan_create": ([vp, ci, ci, vp, vp, vp, ci, ci, ci, cf, vp, vp, ci, vp, vp, cf, cf, cf,
@@ -71,6 +72,10 @@ SIGNATURES = {
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_local_search_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, ci, cf, ci, cf, ci, ci, cf, vp,
                                vp, vp, vp, vp], ci),
+    "fr_ctfrefine_batch": ([vp, ci, ci, vp, vp, vp, ci, vp, ci, cf, vp, vp, vp, vp, cf, cf, cf,
+                            vp, vp, vp], ci),
+    "fr_magrefine_batch": ([vp, ci, ci, vp, vp, vp, ci, vp, vp, vp, vp, cf, cf, cf, vp, vp, vp],
+                           ci),
     "fr_sheltest_noalloc": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_storeshift": ([vp, ci, cd, cd, ci, vp, vp, vp, vp, vp, vp], ci),
     "fr_combine_arrays": ([vp, ci, C.c_longlong, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp,

@@ -536,6 +536,46 @@ class Context:
                 "fr_refine_batch")
         return p, nev
 
+    def ctfrefine(self, nsam: int, spectra, ppar, euler, groups, a3df, pssnr, gpar, *,
+                  rmax1: float, rmax2: float, rbfact: float, ring, fastig: bool = False,
+                  dfstd: float = 100.0, sig2=None, magnification: bool = False):
+        """CTFREFINE (or MAGREFINE with magnification=True) for groups of
+        consecutive particles (fr_ctfrefine_batch / fr_magrefine_batch).
+        spectra: the (np, NSAM, NSAM+2) float32 r2c spectra fr_prep_batch leaves
+        in its image buffer; ppar (np, 8) and euler (np, 3) host; groups: the
+        group sizes.  Returns (np.ndarray (ng, 4): dDF1 dDF2 dANGAST or AMAG 0 0,
+        and the final objective; evaluations per group)."""
+        _dev(spectra, torch.float32, "spectra")
+        _dev(a3df, torch.complex64, "a3df")
+        _dev(pssnr, torch.float32, "pssnr")
+        np_ = spectra.numel() // (nsam * (nsam + 2))
+        pp = np.ascontiguousarray(np.asarray(ppar, np.float32).reshape(np_, 8))
+        eu = np.ascontiguousarray(np.asarray(euler, np.float32).reshape(np_, 3))
+        sizes = np.asarray(groups, np.int64)
+        gs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int32)
+        if gs[-1] != np_:
+            raise ValueError("groups must cover the particles")
+        ng = len(sizes)
+        out = np.zeros((ng, 4), np.float32)
+        nev = np.zeros(ng, np.int32)
+        g = np.asarray(gpar, np.float32)
+        r = np.asarray(ring, np.float32)
+        self._sync_stream()
+        if magnification:
+            N.check(N.lib().fr_magrefine_batch(self.h, nsam, np_, _p(spectra), _p(pp), _p(eu), ng,
+                                               _p(gs), _p(a3df), _p(pssnr), _p(g), float(rmax1),
+                                               float(rmax2), float(rbfact), _p(r), _p(out),
+                                               _p(nev)), "fr_magrefine_batch")
+        else:
+            s2 = np.zeros(ng, np.float32) if sig2 is None else \
+                np.ascontiguousarray(np.asarray(sig2, np.float32).reshape(ng))
+            N.check(N.lib().fr_ctfrefine_batch(self.h, nsam, np_, _p(spectra), _p(pp), _p(eu), ng,
+                                               _p(gs), int(bool(fastig)), float(dfstd), _p(s2),
+                                               _p(a3df), _p(pssnr), _p(g), float(rmax1),
+                                               float(rmax2), float(rbfact), _p(r), _p(out),
+                                               _p(nev)), "fr_ctfrefine_batch")
+        return out, nev
+
     def local_search(self, nsam: int, a3df, spec, ctff, par, *, nang: int, dang: float,
                      nsh: int, dsh: float, maxr1: int, maxr2: int, rbfact: float, fsct, ring,
                      amag=None, sm=None, all_scores: bool = False):

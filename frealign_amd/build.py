@@ -35,6 +35,7 @@ SOURCES = [
     "fr_ccp.hip",
     "fr_psearch.cpp",
     "fr_local.hip",
+    "fr_ctfrefine.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -357,6 +357,25 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
                   const float* d_ctff, const int* d_pidx, const float* d_dm,
                   const float* d_shift, const float* d_amag, int maxr1, int maxr2, float rbfact,
                   const float* d_fsct, const float* ring, float* d_score) {
+  return fr::cc3m_batch(c, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, maxr1,
+                        maxr2, rbfact, d_fsct, ring, d_score, 0);
+}
+
+int fr_cc3_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+                 const float* d_ctff, const int* d_pidx, const float* d_dm, const float* d_shift,
+                 const float* d_amag, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+                 const float* ring, float* d_score) {
+  return fr::cc3m_batch(c, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, maxr1,
+                        maxr2, rbfact, d_fsct, ring, d_score, 1);
+}
+
+}  // extern "C"
+
+namespace fr {
+int cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+               const float* d_ctff, const int* d_pidx, const float* d_dm, const float* d_shift,
+               const float* d_amag, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+               const float* ring, float* d_score, int cc3) {
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(n >= 0 && d_a3df && d_spec && d_ctff && d_pidx && d_dm && d_shift && d_amag &&
@@ -365,6 +384,7 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   FR_CHECK(maxr1 >= 0 && maxr2 > maxr1, "bad resolution band %d..%d", maxr1, maxr2);
   if (n == 0) return 0;
   Cc3mArgs a{nsam, 0, maxr1, maxr2, rbfact, ring[0], ring[1], ring[2], ring[3]};
+  a.cc3 = cc3;
   const fr_ctx::Tune tn = c->tune_for(nsam);
   if (fr::cc3m_fused_ok(nsam, maxr2) && tn.cc3m_fused) {
     Cc3mTables tabs;
@@ -397,6 +417,9 @@ int fr_cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* 
   c->prof_end(pslot);
   return 0;
 }
+}  // namespace fr
+
+extern "C" {
 
 int fr_presb_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
                    const float* d_ctff, const int* d_pidx, const float* d_dm,

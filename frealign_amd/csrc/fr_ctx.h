@@ -47,6 +47,11 @@ struct fr_ctx : public fr::Ctx {
 };
 
 namespace fr {
+// fr_cc3m_batch (cc3 = 0) and fr_cc3_batch (cc3 = 1)
+int cc3m_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
+               const float* d_ctff, const int* d_pidx, const float* d_dm, const float* d_shift,
+               const float* d_amag, int maxr1, int maxr2, float rbfact, const float* d_fsct,
+               const float* ring, float* d_score, int cc3);
 // fr_refine_batch with the symmetry matrix SM applied to every DM (null: identity)
 int refine_items(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d_spec,
                  const float* d_ctff, const int* pidx, const float* amag, float* par,

@@ -63,11 +63,12 @@ int launch_pextract(Ctx& c, int nsam, int n, const float* a3df, const float* dm,
 namespace {
 
 // cc3m.f:74-112: the CTF-multiplied annulus, scaled by 1/NSAM^2, into the
-// real-FFT buffer of each item (zero elsewhere).
+// real-FFT buffer of each item (zero elsewhere).  flip: CC3's phase flip
+// instead (cc3.f:86-89, the section negated where REAL(CTFR+CONJG(CTFL)) < 0).
 __global__ void __launch_bounds__(kThreads)
     k_cc3m_fill(int nsam, int n, int maxr12, int maxr22, const float2* __restrict__ a3df,
                 const float* __restrict__ dm, const int* __restrict__ pidx,
-                const float2* __restrict__ ctff, float* work) {
+                const float2* __restrict__ ctff, float* work, int flip) {
   const int jc = nsam / 2 + 1;
   const size_t ic = (size_t)nsam * jc;
   const size_t gid = blockIdx.x * (size_t)kThreads + threadIdx.x;
@@ -83,7 +84,11 @@ __global__ void __launch_bounds__(kThreads)
     const float2* cp = ctff + (size_t)pidx[it] * 2 * ic;
     cf a = ainterpo_dev(nsam, a3df, d, i, j);
     float2 cr = cp[id], cl = cp[id + ic];
-    a = cmul(a, cf{cr.x + cl.x, cr.y - cl.y});
+    if (flip) {
+      if (cr.x + cl.x < 0.0f) a = cf{-a.re, -a.im};
+    } else {
+      a = cmul(a, cf{cr.x + cl.x, cr.y - cl.y});
+    }
     const float scal = 1.0f / (float)nsam / (float)nsam;
     out = make_float2(a.re * scal, a.im * scal);
   }
@@ -95,7 +100,8 @@ __global__ void __launch_bounds__(kThreads)
 __global__ void __launch_bounds__(kThreads)
     k_cc3m_score(int nsam, int maxr12, int maxr22, float rbfact, const float2* __restrict__ spec,
                  const int* __restrict__ pidx, const float* __restrict__ shift,
-                 const float* __restrict__ fsct, const float* __restrict__ work, float* score) {
+                 const float* __restrict__ fsct, const float* __restrict__ work, float* score,
+                 int flip) {
   __shared__ float bins[4 * 260];
   const int jc = nsam / 2 + 1;
   const size_t ic = (size_t)nsam * jc;
@@ -132,7 +138,7 @@ __global__ void __launch_bounds__(kThreads)
       const float* b = bins + 4 * (i - 1);
       if (b[3] != 0.0f) {
         float r = b[0] / sqrtf(b[2]);
-        if (fsct[i - 1] < 0.95f) r = fabsf(r);
+        if (!flip && fsct[i - 1] < 0.95f) r = fabsf(r);
         float wgt = expf(-rbfact * (float)((i - 1) * (i - 1)));
         s3 += (double)(r * wgt);
         s2 += (double)(wgt * wgt * b[1]);
@@ -243,7 +249,7 @@ __global__ void __launch_bounds__(kFT)
                  const int* __restrict__ pts, const float* __restrict__ ring_tab, int ring_cnt,
                  const float* __restrict__ wmap, float ri2,
                  float ri3, float rih, float halfw, float* score, Cc3mWindow win,
-                 int* __restrict__ wcode) {
+                 int* __restrict__ wcode, int flip) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
   float* Ts = lds;                      // [nsam][kFP]
   float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
@@ -282,7 +288,11 @@ __global__ void __launch_bounds__(kFT)
         size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
         cf a = ainterpo_dev(nsam, a3df, d, i, j);
         float2 cr = cp[id], cl = cp[id + ic];
-        a = cmul(a, cf{cr.x + cl.x, cr.y - cl.y});
+        if (flip) {  // cc3.f:86-89
+          if (cr.x + cl.x < 0.0f) a = cf{-a.re, -a.im};
+        } else {
+          a = cmul(a, cf{cr.x + cl.x, cr.y - cl.y});
+        }
         out = make_float2(a.re * scal, a.im * scal);
       }
       Xs[e] = out;
@@ -548,7 +558,7 @@ __global__ void __launch_bounds__(kFT)
       const float* b = bins + 4 * (i - 1);
       if (b[3] != 0.0f) {
         float r = b[0] / sqrtf(b[2]);
-        if (fsct[i - 1] < 0.95f) r = fabsf(r);
+        if (!flip && fsct[i - 1] < 0.95f) r = fabsf(r);
         float wgt = expf(-rbfact * (float)((i - 1) * (i - 1)));
         s3 += (double)(r * wgt);
         s2 += (double)(wgt * wgt * b[1]);
@@ -658,7 +668,7 @@ __global__ void __launch_bounds__(kFT)
       }
       const float ra = sqrtf(bins[4 * sh + 2]);
       float r0 = q0 / ra, r1 = q1 / ra;
-      if (fsct[sh] < 0.95f) {
+      if (!flip && fsct[sh] < 0.95f) {
         r0 = fabsf(r0);
         r1 = fabsf(r1);
       }
@@ -828,7 +838,8 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
                        reinterpret_cast<const float2*>(a3df), dm, pidx,
                        reinterpret_cast<const float2*>(ctff),
                        reinterpret_cast<const float2*>(spec), shift, amag, fsct, t.dft, t.pts,
-                       t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3, a.rih, a.halfw, score, w, wcode);
+                       t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3, a.rih, a.halfw, score, w, wcode,
+                       a.cc3);
   else
     hipLaunchKernelGGL(k_cc3m_fused<false>, dim3((unsigned)grid), dim3(kFT), lds, c.stream,
                        a.nsam, a.n, a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
@@ -836,7 +847,7 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
                        reinterpret_cast<const float2*>(ctff),
                        reinterpret_cast<const float2*>(spec), shift, amag, fsct, t.dft, t.pts,
                        t.ring, t.ring_cnt, t.wmap, a.ri2, a.ri3, a.rih, a.halfw, score, w,
-                       (int*)nullptr);
+                       (int*)nullptr, a.cc3);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -854,7 +865,7 @@ int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
   hipLaunchKernelGGL(k_cc3m_fill, dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads),
                      0, c.stream, nsam, a.n, a.maxr1 * a.maxr1, a.maxr2 * a.maxr2,
                      reinterpret_cast<const float2*>(a3df), dm, pidx,
-                     reinterpret_cast<const float2*>(ctff), work);
+                     reinterpret_cast<const float2*>(ctff), work, a.cc3);
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, nsam, a.n, work, img)) return r;
   // cc3m.f:114: MASKCOS2 with AMAG of each item (scal 1: already scaled)
@@ -864,7 +875,7 @@ int launch_cc3m(Ctx& c, const Cc3mArgs& a, const float* a3df, const float* spec,
   if (int r = fft_r2c_2d(c, nsam, a.n, work, img)) return r;
   hipLaunchKernelGGL(k_cc3m_score, dim3(a.n), dim3(kThreads), 0, c.stream, nsam,
                      a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,
-                     reinterpret_cast<const float2*>(spec), pidx, shift, fsct, work, score);
+                     reinterpret_cast<const float2*>(spec), pidx, shift, fsct, work, score, a.cc3);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -881,7 +892,7 @@ int launch_presb(Ctx& c, int nsam, int n, const float* a3df, const float* spec, 
   size_t tot = ic * n;
   hipLaunchKernelGGL(k_cc3m_fill, dim3((unsigned)((tot + kThreads - 1) / kThreads)), dim3(kThreads),
                      0, c.stream, nsam, n, 1, nsamh * nsamh, reinterpret_cast<const float2*>(a3df),
-                     dm, pidx, reinterpret_cast<const float2*>(ctff), work);
+                     dm, pidx, reinterpret_cast<const float2*>(ctff), work, 0);
   FR_HIP(hipGetLastError());
   if (int r = fft_c2r_2d(c, nsam, n, work, img)) return r;
   hipLaunchKernelGGL(k_scale_mask, dim3(n), dim3(kMaskThreads), 0, c.stream, nsam, work, img, 1.0f,

@@ -81,6 +81,7 @@ int launch_pextract(Ctx& c, int nsam, int n, const float* a3df, const float* dm,
 struct Cc3mArgs {
   int nsam, n, maxr1, maxr2;
   float rbfact, ri2, ri3, rih, halfw;
+  int cc3 = 0;  // 1: CC3 (cc3.f): the section phase-flipped by the CTF sign, no |R| branch
 };
 // Scale by `scal`, then MASKCOS (origin=0) / MASKCOS2 (origin=1); one workgroup per image.
 constexpr int kMaskThreads = 1024;

@@ -117,6 +117,16 @@ int fr_cc3m_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float
                   const float *d_shift, const float *d_amag, int maxr1, int maxr2, float rbfact,
                   const float *d_fsct, const float *ring, float *d_score);
 
+/* Replaces CC3 (cc3.f:8-159; IEWALD=0, IBUF=0), the score of the defocus and
+ * magnification refinements (calcfx.f:63-140): as fr_cc3m_batch with the
+ * section multiplied by the sign of REAL(CTFR+CONJG(CTFL)) instead of the CTF
+ * (cc3.f:86-89) and R_k signed in every shell (no FSCT branch; d_fsct is not
+ * read). */
+int fr_cc3_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
+                 const float *d_ctff, const int *d_pidx, const float *d_dm, const float *d_shift,
+                 const float *d_amag, int maxr1, int maxr2, float rbfact, const float *d_fsct,
+                 const float *ring, float *d_score);
+
 /* Replaces PRESB (presb.f:8-160; IEWALD=0, non-helical): per-shell
  * correlations of n (particle, section, shift) items over 1 <= r^2 < (NSAM/2)^2.
  * Arguments as fr_cc3m_batch.  d_rbin is n x NSAM floats, in/out: for every
@@ -305,6 +315,39 @@ int fr_sigma_batch(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const floa
 int fr_powell_minimize(int n, float *x, const float *e, float escale, int icon, int maxit,
                        float (*fn)(int n, const float *x, void *user), void *user, float *f,
                        int *nevals);
+
+/* Replaces CTFREFINE (ctfrefine.f:8-100) for ng groups at once: VA04A over
+ * the defocus change (x1, x2, x3) = (dDFMID1, dDFMID2, dANGAST) -- three
+ * parameters when fastig (FASTIG), else one with x2 = x1, x3 = 0 -- of
+ * CALCFX's CTF section (calcfx.f:63-110): the mean over the group's particles
+ * of -CC3 (cc3.f) after CTFAPPLY at DF1+x1, DF2+x2, ANGAST+x3 (ICTF 0), minus
+ * FD(SIG2, x1, x2, DFSTD) (fd.f).  Group g holds particles gstart[g] ..
+ * gstart[g+1]-1 (a film, or one particle when defocus is refined per particle).
+ *   d_spec  np spectra, NSAM x JC complex each (the r2c of fr_prep_batch)
+ *   ppar    host np x 8 as fr_prep_batch's d_ppar; euler host np x 3 (rad)
+ *   sig2    host ng: SIG2 of FD (0 for a film, lmain.f:400-411; SIG2N per
+ *           particle, lmain.f:492-500)
+ *   gpar    11 host floats as fr_prep_batch (CTFAPPLY); ring RI2 RI3 RIH
+ *           HALFW of CC3's MASKCOS2; MAXR = INT(NSAM RMAX |AMAG|) per particle
+ *   out     host ng x 4: dDFMID1, dDFMID2, dANGAST and the final objective
+ * EPAR (1, 1, 1), ESCALE 1000, MAXIT 50 (ctfrefine.f:32-35); the caller adds
+ * the changes to the group's particles (ctfrefine.f:64-84). */
+int fr_ctfrefine_batch(fr_ctx *ctx, int nsam, int np, const float *d_spec, const float *ppar,
+                       const float *euler, int ng, const int *gstart, int fastig, float dfstd,
+                       const float *sig2, const float *d_a3df, const float *d_pssnr,
+                       const float *gpar, float rmax1, float rmax2, float rbfact,
+                       const float *ring, float *out, int *nevals);
+
+/* Replaces MAGREFINE (magrefine.f:8-73) for ng groups (films): VA04A over the
+ * relative magnification AMAG from the group's first particle's AMAGP (EPAR
+ * 0.01, ESCALE 100, MAXIT 50) of CALCFX's magnification section
+ * (calcfx.f:112-140): the mean of -CC3 after CTFAPPLY at AMAG, MAXR =
+ * INT(NSAM RMAX |AMAG|).  Arguments as fr_ctfrefine_batch; out host ng x 4:
+ * AMAG, 0, 0, the final objective. */
+int fr_magrefine_batch(fr_ctx *ctx, int nsam, int np, const float *d_spec, const float *ppar,
+                       const float *euler, int ng, const int *gstart, const float *d_a3df,
+                       const float *d_pssnr, const float *gpar, float rmax1, float rmax2,
+                       float rbfact, const float *ring, float *out, int *nevals);
 
 /* The local grid search of a mode-1 refinement, before PREFINE (no routine of
  * the reference scans a grid here: LMAIN calls PREFINE from the particle's

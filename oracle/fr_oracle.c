@@ -363,11 +363,14 @@ void P(pextract)(int nsam, R *outc, const float *a3df, float phi, float theta, f
     }
 }
 
-/* --------------------------------------------------------------- cc3m.f */
-double P(cc3m)(int nsam, float amag, const R *spec, const float *a3df, int maxr1, int maxr2,
-               float phi, float theta, float psi, float shx, float shy, float rbfact,
-               const float *fsct, const R *ctff, float ri2, float ri3, float rih, float halfw,
-               const float *sm) {
+/* ------------------------------------------------------ cc3m.f and cc3.f */
+/* flip = 0: CC3M (cc3m.f:8-169).  flip = 1: CC3 (cc3.f:8-159), whose section
+ * is negated where REAL(CTFR+CONJG(CTFL)) < 0 instead of multiplied by the
+ * CTF (cc3.f:86-89) and whose R_k keeps its sign in every shell. */
+static double P(cc3_core)(int nsam, float amag, const R *spec, const float *a3df, int maxr1,
+                          int maxr2, float phi, float theta, float psi, float shx, float shy,
+                          float rbfact, const float *fsct, const R *ctff, float ri2, float ri3,
+                          float rih, float halfw, const float *sm, int flip) {
   int jc = nsam / 2 + 1;
   size_t ic = (size_t)nsam * jc;
   int maxr12 = maxr1 * maxr1, maxr22 = maxr2 * maxr2;
@@ -389,7 +392,14 @@ double P(cc3m)(int nsam, float amag, const R *spec, const float *a3df, int maxr1
         float y3 = dm[1] * (float)i + dm[4] * (float)j;
         float z3 = dm[2] * (float)i + dm[5] * (float)j;
         C a = ainterpo3ds_c(nsam, a3df, x3, y3, z3);
-        a = cmul(a, cadd(cr, cconj(cl)));
+        if (flip) {
+          if (cadd(cr, cconj(cl)).re < 0) {
+            a.re = -a.re;
+            a.im = -a.im;
+          }
+        } else {
+          a = cmul(a, cadd(cr, cconj(cl)));
+        }
         buf[2 * id] = a.re * scal;
         buf[2 * id + 1] = a.im * scal;
       }
@@ -423,7 +433,7 @@ double P(cc3m)(int nsam, float amag, const R *spec, const float *a3df, int maxr1
     int ibin = 4 * i;
     if (rbuf[ibin - 1] != 0) {
       R r = rbuf[ibin - 4] / RSQRT(rbuf[ibin - 2]);
-      if (fsct[i - 1] < 0.95f) r = RABS(r);
+      if (!flip && fsct[i - 1] < 0.95f) r = RABS(r);
       float e = -rbfact * (float)((i - 1) * (i - 1));
       R wgt = REXP((R)e);
       sum3 = sum3 + (double)(r * wgt);
@@ -438,6 +448,21 @@ double P(cc3m)(int nsam, float amag, const R *spec, const float *a3df, int maxr1
 #else
   return (double)(float)sum3;
 #endif
+}
+
+double P(cc3m)(int nsam, float amag, const R *spec, const float *a3df, int maxr1, int maxr2,
+               float phi, float theta, float psi, float shx, float shy, float rbfact,
+               const float *fsct, const R *ctff, float ri2, float ri3, float rih, float halfw,
+               const float *sm) {
+  return P(cc3_core)(nsam, amag, spec, a3df, maxr1, maxr2, phi, theta, psi, shx, shy, rbfact,
+                     fsct, ctff, ri2, ri3, rih, halfw, sm, 0);
+}
+
+double P(cc3)(int nsam, float amag, const R *spec, const float *a3df, int maxr1, int maxr2,
+              float phi, float theta, float psi, float shx, float shy, float rbfact,
+              const R *ctff, float ri2, float ri3, float rih, float halfw, const float *sm) {
+  return P(cc3_core)(nsam, amag, spec, a3df, maxr1, maxr2, phi, theta, psi, shx, shy, rbfact,
+                     NULL, ctff, ri2, ri3, rih, halfw, sm, 1);
 }
 
 /* --------------------------------------------------------------- presb.f */

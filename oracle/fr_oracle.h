@@ -113,6 +113,12 @@ void or_irfft2_d(int nsam, double *data);
                  float shx, float shy, float rbfact, const float *fsct,         \
                  const R *ctff, float ri2, float ri3, float rih, float halfw,   \
                  const float *sm);                                              \
+  /* cc3.f:8-159 (IEWALD=0, IBUF=0, IPAD=1, IRAD!=0): CC3M with the section    \
+   * phase-flipped by the CTF sign and signed R_k in every shell */             \
+  double P##cc3(int nsam, float amag, const R *spec, const float *a3df,         \
+                int maxr1, int maxr2, float phi, float theta, float psi,        \
+                float shx, float shy, float rbfact, const R *ctff, float ri2,   \
+                float ri3, float rih, float halfw, const float *sm);            \
   /* presb.f:8-160 (non-helical).  rbin[2*NSAM] updated like the reference. */  \
   void P##presb(int nsam, float amag, const R *spec, const float *a3df,         \
                 float phi, float theta, float psi, float shx, float shy,        \
@@ -193,6 +199,20 @@ float or32_prefine(int nsam, float amag, const float *spec, const float *a3df, i
                    int maxr2, float *par, const int mask[5], float rbfact, const float *fsct,
                    const float *ctff, float ri2, float ri3, float rih, float halfw,
                    const float prior[5], const float *sm, int *nevals);
+
+/* ctfrefine.f:8-100 with calcfx.f:63-110 (IVFLAG 1) for one group of n
+ * particles: spec n half-plane spectra (r2c, interleaved complex), ppar n x 8
+ * (SHX SHY DF1 DF2 ANGAST AMAGP . .), euler n x 3, gpar the 11 CTFAPPLY
+ * globals (CS WL WGH1 WGH2 THETATR RIH HALFW RI2 RI3 TX TY), ring CC3's RI2
+ * RI3 RIH HALFW.  xout: dDF1 dDF2 dANGAST; returns the final objective. */
+float or32_ctfrefine(int nsam, int n, const float *spec, const float *ppar, const float *euler,
+                     int fastig, float dfstd, float sig2, const float *a3df, const float *pssnr,
+                     const float *gpar, float rmax1, float rmax2, float rbfact, const float *ring,
+                     float *xout, int *nevals);
+/* magrefine.f:8-73 with calcfx.f:112-140 (IVFLAG 2); xout[0] = AMAG. */
+float or32_magrefine(int nsam, int n, const float *spec, const float *ppar, const float *euler,
+                     const float *a3df, const float *pssnr, const float *gpar, float rmax1,
+                     float rmax2, float rbfact, const float *ring, float *xout, int *nevals);
 
 /* Closed-form test objectives shared with oracle/ref_va04_obj.f90 (same FP32
  * operation order): 1 coupled quadratic, 2 Rosenbrock chain, 3 rational bump. */

@@ -9,6 +9,8 @@
  * reference on the same objective.  Pinned in tests/test_oracle_ref.py
  * against oracle/_ref (va04.f + the test objective of ref_va04_obj.f90). */
 #include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
 
 #include "fr_oracle.h"
 
@@ -488,4 +490,109 @@ float or32_prefine(int nsam, float amag, const float *spec, const float *a3df, i
     if (mask[j] == 1) par[j] = xmap[ip++];
   if (nevals) *nevals = ne;
   return -f;
+}
+
+/* ---------------------------------------------- ctfrefine.f, magrefine.f */
+/* CTFREFINE (ctfrefine.f:8-100, one group of n particles) and MAGREFINE
+ * (magrefine.f:8-73): VA04A over CALCFX's CTF section (IVFLAG 1,
+ * calcfx.f:63-110) or magnification section (IVFLAG 2, calcfx.f:112-140), each
+ * evaluation re-running CTFAPPLY (ICTF 0) on every particle of the group and
+ * scoring it with CC3 at zero shift, MAXR = INT(NSAM RMAX |AMAG|). */
+struct ctf_ctx {
+  int mode, nsam, n, npar;
+  const float *spec, *ppar, *euler, *a3df, *pssnr, *gpar, *ring;
+  float dfstd, sig2, rmax1, rmax2, rbfact;
+  float *ctff, *outc, *datc;
+};
+
+static float fd(float sig2, float d1, float d2, float dfstd) {
+  return sig2 * (-(d1 * d1) / 2.0f / (dfstd * dfstd) - (d2 * d2) / 2.0f / (dfstd * dfstd));
+}
+
+static float calcfx_ctf(int nx, const float *x, void *user) {
+  struct ctf_ctx *c = (struct ctf_ctx *)user;
+  const int nsam = c->nsam, jc = nsam / 2 + 1;
+  const size_t ic = (size_t)nsam * jc;
+  const float *g = c->gpar;
+  float rft = 0.0f;
+  (void)nx;
+  for (int k = 0; k < c->n; k++) {
+    const float *p = c->ppar + 8 * (size_t)k;
+    const float *e = c->euler + 3 * (size_t)k;
+    float df1 = p[2], df2 = p[3], ast = p[4], amag = p[5];
+    float x1 = 0, x2 = 0;
+    if (c->mode == 1) {
+      x1 = x[0];
+      x2 = c->npar == 1 ? x[0] : x[1];
+      df1 = p[2] + x1;
+      df2 = p[3] + x2;
+      ast = p[4] + (c->npar == 1 ? 0.0f : x[2]);
+    } else {
+      amag = x[0];
+    }
+    const int maxr1 = (int)((float)nsam * c->rmax1 * fabsf(amag));
+    const int maxr2 = (int)((float)nsam * c->rmax2 * fabsf(amag));
+    or32_ctfapply(nsam, c->spec + 2 * ic * (size_t)k, p[0], p[1], g[0], g[1], g[2], g[3], df1, df2,
+                  ast, g[4], amag, g[5], g[6], g[7], g[8], c->pssnr, 0, c->ctff, c->outc,
+                  c->datc);
+    const float cc = (float)or32_cc3(nsam, amag, c->outc, c->a3df, maxr1, maxr2, e[0], e[1], e[2],
+                                     0.0f, 0.0f, c->rbfact, c->ctff, c->ring[0], c->ring[1],
+                                     c->ring[2], c->ring[3], NULL);
+    if (c->mode == 1)
+      rft = rft - cc - fd(c->sig2, x1, x2, c->dfstd);
+    else
+      rft = rft + (-cc);
+  }
+  return rft / (float)c->n;
+}
+
+static float refine_ctf(int mode, int nsam, int n, const float *spec, const float *ppar,
+                        const float *euler, int fastig, float dfstd, float sig2,
+                        const float *a3df, const float *pssnr, const float *gpar, float rmax1,
+                        float rmax2, float rbfact, const float *ring, float *xout, int *nevals) {
+  const int jc = nsam / 2 + 1;
+  const size_t ic = (size_t)nsam * jc;
+  struct ctf_ctx c = {mode, nsam, n, mode == 1 && fastig ? 3 : 1, spec, ppar, euler, a3df, pssnr,
+                      gpar, ring, dfstd, sig2, rmax1, rmax2, rbfact, NULL, NULL, NULL};
+  c.ctff = (float *)calloc(4 * ic, sizeof(float));
+  c.outc = (float *)calloc(2 * ic, sizeof(float));
+  c.datc = (float *)calloc(2 * ic, sizeof(float));
+  float x[3] = {0.0f, 0.0f, 0.0f};
+  float e[3] = {1.0f, 1.0f, 1.0f};
+  float escale = 1000.0f;
+  if (mode == 2) {
+    x[0] = ppar[5];
+    e[0] = 0.01f;
+    escale = 100.0f;
+  }
+  int ne = 0;
+  float f = or_va04a(x, e, c.npar, escale, 0, 1, 50, calcfx_ctf, &c, &ne);
+  free(c.ctff);
+  free(c.outc);
+  free(c.datc);
+  if (mode == 1) {
+    xout[0] = x[0];
+    xout[1] = c.npar == 1 ? x[0] : x[1];
+    xout[2] = c.npar == 1 ? 0.0f : x[2];
+  } else {
+    xout[0] = x[0];
+    xout[1] = xout[2] = 0.0f;
+  }
+  if (nevals) *nevals = ne;
+  return f;
+}
+
+float or32_ctfrefine(int nsam, int n, const float *spec, const float *ppar, const float *euler,
+                     int fastig, float dfstd, float sig2, const float *a3df, const float *pssnr,
+                     const float *gpar, float rmax1, float rmax2, float rbfact, const float *ring,
+                     float *xout, int *nevals) {
+  return refine_ctf(1, nsam, n, spec, ppar, euler, fastig, dfstd, sig2, a3df, pssnr, gpar, rmax1,
+                    rmax2, rbfact, ring, xout, nevals);
+}
+
+float or32_magrefine(int nsam, int n, const float *spec, const float *ppar, const float *euler,
+                     const float *a3df, const float *pssnr, const float *gpar, float rmax1,
+                     float rmax2, float rbfact, const float *ring, float *xout, int *nevals) {
+  return refine_ctf(2, nsam, n, spec, ppar, euler, 0, 1.0f, 0.0f, a3df, pssnr, gpar, rmax1, rmax2,
+                    rbfact, ring, xout, nevals);
 }

@@ -74,6 +74,12 @@ def _declare_oracle(L):
     L.or32_prefine.argtypes = [ci, cf, c64p, c64p, ci, ci, f32p, i32p, cf, f32p, c64p, cf, cf, cf,
                                cf, f32p, C.c_void_p, pi_]
     L.or32_prefine.restype = cf
+    L.or32_ctfrefine.argtypes = [ci, ci, c64p, f32p, f32p, ci, cf, cf, c64p, f32p, f32p, cf, cf,
+                                 cf, f32p, f32p, pi_]
+    L.or32_ctfrefine.restype = cf
+    L.or32_magrefine.argtypes = [ci, ci, c64p, f32p, f32p, c64p, f32p, f32p, cf, cf, cf, f32p,
+                                 f32p, pi_]
+    L.or32_magrefine.restype = cf
     L.or_combine_counters.argtypes = [ci, ci, ci, C.POINTER(C.c_int64), f64p, f64p,
                                       C.POINTER(C.c_int64), pd, pd]
     L.or_rfft2_f.argtypes = [ci, f32p]
@@ -92,6 +98,9 @@ def _declare_oracle(L):
         g("cc3m").argtypes = [ci, cf, cp, c64p, ci, ci, cf, cf, cf, cf, cf, cf, f32p, cp,
                               cf, cf, cf, cf, f32p]
         g("cc3m").restype = cd
+        g("cc3").argtypes = [ci, cf, cp, c64p, ci, ci, cf, cf, cf, cf, cf, cf, cp, cf, cf, cf, cf,
+                             f32p]
+        g("cc3").restype = cd
         g("sigma").argtypes = [ci, cf, c64p, cp, cf, cf, cf, cf, cf, cp, ci, cf, cf, f32p, cf,
                                f32p, rp]
         g("presb").argtypes = [ci, cf, cp, c64p, cf, cf, cf, cf, cf, rp, cp, cf, cf, cf, cf, f32p]

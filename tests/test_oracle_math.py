@@ -577,3 +577,168 @@ def test_prefine_shift_prior_sign_and_presa():
     fsh = f(1.0) * (-(pull[3] * pull[3]) / f(2) / (f(0.05) * f(0.05))
                     - (pull[4] * pull[4]) / f(2) / (f(0.05) * f(0.05)))
     assert abs(presa - (_cc3m32(case, pull) + fsh)) < 1e-6
+
+
+def _flipped_projection(nsam, vol, eul, ctff, maxr1, maxr2, ring):
+    """CC3's AINPO (cc3.f:74-101) from pinned pieces: the section (PEXTRACT with
+    a unit CTF, CTFR = CTFL = 1/2) negated where REAL(CTFR+CONJG(CTFL)) < 0,
+    on the annulus, scaled, c2r, MASKCOS2, r2c -- FP64."""
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    unit = np.full(2 * ic, 0.5, np.complex128)
+    sec = np.zeros(ic, np.complex128)
+    L.or64_pextract(nsam, sec, vol, *eul, unit, O.IDENTITY)
+    sign = np.where((ctff[:ic] + np.conj(ctff[ic:])).real < 0, -1.0, 1.0)
+    i = np.arange(jc)[None, :]
+    m = np.arange(nsam)[:, None]
+    j = np.where(m >= jc, m - nsam, m)
+    it = (i * i + j * j).reshape(-1)
+    keep = (it >= maxr1 * maxr1) & (it < maxr2 * maxr2)
+    buf = np.where(keep, sec * sign, 0) / nsam / nsam
+    d = np.zeros(nsam * (nsam + 2))
+    d.view(np.complex128)[:] = buf
+    L.or_irfft2_d(nsam, d)
+    L.or64_maskcos2(nsam, d, *ring, 1.0)
+    L.or_rfft2_d(nsam, d)
+    return d.view(np.complex128).copy(), it, keep
+
+
+def test_cc3_self_correlation_sign_only_and_no_abs():
+    """CC3 (cc3.f): (1) SPEC = its own phase-flipped masked section gives the
+    closed form sum a_k / sqrt(N sum a_k^2); (2) the score depends on the CTF
+    only through its sign (scaling the CTF by positive factors changes
+    nothing; CC3M's value does change); (3) R_k keeps its sign in every shell:
+    negating the particle negates CC3 exactly, while CC3M with FSCT < 0.95
+    everywhere takes |R_k| and does not change."""
+    nsam = 32
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    vol = _volume(nsam, 12)
+    r = np.random.default_rng(4)
+    ctff = (0.2 + r.standard_normal(2 * ic) + 0.3j * r.standard_normal(2 * ic)).astype(np.complex128)
+    eul = (0.7, 0.4, -1.3)
+    maxr1, maxr2 = 1, 13
+    ri = 0.4 * nsam
+    ring = ((ri - 3) ** 2, (ri + 3) ** 2, ri - 3, 6.0)
+    ainpo, it, keep = _flipped_projection(nsam, vol, eul, ctff, maxr1, maxr2, ring)
+    got = L.or64_cc3(nsam, 1.0, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, ctff, *ring,
+                     O.IDENTITY)
+    kbin = (np.sqrt(it.astype(np.float32)) + 0.5).astype(int)
+    a = np.array([np.sqrt(np.sum(np.abs(ainpo[keep & (kbin == k)]) ** 2))
+                  for k in range(1, nsam // 2) if (keep & (kbin == k)).any()])
+    want = a.sum() / math.sqrt(len(a) * (a ** 2).sum())
+    assert abs(got - want) < 1e-9, (got, want)
+    # only the CTF's sign enters
+    scale = r.uniform(0.5, 3.0, 2 * ic)
+    sc = ctff.copy()
+    sc[:ic] = ctff[:ic] * scale[:ic]
+    sc[ic:] = ctff[ic:] * scale[:ic]  # CTFR + conj(CTFL) scaled by the same positive factor
+    got_s = L.or64_cc3(nsam, 1.0, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, sc, *ring,
+                       O.IDENTITY)
+    assert abs(got_s - got) < 1e-12
+    fs = np.full(jc, 0.5, np.float32)
+    cm = L.or64_cc3m(nsam, 1.0, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, fs, ctff, *ring,
+                     O.IDENTITY)
+    cm_s = L.or64_cc3m(nsam, 1.0, ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, fs, sc, *ring,
+                       O.IDENTITY)
+    assert abs(cm - cm_s) > 1e-4
+    # signed R_k: CC3(-SPEC) = -CC3(SPEC); CC3M with |R_k| unchanged
+    neg = L.or64_cc3(nsam, 1.0, -ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, ctff, *ring,
+                     O.IDENTITY)
+    assert abs(neg + got) < 1e-12
+    cm_neg = L.or64_cc3m(nsam, 1.0, -ainpo, vol, maxr1, maxr2, *eul, 0.0, 0.0, 0.0, fs, ctff,
+                         *ring, O.IDENTITY)
+    assert abs(cm_neg - cm) < 1e-12
+    # the FP32 restatement
+    g32 = L.or32_cc3(nsam, 1.0, ainpo.astype(np.complex64), vol, maxr1, maxr2, *eul, 0.0, 0.0,
+                     0.0, ctff.astype(np.complex64), *ring, O.IDENTITY)
+    assert abs(g32 - got) < 1e-5
+
+
+def _ctf_particles(nsam, vol, eul, df1, df2, ang, gpar):
+    """Noise-free particle spectra (the r2c of the image CTFAPPLY takes): the
+    section (PEXTRACT, unit CTF) times CTFR + conj(CTFL) at the true defocus,
+    times (-1)^(l+m) (CTFAPPLY's centring is then undone exactly)."""
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    l = np.tile(np.arange(jc), nsam)
+    m = np.repeat(np.arange(nsam), jc)
+    mm = np.where(m >= jc, m - nsam, m)
+    sgn = (-1.0) ** (l + mm)
+    unit = np.full(2 * ic, 0.5, np.complex128)
+    out = []
+    for e, d1, d2, a in zip(eul, df1, df2, ang):
+        sec = np.zeros(ic, np.complex128)
+        L.or64_pextract(nsam, sec, vol, *e, unit, O.IDENTITY)
+        ctff = np.zeros(2 * ic, np.complex128)
+        o1, o2 = np.zeros(ic, np.complex128), np.zeros(ic, np.complex128)
+        L.or64_ctfapply(nsam, np.zeros(ic, np.complex64), 0.0, 0.0, *[float(v) for v in gpar[:4]],
+                        float(d1), float(d2), float(a), float(gpar[4]), 1.0,
+                        *[float(v) for v in gpar[5:9]], np.ones(jc + 1, np.float32), 0, ctff, o1,
+                        o2)
+        out.append(sec * (ctff[:ic] + np.conj(ctff[ic:])) * sgn)
+    return np.ascontiguousarray(np.array(out).astype(np.complex64))
+
+
+def _ctfrefine_case():
+    from frealign_amd import synth
+    nsam, psize = 32, 2.0
+    optics = synth.Optics(psize=psize)
+    gpar = optics.gpar(nsam, 12.0)
+    vol = _volume(nsam, 31)
+    eul = [(0.3, 1.0, -0.5), (-1.1, 0.6, 2.0), (2.2, 1.7, 0.4)]
+    df1 = [15000.0, 15000.0, 15000.0]
+    df2 = [14400.0, 14400.0, 14400.0]
+    ang = [0.5, 0.5, 0.5]
+    spec = _ctf_particles(nsam, vol, eul, df1, df2, ang, gpar)
+    jc = nsam // 2 + 1
+    ring = np.array([gpar[7], gpar[8], gpar[5], gpar[6]], np.float32)
+    pssnr = np.ones(jc + 1, np.float32)
+    return nsam, vol, np.array(eul, np.float32), spec, gpar, ring, pssnr, (df1, df2, ang)
+
+
+def test_ctfrefine_recovers_planted_defocus_and_prior_sign():
+    """CTFREFINE (ctfrefine.f over CALCFX's CTF section, CC3, FD; VA04A pinned
+    to va04.f) on three noise-free particles of one film made at a known
+    defocus: started 300 A off in both DF1 and DF2 (coupled, FASTIG = F) it
+    returns -300 A, and started +200/-150 A off with FASTIG = T it returns
+    both; with a strong FD prior (SIG2 > 0) the change is pulled towards zero."""
+    nsam, vol, eul, spec, gpar, ring, pssnr, (df1, df2, ang) = _ctfrefine_case()
+    rmax1, rmax2 = 1.0 / nsam, 13.0 / nsam + 1e-4
+
+    def run(off1, off2, fastig, sig2=0.0, dfstd=100.0):
+        ppar = np.zeros((3, 8), np.float32)
+        ppar[:, 2] = np.array(df1) + off1
+        ppar[:, 3] = np.array(df2) + off2
+        ppar[:, 4] = ang
+        ppar[:, 5] = 1.0
+        x = np.zeros(3, np.float32)
+        ne = C.c_int(0)
+        f = L.or32_ctfrefine(nsam, 3, spec, ppar, eul, fastig, dfstd, sig2, vol, pssnr, gpar,
+                             rmax1, rmax2, 0.0, ring, x, C.byref(ne))
+        return x, f, ne.value
+
+    x, f, ne = run(300.0, 300.0, 0)
+    assert abs(x[0] + 300.0) < 25.0 and x[1] == x[0] and x[2] == 0.0, x
+    assert ne > 3
+    x, f, ne = run(200.0, -150.0, 1)
+    assert abs(x[0] + 200.0) < 40.0 and abs(x[1] - 150.0) < 40.0, x
+    free, _, _ = run(300.0, 300.0, 0, sig2=0.0, dfstd=100.0)
+    pulled, _, _ = run(300.0, 300.0, 0, sig2=1.0, dfstd=100.0)
+    assert abs(pulled[0]) < abs(free[0]) - 20.0, (free, pulled)
+
+
+def test_magrefine_recovers_planted_magnification():
+    """MAGREFINE (magrefine.f over CALCFX's magnification section): particles
+    made at relative magnification 1 and started at 1.02 come back to 1 within
+    0.004 (the EPAR step is 0.01)."""
+    nsam, vol, eul, spec, gpar, ring, pssnr, (df1, df2, ang) = _ctfrefine_case()
+    rmax1, rmax2 = 1.0 / nsam, 13.0 / nsam + 1e-4
+    ppar = np.zeros((3, 8), np.float32)
+    ppar[:, 2], ppar[:, 3], ppar[:, 4] = df1, df2, ang
+    ppar[:, 5] = 1.02
+    x = np.zeros(3, np.float32)
+    ne = C.c_int(0)
+    L.or32_magrefine(nsam, 3, spec, ppar, eul, vol, pssnr, gpar, rmax1, rmax2, 0.0, ring, x,
+                     C.byref(ne))
+    assert abs(x[0] - 1.0) < 0.004, x
