@@ -36,6 +36,7 @@ SOURCES = [
     "fr_psearch.cpp",
     "fr_local.hip",
     "fr_ctfrefine.cpp",
+    "fr_io.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

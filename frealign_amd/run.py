@@ -1,0 +1,422 @@
+"""A Frealign run from its card deck on one GPU: `python -m frealign_amd.run
+< deck` reads cards 1-18 (cards.py), the particle stack and parameter file of
+the dataset, and the reference map, and runs the per-particle hot path of
+LMAIN (lmain.f:509-885) in batches on the device:
+
+  IFLAG 0  reconstruction at the parameters as read;
+  IFLAG 1  refinement from them: CTFREFINE per film or particle (FDEF, FPART,
+           FASTIG) and MAGREFINE per film (FMAG) first, then PREFINE
+           (lmain.f:743-828);
+  IFLAG 3  global search: PSEARCH's view scan as the FP32 GEMM search, the
+           IPMAX best views and the input orientation each refined by
+           PREFINE, the best kept (lmain.f:551-741);
+
+then A3D3's insertion into the two half-maps (odd and even particles), and
+the end of the run of frealign_v9.f:790-950: STORESHIFT, the half-maps'
+c2r, CORRECT3D, MASK3D (written to cards 15 and 16), their r2c and
+SHELTEST's FSC, SHIFTVOL (with the Wiener filter of FIND_FPART's MW closed
+form when FFILT), the final map's c2r, CORRECT3D and MASK3D (card 13), the
+output parameter file (card 11) with the FSC table of OPRESSTATHALF appended
+and the parameter-shift file (card 12).
+
+Supported: MRC files (CFORM M), one dataset, IFSC 0, INTERP 1, IEWALD 0,
+XSTD >= 0, no FBEAUT/FBFACT/FMATCH/FDUMP, point groups C/D/T/O/I; anything
+else stops with a message, as the reference's STOPs do.
+"""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+
+from . import api, cards, fileio, search, synth
+
+F = np.float32
+PI_F = F(3.1415926535897)
+B = 256
+
+
+def _check(deck: cards.Deck):
+    bad = []
+    if deck.cform != "M":
+        bad.append("CFORM must be M (MRC)")
+    if deck.iflag not in (0, 1, 3):
+        bad.append(f"IFLAG {deck.iflag} (0, 1 and 3 are supported)")
+    if deck.ifsc != 0:
+        bad.append("IFSC must be 0")
+    if deck.interp != 1:
+        bad.append("INTERP must be 1 (trilinear)")
+    if deck.iewald != 0:
+        bad.append("IEWALD must be 0")
+    if deck.xstd < 0:
+        bad.append("XSTD < 0 (MASKENV) is not supported")
+    if deck.fbeaut or deck.fbfact or deck.fmatch or deck.fdump:
+        bad.append("FBEAUT, FBFACT, FMATCH and FDUMP must be F")
+    if len(deck.datasets) != 1:
+        bad.append("exactly one dataset (cards 6-12) is supported")
+    if bad:
+        raise cards.CardError("; ".join(bad))
+
+
+class Run:
+    def __init__(self, deck: cards.Deck, ctx: api.Context | None = None, log=None):
+        import torch
+
+        _check(deck)
+        self.torch = torch
+        self.deck = deck
+        self.log = log or (lambda m: print(m, flush=True))
+        self.ctx = ctx or api.Context(0)
+        ds = deck.datasets[0]
+        self.ds = ds
+        info = fileio.mrc_info(ds.finpat1)
+        if info.nx != info.ny or info.nx % 2:
+            raise cards.CardError("particle images must be square with even NSAM")
+        self.nsam = nsam = info.nx
+        self.psize = psize = deck.psize
+        self.ri = deck.ro / psize
+        # cards8and9.f:46-66: resolutions as fractions of the sampling
+        r1, r2 = (ds.rmax1, ds.rmax2) if ds.rmax1 >= ds.rmax2 else (ds.rmax2, ds.rmax1)
+        self.rrec = min(psize / ds.rrec, 0.5)
+        self.rlim = self.rrec
+        self.rmax1 = psize / r1
+        self.rmax2 = min(psize / r2, self.rlim)
+        self.rclas = min(psize / ds.rclas, self.rlim)
+        # card10.f:81-90: default PSSNR and FSCT
+        jc = nsam // 2 + 1
+        self.fsct = np.array([1.0 if (k == 0 or psize * nsam / (k + 1) > 30.0) else 0.0
+                              for k in range(jc)], np.float32)
+        pssnr = np.zeros(jc + 1, np.float32)
+        for i in range(2, nsam // 2 + 1):
+            pssnr[i - 1] = math.exp(-((i - 1) / nsam * self.ri / nsam) ** 2)
+        # optics (card6.f:51-52, cards11and12.f:219, frealign_v9.f:182-183,526-531, lmain.f:275)
+        self.wgh = deck.wgh
+        wl = ds.wl
+        halfw = 6.0
+        rih, rii = self.ri - halfw / 2.0, self.ri + halfw / 2.0
+        self.gpar = np.array([ds.cs * 1.0e7, wl, math.sqrt(1.0 - deck.wgh ** 2), deck.wgh,
+                              wl / (psize * nsam), rih, halfw, rih * rih, rii * rii, ds.tx, ds.ty],
+                             np.float32)
+        self.ring = np.array([rih * rih, rii * rii, rih, halfw], np.float32)
+        # parameters
+        par = fileio.par_read(ds.finpar)
+        if par.part_ssnr is not None:  # card10.f:104-126: the previous run's table
+            for i, v in enumerate(par.part_ssnr[: nsam // 2 - 1]):
+                pssnr[i + 1] = F(v) * F(v)
+            pssnr[0] = pssnr[1]
+        self.pssnr = pssnr
+        keep = [r for r in par.records if deck.ifirst <= r.c <= deck.ilast]
+        if not keep:
+            raise cards.CardError("no particles between IFIRST and ILAST")
+        self.records = keep
+        self.internal = fileio.ParFile(keep).to_internal(nsam, ds.relmag, ds.dstep)
+        self.amagp = np.array([F(ds.relmag * ds.dstep * 10000.0) / (F(abs(r.mag)) * F(psize))
+                               for r in keep], np.float32)
+        # card10.f:319-330,541-566: the dataset's shift statistics (phase units)
+        occ = self.internal[:, 11]
+        shx, shy = self.internal[:, 4], self.internal[:, 5]
+        qm = occ.sum()
+        xm = (occ * shx).sum() / qm if qm else 0.0
+        ym = (occ * shy).sum() / qm if qm else 0.0
+        sx = math.sqrt(abs((occ * shx * shx).sum() / qm - xm * xm)) if qm else 0.0
+        sy = math.sqrt(abs((occ * shy * shy).sum() / qm - ym * ym)) if qm else 0.0
+        if sx < 1e-6 or sy < 1e-6:
+            sx = sy = 99999.0
+        self.shift_stats = (xm, ym, sx, sy)
+        self.sym = api.point_group(*deck.point_group())
+        self.a3df = None
+        if deck.iflag != 0:
+            vol = fileio.mrc_read(deck.f3d)
+            if vol.shape != (nsam, nsam, nsam):
+                raise cards.CardError(f"reference map {vol.shape} does not match NSAM {nsam}")
+            self.a3df = synth.half_volume(torch.from_numpy(vol).to("cuda"))
+        self.log(f" frealign_amd: NSAM {nsam}, {len(keep)} particles, IFLAG {deck.iflag}, "
+                 f"ASYM {deck.asym}")
+
+    # ---------------------------------------------------------------- batches
+    def _images(self, idx):
+        torch = self.torch
+        nsam = self.nsam
+        out = np.zeros((len(idx), nsam, nsam + 2), np.float32)
+        for k, c in enumerate(idx):
+            out[k, :, :nsam] = fileio.mrc_read(self.ds.finpat1, c - 1, 1)[0]
+        return torch.from_numpy(out).to("cuda")
+
+    def _ppar(self, rows):
+        p = np.zeros((len(rows), 8), np.float32)
+        p[:, 0] = self.internal[rows, 4]
+        p[:, 1] = self.internal[rows, 5]
+        p[:, 2] = self.internal[rows, 8]
+        p[:, 3] = self.internal[rows, 9]
+        p[:, 4] = self.internal[rows, 10]
+        p[:, 5] = self.amagp[rows]
+        return p
+
+    def _maxr(self, amag):
+        return (int(F(self.nsam) * F(self.rmax1) * F(abs(amag))),
+                int(F(self.nsam) * F(self.rmax2) * F(abs(amag))))
+
+    def _sig2(self, rows):
+        """SIG2N of FSH and A3D3 (ON = OCC/SIG2N, a3d3_s.f:44): the square of the
+        par file's SIGMA column, 1 where it is zero."""
+        s = np.asarray([self.records[r].sigma for r in rows], np.float32)
+        return np.where(s > 0, s * s, F(1.0)).astype(np.float32)
+
+    def refine_batch(self, rows, gsearch=None):
+        """Refinement of one batch (lmain.f:509-885): returns (par (n, 6) PHI
+        THETA PSI DSHX DSHY PRESA with DSHX, DSHY the change of the shift, the
+        defocus (n, 3), AMAGP (n,), (CTFF, DATC))."""
+        torch, ctx, deck, nsam = self.torch, self.ctx, self.deck, self.nsam
+        n = len(rows)
+        idx = [self.records[r].c for r in rows]
+        pp = self._ppar(rows)
+        am = self.amagp[rows].copy()
+        rbf = self.ds.rbfact / (4.0 * (nsam * self.psize * float(np.mean(am))) ** 2)
+        pss = torch.from_numpy(self.pssnr).cuda()
+        imgs = self._images(idx)
+        ctff, outc, datc = ctx.ctfapply(nsam, imgs, torch.from_numpy(pp).cuda(), pss, self.gpar)
+        euler = np.stack([self.internal[rows, 3], self.internal[rows, 2],
+                          self.internal[rows, 1]], 1).astype(np.float32)  # PHI THETA PSI
+        films = [self.records[r].film for r in rows]
+        if deck.fmag or deck.fdef:
+            if deck.fmag:  # lmain.f:346-381, per film
+                groups = _runs(films)
+                out, _ = ctx.ctfrefine(nsam, imgs, pp, euler, groups, self.a3df, pss, self.gpar,
+                                       rmax1=self.rmax1, rmax2=self.rmax2, rbfact=rbf,
+                                       ring=self.ring, magnification=True)
+                am = np.repeat(out[:, 0], groups).astype(np.float32)
+                pp[:, 5] = am
+            if deck.fdef:  # lmain.f:383-410 per film, 470-505 per particle (FPART)
+                groups = [1] * n if deck.fpart else _runs(films)
+                sig2 = self._sig2(rows) if deck.fpart else np.zeros(len(groups), np.float32)
+                out, _ = ctx.ctfrefine(nsam, imgs, pp, euler, groups, self.a3df, pss, self.gpar,
+                                       rmax1=self.rmax1, rmax2=self.rmax2, rbfact=rbf,
+                                       ring=self.ring, fastig=deck.fastig, dfstd=self.ds.dfstd,
+                                       sig2=sig2)
+                for j in range(3):
+                    pp[:, 2 + j] += np.repeat(out[:, j], groups)
+            ctff, outc, datc = ctx.ctfapply(nsam, self._images(idx), torch.from_numpy(pp).cuda(),
+                                            pss, self.gpar)
+        maxr1, maxr2 = self._maxr(float(np.mean(am)))
+        fs = torch.from_numpy(self.fsct).cuda()
+        xm, ym, sx, sy = self.shift_stats
+        mask = np.asarray(deck.pmask_internal, np.int32)
+        start = np.zeros((n, 6), np.float32)
+        start[:, 0:3] = euler  # DSHX = DSHY = 0: CTFAPPLY has applied SHX, SHY
+        prior = np.zeros((n, 5), np.float32)  # FSH(SIG2N, DSHX, DSHY, XM-SHX, YM-SHY, SX, SY)
+        prior[:, 0] = F(xm) - pp[:, 0]
+        prior[:, 1] = F(ym) - pp[:, 1]
+        prior[:, 2], prior[:, 3] = sx, sy
+        prior[:, 4] = self._sig2(rows)
+        if deck.iflag == 1:
+            par, _ = ctx.refine(nsam, self.a3df, outc, ctff, np.arange(n), am, start, mask, maxr1,
+                                maxr2, rbf, fs, self.ring, prior)
+        else:
+            k = deck.ipmax
+            top_o, top_r, _ = gsearch.candidates(outc, ctff, topk=k)
+            o = top_o.cpu().numpy()
+            r = top_r.cpu().numpy()
+            shifts = gsearch.row_to_shift(np.where(o >= 0, r, 7 * 15 + 7))
+            items = np.zeros((n * (k + 1), 6), np.float32)
+            for q in range(k):
+                items[q::k + 1, 0:3] = gsearch.euler[np.maximum(o[:, q], 0)]
+                items[q::k + 1, 3:5] = shifts[:, q]
+            items[k::k + 1] = start  # lmain.f:559-565: the input orientation as well
+            pidx = np.repeat(np.arange(n), k + 1)
+            ref, _ = ctx.refine(nsam, self.a3df, outc, ctff, pidx, np.repeat(am, k + 1), items,
+                                mask, maxr1, maxr2, rbf, fs, self.ring, np.repeat(prior, k + 1, 0))
+            ref = ref.reshape(n, k + 1, 6)
+            valid = np.concatenate([o >= 0, np.ones((n, 1), bool)], 1)
+            score = np.where(valid, ref[:, :, 5], -np.inf)
+            best = np.argmax(score, 1)  # first maximum, lmain.f:634
+            par = ref[np.arange(n), best]
+        return par, pp[:, 2:5].copy(), am, (ctff, datc)
+
+    # ------------------------------------------------------------------ run
+    def run(self):
+        torch, ctx, deck, nsam = self.torch, self.ctx, self.deck, self.nsam
+        ds = self.ds
+        n = len(self.records)
+        gsearch = None
+        if deck.iflag == 3:
+            letter, nasym = deck.point_group()
+            dang = deck.dang if deck.dang != 0.0 else 15.0
+            maxr1, maxr2 = self._maxr(float(np.mean(self.amagp)))
+            gsearch = search.GlobalSearch(ctx, self.a3df, dang=dang, asym=letter, nasym=nasym,
+                                          maxr1=maxr1, maxr2=maxr2, rbfact=0.0,
+                                          dtype=torch.float32, pmask=tuple(deck.pmask_internal))
+        vols = api.HalfMaps.zeros(nsam)
+        out_recs, shift_recs = [], []
+        for b0 in range(0, n, B):
+            rows = list(range(b0, min(n, b0 + B)))
+            if deck.iflag == 0:
+                idx = [self.records[r].c for r in rows]
+                pp = self._ppar(rows)
+                ctff, outc, datc = ctx.ctfapply(nsam, self._images(idx),
+                                                torch.from_numpy(pp).cuda(),
+                                                torch.from_numpy(self.pssnr).cuda(), self.gpar)
+                par = np.zeros((len(rows), 6), np.float32)
+                par[:, 0] = self.internal[rows, 3]
+                par[:, 1] = self.internal[rows, 2]
+                par[:, 2] = self.internal[rows, 1]
+                par[:, 5] = self.internal[rows, 14]
+                defo, am = pp[:, 2:5].copy(), self.amagp[rows]
+            else:
+                par, defo, am, (ctff, datc) = self.refine_batch(rows, gsearch)
+            # A3D3 (a3d3.f:39-85): odd and even particles into the two halves
+            recb = np.zeros((len(rows), 10), np.float32)
+            recb[:, 0:5] = par[:, 0:5]
+            recb[:, 5] = am
+            recb[:, 6] = self.internal[rows, 11]
+            recb[:, 7] = self._sig2(rows)
+            recb[:, 8] = par[:, 5]
+            recb[:, 9] = self.rrec
+            half = (np.asarray(rows) % 2).astype(np.int32)
+            occ_ok = recb[:, 6] != 0.0
+            if deck.reconstruct and occ_ok.any():
+                sel = np.nonzero(occ_ok)[0]
+                ctx.insert(nsam, datc[sel].contiguous(), ctff[sel].contiguous(), recb[sel],
+                           half[sel], self.sym, deck.pbc, deck.boff, self.psize, vols)
+            if deck.iflag != 0:
+                for k, r in enumerate(rows):
+                    out_recs.append(self._record(r, par[k], defo[k], am[k]))
+                    shift_recs.append(self._shift_record(r, par[k], defo[k], am[k]))
+            self.log(f" frealign_amd: {rows[-1] + 1} of {n} particles")
+        trailer = []
+        if deck.reconstruct:
+            trailer = self.finalise(vols)
+        if deck.iflag != 0:
+            fileio.par_write(ds.foutpar, out_recs, trailer)
+            fileio.par_write(ds.foutsh, shift_recs)
+        elif trailer and ds.foutpar:
+            fileio.par_write(ds.foutpar, [], trailer)
+        return out_recs
+
+    def _record(self, r, par, defo, am):
+        """lmain.f:906-923: the refined particle as its par line."""
+        rec = self.records[r]
+        two_pi = F(2.0) * PI_F
+        ang = []
+        for v in (par[2], par[1], par[0]):  # PSI THETA PHI
+            v = F(v)
+            if v < 0.0:
+                v = F(v + two_pi)
+            ang.append(F(F(v / PI_F) * F(180.0)))
+        psx = F(F(self.psize) * F(abs(am)))
+        ast = F(defo[2])
+        q = float(ast / PI_F)
+        ast = F(ast - F(math.copysign(math.floor(abs(q) + 0.5), q)) * PI_F)  # NINT
+        presa = min(max(float(par[5]), 0.0), 2.0)
+        shx = F(F(self.internal[r, 4]) + F(par[3]))
+        shy = F(F(self.internal[r, 5]) + F(par[4]))
+        return fileio.ParRecord(
+            rec.c, ang[0], ang[1], ang[2],
+            float(F(F(F(shx * psx) * F(self.nsam)) / PI_F) / F(2.0)),
+            float(F(F(F(shy * psx) * F(self.nsam)) / PI_F) / F(2.0)),
+            float(F(self.ds.dstep * 10000.0) / psx), rec.film, float(defo[0]), float(defo[1]),
+            float(F(ast / PI_F) * F(180.0)), float(self.internal[r, 11] * 100.0), rec.logp,
+            rec.sigma, presa * 100.0, (presa - float(self.internal[r, 14])) * 100.0)
+
+    def _shift_record(self, r, par, defo, am):
+        """lmain.f:924-938: the change of each parameter."""
+        new = self._record(r, par, defo, am)
+        old = self.records[r]
+
+        def dang(a, b):
+            d = (a - b) % 360.0
+            return d - 360.0 if d > 180.0 else d
+
+        return fileio.ParRecord(old.c, dang(new.psi, old.psi), dang(new.theta, old.theta),
+                                dang(new.phi, old.phi), new.shx - old.shx, new.shy - old.shy, 0.0,
+                                old.film, new.df1 - old.df1, new.df2 - old.df2,
+                                dang(new.angast, old.angast), 0.0, 0.0, 0.0,
+                                new.score - old.score)
+
+    def finalise(self, vols):
+        """frealign_v9.f:790-950 on the device; returns the FSC table lines."""
+        ctx, deck, nsam, psize = self.ctx, self.deck, self.nsam, self.psize
+        nk = float(vols.nksum.sum().item())
+        if nk == 0:
+            raise cards.CardError("no voxel was inserted")
+        ttd = vols.ttd.cpu().numpy()
+        std, vtd = ttd[0] / nk * 0.1, ttd[1] / nk * 0.1  # frealign_v9.f:792-795 (WC2 = 0.1)
+        iredun = self.sym[3]
+        b, c = ctx.storeshift(nsam, std, vtd, iredun, vols.num[0], vols.ctf2[0], vols.num[1],
+                              vols.ctf2[1])
+        j = np.arange(1, 181, dtype=np.float32)
+        x = j * PI_F / F(180.0)
+        lut = (np.sin(x) / x).astype(np.float32)  # CALCSINC (calcsinc.f:17-19)
+        fmask = 1.0
+        halves = []
+        for k, vol in enumerate((b, c)):
+            ctx.fft3d(nsam, vol, False)
+            ctx.correct3d(nsam, vol, 1, 1, lut)
+            fm, _, _ = ctx.mask3d(nsam, vol, abs(self.ri), deck.ri / psize, 6.0)
+            if k == 0:
+                fmask = fm
+            halves.append(vol.view(self.torch.float32).reshape(nsam, nsam, nsam + 2)[..., :nsam]
+                          .cpu().numpy())
+            ctx.fft3d(nsam, vol, True)
+        title = fileio.title(deck.asym)
+        if deck.map1:
+            fileio.mrc_write(deck.map1, halves[0], psize, title)
+        if deck.map2:
+            fileio.mrc_write(deck.map2, halves[1], psize, title)
+        st = ctx.sheltest(nsam, b, c, vols.ctf2[0], vols.ctf2[1], fmask=fmask)
+        fsc = st["fsc"].astype(np.float32)
+        fpart = 0.0
+        if deck.ffilt and deck.mw > 0:  # FIND_FPART's MW closed form (find_fpart.f:141)
+            import ctypes as C
+
+            from . import _native as N
+            ss = np.zeros(nsam // 2 + 3, np.float32)
+            fp = C.c_float(0.0)
+            N.check(N.lib().fr_fpart_from_mw(nsam, float(deck.mw), 0.81, float(psize),
+                                             ss.ctypes.data, C.byref(fp)), "fr_fpart_from_mw")
+            fpart = fp.value
+        a = vols.num[0].clone()
+        s = vols.ctf2[0].clone()
+        ctx.shiftvol(nsam, std, vtd, self.rlim, iredun, a, vols.num[1], s, vols.ctf2[1], 0,
+                     st["pssnr"], fpart, fsc, fmask)
+        ctx.fft3d(nsam, a, False)
+        ctx.correct3d(nsam, a, 1, 1, lut)
+        if self.ri > 0:
+            ctx.mask3d(nsam, a, self.ri, deck.ri / psize, 6.0)
+        final = a.view(self.torch.float32).reshape(nsam, nsam, nsam + 2)[..., :nsam].cpu().numpy()
+        if deck.f3d:
+            fileio.mrc_write(deck.f3d, final, psize, title)
+        res = 2.0 * psize
+        for k in range(2, len(fsc)):
+            if fsc[k] < 0.143:
+                res = nsam * psize / k
+                break
+        self.log(f" frealign_amd: FSC(0.143) at {res:.2f} A")
+        self.fsc = fsc
+        return fileio.fsc_table(nsam, psize, fsc, st["npixt"], st["pssnr"], rlim=self.rlim,
+                                fmask=fmask, fpart=fpart)
+
+
+def _runs(films):
+    """Sizes of the runs of equal film numbers (consecutive particles)."""
+    out = []
+    for k, f in enumerate(films):
+        if k and f == films[k - 1]:
+            out[-1] += 1
+        else:
+            out.append(1)
+    return out
+
+
+def main(argv=None):
+    text = sys.stdin.read()
+    try:
+        deck = cards.parse_deck(text)
+        Run(deck).run()
+    except cards.CardError as e:
+        print(f" ERROR: {e}", flush=True)
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

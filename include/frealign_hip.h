@@ -502,6 +502,36 @@ int fr_orient_nearest(fr_ctx *ctx, int n, const float *d_dm, int ncls, const flo
 int fr_orient_within(fr_ctx *ctx, int na, const float *d_dma, int nb, const float *d_dmb,
                      int nsym, const float *d_sym, float max_deg, unsigned *d_mask);
 
+/* ------------------------------------------------------------ file formats
+ * The reference's MRC/CCP4 images and maps (iof.f:177-780) and parameter-file
+ * records (card10.f:92-300, lmain.f:913-923), host side. */
+
+/* IOPEN's header read (SET_MHDR, iof.f:994-1060): NX NY NZ, MODE, PSIZE =
+ * CELLA(1)/MX, (DMIN, DMAX, DMEAN, RMS), NSYMBT; any pointer but nxyz may be
+ * NULL.  Modes 0, 1, 2; byte-swapped files by the machine stamp. */
+int fr_mrc_info(const char *path, int *nxyz, int *mode, float *psize, float *stats, int *nsymbt);
+/* IREAD (iof.f:505-576) of sections first .. first+count-1 into float32 (mode
+ * 0 bytes unsigned, mode 1 INTEGER*2, mode 2 REAL*4). */
+int fr_mrc_read(const char *path, long long first, long long count, float *out);
+/* IOPEN/IWRITE/ICLOSE of a new mode-2 file: SET_MHDR's fields (CELLA = PSIZE
+ * N, CELLB 90, MAPCRS 1 2 3, ISPG 1, 'MAP ', little-endian stamp) and ICLOSE's
+ * statistics (FP64 sums of the values and of their FP32 squares); label is
+ * the first 80-character title (may be NULL). */
+int fr_mrc_write(const char *path, int nx, int ny, int nz, float psize, const float *data,
+                 const char *label);
+/* The v9 record of lmain.f:913-923, FORMAT(I7,3F8.2,2F10.2,I8,I6,2F9.1,2F8.2,
+ * I10,F11.4,2F8.2), from the printed values (MAG and LogP as the REALs whose
+ * NINT is printed); out receives 136 characters and a terminating 0. */
+int fr_par_format(int c, float psi, float theta, float phi, float shx, float shy, float mag,
+                  int film, float df1, float df2, float angast, float occ, float logp,
+                  float sigma, float score, float change, char *out, int cap);
+/* card10.f:136-172: one record into rec[15] = C PSI THETA PHI SHX SHY MAG FILM
+ * DF1 DF2 ANGAST OCC LogP SIGMA SCORE as written; *format 0 (pre-v9, up to 95
+ * characters: OCC 100, SIGMA 0, SCORE the old phase residual), 1 (v9 fixed
+ * columns, up to 128: SIGMA 0) or 2 (list-directed).  Returns 1 for a comment
+ * line ('C' in column 1). */
+int fr_par_parse(const char *line, double *rec, int *format);
+
 /* --------------------------------- per-call entry points, Fortran symbols
  * The reference's own routine names and argument lists (Fortran 77 implicit
  * interfaces: every argument by reference, the CHARACTER length of ASYM
