@@ -81,6 +81,7 @@ SIGNATURES = {
     "fr_mrc_write": ([C.c_char_p, ci, ci, ci, cf, vp, C.c_char_p], ci),
     "fr_par_format": ([ci] + [cf] * 6 + [ci] + [cf] * 8 + [C.c_char_p, ci], ci),
     "fr_par_parse": ([C.c_char_p, vp, vp], ci),
+    "fr_calc_occ": ([ci, ci, cf, vp, vp, vp, vp], ci),
     "fr_sheltest_noalloc": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_storeshift": ([vp, ci, cd, cd, ci, vp, vp, vp, vp, vp, vp], ci),
     "fr_combine_arrays": ([vp, ci, C.c_longlong, ci, ci, ci, vp, vp, vp, vp, ci, vp, vp, vp,

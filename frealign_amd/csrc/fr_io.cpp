@@ -99,10 +99,17 @@ struct File {
 constexpr long long kNintOverflow = 1LL << 40;
 
 // Fortran F editing of one value into a field of width w with d decimals
+// (NaN and infinities right-justified as NaN, Inf, -Inf; a value too wide
+// for the field as w asterisks)
 void put_f(std::string& s, double v, int w, int d) {
   char buf[64];
-  snprintf(buf, sizeof buf, "%*.*f", w, d, v);
-  if ((int)strlen(buf) > w || !isfinite(v)) {
+  if (isnan(v))
+    snprintf(buf, sizeof buf, "%*s", w, "NaN");
+  else if (isinf(v))
+    snprintf(buf, sizeof buf, "%*s", w, v > 0 ? "Inf" : "-Inf");
+  else
+    snprintf(buf, sizeof buf, "%*.*f", w, d, v);
+  if ((int)strlen(buf) > w) {
     s.append((size_t)w, '*');
     return;
   }
@@ -370,6 +377,42 @@ int fr_par_parse(const char* line, double* rec, int* format) {
     p = end;
   }
   *format = 2;
+  return 0;
+}
+
+// calc_occ.f:179-217 for np particles in nc classes (arrays [class][particle]):
+// each particle's occupancies renormalised to 100 %, moved by the fraction f
+// towards the LogP-weighted share ON = EXP(LogP - max) A / sum * 100 (classes
+// more than 10 below the maximum get 0), and its SIGMA replaced in every
+// class by the occupancy-weighted mean; avg[i] is class i's average
+// occupancy A(I).  FP32 throughout, in the reference's order.
+int fr_calc_occ(int nc, int np, float f, const float* avg, float* occ, const float* algp,
+                float* sig) {
+  FR_CHECK(nc >= 1 && np >= 0 && avg && occ && algp && sig, "bad arguments");
+  for (int k = 0; k < np; k++) {
+    auto O = [&](int i) -> float& { return occ[(size_t)i * np + k]; };
+    auto L = [&](int i) { return algp[(size_t)i * np + k]; };
+    float sump = 0.0f;
+    for (int i = 0; i < nc; i++) sump = sump + O(i);
+    if (sump == 0.0f) {
+      sump = 100.0f;
+      for (int i = 0; i < nc; i++) O(i) = 100.0f / (float)nc;
+    }
+    for (int i = 0; i < nc; i++) O(i) = O(i) / sump * 100.0f;
+    float amax = -1.0e30f;
+    for (int i = 0; i < nc; i++) amax = fmaxf(L(i), amax);
+    sump = 0.0f;
+    for (int i = 0; i < nc; i++)
+      if (amax - L(i) < 10.0f) sump = sump + expf(L(i) - amax) * avg[i];
+    for (int i = 0; i < nc; i++) {
+      float on = 0.0f;
+      if (amax - L(i) < 10.0f) on = expf(L(i) - amax) * avg[i] / sump * 100.0f;
+      O(i) = f * (on - O(i)) + O(i);
+    }
+    sump = 0.0f;
+    for (int i = 0; i < nc; i++) sump = sump + sig[(size_t)i * np + k] * O(i) / 100.0f;
+    for (int i = 0; i < nc; i++) sig[(size_t)i * np + k] = sump;
+  }
   return 0;
 }
 

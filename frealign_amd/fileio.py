@@ -217,19 +217,28 @@ def par_write(path: str, records, trailer=None) -> None:
             f.write(line + "\n")
 
 
-def fsc_table(nsam: int, psize: float, fsc, npixt, pssnr=None, part_fsc=None, fspr=None,
-              rlim: float = 0.5, fmask: float = 0.0, fpart: float = 0.0):
+def fsc_table(nsam: int, psize: float, fsc, npixt, pssnr=None, npix=None, fspr=None,
+              rlim: float = 0.5, fmask: float = 0.0, fpart: float = 0.0, version: str = ""):
     """The 'C' lines OPRESSTATHALF appends to the output par file
-    (opresstathalf.f:59-115) for shells 2 .. NSTP with RING RAD <= RLIM:
-    NO, RESOL, RING RAD, FSPR, FSC, Part_FSC, sqrt(Part_SSNR), sqrt(Rec_SSNR),
-    CC, EXP. C, SIG C, ERFC, TOTVOX.  Rec_SSNR = 2 FSC / (1 - FSC + 1e-4);
-    columns this build does not compute (FSPR unless given, CC, EXP. C, SIG C,
-    ERFC: the PRESB and variance statistics) are written as zeros."""
+    (opresstathalf.f:46-168): the mask fractions, then for shells 2 .. NSTP
+    with RING RAD <= RLIM the row NO, RESOL, RING RAD, FSPR, FSC, Part_FSC,
+    sqrt(Part_SSNR), sqrt(Rec_SSNR), CC, EXP. C, SIG C, ERFC, TOTVOX
+    (FORMAT 7014), the NPIX-weighted 'Average' row (FORMAT 7015) and the date
+    line.  Rec_SSNR = 2 FSC / (1 - FSC + 1e-4), scaled by FMASK / FPART when
+    FPART is set, which also gives Part_FSC = SSNR / (|SSNR| + 2); FSPR is
+    written when given; CC, EXP. C, SIG C and ERFC (the PRESB and variance
+    statistics of the refinement) are written as zeros.  npix (the shell's
+    voxel count with both halves non-zero) weights the averages; npixt is
+    the shell's voxel count."""
+    import time
     lines = ["C", f"C  Fraction_mask, Fraction_particle = {fmask:11.5f}{fpart:11.5f}", "C",
              "C" + " " * 49 + "sqrt" + " " * 7 + "sqrt",
              "C  NO.  RESOL  RING RAD   FSPR    FSC  Part_FSC  Part_SSNR  Rec_SSNR       CC "
              "  EXP. C    SIG C  ERFC  TOTVOX"]
     f32 = np.float32
+    npix = npixt if npix is None else npix
+    av_pr = av_fsc = av_pfsc = av_ssnr = av_pssnr = f32(0.0)
+    isumv = 0
     for i in range(2, len(fsc) + 1):
         k = i - 1
         ring = f32(f32(k) / f32(nsam))
@@ -237,10 +246,32 @@ def fsc_table(nsam: int, psize: float, fsc, npixt, pssnr=None, part_fsc=None, fs
             break
         fi = f32(fsc[k])
         ssnr = f32(f32(2.0) * fi / (f32(1.0) - fi + f32(0.0001)))
-        pf = f32(0.0) if part_fsc is None else f32(part_fsc[k])
-        ps = f32(0.0) if pssnr is None else f32(math.sqrt(max(float(pssnr[k]), 0.0)))
+        pf = f32(0.0)
+        if fi >= 1.0:
+            pf = f32(1.0)
+        elif fpart != 0.0:
+            ssnr = f32(f32(f32(fmask) / f32(fpart)) * ssnr)
+            pf = f32(ssnr / (abs(ssnr) + f32(2.0)))
+        sq = f32(math.sqrt(abs(float(ssnr))))
+        psn = f32(0.0) if pssnr is None else f32(max(float(pssnr[k]), 0.0))
+        ps = f32(math.sqrt(float(psn)))
         pr = f32(0.0) if fspr is None else f32(fspr[k])
         lines.append("C" + f"{i:4d}{f32(psize) / ring:8.2f}{ring:10.4f}{pr:7.2f}{fi:7.3f}"
-                     f"{pf:10.3f}{ps:11.4f}{math.sqrt(abs(float(ssnr))):10.2f}"
+                     f"{pf:10.3f}{ps:11.4f}{sq:10.2f}"
                      f"{0.0:9.4f}{0.0:9.4f}{0.0:9.4f}{0.0:6.2f}{int(npixt[k]):8d}")
+        w = f32(int(npix[k]))
+        av_pr = f32(av_pr + w * pr)
+        av_fsc = f32(av_fsc + w * fi)
+        av_pfsc = f32(av_pfsc + w * pf)
+        av_ssnr = f32(av_ssnr + w * f32(sq * sq))
+        av_pssnr = f32(av_pssnr + w * psn)
+        isumv += int(npix[k])
+    den = f32(isumv) if isumv else f32(np.nan)
+    lines.append("C  Average             "
+                 f"{av_pr / den:7.2f}{av_fsc / den:7.3f}{av_pfsc / den:10.3f}"
+                 f"{math.sqrt(float(av_pssnr / den)):11.4f}{math.sqrt(float(av_ssnr / den)):10.2f}"
+                 f"{0.0:9.4f}{0.0:9.4f}{0.0:9.4f}{0.0:6.2f}")
+    lines.append("C")
+    lines.append("C Date and time      " + time.strftime("%d-%m-%Y,  %H:%M")
+                 + "    Frealign V" + (version or "frealign_amd").ljust(15)[:15])
     return lines

@@ -393,7 +393,8 @@ class Run:
                 break
         self.log(f" frealign_amd: FSC(0.143) at {res:.2f} A")
         self.fsc = fsc
-        return fileio.fsc_table(nsam, psize, fsc, st["npixt"], st["pssnr"], rlim=self.rlim,
+        return fileio.fsc_table(nsam, psize, fsc, st["npixt"], st["pssnr"], npix=st["npix"],
+                                rlim=self.rlim,
                                 fmask=fmask, fpart=fpart)
 
 

@@ -532,6 +532,13 @@ int fr_par_format(int c, float psi, float theta, float phi, float shx, float shy
  * line ('C' in column 1). */
 int fr_par_parse(const char *line, double *rec, int *format);
 
+/* calc_occ.f:179-217 (the CALC_OCC program's update): np particles in nc
+ * classes, arrays [class][particle] in FP32: OCC (in/out, %), LogP (the
+ * integers of the par files), SIGMA (in/out); avg[i] = A(I), class i's
+ * average occupancy; f the OCC change multiplier. */
+int fr_calc_occ(int nc, int np, float f, const float *avg, float *occ, const float *algp,
+                float *sig);
+
 /* --------------------------------- per-call entry points, Fortran symbols
  * The reference's own routine names and argument lists (Fortran 77 implicit
  * interfaces: every argument by reference, the CHARACTER length of ASYM
