@@ -69,6 +69,8 @@ SIGNATURES = {
     "fr_orient_within": ([vp, ci, vp, ci, vp, ci, vp, cf, vp], ci),
     "fr_insert_batch": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp, vp,
                          vp], ci),
+    "fr_insert_batch_se": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp,
+                            vp, vp, vp, vp, vp, vp, vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),
     "fr_local_search_batch": ([vp, ci, ci, vp, vp, vp, vp, vp, ci, cf, ci, cf, ci, ci, cf, vp,
                                vp, vp, vp, vp], ci),

@@ -204,7 +204,10 @@ class Context:
 
     # ------------------------------------------------------------ insertion
     def insert(self, nsam: int, datc, ctff, recb: np.ndarray, half: np.ndarray, sym, pbc: float,
-               boff: float, psize: float, vols: "HalfMaps"):
+               boff: float, psize: float, vols: "HalfMaps", stats: "VoxelStats | None" = None):
+        """The A3D3 flush for nb particles (fr_insert_batch); with `stats`
+        the _SE insertion (fr_insert_batch_se) that also accumulates the
+        per-voxel statistics."""
         _dev(datc, torch.complex64, "datc")
         _dev(ctff, torch.complex64, "ctff")
         recb = np.ascontiguousarray(recb, dtype=np.float32)
@@ -217,10 +220,16 @@ class Context:
             raise ValueError("half must be (nb,)")
         nsym, symop, jsym, _ = sym
         self._sync_stream()
-        N.check(N.lib().fr_insert_batch(
-            self.h, nsam, nb, _p(datc), _p(ctff), _p(recb), _p(half), nsym, _p(jsym), _p(symop),
-            float(pbc), float(boff), float(psize), _p(vols.num[0]), _p(vols.ctf2[0]),
-            _p(vols.num[1]), _p(vols.ctf2[1]), _p(vols.nksum), _p(vols.ttd)), "fr_insert_batch")
+        args = (self.h, nsam, nb, _p(datc), _p(ctff), _p(recb), _p(half), nsym, _p(jsym),
+                _p(symop), float(pbc), float(boff), float(psize), _p(vols.num[0]),
+                _p(vols.ctf2[0]), _p(vols.num[1]), _p(vols.ctf2[1]), _p(vols.nksum), _p(vols.ttd))
+        if stats is None:
+            N.check(N.lib().fr_insert_batch(*args), "fr_insert_batch")
+            return
+        if stats.nsam != nsam:
+            raise ValueError("stats are for another box size")
+        N.check(N.lib().fr_insert_batch_se(*args, _p(stats.ref), _p(stats.ksum), _p(stats.vsum),
+                                           _p(stats.psum), _p(stats.asum)), "fr_insert_batch_se")
 
     # --------------------------------------------------------------- search
     def search_layout(self, nsam: int, maxr1: int, maxr2: int):
@@ -828,6 +837,31 @@ class HalfMaps:
         ctf2 = tuple(torch.zeros(shp, dtype=torch.float32, device=device) for _ in range(2))
         return HalfMaps(nsam, num, ctf2, torch.zeros(2, dtype=torch.int64, device=device),
                         torch.zeros(2, dtype=torch.float64, device=device))
+
+
+@dataclass
+class VoxelStats:
+    """The per-voxel statistics of the _SE insertion (NNSTAT = 1,
+    pinsert_trilin_se.f:186-194) over both half-maps: KSUM (terms received),
+    VSUM (sum |term|^2), PSUM (sum of phase differences to the reference
+    transform `ref`), ASUM (sum |term|), resident on the device."""
+    nsam: int
+    ref: torch.Tensor
+    ksum: torch.Tensor
+    vsum: torch.Tensor
+    psum: torch.Tensor
+    asum: torch.Tensor
+
+    @staticmethod
+    def zeros(nsam: int, ref: torch.Tensor, device="cuda") -> "VoxelStats":
+        _dev(ref, torch.complex64, "ref")
+        jc = nsam // 2 + 1
+        if ref.numel() != nsam * nsam * jc:
+            raise ValueError("ref must be the (NSAM, NSAM, NSAM/2+1) half-volume")
+        shp = (nsam, nsam, jc)
+        z = lambda dt: torch.zeros(shp, dtype=dt, device=device)  # noqa: E731
+        return VoxelStats(nsam, ref, z(torch.int32), z(torch.float32), z(torch.float32),
+                          z(torch.float32))
 
 
 class Plan:

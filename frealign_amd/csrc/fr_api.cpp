@@ -448,11 +448,41 @@ int fr_presb_batch(fr_ctx* c, int nsam, int n, const float* d_a3df, const float*
 }
 
 // -------------------------------------------------------------- insertion
+namespace {
+int insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const float* d_ctff,
+                 const float* recb, const int* half, int nsym, const int* jsym,
+                 const float* symop, float pbc, float boff, float psize, float* d_num0,
+                 float* d_ctf2_0, float* d_num1, float* d_ctf2_1, long long* d_nksum,
+                 double* d_ttd, const fr::InsertStats* stats);
+}  // namespace
+
 int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const float* d_ctff,
                     const float* recb, const int* half, int nsym, const int* jsym,
                     const float* symop, float pbc, float boff, float psize, float* d_num0,
                     float* d_ctf2_0, float* d_num1, float* d_ctf2_1, long long* d_nksum,
                     double* d_ttd) {
+  return insert_batch(c, nsam, nb, d_datc, d_ctff, recb, half, nsym, jsym, symop, pbc, boff, psize,
+                      d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd, nullptr);
+}
+
+int fr_insert_batch_se(fr_ctx* c, int nsam, int nb, const float* d_datc, const float* d_ctff,
+                       const float* recb, const int* half, int nsym, const int* jsym,
+                       const float* symop, float pbc, float boff, float psize, float* d_num0,
+                       float* d_ctf2_0, float* d_num1, float* d_ctf2_1, long long* d_nksum,
+                       double* d_ttd, const float* d_c3df, int* d_ksum, float* d_vsum,
+                       float* d_psum, float* d_asum) {
+  FR_CHECK(d_c3df && d_ksum && d_vsum && d_psum && d_asum, "null statistics argument");
+  const fr::InsertStats st{d_c3df, d_ksum, d_vsum, d_psum, d_asum};
+  return insert_batch(c, nsam, nb, d_datc, d_ctff, recb, half, nsym, jsym, symop, pbc, boff, psize,
+                      d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd, &st);
+}
+
+namespace {
+int insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const float* d_ctff,
+                 const float* recb, const int* half, int nsym, const int* jsym,
+                 const float* symop, float pbc, float boff, float psize, float* d_num0,
+                 float* d_ctf2_0, float* d_num1, float* d_ctf2_1, long long* d_nksum,
+                 double* d_ttd, const fr::InsertStats* stats) {
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(nb >= 0 && d_datc && d_ctff && recb && half && jsym && symop && d_num0 && d_ctf2_0 &&
@@ -514,10 +544,11 @@ int fr_insert_batch(fr_ctx* c, int nsam, int nb, const float* d_datc, const floa
   int pslot = c->prof_begin(kProfInsert);
   int r = launch_insert(*c, nsam, nb, (int)tasks.size(), d_tasks, d_datc, d_ctff, d_num0,
                         d_ctf2_0, d_num1, d_ctf2_1, reinterpret_cast<unsigned long long*>(d_nksum),
-                        d_ttd, scratch);
+                        d_ttd, scratch, stats);
   c->prof_end(pslot);
   return r;
 }
+}  // namespace
 
 // ------------------------------------------------------------- statistics
 int fr_sheltest(fr_ctx* c, int nsam, int istp, const float* d_a, const float* d_d,

@@ -72,8 +72,10 @@ struct State {
   fr_ctx* ctx = nullptr;
   Dev spec, ctff, fsct, dm, shift, amag, pidx, out;
   Dev vol, vol2, outc, datc, ppar, pssnr, task, sums, work;
+  Dev stat[4];                    // KSUM, VSUM, PSUM, ASUM of PINSERT_TRILIN(_SE)
   Resident ref;                   // the reference transform
   Resident maps[4];               // A3DF, S3DF, D3DF, V3DF of A3D3
+  Resident stats[4];              // KSUM, VSUM, PSUM, ASUM of A3D3 (NNSTAT = 1)
 };
 
 State& st() {
@@ -410,24 +412,25 @@ void ctfapply_(const int* nsam, const float* spec, const float* shx, const float
   download(datc, ddc, sizeof(float) * 2 * ic, R);
 }
 
-// pinsert_trilin.f:8-244 (IRAD = 2 trilinear weights, IEWALD = 0, NNSTAT = 0):
-// the stripe IP of width ST, I = -JC+1+NINT((IP-1)*ST) .. -JC+NINT(IP*ST)
-// (pinsert_trilin.f:44-45), added into B3DF, T3DF, NKSUM and TTD.
-void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, const float* spec,
-                     const float* ctff, const float* occ, float* b3df, float* t3df, double* ttd,
-                     const float* shx, const float* shy, float* /*asum*/, float* /*vsum*/,
-                     float* /*psum*/, float* /*c3df*/, int* /*ksum*/, const float* /*sinclut*/,
-                     const int* /*irada*/, const int* ipad, const float* /*thet*/,
-                     const int* iewald, const int* nnstat, long long* nksum, const int* jc,
-                     const float* dm, const float* tmp, const int* nsamh, const int* ic,
-                     const float* dx, const float* dy, const float* stw /* ST */, const int* ip,
-                     int* /*ns1*/) {
-  const char* R = "PINSERT_TRILIN";
+}  // extern "C"
+
+namespace {
+// pinsert_trilin.f:8-244 and pinsert_trilin_se.f:8-359 (IRAD = 2 trilinear
+// weights, IEWALD = 0, IPAD = 1): the stripe IP of width ST,
+// I = -JC+1+NINT((IP-1)*ST) .. -JC+NINT(IP*ST) (pinsert_trilin.f:44-45),
+// added into B3DF, T3DF, NKSUM and TTD, and with NNSTAT != 0 into the
+// per-voxel KSUM, VSUM, PSUM, ASUM against C3DF (pinsert_trilin_se.f:186-194).
+void insert_stripe(const char* R, const int* nsam, const int* irrec2, const float* spec,
+                   const float* ctff, const float* occ, float* b3df, float* t3df, double* ttd,
+                   const float* shx, const float* shy, float* asum, float* vsum, float* psum,
+                   const float* c3df, int* ksum, const int* ipad, const int* iewald,
+                   const int* nnstat, long long* nksum, const int* jc, const float* dm,
+                   const float* tmp, const int* nsamh, const int* ic, const float* dx,
+                   const float* dy, const float* stw, const int* ip) {
   const int n = *nsam;
   need(n >= 8 && n % 2 == 0 && n <= 1024, R, "NSAM must be even, 8 .. 1024");
   need(*ipad == 1, R, "only IPAD = 1 is supported");
   need(*iewald == 0, R, "Ewald-sphere insertion (IEWALD != 0) is not supported");
-  need(*nnstat == 0, R, "the NNSTAT statistics are not supported");
   const int j = n / 2 + 1;
   need(*jc == j && *nsamh == n / 2 && *ic == n * j, R, "JC, NSAMH or IC inconsistent with NSAM");
   // pinsert_trilin.f:44-45 (NINT: nearest, halves away from zero)
@@ -464,20 +467,68 @@ void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, c
   double* dd = (double*)(dk + 2);
   void* work = s.work.get(fr::insert_scratch_bytes(n, 1, 1));
   need(work != nullptr, R, "device allocation failed");
+  fr::InsertStats stats{};
+  void* sd[4] = {nullptr, nullptr, nullptr, nullptr};
+  void* sh[4] = {ksum, vsum, psum, asum};
+  if (*nnstat != 0) {
+    need(c3df && ksum && vsum && psum && asum, R, "NNSTAT arrays missing");
+    stats.ref = (const float*)resident(s.ref, c3df, sizeof(float) * 2 * hv, R);
+    for (int q = 0; q < 4; q++) sd[q] = upload(s.stat[q], sh[q], sizeof(float) * hv, R);
+    stats.ksum = (int*)sd[0];
+    stats.vsum = (float*)sd[1];
+    stats.psum = (float*)sd[2];
+    stats.asum = (float*)sd[3];
+  }
   // the second half-map is never addressed (t.half = 0): B3DF/T3DF stand in
   check(fr::launch_insert(*c, n, 1, 1, (const fr::InsertTask*)dtask, (const float*)ds,
                           (const float*)dc, (float*)db, (float*)dt, (float*)db, (float*)dt, dk,
-                          dd, work),
+                          dd, work, *nnstat != 0 ? &stats : nullptr),
         R);
   check(fr_synchronize(c), R);
   download(b3df, db, sizeof(float) * 2 * hv, R);
   download(t3df, dt, sizeof(float) * hv, R);
+  if (*nnstat != 0)
+    for (int q = 0; q < 4; q++) download(sh[q], sd[q], sizeof(float) * hv, R);
   unsigned long long k = 0;
   double td = 0.0;
   download(&k, dk, sizeof k, R);
   download(&td, dd, sizeof td, R);
   *nksum += (long long)k;
   *ttd += td;
+}
+}  // namespace
+
+extern "C" {
+
+// pinsert_trilin.f:8-244 (IEWALD = 0, IPAD = 1; the ASUM..KSUM statistics
+// with NNSTAT != 0, as PINSERT only asks of the _SE variant)
+void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, const float* spec,
+                     const float* ctff, const float* occ, float* b3df, float* t3df, double* ttd,
+                     const float* shx, const float* shy, float* asum, float* vsum, float* psum,
+                     float* c3df, int* ksum, const float* /*sinclut*/, const int* /*irada*/,
+                     const int* ipad, const float* /*thet*/, const int* iewald,
+                     const int* nnstat, long long* nksum, const int* jc, const float* dm,
+                     const float* tmp, const int* nsamh, const int* ic, const float* dx,
+                     const float* dy, const float* stw /* ST */, const int* ip, int* /*ns1*/) {
+  insert_stripe("PINSERT_TRILIN", nsam, irrec2, spec, ctff, occ, b3df, t3df, ttd, shx, shy, asum,
+                vsum, psum, c3df, ksum, ipad, iewald, nnstat, nksum, jc, dm, tmp, nsamh, ic, dx,
+                dy, stw, ip);
+}
+
+// pinsert_trilin_se.f:8-359 (IEWALD = 0, IPAD = 1): the statistics variant,
+// same arguments
+void pinsert_trilin_se_(const int* nsam, const float* /*rii*/, const int* irrec2,
+                        const float* spec, const float* ctff, const float* occ, float* b3df,
+                        float* t3df, double* ttd, const float* shx, const float* shy, float* asum,
+                        float* vsum, float* psum, float* c3df, int* ksum,
+                        const float* /*sinclut*/, const int* /*irada*/, const int* ipad,
+                        const float* /*thet*/, const int* iewald, const int* nnstat,
+                        long long* nksum, const int* jc, const float* dm, const float* tmp,
+                        const int* nsamh, const int* ic, const float* dx, const float* dy,
+                        const float* stw, const int* ip, int* /*ns1*/) {
+  insert_stripe("PINSERT_TRILIN_SE", nsam, irrec2, spec, ctff, occ, b3df, t3df, ttd, shx, shy,
+                asum, vsum, psum, c3df, ksum, ipad, iewald, nnstat, nksum, jc, dm, tmp, nsamh, ic,
+                dx, dy, stw, ip);
 }
 
 // a3d3.f:8-146 (INTERP = 1, IEWALD = 0, NNSTAT = 0, NONSYM false): the
@@ -493,8 +544,8 @@ void pinsert_trilin_(const int* nsam, const float* /*rii*/, const int* irrec2, c
 // the driver's PBC is, so it is taken as REAL here.
 void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, const float* /*ri*/,
            const float* rrec, const int* k, const int* ilast, const float* amagp,
-           const float* datc, const float* ctff, float* /*asum*/, float* /*vsum*/,
-           float* /*psum*/, float* /*c3df*/, int* /*ksum*/, const float* phi, const float* theta,
+           const float* datc, const float* ctff, float* asum, float* vsum, float* psum,
+           float* c3df, int* ksum, const float* phi, const float* theta,
            const float* psi, const float* dshx, const float* dshy, const float* presa,
            const float* pbc, const float* boff, float* a3df, float* s3df, double* /*std*/,
            float* d3df, float* v3df, double* /*vtd*/, const int* nn1, const int* /*maxset*/,
@@ -513,8 +564,7 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
   need(n >= 8 && n % 2 == 0 && n <= 1024 && n <= *nn1, R, "NSAM must be even, 8 .. min(NN1, 1024)");
   need(*nvol >= 1 && *nvol1 >= 1 && *nvol2 >= 1, R, "NVOL, NVOL1 and NVOL2 must be >= 1");
   need(*irad != 0 && *interp == 1, R, "only trilinear insertion (INTERP = 1) is supported");
-  need(*ipad == 1 && *iewald == 0 && *nnstat == 0, R,
-       "only IPAD = 1, IEWALD = 0, NNSTAT = 0 are supported");
+  need(*ipad == 1 && *iewald == 0, R, "only IPAD = 1 and IEWALD = 0 are supported");
   need(asym[0] != 'H', R, "helical symmetry is not supported");
   need(*nonsym == 0, R, "the NONSYM insertion is not supported");
   const size_t ic = half_plane(n);
@@ -565,6 +615,24 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
       else
         dev[q] = (float*)resident(r, host[q], bytes[q], R);
     }
+    // NNSTAT != 0: KSUM, VSUM, PSUM, ASUM(NN3*NNSTAT+1, NVOL), column I per
+    // slot (a3d3_s.f:70-71), resident like the half-maps; C3DF the reference
+    void* sh[4] = {ksum, vsum, psum, asum};
+    float* sdev[4] = {nullptr, nullptr, nullptr, nullptr};
+    const float* dref = nullptr;
+    const size_t scol = nvox + 1;
+    if (*nnstat != 0) {
+      need(c3df && ksum && vsum && psum && asum, R, "NNSTAT arrays missing");
+      dref = (const float*)resident(s.ref, c3df, sizeof(float) * 2 * hv, R);
+      const size_t sb = sizeof(float) * scol * *nvol;
+      for (int q = 0; q < 4; q++) {
+        Resident& r = s.stats[q];
+        if (r.dirty && r.host == sh[q] && r.bytes == sb)
+          sdev[q] = (float*)r.dev.p;
+        else
+          sdev[q] = (float*)resident(r, sh[q], sb, R);
+      }
+    }
     void* dsum = s.sums.get(2 * sizeof(long long) + 2 * sizeof(double));
     need(dsum != nullptr, R, "device allocation failed");
     long long* dk = (long long*)dsum;
@@ -583,10 +651,20 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
       need(hipMemset(dsum, 0, 2 * sizeof(long long) + 2 * sizeof(double)) == hipSuccess, R,
            "device memset failed");
       // one half is addressed (ISW): its arrays stand in for the other's
-      check(fr_insert_batch(c, n, 1, (const float*)ds, (const float*)dc, recb + 10 * (size_t)i,
-                            &h, *nsym, jsym, symop, *pbc, *boff, *psize, num, ctf2, num, ctf2, dk,
-                            dd),
-            R);
+      if (*nnstat != 0) {
+        const size_t o = scol * i;
+        check(fr_insert_batch_se(c, n, 1, (const float*)ds, (const float*)dc,
+                                 recb + 10 * (size_t)i, &h, *nsym, jsym, symop, *pbc, *boff,
+                                 *psize, num, ctf2, num, ctf2, dk, dd, dref,
+                                 (int*)sdev[0] + o, sdev[1] + o, sdev[2] + o, sdev[3] + o),
+              R);
+        for (int q = 0; q < 4; q++) s.stats[q].dirty = true;
+      } else {
+        check(fr_insert_batch(c, n, 1, (const float*)ds, (const float*)dc, recb + 10 * (size_t)i,
+                              &h, *nsym, jsym, symop, *pbc, *boff, *psize, num, ctf2, num, ctf2,
+                              dk, dd),
+              R);
+      }
       check(fr_synchronize(c), R);
       long long kc[2];
       double td[2];
@@ -602,7 +680,10 @@ void a3d3_(const int* nset, int* iswitch, const int* nsam, const int* irad, cons
   // the particle loop ends here: the host's half-maps become current
   if (*k == *ilast) {
     State& s = st();
-    for (int q = 0; q < 4; q++) write_back(s.maps[q], R);
+    for (int q = 0; q < 4; q++) {
+      write_back(s.maps[q], R);
+      write_back(s.stats[q], R);
+    }
   }
 }
 
@@ -657,7 +738,10 @@ void fr_percall_invalidate_(void) { fr_percall_invalidate(); }
 // host arrays now (A3D3 itself does so at K = ILAST).  (Fortran: CALL FR_PERCALL_FLUSH)
 void fr_percall_flush(void) {
   State& s = st();
-  for (int q = 0; q < 4; q++) write_back(s.maps[q], "FR_PERCALL_FLUSH");
+  for (int q = 0; q < 4; q++) {
+    write_back(s.maps[q], "FR_PERCALL_FLUSH");
+    write_back(s.stats[q], "FR_PERCALL_FLUSH");
+  }
 }
 void fr_percall_flush_(void) { fr_percall_flush(); }
 

@@ -21,6 +21,11 @@
 // and TTD are deterministic, and no atomics are needed.  The per-point terms
 // (sample x phase ramp x CTF x weight, and CTF^2 x weight) depend only on the
 // particle, so they are formed once per particle.
+//
+// The _SE variant (PINSERT_TRILIN_SE with NNSTAT = 1,
+// pinsert_trilin_se.f:186-194) also sums, per voxel, the count, |term|^2,
+// |term| and the phase difference to the reference transform of the terms it
+// receives; being per voxel, these ride on the same gather for free.
 #include <hip/hip_runtime.h>
 
 #include <math.h>
@@ -241,11 +246,19 @@ __device__ inline bool plane_reaches(const InsertGeo& g, float xl, float xh, flo
   return cx * cx + cy * cy + cz * cz <= g.rmax2;
 }
 
+// pdiff.f:8-24: |phase of b conj(r)|, 0 where the product is 0
+__device__ inline float pdiff_dev(float br, float bi, float2 r) {
+  const float cr = br * r.x + bi * r.y, ci = bi * r.x - br * r.y;
+  if (cr == 0.0f && ci == 0.0f) return 0.0f;
+  return fabsf(atan2f(ci, cr));
+}
+
+template <bool kStats>
 __global__ void __launch_bounds__(kGatherThreads)
     k_insert_gather(int nsam, int nb, int ntask, int tx, int ty, const InsertTask* __restrict__ tasks,
                     const InsertGeo* __restrict__ geo, const float4* __restrict__ pts,
                     float2* num0, float* ctf2_0, float2* num1, float* ctf2_1,
-                    unsigned long long* pcount, double* pttd) {
+                    unsigned long long* pcount, double* pttd, InsertStats st) {
   __shared__ int list[kChunk];
   __shared__ int wave_tot[kGatherThreads / 64];
   const int jc = nsam / 2 + 1, nsamh = nsam / 2, w = 2 * jc - 1;
@@ -270,6 +283,12 @@ __global__ void __launch_bounds__(kGatherThreads)
   float a0r = 0.f, a0i = 0.f, a0c = 0.f, a1r = 0.f, a1i = 0.f, a1c = 0.f;
   unsigned c0 = 0u, c1 = 0u;
   double t0 = 0.0, t1 = 0.0;
+  // _SE statistics of this voxel (both halves), against its reference value
+  float sv = 0.f, sp = 0.f, sa = 0.f;
+  float2 refv = make_float2(0.f, 0.f);
+  if (kStats && mine)
+    refv = reinterpret_cast<const float2*>(
+        st.ref)[(size_t)l + (size_t)jc * ((size_t)ms_ + (size_t)nsam * ns_)];
   for (int base = 0; base < ntask; base += kChunk) {
     __syncthreads();  // the previous chunk's list is no longer read
     const int cnt = min(kChunk, ntask - base);
@@ -372,6 +391,13 @@ __global__ void __launch_bounds__(kGatherThreads)
                 const float box = (1.0f - fabsf(x3 - fl)) * b2 * b3;
                 const float4 pv = tp[(size_t)(j + jc - 1) * w + (i + jc - 1)];
                 const float cv = pv.z * box;
+                if (kStats) {
+                  const float br = pv.x * box, bi = pv.y * box;
+                  const float ab = hypotf(br, bi);
+                  sv += ab * ab;
+                  sa += ab;
+                  sp += pdiff_dev(br, bi, refv);
+                }
                 if (t.half) {
                   a1r += pv.x * box;
                   a1i += pv.y * box;
@@ -402,6 +428,12 @@ __global__ void __launch_bounds__(kGatherThreads)
       float2 o = num1[v];
       num1[v] = make_float2(o.x + a1r, o.y + a1i);
       ctf2_1[v] += a1c;
+    }
+    if (kStats && (c0 | c1)) {
+      st.ksum[v] += (int)(c0 + c1);
+      st.vsum[v] += sv;
+      st.psum[v] += sp;
+      st.asum[v] += sa;
     }
   }
   // the tile's NKSUM and TTD: a fixed reduction order
@@ -490,7 +522,7 @@ size_t insert_scratch_bytes(int nsam, int nb, int ntask) {
 int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks,
                   const float* d_datc, const float* d_ctff, float* num0, float* ctf2_0,
                   float* num1, float* ctf2_1, unsigned long long* d_nksum, double* d_ttd,
-                  void* scratch) {
+                  void* scratch, const InsertStats* stats) {
   if (ntask <= 0 || nb <= 0) return 0;
   FR_CHECK(scratch != nullptr, "insertion scratch missing");
   const int jc = nsam / 2 + 1, w = 2 * jc - 1;
@@ -512,9 +544,19 @@ int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks
   hipLaunchKernelGGL(k_insert_geo, dim3((ntask + kThreads - 1) / kThreads), dim3(kThreads), 0,
                      c.stream, ntask, nsam, d_tasks, s.geo);
   FR_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_insert_gather, dim3(ntiles), dim3(kGatherThreads), 0, c.stream, nsam, nb,
-                     ntask, tx, ty, d_tasks, s.geo, s.pts, reinterpret_cast<float2*>(num0),
-                     ctf2_0, reinterpret_cast<float2*>(num1), ctf2_1, s.pcount, s.pttd);
+  if (stats) {
+    FR_CHECK(stats->ref && stats->ksum && stats->vsum && stats->psum && stats->asum,
+             "insertion statistics arrays missing");
+    hipLaunchKernelGGL(k_insert_gather<true>, dim3(ntiles), dim3(kGatherThreads), 0, c.stream,
+                       nsam, nb, ntask, tx, ty, d_tasks, s.geo, s.pts,
+                       reinterpret_cast<float2*>(num0), ctf2_0, reinterpret_cast<float2*>(num1),
+                       ctf2_1, s.pcount, s.pttd, *stats);
+  } else {
+    hipLaunchKernelGGL(k_insert_gather<false>, dim3(ntiles), dim3(kGatherThreads), 0, c.stream,
+                       nsam, nb, ntask, tx, ty, d_tasks, s.geo, s.pts,
+                       reinterpret_cast<float2*>(num0), ctf2_0, reinterpret_cast<float2*>(num1),
+                       ctf2_1, s.pcount, s.pttd, InsertStats{});
+  }
   FR_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_insert_reduce, dim3(1), dim3(kReduceThreads), 0, c.stream, ntiles,
                      s.pcount, s.pttd, d_nksum, d_ttd);

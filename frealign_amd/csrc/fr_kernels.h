@@ -17,12 +17,24 @@ struct InsertTask {
   short ilo, ihi;  // I stripe of pinsert_trilin.f:44-45 (whole plane: -NSAM/2, NSAM/2)
 };
 
+// The per-voxel statistics of the _SE insertion (NNSTAT = 1,
+// pinsert_trilin_se.f:186-194), over both half-maps' terms: ksum += 1,
+// vsum += |BSAMP|^2, psum += PDIFF(BSAMP, ref), asum += |BSAMP|; ref is the
+// reference transform in the half-volume layout (IPAD = 1).
+struct InsertStats {
+  const float* ref;
+  int* ksum;
+  float* vsum;
+  float* psum;
+  float* asum;
+};
+
 // scratch of launch_insert for nb particles and ntask tasks (grouped by particle)
 size_t insert_scratch_bytes(int nsam, int nb, int ntask);
 int launch_insert(Ctx& c, int nsam, int nb, int ntask, const InsertTask* d_tasks,
                   const float* d_datc, const float* d_ctff, float* num0, float* ctf2_0,
                   float* num1, float* ctf2_1, unsigned long long* d_nksum, double* d_ttd,
-                  void* scratch);
+                  void* scratch, const InsertStats* stats = nullptr);
 
 // FP64 shell sums of sheltest.f:48-88 (istp-wide shells); acc is nstp x 6 doubles:
 // DFSC, MEAN12, MEAN22, PSSNR-sum, NPIX, NPIXT.  acc is cleared first, by a fill

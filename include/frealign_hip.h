@@ -152,6 +152,21 @@ int fr_insert_batch(fr_ctx *ctx, int nsam, int nb, const float *d_datc, const fl
                     float *d_ctf2_0, float *d_num1, float *d_ctf2_1, long long *d_nksum,
                     double *d_ttd);
 
+/* Replaces the same flush with NNSTAT = 1 (FSTAT), where PINSERT dispatches to
+ * PINSERT_TRILIN_SE (pinsert.f:137-176, pinsert_trilin_se.f:8-359; IEWALD=0,
+ * IPAD=1): as fr_insert_batch, and every inserted term BSAMP also adds, at its
+ * voxel, to the per-voxel statistics shared by both half-maps
+ * (pinsert_trilin_se.f:186-194): d_ksum += 1 (int), d_vsum += |BSAMP|^2,
+ * d_psum += PDIFF(BSAMP, C3DF) (pdiff.f), d_asum += |BSAMP|.  d_c3df is the
+ * reference transform C3DF in the half-volume layout (complex, as d_num0);
+ * the four statistics arrays have one entry per half-volume voxel. */
+int fr_insert_batch_se(fr_ctx *ctx, int nsam, int nb, const float *d_datc, const float *d_ctff,
+                       const float *recb, const int *half, int nsym, const int *jsym,
+                       const float *symop, float pbc, float boff, float psize, float *d_num0,
+                       float *d_ctf2_0, float *d_num1, float *d_ctf2_1, long long *d_nksum,
+                       double *d_ttd, const float *d_c3df, int *d_ksum, float *d_vsum,
+                       float *d_psum, float *d_asum);
+
 /* ------------------------------------------------------------- statistics */
 
 /* Replaces COMBINE_ARRAYS (combine_arrays.f:8-69, NNSTAT=0): copies 2..nvol1
@@ -623,9 +638,11 @@ void prefine_(const int *nsam, const int *irad, float *amag, const int *maxr1, c
               const float *spsi, const float *stif, const float *pweights,
               const void *fftw_plans, const float *sm, size_t asym_len);
 
-/* PINSERT_TRILIN (pinsert_trilin.f:8-244; IEWALD = 0, NNSTAT = 0): the
+/* PINSERT_TRILIN (pinsert_trilin.f:8-244; IEWALD = 0, IPAD = 1): the
  * stripe IP of width ST of the plane DM(1..6), phase shift SHX+DX, SHY+DY,
- * added into B3DF, T3DF, NKSUM and TTD (volumes in the batched layout). */
+ * added into B3DF, T3DF, NKSUM and TTD (volumes in the batched layout); with
+ * NNSTAT != 0 also into KSUM, VSUM, PSUM, ASUM against C3DF, as the _SE
+ * variant does. */
 void pinsert_trilin_(const int *nsam, const float *rii, const int *irrec2, const float *spec,
                      const float *ctff, const float *occ, float *b3df, float *t3df, double *ttd,
                      const float *shx, const float *shy, float *asum, float *vsum, float *psum,
@@ -635,12 +652,28 @@ void pinsert_trilin_(const int *nsam, const float *rii, const int *irrec2, const
                      const int *nsamh, const int *ic, const float *dx, const float *dy,
                      const float *st, const int *ip, int *ns1);
 
+/* PINSERT_TRILIN_SE (pinsert_trilin_se.f:8-359; IEWALD = 0, IPAD = 1): the
+ * same stripe, with NNSTAT != 0 adding every term BSAMP to the per-voxel
+ * statistics (pinsert_trilin_se.f:186-194): KSUM += 1, VSUM += |BSAMP|^2,
+ * PSUM += PDIFF(BSAMP, C3DF), ASUM += |BSAMP|.  Same arguments as
+ * PINSERT_TRILIN. */
+void pinsert_trilin_se_(const int *nsam, const float *rii, const int *irrec2, const float *spec,
+                        const float *ctff, const float *occ, float *b3df, float *t3df,
+                        double *ttd, const float *shx, const float *shy, float *asum,
+                        float *vsum, float *psum, float *c3df, int *ksum, const float *sinclut,
+                        const int *irada, const int *ipad, const float *thet, const int *iewald,
+                        const int *nnstat, long long *nksum, const int *jc, const float *dm,
+                        const float *tmp, const int *nsamh, const int *ic, const float *dx,
+                        const float *dy, const float *st, const int *ip, int *ns1);
+
 /* A3D3 (a3d3.f:8-146; INTERP = 1, non-helical, NONSYM false): records the
  * particle in RECBF/CTFBF/RECB/ISW, alternates ISWITCH, and when NVOL
  * particles are buffered or K = ILAST flushes slot I into private volume IS
  * (column IS of A3DF/S3DF(NN3, NVOL1) or D3DF/V3DF(NN3, NVOL2); IS = I, or
- * (I-1)/2+1 when IFSC = 0, a3d3_s.f:39-40), NKSUM1(I) and TTD1/TTD2(IS).
- * PBC is REAL storage (pinsert.f:31 reads it so). */
+ * (I-1)/2+1 when IFSC = 0, a3d3_s.f:39-40), NKSUM1(I) and TTD1/TTD2(IS);
+ * with NNSTAT != 0 also into column I of KSUM/VSUM/PSUM/ASUM(NN3+1, NVOL)
+ * against C3DF (the _SE insertion).  PBC is REAL storage (pinsert.f:31 reads
+ * it so). */
 void a3d3_(const int *nset, int *iswitch, const int *nsam, const int *irad, const float *ri,
            const float *rrec, const int *k, const int *ilast, const float *amagp,
            const float *datc, const float *ctff, float *asum, float *vsum, float *psum,

@@ -45,6 +45,12 @@ static inline float cabs2f(float a, float b) {
 #define IRFFT2(n, d) or_irfft2_f(n, d)
 #endif
 
+#if OR_FP64
+#define RCABS(a, b) hypot(a, b)
+#else
+#define RCABS(a, b) hypotf(a, b)
+#endif
+
 #define PI_F 3.1415926535897f
 #define TWOPI_F 6.2831853071796f
 
@@ -534,10 +540,27 @@ void P(presb)(int nsam, float amag, const R *spec, const float *a3df, float phi,
 /* ------------------------------------------------- pinsert_trilin.f */
 /* pinsert_trilin.f:44-244 for the stripe I = i1..i3 (loops 30 and 31 cover it
  * in order); the phase uses SHX+DX, SHY+DY (pinsert_trilin.f:55,63) */
+/* pdiff.f:8-24: |phase of C1 conj(C2)|, 0 where the product is 0 */
+static R pdiff(C c1, C c2) {
+  C c = cmul(c1, cconj(c2));
+  if (c.re == (R)0 && c.im == (R)0) return (R)0;
+  return RABS(RATAN2(c.im, c.re));
+}
+
+/* the per-voxel statistics of the _SE variants (NNSTAT = 1,
+ * pinsert_trilin_se.f:186-194): KSUM += 1, VSUM += |BSAMP|^2,
+ * PSUM += PDIFF(BSAMP, C3DF), ASUM += |BSAMP|, with IPAD = 1 (C3DF has the
+ * half-volume's layout, ID4 = ID) */
+typedef struct {
+  const R *c3df;
+  int *ksum;
+  R *vsum, *psum, *asum;
+} Stats;
+
 static void pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff, R occ, R *b3df,
                            R *t3df, double *ttd, float shx, float shy, int64_t *nksum, int jc,
                            const float dm[9], float tmp, int nsamh, size_t ic, int i1, int i3,
-                           float dx, float dy) {
+                           float dx, float dy, const Stats *st) {
   const float ri12 = 0.5f, ri2 = 1.0f; /* IRAD=2: (IRAD-1)/2, IRAD/2 */
   const float sx = shx + dx, sy = shy + dy;
   for (int i = i1; i <= i3; i++) {
@@ -603,6 +626,13 @@ static void pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff, R
             b3df[2 * id + 1] = b3df[2 * id + 1] + bs.im;
             t3df[id] = t3df[id] + c2;
             (*nksum)++;
+            if (st) {
+              R ab = RCABS(bs.re, bs.im);
+              st->ksum[id] = st->ksum[id] + 1;
+              st->vsum[id] = st->vsum[id] + ab * ab;
+              st->psum[id] = st->psum[id] + pdiff(bs, cld(st->c3df, id));
+              st->asum[id] = st->asum[id] + ab;
+            }
             *ttd = *ttd + (double)c2;
           }
         }
@@ -612,10 +642,11 @@ static void pinsert_trilin(int nsam, int irrec2, const R *spec, const R *ctff, R
 }
 
 /* ---------------------------------------------------------- pinsert.f */
-void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, R *b3df, R *t3df,
-                double *ttd, float phi, float theta, float psi, float shx, float shy, float presa,
-                float pbc, float boff, int nsym, const int *jsym, const float *symop,
-                int64_t *nksum, float psize, float occ, const float *sm, int nonsym) {
+static void pinsert_common(int nsam, float rrec, float amag, const R *spec, const R *ctff,
+                           R *b3df, R *t3df, double *ttd, float phi, float theta, float psi,
+                           float shx, float shy, float presa, float pbc, float boff, int nsym,
+                           const int *jsym, const float *symop, int64_t *nksum, float psize,
+                           float occ, const float *sm, int nonsym, const Stats *st) {
   if (occ == 0.0f) return;
   float pref = pbc / 100.0f;
   int jc = nsam / 2 + 1;
@@ -642,7 +673,7 @@ void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, 
     }
     /* IP=1, IMP=1 (pinsert.f): the whole plane, I = -JC+1 .. JC-1, DX = DY = 0 */
     pinsert_trilin(nsam, irrec2, spec, ctff, (R)occ, b3df, t3df, ttd, shx, shy, nksum, jc, dm,
-                   tmp, nsamh, ic, -jc + 1, jc - 1, 0.0f, 0.0f);
+                   tmp, nsamh, ic, -jc + 1, jc - 1, 0.0f, 0.0f, st);
     if (nonsym) break;
     int i = 0, done = 0;
     for (;;) {
@@ -659,6 +690,24 @@ void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, 
   }
 }
 
+void P(pinsert)(int nsam, float rrec, float amag, const R *spec, const R *ctff, R *b3df, R *t3df,
+                double *ttd, float phi, float theta, float psi, float shx, float shy, float presa,
+                float pbc, float boff, int nsym, const int *jsym, const float *symop,
+                int64_t *nksum, float psize, float occ, const float *sm, int nonsym) {
+  pinsert_common(nsam, rrec, amag, spec, ctff, b3df, t3df, ttd, phi, theta, psi, shx, shy, presa,
+                 pbc, boff, nsym, jsym, symop, nksum, psize, occ, sm, nonsym, NULL);
+}
+
+void P(pinsert_se)(int nsam, float rrec, float amag, const R *spec, const R *ctff, R *b3df,
+                   R *t3df, double *ttd, float phi, float theta, float psi, float shx, float shy,
+                   float presa, float pbc, float boff, int nsym, const int *jsym,
+                   const float *symop, int64_t *nksum, float psize, float occ, const float *sm,
+                   int nonsym, const R *c3df, int *ksum, R *vsum, R *psum, R *asum) {
+  Stats st = {c3df, ksum, vsum, psum, asum};
+  pinsert_common(nsam, rrec, amag, spec, ctff, b3df, t3df, ttd, phi, theta, psi, shx, shy, presa,
+                 pbc, boff, nsym, jsym, symop, nksum, psize, occ, sm, nonsym, &st);
+}
+
 /* pinsert_trilin.f:8-244 as one call: stripe IP of width ST,
  * I1 = -JC+1+NINT((IP-1)*ST), I3 = -JC+NINT(IP*ST) (pinsert_trilin.f:44-45) */
 void P(pinsert_trilin)(int nsam, int irrec2, const R *spec, const R *ctff, float occ, R *b3df,
@@ -668,7 +717,19 @@ void P(pinsert_trilin)(int nsam, int irrec2, const R *spec, const R *ctff, float
   int i1 = -jc + 1 + (int)lroundf((float)(ip - 1) * st);
   int i3 = -jc + (int)lroundf((float)ip * st);
   pinsert_trilin(nsam, irrec2, spec, ctff, (R)occ, b3df, t3df, ttd, shx, shy, nksum, jc, dm, tmp,
-                 nsam / 2, (size_t)nsam * jc, i1, i3, dx, dy);
+                 nsam / 2, (size_t)nsam * jc, i1, i3, dx, dy, NULL);
+}
+
+void P(pinsert_trilin_se)(int nsam, int irrec2, const R *spec, const R *ctff, float occ,
+                          R *b3df, R *t3df, double *ttd, float shx, float shy, int64_t *nksum,
+                          const float *dm, float tmp, float dx, float dy, float st, int ip,
+                          const R *c3df, int *ksum, R *vsum, R *psum, R *asum) {
+  int jc = nsam / 2 + 1;
+  int i1 = -jc + 1 + (int)lroundf((float)(ip - 1) * st);
+  int i3 = -jc + (int)lroundf((float)ip * st);
+  Stats s = {c3df, ksum, vsum, psum, asum};
+  pinsert_trilin(nsam, irrec2, spec, ctff, (R)occ, b3df, t3df, ttd, shx, shy, nksum, jc, dm, tmp,
+                 nsam / 2, (size_t)nsam * jc, i1, i3, dx, dy, &s);
 }
 
 /* ------------------------------------------------------- storeshift.f */

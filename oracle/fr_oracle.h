@@ -139,6 +139,25 @@ void or_irfft2_d(int nsam, double *data);
                          float occ, R *b3df, R *t3df, double *ttd, float shx,   \
                          float shy, int64_t *nksum, const float *dm, float tmp, \
                          float dx, float dy, float st, int ip);                 \
+  /* pinsert.f:8-195 -> pinsert_trilin_se.f:8-359 (NNSTAT=1, IEWALD=0,       \
+   * IPAD=1): as pinsert, plus the per-voxel statistics of every inserted      \
+   * term BSAMP (pinsert_trilin_se.f:186-194): ksum += 1, vsum += |BSAMP|^2,   \
+   * psum += PDIFF(BSAMP, c3df), asum += |BSAMP|; c3df is the reference        \
+   * transform in the half-volume layout. */                                   \
+  void P##pinsert_se(int nsam, float rrec, float amag, const R *spec,           \
+                     const R *ctff, R *b3df, R *t3df, double *ttd, float phi,   \
+                     float theta, float psi, float shx, float shy, float presa, \
+                     float pbc, float boff, int nsym, const int *jsym,          \
+                     const float *symop, int64_t *nksum, float psize,           \
+                     float occ, const float *sm, int nonsym, const R *c3df,     \
+                     int *ksum, R *vsum, R *psum, R *asum);                     \
+  /* pinsert_trilin_se.f:8-359 as one call (IEWALD=0, NNSTAT=1, IPAD=1) */      \
+  void P##pinsert_trilin_se(int nsam, int irrec2, const R *spec, const R *ctff, \
+                            float occ, R *b3df, R *t3df, double *ttd,           \
+                            float shx, float shy, int64_t *nksum,               \
+                            const float *dm, float tmp, float dx, float dy,     \
+                            float st, int ip, const R *c3df, int *ksum,         \
+                            R *vsum, R *psum, R *asum);                         \
   /* storeshift.f:43-102 (IFSC=0, NNSTAT=0): half-maps A,D -> B,C */            \
   void P##storeshift(int nsam, double std, double vtd, int iredun,              \
                      const R *a3df, const R *s3df, const R *d3df,               \

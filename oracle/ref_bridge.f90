@@ -181,6 +181,55 @@ subroutine ref_pinsert(nsam, rrec, amag, spec, ctff, b3df, t3df, ttd, phi, theta
   ttd = ttd1(1)
 end subroutine
 
+! pinsert.f:8 as ref_pinsert, with NNSTAT=1: PINSERT dispatches to
+! PINSERT_TRILIN_SE (pinsert.f:137-176), which adds the per-voxel statistics
+! KSUM, VSUM, PSUM, ASUM against the reference transform C3DF (IPAD=1).
+subroutine ref_pinsert_se(nsam, rrec, amag, spec, ctff, b3df, t3df, ttd, phi, theta, psi, &
+                          shx, shy, presa, pbc, boff, nsym, jsym, symop, nksum, psize, occ, &
+                          sm, nonsym, c3df, ksum, vsum, psum, asum) bind(C, name="ref_pinsert_se")
+  use iso_c_binding
+  implicit none
+  integer(c_int), intent(in) :: nsam, nsym, nonsym
+  real(c_float), intent(in) :: rrec, amag, phi, theta, psi, shx, shy, presa, pbc, boff, psize, occ
+  complex(c_float_complex), intent(in) :: spec(*), ctff(*), c3df(*)
+  complex(c_float_complex), intent(inout) :: b3df(*)
+  real(c_float), intent(inout) :: t3df(*), vsum(*), psum(*), asum(*)
+  integer(c_int), intent(inout) :: ksum(*)
+  real(c_double), intent(inout) :: ttd
+  integer(c_int), intent(in) :: jsym(*)
+  real(c_float), intent(in) :: symop(3, 3, *), sm(9)
+  integer(c_int64_t), intent(inout) :: nksum
+  integer :: n, irad, irada, ipad, iewald, isymax, nnstat, imp, interp, ns(8), ns1(8), js(8)
+  real :: ri, sinclut(1), thet, smc(9), symc(3, 3, 8)
+  real :: a_rrec, a_amag, a_phi, a_theta, a_psi, a_shx, a_shy, a_presa, a_pbc, a_boff, a_psize, a_occ
+  integer(8) :: nk, nk1(1)
+  double precision :: std, ttd1(1)
+  logical :: lnonsym
+  integer :: nsy, i
+  n = nsam; irad = 1; irada = 1; ipad = 1; iewald = 0; nnstat = 1; imp = 1; interp = 1
+  ri = 0.0; thet = 0.0; sinclut(1) = 0.0
+  nsy = nsym; isymax = max(nsym, 1)
+  do i = 1, nsym
+    js(i) = jsym(i)
+    symc(:, :, i) = symop(:, :, i)
+  end do
+  smc = sm
+  a_rrec = rrec; a_amag = amag; a_phi = phi; a_theta = theta; a_psi = psi
+  a_shx = shx; a_shy = shy; a_presa = presa; a_pbc = pbc; a_boff = boff
+  a_psize = psize; a_occ = occ
+  lnonsym = (nonsym /= 0)
+  std = 0.0d0
+  nk = 0
+  nk1(1) = nksum
+  ttd1(1) = ttd
+  call pinsert(n, irad, ri, a_rrec, a_amag, spec, ctff, b3df, t3df, b3df, t3df, std, &
+               a_phi, a_theta, a_psi, a_shx, a_shy, a_presa, a_pbc, a_boff, asum, vsum, psum, &
+               c3df, ksum, sinclut, irada, ipad, thet, iewald, nsy, isymax, js, symc, nnstat, &
+               nk, imp, a_psize, nk1, ttd1, ns, ns1, a_occ, interp, smc, lnonsym)
+  nksum = nk1(1)
+  ttd = ttd1(1)
+end subroutine
+
 subroutine ref_getsymmat(asymc, nasym, symop, nsym, jsym, iredun) bind(C, name="ref_getsymmat")
   use iso_c_binding
   implicit none

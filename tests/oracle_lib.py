@@ -108,6 +108,12 @@ def _declare_oracle(L):
                                  ci, i32p, f32p, C.POINTER(C.c_int64), cf, cf, f32p, ci]
         g("pinsert_trilin").argtypes = [ci, ci, cp, cp, cf, cp, rp, pd, cf, cf,
                                         C.POINTER(C.c_int64), f32p, cf, cf, cf, cf, ci]
+        g("pinsert_se").argtypes = [ci, cf, cf, cp, cp, cp, rp, pd, cf, cf, cf, cf, cf, cf, cf,
+                                    cf, ci, i32p, f32p, C.POINTER(C.c_int64), cf, cf, f32p, ci,
+                                    cp, i32p, rp, rp, rp]
+        g("pinsert_trilin_se").argtypes = [ci, ci, cp, cp, cf, cp, rp, pd, cf, cf,
+                                           C.POINTER(C.c_int64), f32p, cf, cf, cf, cf, ci, cp,
+                                           i32p, rp, rp, rp]
         g("storeshift").argtypes = [ci, cd, cd, ci, cp, rp, cp, rp, cp, cp]
         g("combine_arrays").argtypes = [ci, C.c_longlong, ci, ci, ci, cp, rp, cp, rp]
         g("shiftvol").argtypes = [ci, pd, cd, cf, ci, cp, cp, rp, rp, ci, f32p, cf, f32p, cf]
@@ -134,6 +140,7 @@ def _declare_ref(L):
     L.ref_pextract.argtypes = [pi_, c64p, c64p, pf, pf, pf, c64p, f32p]
     L.ref_pinsert.argtypes = [pi_, pf, pf, c64p, c64p, c64p, f32p, pd, pf, pf, pf, pf, pf, pf,
                               pf, pf, pi_, i32p, f32p, C.POINTER(C.c_int64), pf, pf, f32p, pi_]
+    L.ref_pinsert_se.argtypes = L.ref_pinsert.argtypes + [c64p, i32p, f32p, f32p, f32p]
     L.ref_getsymmat.argtypes = [C.c_char_p, pi_, f32p, pi_, i32p, pi_]
     L.ref_storeshift.argtypes = [pi_, pd, pd, pi_, c64p, f32p, c64p, f32p, c64p, c64p]
     L.ref_sheltest.argtypes = [pi_, pi_, c64p, c64p, f32p, f32p, pf, i32p, i32p, f64p, f32p,

@@ -260,7 +260,9 @@ def test_a3d3_percall_vs_oracle():
 
 
 def _a3d3_args(k, npart, nsam, nn1, nn3, iswitch, ibuf, datc, ctff, eul, shf, presa, occ, sig2n,
-               rrec, maps, sums, bufs, sym, nvol, ifsc, pbc=5.0, boff=40.0, psize=1.5):
+               rrec, maps, sums, bufs, sym, nvol, ifsc, pbc=5.0, boff=40.0, psize=1.5,
+               stats=None):
+    """a3d3_'s 73 arguments; stats = (ASUM, VSUM, PSUM, C3DF, KSUM) for NNSTAT = 1."""
     f, i = O.byref_f, O.byref_i
     a3df, s3df, d3df, v3df = maps
     nksum1, ttd1, ttd2 = sums
@@ -269,14 +271,16 @@ def _a3d3_args(k, npart, nsam, nn1, nn3, iswitch, ibuf, datc, ctff, eul, shf, pr
     dummy = np.zeros(16, np.float32)
     idummy = np.zeros(64, np.int32)
     nv1 = nv2 = max(1, (nvol + 1) // 2) if ifsc == 0 else nvol
+    sa = [dummy.ctypes] * 4 + [idummy.ctypes] if stats is None else [a.ctypes for a in stats]
     args = [i(2), C.byref(iswitch), i(nsam), i(1), f(0.4 * nsam), rrec.ctypes, i(k + 1),
             i(npart), f(1.0), np.ascontiguousarray(datc).ctypes,
-            np.ascontiguousarray(ctff).ctypes, dummy.ctypes, dummy.ctypes, dummy.ctypes,
-            dummy.ctypes, idummy.ctypes, f(float(eul[0])), f(float(eul[1])), f(float(eul[2])),
+            np.ascontiguousarray(ctff).ctypes] + sa + [
+            f(float(eul[0])), f(float(eul[1])), f(float(eul[2])),
             f(float(shf[0])), f(float(shf[1])), f(float(presa)), f(pbc), f(boff), a3df.ctypes,
             s3df.ctypes, C.byref(C.c_double(0.0)), d3df.ctypes, v3df.ctypes,
             C.byref(C.c_double(0.0)), i(nn1), i(1), dummy.ctypes, i(1), i(1), f(0.0), i(0),
-            i(nsym), i(8), jsym.ctypes, symop.ctypes, i(0), C.byref(C.c_int64(0)), i(ifsc), i(1),
+            i(nsym), i(8), jsym.ctypes, symop.ctypes, i(0 if stats is None else 1),
+            C.byref(C.c_int64(0)), i(ifsc), i(1),
             i(nvol), i(nv1), i(nv2), f(float(occ)), f(float(sig2n)), f(psize),
             C.c_char_p(b"D7 "), f(0.0), f(0.0), i(0), i(0), nksum1.ctypes, ttd1.ctypes,
             ttd2.ctypes, idummy.ctypes, idummy.ctypes, i(1), recbf.ctypes, recb.ctypes,
@@ -375,6 +379,122 @@ def test_a3d3_private_volumes_vs_oracle():
     for h, got in ((0, sums[1]), (1, sums[2])):
         for is_ in range(nvh):
             assert abs(got[is_] - ttd[h][is_].value) <= 1e-6 * ttd[h][is_].value
+
+
+def test_a3d3_statistics_vs_oracle():
+    """a3d3_ with NNSTAT = 1 (NVOL = 2, IFSC = 0, four particles): slot I's
+    per-voxel KSUM/VSUM/PSUM/ASUM go to column I of the (NN3+1, NVOL) arrays
+    (a3d3_s.f:70-71 -> PINSERT_TRILIN_SE) against C3DF, copied back with the
+    half-maps at K = ILAST.  KSUM exact against the oracle's PINSERT_SE, the
+    sums within 1e-5 relative L2, the half-maps as without statistics."""
+    from frealign_amd import api
+
+    L = O.oracle()
+    lib = C.CDLL(LIB)
+    nsam, nn1, nvol = 32, 36, 2   # NN1 > NSAM: the RECBF/CTFBF columns do not overlap
+    jc = nsam // 2 + 1
+    ic = nsam * jc
+    nv = nsam * nsam * jc
+    nn3 = nn1 * nn1 * (nn1 // 2 + 1)
+    r = np.random.default_rng(31)
+    npart = 4
+    datc = (r.standard_normal((npart, ic)) + 1j * r.standard_normal((npart, ic))).astype(np.complex64)
+    ctff = (0.6 + 0.2 * (r.standard_normal((npart, 2 * ic))
+                         + 1j * r.standard_normal((npart, 2 * ic)))).astype(np.complex64)
+    ref = (r.standard_normal(nv) + 1j * r.standard_normal(nv)).astype(np.complex64)
+    eul = r.uniform(-np.pi, np.pi, (npart, 3)).astype(np.float32)
+    shf = r.uniform(-0.3, 0.3, (npart, 2)).astype(np.float32)
+    occ = r.uniform(0.5, 1.0, npart).astype(np.float32)
+    sig2n = r.uniform(0.5, 2.0, npart).astype(np.float32)
+    presa = r.uniform(0.2, 0.8, npart).astype(np.float32)
+    rrec = np.array([0.0, 0.45], np.float32)
+    nsym, symop, jsym, _ = api.point_group("C", 1)
+    maps = (np.zeros(nn3, np.complex64), np.zeros(nn3, np.float32),
+            np.zeros(nn3, np.complex64), np.zeros(nn3, np.float32))
+    sums = (np.zeros(nvol, np.int64), np.zeros(1, np.float64), np.zeros(1, np.float64))
+    bufs = (np.zeros(nn1 // 2 * nn1 * nvol + ic, np.complex64), np.zeros(10 * nvol, np.float32),
+            np.full(nvol, -1, np.int32), np.zeros(nn1 * nn1 * nvol + 2 * ic, np.complex64))
+    col = nn3 + 1
+    stats = (np.zeros(col * nvol, np.float32), np.zeros(col * nvol, np.float32),
+             np.zeros(col * nvol, np.float32), ref, np.zeros(col * nvol, np.int32))
+    iswitch, ibuf = C.c_int(0), C.c_int(0)
+    slot_of = []
+    for k in range(npart):
+        print(f"A3D3 NNSTAT=1 call K={k + 1}", flush=True)
+        slot_of.append(ibuf.value)
+        lib.a3d3_(*_a3d3_args(k, npart, nsam, nn1, nn3, iswitch, ibuf, datc[k], ctff[k], eul[k],
+                              shf[k], presa[k], occ[k], sig2n[k], rrec, maps, sums, bufs,
+                              (nsym, symop, jsym), nvol, 0, stats=stats))
+    assert slot_of == [0, 1, 0, 1]
+    ks = [np.zeros(nv, np.int32) for _ in range(nvol)]
+    vs, ps, as_ = ([np.zeros(nv, np.float32) for _ in range(nvol)] for _ in range(3))
+    for k in range(npart):
+        sl = slot_of[k]
+        on = np.float32(occ[k]) / np.float32(sig2n[k])
+        L.or32_pinsert_se(nsam, float(rrec[1]), 1.0, datc[k], ctff[k], np.zeros(nv, np.complex64),
+                          np.zeros(nv, np.float32), C.byref(C.c_double(0.0)), float(eul[k, 0]),
+                          float(eul[k, 1]), float(eul[k, 2]), float(shf[k, 0]), float(shf[k, 1]),
+                          float(presa[k]), 5.0, 40.0, nsym, jsym, symop, C.byref(C.c_int64(0)),
+                          1.5, float(on), O.IDENTITY, 0, ref, ks[sl], vs[sl], ps[sl], as_[sl])
+    asum, vsum, psum, _, ksum = stats
+    for sl in range(nvol):
+        o = col * sl
+        assert np.array_equal(ksum[o:o + nv], ks[sl]) and not ksum[o + nv:o + col].any()
+        assert int(ksum[o:o + nv].sum()) == int(sums[0][sl])
+        e = [np.linalg.norm(g[o:o + nv] - w) / np.linalg.norm(w)
+             for g, w in ((vsum, vs[sl]), (psum, ps[sl]), (asum, as_[sl]))]
+        print(f"A3D3 NNSTAT slot {sl + 1}: KSUM {ks[sl].sum()} VSUM {e[0]:.1e} PSUM {e[1]:.1e} "
+              f"ASUM {e[2]:.1e}")
+        assert max(e) < 1e-5
+
+
+def test_pinsert_trilin_se_percall_vs_oracle():
+    """pinsert_trilin_se_ with NNSTAT = 1 on one stripe: B3DF/T3DF/NKSUM/TTD
+    and the statistics against the oracle's PINSERT_TRILIN_SE."""
+    L = O.oracle()
+    lib = C.CDLL(LIB)
+    f, i = O.byref_f, O.byref_i
+    nsam, imp, ip = 64, 3, 2
+    jc = nsam // 2 + 1
+    nv = nsam * nsam * jc
+    r = np.random.default_rng(37)
+    spec = (r.standard_normal(nsam * jc) + 1j * r.standard_normal(nsam * jc)).astype(np.complex64)
+    ctff = (0.5 + 0.2 * (r.standard_normal(2 * nsam * jc)
+                         + 1j * r.standard_normal(2 * nsam * jc))).astype(np.complex64)
+    ref = (r.standard_normal(nv) + 1j * r.standard_normal(nv)).astype(np.complex64)
+    dm = np.zeros(9, np.float32)
+    L.or_dm(0.4, 2.2, -1.3, 1.0, None, dm)
+    irrec2 = (nsam * 2 // 5) ** 2
+    tmp, occ, shx, shy, dx, dy = -0.002, 0.7, 0.09, -0.04, 0.0, 0.0
+    st = float(np.float32(2 * jc - 1) / np.float32(imp))
+    out = []
+    for which in ("gpu", "oracle"):
+        b3 = np.zeros(nv, np.complex64)
+        t3 = np.zeros(nv, np.float32)
+        ttd, nk = C.c_double(0.0), C.c_int64(0)
+        ksum = np.zeros(nv, np.int32)
+        vsum, psum, asum = (np.zeros(nv, np.float32) for _ in range(3))
+        if which == "gpu":
+            print("[pinsert_trilin_se_] NNSTAT=1", flush=True)
+            ns1 = np.zeros(4 * nsam, np.int32)
+            lib.pinsert_trilin_se_(i(nsam), f(1.0), i(irrec2), spec.ctypes, ctff.ctypes, f(occ),
+                                   b3.ctypes, t3.ctypes, C.byref(ttd), f(shx), f(shy),
+                                   asum.ctypes, vsum.ctypes, psum.ctypes, ref.ctypes, ksum.ctypes,
+                                   np.zeros(4, np.float32).ctypes, i(1), i(1), f(0.0), i(0), i(1),
+                                   C.byref(nk), i(jc), dm.ctypes, f(tmp), i(nsam // 2),
+                                   i(nsam * jc), f(dx), f(dy), f(st), i(ip), ns1.ctypes)
+        else:
+            L.or32_pinsert_trilin_se(nsam, irrec2, spec, ctff, occ, b3, t3, C.byref(ttd), shx, shy,
+                                     C.byref(nk), dm, tmp, dx, dy, st, ip, ref, ksum, vsum, psum,
+                                     asum)
+        out.append((b3, t3, ttd.value, nk.value, ksum, vsum, psum, asum))
+    g, o = out
+    assert g[3] == o[3] > 0 and np.array_equal(g[4], o[4])
+    e = [np.linalg.norm(a - b) / np.linalg.norm(b) for a, b in
+         ((g[0], o[0]), (g[1], o[1]), (g[5], o[5]), (g[6], o[6]), (g[7], o[7]))]
+    print(f"[pinsert_trilin_se_] NKSUM {g[3]}; rel L2 B3DF {e[0]:.1e} T3DF {e[1]:.1e} "
+          f"VSUM {e[2]:.1e} PSUM {e[3]:.1e} ASUM {e[4]:.1e}")
+    assert max(e) < 1e-5 and abs(g[2] - o[2]) <= 1e-6 * abs(o[2])
 
 
 def test_psearch_and_resident_reference_vs_oracle():

@@ -227,6 +227,114 @@ def ref_pinsert_trilin(ref_lib, nsam, irrec2, spec, ctff, occ, b3, t3, ttd, shx,
                             i(nsam // 2), i(nsam * jc), f(dx), f(dy), f(st), i(ip), ns1.ctypes)
 
 
+def _stats(nv):
+    return (np.zeros(nv, np.int32), np.zeros(nv, np.float32), np.zeros(nv, np.float32),
+            np.zeros(nv, np.float32))
+
+
+def _reference_volume(nsam, seed):
+    """A random half-volume transform, every fifth coefficient zero (PDIFF's 0 case)."""
+    r = np.random.default_rng(seed)
+    nv = nsam * nsam * (nsam // 2 + 1)
+    c = (r.normal(size=nv) + 1j * r.normal(size=nv)).astype(np.complex64)
+    c[::5] = 0
+    return c
+
+
+@pytest.mark.parametrize("nsam,imp", [(16, 1), (32, 3)])
+def test_pinsert_trilin_se_stats(ref_lib, nsam, imp):
+    """PINSERT_TRILIN_SE with NNSTAT = 1 (pinsert_trilin_se.f:186-194): per
+    stripe, the oracle is bit exact to the reference routine in B3DF, T3DF,
+    TTD, NKSUM and the per-voxel KSUM, VSUM, PSUM, ASUM; KSUM sums to NKSUM,
+    and B3DF/T3DF equal PINSERT_TRILIN's."""
+    L = O.oracle()
+    jc = nsam // 2 + 1
+    nv = nsam * nsam * jc
+    spec = half_plane(nsam, 51)
+    ctff = np.concatenate([half_plane(nsam, 52), half_plane(nsam, 53)]) * np.complex64(0.5)
+    c3 = _reference_volume(nsam, 54)
+    dm = np.zeros(9, np.float32)
+    L.or_dm(-1.1, 0.8, 2.6, 1.0, None, dm)
+    irrec2 = (nsam * 2 // 5) ** 2
+    tmp, occ, shx, shy, dx, dy = -0.003, 0.9, -0.05, 0.13, 0.01, -0.02
+    st = np.float32(2 * jc - 1) / np.float32(imp)
+    f, i = O.byref_f, O.byref_i
+    for ip in range(1, imp + 1):
+        bo, br, bp = (np.zeros(nv, np.complex64) for _ in range(3))
+        to, tr, tp = (np.zeros(nv, np.float32) for _ in range(3))
+        tdo, tdr, tdp = C.c_double(0.0), C.c_double(0.0), C.c_double(0.0)
+        nko, nkr, nkp = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        so, sr = _stats(nv), _stats(nv)
+        L.or32_pinsert_trilin_se(nsam, irrec2, spec, ctff, occ, bo, to, C.byref(tdo), shx, shy,
+                                 C.byref(nko), dm, tmp, dx, dy, float(st), ip, c3, *so)
+        ns1 = np.zeros(4 * nsam, np.int32)
+        ref_lib.pinsert_trilin_se_(i(nsam), f(1.0), i(irrec2), spec.ctypes, ctff.ctypes, f(occ),
+                                   br.ctypes, tr.ctypes, C.byref(tdr), f(shx), f(shy),
+                                   sr[3].ctypes, sr[1].ctypes, sr[2].ctypes, c3.ctypes,
+                                   sr[0].ctypes, np.zeros(4, np.float32).ctypes, i(1), i(1),
+                                   f(0.0), i(0), i(1), C.byref(nkr), i(jc), dm.ctypes, f(tmp),
+                                   i(nsam // 2), i(nsam * jc), f(dx), f(dy), f(float(st)), i(ip),
+                                   ns1.ctypes)
+        L.or32_pinsert_trilin(nsam, irrec2, spec, ctff, occ, bp, tp, C.byref(tdp), shx, shy,
+                              C.byref(nkp), dm, tmp, dx, dy, float(st), ip)
+        assert nko.value == nkr.value == nkp.value > 0
+        assert int(so[0].sum()) == nko.value
+        assert np.array_equal(bo.view(np.int32), br.view(np.int32))
+        assert np.array_equal(to.view(np.int32), tr.view(np.int32))
+        assert np.array_equal(bo.view(np.int32), bp.view(np.int32)) and tdo.value == tdp.value
+        assert tdo.value == tdr.value
+        assert np.array_equal(so[0], sr[0])
+        for a, b in zip(so[1:], sr[1:]):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32))
+        hit = so[0] > 0
+        assert np.all(so[3][hit] > 0) and np.all(so[2] <= np.float32(np.pi) * so[0] + 1e-5)
+        assert np.any(so[2][hit] > 0) and np.any(hit[::5])
+        assert np.all(so[2][::5] == 0)
+
+
+@pytest.mark.parametrize("nsam,asym,nasym", [(16, "C", 1), (32, "D", 7)])
+def test_pinsert_se(ref_lib, nsam, asym, nasym):
+    """PINSERT with NNSTAT = 1 over every symmetry copy (pinsert.f:137-176 ->
+    PINSERT_TRILIN_SE): the oracle's volumes, counters and statistics are bit
+    exact to the reference's."""
+    L = O.oracle()
+    nsym, symop, jsym, _ = O.getsymmat(asym, nasym)
+    jc = nsam // 2 + 1
+    nv = nsam * nsam * jc
+    spec = half_plane(nsam, 61)
+    ctff = np.concatenate([half_plane(nsam, 62), half_plane(nsam, 63)]) * np.complex64(0.5)
+    c3 = _reference_volume(nsam, 64)
+    presa, pbc, boff, psize, occ, rrec = 0.6, 5.0, 40.0, 1.5, 0.8, 0.45
+    f = O.byref_f
+    for trial in range(2):
+        phi, theta, psi = (float(v) for v in rng.uniform(-np.pi, np.pi, 3))
+        shx, shy = (float(v) for v in rng.uniform(-0.3, 0.3, 2))
+        out = []
+        for which in ("oracle", "ref"):
+            b3 = np.zeros(nv, np.complex64)
+            t3 = np.zeros(nv, np.float32)
+            ttd, nk = C.c_double(0.0), C.c_int64(0)
+            s = _stats(nv)
+            if which == "oracle":
+                L.or32_pinsert_se(nsam, rrec, 1.0, spec, ctff, b3, t3, C.byref(ttd), phi, theta,
+                                  psi, shx, shy, presa, pbc, boff, nsym, jsym, symop,
+                                  C.byref(nk), psize, occ, O.IDENTITY, 0, c3, *s)
+            else:
+                ref_lib.ref_pinsert_se(O.byref_i(nsam), f(rrec), f(1.0), spec, ctff, b3, t3,
+                                       C.byref(ttd), f(phi), f(theta), f(psi), f(shx), f(shy),
+                                       f(presa), f(pbc), f(boff), O.byref_i(nsym), jsym, symop,
+                                       C.byref(nk), f(psize), f(occ), O.IDENTITY, O.byref_i(0),
+                                       c3, *s)
+            out.append((b3, t3, ttd.value, nk.value, s))
+        (bo, to, tdo, nko, so), (br, tr, tdr, nkr, sr) = out
+        assert nko == nkr == int(so[0].sum())
+        assert np.array_equal(bo.view(np.int32), br.view(np.int32))
+        assert np.array_equal(to.view(np.int32), tr.view(np.int32)) and tdo == tdr
+        assert np.array_equal(so[0], sr[0])
+        for a, b in zip(so[1:], sr[1:]):
+            assert np.array_equal(a.view(np.int32), b.view(np.int32))
+
+
 @pytest.mark.parametrize("nsam,imp", [(16, 1), (32, 3), (64, 4)])
 def test_pinsert_trilin_stripes(ref_lib, nsam, imp):
     """One PINSERT_TRILIN call per stripe IP = 1..IMP (ST = (2JC-1)/IMP, as
