@@ -5,8 +5,11 @@
 // fully connected layer to K orientation-class logits.  Activations are
 // channel-major, [C][B][H][W], so every convolution is one GEMM over the
 // batch: out[C_out][B*P] = W[C_out][C_in*9] x cols[C_in*9][B*P], with the
-// im2col operand gathered inside the GEMM on the way forward and made
-// explicit (im2col / col2im) for the gradients.
+// im2col operand gathered inside the GEMM -- on the way forward, for the
+// weight gradients (positions as the reduction axis, split over many
+// workgroups) and for the input gradients (the transposed convolution over
+// K = C_out*9, the ReLU derivative applied in the epilogue): no column
+// matrix is ever written.
 // The GEMM runs on the matrix cores (v_mfma_f32_16x16x4f32).  Weights are
 // held as BF16-rounded values (wq) with an FP32 master copy for Adam.
 //
@@ -18,6 +21,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "fr_cnn.h"
@@ -48,15 +52,33 @@ struct GemmArgs {
   const float* bias;
   int relu;
   int accumulate;
-  // implicit convolution (k_gemm_t<true>): B[k][n] = in[ci][b][2oy+ky-1][2ox+kx-1]
-  // for k = ci*9 + ky*3 + kx, n = b*Ho*Ho + oy*Ho + ox, zero outside the image
-  int conv_B, conv_H;
+  // implicit convolution operands (B is then the activation or gradient
+  // tensor, [C][batch][H][H], never an explicit column matrix):
+  //   kGemmConv  (forward)        B[k][n] = in[ci][b][2oy+ky-1][2ox+kx-1],
+  //              k = ci*9 + ky*3 + kx, n = b*Ho*Ho + oy*Ho + ox;
+  //   kGemmConvW (weight grad.)   B[k][n] = the same element with the roles
+  //              swapped: k = b*Ho*Ho + oy*Ho + ox, n = ci*9 + ky*3 + kx;
+  //   kGemmConvX (input grad.)    one parity class (py, px) = (conv_cls >> 1,
+  //              conv_cls & 1) of input positions y = 2t+py, x = 2u+px,
+  //              n = b*Ho*Ho + t*Ho + u; its taps are the ky with
+  //              y+1-ky even (ky = 1 for py = 0, ky = 0, 2 for py = 1), kx
+  //              likewise, ntap = |ky set| |kx set|, k = co*ntap + j, and
+  //              B[k][n] = g[co][b][(y+1-ky)/2][(x+1-kx)/2];
+  // zero outside the image.  conv_B is the batch, conv_H the input size H.
+  int conv_B, conv_H, conv_cls;
+  // epilogue (kGemmConvX): C element (ci, n) goes to gin[ci][b][y][x]
+  // (row stride scm), zeroed unless mask at the same place is > 0 (the ReLU
+  // derivative of the layer below)
+  const float* mask;
 };
+
+enum { kGemmPlain = 0, kGemmConv = 1, kGemmConvW = 2, kGemmConvX = 3 };
 
 constexpr int kGP = 80;  // LDS pitch: 16*g + lane -> distinct banks
 
-template <bool kConv>
+template <int kMode>
 __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
+  constexpr bool kConv = kMode == kGemmConv;
   // two LDS stages: the next K tile is fetched into registers while the
   // matrix cores work on the current one (one barrier per K tile)
   __shared__ float As[2][16][kGP];
@@ -97,15 +119,44 @@ __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
   // implicit convolution: this thread's output positions, fixed over K
   long long cb[4];
   int cy[4], cx[4];
-  if (kConv) {
+  if (kConv || kMode == kGemmConvX) {
+    // forward: n -> output position (b, oy, ox), taps add (ky-1, kx-1) to
+    // (2oy, 2ox); input gradient: n -> input position (b, y, x)
     const int H = g.conv_H, Ho = H / 2, P = Ho * Ho;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int n = n0 + bn[q];
-      const int b = n / P, pp = n - b * P, oy = pp / Ho, ox = pp - oy * Ho;
-      cb[q] = (long long)b * H * H;
-      cy[q] = 2 * oy - 1;
-      cx[q] = 2 * ox - 1;
+      if (kConv) {
+        const int b = n / P, pp = n - b * P, oy = pp / Ho, ox = pp - oy * Ho;
+        cb[q] = (long long)b * H * H;
+        cy[q] = 2 * oy - 1;
+        cx[q] = 2 * ox - 1;
+      } else {
+        const int b = n / P, pp = n - b * P, t = pp / Ho, u = pp - t * Ho;
+        cb[q] = (long long)b * P;
+        cy[q] = 2 * t + (g.conv_cls >> 1) + 1;  // y + 1 = 2oy + ky
+        cx[q] = 2 * u + (g.conv_cls & 1) + 1;
+      }
+    }
+  }
+  // weight gradient: this thread's K position (b, oy, ox) walks with the K
+  // tiles (bk fixed per thread, +16 per tile); its four n are fixed taps
+  int wb = 0, woy = 0, wox = 0;
+  int wci[4], wky[4], wkx[4];
+  if (kMode == kGemmConvW) {
+    const int Ho = g.conv_H / 2, P = Ho * Ho;
+    const int k = kbeg + bk[0];
+    wb = k / P;
+    const int pp = k - wb * P;
+    woy = pp / Ho;
+    wox = pp - woy * Ho;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int n = n0 + bn[q];
+      wci[q] = n / 9;
+      const int kk = n - 9 * wci[q];
+      wky[q] = kk / 3;
+      wkx[q] = kk - 3 * wky[q];
     }
   }
   auto fetch = [&](int k0) {
@@ -114,7 +165,29 @@ __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
       const int m = m0 + am[q], k = k0 + ak[q];
       ra[q] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
       const int n = n0 + bn[q], k2 = k0 + bk[q];
-      if (kConv) {
+      if (kMode == kGemmConvW) {
+        // the K position (wb, woy, wox) is this thread's, shared by its four n
+        float v = 0.f;
+        const int H = g.conv_H;
+        if (n < g.N && k2 < kend) {
+          const int y = 2 * woy + wky[q] - 1, x = 2 * wox + wkx[q] - 1;
+          if (y >= 0 && y < H && x >= 0 && x < H)
+            v = g.B[((long long)wci[q] * g.conv_B + wb) * H * H + (long long)y * H + x];
+        }
+        rb[q] = v;
+      } else if (kMode == kGemmConvX) {
+        float v = 0.f;
+        if (n < g.N && k2 < kend) {
+          const int py = g.conv_cls >> 1, px = g.conv_cls & 1;
+          const int nty = py + 1, ntx = px + 1, ntap = nty * ntx;
+          const int co = k2 / ntap, j = k2 - ntap * co, jy = j / ntx, jx = j - jy * ntx;
+          const int ky = py ? 2 * jy : 1, kx = px ? 2 * jx : 1;
+          const int oy = (cy[q] - ky) >> 1, ox = (cx[q] - kx) >> 1, Ho = g.conv_H / 2;
+          if (oy < Ho && ox < Ho)
+            v = g.B[(long long)co * g.conv_B * Ho * Ho + cb[q] + (long long)oy * Ho + ox];
+        }
+        rb[q] = v;
+      } else if (kConv) {
         float v = 0.f;
         if (n < g.N && k2 < kend) {
           const int ci = k2 / 9, kk = k2 - 9 * ci, ky = kk / 3, kx = kk - 3 * ky;
@@ -125,6 +198,17 @@ __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
         rb[q] = v;
       } else {
         rb[q] = (n < g.N && k2 < kend) ? g.B[k2 * g.sbk + n * g.sbn] : 0.f;
+      }
+    }
+    if (kMode == kGemmConvW) {  // the next K tile: 16 positions on
+      const int Ho = g.conv_H / 2;
+      wox += 16;
+      while (wox >= Ho) {
+        wox -= Ho;
+        if (++woy == Ho) {
+          woy = 0;
+          wb++;
+        }
       }
     }
   };
@@ -176,11 +260,37 @@ __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
           float v = acc[i][j][r];
           if (g.bias) v += g.bias[m];
           if (g.relu) v = fmaxf(v, 0.f);
-          float* p = C + m * g.scm + n * g.scn;
+          long long at = m * g.scm + n * g.scn;
+          if (kMode == kGemmConvX) {
+            const int H = g.conv_H, Ho = H / 2, P = Ho * Ho;
+            const int b = n / P, pp = n - b * P, t = pp / Ho, u = pp - t * Ho;
+            at = m * g.scm + (long long)b * H * H + (long long)(2 * t + (g.conv_cls >> 1)) * H +
+                 (2 * u + (g.conv_cls & 1));
+            if (!(g.mask[at] > 0.f)) v = 0.f;
+          }
+          float* p = C + at;
           if (g.accumulate) v += *p;
           *p = v;
         }
       }
+}
+
+// out[i] = sum over s of part[s][i] for many splits and few outputs: one
+// block per output, thread t summing splits t, t+256, ... in order, then a
+// fixed tree (deterministic)
+__global__ void __launch_bounds__(kT)
+    k_split_sum_wide(long long n, int nsplit, const float* __restrict__ part, float* out) {
+  __shared__ float red[kT];
+  const long long i = blockIdx.x;
+  float s = 0.f;
+  for (int q = threadIdx.x; q < nsplit; q += kT) s += part[q * n + i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kT / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i] = red[0];
 }
 
 // out[i] = sum over s of part[s][i], in split order (deterministic)
@@ -191,55 +301,6 @@ __global__ void __launch_bounds__(kT)
   float s = 0.f;
   for (int q = 0; q < nsplit; q++) s += part[q * n + i];
   out[i] = s;
-}
-
-// cols[(ci*9 + ky*3 + kx)][b*P + oy*Wo + ox] = in[ci][b][2oy+ky-1][2ox+kx-1]
-__global__ void __launch_bounds__(kT)
-    k_im2col(int cin, int B, int H, const float* __restrict__ in, float* cols) {
-  const int Ho = H / 2, P = Ho * Ho;
-  const long long NB = (long long)B * P;
-  const long long total = (long long)cin * 9 * NB;
-  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
-  if (idx >= total) return;
-  const long long n = idx % NB;
-  const int r = (int)(idx / NB);
-  const int ci = r / 9, kk = r % 9, ky = kk / 3, kx = kk % 3;
-  const int b = (int)(n / P), p = (int)(n % P), oy = p / Ho, ox = p % Ho;
-  const int y = 2 * oy + ky - 1, x = 2 * ox + kx - 1;
-  float v = 0.f;
-  if (y >= 0 && y < H && x >= 0 && x < H) v = in[(((long long)ci * B + b) * H + y) * H + x];
-  cols[idx] = v;
-}
-
-// gin[ci][b][y][x] = (act[ci][b][y][x] > 0) * sum of the dcols entries that
-// im2col took from (y, x)
-__global__ void __launch_bounds__(kT)
-    k_col2im_relu(int cin, int B, int H, const float* __restrict__ dcols,
-                  const float* __restrict__ act, float* gin) {
-  const int Ho = H / 2, P = Ho * Ho;
-  const long long NB = (long long)B * P;
-  const long long total = (long long)cin * B * H * H;
-  const long long idx = blockIdx.x * (long long)kT + threadIdx.x;
-  if (idx >= total) return;
-  const int x = (int)(idx % H);
-  const int y = (int)((idx / H) % H);
-  const int b = (int)((idx / ((long long)H * H)) % B);
-  const int ci = (int)(idx / ((long long)H * H * B));
-  float s = 0.f;
-  if (act[idx] > 0.f) {
-    for (int ky = 0; ky < 3; ky++) {
-      const int ty = y + 1 - ky;
-      if (ty < 0 || (ty & 1) || ty / 2 >= Ho) continue;
-      const int oy = ty / 2;
-      for (int kx = 0; kx < 3; kx++) {
-        const int tx = x + 1 - kx;
-        if (tx < 0 || (tx & 1) || tx / 2 >= Ho) continue;
-        const int ox = tx / 2;
-        s += dcols[(long long)(ci * 9 + ky * 3 + kx) * NB + (long long)b * P + oy * Ho + ox];
-      }
-    }
-  }
-  gin[idx] = s;
 }
 
 // X[c*P + p][b] = act[c][b*P + p]
@@ -701,10 +762,10 @@ int gemm(Ctx& c, int M, int N, int K, const float* A, long long sam, long long s
          long long sbk, long long sbn, float* C, long long scm, long long scn,
          const float* bias = nullptr, bool relu = false, bool accumulate = false) {
   GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, ((K + 15) / 16) * 16, 0, bias,
-             relu ? 1 : 0, accumulate ? 1 : 0, 0, 0};
+             relu ? 1 : 0, accumulate ? 1 : 0, 0, 0, 0, nullptr};
   dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
   FR_CHECK(grid.y < 65536u, "GEMM too tall");
-  hipLaunchKernelGGL(k_gemm_t<false>, grid, dim3(kT), 0, c.stream, g);
+  hipLaunchKernelGGL(k_gemm_t<kGemmPlain>, grid, dim3(kT), 0, c.stream, g);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -717,38 +778,104 @@ int conv_forward(Ctx& c, int C, int cin, int B, int H, const float* W, const flo
   const long long NB = (long long)B * Ho * Ho;
   FR_CHECK(NB < (1ll << 31), "batch too large");
   const int K = cin * 9, N = (int)NB;
-  GemmArgs g{C, N, K, W, K, 1, in, NB, 1, out, NB, 1, ((K + 15) / 16) * 16, 0, bias, 1, 0, B, H};
+  GemmArgs g{C, N, K, W, K, 1, in, NB, 1, out, NB, 1, ((K + 15) / 16) * 16, 0, bias, 1, 0, B, H,
+             0, nullptr};
   dim3 grid((N + 63) / 64, (C + 63) / 64, 1);
-  hipLaunchKernelGGL(k_gemm_t<true>, grid, dim3(kT), 0, c.stream, g);
+  hipLaunchKernelGGL(k_gemm_t<kGemmConv>, grid, dim3(kT), 0, c.stream, g);
   FR_HIP(hipGetLastError());
   return 0;
 }
 
-// C[M][N] (row-major, dense) = A B with K split over `nsplit` partial
-// products in `part` (nsplit * M * N floats), summed in order
-int gemm_splitk(Ctx& c, int M, int N, int K, const float* A, long long sam, long long sak,
-                const float* B, long long sbk, long long sbn, float* C, float* part,
-                int nsplit) {
-  int kchunk = ((K + nsplit - 1) / nsplit + 15) / 16 * 16;
-  nsplit = (K + kchunk - 1) / kchunk;
-  GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, part, N, 1, kchunk, (long long)M * N, nullptr,
-             0, 0, 0, 0};
-  dim3 grid((N + 63) / 64, (M + 63) / 64, nsplit);
-  hipLaunchKernelGGL(k_gemm_t<false>, grid, dim3(kT), 0, c.stream, g);
+// split-K partial products of the weight gradients: at most this many floats
+constexpr long long kPartFloats = 16ll << 20;
+// workgroups the split-K weight gradient aims for (the GPU's 256 CUs, 8 each)
+constexpr int kWgradWorkgroups = 2048;
+
+// gW[C][cin*9] = sum over the B*Ho*Ho output positions n of
+// g[C][n] x in[ci][b][2oy+ky-1][2ox+kx-1] (the im2col operand gathered in the
+// GEMM), with the positions split over enough workgroups to fill the GPU;
+// the splits' partial products (in `part`) are summed in split order
+int conv_wgrad(Ctx& c, int C, int cin, int B, int H, const float* g_, const float* in,
+               float* gw, float* part) {
+  const int Ho = H / 2, N = cin * 9;
+  const long long NB = (long long)B * Ho * Ho;
+  const int tiles = ((N + 63) / 64) * ((C + 63) / 64);
+  long long nsplit = (kWgradWorkgroups + tiles - 1) / tiles;
+  nsplit = std::min(nsplit, std::max(1ll, NB / 256));
+  nsplit = std::min(nsplit, std::max(1ll, kPartFloats / ((long long)C * N)));
+  int kchunk = (int)(((NB + nsplit - 1) / nsplit + 15) / 16 * 16);
+  nsplit = (NB + kchunk - 1) / kchunk;
+  FR_CHECK(nsplit < 65536 && nsplit * C * N <= kPartFloats, "weight-gradient split too large");
+  GemmArgs g{C, N, (int)NB, g_, NB, 1, in, 1, NB, part, N, 1, kchunk, (long long)C * N,
+             nullptr, 0, 0, B, H, 0, nullptr};
+  dim3 grid((N + 63) / 64, (C + 63) / 64, (unsigned)nsplit);
+  hipLaunchKernelGGL(k_gemm_t<kGemmConvW>, grid, dim3(kT), 0, c.stream, g);
   FR_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_split_sum, dim3(nblk((long long)M * N)), dim3(kT), 0, c.stream,
-                     (long long)M * N, nsplit, part, C);
+  if ((long long)C * N < 16384 && nsplit > 64)
+    hipLaunchKernelGGL(k_split_sum_wide, dim3((unsigned)(C * N)), dim3(kT), 0, c.stream,
+                       (long long)C * N, (int)nsplit, part, gw);
+  else
+    hipLaunchKernelGGL(k_split_sum, dim3(nblk((long long)C * N)), dim3(kT), 0, c.stream,
+                       (long long)C * N, (int)nsplit, part, gw);
   FR_HIP(hipGetLastError());
+  return 0;
+}
+
+// the layer's weights ordered for the input gradient's GEMMs, one block per
+// parity class (py, px) (classes 0..3 at offsets 0, 1, 3, 5 times C*cin):
+// wt_cls[ci][co*ntap + j] = w[co][ci*9 + ky*3 + kx] for the class's j-th tap
+__global__ void __launch_bounds__(kT)
+    k_wgrad_order(int C, int cin, const float* __restrict__ w, float* wt) {
+  const int idx = blockIdx.x * kT + threadIdx.x;
+  if (idx >= C * cin * 9) return;
+  // idx -> class and element: classes hold 1, 2, 2, 4 taps
+  const int per = C * cin;
+  int cls, base;
+  if (idx < per) { cls = 0; base = 0; }
+  else if (idx < 3 * per) { cls = 1; base = per; }
+  else if (idx < 5 * per) { cls = 2; base = 3 * per; }
+  else { cls = 3; base = 5 * per; }
+  const int py = cls >> 1, px = cls & 1, ntx = px + 1, ntap = (py + 1) * ntx;
+  const int e = idx - base;
+  const int ci = e / (C * ntap), r = e - ci * C * ntap, co = r / ntap, j = r - co * ntap;
+  const int jy = j / ntx, jx = j - jy * ntx;
+  const int ky = py ? 2 * jy : 1, kx = px ? 2 * jx : 1;
+  wt[idx] = w[(co * cin + ci) * 9 + ky * 3 + kx];
+}
+
+// gin[ci][b][y][x] = (act[ci][b][y][x] > 0) x sum over co, ky, kx of
+// w[co][ci][ky][kx] g[co][b][oy][ox] with 2oy+ky-1 = y, 2ox+kx-1 = x: the
+// transposed convolution as four GEMMs, one per parity class of (y, x), each
+// over only the taps that reach it (K = C x 1, 2, 2, 4), with the gradient
+// gathered in place (no column matrix) and the ReLU derivative applied on
+// the way out
+int conv_xgrad(Ctx& c, int C, int cin, int B, int H, const float* w, const float* g_,
+               const float* act, float* wt, float* gin) {
+  const int Ho = H / 2;
+  const long long N = (long long)B * Ho * Ho;
+  FR_CHECK((long long)B * H * H < (1ll << 31), "batch too large");
+  hipLaunchKernelGGL(k_wgrad_order, dim3(nblk((long long)C * cin * 9)), dim3(kT), 0, c.stream, C,
+                     cin, w, wt);
+  FR_HIP(hipGetLastError());
+  static const int off[4] = {0, 1, 3, 5};
+  for (int cls = 0; cls < 4; cls++) {
+    const int ntap = ((cls >> 1) + 1) * ((cls & 1) + 1), K = C * ntap;
+    GemmArgs g{cin, (int)N, K, wt + (size_t)off[cls] * C * cin, K, 1, g_, 0, 1, gin,
+               (long long)B * H * H, 1, ((K + 15) / 16) * 16, 0, nullptr, 0, 0, B, H, cls, act};
+    dim3 grid((unsigned)((N + 63) / 64), (cin + 63) / 64, 1);
+    hipLaunchKernelGGL(k_gemm_t<kGemmConvX>, grid, dim3(kT), 0, c.stream, g);
+    FR_HIP(hipGetLastError());
+  }
   return 0;
 }
 
 // workspace offsets (floats) for a batch of B
 struct Ws {
   size_t act[8];  // act[l]: output of convolution l, [C][B][S>>(l+1)]^2
-  size_t X, logits, cols, dcols, gA, gB, gX, gL, part, total;
+  size_t X, logits, wt, gA, gB, gX, gL, part, total;
 };
 
-constexpr int kMaxSplit = 64;
+constexpr int kMaxSplit = 64;  // splits of the bias-gradient row sums (x9)
 
 Ws ws_layout(const CnnLayout& lo, int B, bool backward) {
   Ws w{};
@@ -761,20 +888,16 @@ Ws ws_layout(const CnnLayout& lo, int B, bool backward) {
   o += (size_t)lo.F() * B;
   w.logits = o;
   o += (size_t)lo.K * B;
-  size_t cols = 0, gmax = 0;
+  size_t gmax = 0;
   for (int l = 0; l < lo.L; l++) {
-    size_t cl = (size_t)lo.cin(l) * 9 * B * lo.hout(l) * lo.hout(l);
-    cols = cl > cols ? cl : cols;
     size_t ga = (size_t)lo.C * B * lo.hout(l) * lo.hout(l);
     gmax = ga > gmax ? ga : gmax;
   }
-  // the forward gathers its convolution operands in place; only the
-  // backward's weight and input gradients use the explicit im2col buffers
+  // every convolution GEMM, forward and backward, gathers its operand from
+  // the activation or gradient tensor in place: no column matrices
   if (backward) {
-    w.cols = o;
-    o += cols;
-    w.dcols = o;
-    o += cols;
+    w.wt = o;
+    o += (size_t)lo.C * lo.C * 9;
     w.gA = o;
     o += gmax;
     w.gB = o;
@@ -784,9 +907,9 @@ Ws ws_layout(const CnnLayout& lo, int B, bool backward) {
     w.gL = o;
     o += (size_t)lo.K * B;
     w.part = o;
-    size_t pm = (size_t)lo.K * lo.F();
-    size_t pc = (size_t)lo.C * lo.C * 9;
-    o += (size_t)kMaxSplit * (pm > pc ? pm : pc);
+    // split-K partials of the weight gradients, or the bias row sums' partials
+    size_t pc = (size_t)lo.C * 9 * kMaxSplit;
+    o += (size_t)kPartFloats > pc ? (size_t)kPartFloats : pc;
   }
   w.total = o;
   return w;
@@ -913,14 +1036,8 @@ int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
     const int H = lo.hin(l), Ho = lo.hout(l), cin = lo.cin(l);
     const long long NB = (long long)B * Ho * Ho;
     const float* in = l == 0 ? img : ws + w.act[l - 1];
-    hipLaunchKernelGGL(k_im2col, dim3(nblk((long long)cin * 9 * NB)), dim3(kT), 0, c.stream, cin,
-                       B, H, in, ws + w.cols);
-    FR_HIP(hipGetLastError());
-    // gW[C][cin*9] = sum_n g[C][n] cols[cin*9][n]
-    int nsplit = (int)((NB + 4095) / 4096);
-    nsplit = nsplit < 1 ? 1 : (nsplit > kMaxSplit ? kMaxSplit : nsplit);
-    if (int r = gemm_splitk(c, lo.C, cin * 9, (int)NB, gcur, NB, 1, ws + w.cols, 1, NB,
-                            grad + lo.w_off[l], ws + w.part, nsplit))
+    // gW[C][cin*9] = sum_n g[C][n] im2col(in)[cin*9][n]
+    if (int r = conv_wgrad(c, lo.C, cin, B, H, gcur, in, grad + lo.w_off[l], ws + w.part))
       return r;
     // gb[C] = sum_n g[C][n]: per-split partials in ws.part (free again once
     // the split-K sum above has run), then the splits in order
@@ -934,14 +1051,10 @@ int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
                        ws + w.part, grad + lo.b_off[l]);
     FR_HIP(hipGetLastError());
     if (l > 0) {
-      // dcols[cin*9][n] = sum_co W[co][k] g[co][n]
-      if (int r = gemm(c, cin * 9, (int)NB, lo.C, wq + lo.w_off[l], 1, cin * 9, gcur, NB, 1,
-                       ws + w.dcols, NB, 1))
+      // the input gradient through the layer below's ReLU
+      if (int r = conv_xgrad(c, lo.C, cin, B, H, wq + lo.w_off[l], gcur, ws + w.act[l - 1],
+                             ws + w.wt, gnext))
         return r;
-      const long long ng = (long long)cin * B * H * H;
-      hipLaunchKernelGGL(k_col2im_relu, dim3(nblk(ng)), dim3(kT), 0, c.stream, cin, B, H,
-                         ws + w.dcols, ws + w.act[l - 1], gnext);
-      FR_HIP(hipGetLastError());
       float* t = gcur;
       gcur = gnext;
       gnext = t;
