@@ -51,8 +51,8 @@ SIGNATURES = {
                         ci),
     "fr_search_list_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, ci, vp, ci, ci, vp, vp,
                               vp], ci),
-    "fr_search_ragged_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, vp, vp, vp, ci, ci, vp,
-                                vp, vp], ci),
+    "fr_search_ragged_batch": ([vp, ci, ci, ci, cf, ci, vp, ci, ci, vp, vp, vp, vp, vp, vp, ci, ci,
+                                vp, vp, vp], ci),
     "fr_build_lists": ([vp, ci, ci, vp, vp, cf, ci, vp, vp, ci, ci, ci, ci, vp, vp, vp, vp], ci),
     "fr_rescore_batch": ([vp, ci, ci, ci, vp, vp, vp, vp, vp, ci, vp, ci, ci, ci, cf, vp, vp, vp,
                           vp], ci),

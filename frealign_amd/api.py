@@ -407,10 +407,12 @@ class Context:
 
     def search_ragged(self, nsam: int, maxr1: int, maxr2: int, rbfact: float,
                       bank: torch.Tensor, outc, ctff, loff: torch.Tensor, grp: torch.Tensor,
-                      cand: torch.Tensor, topk: int = 32):
+                      cand: torch.Tensor, topk: int = 32, info: torch.Tensor | None = None):
         """fr_search_ragged_batch: particle p against bank rows
         cand[loff[p]:loff[p+1]] in view groups of grp[p]; returns (bank
-        orientation, shift row, score) of the topk best groups."""
+        orientation, shift row, score) of the topk best groups.  `info`
+        (from build_lists) lets the particles that fall back to the whole
+        bank share its templates."""
         _dev(outc, torch.complex64, "outc")
         _dev(ctff, torch.complex64, "ctff")
         for t, nm in ((loff, "loff"), (grp, "grp"), (cand, "cand")):
@@ -430,8 +432,8 @@ class Context:
         N.check(N.lib().fr_search_ragged_batch(
             self.h, nsam, maxr1, maxr2, float(rbfact), bank.shape[0], _p(bank),
             code, np_, _p(outc), _p(ctff), _p(loff), _p(grp),
-            _p(cand), cand.numel(), topk, _p(top_o), _p(top_r), _p(top_s)),
-            "fr_search_ragged_batch")
+            _p(cand), None if info is None else _p(info), cand.numel(), topk, _p(top_o),
+            _p(top_r), _p(top_s)), "fr_search_ragged_batch")
         return top_o, top_r, top_s
 
     def rescore(self, nsam: int, a3df, outc, ctff, top_o: torch.Tensor, top_r: torch.Tensor,

@@ -1143,8 +1143,8 @@ int fr_build_lists(fr_ctx* c, int np, int ktop, const int* d_top, const float* d
 int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfact, int norient,
                            const void* d_bank, int dtype, int np, const float* d_outc,
                            const float* d_ctff, const int* d_loff, const int* d_grp,
-                           const int* d_cand, int cap, int topk, int* d_top_o, int* d_top_row,
-                           float* d_top_score) {
+                           const int* d_cand, const int* d_info, int cap, int topk, int* d_top_o,
+                           int* d_top_row, float* d_top_score) {
   CTX_CHECK(c);
   SIZE_CHECK(nsam);
   FR_CHECK(d_bank && d_outc && d_ctff && d_loff && d_grp && d_cand && d_top_o && d_top_row &&
@@ -1170,11 +1170,18 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
     if (!brow) return err;
     int* toff = (int*)c->slot(17, sizeof(int) * ((size_t)np + 1), &err);
     if (!toff) return err;
-    if (int r = launch_tile_offsets(*c, np, d_loff, toff)) return r;
+    // with the confidence flags, the fallback particles (whole bank in order)
+    // leave the ragged tiles for the shared-template form
+    int* fb = nullptr;
+    if (d_info) {
+      fb = (int*)c->slot(34, sizeof(int) * ((size_t)np + 1), &err);
+      if (!fb) return err;
+    }
+    if (int r = launch_tile_offsets(*c, np, d_loff, toff, d_info, fb)) return r;
     int pslot = c->prof_begin(kProfSearchList);
     if (int r = launch_search_sep(*c, np, norient, c->nsamr, 0, d_loff, toff, cap,
                                   (int)(max_tiles < (1LL << 30) ? max_tiles : (1LL << 30)), d_cand,
-                                  d_bank, xa, xb, c2s, sum2, score, brow))
+                                  d_bank, xa, xb, c2s, sum2, score, brow, fb))
       return r;
     c->prof_end(pslot);
     return launch_topk(*c, norient, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,

@@ -117,13 +117,24 @@ __global__ void __launch_bounds__(kFill)
 }
 
 // GEMM tile offsets of ragged lists: ceil(len / 256) tiles per particle (a
-// decreasing offset pair counts as an empty list).
-__global__ void __launch_bounds__(kScan) k_tile_offsets(int np, const int* __restrict__ loff,
-                                                        int* toff) {
+// decreasing offset pair counts as an empty list).  With `info` (the
+// confidence flags of k_list_sizes) the particles that fall back to the
+// whole bank get no tiles here; they are listed in fb instead (fb[0] = their
+// count, fb[1..] the particles in order) for the shared-template search.
+__global__ void __launch_bounds__(kScan)
+    k_tile_offsets(int np, const int* __restrict__ loff, const int* __restrict__ info, int* toff,
+                   int* fb) {
   block_scan(np, [&](int i) {
+    if (info && !info[2 * i]) return 0;
     int n = loff[i + 1] - loff[i];
     return n > 0 ? (n + kTile - 1) / kTile : 0;
   }, toff);
+  if (info && fb && threadIdx.x == 0) {
+    int cnt = 0;
+    for (int i = 0; i < np; i++)
+      if (!info[2 * i] && loff[i + 1] > loff[i]) fb[1 + cnt++] = i;
+    fb[0] = cnt;
+  }
 }
 
 // CC3M items of the top-k candidates: pidx, rotation matrix, shift, AMAG.
@@ -186,8 +197,8 @@ int launch_list_build(Ctx& c, int np, int ktop, const int* top, const float* pro
   return 0;
 }
 
-int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff) {
-  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kScan), 0, c.stream, np, loff, toff);
+int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff, const int* info, int* fb) {
+  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kScan), 0, c.stream, np, loff, info, toff, fb);
   FR_HIP(hipGetLastError());
   return 0;
 }

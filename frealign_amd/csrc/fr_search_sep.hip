@@ -163,11 +163,28 @@ struct SepItem {
   int p, base, lo, hi;
 };
 
-template <bool kRagged>
+enum { kSepUniform = 0, kSepRagged = 1, kSepFallback = 2 };
+
+template <int kMode>
 __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
                                 const int* __restrict__ loff, const int* __restrict__ toff,
-                                int cap, SepItem& it) {
-  if (!kRagged) {
+                                int cap, const int* __restrict__ fb, SepItem& it) {
+  if (kMode == kSepFallback) {
+    // the uniform mapping over the fallback particles fb[1 .. fb[0]], whose
+    // lists are the bank in order at loff[p]
+    const int cnt = fb[0], npq = (cnt + 3) / 4;
+    const int xcd = idx & 7, j = idx >> 3;
+    const int tile = (j / npq) * 8 + xcd;
+    const int q = (j % npq) * 4 + wave;
+    if (tile >= ntile || q >= cnt) return false;
+    it.p = fb[1 + q];
+    it.base = loff[it.p];
+    const int end = loff[it.p + 1] < cap ? loff[it.p + 1] : cap;
+    it.lo = tile * kSepTile;
+    it.hi = min(min(nrow, end - it.base), it.lo + kSepTile);
+    return it.base >= 0 && it.lo < it.hi;
+  }
+  if (kMode == kSepUniform) {
     // XCD-aware: blocks are dealt round-robin to the 8 XCDs; each XCD walks
     // its template tiles with the particle quads innermost
     const int npq = (np + 3) / 4;
@@ -195,13 +212,15 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
   return it.base >= 0 && it.lo < it.hi;
 }
 
-template <bool kRagged>
+template <int kMode>
 __global__ void __launch_bounds__(kSepThreads, 2)
     k_search_sep(int np, int nbank, int nsamr, int nrow, int ntile, const int* __restrict__ loff,
                  const int* __restrict__ toff, int cap, const int* __restrict__ cand,
                  const uint4* __restrict__ bank, const uint4* __restrict__ xa,
                  const uint4* __restrict__ xb, const float4* __restrict__ c2s,
-                 const float* __restrict__ sum2, float* score, int* best_row) {
+                 const float* __restrict__ sum2, float* score, int* best_row,
+                 const int* __restrict__ fb) {
+  constexpr bool kRagged = kMode == kSepRagged;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
 
@@ -327,10 +346,15 @@ __global__ void __launch_bounds__(kSepThreads, 2)
     row = bx * 15 + by;
   };
 
-  const int nitem = kRagged ? toff[np] : (int)gridDim.x;
-  for (int idx = blockIdx.x; idx < nitem; idx += kRagged ? (int)gridDim.x : nitem) {
+  // persistent forms loop over their items; the uniform form is one item
+  // per workgroup
+  const int nitem = kRagged ? toff[np]
+                    : kMode == kSepFallback ? (ntile + 7) / 8 * 8 * ((fb[0] + 3) / 4)
+                                            : (int)gridDim.x;
+  const int stride = kMode == kSepUniform ? nitem : (int)gridDim.x;
+  for (int idx = blockIdx.x; idx < nitem; idx += stride) {
     SepItem it;
-    if (!sep_item<kRagged>(idx, wave, np, nrow, ntile, loff, toff, cap, it)) continue;
+    if (!sep_item<kMode>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it)) continue;
     if (it.p != pcur) {
       const size_t b = (size_t)it.p * 512 + lane;
 #pragma unroll
@@ -412,7 +436,7 @@ int launch_sep_feats(Ctx& c, int nsam, int np, int lo2, int hi2, float rbfact, c
 int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int* loff,
                       const int* toff, int cap, int max_tiles, const int* cand, const void* bank,
                       const void* xa, const void* xb, const float* c2s, const float* sum2,
-                      float* score, int* best_row) {
+                      float* score, int* best_row, const int* fb) {
   const uint4* bk = reinterpret_cast<const uint4*>(bank);
   const uint4* a = reinterpret_cast<const uint4*>(xa);
   const uint4* b = reinterpret_cast<const uint4*>(xb);
@@ -422,28 +446,41 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
     const size_t nblk = (size_t)((ntile + 7) / 8 * 8) * ((np + 3) / 4);
     FR_CHECK(nblk < (1u << 31), "search grid too large");
     if (nblk == 0) return 0;
-    hipLaunchKernelGGL(k_search_sep<false>, dim3((unsigned)nblk), dim3(kSepThreads), 0, c.stream,
-                       np, nbank, nsamr, nrow, ntile, nullptr, nullptr, 0, cand, bk, a, b, cc,
-                       sum2, score, best_row);
+    hipLaunchKernelGGL(k_search_sep<kSepUniform>, dim3((unsigned)nblk), dim3(kSepThreads), 0,
+                       c.stream, np, nbank, nsamr, nrow, ntile, nullptr, nullptr, 0, cand, bk, a,
+                       b, cc, sum2, score, best_row, nullptr);
     FR_HIP(hipGetLastError());
     return 0;
   }
   // persistent grid over the ragged tiles
   int per_cu = 0;
-  FR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_search_sep<true>,
-                                                      kSepThreads, 0));
+  FR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void*)k_search_sep<kSepRagged>, kSepThreads, 0));
   if (per_cu < 1) per_cu = 1;
   if (c.cus <= 0) {
     hipDeviceProp_t prop;
     FR_HIP(hipGetDeviceProperties(&prop, c.device));
     c.cus = prop.multiProcessorCount;
   }
-  int grid = c.cus * per_cu;
+  const int full = c.cus * per_cu;
+  int grid = full;
   if (max_tiles < grid) grid = max_tiles;
-  if (grid < 1) return 0;
-  hipLaunchKernelGGL(k_search_sep<true>, dim3(grid), dim3(kSepThreads), 0, c.stream, np, nbank,
-                     nsamr, 0, 0, loff, toff, cap, cand, bk, a, b, cc, sum2, score, best_row);
-  FR_HIP(hipGetLastError());
+  if (grid >= 1) {
+    hipLaunchKernelGGL(k_search_sep<kSepRagged>, dim3(grid), dim3(kSepThreads), 0, c.stream, np,
+                       nbank, nsamr, 0, 0, loff, toff, cap, cand, bk, a, b, cc, sum2, score,
+                       best_row, nullptr);
+    FR_HIP(hipGetLastError());
+  }
+  if (fb) {
+    // the fallback particles: a persistent grid (a multiple of the 8 XCDs, so
+    // that each workgroup keeps its XCD's tiles) over the uniform mapping
+    const int ntile = (nbank + kSepTile - 1) / kSepTile;
+    const int gf = full / 8 * 8 > 8 ? full / 8 * 8 : 8;
+    hipLaunchKernelGGL(k_search_sep<kSepFallback>, dim3(gf), dim3(kSepThreads), 0, c.stream, np,
+                       nbank, nsamr, nbank, ntile, loff, nullptr, cap, nullptr, bk, a, b, cc, sum2,
+                       score, best_row, fb);
+    FR_HIP(hipGetLastError());
+  }
   return 0;
 }
 

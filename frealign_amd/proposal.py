@@ -291,7 +291,7 @@ class LearnedProposal:
             top, prob, self.cfg.threshold, self.cell_off_dev, self.cells_dev, self.cfg.group,
             s.norient, s.group, self.maxlen)
         o, r, sg = self.ctx.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.search_bank,
-                                          outc, ctff, loff, grp, cand, topk)
+                                          outc, ctff, loff, grp, cand, topk, info=info)
         sc, best = self.ctx.rescore(s.nsam, s.a3df, outc, ctff, o, r, s.dm, s.nsamr, s.maxr1,
                                     s.maxr2, s.rbfact, s.fsct_device(fsct), ring)
         npart = o.shape[0]

@@ -417,12 +417,17 @@ int fr_search_list_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfa
  * is the whole bank in order with the bank's view group gives exactly
  * fr_search_batch's answer for that particle.  cap is the length of d_cand;
  * list positions at or beyond it are ignored.  One persistent GEMM launch
- * covers every list. */
+ * covers every list.  d_info (np x 2 from fr_build_lists, or NULL): with the
+ * separable BF16 bank (dtype 2), the particles flagged as falling back
+ * (d_info[2p] == 0; their lists are the whole bank in order) are searched
+ * four to a workgroup against shared templates instead of one list tile at
+ * a time, which reads the bank once per four such particles; the scores are
+ * the same. */
 int fr_search_ragged_batch(fr_ctx *ctx, int nsam, int maxr1, int maxr2, float rbfact,
                            int norient, const void *d_bank, int dtype, int np,
                            const float *d_outc, const float *d_ctff, const int *d_loff,
-                           const int *d_grp, const int *d_cand, int cap, int topk, int *d_top_o,
-                           int *d_top_row, float *d_top_score);
+                           const int *d_grp, const int *d_cand, const int *d_info, int cap,
+                           int topk, int *d_top_o, int *d_top_row, float *d_top_score);
 
 /* Candidate lists of the learned search (no reference counterpart; the
  * fallback list is PSEARCH's full view loop, psearch.f:73-102).  d_top and

@@ -113,6 +113,24 @@ def test_ragged_equals_exhaustive_and_uniform_lists(ctx, setup, kind):
         assert np.array_equal(o_r[p], ref[0]), p
         assert np.array_equal(r_r[p], ref[1]), p
         assert np.array_equal(s_r[p].view(np.uint32), ref[2].view(np.uint32)), p
+    if kind == "separable":
+        # the fallback particles flagged (info[p, 0] = 0) are searched four to
+        # a workgroup against shared templates: the same bits
+        info = np.ones((npart, 2), np.int32)
+        for p in full:
+            info[p, 0] = 0
+        o_i, r_i, s_i = [x.cpu().numpy() for x in ctx.search_ragged(
+            nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff, loff, grp, cand, topk,
+            info=torch.from_numpy(info).to(DEV))]
+        assert np.array_equal(o_i, o_r) and np.array_equal(r_i, r_r)
+        assert np.array_equal(s_i.view(np.uint32), s_r.view(np.uint32))
+        # every particle falling back (three workgroup quads)
+        lf, gf, cf_ = ragged([np.arange(gs.norient, dtype=np.int32)] * npart, [gs.group] * npart)
+        o_a, r_a, s_a = [x.cpu().numpy() for x in ctx.search_ragged(
+            nsam, gs.maxr1, gs.maxr2, gs.rbfact, bank, outc, ctff, lf, gf, cf_, topk,
+            info=torch.zeros((npart, 2), dtype=torch.int32, device=DEV))]
+        assert np.array_equal(o_a, o_f) and np.array_equal(r_a, r_f)
+        assert np.array_equal(s_a.view(np.uint32), s_f.view(np.uint32))
 
 
 @pytest.mark.parametrize("kind", ["annulus", "separable"])
