@@ -67,8 +67,8 @@ def parse():
     ap.add_argument("--cpu-views", type=int, default=12,
                     help="views of PSEARCH per CPU-baseline particle (timed, then scaled to all)")
     ap.add_argument("--exh-steps", type=int, default=4, help="timed steps of the exhaustive BF16 search")
-    ap.add_argument("--train-samples", type=int, default=51_200,
-                    help="synthetic projections the proposal network is trained on (200 Adam "
+    ap.add_argument("--train-samples", type=int, default=102_400,
+                    help="synthetic projections the proposal network is trained on (400 Adam "
                          "steps of 256): near the optimum amortised over a 20,000-particle "
                          "dataset, where more samples raise the confident fraction but cost "
                          "more than the fallbacks they save")
@@ -179,9 +179,9 @@ def main():
 
     ar = np.arange(B)
 
-    def prep(i):
+    def prep(i, c=None):
         img, pp, lo = batch(i)
-        ctff, outc, datc = ctx.ctfapply(NSAM, img.clone(), pp, pssnr, gpar)
+        ctff, outc, datc = (c or ctx).ctfapply(NSAM, img.clone(), pp, pssnr, gpar)
         return ctff, outc, datc, lo
 
     def exhaustive(gs, ctff, outc):
@@ -251,6 +251,33 @@ def main():
     ctx.profile(False)
     ppsl = B * args.steps / dtl
     log(f"learned + bf16: {ppsl:.1f} particles/s")
+
+    # the same steps on two library contexts (each its own workspace) and two
+    # streams of default priority, consecutive batches alternating: one
+    # batch's rescoring overlaps the next one's preparation and search
+    ctx2 = api.Context(0)
+    ctxs = (ctx, ctx2)
+    streams = (torch.cuda.Stream(), torch.cuda.Stream())
+    torch.cuda.synchronize()
+    for i in range(2):
+        with torch.cuda.stream(streams[i]):
+            ctff, outc, datc, lo = prep(i, ctxs[i])
+            lp.run(outc, ctff, datc, fsct, ring, TOPK, ctx=ctxs[i])
+    pending = None
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for i in range(args.warmup, args.warmup + args.steps):
+        with torch.cuda.stream(streams[i % 2]):
+            ctff, outc, datc, lo = prep(i, ctxs[i % 2])
+            nxt = lp.launch(outc, ctff, datc, fsct, ring, TOPK, ctx=ctxs[i % 2])
+        if pending is not None:
+            lp.finish(pending)
+        pending = nxt
+    lp.finish(pending)
+    torch.cuda.synchronize()
+    pps_two = B * args.steps / (time.perf_counter() - t2)
+    log(f"learned + bf16, two streams: {pps_two:.1f} particles/s")
+    del ctx2, ctxs
 
     # ------------------------------------------------------- exhaustive BF16
     step_exhaustive(gs16, 0)
@@ -460,6 +487,9 @@ def main():
                                       "SHELTEST"},
         "insertion_GBps": ins_gbps,
         "insertion_frac_of_hbm_peak": ins_gbps * 1e9 / PEAK_HBM,
+        "learned_two_streams_particles_per_s": pps_two,
+        "learned_two_streams_note": "the same learned steps on two contexts and streams, "
+                                    "consecutive batches overlapping on the GPU",
         "kernel_ms_per_step": {"search_gemm_lists": gemm_ms / max(args.steps, 1),
                                "search_gemm_fallback": fb_ms / max(args.steps, 1),
                                "cnn_forward": cnn_ms / max(args.steps, 1),

@@ -61,6 +61,7 @@ SIGNATURES = {
     "fr_cnn_forward": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),
     "fr_cnn_forward_bf16": ([vp, ci, ci, ci, ci, vp, ci, vp, vp], ci),
     "fr_cnn_gradients": ([vp, ci, ci, ci, ci, vp, ci, vp, vp, vp, vp], ci),
+    "fr_cnn_gradients_bf16": ([vp, ci, ci, ci, ci, vp, ci, vp, vp, vp, vp], ci),
     "fr_cnn_adam": ([vp, ci, ci, ci, ci, vp, vp, vp, vp, vp, ci, cf], ci),
     "fr_cnn_propose": ([vp, ci, ci, vp, ci, vp, vp], ci),
     "fr_cnn_images": ([vp, ci, ci, ci, vp, vp, vp], ci),

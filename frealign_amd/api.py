@@ -665,7 +665,10 @@ class Context:
         return out
 
     def cnn_gradients(self, shape, wq: torch.Tensor, img: torch.Tensor, label: torch.Tensor,
-                      grad: torch.Tensor) -> torch.Tensor:
+                      grad: torch.Tensor, bf16: bool = False) -> torch.Tensor:
+        """Per-image losses and the gradient of their mean (fr_cnn_gradients);
+        with `bf16` the mixed-precision step (fr_cnn_gradients_bf16: BF16 GEMM
+        operands, FP32 accumulation)."""
         _dev(wq, torch.float32, "wq")
         _dev(img, torch.float32, "img")
         _dev(label, torch.int32, "label")
@@ -676,8 +679,9 @@ class Context:
             raise ValueError("one label in 0..K-1 per image")
         loss = torch.empty(nb, dtype=torch.float32, device=img.device)
         self._sync_stream()
-        N.check(N.lib().fr_cnn_gradients(self.h, *shape, _p(wq), nb, _p(img), _p(label),
-                                         _p(loss), _p(grad)), "fr_cnn_gradients")
+        fn = "fr_cnn_gradients_bf16" if bf16 else "fr_cnn_gradients"
+        N.check(getattr(N.lib(), fn)(self.h, *shape, _p(wq), nb, _p(img), _p(label), _p(loss),
+                                     _p(grad)), fn)
         return loss
 
     def cnn_adam(self, shape, master, wq, grad, m, v, step: int, lr: float):

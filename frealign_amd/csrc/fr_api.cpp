@@ -1001,6 +1001,17 @@ int fr_cnn_gradients(fr_ctx* c, int S, int C, int nconv, int K, const float* d_w
   return cnn_backward(*c, lo, d_wq, nb, d_img, d_label, ws, d_loss, d_grad);
 }
 
+int fr_cnn_gradients_bf16(fr_ctx* c, int S, int C, int nconv, int K, const float* d_wq, int nb,
+                          const float* d_img, const int* d_label, float* d_loss, float* d_grad) {
+  CTX_CHECK(c);
+  FR_CHECK(d_wq && d_img && d_label && d_loss && d_grad && nb >= 1, "null argument");
+  CNN_SHAPE(lo);
+  int err = 0;
+  float* ws = (float*)c->slot(9, sizeof(float) * cnn_workspace(lo, nb, true), &err);
+  if (!ws) return err;
+  return cnn_backward(*c, lo, d_wq, nb, d_img, d_label, ws, d_loss, d_grad, true);
+}
+
 int fr_cnn_adam(fr_ctx* c, int S, int C, int nconv, int K, float* d_master, float* d_wq,
                 const float* d_grad, float* d_m, float* d_v, int step, float lr) {
   CTX_CHECK(c);

@@ -35,8 +35,10 @@ bool cnn_bf16_ok(const CnnLayout& lo);
 size_t cnn_workspace_bf16(const CnnLayout& lo, int B);
 int cnn_forward_bf16(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
                      void* ws, float* logits);
+// bf16: every GEMM of the step (forward included) with BF16 operands and
+// FP32 accumulation (the mixed-precision training step); else FP32 throughout
 int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
-                 const int* label, float* ws, float* loss, float* grad);
+                 const int* label, float* ws, float* loss, float* grad, bool bf16 = false);
 int cnn_adam(Ctx& c, const CnnLayout& lo, float* master, float* wq, const float* grad, float* m,
              float* v, int step, float lr);
 int cnn_propose(Ctx& c, int K, int B, const float* logits, int topk, int* top, float* prob);

@@ -275,6 +275,196 @@ __global__ void __launch_bounds__(kT) k_gemm_t(GemmArgs g) {
       }
 }
 
+// The same GEMM with BF16 operands (each FP32 element rounded to nearest
+// even as it is staged) on v_mfma_f32_16x16x32_bf16, FP32 accumulation and
+// epilogue: the training step's mixed-precision form.  K is staged 32 at a
+// time; LDS holds the tiles k-contiguous per row (A[m][k], B[n][k]) so that
+// each lane's 8-element operand is one 16-byte read.  Same modes, arguments
+// and epilogue as k_gemm_t.
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 gbf16x8;
+constexpr int kBP = 40;  // LDS row pitch (bf16): 80 bytes
+
+template <int kMode>
+__global__ void __launch_bounds__(kT) k_gemm_b(GemmArgs g) {
+  constexpr bool kConv = kMode == kGemmConv;
+  __shared__ __attribute__((aligned(16))) __bf16 As[2][64][kBP];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[2][64][kBP];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int la = lane & 15, lg = lane >> 4;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  const int kbeg = blockIdx.z * g.kchunk;
+  const int kend = min(g.K, kbeg + g.kchunk);
+  const bool a_kfast = g.sak == 1;
+  const bool b_nfast = g.sbn == 1;
+  fx4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++) acc[i][j] = fx4{0.f, 0.f, 0.f, 0.f};
+  // this thread's eight A and eight B elements of a K tile of 32
+  int am[8], ak[8], bn[8], bk[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int e = tid + kT * q;
+    if (a_kfast) {
+      ak[q] = e & 31;
+      am[q] = e >> 5;
+    } else {
+      am[q] = e & 63;
+      ak[q] = e >> 6;
+    }
+    if (b_nfast) {
+      bn[q] = e & 63;
+      bk[q] = e >> 6;
+    } else {
+      bk[q] = e & 31;
+      bn[q] = e >> 5;
+    }
+  }
+  // implicit convolution: the n of an n-fast B operand is fixed per thread
+  long long cb = 0;
+  int cy = 0, cx = 0;
+  const int H = g.conv_H, Ho = H / 2, P = Ho * Ho;
+  if (kConv || kMode == kGemmConvX) {
+    const int n = n0 + bn[0];
+    if (kConv) {
+      const int b = n / P, pp = n - b * P, oy = pp / Ho, ox = pp - oy * Ho;
+      cb = (long long)b * H * H;
+      cy = 2 * oy - 1;
+      cx = 2 * ox - 1;
+    } else {
+      const int b = n / P, pp = n - b * P, t = pp / Ho, u = pp - t * Ho;
+      cb = (long long)b * P;
+      cy = 2 * t + (g.conv_cls >> 1) + 1;
+      cx = 2 * u + (g.conv_cls & 1) + 1;
+    }
+  }
+  // weight gradient: the k of a k-fast B operand is fixed per thread; its
+  // eight n are taps (ci, ky, kx)
+  int wci[8], wky[8], wkx[8];
+  if (kMode == kGemmConvW) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int n = n0 + bn[q];
+      wci[q] = n / 9;
+      const int kk = n - 9 * wci[q];
+      wky[q] = kk / 3;
+      wkx[q] = kk - 3 * wky[q];
+    }
+  }
+  float ra[8], rb[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      const int m = m0 + am[q], k = k0 + ak[q];
+      ra[q] = (m < g.M && k < kend) ? g.A[m * g.sam + k * g.sak] : 0.f;
+    }
+    if (kMode == kGemmConvW) {
+      const int k2 = k0 + bk[0];
+      int wb = 0, woy = 0, wox = 0;
+      if (k2 < kend) {
+        wb = k2 / P;
+        const int pp = k2 - wb * P;
+        woy = pp / Ho;
+        wox = pp - woy * Ho;
+      }
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        float v = 0.f;
+        if (n0 + bn[q] < g.N && k2 < kend) {
+          const int y = 2 * woy + wky[q] - 1, x = 2 * wox + wkx[q] - 1;
+          if (y >= 0 && y < H && x >= 0 && x < H)
+            v = g.B[((long long)wci[q] * g.conv_B + wb) * H * H + (long long)y * H + x];
+        }
+        rb[q] = v;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; q++) {
+        const int n = n0 + bn[q], k2 = k0 + bk[q];
+        float v = 0.f;
+        if (n < g.N && k2 < kend) {
+          if (kMode == kGemmConvX) {
+            const int py = g.conv_cls >> 1, px = g.conv_cls & 1;
+            const int ntx = px + 1, ntap = (py + 1) * ntx;
+            const int co = k2 / ntap, j = k2 - ntap * co, jy = j / ntx, jx = j - jy * ntx;
+            const int ky = py ? 2 * jy : 1, kx = px ? 2 * jx : 1;
+            const int oy = (cy - ky) >> 1, ox = (cx - kx) >> 1;
+            if (oy < Ho && ox < Ho)
+              v = g.B[(long long)co * g.conv_B * P + cb + (long long)oy * Ho + ox];
+          } else if (kConv) {
+            const int ci = k2 / 9, kk = k2 - 9 * ci, ky = kk / 3, kx = kk - 3 * ky;
+            const int y = cy + ky, x = cx + kx;
+            if (y >= 0 && y < H && x >= 0 && x < H)
+              v = g.B[(long long)ci * g.conv_B * H * H + cb + (long long)y * H + x];
+          } else {
+            v = g.B[k2 * g.sbk + n * g.sbn];
+          }
+        }
+        rb[q] = v;
+      }
+    }
+  };
+  auto put = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      As[buf][am[q]][ak[q]] = (__bf16)ra[q];
+      Bs[buf][bn[q]][bk[q]] = (__bf16)rb[q];
+    }
+  };
+  int buf = 0;
+  if (kbeg < kend) {
+    fetch(kbeg);
+    put(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += 32) {
+    const bool more = k0 + 32 < kend;
+    if (more) fetch(k0 + 32);
+    gbf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+      a[i] = *reinterpret_cast<const gbf16x8*>(&As[buf][wm * 32 + i * 16 + la][8 * lg]);
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+      b[j] = *reinterpret_cast<const gbf16x8*>(&Bs[buf][wn * 32 + j * 16 + la][8 * lg]);
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+      for (int j = 0; j < 2; j++)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) put(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+  float* C = g.C + blockIdx.z * g.csplit;
+#pragma unroll
+  for (int i = 0; i < 2; i++)
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int m = m0 + wm * 32 + i * 16 + 4 * lg + r;
+        const int n = n0 + wn * 32 + j * 16 + la;
+        if (m < g.M && n < g.N) {
+          float v = acc[i][j][r];
+          if (g.bias) v += g.bias[m];
+          if (g.relu) v = fmaxf(v, 0.f);
+          long long at = m * g.scm + n * g.scn;
+          if (kMode == kGemmConvX) {
+            const int b = n / P, pp = n - b * P, t = pp / Ho, u = pp - t * Ho;
+            at = m * g.scm + (long long)b * H * H + (long long)(2 * t + (g.conv_cls >> 1)) * H +
+                 (2 * u + (g.conv_cls & 1));
+            if (!(g.mask[at] > 0.f)) v = 0.f;
+          }
+          float* p = C + at;
+          if (g.accumulate) v += *p;
+          *p = v;
+        }
+      }
+}
+
 // out[i] = sum over s of part[s][i] for many splits and few outputs: one
 // block per output, thread t summing splits t, t+256, ... in order, then a
 // fixed tree (deterministic)
@@ -758,14 +948,24 @@ __global__ void __launch_bounds__(kT)
 
 inline unsigned nblk(long long n) { return (unsigned)((n + kT - 1) / kT); }
 
+// the FP32 (k_gemm_t) or BF16-operand (k_gemm_b) form of one GEMM launch
+template <int kMode>
+void launch_gemm(Ctx& c, dim3 grid, const GemmArgs& g, bool bf16) {
+  if (bf16)
+    hipLaunchKernelGGL(k_gemm_b<kMode>, grid, dim3(kT), 0, c.stream, g);
+  else
+    hipLaunchKernelGGL(k_gemm_t<kMode>, grid, dim3(kT), 0, c.stream, g);
+}
+
 int gemm(Ctx& c, int M, int N, int K, const float* A, long long sam, long long sak, const float* B,
          long long sbk, long long sbn, float* C, long long scm, long long scn,
-         const float* bias = nullptr, bool relu = false, bool accumulate = false) {
+         const float* bias = nullptr, bool relu = false, bool accumulate = false,
+         bool bf16 = false) {
   GemmArgs g{M, N, K, A, sam, sak, B, sbk, sbn, C, scm, scn, ((K + 15) / 16) * 16, 0, bias,
              relu ? 1 : 0, accumulate ? 1 : 0, 0, 0, 0, nullptr};
   dim3 grid((N + 63) / 64, (M + 63) / 64, 1);
   FR_CHECK(grid.y < 65536u, "GEMM too tall");
-  hipLaunchKernelGGL(k_gemm_t<kGemmPlain>, grid, dim3(kT), 0, c.stream, g);
+  launch_gemm<kGemmPlain>(c, grid, g, bf16);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -773,7 +973,7 @@ int gemm(Ctx& c, int M, int N, int K, const float* A, long long sam, long long s
 // out[C][B*Ho*Ho] = W[C][cin*9] x im2col(in) + bias, ReLU, with the im2col
 // operand gathered inside the GEMM (the same K order as the explicit form)
 int conv_forward(Ctx& c, int C, int cin, int B, int H, const float* W, const float* in,
-                 const float* bias, float* out) {
+                 const float* bias, float* out, bool bf16) {
   const int Ho = H / 2;
   const long long NB = (long long)B * Ho * Ho;
   FR_CHECK(NB < (1ll << 31), "batch too large");
@@ -781,7 +981,7 @@ int conv_forward(Ctx& c, int C, int cin, int B, int H, const float* W, const flo
   GemmArgs g{C, N, K, W, K, 1, in, NB, 1, out, NB, 1, ((K + 15) / 16) * 16, 0, bias, 1, 0, B, H,
              0, nullptr};
   dim3 grid((N + 63) / 64, (C + 63) / 64, 1);
-  hipLaunchKernelGGL(k_gemm_t<kGemmConv>, grid, dim3(kT), 0, c.stream, g);
+  launch_gemm<kGemmConv>(c, grid, g, bf16);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -796,20 +996,22 @@ constexpr int kWgradWorkgroups = 2048;
 // GEMM), with the positions split over enough workgroups to fill the GPU;
 // the splits' partial products (in `part`) are summed in split order
 int conv_wgrad(Ctx& c, int C, int cin, int B, int H, const float* g_, const float* in,
-               float* gw, float* part) {
+               float* gw, float* part, bool bf16) {
   const int Ho = H / 2, N = cin * 9;
   const long long NB = (long long)B * Ho * Ho;
   const int tiles = ((N + 63) / 64) * ((C + 63) / 64);
   long long nsplit = (kWgradWorkgroups + tiles - 1) / tiles;
   nsplit = std::min(nsplit, std::max(1ll, NB / 256));
   nsplit = std::min(nsplit, std::max(1ll, kPartFloats / ((long long)C * N)));
-  int kchunk = (int)(((NB + nsplit - 1) / nsplit + 15) / 16 * 16);
+  // whole K tiles per split (16 for FP32, 32 for BF16 operands)
+  const int kt = bf16 ? 32 : 16;
+  int kchunk = (int)(((NB + nsplit - 1) / nsplit + kt - 1) / kt * kt);
   nsplit = (NB + kchunk - 1) / kchunk;
   FR_CHECK(nsplit < 65536 && nsplit * C * N <= kPartFloats, "weight-gradient split too large");
   GemmArgs g{C, N, (int)NB, g_, NB, 1, in, 1, NB, part, N, 1, kchunk, (long long)C * N,
              nullptr, 0, 0, B, H, 0, nullptr};
   dim3 grid((N + 63) / 64, (C + 63) / 64, (unsigned)nsplit);
-  hipLaunchKernelGGL(k_gemm_t<kGemmConvW>, grid, dim3(kT), 0, c.stream, g);
+  launch_gemm<kGemmConvW>(c, grid, g, bf16);
   FR_HIP(hipGetLastError());
   if ((long long)C * N < 16384 && nsplit > 64)
     hipLaunchKernelGGL(k_split_sum_wide, dim3((unsigned)(C * N)), dim3(kT), 0, c.stream,
@@ -850,7 +1052,7 @@ __global__ void __launch_bounds__(kT)
 // gathered in place (no column matrix) and the ReLU derivative applied on
 // the way out
 int conv_xgrad(Ctx& c, int C, int cin, int B, int H, const float* w, const float* g_,
-               const float* act, float* wt, float* gin) {
+               const float* act, float* wt, float* gin, bool bf16) {
   const int Ho = H / 2;
   const long long N = (long long)B * Ho * Ho;
   FR_CHECK((long long)B * H * H < (1ll << 31), "batch too large");
@@ -863,7 +1065,7 @@ int conv_xgrad(Ctx& c, int C, int cin, int B, int H, const float* w, const float
     GemmArgs g{cin, (int)N, K, wt + (size_t)off[cls] * C * cin, K, 1, g_, 0, 1, gin,
                (long long)B * H * H, 1, ((K + 15) / 16) * 16, 0, nullptr, 0, 0, B, H, cls, act};
     dim3 grid((unsigned)((N + 63) / 64), (cin + 63) / 64, 1);
-    hipLaunchKernelGGL(k_gemm_t<kGemmConvX>, grid, dim3(kT), 0, c.stream, g);
+    launch_gemm<kGemmConvX>(c, grid, g, bf16);
     FR_HIP(hipGetLastError());
   }
   return 0;
@@ -979,13 +1181,13 @@ int cnn_init(Ctx& c, const CnnLayout& lo, unsigned long long seed, float* master
 namespace {
 
 int forward_impl(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
-                 float* ws, const Ws& w) {
+                 float* ws, const Ws& w, bool bf16) {
   const float* in = img;
   for (int l = 0; l < lo.L; l++) {
     const int H = lo.hin(l), cin = lo.cin(l);
     // out[C][NB] = W[C][cin*9] x cols[cin*9][NB] + b, ReLU
     if (int r = conv_forward(c, lo.C, cin, B, H, wq + lo.w_off[l], in, wq + lo.b_off[l],
-                             ws + w.act[l]))
+                             ws + w.act[l], bf16))
       return r;
     in = ws + w.act[l];
   }
@@ -995,7 +1197,7 @@ int forward_impl(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
   FR_HIP(hipGetLastError());
   // logits[b][k] = sum_f Wfc[k][f] X[f][b] + bfc[k]  (C element (k, b) at b*K + k)
   return gemm(c, lo.K, B, lo.F(), wq + lo.fw_off, lo.F(), 1, ws + w.X, B, 1, ws + w.logits, 1,
-              lo.K, wq + lo.fb_off, false);
+              lo.K, wq + lo.fb_off, false, false, bf16);
 }
 
 }  // namespace
@@ -1003,28 +1205,32 @@ int forward_impl(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
 int cnn_forward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img, float* ws,
                 float* logits) {
   Ws w = ws_layout(lo, B, false);
-  if (int r = forward_impl(c, lo, wq, B, img, ws, w)) return r;
+  if (int r = forward_impl(c, lo, wq, B, img, ws, w, false)) return r;
   FR_HIP(hipMemcpyAsync(logits, ws + w.logits, sizeof(float) * (size_t)lo.K * B,
                         hipMemcpyDeviceToDevice, c.stream));
   return 0;
 }
 
 int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const float* img,
-                 const int* label, float* ws, float* loss, float* grad) {
+                 const int* label, float* ws, float* loss, float* grad, bool bf16) {
   Ws w = ws_layout(lo, B, true);
-  if (int r = forward_impl(c, lo, wq, B, img, ws, w)) return r;
+  if (int r = forward_impl(c, lo, wq, B, img, ws, w, bf16)) return r;
   const int K = lo.K, F = lo.F();
   hipLaunchKernelGGL(k_softmax_xent, dim3(B), dim3(kT), 0, c.stream, K, B, ws + w.logits, label,
                      loss, ws + w.gL);
   FR_HIP(hipGetLastError());
   // gWfc[k][f] = sum_b gL[b][k] X[f][b]
-  if (int r = gemm(c, K, F, B, ws + w.gL, 1, K, ws + w.X, 1, B, grad + lo.fw_off, F, 1)) return r;
+  if (int r = gemm(c, K, F, B, ws + w.gL, 1, K, ws + w.X, 1, B, grad + lo.fw_off, F, 1, nullptr,
+                   false, false, bf16))
+    return r;
   // gbfc[k] = sum_b gL[b][k]
   hipLaunchKernelGGL(k_rowsum, dim3(K), dim3(kT), 0, c.stream, B, 1ll, (long long)K, ws + w.gL,
                      grad + lo.fb_off);
   FR_HIP(hipGetLastError());
   // gX[f][b] = sum_k Wfc[k][f] gL[b][k]
-  if (int r = gemm(c, F, B, K, wq + lo.fw_off, 1, F, ws + w.gL, 1, K, ws + w.gX, B, 1)) return r;
+  if (int r = gemm(c, F, B, K, wq + lo.fw_off, 1, F, ws + w.gL, 1, K, ws + w.gX, B, 1, nullptr,
+                   false, false, bf16))
+    return r;
   const int PL = lo.hout(lo.L - 1) * lo.hout(lo.L - 1);
   const long long nl = (long long)lo.C * B * PL;
   float* gcur = ws + w.gA;
@@ -1037,7 +1243,7 @@ int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
     const long long NB = (long long)B * Ho * Ho;
     const float* in = l == 0 ? img : ws + w.act[l - 1];
     // gW[C][cin*9] = sum_n g[C][n] im2col(in)[cin*9][n]
-    if (int r = conv_wgrad(c, lo.C, cin, B, H, gcur, in, grad + lo.w_off[l], ws + w.part))
+    if (int r = conv_wgrad(c, lo.C, cin, B, H, gcur, in, grad + lo.w_off[l], ws + w.part, bf16))
       return r;
     // gb[C] = sum_n g[C][n]: per-split partials in ws.part (free again once
     // the split-K sum above has run), then the splits in order
@@ -1053,7 +1259,7 @@ int cnn_backward(Ctx& c, const CnnLayout& lo, const float* wq, int B, const floa
     if (l > 0) {
       // the input gradient through the layer below's ReLU
       if (int r = conv_xgrad(c, lo.C, cin, B, H, wq + lo.w_off[l], gcur, ws + w.act[l - 1],
-                             ws + w.wt, gnext))
+                             ws + w.wt, gnext, bf16))
         return r;
       float* t = gcur;
       gcur = gnext;

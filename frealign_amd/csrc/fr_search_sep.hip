@@ -38,6 +38,8 @@ namespace {
 
 constexpr int kSepThreads = 256;  // 4 waves; a wave takes one template at a time
 constexpr int kSepTile = 256;     // list entries per work item
+constexpr int kSepFbTile = 64;    // templates per work item of the fallback form (few
+                                  // particles: short items keep the GPU full)
 constexpr int kSepElems = 2048;   // grid points per template (8 x 64 x 4)
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -180,8 +182,8 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
     it.p = fb[1 + q];
     it.base = loff[it.p];
     const int end = loff[it.p + 1] < cap ? loff[it.p + 1] : cap;
-    it.lo = tile * kSepTile;
-    it.hi = min(min(nrow, end - it.base), it.lo + kSepTile);
+    it.lo = tile * kSepFbTile;
+    it.hi = min(min(nrow, end - it.base), it.lo + kSepFbTile);
     return it.base >= 0 && it.lo < it.hi;
   }
   if (kMode == kSepUniform) {
@@ -474,7 +476,7 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
   if (fb) {
     // the fallback particles: a persistent grid (a multiple of the 8 XCDs, so
     // that each workgroup keeps its XCD's tiles) over the uniform mapping
-    const int ntile = (nbank + kSepTile - 1) / kSepTile;
+    const int ntile = (nbank + kSepFbTile - 1) / kSepFbTile;
     const int gf = full / 8 * 8 > 8 ? full / 8 * 8 : 8;
     hipLaunchKernelGGL(k_search_sep<kSepFallback>, dim3(gf), dim3(kSepThreads), 0, c.stream, np,
                        nbank, nsamr, nbank, ntile, loff, nullptr, cap, nullptr, bk, a, b, cc, sum2,

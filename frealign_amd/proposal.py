@@ -30,9 +30,12 @@ from .search import GlobalSearch, SearchResult
 
 @dataclass
 class ProposalConfig:
-    S: int = 128            # network input (Fourier crop of the particle)
+    # network input (Fourier crop of the particle): 64^2 and four stride-2
+    # convolutions end at 4 x 4 x 64 features; at 15-degree classes it
+    # proposes as well as a 128^2 input with five, at a quarter of the cost
+    S: int = 64
     channels: int = 64
-    nconv: int = 5
+    nconv: int = 4
     class_deg: float = 15.0
     topk: int = 8
     threshold: float = 0.3  # max softmax below which the exhaustive search runs
@@ -103,15 +106,20 @@ class Network:
     def forward(self, img: torch.Tensor) -> torch.Tensor:
         return self.ctx.cnn_forward(self.shape, self.wq, img)
 
-    def infer(self, img: torch.Tensor) -> torch.Tensor:
+    def infer(self, img: torch.Tensor, ctx: api.Context | None = None) -> torch.Tensor:
         """Logits for proposing: BF16 activations where the shape allows (64
-        channels), else the FP32 forward."""
+        channels), else the FP32 forward; on `ctx` (its stream and workspace)
+        when given."""
+        c = ctx or self.ctx
         if self.shape[1] == 64 and self.shape[2] >= 2:
-            return self.ctx.cnn_forward_bf16(self.shape, self.wq, img)
-        return self.forward(img)
+            return c.cnn_forward_bf16(self.shape, self.wq, img)
+        return c.cnn_forward(self.shape, self.wq, img)
 
-    def train_step(self, img: torch.Tensor, label: torch.Tensor, lr: float) -> torch.Tensor:
-        loss = self.ctx.cnn_gradients(self.shape, self.wq, img, label, self.grad)
+    def train_step(self, img: torch.Tensor, label: torch.Tensor, lr: float,
+                   bf16: bool = True) -> torch.Tensor:
+        """One Adam step on a batch; the gradients in mixed precision (BF16
+        GEMM operands, FP32 accumulation) unless bf16 is False."""
+        loss = self.ctx.cnn_gradients(self.shape, self.wq, img, label, self.grad, bf16=bf16)
         self.step += 1
         self.ctx.cnn_adam(self.shape, self.master, self.wq, self.grad, self.m, self.v, self.step,
                           lr)
@@ -273,27 +281,32 @@ class LearnedProposal:
         return hit1 / n, hitk / n
 
     # ------------------------------------------------------------ search
-    def propose(self, datc: torch.Tensor, ctff: torch.Tensor):
-        img = self.ctx.cnn_images(self.search.nsam, self.cfg.S, datc, ctff)
-        return self.ctx.cnn_propose(self.net.infer(img), self.cfg.topk)
+    def propose(self, datc: torch.Tensor, ctff: torch.Tensor, ctx: api.Context | None = None):
+        c = ctx or self.ctx
+        img = c.cnn_images(self.search.nsam, self.cfg.S, datc, ctff)
+        return c.cnn_propose(self.net.infer(img, c), self.cfg.topk)
 
-    def launch(self, outc, ctff, datc, fsct, ring, topk: int = 32):
+    def launch(self, outc, ctff, datc, fsct, ring, topk: int = 32,
+               ctx: api.Context | None = None):
         """Enqueue the learned search of one batch on the current stream, from
         images to winners: proposal -> ragged candidate lists (the proposed
         classes' cells, or the whole bank for a particle below the confidence
         threshold) -> one GEMM search launch -> FP32 CC3M rescoring of the
         top-k.  The results are copied into a pinned host buffer
         asynchronously; `finish` waits for them.  A caller may launch the next
-        batch before finishing this one (two batches can be in flight)."""
+        batch before finishing this one (two batches can be in flight); with
+        a second context (its own workspace) on a second stream, the two
+        batches also run on the GPU at the same time."""
         s = self.search
-        top, prob = self.propose(datc, ctff)
-        loff, grp, cand, info = self.ctx.build_lists(
+        c = ctx or self.ctx
+        top, prob = self.propose(datc, ctff, c)
+        loff, grp, cand, info = c.build_lists(
             top, prob, self.cfg.threshold, self.cell_off_dev, self.cells_dev, self.cfg.group,
             s.norient, s.group, self.maxlen)
-        o, r, sg = self.ctx.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.search_bank,
-                                          outc, ctff, loff, grp, cand, topk, info=info)
-        sc, best = self.ctx.rescore(s.nsam, s.a3df, outc, ctff, o, r, s.dm, s.nsamr, s.maxr1,
-                                    s.maxr2, s.rbfact, s.fsct_device(fsct), ring)
+        o, r, sg = c.search_ragged(s.nsam, s.maxr1, s.maxr2, s.rbfact, s.search_bank,
+                                   outc, ctff, loff, grp, cand, topk, info=info)
+        sc, best = c.rescore(s.nsam, s.a3df, outc, ctff, o, r, s.dm, s.nsamr, s.maxr1,
+                             s.maxr2, s.rbfact, s.fsct_device(fsct), ring)
         npart = o.shape[0]
         f32 = torch.float32
         pack = torch.cat([o.view(f32).reshape(-1), r.view(f32).reshape(-1), sg.reshape(-1),
@@ -334,6 +347,6 @@ class LearnedProposal:
                            o_h, sg_h)
         return res, confident
 
-    def run(self, outc, ctff, datc, fsct, ring, topk: int = 32):
+    def run(self, outc, ctff, datc, fsct, ring, topk: int = 32, ctx: api.Context | None = None):
         """Learned search of one batch (launch + finish)."""
-        return self.finish(self.launch(outc, ctff, datc, fsct, ring, topk))
+        return self.finish(self.launch(outc, ctff, datc, fsct, ring, topk, ctx))

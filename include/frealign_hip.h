@@ -484,6 +484,14 @@ int fr_cnn_forward_bf16(fr_ctx *ctx, int S, int C, int nconv, int K, const float
 int fr_cnn_gradients(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq, int nb,
                      const float *d_img, const int *d_label, float *d_loss, float *d_grad);
 
+/* The same step in mixed precision, as training uses it: every GEMM of the
+ * forward and backward passes takes its operands rounded to BF16 (nearest
+ * even) and accumulates in FP32; elementwise work, the loss and the outputs
+ * are FP32. */
+int fr_cnn_gradients_bf16(fr_ctx *ctx, int S, int C, int nconv, int K, const float *d_wq,
+                          int nb, const float *d_img, const int *d_label, float *d_loss,
+                          float *d_grad);
+
 /* One Adam step (beta 0.9/0.999, eps 1e-8, bias-corrected; step >= 1) on the
  * master copy; d_wq is refreshed from it. */
 int fr_cnn_adam(fr_ctx *ctx, int S, int C, int nconv, int K, float *d_master, float *d_wq,

@@ -63,9 +63,18 @@ def col2im(cols: np.ndarray, Ci: int, H: int) -> np.ndarray:
     return xp[:, :, 1:H + 1, 1:H + 1]
 
 
-def forward(p: np.ndarray, shape, img: np.ndarray):
-    """Logits (B, K) and the cache for backward."""
+def _ops(bf16_ops: bool):
+    """The operand rounding of a GEMM: BF16 (via FP32) for the mixed-precision
+    step, none otherwise."""
+    return (lambda a: bf16_round(a).astype(np.float64)) if bf16_ops else (lambda a: a)
+
+
+def forward(p: np.ndarray, shape, img: np.ndarray, bf16_ops: bool = False):
+    """Logits (B, K) and the cache for backward.  bf16_ops: every GEMM's
+    operands rounded to BF16 (the activations where a product reads them),
+    as fr_cnn_gradients_bf16 stages them; the arithmetic stays FP64."""
     S, C, L, K = shape
+    r = _ops(bf16_ops)
     offs, fw, fb, F, _ = layout(S, C, L, K)
     p = np.asarray(p, np.float64)
     x = np.asarray(img, np.float64).reshape(-1, 1, S, S)
@@ -73,15 +82,15 @@ def forward(p: np.ndarray, shape, img: np.ndarray):
     cache = []
     for (w, b, ci) in offs:
         W = p[w[0]:w[1]].reshape(C, ci * 9)
-        cols = im2col(x)
-        z = np.einsum("ok,bkn->bon", W, cols) + p[b[0]:b[1]][None, :, None]
+        cols = im2col(r(x))
+        z = np.einsum("ok,bkn->bon", r(W), cols) + p[b[0]:b[1]][None, :, None]
         a = np.maximum(z, 0.0)
         Ho = x.shape[2] // 2
         cache.append((x, cols, a))
         x = a.reshape(B, C, Ho, Ho)
     X = x.reshape(B, F)
     Wf = p[fw[0]:fw[1]].reshape(K, F)
-    logits = X @ Wf.T + p[fb[0]:fb[1]][None, :]
+    logits = r(X) @ r(Wf).T + p[fb[0]:fb[1]][None, :]
     return logits, (cache, X)
 
 
@@ -104,12 +113,16 @@ def forward_bf16_act(p: np.ndarray, shape, img: np.ndarray):
     return x.reshape(B, F) @ Wf.T + p[fb[0]:fb[1]][None, :]
 
 
-def loss_grad(p: np.ndarray, shape, img: np.ndarray, label: np.ndarray):
-    """Per-image softmax cross-entropy and the gradient of their mean."""
+def loss_grad(p: np.ndarray, shape, img: np.ndarray, label: np.ndarray, bf16_ops: bool = False):
+    """Per-image softmax cross-entropy and the gradient of their mean;
+    bf16_ops as in forward (the backward products too: the loss gradient,
+    the layer gradients and the weights where a product reads them; the bias
+    sums and the ReLU masks stay exact)."""
     S, C, L, K = shape
+    r = _ops(bf16_ops)
     offs, fw, fb, F, n = layout(S, C, L, K)
     p = np.asarray(p, np.float64)
-    logits, (cache, X) = forward(p, shape, img)
+    logits, (cache, X) = forward(p, shape, img, bf16_ops)
     B = logits.shape[0]
     z = logits - logits.max(axis=1, keepdims=True)
     e = np.exp(z)
@@ -122,19 +135,19 @@ def loss_grad(p: np.ndarray, shape, img: np.ndarray, label: np.ndarray):
     gL /= B
     g = np.zeros(n, np.float64)
     Wf = p[fw[0]:fw[1]].reshape(K, F)
-    g[fw[0]:fw[1]] = (gL.T @ X).reshape(-1)
+    g[fw[0]:fw[1]] = (r(gL).T @ r(X)).reshape(-1)
     g[fb[0]:fb[1]] = gL.sum(axis=0)
-    gX = gL @ Wf
+    gX = r(gL) @ r(Wf)
     ga = gX.reshape(cache[-1][2].shape)  # (B, C, P)
     for l in range(L - 1, -1, -1):
         (w, b, ci) = offs[l]
         x, cols, a = cache[l]
         gz = ga * (a > 0)
         W = p[w[0]:w[1]].reshape(C, ci * 9)
-        g[w[0]:w[1]] = np.einsum("bon,bkn->ok", gz, cols).reshape(-1)
+        g[w[0]:w[1]] = np.einsum("bon,bkn->ok", r(gz), cols).reshape(-1)
         g[b[0]:b[1]] = gz.sum(axis=(0, 2))
         if l > 0:
-            gcols = np.einsum("ok,bon->bkn", W, gz)
+            gcols = np.einsum("ok,bon->bkn", r(W), r(gz))
             gx = col2im(gcols, ci, x.shape[2])
             ga = gx.reshape(x.shape[0], ci, -1)
     return loss, g

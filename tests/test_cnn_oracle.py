@@ -128,3 +128,40 @@ def test_forward_bf16_act_rounds_only_the_activations():
     b = CO.forward_bf16_act(p2, shape, img2)
     rel = np.linalg.norm(a - b) / np.linalg.norm(a)
     assert 1e-6 < rel < 2e-2, rel
+
+
+def test_bf16_operand_step_rounds_only_the_products():
+    """The mixed-precision step (loss_grad with bf16_ops): with integer
+    operands exact in BF16 its forward is the exact one, and the gradients
+    differ from the exact ones by BF16 rounding of the backward operands
+    only (the FC bias gradient, a plain sum of the loss gradient, is
+    unchanged); on random values the gradient differs by BF16 rounding (a
+    segment by more only where a rounded operand moves a pre-activation
+    across a ReLU's zero)."""
+    shape = (16, 4, 2, 5)
+    offs, fw, fb, F, n = CO.layout(*shape)
+    rng = np.random.default_rng(5)
+    p = np.zeros(n)
+    for (w, b, ci) in offs:
+        p[w[0]:w[1]] = rng.integers(-1, 2, w[1] - w[0])
+        p[b[0]:b[1]] = rng.integers(0, 2, b[1] - b[0])
+    p[fw[0]:fw[1]] = rng.integers(-2, 3, fw[1] - fw[0]) * 0.0625
+    img = rng.integers(0, 3, (3, 16, 16)).astype(np.float64)
+    lab = np.array([0, 3, 4])
+    exact, _ = CO.forward(p, shape, img)
+    mixed, _ = CO.forward(p, shape, img, bf16_ops=True)
+    assert np.array_equal(exact, mixed)
+    l0, g0 = CO.loss_grad(p, shape, img, lab)
+    l1, g1 = CO.loss_grad(p, shape, img, lab, bf16_ops=True)
+    assert np.array_equal(l0, l1)
+    assert np.array_equal(g0[fb[0]:fb[1]], g1[fb[0]:fb[1]])
+    assert 0 < np.linalg.norm(g1 - g0) <= 2e-2 * np.linalg.norm(g0)
+    p2 = rng.standard_normal(n) * 0.3
+    img2 = rng.standard_normal((3, 16, 16))
+    _, a = CO.loss_grad(p2, shape, img2, lab)
+    _, b = CO.loss_grad(p2, shape, img2, lab, bf16_ops=True)
+    # three images of a 4-channel network: a single flipped ReLU moves a
+    # segment by a few per cent, so the bar here is loose
+    for (w, bb, ci) in offs + [(fw, fb, 0)]:
+        rel = np.linalg.norm(a[w[0]:w[1]] - b[w[0]:w[1]]) / np.linalg.norm(a[w[0]:w[1]])
+        assert 1e-6 < rel < 0.2, rel
