@@ -129,12 +129,12 @@ int upload_table(const std::vector<T>& h, T** d) {
 }  // namespace
 
 int fr_ctx::cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t) {
-  std::vector<float> dft;
   auto di = dft_tabs.find(a.nsam);
   if (di == dft_tabs.end()) {
-    fr::cc3m_fused_table(a.nsam, dft);
+    std::vector<float> folded;
+    fr::cc3m_fused_table_folded(a.nsam, folded);
     float* d = nullptr;
-    if (int r = upload_table(dft, &d)) return r;
+    if (int r = upload_table(folded, &d)) return r;
     di = dft_tabs.emplace(a.nsam, d).first;
   }
   t->dft = di->second;
@@ -151,10 +151,9 @@ int fr_ctx::cc3m_tables(const fr::Cc3mArgs& a, fr::Cc3mTables* t) {
   const auto rk = std::make_tuple(a.nsam, a.ri2, a.ri3);
   auto ri = ring_tabs.find(rk);
   if (ri == ring_tabs.end()) {
-    if (dft.empty()) fr::cc3m_fused_table(a.nsam, dft);
     std::vector<float> ring;
     RingTab rt;
-    rt.cnt = fr::cc3m_fused_ring(a.nsam, a.ri2, a.ri3, dft, ring);
+    rt.cnt = fr::cc3m_fused_ring_q(a.nsam, a.ri2, a.ri3, ring);
     if (int r = upload_table(ring, &rt.d)) return r;
     ri = ring_tabs.emplace(rk, rt).first;
   }

@@ -211,11 +211,16 @@ __global__ void __launch_bounds__(kThreads)
 //   row r2c      Z^T (64 x 16m) = T^T x x'
 //   column r2c   out(64 x 64) = T^T x Z         (C +- i S' gives +-J)
 // One workgroup holds one item: T, Y/Z and the small operands live in LDS.
+// T is held folded (rows 0 .. N/2: row N - n of T is row n with its sine
+// columns negated), and the row passes use the fold: pixel rows n and N - n
+// come from the same cosine and sine partial sums (x = E +- O) and go back
+// through the same table row (cosine columns take y[n] + y[N-n], sine
+// columns y[n] - y[N-n]), which halves their matrix-core work.
 // The row-column order and the c2r conventions (Im ignored at I = 0) are
 // those of the FFT path, so the result agrees with it to FP32 rounding.
 namespace {
 
-constexpr int kFT = 512;     // threads: 8 waves
+constexpr int kFT = 512;     // threads: 8 waves (the column r2c maps waves to its 8 tile pairs)
 constexpr int kFK = 64;      // reals per row: 32 frequencies
 constexpr int kFP = 68;      // LDS row pitch (floats): conflict-free MFMA operand reads
 constexpr int kFJ = 31;      // |J| <= 31
@@ -251,24 +256,39 @@ __global__ void __launch_bounds__(kFT)
                  float ri3, float rih, float halfw, float* score, Cc3mWindow win,
                  int* __restrict__ wcode, int flip) {
   extern __shared__ __attribute__((aligned(16))) float lds[];
-  float* Ts = lds;                      // [nsam][kFP]
-  float* Ys = Ts + nsam * kFP;          // [nsam][kFP]: Y, then Z
+  const int hrow = nsam / 2 + 1;
+  float* Ts = lds;                      // [nsam/2 + 1][kFP]: the folded table TF
+  float* Ys = Ts + hrow * kFP;          // [nsam][kFP]: Y, then Z (columns Re | Im)
   float* Bs = Ys + nsam * kFP;          // [64][kFP]: B(X)^T, then out
-  float* bins = Bs + 64 * kFP;          // [4][nsam/2 + 2]
+  float* bins = Bs + 64 * kFP;          // [4][nb]
+  const int nb = nsam / 2 + 2 > kCc3mShells ? nsam / 2 + 2 : kCc3mShells;
+  int* pts_s = reinterpret_cast<int*>(bins + 4 * nb);  // the band point table
   __shared__ float red_s[kFT / 64];
   __shared__ int red_c[kFT / 64];
+  // e^{i SHX I} and e^{i SHY J} of the item: the shift phase of a point is
+  // their product
+  __shared__ float2 phx[32], phy[2 * kFJ + 1];
 
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int la = lane & 15, lg = lane >> 4;
   const int jc = nsam / 2 + 1, nsamh = nsam / 2;
   const size_t ic = (size_t)nsam * jc;
 
-  // ---- stage T once; the workgroup then takes items blockIdx.x, +gridDim.x, ...
-  for (int e = tid; e < nsam * (kFK / 4); e += kFT) {
+  // T[n][k] (k = 2i: cos, 2i + 1: -sin) for any row n, from the folded table
+  auto tval = [&](int n, int k) -> float {
+    const bool up = n > nsamh;
+    const float v = Ts[(up ? nsam - n : n) * kFP + (k & 1) * (kFK / 2) + (k >> 1)];
+    return (up && (k & 1)) ? -v : v;
+  };
+
+  // ---- stage TF once; the workgroup then takes items blockIdx.x, +gridDim.x, ...
+  for (int e = tid; e < hrow * (kFK / 4); e += kFT) {
     int n = e / (kFK / 4), q = e % (kFK / 4);
     *reinterpret_cast<float4*>(Ts + n * kFP + 4 * q) =
         *reinterpret_cast<const float4*>(tab + (size_t)n * kFK + 4 * q);
   }
+  // the band point table, once (item independent)
+  for (int e = tid; e < kCc3mPtsHdr + pts[kCc3mShells]; e += kFT) pts_s[e] = pts[e];
   for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
   const int p = pidx[it];
   const float amagp = amag_arr[it];
@@ -276,6 +296,16 @@ __global__ void __launch_bounds__(kFT)
 
   // ---- fill X (annulus x CTF / NSAM^2) into Ys
   float2* Xs = reinterpret_cast<float2*>(Ys);  // [32][63]: I, J + 31
+  if (!kWin && tid < 32 + 2 * kFJ + 1) {
+    float sn, cs;
+    if (tid < 32) {
+      sincosf(shift[2 * it] * (float)tid, &sn, &cs);
+      phx[tid] = make_float2(cs, sn);
+    } else {
+      sincosf(shift[2 * it + 1] * (float)(tid - 32 - kFJ), &sn, &cs);
+      phy[tid - 32] = make_float2(cs, sn);
+    }
+  }
   {
     const float* d = dm + 9 * (size_t)it;
     const float2* cp = ctff + (size_t)p * 2 * ic;
@@ -299,7 +329,12 @@ __global__ void __launch_bounds__(kFT)
     }
   }
   __syncthreads();
-  // B(X)^T[c][k]: column c = (I, re|im), k = (|J|, cos|sin)
+  // B(X)^T[c][k]: column c = (I, re|im), k = (|J|, cos|sin).  At AMAG = 1
+  // the MASKCOS2 ring sum is linear in B: sum_{c,k} Q[c][k] B^T[c][k] with the
+  // precomputed Q (the ring's rows of T through both transforms)
+  const bool ringq = ring_tab != nullptr && amagp == 1.0f;
+  float rsum = 0.f;
+  int rcnt = 0;
   for (int e = tid; e < 64 * 64; e += kFT) {
     int c = e >> 6, k = e & 63;
     int i = c >> 1, jj = k >> 1;
@@ -311,32 +346,77 @@ __global__ void __launch_bounds__(kFT)
     if (!(c & 1)) v = (k & 1) ? Q.y : P.x;   // Y_re = P_re T_re + Q_im T_im
     else v = (k & 1) ? -Q.x : P.y;           // Y_im = P_im T_re - Q_re T_im
     Bs[c * kFP + k] = v;
+    if (ringq) rsum += ring_tab[e] * v;
   }
   __syncthreads();
 
-  // ---- column c2c: Y[m][c] = sum_k T[m][k] B[k][c], times the c2r weight
+  // ---- column c2c: Y[m][c] = sum_k T[m][k] B[k][c], times the c2r weight,
+  // folded like the rows: for m < N/2, E = TF[m][0:32] against the cosine
+  // rows of B and O = TF[m][32:64] against its sine rows give Y[m] = E + O
+  // and Y[N-m] = E - O (row 0 once); row N/2, which has no partner, on the
+  // vector ALU.  Folded tiles (16 m x 16 c) go two at a time (four chains).
   const int mt = nsam / 16;
-  for (int t = wave; t < mt * 4; t += kFT / 64) {
-    const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
-    fx4 acc = {0.f, 0.f, 0.f, 0.f};
+  {
+    const int nct = (nsamh / 16) * 4;
+    auto ctile = [&](int t, fx4& e, fx4& o) {
+      const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
 #pragma unroll
-    for (int ks = 0; ks < kFK / 4; ks++)
-      acc = mfma4(Ts[(m0 + la) * kFP + 4 * ks + lg], Bs[(c0 + la) * kFP + 4 * ks + lg], acc);
-    const float w = (c0 + la) < 2 ? 1.0f : 2.0f;
+      for (int ks = 0; ks < kFK / 8; ks++) {
+        e = mfma4(Ts[(m0 + la) * kFP + 4 * ks + lg], Bs[(c0 + la) * kFP + 8 * ks + 2 * lg], e);
+        o = mfma4(Ts[(m0 + la) * kFP + kFK / 2 + 4 * ks + lg],
+                  Bs[(c0 + la) * kFP + 8 * ks + 2 * lg + 1], o);
+      }
+    };
+    auto cstore = [&](int t, const fx4& e, const fx4& o) {
+      const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
+      const float w = (c0 + la) < 2 ? 1.0f : 2.0f;
 #pragma unroll
-    for (int i = 0; i < 4; i++) Ys[(m0 + 4 * lg + i) * kFP + c0 + la] = acc[i] * w;
+      for (int i = 0; i < 4; i++) {
+        const int m = m0 + 4 * lg + i;
+        Ys[m * kFP + c0 + la] = (e[i] + o[i]) * w;
+        if (m) Ys[(nsam - m) * kFP + c0 + la] = (e[i] - o[i]) * w;
+      }
+    };
+    for (int t = wave; t < nct; t += 2 * (kFT / 64)) {
+      const int t1 = t + kFT / 64;
+      fx4 e0 = {0.f, 0.f, 0.f, 0.f}, o0 = e0, e1 = e0, o1 = e0;
+      if (t1 < nct) {
+        const int m0 = (t >> 2) * 16, c0 = (t & 3) * 16;
+        const int m1 = (t1 >> 2) * 16, c1 = (t1 & 3) * 16;
+#pragma unroll
+        for (int ks = 0; ks < kFK / 8; ks++) {
+          e0 = mfma4(Ts[(m0 + la) * kFP + 4 * ks + lg], Bs[(c0 + la) * kFP + 8 * ks + 2 * lg],
+                     e0);
+          o0 = mfma4(Ts[(m0 + la) * kFP + kFK / 2 + 4 * ks + lg],
+                     Bs[(c0 + la) * kFP + 8 * ks + 2 * lg + 1], o0);
+          e1 = mfma4(Ts[(m1 + la) * kFP + 4 * ks + lg], Bs[(c1 + la) * kFP + 8 * ks + 2 * lg],
+                     e1);
+          o1 = mfma4(Ts[(m1 + la) * kFP + kFK / 2 + 4 * ks + lg],
+                     Bs[(c1 + la) * kFP + 8 * ks + 2 * lg + 1], o1);
+        }
+        cstore(t, e0, o0);
+        cstore(t1, e1, o1);
+      } else {
+        ctile(t, e0, o0);
+        cstore(t, e0, o0);
+      }
+    }
+    // row N/2: Y[N/2][c] = sum_k T[N/2][k] B[k][c]
+    for (int c = tid; c < kFK; c += kFT) {
+      float y = 0.f;
+      for (int j = 0; j < kFK / 2; j++)
+        y += Ts[nsamh * kFP + j] * Bs[c * kFP + 2 * j] +
+             Ts[nsamh * kFP + kFK / 2 + j] * Bs[c * kFP + 2 * j + 1];
+      Ys[nsamh * kFP + c] = y * (c < 2 ? 1.0f : 2.0f);
+    }
   }
   __syncthreads();
 
   const bool use_wmap = wmap != nullptr && amagp == 1.0f;
-  // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3'.  x is linear in Y,
-  // so at AMAG = 1 the ring sum is sum_{m,k} R[m][k] Y[m][k] with the
-  // precomputed R; otherwise a pass over the ring tiles on the matrix cores.
-  float rsum = 0.f;
-  int rcnt = 0;
-  if (ring_tab && amagp == 1.0f) {
-    for (int e = tid; e < nsam * kFK; e += kFT)
-      rsum += ring_tab[e] * Ys[(e / kFK) * kFP + (e % kFK)];
+  // ---- MASKCOS2 ring mean: x over ri2' < r^2 <= ri3'.  At AMAG = 1 the
+  // ring sum was taken from B above; otherwise a pass over the ring tiles on
+  // the matrix cores.
+  if (ringq) {
     rcnt = tid == 0 ? ring_cnt : 0;
   } else
   for (int rb = wave; rb < mt; rb += kFT / 64) {
@@ -356,7 +436,7 @@ __global__ void __launch_bounds__(kFT)
       fx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int ks = 0; ks < kFK / 4; ks++)
-        acc = mfma4(Ts[(n0 + la) * kFP + 4 * ks + lg], yv[ks], acc);
+        acc = mfma4(tval(n0 + la, 4 * ks + lg), yv[ks], acc);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
         int n = n0 + 4 * lg + i, dx = n < nsamh ? n : n - nsam;
@@ -390,23 +470,34 @@ __global__ void __launch_bounds__(kFT)
   // x' = mean + (x - mean) w, with w = 1 inside RI2', the cosine edge in the
   // ring and 0 outside RI3'; since sum_n T[n][k] is N for the DC term (k = 0)
   // and 0 for every other k, Z^T = N mean e_0 + T^T ((x - mean) w), and tiles
-  // wholly outside the mask add nothing.  The active tiles of a row block go
-  // through the matrix cores four at a time, their chains interleaved.
+  // wholly outside the mask add nothing.  Folded: a tile of table rows
+  // r = 16t .. 16t+15 (r < N/2) gives E = TF[r][0:32] wYre and
+  // O = TF[r][32:64] wYim, x[r] = E + O, x[N-r] = E - O; y = (x - mean) w
+  // (the same w at r and N - r: equal wrapped distances); then
+  // Z[m][i] += TF[r][i] (y[r] + y[N-r]), Z[m][32+i] += TF[r][32+i] (y[r] -
+  // y[N-r]).  Row 0 is its own partner (counted once); row N/2 has none and
+  // is added on the vector ALU.  Z goes back with its columns blocked (Re |
+  // Im).  The active tiles of a row block go through the matrix cores two at
+  // a time, their four chains interleaved.
   for (int rb = wave; rb < mt; rb += kFT / 64) {
     const int m0 = rb * 16;
     int mlo, mhi;
     tile_r2(m0, nsamh, nsam, mlo, mhi);
-    float yv[kFK / 4];
-#pragma unroll
-    for (int ks = 0; ks < kFK / 4; ks++) yv[ks] = Ys[(m0 + la) * kFP + 4 * ks + lg];
     const int m = m0 + la;
     const int dy = m < nsamh ? m : m - nsam;
+    // wY of row m in the E/O operand layout: I = 4 ks + lg
+    float yre[kFK / 8], yim[kFK / 8];
+#pragma unroll
+    for (int ks = 0; ks < kFK / 8; ks++) {
+      yre[ks] = Ys[m * kFP + 8 * ks + 2 * lg];
+      yim[ks] = Ys[m * kFP + 8 * ks + 2 * lg + 1];
+    }
     fx4 z[4];
 #pragma unroll
     for (int kt = 0; kt < 4; kt++) z[kt] = fx4{0.f, 0.f, 0.f, 0.f};
-    // active tiles of this row block as a bit set (wave-uniform)
+    // active folded tiles of this row block as a bit set (wave-uniform)
     unsigned act = 0u;
-    for (int nt = 0; nt < mt; nt++) {
+    for (int nt = 0; nt < nsamh / 16; nt++) {
       int nlo, nhi;
       tile_r2(nt * 16, nsamh, nsam, nlo, nhi);
       if (!((float)(nlo + mlo) > ri3p)) act |= 1u << nt;
@@ -430,56 +521,89 @@ __global__ void __launch_bounds__(kFT)
       }
       return w;
     };
-    auto maskv = [&](const fx4& w, fx4& x) {
+    // E and O of folded tile t on the matrix cores (two independent chains)
+    auto eo = [&](int t, fx4& e, fx4& o) {
+      e = fx4{0.f, 0.f, 0.f, 0.f};
+      o = e;
 #pragma unroll
-      for (int i = 0; i < 4; i++) x[i] = (x[i] - mean) * w[i];
+      for (int ks = 0; ks < kFK / 8; ks++) {
+        e = mfma4(Ts[(16 * t + la) * kFP + 4 * ks + lg], yre[ks], e);
+        o = mfma4(Ts[(16 * t + la) * kFP + kFK / 2 + 4 * ks + lg], yim[ks], o);
+      }
     };
-    // k-step i takes n = n0 + 4*lg + i: the C layout of x is the B layout
-    auto zacc = [&](int n0, const fx4& x) {
+    // masked sums and differences of the pixel pairs (r, N - r), r = 16t+4lg+i
+    auto fold = [&](int t, const fx4& w, const fx4& e, const fx4& o, fx4& sp, fx4& sm) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const float ya = (e[i] + o[i] - mean) * w[i];
+        float yb = (e[i] - o[i] - mean) * w[i];
+        if (t == 0 && lg == 0 && i == 0) yb = 0.f;  // r = 0: its own partner
+        sp[i] = ya + yb;
+        sm[i] = ya - yb;
+      }
+    };
+    // k-step i takes r = 16t + 4 lg + i: the C layout of E, O is the B layout
+    auto zacc = [&](int t, const fx4& sp, const fx4& sm) {
 #pragma unroll
       for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int kt = 0; kt < 4; kt++)
-          z[kt] = mfma4(Ts[(n0 + 4 * lg + i) * kFP + 16 * kt + la], x[i], z[kt]);
+          z[kt] = mfma4(Ts[(16 * t + 4 * lg + i) * kFP + 16 * kt + la], kt < 2 ? sp[i] : sm[i],
+                        z[kt]);
     };
-    while (__builtin_popcount(act) >= 4) {
+    while (__builtin_popcount(act) >= 2) {
       const int t0 = __builtin_ctz(act);
       act &= act - 1u;
       const int t1 = __builtin_ctz(act);
       act &= act - 1u;
-      const int t2 = __builtin_ctz(act);
-      act &= act - 1u;
-      const int t3 = __builtin_ctz(act);
-      act &= act - 1u;
-      const fx4 w0 = weights(16 * t0), w1 = weights(16 * t1), w2 = weights(16 * t2),
-                w3 = weights(16 * t3);
-      fx4 x0 = {0.f, 0.f, 0.f, 0.f}, x1 = x0, x2 = x0, x3 = x0;
+      fx4 e0, o0, e1, o1, p0, q0, p1, q1;
+      // the weights first: their loads are in flight under the products
+      const fx4 w0 = weights(16 * t0), w1 = weights(16 * t1);
+      e0 = fx4{0.f, 0.f, 0.f, 0.f};
+      o0 = e0;
+      e1 = e0;
+      o1 = e0;
 #pragma unroll
-      for (int ks = 0; ks < kFK / 4; ks++) {
-        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
-        x1 = mfma4(Ts[(16 * t1 + la) * kFP + 4 * ks + lg], yv[ks], x1);
-        x2 = mfma4(Ts[(16 * t2 + la) * kFP + 4 * ks + lg], yv[ks], x2);
-        x3 = mfma4(Ts[(16 * t3 + la) * kFP + 4 * ks + lg], yv[ks], x3);
+      for (int ks = 0; ks < kFK / 8; ks++) {
+        e0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yre[ks], e0);
+        o0 = mfma4(Ts[(16 * t0 + la) * kFP + kFK / 2 + 4 * ks + lg], yim[ks], o0);
+        e1 = mfma4(Ts[(16 * t1 + la) * kFP + 4 * ks + lg], yre[ks], e1);
+        o1 = mfma4(Ts[(16 * t1 + la) * kFP + kFK / 2 + 4 * ks + lg], yim[ks], o1);
       }
-      maskv(w0, x0);
-      maskv(w1, x1);
-      maskv(w2, x2);
-      maskv(w3, x3);
-      zacc(16 * t0, x0);
-      zacc(16 * t1, x1);
-      zacc(16 * t2, x2);
-      zacc(16 * t3, x3);
+      fold(t0, w0, e0, o0, p0, q0);
+      fold(t1, w1, e1, o1, p1, q1);
+      zacc(t0, p0, q0);
+      zacc(t1, p1, q1);
     }
-    while (act) {
+    if (act) {
       const int t0 = __builtin_ctz(act);
-      act &= act - 1u;
+      fx4 e0, o0, p0, q0;
       const fx4 w0 = weights(16 * t0);
-      fx4 x0 = {0.f, 0.f, 0.f, 0.f};
+      eo(t0, e0, o0);
+      fold(t0, w0, e0, o0, p0, q0);
+      zacc(t0, p0, q0);
+    }
+    // row N/2: x = sum_I (-1)^I wYre[m][I] (its sine terms vanish), added to
+    // the cosine rows of Z as (-1)^i y
+    {
+      const float crad2 = (float)(nsamh * nsamh + dy * dy);
+      if (__ballot(crad2 <= ri3p)) {
+        float xs = 0.f;
 #pragma unroll
-      for (int ks = 0; ks < kFK / 4; ks++)
-        x0 = mfma4(Ts[(16 * t0 + la) * kFP + 4 * ks + lg], yv[ks], x0);
-      maskv(w0, x0);
-      zacc(16 * t0, x0);
+        for (int ks = 0; ks < kFK / 8; ks++) xs += yre[ks];
+        xs = (lg & 1) ? -xs : xs;  // I = 4 ks + lg: the sign of lane group lg
+        xs += __shfl_xor(xs, 16, 64);
+        xs += __shfl_xor(xs, 32, 64);
+        float w = 1.0f;
+        if (crad2 > ri3p) w = 0.0f;
+        else if (use_wmap) w = wmap[(size_t)m * nsam + nsamh];
+        else if (crad2 > ri2p) w = (1.0f + cosf(kPi * (sqrtf(crad2) - rihp) / halfw)) / 2.0f;
+        const float y = (xs - mean) * w;
+#pragma unroll
+        for (int kt = 0; kt < 2; kt++)
+#pragma unroll
+          for (int i = 0; i < 4; i++) z[kt][i] += (i & 1) ? -y : y;
+      }
     }
     if (lg == 0) z[0][0] += mean * (float)nsam;  // row k = 0, the DC term
     // Z[m][k]: this wave alone read rows m0..m0+15 of Ys
@@ -490,20 +614,47 @@ __global__ void __launch_bounds__(kFT)
   }
   __syncthreads();
 
-  // ---- column r2c: out[r][c] = sum_m T[m][r] Z[m][c]  (into Bs)
-  for (int t = wave; t < 16; t += kFT / 64) {
-    const int r0 = (t >> 2) * 16, c0 = (t & 3) * 16;
-    fx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-    for (int m0 = 0; m0 < nsam; m0 += 16) {
+  // ---- column r2c: out[r][c] = sum_m T[m][r] Z[m][c]  (into Bs), folded over
+  // m: the cosine rows r = j take P[m] = Z[m] + Z[N-m] and the sine rows
+  // r = 32 + j take M[m] = Z[m] - Z[N-m] against TF rows m = 0 .. N/2 (P = Z
+  // and M = 0 at m = 0 and N/2, their own partners).  out is written with its
+  // rows blocked (cos | sin) like its columns (Re | Im).  Wave w takes the
+  // cosine and sine tiles of rows j0 = 16 (w >> 2) and columns 16 (w & 3).
+  static_assert(kFT == 512, "eight waves: one (cosine, sine) tile pair each");
+  {
+    const int c0 = (wave & 3) * 16, j0 = (wave >> 2) * 16;
+    fx4 ac0 = {0.f, 0.f, 0.f, 0.f}, ac1 = ac0, as0 = ac0, as1 = ac0;
+    for (int s4 = 0; s4 <= nsamh / 4; s4 += 2) {
 #pragma unroll
-      for (int i = 0; i < 4; i += 2) {
-        const int ma = m0 + 4 * lg + i, mb = ma + 1;
-        acc0 = mfma4(Ts[ma * kFP + r0 + la], Ys[ma * kFP + c0 + la], acc0);
-        acc1 = mfma4(Ts[mb * kFP + r0 + la], Ys[mb * kFP + c0 + la], acc1);
+      for (int h = 0; h < 2; h++) {
+        const int m = 4 * (s4 + h) + lg;
+        float p = 0.f, q = 0.f, tc = 0.f, ts = 0.f;
+        if (m <= nsamh) {
+          const float za = Ys[m * kFP + c0 + la];
+          if (m == 0 || m == nsamh) {
+            p = za;
+          } else {
+            const float zb = Ys[(nsam - m) * kFP + c0 + la];
+            p = za + zb;
+            q = za - zb;
+          }
+          tc = Ts[m * kFP + j0 + la];
+          ts = Ts[m * kFP + kFK / 2 + j0 + la];
+        }
+        if (h) {
+          ac1 = mfma4(tc, p, ac1);
+          as1 = mfma4(ts, q, as1);
+        } else {
+          ac0 = mfma4(tc, p, ac0);
+          as0 = mfma4(ts, q, as0);
+        }
       }
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++) Bs[(r0 + 4 * lg + i) * kFP + c0 + la] = acc0[i] + acc1[i];
+    for (int i = 0; i < 4; i++) {
+      Bs[(j0 + 4 * lg + i) * kFP + c0 + la] = ac0[i] + ac1[i];
+      Bs[(kFK / 2 + j0 + 4 * lg + i) * kFP + c0 + la] = as0[i] + as1[i];
+    }
   }
   __syncthreads();
 
@@ -514,21 +665,20 @@ __global__ void __launch_bounds__(kFT)
   // a fixed butterfly: no atomics, the same sums on every run.
   {
     const float2* sp = spec + (size_t)p * ic;
-    const float shx = shift[2 * it], shy = shift[2 * it + 1];
     for (int sh = 0; sh < kCc3mShells; sh++) {
       const int z = sh & 15;
       if ((z < 8 ? z : 15 - z) != wave) continue;
-      const int b0 = pts[sh], b1 = pts[sh + 1];
+      const int b0 = pts_s[sh], b1 = pts_s[sh + 1];
       float q = 0.f, s2 = 0.f, a2 = 0.f;
       for (int e = b0 + lane; e < b1; e += 64) {
-        const int pk = pts[kCc3mPtsHdr + e];
+        const int pk = pts_s[kCc3mPtsHdr + e];
         const int i = pk & 0xff, j = (pk >> 8) - 64;
         int jj = j < 0 ? -j : j;
-        float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
-        float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
+        float cre = Bs[jj * kFP + i], cim = Bs[jj * kFP + kFK / 2 + i];
+        float sre = Bs[(kFK / 2 + jj) * kFP + i], sim = Bs[(kFK / 2 + jj) * kFP + kFK / 2 + i];
         float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
-        float sn, cs;
-        sincosf(shx * (float)i + shy * (float)j, &sn, &cs);
+        const float2 ex = phx[i], ey = phy[j + kFJ];
+        const float cs = ex.x * ey.x - ex.y * ey.y, sn = ex.x * ey.y + ex.y * ey.x;
         float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
         cf samp = cmul(cf{sv.x, sv.y}, cf{cs, sn});
         q += samp.re * a.x + samp.im * a.y;
@@ -589,14 +739,14 @@ __global__ void __launch_bounds__(kFT)
     for (int sh = 0; sh < kCc3mShells; sh++) {
       const int z = sh & 15;
       if ((z < 8 ? z : 15 - z) != wave) continue;
-      const int b0 = pts[sh], b1 = pts[sh + 1];
+      const int b0 = pts_s[sh], b1 = pts_s[sh + 1];
       float s2 = 0.f, a2 = 0.f;
       for (int e = b0 + lane; e < b1; e += 64) {
-        const int pk = pts[kCc3mPtsHdr + e];
+        const int pk = pts_s[kCc3mPtsHdr + e];
         const int i = pk & 0xff, j = (pk >> 8) - 64;
         int jj = j < 0 ? -j : j;
-        float cre = Bs[(2 * jj) * kFP + 2 * i], cim = Bs[(2 * jj) * kFP + 2 * i + 1];
-        float sre = Bs[(2 * jj + 1) * kFP + 2 * i], sim = Bs[(2 * jj + 1) * kFP + 2 * i + 1];
+        float cre = Bs[jj * kFP + i], cim = Bs[jj * kFP + kFK / 2 + i];
+        float sre = Bs[(kFK / 2 + jj) * kFP + i], sim = Bs[(kFK / 2 + jj) * kFP + kFK / 2 + i];
         float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
         float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
         Cs[e] = make_float2(sv.x * a.x + sv.y * a.y, sv.y * a.x - sv.x * a.y);
@@ -649,10 +799,10 @@ __global__ void __launch_bounds__(kFT)
       if ((z < 8 ? z : 15 - z) != wave) continue;
       const float cnt = bins[4 * sh + 3];
       if (cnt == 0.0f) continue;
-      const int b0 = pts[sh], b1 = pts[sh + 1];
+      const int b0 = pts_s[sh], b1 = pts_s[sh + 1];
       float q0 = 0.f, q1 = 0.f;
       for (int e = b0; e < b1; e++) {
-        const int pk = pts[kCc3mPtsHdr + e];
+        const int pk = pts_s[kCc3mPtsHdr + e];
         const int i = pk & 0xff, jj = (pk >> 8) - 64 + kFJ;
         const float2 c = Cs[e];
         {
@@ -733,7 +883,10 @@ bool cc3m_fused_ok(int nsam, int maxr2) {
 
 size_t cc3m_fused_lds(int nsam) {
   const int nb = nsam / 2 + 2 > kCc3mShells ? nsam / 2 + 2 : kCc3mShells;
-  return sizeof(float) * ((size_t)2 * nsam * kFP + 64 * kFP + 4 * nb);
+  // TF, Y/Z, B/out, the shell bins and the band point table (header and at
+  // most 32 x 63 points)
+  return sizeof(float) * ((size_t)(nsam / 2 + 1) * kFP + (size_t)nsam * kFP + 64 * kFP + 4 * nb) +
+         sizeof(int) * (kCc3mPtsHdr + 32 * (2 * kFJ + 1));
 }
 
 // T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N) for i < 32, from
@@ -745,6 +898,20 @@ void cc3m_fused_table(int nsam, std::vector<float>& tab) {
       double ph = 2.0 * 3.14159265358979323846 * (double)((i * n) % nsam) / (double)nsam;
       tab[(size_t)n * kFK + 2 * i] = (float)std::cos(ph);
       tab[(size_t)n * kFK + 2 * i + 1] = (float)-std::sin(ph);
+    }
+}
+
+// The folded table of the kernel: TF[r][i] = cos(2 pi i r/N) and
+// TF[r][32 + i] = -sin(2 pi i r/N) for r = 0 .. N/2, i < 32 (rows N/2+1 .. N-1
+// of T are rows N - r of it, the sine columns negated).
+void cc3m_fused_table_folded(int nsam, std::vector<float>& tab) {
+  const int h = nsam / 2;
+  tab.assign((size_t)(h + 1) * kFK, 0.f);
+  for (int r = 0; r <= h; r++)
+    for (int i = 0; i < kFK / 2; i++) {
+      double ph = 2.0 * 3.14159265358979323846 * (double)((i * r) % nsam) / (double)nsam;
+      tab[(size_t)r * kFK + i] = (float)std::cos(ph);
+      tab[(size_t)r * kFK + kFK / 2 + i] = (float)-std::sin(ph);
     }
 }
 
@@ -788,6 +955,31 @@ int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& df
   }
   ring.resize(acc.size());
   for (size_t e = 0; e < acc.size(); e++) ring[e] = (float)acc[e];
+  return cnt;
+}
+
+// The ring sum of the kernel as a function of B (the column transform's
+// input): sum_{(n,m) in ring} x[n][m] = sum_{c,k} Q[c][k] B^T[c][k] with
+// Q[c][k] = w_c sum_m T[m][k] R[m][c] (w_c = 1 for c < 2, else 2: the c2r
+// weight on Y), R from cc3m_fused_ring; in double, q is 64 x 64 [c][k].
+int cc3m_fused_ring_q(int nsam, float ri2, float ri3, std::vector<float>& q) {
+  std::vector<float> dft, ring;
+  cc3m_fused_table(nsam, dft);
+  const int cnt = cc3m_fused_ring(nsam, ri2, ri3, dft, ring);
+  std::vector<double> t((size_t)nsam * kFK);
+  for (int m = 0; m < nsam; m++)
+    for (int i = 0; i < kFK / 2; i++) {
+      double ph = 2.0 * 3.14159265358979323846 * (double)((i * m) % nsam) / (double)nsam;
+      t[(size_t)m * kFK + 2 * i] = std::cos(ph);
+      t[(size_t)m * kFK + 2 * i + 1] = -std::sin(ph);
+    }
+  q.assign((size_t)kFK * kFK, 0.f);
+  for (int c = 0; c < kFK; c++)
+    for (int k = 0; k < kFK; k++) {
+      double acc = 0.0;
+      for (int m = 0; m < nsam; m++) acc += t[(size_t)m * kFK + k] * (double)ring[(size_t)m * kFK + c];
+      q[(size_t)c * kFK + k] = (float)((c < 2 ? 1.0 : 2.0) * acc);
+    }
   return cnt;
 }
 

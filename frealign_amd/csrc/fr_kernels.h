@@ -119,11 +119,15 @@ int launch_sigma(Ctx& c, int nsam, int n, const float* a3df, const SigmaItem* it
 bool cc3m_fused_ok(int nsam, int maxr2);
 size_t cc3m_fused_lds(int nsam);
 // Constant tables of the fused kernel (device pointers, owned by the context):
-//   dft  T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N), i < 32;
+//   dft  the folded DFT table TF[r][i] = cos(2 pi i r/N), TF[r][32+i] =
+//        -sin(2 pi i r/N), r = 0..N/2, i < 32 (T[n][2i] = cos(2 pi i n/N),
+//        T[n][2i+1] = -sin(2 pi i n/N) for every n is read from it);
 //   pts  kCc3mShells+1 shell offsets, then the band points (I | (J+64) << 8)
 //        sorted by shell INT(SQRT(I*I+J*J)+0.5), (I, J) order within a shell;
-//   ring R[m][k] = sum over MASKCOS2 ring pixels n of row m of T[n][k], for
-//        AMAG = 1 (null: the ring mean is taken by a pass on the matrix cores);
+//   ring Q[c][k] = w_c sum_m T[m][k] R[m][c], R[m][k] the sum over the
+//        MASKCOS2 ring pixels n of row m of T[n][k] (w_c the c2r weight): the
+//        ring sum is sum Q B^T, for AMAG = 1 (null: the ring mean is taken by a
+//        pass on the matrix cores);
 //   wmap the MASKCOS2 weight of every pixel [m][n] at AMAG = 1 (null:
 //        evaluated per pixel).
 constexpr int kCc3mShells = 34;
@@ -136,9 +140,11 @@ struct Cc3mTables {
   const float* wmap = nullptr;
 };
 void cc3m_fused_table(int nsam, std::vector<float>& tab);
+void cc3m_fused_table_folded(int nsam, std::vector<float>& tab);
 void cc3m_fused_points(int maxr12, int maxr22, std::vector<int>& pts);
 int cc3m_fused_ring(int nsam, float ri2, float ri3, const std::vector<float>& dft,
                     std::vector<float>& ring);
+int cc3m_fused_ring_q(int nsam, float ri2, float ri3, std::vector<float>& q);
 void cc3m_fused_wmap(int nsam, float ri2, float ri3, float rih, float halfw,
                      std::vector<float>& w);
 // A window of nsx x nsy shifts (odd sizes up to kCc3mWinMax) centred on each
