@@ -815,6 +815,29 @@ class Context:
         fn = N.lib().fr_fft3d_r2c if forward else N.lib().fr_fft3d_c2r
         N.check(fn(self.h, nsam, _p(vol)), "fr_fft3d")
 
+    def find_fpart(self, nsam: int, std: float, vtd: float, a, d, s, v, iredun: int, ri: float,
+                   ric: float, psize: float, mw: float, dalt: float, fsc, pssnr):
+        """FIND_FPART (fr_find_fpart): the particle fraction of the Wiener
+        filter from the half-maps (the MW = 0 search, or the MW closed form);
+        returns (FPART, PSSNR / FPART).  a, d: complex half-volumes; s, v:
+        their CTF^2 sums; fsc, pssnr: SHELTEST's shells."""
+        for t, nm, dt in ((a, "a", torch.complex64), (d, "d", torch.complex64),
+                          (s, "s", torch.float32), (v, "v", torch.float32)):
+            _dev(t, dt, nm)
+        jc = nsam // 2 + 1
+        n = max(len(fsc), len(pssnr), jc)
+        fs = np.zeros(n, np.float32)
+        fs[:len(fsc)] = fsc
+        ps = np.zeros(n, np.float32)
+        ps[:len(pssnr)] = pssnr
+        fp = C.c_float(0.0)
+        self._sync_stream()
+        N.check(N.lib().fr_find_fpart(self.h, nsam, float(std), float(vtd), _p(a), _p(d), _p(s),
+                                      _p(v), int(iredun), float(ri), float(ric), float(psize),
+                                      float(mw), float(dalt), fs.ctypes.data, ps.ctypes.data, n,
+                                      C.byref(fp)), "fr_find_fpart")
+        return fp.value, ps[:len(pssnr)]
+
     def mask3d(self, nsam: int, a3dv, rio: float, ric: float, halfw: float):
         """MASK3D (mask3d.f) in place on the padded real volume; returns
         (FMASK, AVE, STD)."""

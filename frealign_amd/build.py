@@ -37,6 +37,7 @@ SOURCES = [
     "fr_local.hip",
     "fr_ctfrefine.cpp",
     "fr_io.cpp",
+    "fr_fpart.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -310,6 +310,8 @@ __global__ void __launch_bounds__(kFT)
     const float* d = dm + 9 * (size_t)it;
     const float2* cp = ctff + (size_t)p * 2 * ic;
     const float scal = 1.0f / (float)nsam / (float)nsam;
+    // unrolled: the gathers of a thread's points are in flight together
+#pragma unroll 4
     for (int e = tid; e < 32 * (2 * kFJ + 1); e += kFT) {
       int i = e / (2 * kFJ + 1), j = e % (2 * kFJ + 1) - kFJ;
       int itemp = i * i + j * j;
@@ -670,6 +672,7 @@ __global__ void __launch_bounds__(kFT)
       if ((z < 8 ? z : 15 - z) != wave) continue;
       const int b0 = pts_s[sh], b1 = pts_s[sh + 1];
       float q = 0.f, s2 = 0.f, a2 = 0.f;
+#pragma unroll 4
       for (int e = b0 + lane; e < b1; e += 64) {
         const int pk = pts_s[kCc3mPtsHdr + e];
         const int i = pk & 0xff, j = (pk >> 8) - 64;

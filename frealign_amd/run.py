@@ -365,16 +365,17 @@ class Run:
             fileio.mrc_write(deck.map2, halves[1], psize, title)
         st = ctx.sheltest(nsam, b, c, vols.ctf2[0], vols.ctf2[1], fmask=fmask)
         fsc = st["fsc"].astype(np.float32)
-        fpart = 0.0
-        if deck.ffilt and deck.mw > 0:  # FIND_FPART's MW closed form (find_fpart.f:141)
-            import ctypes as C
-
-            from . import _native as N
-            ss = np.zeros(nsam // 2 + 3, np.float32)
-            fp = C.c_float(0.0)
-            N.check(N.lib().fr_fpart_from_mw(nsam, float(deck.mw), 0.81, float(psize),
-                                             ss.ctypes.data, C.byref(fp)), "fr_fpart_from_mw")
-            fpart = fp.value
+        # FIND_FPART (frealign_v9.f:871-874; DALT 0.81): FPART and PSSNR / FPART
+        # (the Part_SSNR of the table); the Wiener filter uses FPART only
+        # with FFILT (frealign_v9.f:888)
+        fpart, pssnr = ctx.find_fpart(nsam, std, vtd, vols.num[0], vols.num[1], vols.ctf2[0],
+                                      vols.ctf2[1], iredun, abs(self.ri), deck.ri / psize,
+                                      psize, deck.mw, 0.81, fsc, st["pssnr"])
+        self.log(f" Fraction_mask, Fraction_particle = {fmask:10.4f}{fpart:10.4f}")
+        st["pssnr"] = pssnr
+        self.fpart = fpart
+        if not deck.ffilt:
+            fpart = 0.0
         a = vols.num[0].clone()
         s = vols.ctf2[0].clone()
         ctx.shiftvol(nsam, std, vtd, self.rlim, iredun, a, vols.num[1], s, vols.ctf2[1], 0,
@@ -394,8 +395,7 @@ class Run:
         self.log(f" frealign_amd: FSC(0.143) at {res:.2f} A")
         self.fsc = fsc
         return fileio.fsc_table(nsam, psize, fsc, st["npixt"], st["pssnr"], npix=st["npix"],
-                                rlim=self.rlim,
-                                fmask=fmask, fpart=fpart)
+                                rlim=self.rlim, fmask=fmask, fpart=fpart)
 
 
 def _runs(films):

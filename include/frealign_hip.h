@@ -179,6 +179,25 @@ int fr_combine_arrays(fr_ctx *ctx, int nsam, long long nn3, int nvol1, int nvol2
                       const long long *nksum1, const double *ttd1, const double *ttd2,
                       long long *nksum, double *std_, double *vtd);
 
+/* Replaces FIND_FPART (find_fpart.f:8-150) with FPART_CC (fpart_cc.f:8-94)
+ * and VOLMEASURE (volmeasure.f:8-69): the particle fraction FPART of the
+ * Wiener filter.  mw != 0: the closed form 1000 MW / DALT / (PSIZE NSAM)^3.
+ * mw == 0: the search, on the device -- B = A/(S + STD IREDUN) and the
+ * low-passed C = (A+D)/(S+V + (STD+VTD) IREDUN) to real space, MASK3D of C
+ * (RI, RIC, HALFW 1), the voxels above the first threshold MAVE + (1.5 -
+ * 0.1K) MSTD that holds more than 0.2 N^3 FMASK of them, VOLMEASURE's start
+ * and FPART_CC's correlation of B with D/(V + FPART/PSSNR) over them,
+ * stepped by 0.2, 0.067, ... while it rises.  The half-maps d_a3df, d_d3df
+ * (complex), d_s3df, d_v3df are read only; fsc and pssnr are host arrays of
+ * nshell >= NSAM/2+1 shells (SHELTEST's); pssnr is divided by FPART in place
+ * (find_fpart.f:143-149) and *fpart returned.  STD or VTD zero is an error
+ * (find_fpart.f:37-43).  VOLMEASURE's class sums are taken in double where
+ * the reference adds in FP32. */
+int fr_find_fpart(fr_ctx *ctx, int nsam, double std_, double vtd, const float *d_a3df,
+                  const float *d_d3df, const float *d_s3df, const float *d_v3df, int iredun,
+                  float ri, float ric, float psize, float mw, float dalt, const float *fsc,
+                  float *pssnr, int nshell, float *fpart);
+
 /* Replaces SHIFTVOL (shiftvol.f:8-77, NNSTAT=0), in place: *std_ += vtd; for
  * ifsc == 0 A += D and S += V; then within INT(NSAM*RLIM) A = A(-1)^(l+m+n) /
  * (S + STD*IREDUN) (fpart == 0) or that / (1 + FPART/A1/FMASK) with
