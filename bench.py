@@ -92,10 +92,28 @@ TOPK = 32
 PEAK_BF16 = 2.5e15   # MI355X dense BF16 MFMA, FLOP/s
 PEAK_FP32_MFMA = 157.3e12
 PEAK_HBM = 8.0e12
-# FP32 MFMA work of one fused CC3M item at 256^2 with MAXR2 = 32 (16x16x4
-# products of the band-pruned transforms: column 1024 + rows 16 x 16 x 32 +
-# column 1024, 2048 flops each); the ring-mean pass is not counted
-CC3M_FLOPS = (1024 + 16 * 16 * 32 + 1024) * 2048.0
+
+
+def cc3m_issued_mfma(nsam, ri3):
+    """16x16x4 FP32 MFMA issued per fused CC3M item with MAXR2 = 32 (the
+    band-pruned transforms, folded over the cos/sin symmetry of the DFT
+    table): the column c2c takes (N/2 / 16) x 4 folded tiles of 16 products,
+    the row pass 32 per folded tile (16 forward, 16 back) for each 16-row
+    block and each tile that reaches inside the mask radius^2 ri3 (the
+    kernel's own test), the column r2c 8 waves x (N/4 / 2 + 1) x 4; 2048
+    flops each.  The ring-mean pass (AMAG != 1 only) is not counted."""
+    h = nsam // 2
+
+    def lo(v0):
+        a = v0 if v0 < h else v0 - nsam
+        b = a + 15
+        return 0 if a <= 0 <= b else min(a * a, b * b)
+    rows = sum(1 for rb in range(nsam // 16) for nt in range(h // 16)
+               if not lo(16 * nt) + lo(16 * rb) > ri3)
+    return (h // 16) * 4 * 16 + rows * 32 + 8 * (nsam // 8 + 1) * 4
+
+
+CC3M_FLOPS = cc3m_issued_mfma(NSAM, (RI + 3.0) ** 2) * 2048.0
 # the algorithmic unit of one CC3M evaluation: its two NSAM^2 FFTs (the c2r
 # and r2c of cc3m.f:113-115), 5 N^2 log2(N^2) flops each
 CC3M_FFT_FLOPS = 2 * 5.0 * NSAM * NSAM * math.log2(NSAM * NSAM)
