@@ -17,7 +17,11 @@
 // inverse Gram matrix of the plane's axes -- and re-derives for each the
 // reference's FP32 coordinates and voxel range (same operation order, no FMA
 // contraction), so exactly the scatter's (point, voxel) pairs contribute,
-// with bit-identical terms.  Sums run in a fixed order: the volumes, NKSUM
+// with bit-identical terms (the voxel-range test is done as two float
+// comparisons per axis, equivalent to the reference's INT() rules; the J of a
+// candidate row are bounded per axis by a form linear in I).  Tiles near the
+// origin, which every plane crosses, are dispatched first so that they do not
+// trail the launch.  Sums run in a fixed order: the volumes, NKSUM
 // and TTD are deterministic, and no atomics are needed.  The per-point terms
 // (sample x phase ramp x CTF x weight, and CTF^2 x weight) depend only on the
 // particle, so they are formed once per particle.
@@ -51,8 +55,14 @@ struct InsertGeo {
   float h2x, h2y, h2z, r2;  // J = h2 . P; r2 = |h2|_1
   float rmax2;              // squared reach of the plane's points (+ margin)
   int valid;
-  float pad_[2];
+  int zaxes;                // bit k: |b_k| too small for 1/b_k (axis k bounds no J)
+  float pad_;
   float ibx, iby, ibz, pad2_;  // 1/b per axis (0 where |b| is tiny)
+  // per axis k, the J of a point I a + J b within 1.02 of coordinate v_k:
+  // J in v_k / b_k - I a_k / b_k -+ 1.02 / |b_k|
+  float sx, sy, sz;         // a_k / b_k
+  float ox, oy, oz;         // 1.02 / |b_k| (3e38 where b_k ~ 0)
+  float pad3_[2];
 };
 
 __host__ __device__ inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -191,11 +201,21 @@ __global__ void __launch_bounds__(kThreads)
   const double r2pts = t.irrec2 < 2.0 * jcm * jcm ? (double)t.irrec2 : 2.0 * jcm * jcm;
   const double reach = sqrt(r2pts * lmax) + 1.7320508 + 0.01;
   g.rmax2 = (float)(reach * reach);
-  g.pad_[0] = g.pad_[1] = 0.f;
-  g.ibx = fabs(bx) > 1e-6 ? (float)(1.0 / bx) : 0.f;
-  g.iby = fabs(by) > 1e-6 ? (float)(1.0 / by) : 0.f;
-  g.ibz = fabs(bz) > 1e-6 ? (float)(1.0 / bz) : 0.f;
+  g.pad_ = 0.f;
+  const bool zx = !(fabs(bx) > 1e-6), zy = !(fabs(by) > 1e-6), zz = !(fabs(bz) > 1e-6);
+  g.zaxes = (zx ? 1 : 0) | (zy ? 2 : 0) | (zz ? 4 : 0);
+  g.ibx = zx ? 0.f : (float)(1.0 / bx);
+  g.iby = zy ? 0.f : (float)(1.0 / by);
+  g.ibz = zz ? 0.f : (float)(1.0 / bz);
   g.pad2_ = 0.f;
+  g.sx = zx ? 0.f : (float)(ax / bx);
+  g.sy = zy ? 0.f : (float)(ay / by);
+  g.sz = zz ? 0.f : (float)(az / bz);
+  // an axis with b_k ~ 0 leaves J free: c_k = s_k = 0 and an unbounded o_k
+  g.ox = zx ? 3.0e38f : (float)(1.02 / fabs(bx));
+  g.oy = zy ? 3.0e38f : (float)(1.02 / fabs(by));
+  g.oz = zz ? 3.0e38f : (float)(1.02 / fabs(bz));
+  g.pad3_[0] = g.pad3_[1] = 0.f;
   geo[k] = g;
 }
 
@@ -246,6 +266,26 @@ __device__ inline bool plane_reaches(const InsertGeo& g, float xl, float xh, flo
   return cx * cx + cy * cy + cz * cz <= g.rmax2;
 }
 
+// pinsert_trilin.f:88-111 sets, per axis, S = INT(A1) (minus 1 when A1 < 0)
+// with A1 = (X3 + 0.5) - 0.5 and T = INT(A2) (minus 1 when A2 < 0) with
+// A2 = X3 + 1, clamped to the axis's range, and fills the voxels S..T.  For
+// an integer coordinate v inside the range, v >= S exactly when A1 < v + 1
+// (A1 <= v + 1 when v + 1 < 0), and v <= T exactly when A2 >= v (A2 > v when
+// v < 0); the clamps never exclude such a v.  The strict forms are the
+// non-strict ones against the next float toward +infinity, so each bound is
+// one comparison: A1 < range_lo(v) and A2 >= range_hi(v).
+__device__ inline float next_up_neg(float x) {  // x < 0
+  return __int_as_float(__float_as_int(x) - 1);
+}
+__device__ inline float range_lo(int v) {
+  const float f = (float)(v + 1);
+  return v + 1 >= 0 ? f : next_up_neg(f);
+}
+__device__ inline float range_hi(int v) {
+  const float f = (float)v;
+  return v >= 0 ? f : next_up_neg(f);
+}
+
 // pdiff.f:8-24: |phase of b conj(r)|, 0 where the product is 0
 __device__ inline float pdiff_dev(float br, float bi, float2 r) {
   const float cr = br * r.x + bi * r.y, ci = bi * r.x - br * r.y;
@@ -255,16 +295,22 @@ __device__ inline float pdiff_dev(float br, float bi, float2 r) {
 
 template <bool kStats>
 __global__ void __launch_bounds__(kGatherThreads)
-    k_insert_gather(int nsam, int nb, int ntask, int tx, int ty, const InsertTask* __restrict__ tasks,
-                    const InsertGeo* __restrict__ geo, const float4* __restrict__ pts,
-                    float2* num0, float* ctf2_0, float2* num1, float* ctf2_1,
-                    unsigned long long* pcount, double* pttd, InsertStats st) {
+    k_insert_gather(int nsam, int nb, int ntask, int tx, int ty,
+                    const InsertTask* __restrict__ tasks, const InsertGeo* __restrict__ geo,
+                    const float4* __restrict__ pts, float2* num0, float* ctf2_0, float2* num1,
+                    float* ctf2_1, unsigned long long* pcount, double* pttd, InsertStats st) {
   __shared__ int list[kChunk];
   __shared__ int wave_tot[kGatherThreads / 64];
-  const int jc = nsam / 2 + 1, nsamh = nsam / 2, w = 2 * jc - 1;
+  const int jc = nsam / 2 + 1, w = 2 * jc - 1;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int b = blockIdx.x;
-  const int l0 = (b % tx) * kTile, m0 = ((b / tx) % ty) * kTile, n0 = (b / (tx * ty)) * kTile;
+  // tiles go out slab by slab in order of |logical n|, rows in order of
+  // |logical m| (storage rows 0, last, 1, last - 1, ...): the tiles near the
+  // origin, which every plane crosses, start first instead of last
+  const int tmf = (b / tx) % ty, tnf = b / (tx * ty);
+  const int l0 = (b % tx) * kTile;
+  const int m0 = ((tmf & 1) ? ty - 1 - (tmf >> 1) : (tmf >> 1)) * kTile;
+  const int n0 = ((tnf & 1) ? ty - 1 - (tnf >> 1) : (tnf >> 1)) * kTile;
   // the tile's logical boxes (a wrapped axis may split in two)
   const int l1 = min(l0 + kTile - 1, jc - 1);
   int mlo[2], mhi[2], nlo[2], nhi[2];
@@ -280,6 +326,8 @@ __global__ void __launch_bounds__(kGatherThreads)
   int mv[2], nv[2];
   const int nm = mine ? logical_axis(ms_, nsam, mv) : 0;
   const int nn = mine ? logical_axis(ns_, nsam, nv) : 0;
+  // the voxel's range tests as thresholds on X3 (see range_lo, range_hi)
+  const float ltl = range_lo(l), utl = range_hi(l);
   float a0r = 0.f, a0i = 0.f, a0c = 0.f, a1r = 0.f, a1i = 0.f, a1c = 0.f;
   unsigned c0 = 0u, c1 = 0u;
   double t0 = 0.0, t1 = 0.0;
@@ -340,52 +388,56 @@ __global__ void __launch_bounds__(kGatherThreads)
             ihi = min(ihi, (int)t.ihi);
             jlo = max(jlo, -(jc - 1));
             jhi = min(jhi, jc - 1);
+            // per axis, the J interval of the points within 1.02 of the
+            // voxel, linear in I: [c_k - I s_k - o_k, c_k - I s_k + o_k]
+            // (an axis with b_k ~ 0 bounds no J: tested per I below)
+            const float cx = (g.zaxes & 1) ? 0.f : fl * g.ibx;
+            const float cy = (g.zaxes & 2) ? 0.f : fm * g.iby;
+            const float cz = (g.zaxes & 4) ? 0.f : fn * g.ibz;
+            const float ltm_ = range_lo(m), utm_ = range_hi(m), ltn_ = range_lo(n), utn_ = range_hi(n);
             for (int i = ilo; i <= ihi; i++) {
-              // the J of this I whose point lies in the voxel's reach
-              // |I a + J b - v| <= 1 on every axis (with margin)
-              float jl = (float)jlo, jh = (float)jhi;
-              bool none = false;
               const float fi = (float)i;
-              auto axis = [&](float vk, float ak, float bk, float ibk) {
-                const float o = vk - fi * ak;
-                if (ibk != 0.f) {
-                  float e1 = (o - 1.02f) * ibk, e2 = (o + 1.02f) * ibk;
-                  if (e1 > e2) { float tt = e1; e1 = e2; e2 = tt; }
-                  jl = fmaxf(jl, e1);
-                  jh = fminf(jh, e2);
-                } else if (fabsf(o) > 1.02f + 1e-3f * fabsf(bk) * (float)(jc)) {
+              const float ex = cx - fi * g.sx, ey = cy - fi * g.sy, ez = cz - fi * g.sz;
+              const float jl = fmaxf(fmaxf((float)jlo, ex - g.ox), fmaxf(ey - g.oy, ez - g.oz));
+              const float jh = fminf(fminf((float)jhi, ex + g.ox), fminf(ey + g.oy, ez + g.oz));
+              if (g.zaxes) {
+                bool none = false;
+                const float slack = 1e-3f * (float)jc;
+                if ((g.zaxes & 1) && fabsf(fl - fi * t.dm[0]) > 1.02f + slack * fabsf(t.dm[3]))
                   none = true;
-                }
-              };
-              axis(fl, t.dm[0], t.dm[3], g.ibx);
-              axis(fm, t.dm[1], t.dm[4], g.iby);
-              axis(fn, t.dm[2], t.dm[5], g.ibz);
-              if (none) continue;
-              const int j0 = (int)ceilf(jl - 0.01f), j1 = (int)floorf(jh + 0.01f);
+                if ((g.zaxes & 2) && fabsf(fm - fi * t.dm[1]) > 1.02f + slack * fabsf(t.dm[4]))
+                  none = true;
+                if ((g.zaxes & 4) && fabsf(fn - fi * t.dm[2]) > 1.02f + slack * fabsf(t.dm[5]))
+                  none = true;
+                if (none) continue;
+              }
+              const int j0 = max((int)ceilf(jl - 0.01f), jlo);
+              const int j1 = min((int)floorf(jh + 0.01f), jhi);
+              float xi, yi, zi;
+              {
+#pragma clang fp contract(off)
+                xi = t.dm[0] * fi;
+                yi = t.dm[1] * fi;
+                zi = t.dm[2] * fi;
+              }
+              const int ii = i * i;
               for (int j = j0; j <= j1; j++) {
+                // pinsert_trilin.f:88-111 in the reference's FP32 order: the
+                // point's X3 and whether this voxel is in its range
                 float x3, y3, z3;
-                int ls, lt, ms, mt, ns, nt;
-                bool inside;
+                bool in;
                 {
 #pragma clang fp contract(off)
-                  float g2 = (float)(i * i + j * j);
-                  inside = g2 <= (float)t.irrec2;
-                  float fi = (float)i, fj = (float)j;
-                  float dm1 = t.dm[0] * fi, dm2 = t.dm[1] * fi, dm3 = t.dm[2] * fi;
-                  x3 = dm1 + t.dm[3] * fj;
-                  y3 = dm2 + t.dm[4] * fj;
-                  z3 = dm3 + t.dm[5] * fj;
-                  // pinsert_trilin.f:88-111: A=X3+0.5-RI12 (RI12=0.5), A=X3+RI2 (RI2=1)
-                  float a;
-                  a = (x3 + 0.5f) - 0.5f; ls = (int)a; if (a < 0.0f) ls--; if (ls < 0) ls = 0;
-                  a = x3 + 1.0f;          lt = (int)a; if (a < 0.0f) lt--; if (lt > nsamh) lt = nsamh;
-                  a = (y3 + 0.5f) - 0.5f; ms = (int)a; if (a < 0.0f) ms--; if (ms < -nsamh) ms = -nsamh;
-                  a = y3 + 1.0f;          mt = (int)a; if (a < 0.0f) mt--; if (mt > nsamh) mt = nsamh;
-                  a = (z3 + 0.5f) - 0.5f; ns = (int)a; if (a < 0.0f) ns--; if (ns < -nsamh) ns = -nsamh;
-                  a = z3 + 1.0f;          nt = (int)a; if (a < 0.0f) nt--; if (nt > nsamh) nt = nsamh;
+                  const float fj = (float)j;
+                  x3 = xi + t.dm[3] * fj;
+                  y3 = yi + t.dm[4] * fj;
+                  z3 = zi + t.dm[5] * fj;
+                  in = ((x3 + 0.5f) - 0.5f < ltl) & (x3 + 1.0f >= utl) &
+                       ((y3 + 0.5f) - 0.5f < ltm_) & (y3 + 1.0f >= utm_) &
+                       ((z3 + 0.5f) - 0.5f < ltn_) & (z3 + 1.0f >= utn_);
                 }
-                if (!inside || l < ls || l > lt || m < ms || m > mt || n < ns || n > nt) continue;
-                if (j < jlo || j > jhi) continue;
+                // I^2 + J^2 <= IRREC^2 (exact in FP32 at these sizes)
+                if (!in || ii + j * j > t.irrec2) continue;
                 const float b3 = 1.0f - fabsf(z3 - fn);
                 const float b2 = (1.0f - fabsf(y3 - fm));
                 const float box = (1.0f - fabsf(x3 - fl)) * b2 * b3;

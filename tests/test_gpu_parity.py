@@ -272,8 +272,15 @@ def _oracle_insert(L, pre, nsam, datc, ctff, recb, half, sym, pbc, boff, psize):
     return num, c2, [t.value for t in ttd], [k.value for k in nk]
 
 
-@pytest.mark.parametrize("nsam,asym,nasym", [(32, "C", 1), (64, "C", 1), (64, "D", 7)])
-def test_insert_vs_oracle(ctx, nsam, asym, nasym):
+@pytest.mark.parametrize("nsam,asym,nasym,aligned", [(32, "C", 1, False), (64, "C", 1, False),
+                                                     (64, "D", 7, False), (64, "C", 1, True),
+                                                     (40, "D", 4, True)])
+def test_insert_vs_oracle(ctx, nsam, asym, nasym, aligned):
+    """PINSERT for a batch against the oracle: NKSUM and the voxel set exact,
+    the sums to FP32 rounding.  `aligned`: planes along the axes and on the
+    diagonals (in-plane axes with zero components) at magnifications other
+    than 1, so that points fall on the voxel grid and the plane geometry is
+    not orthonormal."""
     jc = nsam // 2 + 1
     L = O.oracle()
     nb = 6
@@ -290,6 +297,10 @@ def test_insert_vs_oracle(ctx, nsam, asym, nasym):
     recb[:, 7] = rng.uniform(0.5, 2.0, nb)
     recb[:, 8] = rng.uniform(0.2, 0.8, nb)
     recb[:, 9] = 0.5 if nsam < 64 else 0.45
+    if aligned:
+        q = np.float32(np.pi / 2)
+        recb[:, 0:3] = [[0, 0, 0], [q, 0, 0], [0, q, 0], [q, q, 0], [0, q, q], [q / 2, q, 0]]
+        recb[:, 5] = [1.0, 0.97, 1.03, 0.95, 1.05, 1.0]
     half = (np.arange(nb) % 2).astype(np.int32)
     sym = api.point_group(asym, nasym)
     pbc, boff, psize = 5.0, 40.0, 1.5
