@@ -289,11 +289,11 @@ module frealign_hip
 
     ! PSEARCH over per-particle orientation lists d_cand(d_loff(p)+1 : d_loff(p+1))
     function fr_search_ragged_batch(ctx, nsam, maxr1, maxr2, rbfact, norient, d_bank, dtype, &
-                                    np, d_outc, d_ctff, d_loff, d_grp, d_cand, cap, topk, &
+                                    np, d_outc, d_ctff, d_loff, d_grp, d_cand, d_info, cap, topk, &
                                     d_top_o, d_top_row, d_top_score) &
         bind(C, name="fr_search_ragged_batch")
       import :: c_int, c_float, c_ptr
-      type(c_ptr), value :: ctx, d_bank, d_outc, d_ctff, d_loff, d_grp, d_cand
+      type(c_ptr), value :: ctx, d_bank, d_outc, d_ctff, d_loff, d_grp, d_cand, d_info
       type(c_ptr), value :: d_top_o, d_top_row, d_top_score
       integer(c_int), value :: nsam, maxr1, maxr2, norient, dtype, np, cap, topk
       real(c_float), value :: rbfact
@@ -353,6 +353,201 @@ module frealign_hip
       real(c_float), value :: rbfact
       real(c_float), intent(in) :: ring(4)
       integer(c_int) :: fr_rescore_batch
+    end function
+    ! PINSERT_TRILIN_SE with NNSTAT = 1 (pinsert_trilin_se.f:186-194): as
+    ! fr_insert_batch plus the per-voxel KSUM, VSUM, PSUM, ASUM
+    function fr_insert_batch_se(ctx, nsam, nb, d_datc, d_ctff, recb, half, nsym, jsym, symop, &
+                                pbc, boff, psize, d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, &
+                                d_ttd, d_c3df, d_ksum, d_vsum, d_psum, d_asum) &
+        bind(C, name="fr_insert_batch_se")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_datc, d_ctff, d_num0, d_ctf2_0, d_num1, d_ctf2_1, d_nksum, d_ttd
+      type(c_ptr), value :: d_c3df, d_ksum, d_vsum, d_psum, d_asum
+      integer(c_int), value :: nsam, nb, nsym
+      real(c_float) :: recb(10, *), symop(3, 3, *)
+      integer(c_int) :: half(*), jsym(*)
+      real(c_float), value :: pbc, boff, psize
+      integer(c_int) :: fr_insert_batch_se
+    end function
+
+    ! SHELTEST with its accumulators in a fixed device array (no allocation)
+    function fr_sheltest_noalloc(ctx, nsam, istp, d_a, d_d, d_s, d_v, fmask, npix, npixt, dfsc, &
+                                 pssnr, nstp) bind(C, name="fr_sheltest_noalloc")
+      import :: c_int, c_float, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a, d_d, d_s, d_v
+      integer(c_int), value :: nsam, istp
+      real(c_float), value :: fmask
+      integer(c_int) :: npix(*), npixt(*), nstp
+      real(c_double) :: dfsc(*)
+      real(c_float) :: pssnr(*)
+      integer(c_int) :: fr_sheltest_noalloc
+    end function
+
+    ! FIND_FPART (find_fpart.f:8-150, fpart_cc.f, volmeasure.f); pssnr in/out
+    function fr_find_fpart(ctx, nsam, std, vtd, d_a3df, d_d3df, d_s3df, d_v3df, iredun, ri, ric, &
+                           psize, mw, dalt, fsc, pssnr, nshell, fpart) bind(C, name="fr_find_fpart")
+      import :: c_int, c_float, c_double, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_d3df, d_s3df, d_v3df
+      integer(c_int), value :: nsam, iredun, nshell
+      real(c_double), value :: std, vtd
+      real(c_float), value :: ri, ric, psize, mw, dalt
+      real(c_float) :: fsc(*), pssnr(*), fpart
+      integer(c_int) :: fr_find_fpart
+    end function
+
+    ! CC3 (cc3.f:8-159), the score of CTFREFINE and MAGREFINE
+    function fr_cc3_batch(ctx, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, &
+                          maxr1, maxr2, rbfact, d_fsct, ring, d_score) bind(C, name="fr_cc3_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, d_fsct, &
+                            d_score
+      integer(c_int), value :: nsam, n, maxr1, maxr2
+      real(c_float), value :: rbfact
+      real(c_float) :: ring(4)
+      integer(c_int) :: fr_cc3_batch
+    end function
+
+    ! CTFREFINE (ctfrefine.f:8-100) and MAGREFINE (magrefine.f:8-73) per group
+    function fr_ctfrefine_batch(ctx, nsam, np, d_spec, ppar, euler, ng, gstart, fastig, dfstd, &
+                                sig2, d_a3df, d_pssnr, gpar, rmax1, rmax2, rbfact, ring, out, &
+                                nevals) bind(C, name="fr_ctfrefine_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_spec, d_a3df, d_pssnr
+      integer(c_int), value :: nsam, np, ng, fastig
+      real(c_float), value :: dfstd, rmax1, rmax2, rbfact
+      real(c_float) :: ppar(8, *), euler(3, *), sig2(*), gpar(11), ring(4), out(4, *)
+      integer(c_int) :: gstart(*), nevals(*)
+      integer(c_int) :: fr_ctfrefine_batch
+    end function
+
+    function fr_magrefine_batch(ctx, nsam, np, d_spec, ppar, euler, ng, gstart, d_a3df, d_pssnr, &
+                                gpar, rmax1, rmax2, rbfact, ring, out, nevals) &
+        bind(C, name="fr_magrefine_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_spec, d_a3df, d_pssnr
+      integer(c_int), value :: nsam, np, ng
+      real(c_float), value :: rmax1, rmax2, rbfact
+      real(c_float) :: ppar(8, *), euler(3, *), gpar(11), ring(4), out(4, *)
+      integer(c_int) :: gstart(*), nevals(*)
+      integer(c_int) :: fr_magrefine_batch
+    end function
+
+    ! mode-1 local grid search before PREFINE (lmain.f:743-828); par(6, np) in/out
+    function fr_local_search_batch(ctx, nsam, np, d_a3df, d_spec, d_ctff, amag, par, nang, dang, &
+                                   nsh, dsh, maxr1, maxr2, rbfact, d_fsct, ring, sm, d_scores, &
+                                   d_codes) bind(C, name="fr_local_search_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_fsct, sm, d_scores, d_codes
+      integer(c_int), value :: nsam, np, nang, nsh, maxr1, maxr2
+      real(c_float), value :: dang, dsh, rbfact
+      real(c_float) :: amag(*), par(6, *), ring(4)
+      integer(c_int) :: fr_local_search_batch
+    end function
+
+    ! reference-exact PSEARCH -> CCP with PAD and ROTATE2D (psearch.f:8-121,
+    ! ccp.f:8-275); sm may be c_null_ptr (identity)
+    function fr_psearch(ctx, nsam, np, amag, d_a3df, d_spec, d_ctff, dangin, maxr1, maxr2, par, &
+                        sang, nsang, iquadmax, mask, rbfact, testpar, ipmax, iran, sm, asym) &
+        bind(C, name="fr_psearch")
+      import :: c_int, c_float, c_ptr, c_char
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, sm
+      integer(c_int), value :: nsam, np, maxr1, maxr2, nsang, iquadmax, ipmax
+      real(c_float), value :: dangin, rbfact
+      real(c_float) :: amag(*), par(5, *), sang(3, *), testpar(6, ipmax, *)
+      integer(c_int) :: mask(5), iran
+      character(kind=c_char), value :: asym
+      integer(c_int) :: fr_psearch
+    end function
+
+    function fr_ccp_view_batch(ctx, nsam, np, nview, d_a3df, d_spec, d_ctff, d_dm, maxr1, maxr2, &
+                               rbfact, dang, iquadmax, mask, d_score, d_code) &
+        bind(C, name="fr_ccp_view_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_spec, d_ctff, d_dm, d_score, d_code
+      integer(c_int), value :: nsam, np, nview, maxr1, maxr2, iquadmax
+      real(c_float), value :: rbfact, dang
+      integer(c_int) :: mask(5)
+      integer(c_int) :: fr_ccp_view_batch
+    end function
+
+    ! PSEARCH over lists of one length (ncand per particle, in groups)
+    function fr_search_list_batch(ctx, nsam, maxr1, maxr2, rbfact, norient, d_bank, dtype, np, &
+                                  d_outc, d_ctff, ncand, d_cand, group, topk, d_top_o, d_top_row, &
+                                  d_top_score) bind(C, name="fr_search_list_batch")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_bank, d_outc, d_ctff, d_cand, d_top_o, d_top_row, d_top_score
+      integer(c_int), value :: nsam, maxr1, maxr2, norient, dtype, np, ncand, group, topk
+      real(c_float), value :: rbfact
+      integer(c_int) :: fr_search_list_batch
+    end function
+
+    ! learned proposal: parameters, training images, one training step, Adam
+    function fr_cnn_param_count(s, c, nconv, k, nparam) bind(C, name="fr_cnn_param_count")
+      import :: c_int, c_long_long
+      integer(c_int), value :: s, c, nconv, k
+      integer(c_long_long) :: nparam
+      integer(c_int) :: fr_cnn_param_count
+    end function
+
+    function fr_cnn_init(ctx, s, c, nconv, k, seed, d_master, d_wq) bind(C, name="fr_cnn_init")
+      import :: c_int, c_long_long, c_ptr
+      type(c_ptr), value :: ctx, d_master, d_wq
+      integer(c_int), value :: s, c, nconv, k
+      integer(c_long_long), value :: seed
+      integer(c_int) :: fr_cnn_init
+    end function
+
+    function fr_cnn_synth(ctx, nsam, s, nb, d_a3df, d_dm, d_spar, gpar, disc, seed, d_img) &
+        bind(C, name="fr_cnn_synth")
+      import :: c_int, c_float, c_long_long, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_dm, d_spar, d_img
+      integer(c_int), value :: nsam, s, nb
+      real(c_float) :: gpar(11)
+      real(c_float), value :: disc
+      integer(c_long_long), value :: seed
+      integer(c_int) :: fr_cnn_synth
+    end function
+
+    function fr_cnn_gradients(ctx, s, c, nconv, k, d_wq, nb, d_img, d_label, d_loss, d_grad) &
+        bind(C, name="fr_cnn_gradients")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_wq, d_img, d_label, d_loss, d_grad
+      integer(c_int), value :: s, c, nconv, k, nb
+      integer(c_int) :: fr_cnn_gradients
+    end function
+
+    function fr_cnn_gradients_bf16(ctx, s, c, nconv, k, d_wq, nb, d_img, d_label, d_loss, d_grad) &
+        bind(C, name="fr_cnn_gradients_bf16")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_wq, d_img, d_label, d_loss, d_grad
+      integer(c_int), value :: s, c, nconv, k, nb
+      integer(c_int) :: fr_cnn_gradients_bf16
+    end function
+
+    function fr_cnn_adam(ctx, s, c, nconv, k, d_master, d_wq, d_grad, d_m, d_v, step, lr) &
+        bind(C, name="fr_cnn_adam")
+      import :: c_int, c_float, c_ptr
+      type(c_ptr), value :: ctx, d_master, d_wq, d_grad, d_m, d_v
+      integer(c_int), value :: s, c, nconv, k, step
+      real(c_float), value :: lr
+      integer(c_int) :: fr_cnn_adam
+    end function
+
+    function fr_orient_nearest(ctx, n, d_dm, ncls, d_dmc, nsym, d_sym, d_label) &
+        bind(C, name="fr_orient_nearest")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, d_dm, d_dmc, d_sym, d_label
+      integer(c_int), value :: n, ncls, nsym
+      integer(c_int) :: fr_orient_nearest
+    end function
+
+    ! CALC_OCC's occupancy update (calc_occ.f:179-217); arrays (np, nc)
+    function fr_calc_occ(nc, np, f, avg, occ, algp, sig) bind(C, name="fr_calc_occ")
+      import :: c_int, c_float
+      integer(c_int), value :: nc, np
+      real(c_float), value :: f
+      real(c_float) :: avg(*), occ(*), algp(*), sig(*)
+      integer(c_int) :: fr_calc_occ
     end function
   end interface
 

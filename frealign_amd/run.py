@@ -14,10 +14,11 @@ LMAIN (lmain.f:509-885) in batches on the device:
 then A3D3's insertion into the two half-maps (odd and even particles), and
 the end of the run of frealign_v9.f:790-950: STORESHIFT, the half-maps'
 c2r, CORRECT3D, MASK3D (written to cards 15 and 16), their r2c and
-SHELTEST's FSC, SHIFTVOL (with the Wiener filter of FIND_FPART's MW closed
-form when FFILT), the final map's c2r, CORRECT3D and MASK3D (card 13), the
-output parameter file (card 11) with the FSC table of OPRESSTATHALF appended
-and the parameter-shift file (card 12).
+SHELTEST's FSC, FIND_FPART (the particle fraction: the MW = 0 search over
+the half-maps, or the MW closed form; PSSNR divided by it), SHIFTVOL (with
+its Wiener filter when FFILT), the final map's c2r, CORRECT3D and MASK3D
+(card 13), the output parameter file (card 11) with the FSC table of
+OPRESSTATHALF appended and the parameter-shift file (card 12).
 
 Supported: MRC files (CFORM M), one dataset, IFSC 0, INTERP 1, IEWALD 0,
 XSTD >= 0, no FBEAUT/FBFACT/FMATCH/FDUMP, point groups C/D/T/O/I; anything

@@ -1,0 +1,71 @@
+"""The Fortran binding (frealign_amd/fortran/frealign_hip_mod.f90) against the
+C declarations (include/frealign_hip.h): every interface binds a declared
+entry point with the same number of arguments, and the entry points on the
+per-particle path all have an interface.  Needs no GPU."""
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HDR = os.path.join(ROOT, "include", "frealign_hip.h")
+MOD = os.path.join(ROOT, "frealign_amd", "fortran", "frealign_hip_mod.f90")
+
+
+def _split_top(s):
+    parts, depth, cur = [], 0, ""
+    for ch in s:
+        if ch == "(":
+            depth += 1
+        elif ch == ")":
+            depth -= 1
+        if ch == "," and depth == 0:
+            parts.append(cur)
+            cur = ""
+        else:
+            cur += ch
+    if cur.strip():
+        parts.append(cur)
+    return [p.strip() for p in parts if p.strip() and p.strip() != "void"]
+
+
+def c_decls():
+    text = re.sub(r"/\*.*?\*/", "", open(HDR).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^[A-Za-z_][A-Za-z0-9_ \*]*?\b(fr_[a-z0-9_]+)\s*\(", text, flags=re.M):
+        name, i, depth = m.group(1), m.end(), 1
+        j = i
+        while depth:
+            depth += {"(": 1, ")": -1}.get(text[j], 0)
+            j += 1
+        out[name] = len(_split_top(text[i:j - 1]))
+    return out
+
+
+def f_interfaces():
+    text = open(MOD).read()
+    text = re.sub(r"!.*", "", text)
+    text = re.sub(r"&\s*\n\s*", " ", text)
+    out = {}
+    for m in re.finditer(r"function\s+(\w+)\s*\(([^)]*)\)\s*bind\s*\(\s*C\s*,\s*name\s*=\s*\"(\w+)\"",
+                         text, flags=re.I):
+        args = [a for a in m.group(2).split(",") if a.strip()]
+        out[m.group(3)] = len(args)
+    return out
+
+
+def test_interfaces_match_declarations():
+    c, f = c_decls(), f_interfaces()
+    assert f, "no interfaces parsed"
+    for name, n in f.items():
+        assert name in c, f"{name}: bound in Fortran but not declared in the header"
+        assert n == c[name], f"{name}: {n} Fortran arguments, {c[name]} in C"
+
+
+def test_path_entry_points_have_interfaces():
+    f = f_interfaces()
+    need = ["fr_prep_batch", "fr_pextract_batch", "fr_cc3m_batch", "fr_cc3_batch", "fr_presb_batch",
+            "fr_sigma_batch", "fr_refine_batch", "fr_insert_batch", "fr_insert_batch_se",
+            "fr_search_batch", "fr_search_ragged_batch", "fr_psearch", "fr_local_search_batch",
+            "fr_ctfrefine_batch", "fr_magrefine_batch", "fr_storeshift", "fr_sheltest",
+            "fr_find_fpart", "fr_shiftvol", "fr_correct3d", "fr_mask3d", "fr_combine_arrays"]
+    missing = [n for n in need if n not in f]
+    assert not missing, missing
