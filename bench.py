@@ -27,7 +27,9 @@ reference; `particles_per_s_including_training` charges its time to the
 
 Accuracy against the truth and between searches, and the FSC(0.143) of the
 half-maps each search's parameters give, are measured on every particle of
-the pool.  Also reported: insertion GB/s (the refined particles inserted into
+the pool.  Also reported: LMAIN's steps after the search on the learned
+winners (PREFINE, PRESB, SIGMA) per batch and the rates with them, insertion
+GB/s (the refined particles inserted into
 the two half-maps with the 14 D7 copies), the wall time of the device
 finalisation of the learned half-maps (STORESHIFT, 3D FFTs, CORRECT3D, MASK3D,
 SHELTEST at 256^3) with its masked FSC, the mode-1 local refinement at 128^2
@@ -77,6 +79,9 @@ def parse():
                          "HBM byte counters against a known size)")
     ap.add_argument("--c2-particles", type=int, default=10000,
                     help="particles of the mode-1 local refinement at 128^2 (0: skip)")
+    ap.add_argument("--post-batches", type=int, default=2,
+                    help="batches on which LMAIN's steps after the search (PREFINE of the "
+                         "winner, PRESB, SIGMA) are timed (0: skip)")
     ap.add_argument("--uneven-batches", type=int, default=1,
                     help="batches of 256 per 'unit' of the 3:2:1 uneven workload (0: skip)")
     return ap.parse_args()
@@ -404,6 +409,21 @@ def main():
     log(f"FSC(0.143): learned {res_fsc['learned']:.2f} A, plain fp32 {res_fsc['fp32']:.2f} A "
         f"({npool} particles)")
 
+    # ------------------------ LMAIN's per-particle steps after the search
+    post = None
+    if args.post_batches > 0:
+        post = after_search(ctx, lp, gs16, prep, a3df, fsct, ring, maxr1, maxr2,
+                            min(args.post_batches, nbatch))
+        post_ms = sum(post["ms_per_batch"].values())
+        post["learned_with_these_particles_per_s"] = B / (dtl / args.steps + post_ms / 1e3)
+        post["plain_fp32_with_these_particles_per_s"] = B / (B / pps32 + post_ms / 1e3)
+        post["speedup_with_these"] = (post["learned_with_these_particles_per_s"] /
+                                      post["plain_fp32_with_these_particles_per_s"])
+        log(f"after the search: PREFINE {post['ms_per_batch']['prefine']:.1f}, PRESB "
+            f"{post['ms_per_batch']['presb']:.2f}, SIGMA {post['ms_per_batch']['sigma']:.2f} ms "
+            f"per batch; learned with them {post['learned_with_these_particles_per_s']:.0f} "
+            f"particles/s")
+
     # ------------------------------------------------------------ roofline
     # dominant kernel of the learned step: the FP32 CC3M rescoring or the search GEMM
     ncand = float(np.mean(valid))
@@ -516,11 +536,67 @@ def main():
         "roofline": roof,
         "exhaustive_gemm": exh_roof,
         "cpu_baseline": cpu,
+        "after_search_256": post,
         "local_refinement_128": c2,
         "uneven_batches": uneven,
         "setup_s": setup_s,
     }
     print(json.dumps(out))
+
+
+def after_search(ctx, lp, gs, prep, a3df, fsct, ring, maxr1, maxr2, nbatch):
+    """LMAIN's per-particle steps after a mode-3 search, on the learned
+    winners of nbatch batches: PREFINE of the winner (lmain.f:615-642; all
+    five parameters, a shift prior of 0.05 phase units), PRESB at the refined
+    parameters (lmain.f:731-736) and SIGMA (lmain.f:897-903, the 8 A band and
+    the particle disc).  They are the same for the learned and the plain
+    search, so they are reported beside `value` with the rates both searches
+    would have with them."""
+    import torch
+
+    from frealign_amd import api
+    dev = "cuda"
+    fs_d = torch.from_numpy(np.asarray(fsct, np.float32)).to(dev)
+    mask = np.ones(5, np.int32)
+    dmask = np.array([NSAM / 2 + 1] * 3 + [RI], np.float32)
+    tms = {"prefine": 0.0, "presb": 0.0, "sigma": 0.0}
+    nev = []
+    npart = 0
+    for i in range(nbatch):
+        ctff, outc, datc, lo = prep(i)
+        res, _ = lp.run(outc, ctff, datc, fsct, ring, TOPK)
+        n = outc.shape[0]
+        npart += n
+        par = np.zeros((n, 6), np.float32)
+        par[:, 0:3] = gs.euler[np.maximum(res.view, 0)]
+        par[:, 3:5] = res.shift
+        prior = np.zeros((n, 5), np.float32)
+        prior[:, 2:4] = 0.05
+        prior[:, 4] = 0.01
+        ones = np.ones(n, np.float32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ref, ne = ctx.refine(NSAM, a3df, outc, ctff, np.arange(n), ones, par, mask, maxr1, maxr2,
+                             0.0, fs_d, ring, prior)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        dm = torch.from_numpy(api.rotation(ref[:, 0:3], amag=1.0)).to(dev)
+        sh = torch.from_numpy(np.ascontiguousarray(ref[:, 3:5])).to(dev)
+        ctx.presb(NSAM, a3df, outc, ctff, torch.arange(n, dtype=torch.int32, device=dev), dm, sh,
+                  torch.ones(n, dtype=torch.float32, device=dev), ring)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        ctx.sigma(NSAM, a3df, datc, ctff, np.arange(n), ref[:, 0:5], ones, maxr1,
+                  PSIZE / RMAX2_A, dmask, 1.0)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        tms["prefine"] += t1 - t0
+        tms["presb"] += t2 - t1
+        tms["sigma"] += t3 - t2
+        nev.append(ne)
+    return {"batches": nbatch, "particles": npart,
+            "ms_per_batch": {k: v / nbatch * 1e3 for k, v in tms.items()},
+            "prefine_cc3m_evaluations_per_particle": float(np.mean(np.concatenate(nev)))}
 
 
 def uneven_workload(ctx, units, log):

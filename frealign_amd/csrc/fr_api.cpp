@@ -116,6 +116,24 @@ void* fr_ctx::slot(int k, size_t bytes, int* err) {
   return slots[k];
 }
 
+void* fr_ctx::host_pinned(size_t bytes, int* err) {
+  if (bytes <= hpin_bytes) return hpin;
+  if (hpin) {
+    if (hipStreamSynchronize(stream) != hipSuccess || hipHostFree(hpin) != hipSuccess) {
+      *err = fr::fail("pinned staging release failed");
+      return nullptr;
+    }
+    hpin = nullptr;
+    hpin_bytes = 0;
+  }
+  if (hipHostMalloc(&hpin, bytes, hipHostMallocDefault) != hipSuccess) {
+    *err = fr::fail("hipHostMalloc of %zu staging bytes failed", bytes);
+    return nullptr;
+  }
+  hpin_bytes = bytes;
+  return hpin;
+}
+
 namespace {
 void sheltest_finish(const double* h, int ns, float fmask, int* npix, int* npixt, double* dfsc,
                      float* pssnr);
@@ -241,6 +259,7 @@ int fr_destroy(fr_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   for (int k = 0; k < fr_ctx::kSlots; k++)
     if (c->slots[k]) (void)hipFree(c->slots[k]);
+  if (c->hpin) (void)hipHostFree(c->hpin);
   if (c->d_kidx) (void)hipFree(c->d_kidx);
   for (auto& e : c->dft_tabs) (void)hipFree(e.second);
   for (auto& e : c->pts_tabs) (void)hipFree(e.second);
@@ -1306,28 +1325,33 @@ int refine_items(fr_ctx* c, int nsam, int n, const float* d_a3df, const float* d
   a.prior = prior;
   a.maxit = maxit;
   a.sm = sm;
+  // one round's items go over in one pinned copy (PIDX | DM | SHIFT | AMAG,
+  // 13 words per item) and its scores come back in one
   auto cc3m = [&](int m, const int* hp, const float* hdm, const float* hsh, const float* ham,
                   float* hscore) -> int {
     int err = 0;
-    int* dp = (int*)c->slot(11, sizeof(int) * (size_t)m, &err);
-    if (!dp) return err;
-    float* ddm = (float*)c->slot(12, sizeof(float) * 9 * (size_t)m, &err);
-    if (!ddm) return err;
-    float* dsh = (float*)c->slot(13, sizeof(float) * 2 * (size_t)m, &err);
-    if (!dsh) return err;
-    float* dam = (float*)c->slot(14, sizeof(float) * (size_t)m, &err);
-    if (!dam) return err;
+    const size_t words = 13 * (size_t)m;
+    char* dbuf = (char*)c->slot(11, sizeof(float) * words, &err);
+    if (!dbuf) return err;
     float* dsc = (float*)c->slot(15, sizeof(float) * (size_t)m, &err);
     if (!dsc) return err;
-    FR_HIP(hipMemcpyAsync(dp, hp, sizeof(int) * m, hipMemcpyHostToDevice, c->stream));
-    FR_HIP(hipMemcpyAsync(ddm, hdm, sizeof(float) * 9 * m, hipMemcpyHostToDevice, c->stream));
-    FR_HIP(hipMemcpyAsync(dsh, hsh, sizeof(float) * 2 * m, hipMemcpyHostToDevice, c->stream));
-    FR_HIP(hipMemcpyAsync(dam, ham, sizeof(float) * m, hipMemcpyHostToDevice, c->stream));
-    if (int r = fr_cc3m_batch(c, nsam, m, d_a3df, d_spec, d_ctff, dp, ddm, dsh, dam, maxr1, maxr2,
-                              rbfact, d_fsct, ring, dsc))
+    char* h = (char*)c->host_pinned(sizeof(float) * (words + m), &err);
+    if (!h) return err;
+    const size_t o_dm = sizeof(int) * m, o_sh = o_dm + sizeof(float) * 9 * m,
+                 o_am = o_sh + sizeof(float) * 2 * m, o_sc = o_am + sizeof(float) * m;
+    memcpy(h, hp, sizeof(int) * m);
+    memcpy(h + o_dm, hdm, sizeof(float) * 9 * m);
+    memcpy(h + o_sh, hsh, sizeof(float) * 2 * m);
+    memcpy(h + o_am, ham, sizeof(float) * m);
+    FR_HIP(hipMemcpyAsync(dbuf, h, o_sc, hipMemcpyHostToDevice, c->stream));
+    if (int r = fr_cc3m_batch(c, nsam, m, d_a3df, d_spec, d_ctff, (const int*)dbuf,
+                              (const float*)(dbuf + o_dm), (const float*)(dbuf + o_sh),
+                              (const float*)(dbuf + o_am), maxr1, maxr2, rbfact, d_fsct, ring,
+                              dsc))
       return r;
-    FR_HIP(hipMemcpyAsync(hscore, dsc, sizeof(float) * m, hipMemcpyDeviceToHost, c->stream));
+    FR_HIP(hipMemcpyAsync(h + o_sc, dsc, sizeof(float) * m, hipMemcpyDeviceToHost, c->stream));
     FR_HIP(hipStreamSynchronize(c->stream));
+    memcpy(hscore, h + o_sc, sizeof(float) * m);
     return 0;
   };
   return fr::refine_batch(*c, a, par, nevals, cc3m);

@@ -15,6 +15,11 @@ struct fr_ctx : public fr::Ctx {
   void* slots[kSlots] = {};
   size_t slot_bytes[kSlots] = {};
   void* slot(int k, size_t bytes, int* err);
+  // grow-only pinned host staging (one buffer), for the small per-round
+  // transfers of the lock-stepped minimisations
+  void* hpin = nullptr;
+  size_t hpin_bytes = 0;
+  void* host_pinned(size_t bytes, int* err);
   // search annulus and shift-phase tables, cached per (NSAM, MAXR1, MAXR2)
   int skey[3] = {0, 0, 0};
   int kc = 0, kpad = 0, nsamr = 0;
