@@ -32,7 +32,8 @@ int launch_sep_feats(Ctx& c, int nsam, int np, int lo2, int hi2, float rbfact, c
                      const float* ctff, void* xa, void* xb, float* c2s);
 // fb (ragged lists only, may be null): the fallback particles (fb[0] = count,
 // fb[1..] particles) whose lists are the whole bank in order; they are
-// searched four to a workgroup against shared templates, as the uniform form
+// searched four to a workgroup against shared templates, as the uniform form,
+// in the same persistent launch as the listed particles' tiles
 int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int* loff,
                       const int* toff, int cap, int max_tiles, const int* cand, const void* bank,
                       const void* xa, const void* xb, const float* c2s, const float* sum2,

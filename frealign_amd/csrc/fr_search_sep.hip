@@ -163,9 +163,12 @@ __global__ void __launch_bounds__(256)
 // tile.
 struct SepItem {
   int p, base, lo, hi;
+  bool listed;  // the entries are list positions (cand), else bank rows in order
 };
 
-enum { kSepUniform = 0, kSepRagged = 1, kSepFallback = 2 };
+// kSepMixed: the ragged tiles of the listed particles, then the fallback
+// items, in one persistent launch
+enum { kSepUniform = 0, kSepRagged = 1, kSepFallback = 2, kSepMixed = 3 };
 
 template <int kMode>
 __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
@@ -184,6 +187,7 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
     const int end = loff[it.p + 1] < cap ? loff[it.p + 1] : cap;
     it.lo = tile * kSepFbTile;
     it.hi = min(min(nrow, end - it.base), it.lo + kSepFbTile);
+    it.listed = false;
     return it.base >= 0 && it.lo < it.hi;
   }
   if (kMode == kSepUniform) {
@@ -197,6 +201,7 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
     it.base = it.p * nrow;
     it.lo = tile * kSepTile;
     it.hi = min(nrow, it.lo + kSepTile);
+    it.listed = true;
     return true;
   }
   // the particle whose tile range holds idx: the largest p with toff[p] <= idx
@@ -211,6 +216,7 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
   const int end = loff[lo + 1] < cap ? loff[lo + 1] : cap;
   it.lo = (idx - toff[lo]) * kSepTile + wave * 64;
   it.hi = min(end - it.base, it.lo + 64);
+  it.listed = true;
   return it.base >= 0 && it.lo < it.hi;
 }
 
@@ -350,13 +356,22 @@ __global__ void __launch_bounds__(kSepThreads, 2)
 
   // persistent forms loop over their items; the uniform form is one item
   // per workgroup
-  const int nitem = kRagged ? toff[np]
-                    : kMode == kSepFallback ? (ntile + 7) / 8 * 8 * ((fb[0] + 3) / 4)
-                                            : (int)gridDim.x;
+  const int nfb = kMode == kSepFallback || kMode == kSepMixed
+                      ? (ntile + 7) / 8 * 8 * ((fb[0] + 3) / 4) : 0;
+  const int nlist = kRagged || kMode == kSepMixed ? toff[np] : 0;
+  const int nitem = kMode == kSepUniform ? (int)gridDim.x : nlist + nfb;
   const int stride = kMode == kSepUniform ? nitem : (int)gridDim.x;
   for (int idx = blockIdx.x; idx < nitem; idx += stride) {
     SepItem it;
-    if (!sep_item<kMode>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it)) continue;
+    bool ok;
+    if (kMode == kSepMixed) {
+      ok = idx < nlist ? sep_item<kSepRagged>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it)
+                       : sep_item<kSepFallback>(idx - nlist, wave, np, nrow, ntile, loff, toff,
+                                                cap, fb, it);
+    } else {
+      ok = sep_item<kMode>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it);
+    }
+    if (!ok) continue;
     if (it.p != pcur) {
       const size_t b = (size_t)it.p * 512 + lane;
 #pragma unroll
@@ -368,7 +383,7 @@ __global__ void __launch_bounds__(kSepThreads, 2)
       s2 = sum2[it.p];
       pcur = it.p;
     }
-    const int* cl = cand ? cand + it.base : nullptr;
+    const int* cl = cand && it.listed ? cand + it.base : nullptr;
     // the score of an empty column (zero map, no norm): the GEMM form's
     auto put = [&](int pos, int o, float sc, int row) {
       if (lane == 0) {
@@ -465,22 +480,23 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
     c.cus = prop.multiProcessorCount;
   }
   const int full = c.cus * per_cu;
+  if (fb) {
+    // the listed particles' tiles and the fallback particles' items (four
+    // particles against shared 64-template tiles, the uniform mapping) in one
+    // persistent launch, so that neither pass leaves a tail of idle CUs
+    const int ntile = (nbank + kSepFbTile - 1) / kSepFbTile;
+    hipLaunchKernelGGL(k_search_sep<kSepMixed>, dim3(full), dim3(kSepThreads), 0, c.stream, np,
+                       nbank, nsamr, nbank, ntile, loff, toff, cap, cand, bk, a, b, cc, sum2,
+                       score, best_row, fb);
+    FR_HIP(hipGetLastError());
+    return 0;
+  }
   int grid = full;
   if (max_tiles < grid) grid = max_tiles;
   if (grid >= 1) {
     hipLaunchKernelGGL(k_search_sep<kSepRagged>, dim3(grid), dim3(kSepThreads), 0, c.stream, np,
                        nbank, nsamr, 0, 0, loff, toff, cap, cand, bk, a, b, cc, sum2, score,
                        best_row, nullptr);
-    FR_HIP(hipGetLastError());
-  }
-  if (fb) {
-    // the fallback particles: a persistent grid (a multiple of the 8 XCDs, so
-    // that each workgroup keeps its XCD's tiles) over the uniform mapping
-    const int ntile = (nbank + kSepFbTile - 1) / kSepFbTile;
-    const int gf = full / 8 * 8 > 8 ? full / 8 * 8 : 8;
-    hipLaunchKernelGGL(k_search_sep<kSepFallback>, dim3(gf), dim3(kSepThreads), 0, c.stream, np,
-                       nbank, nsamr, nbank, ntile, loff, nullptr, cap, nullptr, bk, a, b, cc, sum2,
-                       score, best_row, fb);
     FR_HIP(hipGetLastError());
   }
   return 0;
