@@ -72,6 +72,8 @@ SIGNATURES = {
                          vp], ci),
     "fr_insert_batch_se": ([vp, ci, ci, vp, vp, vp, vp, ci, vp, vp, cf, cf, cf, vp, vp, vp, vp,
                             vp, vp, vp, vp, vp, vp, vp], ci),
+    "fr_finalize": ([vp, ci, vp, vp, vp, vp, vp, vp, ci, cf, cf, cf, cf, cf, ci, cf, vp, vp,
+                     ci, vp, vp, vp, vp, vp, vp, vp], ci),
     "fr_find_fpart": ([vp, ci, C.c_double, C.c_double, vp, vp, vp, vp, ci, cf, cf, cf, cf, cf,
                        vp, vp, ci, vp], ci),
     "fr_sheltest": ([vp, ci, ci, vp, vp, vp, vp, cf, vp, vp, vp, vp, pi_], ci),

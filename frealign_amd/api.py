@@ -838,6 +838,36 @@ class Context:
                                       C.byref(fp)), "fr_find_fpart")
         return fp.value, ps[:len(pssnr)]
 
+    def finalize(self, nsam: int, vols: "HalfMaps", iredun: int, psize: float, ri: float,
+                 ric: float, mw: float, dalt: float, ffilt: bool, rlim: float):
+        """fr_finalize: the end of a refinement cycle (frealign_v9.f:790-950) in
+        one call.  vols.num[0] and vols.ctf2[0] are overwritten (the final
+        real-space map ends in vols.num[0], padded layout); returns a dict with
+        the final map as an (NSAM, NSAM, NSAM) view, the masked half-maps'
+        transforms (b, c) and the FSC table's numbers."""
+        jc = nsam // 2 + 1
+        cap = jc + 2
+        b = torch.empty_like(vols.num[0])
+        c = torch.empty_like(vols.num[1])
+        fsc = np.zeros(cap, np.float32)
+        pssnr = np.zeros(cap, np.float32)
+        npix = np.zeros(cap, np.int32)
+        npixt = np.zeros(cap, np.int32)
+        nk = np.ascontiguousarray(vols.nksum.cpu().numpy(), np.int64)
+        td = np.ascontiguousarray(vols.ttd.cpu().numpy(), np.float64)
+        nstp, fm, fp = C.c_int(0), C.c_float(0.0), C.c_float(0.0)
+        self._sync_stream()
+        N.check(N.lib().fr_finalize(
+            self.h, nsam, _p(vols.num[0]), _p(vols.ctf2[0]), _p(vols.num[1]), _p(vols.ctf2[1]),
+            nk.ctypes.data, td.ctypes.data, int(iredun), float(psize), float(ri), float(ric),
+            float(mw), float(dalt), int(bool(ffilt)), float(rlim), _p(b), _p(c), cap,
+            fsc.ctypes.data, pssnr.ctypes.data, npix.ctypes.data, npixt.ctypes.data,
+            C.byref(nstp), C.byref(fm), C.byref(fp)), "fr_finalize")
+        k = nstp.value
+        final = vols.num[0].view(torch.float32).reshape(nsam, nsam, nsam + 2)[..., :nsam]
+        return dict(final=final, b=b, c=c, fsc=fsc[:k], pssnr=pssnr[:k], npix=npix[:k],
+                    npixt=npixt[:k], fmask=fm.value, fpart=fp.value)
+
     def mask3d(self, nsam: int, a3dv, rio: float, ric: float, halfw: float):
         """MASK3D (mask3d.f) in place on the padded real volume; returns
         (FMASK, AVE, STD)."""

@@ -38,6 +38,7 @@ SOURCES = [
     "fr_ctfrefine.cpp",
     "fr_io.cpp",
     "fr_fpart.hip",
+    "fr_finalize.cpp",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -395,6 +395,21 @@ module frealign_hip
       integer(c_int) :: fr_find_fpart
     end function
 
+    ! the end of a refinement cycle (frealign_v9.f:790-950) in one call
+    function fr_finalize(ctx, nsam, d_a3df, d_s3df, d_d3df, d_v3df, nksum, ttd, iredun, psize, &
+                         ri, ric, mw, dalt, ffilt, rlim, d_b, d_c, nshell, fsc, pssnr, npix, &
+                         npixt, nstp, fmask, fpart) bind(C, name="fr_finalize")
+      import :: c_int, c_float, c_double, c_long_long, c_ptr
+      type(c_ptr), value :: ctx, d_a3df, d_s3df, d_d3df, d_v3df, d_b, d_c
+      integer(c_int), value :: nsam, iredun, ffilt, nshell
+      integer(c_long_long) :: nksum(2)
+      real(c_double) :: ttd(2)
+      real(c_float), value :: psize, ri, ric, mw, dalt, rlim
+      real(c_float) :: fsc(*), pssnr(*), fmask, fpart
+      integer(c_int) :: npix(*), npixt(*), nstp
+      integer(c_int) :: fr_finalize
+    end function
+
     ! CC3 (cc3.f:8-159), the score of CTFREFINE and MAGREFINE
     function fr_cc3_batch(ctx, nsam, n, d_a3df, d_spec, d_ctff, d_pidx, d_dm, d_shift, d_amag, &
                           maxr1, maxr2, rbfact, d_fsct, ring, d_score) bind(C, name="fr_cc3_batch")

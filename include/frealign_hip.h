@@ -198,6 +198,24 @@ int fr_find_fpart(fr_ctx *ctx, int nsam, double std_, double vtd, const float *d
                   float ri, float ric, float psize, float mw, float dalt, const float *fsc,
                   float *pssnr, int nshell, float *fpart);
 
+/* The end of a refinement cycle (frealign_v9.f:790-950, IFSC 0, INTERP 1,
+ * IPAD 1) in one call, by the entry points of this header in the reference's
+ * order: STD, VTD = TTD / (NKSUM(1) + NKSUM(2)) * 0.1; STORESHIFT into d_b,
+ * d_c; each half: 3D c2r, CORRECT3D (CALCSINC's table), MASK3D at |ri| with
+ * ric and HALFW 6 (FMASK from the first), 3D r2c; SHELTEST; FIND_FPART (mw,
+ * dalt; PSSNR divided by FPART); SHIFTVOL into d_a3df/d_s3df (overwritten, as
+ * in the reference) with the Wiener filter when ffilt, RLIM rlim; the final
+ * map's 3D c2r and CORRECT3D, and MASK3D when ri > 0.  On return d_a3df holds
+ * the final real-space map (padded layout), d_b/d_c the masked half-maps'
+ * transforms.  Host: nksum[2], ttd[2] (A3D3's counters), and per shell
+ * (nshell >= NSAM/2 + 3 entries) fsc, pssnr (divided by FPART), npix, npixt;
+ * *nstp shells are set; *fmask, *fpart. */
+int fr_finalize(fr_ctx *ctx, int nsam, float *d_a3df, float *d_s3df, const float *d_d3df,
+                const float *d_v3df, const long long *nksum, const double *ttd, int iredun,
+                float psize, float ri, float ric, float mw, float dalt, int ffilt, float rlim,
+                float *d_b, float *d_c, int nshell, float *fsc, float *pssnr, int *npix,
+                int *npixt, int *nstp, float *fmask, float *fpart);
+
 /* Replaces SHIFTVOL (shiftvol.f:8-77, NNSTAT=0), in place: *std_ += vtd; for
  * ifsc == 0 A += D and S += V; then within INT(NSAM*RLIM) A = A(-1)^(l+m+n) /
  * (S + STD*IREDUN) (fpart == 0) or that / (1 + FPART/A1/FMASK) with
@@ -224,8 +242,8 @@ int fr_mask3d(fr_ctx *ctx, int nsam, float *d_a3dv, float rio, float ric, float 
               float *fmask, float *ave, float *std_);
 
 /* FIND_FPART's closed form for MW != 0 (find_fpart.f:141-149): FPART =
- * |1000 MW / DALT / (PSIZE NSAM)^3| and SSNR(1..JC) /= FPART (host).  The MW = 0
- * search (3D FFTs, MASK3D, VOLMEASURE, FPART_CC) is not provided. */
+ * |1000 MW / DALT / (PSIZE NSAM)^3| and SSNR(1..JC) /= FPART (host).
+ * fr_find_fpart covers both branches, the MW = 0 search on the device. */
 int fr_fpart_from_mw(int nsam, float mw, float dalt, float psize, float *ssnr, float *fpart);
 
 /* Replaces SHELTEST's FSC/PSSNR part (sheltest.f:8-122, NNSTAT=0) with FP64

@@ -66,6 +66,7 @@ def test_path_entry_points_have_interfaces():
             "fr_sigma_batch", "fr_refine_batch", "fr_insert_batch", "fr_insert_batch_se",
             "fr_search_batch", "fr_search_ragged_batch", "fr_psearch", "fr_local_search_batch",
             "fr_ctfrefine_batch", "fr_magrefine_batch", "fr_storeshift", "fr_sheltest",
-            "fr_find_fpart", "fr_shiftvol", "fr_correct3d", "fr_mask3d", "fr_combine_arrays"]
+            "fr_find_fpart", "fr_shiftvol", "fr_correct3d", "fr_mask3d", "fr_combine_arrays",
+            "fr_finalize"]
     missing = [n for n in need if n not in f]
     assert not missing, missing
