@@ -8,7 +8,7 @@ at uniform random orientations, +-3 A shifts, CTF at 300 kV / Cs 2.7 mm /
 in-plane angles = 46,800 orientations, 15x15 NSAMR shift hypotheses, 8 A
 band; the 32 best views are rescored with the FP32 CC3M.
 
-One step = one batch of B particles already resident in HBM through
+One step = one batch of B particles (1,024 by default) already resident in HBM through
 particle preparation (r2c + CTFAPPLY), the search and the FP32 rescoring;
 the timed steps cycle over a pool of 5,120 distinct particles.  Searches:
   * learned + BF16 (`value`): the proposal network's 8 most probable
@@ -56,7 +56,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=8)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=1024,
+                    help="particles per step (the learned, exhaustive and plain FP32 searches "
+                         "alike)")
     ap.add_argument("--fp32-steps", type=int, default=4)
     ap.add_argument("--pool", type=int, default=5120,
                     help="distinct particles: the timed steps cycle over them; accuracy and "
