@@ -69,6 +69,12 @@ const float2* Ctx::twiddles(int n) {
   return d;
 }
 
+float2* Ctx::prep_partials() {
+  if (!prep_part && hipMalloc(&prep_part, sizeof(float2) * 65535 * 16) != hipSuccess)
+    prep_part = nullptr;
+  return prep_part;
+}
+
 int Ctx::prof_begin(int name) {
   if (!prof_on || nprof >= 4096) return -1;
   ProfRec& r = prof[nprof];
@@ -92,6 +98,7 @@ Ctx::~Ctx() {
     (void)hipEventDestroy(prof[i].b);
   }
   if (scratch) (void)hipFree(scratch);
+  if (prep_part) (void)hipFree(prep_part);
   for (auto* p : tw)
     if (p) (void)hipFree(p);
 }

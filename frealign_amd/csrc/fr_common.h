@@ -43,6 +43,12 @@ struct Ctx {
   void prof_end(int slot);
   int ensure_scratch(size_t bytes);
   const float2* twiddles(int n);
+  // the particle preparation's per-row-block ring sums (fr_prep.hip) for
+  // calls on the context's stream: one fixed allocation for the largest
+  // batch (65535 images x 16 row blocks), made on first use and kept (plans,
+  // whose graphs replay on their own streams, bring their own buffer)
+  float2* prep_part = nullptr;
+  float2* prep_partials();
   ~Ctx();
 };
 
@@ -86,5 +92,10 @@ int fft_c2r_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride);
 int fft_r2c_3d(Ctx& c, int nsam, float* data);
 int fft_c2r_3d(Ctx& c, int nsam, float* data);
 bool fft_supported(int nsam);
+// the row pass alone (forward r2c of the rows, or backward c2r), for callers
+// that run the column pass themselves
+int fft_rows_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, bool fwd);
+// the column pass alone (sign -1 forward, +1 backward)
+int fft_cols_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, int sign);
 
 }  // namespace fr

@@ -14,12 +14,14 @@
 #include <vector>
 
 #include "fr_common.h"
+#include "fr_fft_dev.h"
 
 namespace fr {
 
 namespace {
 
-constexpr int kThreads = 256;
+using namespace fftdev;
+constexpr int kThreads = kFftThreads;
 constexpr int kMaxPass = 12;
 
 struct Plan {
@@ -38,56 +40,6 @@ bool make_plan(int n, Plan* p) {
   while (m % 3 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 3; m /= 3; }
   while (m % 5 == 0 && p->npass < kMaxPass) { p->radix[p->npass++] = 5; m /= 5; }
   return m == 1;
-}
-
-__device__ inline float2 f2(float a, float b) { return make_float2(a, b); }
-__device__ inline float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
-__device__ inline float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
-__device__ inline float2 mul2(float2 a, float2 b) {
-  return f2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ inline float2 scl2(float2 a, float s) { return f2(a.x * s, a.y * s); }
-// multiply by i*sign
-__device__ inline float2 muli(float2 a, int sign) { return sign > 0 ? f2(-a.y, a.x) : f2(a.y, -a.x); }
-
-template <int R>
-__device__ inline void butterfly(float2* u, int sign) {
-  if constexpr (R == 2) {
-    float2 a = u[0], b = u[1];
-    u[0] = add2(a, b);
-    u[1] = sub2(a, b);
-  } else if constexpr (R == 4) {
-    float2 a0 = add2(u[0], u[2]), a1 = sub2(u[0], u[2]);
-    float2 b0 = add2(u[1], u[3]), jt = muli(sub2(u[1], u[3]), sign);
-    u[0] = add2(a0, b0);
-    u[2] = sub2(a0, b0);
-    u[1] = add2(a1, jt);
-    u[3] = sub2(a1, jt);
-  } else if constexpr (R == 3) {
-    const float h = 0.86602540378443864676f;
-    float2 s = add2(u[1], u[2]), d = sub2(u[1], u[2]);
-    float2 v0 = add2(u[0], s);
-    float2 m = sub2(u[0], scl2(s, 0.5f));
-    float2 nn = scl2(muli(d, sign), h);
-    u[0] = v0;
-    u[1] = add2(m, nn);
-    u[2] = sub2(m, nn);
-  } else if constexpr (R == 5) {
-    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
-    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
-    float2 a1 = add2(u[1], u[4]), b1 = sub2(u[1], u[4]);
-    float2 a2 = add2(u[2], u[3]), b2 = sub2(u[2], u[3]);
-    float2 v0 = add2(add2(u[0], a1), a2);
-    float2 p1 = add2(u[0], add2(scl2(a1, c1), scl2(a2, c2)));
-    float2 q1 = muli(add2(scl2(b1, s1), scl2(b2, s2)), sign);
-    float2 p2 = add2(u[0], add2(scl2(a1, c2), scl2(a2, c1)));
-    float2 q2 = muli(sub2(scl2(b1, s2), scl2(b2, s1)), sign);
-    u[0] = v0;
-    u[1] = add2(p1, q1);
-    u[4] = sub2(p1, q1);
-    u[2] = add2(p2, q2);
-    u[3] = sub2(p2, q2);
-  }
 }
 
 template <int R>
@@ -220,14 +172,20 @@ __global__ void __launch_bounds__(kThreads) k_rows_c2r(float* data, Plan pl, int
 
 // ---- column pass over the half-plane: JC columns of length n (complex, stride JC)
 __global__ void __launch_bounds__(kThreads) k_cols(float* data, Plan pl, size_t img_stride, int cw,
-                                                   int sign, const float2* __restrict__ tw) {
+                                                   int nimg, int sign,
+                                                   const float2* __restrict__ tw) {
   extern __shared__ float2 smem[];
   const int n = pl.n, ld = n + 1, jc = n / 2 + 1;
   float2* buf = smem;
   float2* tmp = smem + cw * ld;
-  const int x0 = blockIdx.x * cw;
+  // column blocks of one image in XCD-contiguous order
+  const int jb = (jc + cw - 1) / cw;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int img = lb / jb;
+  if (img >= nimg) return;
+  const int x0 = (lb - img * jb) * cw;
   const int ncol = min(cw, jc - x0);
-  float2* base = reinterpret_cast<float2*>(data + blockIdx.y * img_stride);
+  float2* base = reinterpret_cast<float2*>(data + img * img_stride);
   for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
     int y = e / cw, c = e - y * cw;
     float2 v = f2(0.f, 0.f);
@@ -251,7 +209,9 @@ __global__ void __launch_bounds__(kThreads) k_cols_z(float* data, Plan pl, int n
   const int n = pl.n, ld = n + 1;
   float2* buf = smem;
   float2* tmp = smem + cw * ld;
-  const int x0 = blockIdx.x * cw;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int x0 = lb * cw;
+  if (x0 >= ncols) return;
   const int ncol = min(cw, ncols - x0);
   float2* base = reinterpret_cast<float2*>(data);
   for (int e = threadIdx.x; e < n * cw; e += blockDim.x) {
@@ -276,54 +236,6 @@ int row_seqs(int n) {
 // ---- the same three kernels with the length known at compile time (the
 // box sizes of the bench and the proposal: 64, 128, 256): every index
 // division and modulus becomes a shift or a constant multiply.
-// Radix of pass i in make_plan's order (0 past the last pass).
-constexpr int radix_at(int n, int i) {
-  int m = n, k = 0;
-  while (m % 4 == 0) { if (k == i) return 4; m /= 4; k++; }
-  while (m % 2 == 0) { if (k == i) return 2; m /= 2; k++; }
-  while (m % 3 == 0) { if (k == i) return 3; m /= 3; k++; }
-  while (m % 5 == 0) { if (k == i) return 5; m /= 5; k++; }
-  return 0;
-}
-
-template <int R, int N, int P, int NSEQ, int LD, int SIGN>
-__device__ inline void stockham_pass_t(const float2* in, float2* out,
-                                       const float2* __restrict__ tw) {
-  constexpr int T = N / R;
-  constexpr int stride = N / (P * R);
-  for (int b = threadIdx.x; b < NSEQ * T; b += kThreads) {
-    const int s = b / T, i = b - s * T;
-    const int k = i % P;
-    const float2* x = in + s * LD;
-    float2 u[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) u[r] = x[i + r * T];
-#pragma unroll
-    for (int r = 1; r < R; r++) {
-      float2 w = tw[(r * k * stride) % N];
-      if (SIGN > 0) w.y = -w.y;
-      u[r] = mul2(u[r], w);
-    }
-    butterfly<R>(u, SIGN);
-    float2* y = out + s * LD;
-    const int base = (i - k) * R + k;
-#pragma unroll
-    for (int q = 0; q < R; q++) y[base + q * P] = u[q];
-  }
-}
-
-template <int N, int NSEQ, int LD, int SIGN, int I = 0, int P = 1>
-__device__ inline float2* lds_fft_t(float2* buf, float2* tmp, const float2* __restrict__ tw) {
-  constexpr int R = radix_at(N, I);
-  if constexpr (R == 0) {
-    return buf;
-  } else {
-    stockham_pass_t<R, N, P, NSEQ, LD, SIGN>(buf, tmp, tw);
-    __syncthreads();
-    return lds_fft_t<N, NSEQ, LD, SIGN, I + 1, P * R>(tmp, buf, tw);
-  }
-}
-
 template <int N>
 __global__ void __launch_bounds__(kThreads) k_rows_r2c_t(float* data, size_t img_stride,
                                                          int npairs, const float2* __restrict__ tw) {
@@ -398,24 +310,23 @@ __global__ void __launch_bounds__(kThreads) k_rows_c2r_t(float* data, size_t img
   }
 }
 
-// columns per workgroup of the column pass (16 at 256 measured no faster)
-template <int N>
-constexpr int cols_cw() {
-  constexpr int JC = N / 2 + 1;
-  return (2048 / N) < JC ? (2048 / N) : JC;
-}
 
 template <int N, int SIGN>
-__global__ void __launch_bounds__(kThreads) k_cols_t(float* data, size_t img_stride,
+__global__ void __launch_bounds__(kThreads) k_cols_t(float* data, size_t img_stride, int nimg,
                                                      const float2* __restrict__ tw) {
   constexpr int JC = N / 2 + 1, LD = N + 1;
   constexpr int CW = cols_cw<N>();
+  constexpr int JB = (JC + CW - 1) / CW;
   extern __shared__ float2 smem[];
   float2* buf = smem;
   float2* tmp = smem + CW * LD;
-  const int x0 = blockIdx.x * CW;
+  // column blocks of one image in XCD-contiguous order
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int img = lb / JB;
+  if (img >= nimg) return;
+  const int x0 = (lb - img * JB) * CW;
   const int ncol = min(CW, JC - x0);
-  float2* base = reinterpret_cast<float2*>(data + blockIdx.y * img_stride);
+  float2* base = reinterpret_cast<float2*>(data + img * img_stride);
   for (int e = threadIdx.x; e < N * CW; e += kThreads) {
     const int y = e / CW, c = e - y * CW;
     float2 v = f2(0.f, 0.f);
@@ -451,13 +362,15 @@ int cols_pass_t(Ctx& c, int nimg, float* data, size_t img_stride, int sign, cons
   constexpr int JC = N / 2 + 1;
   constexpr int CW = cols_cw<N>();
   const size_t lds = sizeof(float2) * 2 * CW * (N + 1);
-  dim3 grid((JC + CW - 1) / CW, nimg);
+  const long long nblk = (long long)((JC + CW - 1) / CW) * nimg;
+  FR_CHECK(nblk <= (1ll << 30), "too many images in one FFT batch (%d)", nimg);
+  dim3 grid(round8((int)nblk));
   if (sign < 0)
     hipLaunchKernelGGL((k_cols_t<N, -1>), grid, dim3(kThreads), lds, c.stream, data, img_stride,
-                       tw);
+                       nimg, tw);
   else
     hipLaunchKernelGGL((k_cols_t<N, +1>), grid, dim3(kThreads), lds, c.stream, data, img_stride,
-                       tw);
+                       nimg, tw);
   FR_HIP(hipGetLastError());
   return 0;
 }
@@ -511,11 +424,23 @@ static int cols_pass(Ctx& c, int nsam, int nimg, float* data, size_t img_stride,
   if (cw < 1) cw = 1;
   if (cw > jc) cw = jc;
   size_t lds = sizeof(float2) * 2 * cw * (nsam + 1);
-  FR_CHECK(nimg <= 65535, "too many images in one FFT batch (%d)", nimg);
-  dim3 grid((jc + cw - 1) / cw, nimg);
-  hipLaunchKernelGGL(k_cols, grid, dim3(kThreads), lds, c.stream, data, pl, img_stride, cw, sign, tw);
+  const long long nblk = (long long)((jc + cw - 1) / cw) * nimg;
+  FR_CHECK(nblk <= (1ll << 30), "too many images in one FFT batch (%d)", nimg);
+  dim3 grid(round8((int)nblk));
+  hipLaunchKernelGGL(k_cols, grid, dim3(kThreads), lds, c.stream, data, pl, img_stride, cw, nimg,
+                     sign, tw);
   FR_HIP(hipGetLastError());
   return 0;
+}
+
+int fft_rows_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, bool fwd) {
+  if (nimg <= 0) return 0;
+  return rows_pass(c, nsam, nimg, data, img_stride, fwd);
+}
+
+int fft_cols_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride, int sign) {
+  if (nimg <= 0) return 0;
+  return cols_pass(c, nsam, nimg, data, img_stride, sign);
 }
 
 int fft_r2c_2d(Ctx& c, int nsam, int nimg, float* data, size_t img_stride) {
@@ -540,7 +465,7 @@ static int z_pass(Ctx& c, int nsam, float* data, int sign) {
   int cw = 2048 / nsam;
   if (cw < 1) cw = 1;
   const size_t lds = sizeof(float2) * 2 * cw * (nsam + 1);
-  dim3 grid((ncols + cw - 1) / cw);
+  dim3 grid(round8((ncols + cw - 1) / cw));
   hipLaunchKernelGGL(k_cols_z, grid, dim3(kThreads), lds, c.stream, data, pl, ncols,
                      (size_t)ncols, cw, sign, tw);
   FR_HIP(hipGetLastError());

@@ -78,12 +78,16 @@ struct PrepGlobals {
   float cs, wl, wgh1, wgh2, thetatr, rih, halfw, ri2, ri3, tx, ty;
   int ictf;
 };
+// part: the ring partial sums of MASKCOS's mean (prep_partials_count(nsam,
+// nb) float2; null: the context's buffer, for calls on the context's stream)
 int launch_prep(Ctx& c, int nsam, int nb, float* data, const float* ppar, const float* pssnr,
-                const PrepGlobals& g, float* ctff, float* outc, float* datc);
+                const PrepGlobals& g, float* ctff, float* outc, float* datc,
+                float2* part = nullptr);
 // ctfapply.f:8-161 alone, from nb spectra (NSAM x JC complex each, as the r2c leaves them)
 int launch_ctfapply(Ctx& c, int nsam, int nb, const float* spec, const float* ppar,
                     const float* pssnr, const PrepGlobals& g, float* ctff, float* outc,
-                    float* datc);
+                    float* datc, float2* part = nullptr);
+size_t prep_partials_count(int nsam, int nb);
 
 // Central-section extraction (pextract.f:8-101), one output plane per item.
 int launch_pextract(Ctx& c, int nsam, int n, const float* a3df, const float* dm, const int* pidx,

@@ -53,6 +53,7 @@ struct fr_plan {
   int* d_pidx = nullptr;
   InsertTask* d_tasks = nullptr;
   unsigned char* d_ins = nullptr;  // insertion scratch
+  float* d_part = nullptr;         // the preparation's ring sums (own: plans run concurrently)
   // pinned staging
   float *h_ppar = nullptr, *h_dm = nullptr, *h_shift = nullptr, *h_amag = nullptr,
         *h_score = nullptr;
@@ -74,7 +75,7 @@ int enqueue(fr_plan* p) {
   FR_HIP(hipMemcpyAsync(p->d_ppar, p->h_ppar, sizeof(float) * 8 * nb, hipMemcpyHostToDevice,
                         c->stream));
   if (int r = launch_prep(*c, nsam, nb, p->d_img, p->d_ppar, p->pssnr, p->g, p->d_ctff, p->d_outc,
-                          p->d_datc))
+                          p->d_datc, reinterpret_cast<float2*>(p->d_part)))
     return r;
   FR_HIP(hipMemcpyAsync(p->d_dm, p->h_dm, sizeof(float) * 9 * nb, hipMemcpyHostToDevice,
                         c->stream));
@@ -226,6 +227,7 @@ int fr_plan_create(fr_ctx* c, int nsam, int nb, const float* d_a3df, const float
     return fail(r);
   if (!p->fused && (r = dalloc(&p->d_work, img * nb))) return fail(r);
   if ((r = dalloc(&p->d_ins, insert_scratch_bytes(nsam, nb, nb * ncopy)))) return fail(r);
+  if ((r = dalloc(&p->d_part, 2 * prep_partials_count(nsam, nb)))) return fail(r);
   if (p->fused && (r = c->cc3m_tables(p->ca, &p->tabs))) return fail(r);
   std::vector<int> pid(nb);
   for (int b = 0; b < nb; b++) pid[b] = b;
@@ -306,7 +308,8 @@ int fr_plan_destroy(fr_plan* p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   void* dev[] = {p->d_img, p->d_ppar, p->d_ctff, p->d_outc, p->d_datc, p->d_dm, p->d_shift,
-                 p->d_amag, p->d_score, p->d_work, p->d_pidx, p->d_tasks, p->d_ins};
+                 p->d_amag, p->d_score, p->d_work, p->d_pidx, p->d_tasks, p->d_ins,
+                 p->d_part};
   for (void* d : dev)
     if (d) (void)hipFree(d);
   void* host[] = {p->h_ppar, p->h_dm, p->h_shift, p->h_amag, p->h_score, p->h_tasks};

@@ -102,10 +102,15 @@ def test_ctfapply_vs_oracle(ctx, nsam):
     ppar[:, 3] = ppar[:, 2] + rng.uniform(0, 1000, nb)
     ppar[:, 4] = rng.uniform(-1, 1, nb)
     ppar[:, 5] = 1.0
-    ctff, outc, datc = ctx.ctfapply(nsam, torch.from_numpy(buf.copy()).to(DEV),
-                                    torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV),
-                                    g)
+    imgs = torch.from_numpy(buf.copy()).to(DEV)
+    ctff, outc, datc = ctx.ctfapply(nsam, imgs, torch.from_numpy(ppar).to(DEV),
+                                    torch.from_numpy(ps).to(DEV), g)
     ctff, outc, datc = ctff.cpu().numpy(), outc.cpu().numpy(), datc.cpu().numpy()
+    # the images hold their r2c spectra afterwards (lmain.f:459, in place),
+    # the same bits as the FFT entry point gives
+    ref = torch.from_numpy(buf.copy()).to(DEV)
+    ctx.fft_r2c(nsam, ref)
+    assert torch.equal(imgs.view(torch.int32), ref.view(torch.int32))
     # PWEIGHT's shell sums run in a fixed order: a rerun gives the same bits
     again = ctx.ctfapply(nsam, torch.from_numpy(buf.copy()).to(DEV),
                          torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV), g)
@@ -132,6 +137,53 @@ def test_ctfapply_vs_oracle(ctx, nsam):
             e64 = rel_l2(gpu, res["or64_"][k])
             print(f"nsam={nsam} particle {b} {name}: rel L2 vs FP32 oracle {e32:.2e}, vs FP64 {e64:.2e}")
             assert e32 < 1e-5, name
+
+
+@pytest.mark.parametrize("nsam", [64, 96, 256])
+def test_ctfapply_from_spectrum_equals_prep(ctx, nsam):
+    """CTFAPPLY from a particle's spectrum (the per-call ctfapply_, which the
+    defocus refinement's path shares) against fr_prep_batch from the image, on
+    the spectrum fr_prep_batch leaves: CTFF, OUTC and DATC agree bit for bit
+    (the same transforms and point operations, whether the column pass
+    starts from the image's row transforms or from the whole spectrum)."""
+    from frealign_amd import _native as N
+    nb = 2
+    jc = nsam // 2 + 1
+    g = gpar_for(nsam)
+    ps = pssnr_for(nsam)
+    buf = np.zeros((nb, nsam, nsam + 2), np.float32)
+    buf[:, :, :nsam] = rng.standard_normal((nb, nsam, nsam))
+    ppar = np.zeros((nb, 8), np.float32)
+    ppar[:, 0:2] = rng.uniform(-0.3, 0.3, (nb, 2))
+    ppar[:, 2] = rng.uniform(10000, 30000, nb)
+    ppar[:, 3] = ppar[:, 2] + rng.uniform(0, 1000, nb)
+    ppar[:, 4] = rng.uniform(-1, 1, nb)
+    ppar[:, 5] = 1.0
+    imgs = torch.from_numpy(buf).to(DEV)
+    ctff, outc, datc = [t.cpu().numpy() for t in ctx.ctfapply(
+        nsam, imgs, torch.from_numpy(ppar).to(DEV), torch.from_numpy(ps).to(DEV), g)]
+    spec = imgs.cpu().numpy()
+    lib = N.lib()
+    fp = lambda v: C.byref(C.c_float(float(v)))  # noqa: E731
+    for b in range(nb):
+        sp = np.ascontiguousarray(spec[b])
+        cf_ = np.zeros(4 * nsam * jc, np.float32)
+        od = np.zeros(2 * nsam * jc, np.float32)
+        dc = np.zeros(2 * nsam * jc, np.float32)
+        scratch = np.zeros(4 * nsam * jc, np.float32)
+        args = [C.byref(C.c_int(nsam)), sp.ctypes, fp(ppar[b, 0]), fp(ppar[b, 1]), fp(g[0]),
+                fp(g[1]), fp(g[2]), fp(g[3]), fp(ppar[b, 2]), fp(ppar[b, 3]), fp(ppar[b, 4]),
+                fp(g[4]), cf_.ctypes, od.ctypes, scratch.ctypes, fp(ppar[b, 5]), fp(g[5]),
+                fp(g[6]), fp(g[7]), fp(g[8]), fp(0.0), scratch.ctypes, dc.ctypes,
+                scratch.ctypes, fp(0.0), fp(0.0), fp(0.0), scratch.ctypes, fp(0.0),
+                C.byref(C.c_int(0)), fp(g[9]), fp(g[10]), C.c_char_p(b"C1 "), ps.ctypes,
+                scratch.ctypes, scratch.ctypes, C.byref(C.c_int(1)), scratch.ctypes,
+                C.c_size_t(3)]
+        lib.ctfapply_(*args)
+        for name, got, want in (("ctff", cf_, ctff[b]), ("outc", od, outc[b]),
+                                ("datc", dc, datc[b])):
+            assert np.array_equal(got.view(np.uint32), np.ascontiguousarray(want).reshape(
+                -1).view(np.uint32)), f"nsam={nsam} particle {b} {name}"
 
 
 # ------------------------------------------------------------- PEXTRACT
