@@ -241,6 +241,10 @@ __device__ inline void tile_r2(int v0, int nsamh, int nsam, int& lo, int& hi) {
   hi = la > lb ? la : lb;
 }
 
+// words of the band point table in LDS (header and at most 32 x 63 points),
+// even so that the float2 band after it is 8-byte aligned
+constexpr int cc3m_pts_words() { return (kCc3mPtsHdr + 32 * (2 * kFJ + 1) + 1) & ~1; }
+
 // kWin: the score at every shift of a window around the item's shift (the
 // best of them and its index), instead of at the item's shift alone
 template <bool kWin>
@@ -263,6 +267,9 @@ __global__ void __launch_bounds__(kFT)
   float* bins = Bs + 64 * kFP;          // [4][nb]
   const int nb = nsam / 2 + 2 > kCc3mShells ? nsam / 2 + 2 : kCc3mShells;
   int* pts_s = reinterpret_cast<int*>(bins + 4 * nb);  // the band point table
+  // the particle's spectrum at the band points, [I][J + 31], read once in the
+  // band fill (its loads in flight with the gathers) for the shell sums
+  float2* Sp = reinterpret_cast<float2*>(pts_s + cc3m_pts_words());
   __shared__ float red_s[kFT / 64];
   __shared__ int red_c[kFT / 64];
   // e^{i SHX I} and e^{i SHY J} of the item: the shift phase of a point is
@@ -309,6 +316,7 @@ __global__ void __launch_bounds__(kFT)
   {
     const float* d = dm + 9 * (size_t)it;
     const float2* cp = ctff + (size_t)p * 2 * ic;
+    const float2* sps = spec + (size_t)p * ic;
     const float scal = 1.0f / (float)nsam / (float)nsam;
     // unrolled: the gathers of a thread's points are in flight together
 #pragma unroll 4
@@ -320,6 +328,7 @@ __global__ void __launch_bounds__(kFT)
         size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
         cf a = ainterpo_dev(nsam, a3df, d, i, j);
         float2 cr = cp[id], cl = cp[id + ic];
+        Sp[e] = sps[id];
         if (flip) {  // cc3.f:86-89
           if (cr.x + cl.x < 0.0f) a = cf{-a.re, -a.im};
         } else {
@@ -553,12 +562,29 @@ __global__ void __launch_bounds__(kFT)
           z[kt] = mfma4(Ts[(16 * t + 4 * lg + i) * kFP + 16 * kt + la], kt < 2 ? sp[i] : sm[i],
                         z[kt]);
     };
+    // the table operands of a tile's Z products, loaded ahead of the
+    // products and the folds they wait for
+    auto zload = [&](int t, float (&ta)[16]) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) ta[4 * i + kt] = Ts[(16 * t + 4 * lg + i) * kFP + 16 * kt + la];
+    };
+    auto zacc_r = [&](const float (&ta)[16], const fx4& sp, const fx4& sm) {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int kt = 0; kt < 4; kt++) z[kt] = mfma4(ta[4 * i + kt], kt < 2 ? sp[i] : sm[i], z[kt]);
+    };
     while (__builtin_popcount(act) >= 2) {
       const int t0 = __builtin_ctz(act);
       act &= act - 1u;
       const int t1 = __builtin_ctz(act);
       act &= act - 1u;
       fx4 e0, o0, e1, o1, p0, q0, p1, q1;
+      float za0[16], za1[16];
+      zload(t0, za0);
+      zload(t1, za1);
       // the weights first: their loads are in flight under the products
       const fx4 w0 = weights(16 * t0), w1 = weights(16 * t1);
       e0 = fx4{0.f, 0.f, 0.f, 0.f};
@@ -574,8 +600,8 @@ __global__ void __launch_bounds__(kFT)
       }
       fold(t0, w0, e0, o0, p0, q0);
       fold(t1, w1, e1, o1, p1, q1);
-      zacc(t0, p0, q0);
-      zacc(t1, p1, q1);
+      zacc_r(za0, p0, q0);
+      zacc_r(za1, p1, q1);
     }
     if (act) {
       const int t0 = __builtin_ctz(act);
@@ -666,7 +692,6 @@ __global__ void __launch_bounds__(kFT)
   // balance), walks their points from the sorted table and reduces them with
   // a fixed butterfly: no atomics, the same sums on every run.
   {
-    const float2* sp = spec + (size_t)p * ic;
     for (int sh = 0; sh < kCc3mShells; sh++) {
       const int z = sh & 15;
       if ((z < 8 ? z : 15 - z) != wave) continue;
@@ -682,7 +707,7 @@ __global__ void __launch_bounds__(kFT)
         float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
         const float2 ex = phx[i], ey = phy[j + kFJ];
         const float cs = ex.x * ey.x - ex.y * ey.y, sn = ex.x * ey.y + ex.y * ey.x;
-        float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
+        const float2 sv = Sp[i * (2 * kFJ + 1) + j + kFJ];
         cf samp = cmul(cf{sv.x, sv.y}, cf{cs, sn});
         q += samp.re * a.x + samp.im * a.y;
         s2 += samp.re * samp.re + samp.im * samp.im;
@@ -738,7 +763,6 @@ __global__ void __launch_bounds__(kFT)
   // two small tables, and adds R_k WGT into its FP64 partial per shift.
   float2* Cs = reinterpret_cast<float2*>(Ys);
   {
-    const float2* sp = spec + (size_t)p * ic;
     for (int sh = 0; sh < kCc3mShells; sh++) {
       const int z = sh & 15;
       if ((z < 8 ? z : 15 - z) != wave) continue;
@@ -751,7 +775,7 @@ __global__ void __launch_bounds__(kFT)
         float cre = Bs[jj * kFP + i], cim = Bs[jj * kFP + kFK / 2 + i];
         float sre = Bs[(kFK / 2 + jj) * kFP + i], sim = Bs[(kFK / 2 + jj) * kFP + kFK / 2 + i];
         float2 a = j >= 0 ? make_float2(cre - sim, cim + sre) : make_float2(cre + sim, cim - sre);
-        float2 sv = sp[(size_t)i + (size_t)jc * wrapn(j, nsam)];
+        const float2 sv = Sp[i * (2 * kFJ + 1) + j + kFJ];
         Cs[e] = make_float2(sv.x * a.x + sv.y * a.y, sv.y * a.x - sv.x * a.y);
         s2 += sv.x * sv.x + sv.y * sv.y;
         a2 += a.x * a.x + a.y * a.y;
@@ -886,10 +910,10 @@ bool cc3m_fused_ok(int nsam, int maxr2) {
 
 size_t cc3m_fused_lds(int nsam) {
   const int nb = nsam / 2 + 2 > kCc3mShells ? nsam / 2 + 2 : kCc3mShells;
-  // TF, Y/Z, B/out, the shell bins and the band point table (header and at
-  // most 32 x 63 points)
+  // TF, Y/Z, B/out, the shell bins, the band point table (header and at
+  // most 32 x 63 points, padded to 8 bytes) and the particle's band
   return sizeof(float) * ((size_t)(nsam / 2 + 1) * kFP + (size_t)nsam * kFP + 64 * kFP + 4 * nb) +
-         sizeof(int) * (kCc3mPtsHdr + 32 * (2 * kFJ + 1));
+         sizeof(int) * cc3m_pts_words() + sizeof(float2) * 32 * (2 * kFJ + 1);
 }
 
 // T[n][2i] = cos(2 pi i n/N), T[n][2i+1] = -sin(2 pi i n/N) for i < 32, from
