@@ -86,7 +86,7 @@ def test_fft_r2c_c2r(ctx, nsam):
 
 
 # ------------------------------------------------------------- CTFAPPLY
-@pytest.mark.parametrize("nsam", [64, 96, 256])
+@pytest.mark.parametrize("nsam", [64, 96, 128, 256])
 def test_ctfapply_vs_oracle(ctx, nsam):
     nb = 3
     jc = nsam // 2 + 1

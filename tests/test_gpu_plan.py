@@ -1,11 +1,12 @@
 """Graph-captured batches (fr_plan_*) and the CC3M autotuner.
 
 A plan's replay must equal the eager sequence fr_prep_batch ->
-fr_cc3m_batch -> fr_insert_batch on the same batch: NKSUM exactly; CC3M
-scores, TTD and the half-maps up to the order of float atomics (the shell
-sums of PWEIGHT and CC3M accumulate in LDS with float atomics, so two runs
-may differ in the last bit: scores within 1e-6 relative, maps 1e-6 rel L2).  The graph is replayed several times on different batches,
-with eager work between the replays, and every replay is checked.  The
+fr_cc3m_batch -> fr_insert_batch on the same batch bit for bit (NKSUM, CC3M
+scores, TTD and the half-maps): every kernel on the path sums in a fixed
+order.  Box sizes 64 and 128 take the fused preparation (its ring sums in
+the plan's own buffer), 96 the one-kernel-per-step form.  The graph is
+replayed several times on different batches, with eager work between the
+replays, and every replay is checked.  The
 autotuner's choice must be one of its measured candidates, and each CC3M
 variant it can choose must give the same scores (1e-6 between persistent
 grid sizes of the fused kernel, whose shell sums use LDS float atomics; 1e-4
@@ -46,7 +47,7 @@ def _records(parts, rng, nb):
     return recb, half
 
 
-@pytest.mark.parametrize("nsam,asym,nasym", [(64, "C", 1), (96, "D", 7)])
+@pytest.mark.parametrize("nsam,asym,nasym", [(64, "C", 1), (96, "D", 7), (128, "C", 2)])
 def test_plan_replays_equal_eager(ctx, nsam, asym, nasym):
     psize, ri, nb = 2.0, 0.4 * nsam, 8
     optics = synth.Optics(psize=psize)
