@@ -296,6 +296,12 @@ __global__ void __launch_bounds__(kFT)
   }
   // the band point table, once (item independent)
   for (int e = tid; e < kCc3mPtsHdr + pts[kCc3mShells]; e += kFT) pts_s[e] = pts[e];
+  // this thread's entries e = tid + 512 r of the ring's Q table (the same for
+  // every item), held in registers across the items
+  static_assert(64 * 64 % kFT == 0, "Q entries per thread");
+  float qv[64 * 64 / kFT];
+#pragma unroll
+  for (int r = 0; r < 64 * 64 / kFT; r++) qv[r] = ring_tab ? ring_tab[tid + r * kFT] : 0.f;
   for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
   const int p = pidx[it];
   const float amagp = amag_arr[it];
@@ -346,7 +352,9 @@ __global__ void __launch_bounds__(kFT)
   const bool ringq = ring_tab != nullptr && amagp == 1.0f;
   float rsum = 0.f;
   int rcnt = 0;
-  for (int e = tid; e < 64 * 64; e += kFT) {
+#pragma unroll
+  for (int r = 0; r < 64 * 64 / kFT; r++) {
+    const int e = tid + r * kFT;
     int c = e >> 6, k = e & 63;
     int i = c >> 1, jj = k >> 1;
     float2 xp = Xs[i * (2 * kFJ + 1) + kFJ + jj];
@@ -357,7 +365,7 @@ __global__ void __launch_bounds__(kFT)
     if (!(c & 1)) v = (k & 1) ? Q.y : P.x;   // Y_re = P_re T_re + Q_im T_im
     else v = (k & 1) ? -Q.x : P.y;           // Y_im = P_im T_re - Q_re T_im
     Bs[c * kFP + k] = v;
-    if (ringq) rsum += ring_tab[e] * v;
+    if (ringq) rsum += qv[r] * v;
   }
   __syncthreads();
 
