@@ -41,7 +41,7 @@ struct fr_ctx : public fr::Ctx {
   // launch choices per box size (set by fr_autotune_cc3m)
   struct Tune {
     int cc3m_fused = 1;   // band-pruned fused CC3M where it applies
-    int cc3m_blocks = 0;  // persistent workgroups (0: one per item)
+    int cc3m_blocks = -1;  // persistent workgroups (0: one per item; -1: one full wave)
   };
   std::map<int, Tune> tune;
   Tune tune_for(int nsam) const;

@@ -1057,8 +1057,21 @@ int launch_cc3m_fused(Ctx& c, const Cc3mArgs& a, const float* a3df, const float*
   }
   const void* fn = win ? (const void*)k_cc3m_fused<true> : (const void*)k_cc3m_fused<false>;
   FR_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  // persistent grid: `blocks` workgroups (0: one per item), each looping over items
-  const int grid = (blocks > 0 && blocks < a.n) ? blocks : a.n;
+  // persistent grid: `blocks` workgroups (0: one per item; below 0: one full
+  // wave of workgroups), each looping over items, so that the tables are
+  // staged once per workgroup rather than once per item
+  int grid = (blocks > 0 && blocks < a.n) ? blocks : a.n;
+  if (blocks < 0) {
+    if (c.cus <= 0) {
+      hipDeviceProp_t prop;
+      FR_HIP(hipGetDeviceProperties(&prop, c.device));
+      c.cus = prop.multiProcessorCount;
+    }
+    int per_cu = 0;
+    FR_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kFT, lds));
+    const long long full = (long long)c.cus * (per_cu > 0 ? per_cu : 1);
+    if (full < a.n) grid = (int)full;
+  }
   if (win)
     hipLaunchKernelGGL(k_cc3m_fused<true>, dim3((unsigned)grid), dim3(kFT), lds, c.stream, a.nsam,
                        a.n, a.maxr1 * a.maxr1, a.maxr2 * a.maxr2, a.rbfact,

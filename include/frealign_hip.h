@@ -802,7 +802,9 @@ int fr_plan_destroy(fr_plan *plan);
  * NSAM in the context and writes the timings and the choice as JSON into
  * log (logcap bytes, may be NULL). */
 /* Sets the CC3M variant for NSAM by hand: fused (1) or FFT path (0), and the
- * fused kernel's persistent workgroup count (0: one per item). */
+ * fused kernel's persistent workgroup count (0: one per item).  With no
+ * choice made for NSAM the fused kernel runs one full wave of workgroups (as
+ * many as the device holds at once), each looping over the items. */
 int fr_set_cc3m_variant(fr_ctx *ctx, int nsam, int fused, int workgroups);
 int fr_autotune_cc3m(fr_ctx *ctx, int nsam, int n, const float *d_a3df, const float *d_spec,
                      const float *d_ctff, const int *d_pidx, const float *d_dm,
