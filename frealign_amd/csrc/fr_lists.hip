@@ -55,6 +55,30 @@ __device__ void block_scan(int n, F val, int* out) {
   if (tid == kScan - 1) out[n] = (int)part[kScan - 1];
 }
 
+// Stream compaction by the same contiguous runs: out[1 + k] = the k-th index
+// i < n (ascending) with keep(i), out[0] = their count.
+template <typename F>
+__device__ void block_compact(int n, F keep, int* out) {
+  __shared__ int cpart[kScan];
+  const int tid = threadIdx.x;
+  const int per = (n + kScan - 1) / kScan;
+  const int i0 = tid * per, i1 = min(n, i0 + per);
+  int s = 0;
+  for (int i = i0; i < i1; i++) s += keep(i) ? 1 : 0;
+  cpart[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < kScan; off <<= 1) {
+    int v = tid >= off ? cpart[tid - off] : 0;
+    __syncthreads();
+    cpart[tid] += v;
+    __syncthreads();
+  }
+  int run = cpart[tid] - s;
+  for (int i = i0; i < i1; i++)
+    if (keep(i)) out[1 + run++] = i;
+  if (tid == kScan - 1) out[0] = cpart[kScan - 1];
+}
+
 // List length, group and confidence of every particle, and the list offsets.
 __global__ void __launch_bounds__(kScan)
     k_list_sizes(int np, int ktop, const int* __restrict__ top, const float* __restrict__ prob,
@@ -129,12 +153,9 @@ __global__ void __launch_bounds__(kScan)
     int n = loff[i + 1] - loff[i];
     return n > 0 ? (n + kTile - 1) / kTile : 0;
   }, toff);
-  if (info && fb && threadIdx.x == 0) {
-    int cnt = 0;
-    for (int i = 0; i < np; i++)
-      if (!info[2 * i] && loff[i + 1] > loff[i]) fb[1 + cnt++] = i;
-    fb[0] = cnt;
-  }
+  // the fallback particles (not confident, non-empty list) in index order
+  if (info && fb)
+    block_compact(np, [&](int i) { return !info[2 * i] && loff[i + 1] > loff[i]; }, fb);
 }
 
 // CC3M items of the top-k candidates: pidx, rotation matrix, shift, AMAG.
