@@ -1213,11 +1213,14 @@ int fr_search_ragged_batch(fr_ctx* c, int nsam, int maxr1, int maxr2, float rbfa
       fb = (int*)c->slot(34, sizeof(int) * ((size_t)np + 1), &err);
       if (!fb) return err;
     }
-    if (int r = launch_tile_offsets(*c, np, d_loff, toff, d_info, fb)) return r;
+    // the tiles in the order of the particles' first list entries
+    int* ord = (int*)c->slot(35, sizeof(int) * (size_t)np, &err);
+    if (!ord) return err;
+    if (int r = launch_tile_offsets(*c, np, d_loff, toff, d_info, fb, d_cand, ord)) return r;
     int pslot = c->prof_begin(kProfSearchList);
     if (int r = launch_search_sep(*c, np, norient, c->nsamr, 0, d_loff, toff, cap,
                                   (int)(max_tiles < (1LL << 30) ? max_tiles : (1LL << 30)), d_cand,
-                                  d_bank, xa, xb, c2s, sum2, score, brow, fb))
+                                  d_bank, xa, xb, c2s, sum2, score, brow, fb, ord))
       return r;
     c->prof_end(pslot);
     return launch_topk(*c, norient, np, 0, 0, d_loff, d_grp, d_cand, cap, topk, score, brow, d_top_o,

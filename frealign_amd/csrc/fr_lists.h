@@ -8,9 +8,11 @@ int launch_list_build(Ctx& c, int np, int ktop, const int* top, const float* pro
                       int bank_group, int maxlen, int* len_scratch, int* loff, int* grp, int* info,
                       int* cand);
 // toff[0..np] tile offsets of the lists; with info (k_list_sizes' flags) the
-// fallback particles get no tiles and are listed in fb (fb[0] = count)
+// fallback particles get no tiles and are listed in fb (fb[0] = count); with
+// cand and ord the tiles follow the particles ord[0..np) (sorted by their
+// first list entry), toff[q] being the first tile of particle ord[q]
 int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff, const int* info = nullptr,
-                        int* fb = nullptr);
+                        int* fb = nullptr, const int* cand = nullptr, int* ord = nullptr);
 int launch_cand_items(Ctx& c, int np, int k, const int* top_o, const int* top_row, int nbank,
                       const float* dm_bank, int nsamr, int* pidx, float* dm, float* shift,
                       float* amag);

@@ -145,14 +145,56 @@ __global__ void __launch_bounds__(kFill)
 // confidence flags of k_list_sizes) the particles that fall back to the
 // whole bank get no tiles here; they are listed in fb instead (fb[0] = their
 // count, fb[1..] the particles in order) for the shared-template search.
+// With `ord` the tiles are laid out in the order of the particles' first
+// list entries (their most probable class's first cell; ties by index):
+// ord[q] is the particle whose tiles come q-th, so that particles proposing
+// the same classes are searched at the same time and share their template
+// reads in the caches.  Batches above kOrdMax particles keep index order.
+constexpr int kOrdMax = 4096;
+
 __global__ void __launch_bounds__(kScan)
     k_tile_offsets(int np, const int* __restrict__ loff, const int* __restrict__ info, int* toff,
-                   int* fb) {
-  block_scan(np, [&](int i) {
+                   int* fb, const int* __restrict__ cand, int* ord) {
+  __shared__ long long skey[kOrdMax];
+  const int tid = threadIdx.x;
+  auto tiles = [&](int i) {
     if (info && !info[2 * i]) return 0;
     int n = loff[i + 1] - loff[i];
     return n > 0 ? (n + kTile - 1) / kTile : 0;
-  }, toff);
+  };
+  const bool sorted = ord && cand && np <= kOrdMax;
+  if (sorted) {
+    int n2 = 1;
+    while (n2 < np) n2 <<= 1;
+    for (int i = tid; i < n2; i += kScan) {
+      long long key = 0x7fffffffffffffffLL;
+      if (i < np) {
+        const int k0 = tiles(i) > 0 ? cand[loff[i]] : 0x7fffffff;
+        key = (long long)k0 * 4294967296LL + i;
+      }
+      skey[i] = key;
+    }
+    __syncthreads();
+    // bitonic sort, ascending
+    for (int k = 2; k <= n2; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        for (int i = tid; i < n2; i += kScan) {
+          const int l = i ^ j;
+          if (l > i) {
+            const long long x = skey[i], y = skey[l];
+            if ((x > y) == ((i & k) == 0)) {
+              skey[i] = y;
+              skey[l] = x;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    for (int q = tid; q < np; q += kScan) ord[q] = (int)(skey[q] & 0xffffffffLL);
+  } else if (ord) {
+    for (int q = tid; q < np; q += kScan) ord[q] = q;
+  }
+  block_scan(np, [&](int q) { return tiles(sorted ? (int)(skey[q] & 0xffffffffLL) : q); }, toff);
   // the fallback particles (not confident, non-empty list) in index order
   if (info && fb)
     block_compact(np, [&](int i) { return !info[2 * i] && loff[i + 1] > loff[i]; }, fb);
@@ -218,8 +260,10 @@ int launch_list_build(Ctx& c, int np, int ktop, const int* top, const float* pro
   return 0;
 }
 
-int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff, const int* info, int* fb) {
-  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kScan), 0, c.stream, np, loff, info, toff, fb);
+int launch_tile_offsets(Ctx& c, int np, const int* loff, int* toff, const int* info, int* fb,
+                        const int* cand, int* ord) {
+  hipLaunchKernelGGL(k_tile_offsets, dim3(1), dim3(kScan), 0, c.stream, np, loff, info, toff, fb,
+                     cand, ord);
   FR_HIP(hipGetLastError());
   return 0;
 }

@@ -33,11 +33,13 @@ int launch_sep_feats(Ctx& c, int nsam, int np, int lo2, int hi2, float rbfact, c
 // fb (ragged lists only, may be null): the fallback particles (fb[0] = count,
 // fb[1..] particles) whose lists are the whole bank in order; they are
 // searched four to a workgroup against shared templates, as the uniform form,
-// in the same persistent launch as the listed particles' tiles
+// in the same persistent launch as the listed particles' tiles.  ord (may be
+// null): the particle of each toff slot (launch_tile_offsets' order)
 int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int* loff,
                       const int* toff, int cap, int max_tiles, const int* cand, const void* bank,
                       const void* xa, const void* xb, const float* c2s, const float* sum2,
-                      float* score, int* best_row, const int* fb = nullptr);
+                      float* score, int* best_row, const int* fb = nullptr,
+                      const int* ord = nullptr);
 int launch_topk(Ctx& c, int nbank, int np, int n, int group, const int* loff, const int* grp,
                 const int* cand, int cap, int topk, const float* score, const int* best_row,
                 int* top_o, int* top_row, float* top_s);

@@ -173,7 +173,8 @@ enum { kSepUniform = 0, kSepRagged = 1, kSepFallback = 2, kSepMixed = 3 };
 template <int kMode>
 __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
                                 const int* __restrict__ loff, const int* __restrict__ toff,
-                                int cap, const int* __restrict__ fb, SepItem& it) {
+                                int cap, const int* __restrict__ fb,
+                                const int* __restrict__ ord, SepItem& it) {
   if (kMode == kSepFallback) {
     // the uniform mapping over the fallback particles fb[1 .. fb[0]], whose
     // lists are the bank in order at loff[p]
@@ -204,16 +205,17 @@ __device__ inline bool sep_item(int idx, int wave, int np, int nrow, int ntile,
     it.listed = true;
     return true;
   }
-  // the particle whose tile range holds idx: the largest p with toff[p] <= idx
+  // the slot whose tile range holds idx: the largest q with toff[q] <= idx;
+  // its particle is ord[q] (q itself without an order)
   int lo = 0, hi = np - 1;
   while (lo < hi) {
     int mid = (lo + hi + 1) >> 1;
     if (toff[mid] <= idx) lo = mid;
     else hi = mid - 1;
   }
-  it.p = lo;
-  it.base = loff[lo];
-  const int end = loff[lo + 1] < cap ? loff[lo + 1] : cap;
+  it.p = ord ? ord[lo] : lo;
+  it.base = loff[it.p];
+  const int end = loff[it.p + 1] < cap ? loff[it.p + 1] : cap;
   it.lo = (idx - toff[lo]) * kSepTile + wave * 64;
   it.hi = min(end - it.base, it.lo + 64);
   it.listed = true;
@@ -227,7 +229,7 @@ __global__ void __launch_bounds__(kSepThreads, 2)
                  const uint4* __restrict__ bank, const uint4* __restrict__ xa,
                  const uint4* __restrict__ xb, const float4* __restrict__ c2s,
                  const float* __restrict__ sum2, float* score, int* best_row,
-                 const int* __restrict__ fb) {
+                 const int* __restrict__ fb, const int* __restrict__ ord) {
   constexpr bool kRagged = kMode == kSepRagged;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int g = lane >> 4, n = lane & 15;
@@ -365,11 +367,12 @@ __global__ void __launch_bounds__(kSepThreads, 2)
     SepItem it;
     bool ok;
     if (kMode == kSepMixed) {
-      ok = idx < nlist ? sep_item<kSepRagged>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it)
-                       : sep_item<kSepFallback>(idx - nlist, wave, np, nrow, ntile, loff, toff,
-                                                cap, fb, it);
+      ok = idx < nlist
+               ? sep_item<kSepRagged>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, ord, it)
+               : sep_item<kSepFallback>(idx - nlist, wave, np, nrow, ntile, loff, toff, cap, fb,
+                                        ord, it);
     } else {
-      ok = sep_item<kMode>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, it);
+      ok = sep_item<kMode>(idx, wave, np, nrow, ntile, loff, toff, cap, fb, ord, it);
     }
     if (!ok) continue;
     if (it.p != pcur) {
@@ -453,7 +456,7 @@ int launch_sep_feats(Ctx& c, int nsam, int np, int lo2, int hi2, float rbfact, c
 int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int* loff,
                       const int* toff, int cap, int max_tiles, const int* cand, const void* bank,
                       const void* xa, const void* xb, const float* c2s, const float* sum2,
-                      float* score, int* best_row, const int* fb) {
+                      float* score, int* best_row, const int* fb, const int* ord) {
   const uint4* bk = reinterpret_cast<const uint4*>(bank);
   const uint4* a = reinterpret_cast<const uint4*>(xa);
   const uint4* b = reinterpret_cast<const uint4*>(xb);
@@ -465,7 +468,7 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
     if (nblk == 0) return 0;
     hipLaunchKernelGGL(k_search_sep<kSepUniform>, dim3((unsigned)nblk), dim3(kSepThreads), 0,
                        c.stream, np, nbank, nsamr, nrow, ntile, nullptr, nullptr, 0, cand, bk, a,
-                       b, cc, sum2, score, best_row, nullptr);
+                       b, cc, sum2, score, best_row, nullptr, nullptr);
     FR_HIP(hipGetLastError());
     return 0;
   }
@@ -487,7 +490,7 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
     const int ntile = (nbank + kSepFbTile - 1) / kSepFbTile;
     hipLaunchKernelGGL(k_search_sep<kSepMixed>, dim3(full), dim3(kSepThreads), 0, c.stream, np,
                        nbank, nsamr, nbank, ntile, loff, toff, cap, cand, bk, a, b, cc, sum2,
-                       score, best_row, fb);
+                       score, best_row, fb, ord);
     FR_HIP(hipGetLastError());
     return 0;
   }
@@ -496,7 +499,7 @@ int launch_search_sep(Ctx& c, int np, int nbank, int nsamr, int nrow, const int*
   if (grid >= 1) {
     hipLaunchKernelGGL(k_search_sep<kSepRagged>, dim3(grid), dim3(kSepThreads), 0, c.stream, np,
                        nbank, nsamr, 0, 0, loff, toff, cap, cand, bk, a, b, cc, sum2, score,
-                       best_row, nullptr);
+                       best_row, nullptr, ord);
     FR_HIP(hipGetLastError());
   }
   return 0;
