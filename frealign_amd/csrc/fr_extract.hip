@@ -302,8 +302,15 @@ __global__ void __launch_bounds__(kFT)
   float qv[64 * 64 / kFT];
 #pragma unroll
   for (int r = 0; r < 64 * 64 / kFT; r++) qv[r] = ring_tab ? ring_tab[tid + r * kFT] : 0.f;
-  for (int it = blockIdx.x; it < nitem; it += gridDim.x) {
+  // a contiguous run of items per workgroup: consecutive items of one
+  // particle (a particle's candidates) reuse its band spectrum in LDS
+  const int per = (nitem + (int)gridDim.x - 1) / (int)gridDim.x;
+  const int it0 = blockIdx.x * per, it1 = min(nitem, it0 + per);
+  int pprev = -1;
+  for (int it = it0; it < it1; it++) {
   const int p = pidx[it];
+  const bool newp = p != pprev;
+  pprev = p;
   const float amagp = amag_arr[it];
   const float rihp = rih / amagp, ri2p = ri2 / amagp / amagp, ri3p = ri3 / amagp / amagp;
 
@@ -334,7 +341,7 @@ __global__ void __launch_bounds__(kFT)
         size_t id = (size_t)i + (size_t)jc * wrapn(j, nsam);
         cf a = ainterpo_dev(nsam, a3df, d, i, j);
         float2 cr = cp[id], cl = cp[id + ic];
-        Sp[e] = sps[id];
+        if (newp) Sp[e] = sps[id];
         if (flip) {  // cc3.f:86-89
           if (cr.x + cl.x < 0.0f) a = cf{-a.re, -a.im};
         } else {
