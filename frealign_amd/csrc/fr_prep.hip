@@ -238,11 +238,14 @@ __global__ void __launch_bounds__(fftdev::kFftThreads)
     if (pr >= npairs) continue;
     const int img = (2 * pr) / N, row = (2 * pr) % N;
     float* ra = data + img * img_stride + (size_t)row * PITCH;
-    const float2 z = x < N ? res[s * LD + x] : make_float2(0.f, 0.f);
-    ra[x] = z.x;
-    ra[PITCH + x] = z.y;
-    if (x < N) {
-      const float ca = maskcos_r2(row, x, nmid), cb = maskcos_r2(row + 1, x, nmid);
+    if (x >= N) continue;  // the row padding is not read again
+    const float2 z = res[s * LD + x];
+    // pixels beyond RI3' become the ring mean in the masked r2c, which does
+    // not read them: only those within RI3' are stored
+    const float ca = maskcos_r2(row, x, nmid), cb = maskcos_r2(row + 1, x, nmid);
+    if (ca <= ri3p) ra[x] = z.x;
+    if (cb <= ri3p) ra[PITCH + x] = z.y;
+    {
       if (ca > ri2p && ca <= ri3p) {
         sum += z.x * scal;
         cnt++;
@@ -318,8 +321,10 @@ __global__ void __launch_bounds__(fftdev::kFftThreads)
     if (pr < npairs) {
       const int img = (2 * pr) / N, row = (2 * pr) % N;
       const float* ra = data + img * img_stride + (size_t)row * PITCH;
-      v = make_float2(mask(ra[x] * scal, maskcos_r2(row, x, nmid)),
-                      mask(ra[x + PITCH] * scal, maskcos_r2(row + 1, x, nmid)));
+      // beyond RI3' the pixel is the mean: k_rows_c2r_ring did not store it
+      const float ca = maskcos_r2(row, x, nmid), cb = maskcos_r2(row + 1, x, nmid);
+      v = make_float2(ca > ri3p ? mean : mask(ra[x] * scal, ca),
+                      cb > ri3p ? mean : mask(ra[x + PITCH] * scal, cb));
     }
     buf[s * LD + x] = v;
   }
